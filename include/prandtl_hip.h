@@ -1,0 +1,107 @@
+/*
+ * prandtl_hip.h — C ABI of the MI355X-native DGSEM right-hand side and
+ * SSP-RK3 stage update (library: prandtl_amd/libprandtl_hip.so).
+ *
+ * Drop-in boundary for Prandtl's semi-discrete operator on one GPU.  Every
+ * entry point returns an int status (PRANDTL_OK = 0); no exception crosses
+ * the ABI.  State vectors are device pointers in the reference's byNODES
+ * layout: index = var*NE*K + elem*K + node, nodes lexicographic, x fastest
+ * (Simulation.cpp:338-346 builds the vector space with Ordering::byNODES).
+ * `stream` is a hipStream_t (NULL = default stream); calls are asynchronous
+ * on it unless stated otherwise.  One context per stream/thread; a context
+ * owns its device scratch, the caller owns the state buffers.
+ */
+#ifndef PRANDTL_HIP_H
+#define PRANDTL_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PRANDTL_OK = 0,
+    PRANDTL_ERR_ARG = 1,          /* bad argument or size */
+    PRANDTL_ERR_UNSUPPORTED = 2,  /* configuration not built */
+    PRANDTL_ERR_HIP = 3,          /* HIP runtime error (see prandtl_last_error) */
+    PRANDTL_ERR_STATE = 4,        /* non-finite or non-positive density/pressure */
+    PRANDTL_ERR_NOMEM = 5
+};
+
+enum { PRANDTL_FP64 = 0, PRANDTL_MIXED = 1 };
+
+typedef struct prandtl_ctx prandtl_ctx;
+
+/* Periodic Cartesian box of hexahedra/quadrilaterals with LGL nodes.
+ * Replaces the mesh/space set-up the operator receives in
+ * DGSEMOperator::DGSEMOperator (DGSEMOperator.hpp:87-99; spaces built at
+ * Simulation.cpp:338-346).  Element index = (ez*ny + ey)*nx + ex. */
+typedef struct prandtl_mesh_desc {
+    int dim;            /* 2 or 3 */
+    int order;          /* polynomial degree N (Np = N+1): 2..5 in 2-D, 2..7 in 3-D */
+    int nelem[3];       /* elements per direction */
+    double lo[3], hi[3];
+} prandtl_mesh_desc;
+
+/* Physics and scheme switches: PhysicsConstants (Physics.hpp:10-37), the
+ * compile-time PARABOLIC / SUBCELL_FV_BLENDING flags (CMakeLists.txt:36-50)
+ * and DGSEMOperator's alpha_max/alpha_min (DGSEMOperator.hpp:99). */
+typedef struct prandtl_params {
+    double gamma, Pr, mu;
+    double alpha_max, alpha_min;
+    int parabolic;      /* Navier-Stokes with BR1 lifting */
+    int blending;       /* modal indicator + subcell FV blending */
+    int ec_surface;     /* 1: entropy-conservative two-point flux at faces */
+    int quirks;         /* 1: reproduce the reference's defects, 0: corrected */
+    int precision;      /* PRANDTL_FP64 or PRANDTL_MIXED */
+} prandtl_params;
+
+/* DGSEMOperator constructor (DGSEMOperator.hpp:87-99): builds LGL rule,
+ * derivative matrices (DGSEMIntegrator.cpp:8-82), modal basis
+ * (ModalBasis.cpp:5-22) and the device scratch.  Uses the current device. */
+int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *params,
+                   prandtl_ctx **out);
+void prandtl_destroy(prandtl_ctx *ctx);
+
+/* NE, K (nodes/element), nv (variables) */
+int prandtl_sizes(const prandtl_ctx *ctx, int *NE, int *K, int *nv);
+
+/* DGSEMOperator::Mult(const Vector &u, Vector &dudt) const
+ * (DGSEMOperator.hpp:105, DGSEMOperator.cpp:537-591): dudt is overwritten
+ * with the semi-discrete RHS of u.  u and dudt must not alias. */
+int prandtl_rhs(prandtl_ctx *ctx, const double *u_dev, double *dudt_dev, void *stream);
+
+/* RK3SSPSolver::Step(x, t, dt) as driven at Simulation.cpp:1010: three RHS
+ * evaluations with the Shu-Osher stage combinations fused into the RHS
+ * kernel; u is advanced in place. */
+int prandtl_rk3_step(prandtl_ctx *ctx, double *u_dev, double dt, void *stream);
+
+/* DGSEMOperator::GetMaxCharSpeed (DGSEMOperator.hpp:106-109): running
+ * maximum of |v|+c over every state the operator has evaluated since
+ * creation or the last reset (DGSEMIntegrator.cpp:39,188,270).  Synchronises
+ * the stream.  reset != 0 clears it after reading. */
+int prandtl_max_wave_speed(prandtl_ctx *ctx, double *out, int reset, void *stream);
+
+/* Blending coefficient alpha_e and indicator eta_e of the last RHS
+ * (the `alpha` / `eta` grid functions of DGSEMOperator.cpp:60-80), copied to
+ * host arrays of NE doubles.  Synchronises the stream. */
+int prandtl_alpha(prandtl_ctx *ctx, double *alpha_host, double *eta_host, void *stream);
+
+/* Simulation.cpp:1025-1035 nancheck, extended: returns PRANDTL_ERR_STATE if
+ * any node has a non-finite value, rho <= 0 or p <= 0.  Synchronises. */
+int prandtl_check_state(prandtl_ctx *ctx, const double *u_dev, void *stream);
+
+/* LGL nodes and weights on [0,1] used by the operator (DGSEMIntegrator.cpp:18-22). */
+int prandtl_lgl(int order, double *x, double *w);
+
+/* Diagnostics of the parabolic path: primitive-variable gradients
+ * [dim][nv][NE][K] of u after BR1 lifting and EntropyGrad2PrimGrad
+ * (DGSEMOperator.cpp:96-159), written to a device buffer. */
+int prandtl_debug_gradients(prandtl_ctx *ctx, const double *u_dev, double *grad_dev, void *stream);
+
+/* Last error message of this context (or of create when ctx is NULL). */
+const char *prandtl_last_error(const prandtl_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

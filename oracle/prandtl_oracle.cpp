@@ -1,0 +1,1151 @@
+/*
+ * TEST INFRASTRUCTURE — not part of the product.
+ *
+ * CPU FP64 restatement ("the oracle") of Prandtl's DGSEM right-hand side,
+ * its BR1 lifting, modal indicator with subcell-FV blending, and MFEM's
+ * SSP-RK3 step.  It is the checker that the HIP path in prandtl_amd/ is
+ * compared against; the product never links or calls it.
+ *
+ * Every routine cites the reference lines it restates (paths relative to the
+ * reference tree).  The arithmetic keeps the reference's operation order and
+ * association wherever the reference fixes it (for example MFEM's
+ * `Vector /= c` multiplies by 1/c, `a*b*c` is (a*b)*c), so that the oracle and
+ * the reference differ only where MFEM internals that are not in the tree
+ * (LGL node rounding, dense inverse rounding) fix the last bits.
+ *
+ * All of it is templated on the scalar type: `double` for values and `Cnt`,
+ * an op-counting scalar, for the algorithmic flop count (each +,-,*,/, sqrt,
+ * log, exp, pow counts one).
+ *
+ * Geometry is general per-node (J, adjugate rows, face normals, subcell
+ * normals) so the restatement follows the reference's metric use; the mesh
+ * generator provided here builds periodic Cartesian boxes.
+ */
+#include "prandtl_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+/* ------------------------------------------------------------------ */
+/* op-counting scalar                                                   */
+/* ------------------------------------------------------------------ */
+long long g_ops[8];
+int g_cat = 0;
+
+struct Cnt {
+    double v;
+    Cnt() : v(0.0) {}
+    Cnt(double x) : v(x) {}
+};
+inline Cnt operator+(Cnt a, Cnt b) { ++g_ops[g_cat]; return Cnt(a.v + b.v); }
+inline Cnt operator-(Cnt a, Cnt b) { ++g_ops[g_cat]; return Cnt(a.v - b.v); }
+inline Cnt operator*(Cnt a, Cnt b) { ++g_ops[g_cat]; return Cnt(a.v * b.v); }
+inline Cnt operator/(Cnt a, Cnt b) { ++g_ops[g_cat]; return Cnt(a.v / b.v); }
+inline Cnt operator-(Cnt a) { return Cnt(-a.v); }
+inline Cnt &operator+=(Cnt &a, Cnt b) { a = a + b; return a; }
+inline Cnt &operator-=(Cnt &a, Cnt b) { a = a - b; return a; }
+inline Cnt &operator*=(Cnt &a, Cnt b) { a = a * b; return a; }
+inline bool operator<(Cnt a, Cnt b) { return a.v < b.v; }
+inline bool operator>(Cnt a, Cnt b) { return a.v > b.v; }
+inline bool operator<=(Cnt a, Cnt b) { return a.v <= b.v; }
+inline bool operator!=(Cnt a, Cnt b) { return a.v != b.v; }
+
+inline double val(double x) { return x; }
+inline double val(Cnt x) { return x.v; }
+inline double mlog(double x) { return std::log(x); }
+inline Cnt mlog(Cnt x) { ++g_ops[g_cat]; return Cnt(std::log(x.v)); }
+inline double msqrt(double x) { return std::sqrt(x); }
+inline Cnt msqrt(Cnt x) { ++g_ops[g_cat]; return Cnt(std::sqrt(x.v)); }
+inline double mexp(double x) { return std::exp(x); }
+inline Cnt mexp(Cnt x) { ++g_ops[g_cat]; return Cnt(std::exp(x.v)); }
+inline double mpow(double x, double y) { return std::pow(x, y); }
+inline Cnt mpow(Cnt x, double y) { ++g_ops[g_cat]; return Cnt(std::pow(x.v, y)); }
+inline double mabs(double x) { return std::fabs(x); }
+inline Cnt mabs(Cnt x) { return Cnt(std::fabs(x.v)); }
+/* std::max semantics: max(a,b) = (a < b) ? b : a */
+template <class T> inline T mmax(T a, T b) { return (a < b) ? b : a; }
+
+enum { CAT_TOTAL = 0, CAT_IND = 1, CAT_LIFT = 2, CAT_VISC = 3, CAT_VOL = 4,
+       CAT_FACE = 5, CAT_FV = 6, CAT_ASM = 7 };
+
+/* ------------------------------------------------------------------ */
+/* 1-D rules (DGSEMIntegrator.cpp:18-82, ModalBasis.cpp:110-135)         */
+/* ------------------------------------------------------------------ */
+
+/* Legendre-Gauss-Lobatto nodes and weights on [0,1]: the rule MFEM returns
+ * for IntegrationRules(0, GaussLobatto).Get(SEGMENT, 2Np-3)
+ * (DGSEMIntegrator.cpp:18-22).  Interior nodes are the roots of P_N',
+ * found by Newton's method in long double from Chebyshev-Lobatto guesses;
+ * weights 2/(N(N+1) P_N(x)^2) on [-1,1], halved on [0,1]. */
+void lgl_rule(int N, std::vector<double> &x, std::vector<double> &w)
+{
+    const int Np = N + 1;
+    x.assign(Np, 0.0);
+    w.assign(Np, 0.0);
+    if (N == 0) { x[0] = 0.5; w[0] = 1.0; return; }
+    std::vector<long double> z(Np), wz(Np);
+    for (int i = 0; i < Np; i++) {
+        long double t = -std::cos((long double)M_PI * i / N);
+        if (i == 0) t = -1.0L;
+        else if (i == N) t = 1.0L;
+        else {
+            for (int it = 0; it < 100; it++) {
+                /* P_N, P_N' by recurrence */
+                long double p0 = 1.0L, p1 = t, dp0 = 0.0L, dp1 = 1.0L;
+                for (int n = 1; n < N; n++) {
+                    long double p2 = ((2 * n + 1) * t * p1 - n * p0) / (n + 1);
+                    long double dp2 = dp0 + (2 * n + 1) * p1;
+                    p0 = p1; p1 = p2; dp0 = dp1; dp1 = dp2;
+                }
+                long double PN = (N == 1) ? t : p1;
+                long double dPN = (N == 1) ? 1.0L : dp1;
+                long double d2PN = (2.0L * t * dPN - N * (N + 1) * PN) / (1.0L - t * t);
+                long double dt = dPN / d2PN;
+                t -= dt;
+                if (std::fabs((double)dt) < 1e-22) break;
+            }
+        }
+        long double p0 = 1.0L, p1 = t;
+        for (int n = 1; n < N; n++) {
+            long double p2 = ((2 * n + 1) * t * p1 - n * p0) / (n + 1);
+            p0 = p1; p1 = p2;
+        }
+        long double PN = (N == 1) ? t : p1;
+        z[i] = t;
+        wz[i] = 2.0L / (N * (N + 1) * PN * PN);
+    }
+    /* symmetric pairs share magnitudes; the middle node of odd Np is 1/2 */
+    for (int i = 0; i < Np; i++) {
+        long double zs = 0.5L * (z[i] - z[N - i]);
+        long double ws = 0.5L * (wz[i] + wz[N - i]);
+        x[i] = (double)(0.5L * (1.0L + zs));
+        w[i] = (double)(0.5L * ws);
+    }
+    x[0] = 0.0;
+    x[N] = 1.0;
+    if (Np % 2 == 1) x[N / 2] = 0.5;
+}
+
+/* D, Dhat = D + E, Dhat2 = 2D + E with E = diag(+1/w_0, 0, ..., -1/w_N);
+ * barycentric weights and negative-sum diagonal exactly as
+ * DGSEMIntegrator.cpp:45-80 (stored here un-transposed, row-major). */
+void dmats(int N, const std::vector<double> &x, const std::vector<double> &w,
+           std::vector<double> &D, std::vector<double> &Dh, std::vector<double> &Dh2)
+{
+    const int Np = N + 1;
+    std::vector<double> wb(Np, 1.0);
+    for (int i = 1; i < Np; i++)
+        for (int j = 0; j < i; j++) {
+            wb[j] *= (x[j] - x[i]);
+            wb[i] *= (x[i] - x[j]);
+        }
+    for (int i = 0; i < Np; i++) wb[i] = 1.0 / wb[i];
+    D.assign(Np * Np, 0.0);
+    for (int iL = 0; iL < Np; iL++)
+        for (int i = 0; i < Np; i++)
+            if (iL != i) {
+                D[i * Np + iL] = wb[iL] / wb[i] / (x[i] - x[iL]);
+                D[i * Np + i] -= D[i * Np + iL];
+            }
+    Dh = D;
+    Dh[0] += 1.0 / w[0];
+    Dh[N * Np + N] -= 1.0 / w[N];
+    Dh2.assign(Np * Np, 0.0);
+    for (int i = 0; i < Np * Np; i++) Dh2[i] = D[i] * 2.0;
+    Dh2[0] += 1.0 / w[0];
+    Dh2[N * Np + N] -= 1.0 / w[N];
+}
+
+/* shifted Legendre P_n(2x-1), MFEM Poly_1D::CalcLegendre (ModalBasis.cpp:123) */
+void calc_legendre(int p, double x, double *u)
+{
+    u[0] = 1.0;
+    if (p == 0) return;
+    double z = 2.0 * x - 1.0;
+    u[1] = z;
+    for (int n = 1; n < p; n++) u[n + 1] = ((2 * n + 1) * z * u[n] - n * u[n - 1]) / (n + 1);
+}
+
+/* dense inverse by Gauss-Jordan with partial pivoting, long double */
+bool invert(int n, const std::vector<double> &A, std::vector<double> &Ai)
+{
+    std::vector<long double> a(n * 2 * n, 0.0L);
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) a[i * 2 * n + j] = A[i * n + j];
+        a[i * 2 * n + n + i] = 1.0L;
+    }
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::fabs((double)a[r * 2 * n + c]) > std::fabs((double)a[piv * 2 * n + c])) piv = r;
+        if (a[piv * 2 * n + c] == 0.0L) return false;
+        if (piv != c)
+            for (int j = 0; j < 2 * n; j++) std::swap(a[c * 2 * n + j], a[piv * 2 * n + j]);
+        long double d = a[c * 2 * n + c];
+        for (int j = 0; j < 2 * n; j++) a[c * 2 * n + j] /= d;
+        for (int r = 0; r < n; r++)
+            if (r != c) {
+                long double f = a[r * 2 * n + c];
+                if (f != 0.0L)
+                    for (int j = 0; j < 2 * n; j++) a[r * 2 * n + j] -= f * a[c * 2 * n + j];
+            }
+    }
+    Ai.assign(n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) Ai[i * n + j] = (double)a[i * 2 * n + n + j];
+    return true;
+}
+
+/* ------------------------------------------------------------------ */
+/* point physics                                                        */
+/* ------------------------------------------------------------------ */
+
+/* BasicOperations.cpp:6-12, eps = 1e-4 (BasicOperations.hpp:9) */
+template <class T> T logmean(T x, T y)
+{
+    T xi = y / x;
+    T u = (xi * (xi - 2.0) + 1.0) / (xi * (xi + 2.0) + 1.0);
+    return (u < T(1e-4)) ? (x + y) * 52.5 / (105.0 + u * (35.0 + u * (21.0 + 15.0 * u)))
+                         : (y - x) / mlog(xi);
+}
+
+/* BasicOperations.cpp:53-56 with b = 0 */
+template <class T> T sound_speed(T p, T rho, double gamma) { return msqrt(gamma * p / rho); }
+
+/* MFEM Vector::Norml2 (scaled, overflow-safe) used at ChandrashekarFlux.cpp:88 */
+template <class T> T norml2(int n, const T *d)
+{
+    if (n == 1) return mabs(d[0]);
+    T scale = 0.0, sum = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (d[i] != T(0.0)) {
+            T a = mabs(d[i]);
+            if (scale <= a) {
+                T s = scale / a;
+                sum = 1.0 + sum * (s * s);
+                scale = a;
+                continue;
+            }
+            T s = a / scale;
+            sum += s * s;
+        }
+    }
+    return scale * msqrt(sum);
+}
+
+/* ChandrashekarFlux::ComputeVolumeFlux, ChandrashekarFlux.cpp:11-83 */
+template <class T>
+T volume_flux(int dim, double gamma, const T *s1, const T *s2, const T *m1, const T *m2, T *F)
+{
+    const double gm1 = gamma - 1.0, gm1i = 1.0 / gm1;
+    const int nv = dim + 2;
+    T metric[3], mom[3];
+    for (int d = 0; d < dim; d++) metric[d] = (m1[d] + m2[d]) * 0.5;
+    T rho1 = s1[0], rho2 = s2[0];
+    T rho_ln = logmean(rho1, rho2);
+    T Vsq1 = 0.0, Vsq2 = 0.0, Vn = 0.0, h_hat = 0.0;
+    for (int d = 0; d < dim; d++) {
+        T a1 = s1[1 + d] / rho1, a2 = s2[1 + d] / rho2;
+        T am = 0.5 * (a1 + a2);
+        if (d == 0) {
+            Vsq1 = a1 * a1; Vsq2 = a2 * a2; Vn = am * metric[0];
+            h_hat = -0.25 * (a1 * a1 + a2 * a2) + am * am;
+        } else {
+            Vsq1 += a1 * a1; Vsq2 += a2 * a2; Vn += am * metric[d];
+            h_hat += -0.25 * (a1 * a1 + a2 * a2) + am * am;
+        }
+        mom[d] = rho_ln * am;
+    }
+    T p1 = gm1 * (s1[nv - 1] - 0.5 * rho1 * Vsq1);
+    T p2 = gm1 * (s2[nv - 1] - 0.5 * rho2 * Vsq2);
+    T lam = msqrt(Vsq1) + sound_speed(p1, rho1, gamma);
+    lam = mmax(lam, msqrt(Vsq2) + sound_speed(p2, rho2, gamma));
+    T beta1 = 0.5 * rho1 / p1, beta2 = 0.5 * rho2 / p2;
+    T beta_ln = logmean(beta1, beta2);
+    T p_hat = 0.5 * (rho1 + rho2) / (beta1 + beta2);
+    h_hat += 0.5 / beta_ln * gm1i + p_hat / rho_ln;
+    F[0] = rho_ln * Vn;
+    for (int d = 0; d < dim; d++) F[1 + d] = Vn * mom[d] + p_hat * metric[d];
+    F[1 + dim] = rho_ln * Vn * h_hat;
+    return lam;
+}
+
+/* ChandrashekarFlux::ComputeFaceFlux, ChandrashekarFlux.cpp:85-161.
+ * quirks=1 keeps the 3-D energy-dissipation term of line 131
+ * ("+ w_mean" where the 2-D line 118 multiplies). */
+template <class T>
+T face_flux(int dim, double gamma, int quirks, const T *s1, const T *s2, const T *nor, T *F)
+{
+    const double gm1 = gamma - 1.0, gm1i = 1.0 / gm1;
+    const int nv = dim + 2;
+    T nor_mag = norml2(dim, nor);
+    T rho1 = s1[0], rho2 = s2[0];
+    T rho_mean = 0.5 * (rho1 + rho2);
+    T rho_ln = logmean(rho1, rho2);
+    T drho = rho2 - rho1;
+    T Vsq1 = 0.0, Vsq2 = 0.0, Vn = 0.0, h_hat = 0.0, diss = 0.0, mom[3];
+    for (int d = 0; d < dim; d++) {
+        T a1 = s1[1 + d] / rho1, a2 = s2[1 + d] / rho2;
+        T am = 0.5 * (a1 + a2);
+        T da = a2 - a1;
+        if (d == 0) {
+            Vsq1 = a1 * a1; Vsq2 = a2 * a2; Vn = am * nor[0];
+            h_hat = -0.25 * (a1 * a1 + a2 * a2) + am * am;
+            diss = 0.5 * drho * a1 * a2 + rho_mean * da * am;
+        } else {
+            Vsq1 += a1 * a1; Vsq2 += a2 * a2; Vn += am * nor[d];
+            h_hat += -0.25 * (a1 * a1 + a2 * a2) + am * am;
+            if (d == 2 && quirks)
+                diss += 0.5 * drho * a1 * a2 + rho_mean * da + am;
+            else
+                diss += 0.5 * drho * a1 * a2 + rho_mean * da * am;
+        }
+        mom[d] = rho_ln * am;
+    }
+    T p1 = gm1 * (s1[nv - 1] - 0.5 * rho1 * Vsq1);
+    T p2 = gm1 * (s2[nv - 1] - 0.5 * rho2 * Vsq2);
+    T lam = msqrt(Vsq1) + sound_speed(p1, rho1, gamma);
+    lam = mmax(lam, msqrt(Vsq2) + sound_speed(p2, rho2, gamma));
+    T beta1 = 0.5 * rho1 / p1, beta2 = 0.5 * rho2 / p2;
+    T beta_ln = logmean(beta1, beta2);
+    T p_hat = 0.5 * (rho1 + rho2) / (beta1 + beta2);
+    h_hat += 0.5 / beta_ln * gm1i + p_hat / rho_ln;
+    diss += 0.5 * drho * gm1i / beta_ln + 0.5 * rho_mean * gm1i * (1.0 / beta2 - 1.0 / beta1);
+    F[0] = rho_ln * Vn - 0.5 * lam * (rho2 - rho1) * nor_mag;
+    for (int d = 0; d < dim; d++)
+        F[1 + d] = Vn * mom[d] + p_hat * nor[d] - 0.5 * lam * (s2[1 + d] - s1[1 + d]) * nor_mag;
+    F[1 + dim] = rho_ln * Vn * h_hat - 0.5 * lam * diss * nor_mag;
+    return lam;
+}
+
+/* ComputePressure, BasicOperations.cpp:31-37 (V /= rho multiplies by 1/rho) */
+template <class T> T pressure(int dim, double gamma, const T *s)
+{
+    T r = 1.0 / s[0];
+    T vv = 0.0;
+    for (int d = 0; d < dim; d++) {
+        T v = s[1 + d] * r;
+        vv += v * v;
+    }
+    return (gamma - 1.0) * (s[dim + 1] - 0.5 * s[0] * vv);
+}
+
+/* Conserv2Entropy, BasicOperations.cpp:81-104 (beta = rho/p here) */
+template <class T> void cons2entropy(int dim, double gamma, const T *s, T *w)
+{
+    const double gm1 = gamma - 1.0, gm1i = 1.0 / gm1;
+    T r = 1.0 / s[0];
+    T vel[3];
+    T vv = 0.0;
+    for (int d = 0; d < dim; d++) { vel[d] = s[1 + d] * r; vv += vel[d] * vel[d]; }
+    T v_sq = 0.5 * vv;
+    T p = gm1 * (s[dim + 1] - s[0] * v_sq);
+    T beta = s[0] / p;
+    T sent = mlog(p * mpow(s[0], -gamma));
+    w[0] = (gamma - sent) * gm1i - beta * v_sq;
+    for (int d = 0; d < dim; d++) w[1 + d] = beta * vel[d];
+    w[dim + 1] = -beta;
+}
+
+/* EntropyGrad2PrimGrad, BasicOperations.cpp:126-170, one direction, in place.
+ * quirks=1: `Vector vel(state.GetData()+1, dim); vel /= state(0)` (lines
+ * 137-138) aliases the row copy, so the density-gradient term at line 164
+ * multiplies velocities where momenta are meant.  That term cancels from
+ * grad p - (p/rho) grad rho, so the viscous flux is unchanged up to
+ * rounding; only grad rho and grad p themselves differ. */
+template <class T> void entgrad2primgrad(int dim, double gamma, int quirks, const T *s, T *g)
+{
+    const double gm1 = gamma - 1.0, gm1i = 1.0 / gm1;
+    T r = 1.0 / s[0];
+    T vel[3];
+    T vv = 0.0;
+    for (int d = 0; d < dim; d++) { vel[d] = s[1 + d] * r; vv += vel[d] * vel[d]; }
+    T v_sq = 0.5 * vv;
+    T p = gm1 * (s[dim + 1] - s[0] * v_sq);
+    T KE = v_sq * s[0];
+    T gs[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int v = 0; v < dim + 2; v++) gs[v] = g[v];
+    const int L = dim + 1;
+    T mdot = 0.0;
+    for (int d = 0; d < dim; d++) {
+        g[1 + d] = p / s[0] * (gs[1 + d] + vel[d] * gs[L]);
+        T md = quirks ? vel[d] : s[1 + d];
+        if (d == 0) mdot = md * g[1];
+        else mdot += md * g[1 + d];
+    }
+    g[0] = s[0] * gs[0] - gs[L] * (KE - p * gm1i) + s[0] / p * mdot;
+    g[L] = p / s[0] * (g[0] + p * gs[L]);
+}
+
+/* NavierStokesFlux::ComputeViscousFlux, NavierStokesFlux.cpp:12-90 (constant
+ * mu, mu_bulk = 2/3 from Physics.hpp:34).  flux is nv x dim, row 0 = 0. */
+template <class T>
+void viscous_flux(int dim, double gamma, double Pr, double mu, const T *s,
+                  const T *const *dq, T *flux)
+{
+    const double gm1 = gamma - 1.0, gm1i = 1.0 / gm1;
+    const double mu_bulk = 2.0 / 3.0;
+    const int nv = dim + 2;
+    T prim[5];
+    prim[0] = s[0];
+    for (int d = 0; d < dim; d++) prim[1 + d] = s[1 + d] / s[0];
+    prim[nv - 1] = pressure(dim, gamma, s);
+    T lambda = mu * gamma * (1.0 / Pr);
+    T div = dq[0][1];
+    for (int d = 1; d < dim; d++) div += dq[d][1 + d];
+    T cvdT[3];
+    for (int c = 0; c < dim; c++)
+        cvdT[c] = gm1i / prim[0] * (dq[c][nv - 1] - prim[nv - 1] / prim[0] * dq[c][0]);
+    for (int c = 0; c < dim; c++) flux[0 * dim + c] = 0.0;
+    for (int c = 0; c < dim; c++) {
+        for (int a = 0; a < dim; a++) {
+            /* tau(a, c): mu (2 du_a/dx_a - mu_bulk div) on the diagonal,
+             * mu (du_a/dx_c + du_c/dx_a) off it, with the reference's
+             * operand order (first index is the row of the flux matrix) */
+            if (a == c) flux[(1 + a) * dim + c] = mu * (2.0 * dq[a][1 + a] - mu_bulk * div);
+            else if (c == 0) flux[(1 + a) * dim + c] = mu * (dq[a][1 + 0] + dq[0][1 + a]);
+            else flux[(1 + a) * dim + c] = mu * (dq[c][1 + a] + dq[a][1 + c]);
+        }
+        T e = prim[1] * flux[1 * dim + c];
+        for (int a = 1; a < dim; a++) e = e + prim[1 + a] * flux[(1 + a) * dim + c];
+        flux[(nv - 1) * dim + c] = e + lambda * cvdT[c];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* mesh                                                                 */
+/* ------------------------------------------------------------------ */
+struct Mesh {
+    int dim, N, Np, K, NE, Nfp, nv;
+    int n[3];
+    double h[3], lo[3];
+    std::vector<double> J, adj;      /* [NE*K], [NE*K*dim*dim] row r = Ja^r */
+    int nf;
+    std::vector<int> fe1, fe2, fn1, fn2;
+    std::vector<double> fnor;        /* [nf*Nfp*dim] */
+    std::vector<double> sub;         /* [NE][dim][Nfp][N][dim] interior subcell normals */
+    double hmin;
+};
+
+inline int line_node(int dim, int Np, int r, int l, int m)
+{
+    if (dim == 2) return r == 0 ? l * Np + m : m * Np + l;
+    int a = l % Np, b = l / Np;
+    if (r == 0) return (b * Np + a) * Np + m;
+    if (r == 1) return (b * Np + m) * Np + a;
+    return (m * Np + b) * Np + a;
+}
+
+} /* namespace */
+
+struct orc_ctx {
+    orc_params p;
+    Mesh m;
+    std::vector<double> x, w, D, Dh, Dh2;
+    std::vector<double> Vinv;        /* K x K */
+    std::vector<int> degs;           /* K x dim (ModalBasis ubdegs) */
+    std::vector<double> alpha, eta, wJ;
+    std::vector<double> grad;        /* [dim][nv][NE][K] */
+    std::vector<double> stage_y, stage_k;
+    double lam;
+    double thresh;
+};
+
+namespace {
+
+void build_mesh(orc_ctx *c)
+{
+    Mesh &m = c->m;
+    const orc_params &p = c->p;
+    m.dim = p.dim; m.N = p.order; m.Np = p.order + 1;
+    m.nv = p.dim + 2;
+    m.K = m.Np * m.Np * (p.dim == 3 ? m.Np : 1);
+    m.Nfp = (p.dim == 3) ? m.Np * m.Np : m.Np;
+    m.NE = 1;
+    for (int r = 0; r < 3; r++) m.n[r] = (r < p.dim) ? p.nelem[r] : 1;
+    for (int r = 0; r < p.dim; r++) {
+        m.NE *= m.n[r];
+        m.h[r] = (p.hi[r] - p.lo[r]) / m.n[r];
+        m.lo[r] = p.lo[r];
+    }
+    const int dim = m.dim, K = m.K, NE = m.NE, Np = m.Np, N = m.N, Nfp = m.Nfp;
+    /* Cartesian Jacobian diag(h); det and adjugate with MFEM's association */
+    double Jd, ad[3] = {0, 0, 0};
+    if (dim == 2) { Jd = m.h[0] * m.h[1]; ad[0] = m.h[1]; ad[1] = m.h[0]; }
+    else {
+        Jd = m.h[0] * (m.h[1] * m.h[2]);
+        ad[0] = m.h[1] * m.h[2]; ad[1] = m.h[0] * m.h[2]; ad[2] = m.h[0] * m.h[1];
+    }
+    m.hmin = m.h[0];
+    for (int r = 1; r < dim; r++) m.hmin = std::min(m.hmin, m.h[r]);
+    m.J.assign((size_t)NE * K, Jd);
+    m.adj.assign((size_t)NE * K * dim * dim, 0.0);
+    for (size_t q = 0; q < (size_t)NE * K; q++)
+        for (int r = 0; r < dim; r++) m.adj[q * dim * dim + r * dim + r] = ad[r];
+    /* interior faces: every element is elem1 of its +r face, elem2 is its
+     * periodic +r neighbour, normal = +|face| e_r */
+    m.nf = dim * NE;
+    m.fe1.resize(m.nf); m.fe2.resize(m.nf);
+    m.fn1.resize((size_t)m.nf * Nfp); m.fn2.resize((size_t)m.nf * Nfp);
+    m.fnor.assign((size_t)m.nf * Nfp * dim, 0.0);
+    int f = 0;
+    for (int r = 0; r < dim; r++)
+        for (int e = 0; e < NE; e++, f++) {
+            int ex = e % m.n[0], ey = (e / m.n[0]) % m.n[1], ez = e / (m.n[0] * m.n[1]);
+            int idx[3] = {ex, ey, ez};
+            idx[r] = (idx[r] + 1) % m.n[r];
+            int e2 = (idx[2] * m.n[1] + idx[1]) * m.n[0] + idx[0];
+            m.fe1[f] = e; m.fe2[f] = e2;
+            for (int q = 0; q < Nfp; q++) {
+                m.fn1[(size_t)f * Nfp + q] = line_node(dim, Np, r, q, N);
+                m.fn2[(size_t)f * Nfp + q] = line_node(dim, Np, r, q, 0);
+                m.fnor[((size_t)f * Nfp + q) * dim + r] = ad[r];
+            }
+        }
+    /* subcell normals, DGSEMIntegrator.cpp:1199-1304:
+     * n_{m+1/2} = Ja^r(line start) + sum_{l<=m} w_l sum_q D_{lq} Ja^r_q */
+    if (c->p.blending) {
+        m.sub.assign((size_t)NE * dim * Nfp * N * dim, 0.0);
+        for (int e = 0; e < NE; e++)
+            for (int r = 0; r < dim; r++)
+                for (int l = 0; l < Nfp; l++) {
+                    int n0 = line_node(dim, Np, r, l, 0);
+                    const double *m1 = &m.adj[((size_t)e * K + n0) * dim * dim + r * dim];
+                    for (int i = 1; i < Np; i++) {
+                        double nor[3];
+                        for (int d = 0; d < dim; d++) nor[d] = m1[d];
+                        for (int ll = 0; ll < i; ll++) {
+                            double tmp[3] = {0, 0, 0};
+                            for (int q = 0; q < Np; q++) {
+                                int nq = line_node(dim, Np, r, l, q);
+                                const double *m2 = &m.adj[((size_t)e * K + nq) * dim * dim + r * dim];
+                                for (int d = 0; d < dim; d++) tmp[d] += m2[d] * c->D[ll * Np + q];
+                            }
+                            for (int d = 0; d < dim; d++) { tmp[d] *= c->w[ll]; nor[d] += tmp[d]; }
+                        }
+                        double *dst = &m.sub[((((size_t)e * dim + r) * Nfp + l) * N + (i - 1)) * dim];
+                        for (int d = 0; d < dim; d++) dst[d] = nor[d];
+                    }
+                }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* the RHS engine                                                       */
+/* ------------------------------------------------------------------ */
+template <class T> struct Engine {
+    orc_ctx *c;
+    const Mesh &m;
+    int dim, N, Np, K, NE, nv, Nfp;
+    size_t NK;
+    double gamma;
+    std::vector<T> U, W, G;  /* state copy (counted only), entropy vars, gradients */
+
+    explicit Engine(orc_ctx *cc) : c(cc), m(cc->m)
+    {
+        dim = m.dim; N = m.N; Np = m.Np; K = m.K; NE = m.NE; nv = m.nv; Nfp = m.Nfp;
+        NK = (size_t)NE * K;
+        gamma = c->p.gamma;
+    }
+
+    inline void state(const T *u, int e, int i, T *s) const
+    {
+        for (int v = 0; v < nv; v++) s[v] = u[v * NK + (size_t)e * K + i];
+    }
+
+    /* Indicator::CheckSmoothness (PerssonPeraireIndicator.cpp:21-58) and
+     * DGSEMOperator::ComputeBlendingCoefficient (DGSEMOperator.cpp:60-80) */
+    void blending(const T *u)
+    {
+        g_cat = CAT_IND;
+        std::vector<T> q(K), md(K);
+        const double s = 9.21024, thr = c->thresh;
+        for (int e = 0; e < NE; e++) {
+            for (int i = 0; i < K; i++) {
+                T st[5];
+                state(u, e, i, st);
+                q[i] = st[0] * pressure(dim, gamma, st);
+            }
+            for (int a = 0; a < K; a++) {
+                T y = c->Vinv[(size_t)a * K + 0] * q[0];
+                for (int i = 1; i < K; i++) y += c->Vinv[(size_t)a * K + i] * q[i];
+                md[a] = y;
+            }
+            T mm = 0.0, m1 = 0.0, m2 = 0.0;
+            for (int a = 0; a < K; a++) {
+                bool in1 = true, in2 = true;
+                for (int d = 0; d < dim; d++) {
+                    int dg = c->degs[a * dim + d];
+                    if (dg > N - 2) { in2 = false; if (dg > N - 1) in1 = false; }
+                }
+                mm += md[a] * md[a];
+                m1 += (in1 ? md[a] : T(0.0)) * (in1 ? md[a] : T(0.0));
+                m2 += (in2 ? md[a] : T(0.0)) * (in2 ? md[a] : T(0.0));
+            }
+            T eta = 1.0 - m1 / mm;
+            eta = mmax(eta, 1.0 - m2 / m1);
+            c->eta[e] = val(eta);
+            T al = 1.0 / (1.0 + mexp(-s * (eta - thr) / thr));
+            double a = val(al);
+            if (a < c->p.alpha_min) a = 0.0;
+            else if (a > 1.0 - c->p.alpha_min) a = 1.0;
+            a = std::min(a, c->p.alpha_max);
+            c->alpha[e] = a;
+        }
+    }
+
+    /* DGSEMOperator.cpp:82-94, DGSEMNonlinearForm.cpp:424-645,
+     * LiftingBR1.cpp:10-66 (faces) and 156-272 (elements),
+     * DGSEMOperator.cpp:96-159 (gradient conversion) */
+    void lifting(const T *u)
+    {
+        g_cat = CAT_LIFT;
+        W.assign(nv * NK, T(0.0));
+        G.assign(dim * nv * NK, T(0.0));
+        for (int e = 0; e < NE; e++)
+            for (int i = 0; i < K; i++) {
+                T st[5], wv[5];
+                state(u, e, i, st);
+                cons2entropy(dim, gamma, st, wv);
+                for (int v = 0; v < nv; v++) W[v * NK + (size_t)e * K + i] = wv[v];
+            }
+        const double *D = c->D.data();
+        for (int e = 0; e < NE; e++)
+            for (int id = 0; id < K; id++) {
+                int ii[3] = {id % Np, (id / Np) % Np, id / (Np * Np)};
+                T g1[5][3];
+                for (int v = 0; v < nv; v++)
+                    for (int r = 0; r < dim; r++) g1[v][r] = 0.0;
+                for (int l = 0; l < Np; l++)
+                    for (int r = 0; r < dim; r++) {
+                        int jj[3] = {ii[0], ii[1], ii[2]};
+                        jj[r] = l;
+                        int nl = (jj[2] * Np + jj[1]) * Np + jj[0];
+                        for (int v = 0; v < nv; v++)
+                            g1[v][r] += W[v * NK + (size_t)e * K + nl] * D[ii[r] * Np + l];
+                    }
+                const double *adj = &m.adj[((size_t)e * K + id) * dim * dim];
+                T invJ = 1.0 / T(m.J[(size_t)e * K + id]);
+                for (int cc = 0; cc < dim; cc++)
+                    for (int v = 0; v < nv; v++) {
+                        T a = g1[v][0] * adj[0 * dim + cc];
+                        for (int r = 1; r < dim; r++) a += g1[v][r] * adj[r * dim + cc];
+                        G[(cc * nv + v) * NK + (size_t)e * K + id] = a * invJ;
+                    }
+            }
+        const double w0 = c->w[0];
+        for (int f = 0; f < m.nf; f++)
+            for (int q = 0; q < Nfp; q++) {
+                int e1 = m.fe1[f], e2 = m.fe2[f];
+                int n1 = m.fn1[(size_t)f * Nfp + q], n2 = m.fn2[(size_t)f * Nfp + q];
+                const double *nor = &m.fnor[((size_t)f * Nfp + q) * dim];
+                T a1 = 1.0 / (w0 * T(m.J[(size_t)e1 * K + n1]));
+                T a2 = 1.0 / (w0 * T(m.J[(size_t)e2 * K + n2]));
+                for (int v = 0; v < nv; v++) {
+                    T fv = 0.5 * (W[v * NK + (size_t)e2 * K + n2] - W[v * NK + (size_t)e1 * K + n1]);
+                    for (int cc = 0; cc < dim; cc++) {
+                        T fc = fv * nor[cc];
+                        G[(cc * nv + v) * NK + (size_t)e1 * K + n1] += a1 * fc;
+                        G[(cc * nv + v) * NK + (size_t)e2 * K + n2] += a2 * fc;
+                    }
+                }
+            }
+        for (int e = 0; e < NE; e++)
+            for (int i = 0; i < K; i++) {
+                T st[5];
+                state(u, e, i, st);
+                for (int cc = 0; cc < dim; cc++) {
+                    T g[5];
+                    for (int v = 0; v < nv; v++) g[v] = G[(cc * nv + v) * NK + (size_t)e * K + i];
+                    entgrad2primgrad(dim, gamma, c->p.quirks, st, g);
+                    for (int v = 0; v < nv; v++) G[(cc * nv + v) * NK + (size_t)e * K + i] = g[v];
+                }
+            }
+    }
+
+    /* DGSEMIntegrator::ComputeFVFluxes, DGSEMIntegrator.cpp:972-1196.
+     * fv (K x nv, node-major) is overwritten with alpha * FV. */
+    T fv_element(const T *u, int e, T alpha_e, T *fv, T lam)
+    {
+        g_cat = CAT_FV;
+        const int q = c->p.quirks;
+        for (int i = 0; i < K * nv; i++) fv[i] = 0.0;
+        for (int r = 0; r < dim; r++)
+            for (int l = 0; l < Nfp; l++) {
+                T dU[5], s1[5], s2[5], F[5];
+                for (int v = 0; v < nv; v++) dU[v] = 0.0;
+                int id1 = line_node(dim, Np, r, l, 0);
+                state(u, e, id1, s1);
+                for (int i = 0; i < Np - 1; i++) {
+                    int id2 = line_node(dim, Np, r, l, i + 1);
+                    state(u, e, id2, s2);
+                    T nor[3];
+                    const double *sn = &m.sub[((((size_t)e * dim + r) * Nfp + l) * N + i) * dim];
+                    for (int d = 0; d < dim; d++) nor[d] = sn[d];
+                    lam = mmax(lam, face_flux(dim, gamma, q, s1, s2, nor, F));
+                    T sc = 1.0 / (T(m.J[(size_t)e * K + id1]) * c->w[i]);
+                    for (int v = 0; v < nv; v++) {
+                        dU[v] -= F[v];
+                        dU[v] *= sc;
+                        if (r == 0) fv[id1 * nv + v] = dU[v];
+                        else fv[id1 * nv + v] += dU[v];
+                        dU[v] = F[v];
+                        s1[v] = s2[v];
+                    }
+                    id1 = id2;
+                }
+                T sc = 1.0 / (T(m.J[(size_t)e * K + id1]) * c->w[Np - 1]);
+                for (int v = 0; v < nv; v++) {
+                    dU[v] *= sc;
+                    if (r == 0) fv[id1 * nv + v] = dU[v];
+                    else fv[id1 * nv + v] += dU[v];
+                }
+            }
+        for (int i = 0; i < K * nv; i++) fv[i] *= alpha_e;
+        return lam;
+    }
+
+    /* AssembleElementVector: Euler DGSEMIntegrator.cpp:224-433,
+     * NS-3D 601-750, NS-2D 752-864.  Adds the element vector to dudt. */
+    T element(const T *u, int e, T *dudt, T lam, std::vector<T> &Fi, std::vector<T> &Fv,
+              std::vector<T> &el)
+    {
+        const bool par = c->p.parabolic != 0, blend = c->p.blending != 0;
+        const int quirks = c->p.quirks;
+        const T alpha_e = blend ? T(c->alpha[e]) : T(0.0);
+        if (blend) lam = fv_element(u, e, alpha_e, el.data(), lam);
+        else for (int i = 0; i < K * nv; i++) el[i] = 0.0;
+        /* viscous volume flux, contravariant (DGSEMIntegrator.cpp:617-641) */
+        if (par) {
+            g_cat = CAT_VISC;
+            for (int i = 0; i < K; i++) {
+                int src = quirks ? K - 1 : i; /* stale member id1 (line 620) */
+                T st[5], dq[3][5], fl[5 * 3];
+                state(u, e, src, st);
+                const T *dqp[3];
+                for (int cc = 0; cc < dim; cc++) {
+                    for (int v = 0; v < nv; v++) dq[cc][v] = G[(cc * nv + v) * NK + (size_t)e * K + i];
+                    dqp[cc] = dq[cc];
+                }
+                viscous_flux(dim, gamma, c->p.Pr, c->p.mu, st, dqp, fl);
+                const double *adj = &m.adj[((size_t)e * K + i) * dim * dim];
+                for (int r = 0; r < dim; r++)
+                    for (int v = 0; v < nv; v++) {
+                        T a = adj[r * dim + 0] * fl[v * dim + 0];
+                        for (int cc = 1; cc < dim; cc++) a += adj[r * dim + cc] * fl[v * dim + cc];
+                        Fv[((size_t)r * K + i) * nv + v] = a;
+                    }
+            }
+        }
+        /* inviscid pairs, node loop (k, j, i) as the reference */
+        const int Nz = dim == 3 ? Np : 1;
+        auto FI = [&](int r, int node, int part) -> T * {
+            return &Fi[(((size_t)r * K + node) * Np + part) * nv];
+        };
+        for (int k = 0; k < Nz; k++)
+            for (int j = 0; j < Np; j++)
+                for (int i = 0; i < Np; i++) {
+                    g_cat = CAT_VOL;
+                    int id1 = (k * Np + j) * Np + i;
+                    int ii[3] = {i, j, k};
+                    T s1[5], s2[5], m1[3], m2[3];
+                    state(u, e, id1, s1);
+                    const double *adj1 = &m.adj[((size_t)e * K + id1) * dim * dim];
+                    T glast[5];
+                    for (int r = 0; r < dim; r++) {
+                        for (int d = 0; d < dim; d++) m1[d] = adj1[r * dim + d];
+                        T f[5];
+                        for (int v = 0; v < nv; v++) f[v] = 0.0;
+                        T *self = FI(r, id1, ii[r]);
+                        for (int v = 0; v < nv; v++) self[v] = 0.0;
+                        for (int mm = ii[r] + 1; mm < Np; mm++) {
+                            int jj[3] = {i, j, k};
+                            jj[r] = mm;
+                            int id2 = (jj[2] * Np + jj[1]) * Np + jj[0];
+                            state(u, e, id2, s2);
+                            const double *adj2 = &m.adj[((size_t)e * K + id2) * dim * dim];
+                            for (int d = 0; d < dim; d++) m2[d] = adj2[r * dim + d];
+                            lam = mmax(lam, volume_flux(dim, gamma, s1, s2, m1, m2, f));
+                            T *a = FI(r, id1, mm), *b = FI(r, id2, ii[r]);
+                            for (int v = 0; v < nv; v++) {
+                                a[v] = f[v];
+                                /* NS-3D: H_inviscid(id2).SetCol(k, g) (line 702) */
+                                b[v] = (par && quirks && dim == 3 && r == 2) ? glast[v] : f[v];
+                            }
+                        }
+                        if (r == 1)
+                            for (int v = 0; v < nv; v++) glast[v] = f[v];
+                    }
+                    g_cat = CAT_ASM;
+                    T dU[5], dV[5];
+                    for (int v = 0; v < nv; v++) { dU[v] = 0.0; dV[v] = 0.0; }
+                    for (int r = 0; r < dim; r++) {
+                        const double *Drow = &c->Dh2[ii[r] * Np];
+                        for (int mm = 0; mm < Np; mm++) {
+                            const T *fp = FI(r, id1, mm);
+                            for (int v = 0; v < nv; v++) {
+                                if (r == 0 && mm == 0) dU[v] = Drow[mm] * fp[v];
+                                else dU[v] += Drow[mm] * fp[v];
+                            }
+                        }
+                        if (par) {
+                            const double *Dr = &c->Dh[ii[r] * Np];
+                            for (int l = 0; l < Np; l++) {
+                                int jj[3] = {i, j, k};
+                                jj[r] = l;
+                                int nl = (jj[2] * Np + jj[1]) * Np + jj[0];
+                                const T *fp = &Fv[((size_t)r * K + nl) * nv];
+                                for (int v = 0; v < nv; v++) dV[v] += fp[v] * Dr[l];
+                            }
+                        }
+                    }
+                    T invJ = 1.0 / T(m.J[(size_t)e * K + id1]);
+                    for (int v = 0; v < nv; v++) {
+                        T a = -dU[v];
+                        if (blend) a *= (1.0 - alpha_e);
+                        if (par) a = a + dV[v];
+                        a *= invJ;
+                        el[id1 * nv + v] += a;
+                    }
+                }
+        for (int i = 0; i < K; i++)
+            for (int v = 0; v < nv; v++) dudt[v * NK + (size_t)e * K + i] += el[i * nv + v];
+        return lam;
+    }
+
+    /* interior faces of direction r, faces [f0, f1):
+     * AssembleFaceVector Euler DGSEMIntegrator.cpp:151-222,
+     * NS-3D 435-494, NS-2D 496-549 */
+    T faces(const T *u, int f0, int f1, T *dudt, T lam)
+    {
+        const bool par = c->p.parabolic != 0;
+        const int quirks = c->p.quirks, ec = c->p.ec_surface;
+        const double w0 = c->w[0];
+        for (int f = f0; f < f1; f++)
+            for (int q = 0; q < Nfp; q++) {
+                int e1 = m.fe1[f], e2 = m.fe2[f];
+                int n1 = m.fn1[(size_t)f * Nfp + q], n2 = m.fn2[(size_t)f * Nfp + q];
+                T s1[5], s2[5], nor[3], F[5];
+                state(u, e1, n1, s1);
+                state(u, e2, n2, s2);
+                for (int d = 0; d < dim; d++) nor[d] = m.fnor[((size_t)f * Nfp + q) * dim + d];
+                T d1[5], d2[5], fn[5];
+                if (par) {
+                    /* viscous fluxes first, as DGSEMIntegrator.cpp:467-477 */
+                    g_cat = CAT_VISC;
+                    T fl1[15], fl2[15], dq[3][5];
+                    const T *dqp[3];
+                    for (int cc = 0; cc < dim; cc++) {
+                        for (int v = 0; v < nv; v++) dq[cc][v] = G[(cc * nv + v) * NK + (size_t)e1 * K + n1];
+                        dqp[cc] = dq[cc];
+                    }
+                    viscous_flux(dim, gamma, c->p.Pr, c->p.mu, s1, dqp, fl1);
+                    for (int cc = 0; cc < dim; cc++)
+                        for (int v = 0; v < nv; v++) dq[cc][v] = G[(cc * nv + v) * NK + (size_t)e2 * K + n2];
+                    viscous_flux(dim, gamma, c->p.Pr, c->p.mu, s2, dqp, fl2);
+                    for (int v = 0; v < nv; v++) {
+                        fn[v] = 0.0;
+                        for (int cc = 0; cc < dim; cc++) {
+                            T avg = (fl1[v * dim + cc] + fl2[v * dim + cc]) * 0.5;
+                            fn[v] = (cc == 0) ? avg * nor[0] : fn[v] + avg * nor[cc];
+                        }
+                    }
+                    if (quirks) {
+                        /* ComputeViscousFlux -> Conserv2Prim -> ComputePressure
+                         * (NavierStokesFlux.cpp:14, BasicOperations.cpp:33-34,185)
+                         * divides the momentum slots of state1/state2 in place,
+                         * and ComputeFaceFlux (line 481) then reads them */
+                        T r1 = 1.0 / s1[0], r2 = 1.0 / s2[0];
+                        for (int d = 0; d < dim; d++) { s1[1 + d] *= r1; s2[1 + d] *= r2; }
+                    }
+                }
+                g_cat = CAT_FACE;
+                if (ec) lam = mmax(lam, volume_flux(dim, gamma, s1, s2, nor, nor, F));
+                else lam = mmax(lam, face_flux(dim, gamma, quirks, s1, s2, nor, F));
+                for (int v = 0; v < nv; v++) { d1[v] = -F[v]; d2[v] = F[v]; }
+                if (par)
+                    for (int v = 0; v < nv; v++) { d1[v] += fn[v]; d2[v] -= fn[v]; }
+                g_cat = CAT_ASM;
+                T a1 = 1.0 / (w0 * T(m.J[(size_t)e1 * K + n1]));
+                T a2 = 1.0 / (w0 * T(m.J[(size_t)e2 * K + n2]));
+                for (int v = 0; v < nv; v++) {
+                    dudt[v * NK + (size_t)e1 * K + n1] += a1 * d1[v];
+                    dudt[v * NK + (size_t)e2 * K + n2] += a2 * d2[v];
+                }
+            }
+        return lam;
+    }
+
+    /* DGSEMOperator::Mult (DGSEMOperator.cpp:537-591) with
+     * DGSEMNonlinearForm::Mult (DGSEMNonlinearForm.cpp:647-849, 1302-1545):
+     * dudt is overwritten. */
+    void rhs(const T *u, T *dudt, int threads)
+    {
+        if (c->p.blending) blending(u);
+        if (c->p.parabolic) lifting(u);
+        for (size_t i = 0; i < nv * NK; i++) dudt[i] = 0.0;
+        double lam = c->lam;
+        int nth = threads > 0 ? threads : 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nth) reduction(max : lam)
+#endif
+        {
+            std::vector<T> Fi((size_t)dim * K * Np * nv), Fv((size_t)dim * K * nv), el((size_t)K * nv);
+            T l = lam;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+            for (int e = 0; e < NE; e++) l = element(u, e, dudt, l, Fi, Fv, el);
+            /* faces of one direction touch disjoint nodes of each element */
+            for (int r = 0; r < dim; r++) {
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+                for (int b = 0; b < NE; b++) l = faces(u, r * NE + b, r * NE + b + 1, dudt, l);
+            }
+            lam = std::max(lam, val(l));
+        }
+        (void)nth;
+        c->lam = lam;
+    }
+};
+
+/* MFEM RK3SSPSolver::Step (Shu-Osher form), called at Simulation.cpp:1010 */
+void rk3_step(orc_ctx *c, double *x, double dt)
+{
+    Engine<double> E(c);
+    const size_t n = (size_t)c->m.nv * c->m.NE * c->m.K;
+    std::vector<double> &y = c->stage_y, &k = c->stage_k;
+    y.resize(n); k.resize(n);
+    E.rhs(x, k.data(), c->p.threads);
+    for (size_t i = 0; i < n; i++) y[i] = x[i] + dt * k[i];
+    E.rhs(y.data(), k.data(), c->p.threads);
+    for (size_t i = 0; i < n; i++) y[i] += dt * k[i];
+    for (size_t i = 0; i < n; i++) y[i] = 3. / 4 * x[i] + 1. / 4 * y[i];
+    E.rhs(y.data(), k.data(), c->p.threads);
+    for (size_t i = 0; i < n; i++) y[i] += dt * k[i];
+    for (size_t i = 0; i < n; i++) x[i] = 1. / 3 * x[i] + 2. / 3 * y[i];
+}
+
+} /* namespace */
+
+/* ------------------------------------------------------------------ */
+/* C ABI                                                                */
+/* ------------------------------------------------------------------ */
+extern "C" {
+
+int orc_lgl(int order, double *x, double *w)
+{
+    if (order < 1 || order > 16) return 1;
+    std::vector<double> xv, wv;
+    lgl_rule(order, xv, wv);
+    std::memcpy(x, xv.data(), xv.size() * sizeof(double));
+    std::memcpy(w, wv.data(), wv.size() * sizeof(double));
+    return 0;
+}
+
+int orc_dmat(int order, double *D, double *Dhat, double *Dhat2)
+{
+    if (order < 1 || order > 16) return 1;
+    std::vector<double> x, w, a, b, cc;
+    lgl_rule(order, x, w);
+    dmats(order, x, w, a, b, cc);
+    std::memcpy(D, a.data(), a.size() * sizeof(double));
+    std::memcpy(Dhat, b.data(), b.size() * sizeof(double));
+    std::memcpy(Dhat2, cc.data(), cc.size() * sizeof(double));
+    return 0;
+}
+
+int orc_legendre_vdm(int order, double *V1, double *V1inv)
+{
+    if (order < 1 || order > 16) return 1;
+    const int Np = order + 1;
+    std::vector<double> x, w, V(Np * Np), Vi;
+    lgl_rule(order, x, w);
+    std::vector<double> L(Np);
+    for (int i = 0; i < Np; i++) {
+        calc_legendre(order, x[i], L.data());
+        for (int a = 0; a < Np; a++) V[i * Np + a] = L[a];
+    }
+    if (!invert(Np, V, Vi)) return 2;
+    std::memcpy(V1, V.data(), V.size() * sizeof(double));
+    std::memcpy(V1inv, Vi.data(), Vi.size() * sizeof(double));
+    return 0;
+}
+
+double orc_logmean(double x, double y) { return logmean<double>(x, y); }
+
+double orc_volume_flux(int dim, double gamma, const double *s1, const double *s2,
+                       const double *m1, const double *m2, double *F)
+{
+    return volume_flux<double>(dim, gamma, s1, s2, m1, m2, F);
+}
+
+double orc_face_flux(int dim, double gamma, int quirks, const double *s1, const double *s2,
+                     const double *nor, double *F)
+{
+    return face_flux<double>(dim, gamma, quirks, s1, s2, nor, F);
+}
+
+void orc_cons2entropy(int dim, double gamma, const double *s, double *w)
+{
+    cons2entropy<double>(dim, gamma, s, w);
+}
+
+void orc_entgrad2primgrad(int dim, double gamma, int quirks, const double *s, double *g)
+{
+    entgrad2primgrad<double>(dim, gamma, quirks, s, g);
+}
+
+void orc_viscous_flux(int dim, double gamma, double Pr, double mu, const double *s,
+                      const double *dqdx, const double *dqdy, const double *dqdz, double *flux)
+{
+    const double *dq[3] = {dqdx, dqdy, dqdz};
+    viscous_flux<double>(dim, gamma, Pr, mu, s, dq, flux);
+}
+
+int orc_create(const orc_params *p, orc_ctx **out)
+{
+    if (!p || !out) return 1;
+    if (p->dim < 2 || p->dim > 3 || p->order < 1 || p->order > 12) return 2;
+    for (int r = 0; r < p->dim; r++)
+        if (p->nelem[r] < 1 || !(p->hi[r] > p->lo[r])) return 3;
+    orc_ctx *c = new (std::nothrow) orc_ctx();
+    if (!c) return 4;
+    c->p = *p;
+    lgl_rule(p->order, c->x, c->w);
+    dmats(p->order, c->x, c->w, c->D, c->Dh, c->Dh2);
+    build_mesh(c);
+    const Mesh &m = c->m;
+    c->alpha.assign(m.NE, 0.0);
+    c->eta.assign(m.NE, 0.0);
+    c->lam = -1.0; /* DGSEMIntegrator.cpp:39 */
+    c->thresh = 0.5 * std::pow(10.0, -1.8 * std::pow((double)p->order, 0.25)); /* DGSEMOperator.cpp:25 */
+    /* ModalBasis (ModalBasis.cpp:42-135): mode n <-> per-dim degrees,
+     * V(node, n) = prod_d P_deg(x_d); V^-1 = kron of the 1-D inverses */
+    const int Np = m.Np, K = m.K, dim = m.dim;
+    c->degs.assign(K * dim, 0);
+    for (int n = 0; n < K; n++) {
+        if (dim == 2) { c->degs[n * 2 + 0] = n / Np; c->degs[n * 2 + 1] = n % Np; }
+        else {
+            c->degs[n * 3 + 0] = n / (Np * Np);
+            c->degs[n * 3 + 1] = (n / Np) % Np;
+            c->degs[n * 3 + 2] = n % Np;
+        }
+    }
+    std::vector<double> V1(Np * Np), V1i(Np * Np);
+    orc_legendre_vdm(p->order, V1.data(), V1i.data());
+    c->Vinv.assign((size_t)K * K, 0.0);
+    for (int n = 0; n < K; n++)
+        for (int i = 0; i < K; i++) {
+            int id[3] = {i % Np, (i / Np) % Np, i / (Np * Np)};
+            double v = 1.0;
+            for (int d = 0; d < dim; d++) v *= V1i[c->degs[n * dim + d] * Np + id[d]];
+            c->Vinv[(size_t)n * K + i] = v;
+        }
+    c->wJ.assign((size_t)m.NE * K, 0.0);
+    for (int e = 0; e < m.NE; e++)
+        for (int i = 0; i < K; i++) {
+            double wt = c->w[i % Np] * c->w[(i / Np) % Np];
+            if (dim == 3) wt *= c->w[i / (Np * Np)];
+            c->wJ[(size_t)e * K + i] = wt * m.J[(size_t)e * K + i];
+        }
+    *out = c;
+    return 0;
+}
+
+void orc_destroy(orc_ctx *c) { delete c; }
+
+int orc_sizes(const orc_ctx *c, int *NE, int *K, int *nv)
+{
+    *NE = c->m.NE; *K = c->m.K; *nv = c->m.nv;
+    return 0;
+}
+
+int orc_coords(const orc_ctx *c, double *X)
+{
+    const Mesh &m = c->m;
+    const size_t NK = (size_t)m.NE * m.K;
+    for (int e = 0; e < m.NE; e++) {
+        int idx[3] = {e % m.n[0], (e / m.n[0]) % m.n[1], e / (m.n[0] * m.n[1])};
+        for (int i = 0; i < m.K; i++) {
+            int ii[3] = {i % m.Np, (i / m.Np) % m.Np, i / (m.Np * m.Np)};
+            for (int d = 0; d < m.dim; d++)
+                X[d * NK + (size_t)e * m.K + i] = m.lo[d] + (idx[d] + c->x[ii[d]]) * m.h[d];
+        }
+    }
+    return 0;
+}
+
+int orc_weights(const orc_ctx *c, double *WJ)
+{
+    std::memcpy(WJ, c->wJ.data(), c->wJ.size() * sizeof(double));
+    return 0;
+}
+
+int orc_rhs(orc_ctx *c, const double *u, double *dudt)
+{
+    Engine<double> E(c);
+    E.rhs(u, dudt, c->p.threads);
+    if (c->p.parabolic) c->grad.assign(E.G.begin(), E.G.end());
+    return 0;
+}
+
+int orc_rk3_step(orc_ctx *c, double *u, double dt)
+{
+    rk3_step(c, u, dt);
+    return 0;
+}
+
+double orc_max_wave_speed(const orc_ctx *c) { return c->lam; }
+void orc_reset_wave_speed(orc_ctx *c) { c->lam = -1.0; }
+double orc_hmin(const orc_ctx *c) { return c->m.hmin; }
+
+int orc_alpha(const orc_ctx *c, double *alpha)
+{
+    std::memcpy(alpha, c->alpha.data(), c->alpha.size() * sizeof(double));
+    return 0;
+}
+
+int orc_eta(const orc_ctx *c, double *eta)
+{
+    std::memcpy(eta, c->eta.data(), c->eta.size() * sizeof(double));
+    return 0;
+}
+
+int orc_last_gradients(const orc_ctx *c, double *grad)
+{
+    if (c->grad.empty()) return 1;
+    std::memcpy(grad, c->grad.data(), c->grad.size() * sizeof(double));
+    return 0;
+}
+
+int orc_count_rhs_flops(orc_ctx *c, const double *u, double *out)
+{
+    const size_t n = (size_t)c->m.nv * c->m.NE * c->m.K;
+    std::vector<Cnt> U(n), dU(n);
+    for (size_t i = 0; i < n; i++) U[i] = Cnt(u[i]);
+    for (int i = 0; i < 8; i++) g_ops[i] = 0;
+    double lam = c->lam;
+    std::vector<double> al = c->alpha, et = c->eta;
+    Engine<Cnt> E(c);
+    E.rhs(U.data(), dU.data(), 1);
+    c->lam = lam; c->alpha = al; c->eta = et;
+    long long tot = 0;
+    for (int i = 1; i < 8; i++) tot += g_ops[i];
+    out[0] = (double)tot;
+    for (int i = 1; i < 8; i++) out[i] = (double)g_ops[i];
+    return 0;
+}
+
+} /* extern "C" */

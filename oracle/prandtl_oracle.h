@@ -1,0 +1,77 @@
+/*
+ * TEST INFRASTRUCTURE — not part of the product.
+ *
+ * C ABI of the CPU FP64 restatement ("the oracle") of Prandtl's DGSEM
+ * right-hand side and SSP-RK3 step.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load this library, and only as the
+ * checker / CPU baseline; the product path never links or calls it.
+ *
+ * Layout of every state vector: byNODES SoA, index = var*NE*K + e*K + node,
+ * nodes lexicographic with x fastest (Simulation.cpp:338-346 in the
+ * reference; vdim = nv, Ordering::byNODES).
+ */
+#ifndef PRANDTL_ORACLE_H
+#define PRANDTL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+typedef struct orc_params {
+    int dim;            /* 2 or 3 */
+    int order;          /* polynomial degree N; Np = N+1 LGL nodes per direction */
+    int nelem[3];       /* periodic Cartesian mesh: elements per direction */
+    double lo[3], hi[3];/* box */
+    double gamma, Pr, mu;
+    int parabolic;      /* Navier-Stokes with BR1 lifting (PARABOLIC build) */
+    int blending;       /* SUBCELL_FV_BLENDING build */
+    double alpha_max, alpha_min;
+    int ec_surface;     /* 1: entropy-conservative F# at faces (entropy test) */
+    int quirks;         /* 1: reproduce the reference's defects, 0: corrected */
+    int threads;        /* OpenMP threads for the double instantiation (<=0: 1) */
+} orc_params;
+
+/* 1-D rules and operators (setup arithmetic of DGSEMIntegrator.cpp:8-82) */
+int orc_lgl(int order, double *x, double *w);
+int orc_dmat(int order, double *D, double *Dhat, double *Dhat2); /* row-major Np x Np */
+int orc_legendre_vdm(int order, double *V1, double *V1inv);      /* row-major 1-D shifted-Legendre Vandermonde */
+
+/* point physics (ChandrashekarFlux.cpp, BasicOperations.cpp, NavierStokesFlux.cpp) */
+double orc_logmean(double x, double y);
+double orc_volume_flux(int dim, double gamma, const double *s1, const double *s2,
+                       const double *m1, const double *m2, double *F);
+double orc_face_flux(int dim, double gamma, int quirks, const double *s1, const double *s2,
+                     const double *nor, double *F);
+void orc_cons2entropy(int dim, double gamma, const double *s, double *w);
+void orc_entgrad2primgrad(int dim, double gamma, int quirks, const double *s, double *g);
+void orc_viscous_flux(int dim, double gamma, double Pr, double mu, const double *s,
+                      const double *dqdx, const double *dqdy, const double *dqdz,
+                      double *flux /* nv x dim row-major */);
+
+/* operator */
+int orc_create(const orc_params *p, orc_ctx **out);
+void orc_destroy(orc_ctx *c);
+int orc_sizes(const orc_ctx *c, int *NE, int *K, int *nv);
+int orc_coords(const orc_ctx *c, double *X /* [dim][NE][K] */);
+int orc_weights(const orc_ctx *c, double *WJ /* [NE][K] omega_i * J_i */);
+int orc_rhs(orc_ctx *c, const double *u, double *dudt);
+int orc_rk3_step(orc_ctx *c, double *u, double dt);
+double orc_max_wave_speed(const orc_ctx *c);
+void orc_reset_wave_speed(orc_ctx *c);
+int orc_alpha(const orc_ctx *c, double *alpha /* NE */);
+int orc_eta(const orc_ctx *c, double *eta /* NE */);
+double orc_hmin(const orc_ctx *c);
+/* BR1 primitive gradients of the last parabolic RHS, [dim][nv][NE][K] */
+int orc_last_gradients(const orc_ctx *c, double *grad);
+/* algorithmic op count of one RHS on state u (counted instantiation):
+ * out[0]=total, [1]=indicator+alpha, [2]=entropy vars+lifting+grad conversion,
+ * [3]=viscous volume+face viscous fluxes, [4]=inviscid volume pairs,
+ * [5]=surface fluxes, [6]=FV subcell, [7]=assembly (scaling/sums) */
+int orc_count_rhs_flops(orc_ctx *c, const double *u, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,83 @@
+"""ctypes binding of libprandtl_hip.so (the C ABI declared in include/prandtl_hip.h).
+
+The HIP library is the product path: when it is missing this module raises,
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libprandtl_hip.so")
+
+PRANDTL_OK = 0
+PRANDTL_ERR_ARG = 1
+PRANDTL_ERR_UNSUPPORTED = 2
+PRANDTL_ERR_HIP = 3
+PRANDTL_ERR_STATE = 4
+PRANDTL_ERR_NOMEM = 5
+PRANDTL_FP64 = 0
+PRANDTL_MIXED = 1
+
+EXPORTED = (
+    "prandtl_create", "prandtl_destroy", "prandtl_sizes", "prandtl_rhs", "prandtl_rk3_step",
+    "prandtl_max_wave_speed", "prandtl_alpha", "prandtl_check_state", "prandtl_lgl",
+    "prandtl_debug_gradients", "prandtl_last_error",
+)
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("nelem", ctypes.c_int * 3),
+                ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("Pr", ctypes.c_double), ("mu", ctypes.c_double),
+                ("alpha_max", ctypes.c_double), ("alpha_min", ctypes.c_double),
+                ("parabolic", ctypes.c_int), ("blending", ctypes.c_int),
+                ("ec_surface", ctypes.c_int), ("quirks", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class PrandtlError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("prandtl error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libprandtl_hip.so is not built; run __graft_entry__.build()")
+    # torch ships its own libamdhip64.so.7; importing it first makes the
+    # dynamic loader resolve this library's libamdhip64.so.7 to that same
+    # runtime, so device pointers and streams are shared with torch.
+    import torch  # noqa: F401
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.prandtl_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.prandtl_destroy.argtypes = [vp]
+    L.prandtl_destroy.restype = None
+    L.prandtl_sizes.argtypes = [vp, ip, ip, ip]
+    L.prandtl_rhs.argtypes = [vp, vp, vp, vp]
+    L.prandtl_rk3_step.argtypes = [vp, vp, ctypes.c_double, vp]
+    L.prandtl_max_wave_speed.argtypes = [vp, dp, ctypes.c_int, vp]
+    L.prandtl_alpha.argtypes = [vp, dp, dp, vp]
+    L.prandtl_check_state.argtypes = [vp, vp, vp]
+    L.prandtl_lgl.argtypes = [ctypes.c_int, dp, dp]
+    L.prandtl_debug_gradients.argtypes = [vp, vp, vp, vp]
+    L.prandtl_last_error.argtypes = [vp]
+    L.prandtl_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc, ctx=None):
+    if rc != PRANDTL_OK:
+        msg = lib().prandtl_last_error(ctx)
+        raise PrandtlError(rc, msg.decode() if msg else "")
+    return rc

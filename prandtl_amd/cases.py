@@ -1,0 +1,184 @@
+"""Synthetic meshes and initial states of the benchmarked configurations.
+
+All states are numpy float64 arrays in byNODES layout [var][element][node]
+(element index (ez*ny + ey)*nx + ex, nodes lexicographic x fastest) on the
+periodic Cartesian boxes the operator builds.
+
+ - isentropic_vortex: 2-D Euler, Shu vortex of strength beta on [-5,5]^2
+   around the freestream rho=1, u=v=1, p=1.
+ - taylor_green: 3-D Taylor-Green vortex, rho=1, V0=1, L=1,
+   p = p0 + (cos 2x + cos 2y)(cos 2z + 2)/16, p0 = 1/(gamma Ma^2).
+ - kelvin_helmholtz: 2-D shear layer, B = tanh(s y + s/2) - tanh(s y - s/2),
+   rho = 0.5 + 0.75 B, u = 0.5 (B - 1), p = 1,
+   v = 0.1 sin(2 pi x) + 1e-3 (2 r - 1) with r from std::mt19937_64
+   (seed 20261202) as (x >> 11) * 2^-53, drawn in global node order.
+"""
+import numpy as np
+
+from .operator import lgl_nodes
+
+MASK64 = (1 << 64) - 1
+
+
+class MT19937_64:
+    """std::mt19937_64 (the C++ standard's 64-bit Mersenne Twister)."""
+
+    def __init__(self, seed=5489):
+        self.mt = [0] * 312
+        self.mt[0] = seed & MASK64
+        for i in range(1, 312):
+            self.mt[i] = (6364136223846793005 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 62)) + i) & MASK64
+        self.idx = 312
+
+    def _twist(self):
+        mt = self.mt
+        for i in range(312):
+            x = (mt[i] & 0xFFFFFFFF80000000) | (mt[(i + 1) % 312] & 0x7FFFFFFF)
+            xa = x >> 1
+            if x & 1:
+                xa ^= 0xB5026F5AA96619E9
+            mt[i] = mt[(i + 156) % 312] ^ xa
+        self.idx = 0
+
+    def __call__(self):
+        if self.idx >= 312:
+            self._twist()
+        y = self.mt[self.idx]
+        self.idx += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEDA60000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & MASK64
+
+    def uniform(self):
+        """(x >> 11) * 2^-53 in [0, 1)."""
+        return (self() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def node_coords(dim, order, nelem, lo, hi):
+    """Physical node coordinates X[dim][NE][K]."""
+    x, _ = lgl_nodes(order)
+    Np = order + 1
+    n = [int(nelem[r]) for r in range(dim)] + [1] * (3 - dim)
+    h = [(hi[r] - lo[r]) / n[r] for r in range(dim)]
+    NE = int(np.prod(n))
+    K = Np ** dim
+    e = np.arange(NE)
+    eidx = [e % n[0], (e // n[0]) % n[1], e // (n[0] * n[1])]
+    i = np.arange(K)
+    iidx = [i % Np, (i // Np) % Np, i // (Np * Np)]
+    X = np.zeros((dim, NE, K))
+    for r in range(dim):
+        X[r] = lo[r] + (eidx[r][:, None] + x[iidx[r]][None, :]) * h[r]
+    return X
+
+
+def prim_to_cons(rho, vel, p, gamma):
+    dim = len(vel)
+    U = np.zeros((dim + 2,) + np.shape(rho))
+    U[0] = rho
+    ke = np.zeros_like(rho)
+    for d in range(dim):
+        U[1 + d] = rho * vel[d]
+        ke = ke + vel[d] * vel[d]
+    U[dim + 1] = p / (gamma - 1.0) + 0.5 * rho * ke
+    return U
+
+
+def isentropic_vortex(X, gamma=1.4, beta=5.0, center=(0.0, 0.0), uinf=(1.0, 1.0)):
+    x = X[0] - center[0]
+    y = X[1] - center[1]
+    r2 = x * x + y * y
+    f = beta / (2.0 * np.pi) * np.exp(0.5 * (1.0 - r2))
+    u = uinf[0] - f * y
+    v = uinf[1] + f * x
+    T = 1.0 - (gamma - 1.0) * beta ** 2 / (8.0 * gamma * np.pi ** 2) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / (gamma - 1.0))
+    p = rho * T
+    return prim_to_cons(rho, [u, v], p, gamma)
+
+
+def isentropic_vortex_exact(X, t, lo, hi, gamma=1.4, beta=5.0, uinf=(1.0, 1.0)):
+    """Exact solution at time t: the vortex advected with uinf, periodically wrapped."""
+    L = [hi[0] - lo[0], hi[1] - lo[1]]
+    c = [uinf[0] * t, uinf[1] * t]
+    x = X[0] - c[0]
+    y = X[1] - c[1]
+    x = (x - lo[0]) % L[0] + lo[0]
+    y = (y - lo[1]) % L[1] + lo[1]
+    return isentropic_vortex(np.stack([x, y]), gamma, beta, (0.0, 0.0), uinf)
+
+
+def taylor_green(X, gamma=1.4, Ma=0.1):
+    x, y, z = X[0], X[1], X[2]
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(u)
+    p0 = 1.0 / (gamma * Ma * Ma)
+    p = p0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
+    rho = np.ones_like(u)
+    return prim_to_cons(rho, [u, v, w], p, gamma)
+
+
+def kelvin_helmholtz(X, gamma=1.4, s=15.0, seed=20261202, noise=1e-3):
+    x, y = X[0], X[1]
+    B = np.tanh(s * y + 0.5 * s) - np.tanh(s * y - 0.5 * s)
+    rho = 0.5 + 0.75 * B
+    u = 0.5 * (B - 1.0)
+    g = MT19937_64(seed)
+    r = np.array([g.uniform() for _ in range(x.size)]).reshape(x.shape)
+    v = 0.1 * np.sin(2.0 * np.pi * x) + noise * (2.0 * r - 1.0)
+    p = np.ones_like(x)
+    return prim_to_cons(rho, [u, v], p, gamma)
+
+
+def random_smooth_state(X, dim, seed, amp=0.2, gamma=1.4, rho0=1.0, p0=1.0, L=None):
+    """Smooth positive periodic state: a few random Fourier modes on each primitive."""
+    rng = np.random.default_rng(seed)
+    if L is None:
+        L = [X[r].max() - X[r].min() for r in range(dim)]
+
+    def field(scale):
+        f = np.zeros_like(X[0])
+        for _ in range(3):
+            k = rng.integers(1, 3, size=dim)
+            ph = rng.uniform(0, 2 * np.pi)
+            arg = ph + sum(2 * np.pi * k[r] * X[r] / L[r] for r in range(dim))
+            f = f + rng.uniform(-1, 1) * np.sin(arg)
+        return scale * f / 3.0
+
+    rho = rho0 * (1.0 + field(amp))
+    vel = [field(1.0) for _ in range(dim)]
+    p = p0 * (1.0 + field(amp))
+    return prim_to_cons(rho, vel, p, gamma)
+
+
+def kinetic_energy(U, WJ, volume):
+    """E_k = (1/|Omega|) sum_e sum_i w_i J_i rho |v|^2 / 2 (LGL quadrature)."""
+    dim = U.shape[0] - 2
+    ke = 0.5 * sum(U[1 + d] ** 2 for d in range(dim)) / U[0]
+    return float(np.sum(WJ * ke) / volume)
+
+
+def kinetic_energy_rate(U, dU, WJ, volume):
+    """dE_k/dt from the RHS: integral of v.d(rho v)/dt - |v|^2/2 drho/dt."""
+    dim = U.shape[0] - 2
+    rho = U[0]
+    vel = [U[1 + d] / rho for d in range(dim)]
+    integrand = sum(vel[d] * dU[1 + d] for d in range(dim)) - 0.5 * sum(v * v for v in vel) * dU[0]
+    return float(np.sum(WJ * integrand) / volume)
+
+
+def quadrature_weights(dim, order, nelem, lo, hi):
+    """w_i J_i per node, [NE][K]."""
+    _, w = lgl_nodes(order)
+    Np = order + 1
+    h = [(hi[r] - lo[r]) / nelem[r] for r in range(dim)]
+    J = h[0] * h[1] if dim == 2 else h[0] * (h[1] * h[2])
+    i = np.arange(Np ** dim)
+    wt = w[i % Np] * w[(i // Np) % Np]
+    if dim == 3:
+        wt = wt * w[i // (Np * Np)]
+    NE = int(np.prod([nelem[r] for r in range(dim)]))
+    return np.broadcast_to(wt * J, (NE, Np ** dim)).copy()

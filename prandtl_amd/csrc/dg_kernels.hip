@@ -1,0 +1,1085 @@
+// DGSEM right-hand side and fused SSP-RK3 stage update for MI355X (gfx950).
+//
+// Two kernels per RHS evaluation:
+//
+//  k_visc (parabolic only)  u -> BR1 lifting of the entropy variables,
+//      EntropyGrad2PrimGrad, viscous flux, contravariant transform and the
+//      weak-form derivative sum D^ Fv of the volume term; writes the raw
+//      viscous volume sum (nv-1 per node) and the face-normal component of
+//      the viscous flux at every face node of the element (nv-1 per face
+//      node).  Restates DGSEMOperator.cpp:82-159, LiftingBR1.cpp:10-272 and
+//      DGSEMIntegrator.cpp:617-641,705-738 (and 435-549 for the face side).
+//
+//  k_rhs  the inviscid flux differencing, surface fluxes, modal indicator and
+//      subcell FV blending, plus the viscous volume/face terms from k_visc,
+//      then either writes the RHS (Operator::Mult) or the RK3 stage
+//      combination.  Restates DGSEMIntegrator.cpp:151-433,435-549,601-864,
+//      972-1196, PerssonPeraireIndicator.cpp:21-58, DGSEMOperator.cpp:60-80
+//      and MFEM's RK3SSPSolver stage arithmetic.
+//
+// Work decomposition: a workgroup owns EPB whole elements (one node per
+// thread in the node phases).  For each direction the element's two-point
+// fluxes are "jobs" — symmetric volume pairs (computed once, as the
+// reference does), surface points and subcell interfaces — computed by all
+// threads into LDS, then reduced per node in the reference's summation
+// order.  Face jobs are computed by both neighbouring elements ("owner
+// computes") with the face's canonical orientation, so each face point's flux
+// is bitwise the same on both sides and no atomics or scatter are needed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/prandtl_hip.h"
+#include "dg_physics.h"
+
+namespace prandtl {
+
+constexpr int MAXNP = 8;
+
+struct Tables {
+    double D[MAXNP * MAXNP];    // D_ij (row-major), DGSEMIntegrator.cpp:58-69
+    double Dh[MAXNP * MAXNP];   // D + E            (lines 71-74, un-transposed)
+    double Dh2[MAXNP * MAXNP];  // 2D + E           (lines 76-80, un-transposed)
+    double V1i[MAXNP * MAXNP];  // inverse 1-D shifted-Legendre Vandermonde
+    double fvs[MAXNP];          // 1/(J w_i)            (line 1046)
+    double subn[3][MAXNP];      // subcell normal magnitude per interface (1199-1304)
+};
+
+struct Geo {
+    int n[3];
+    int NE;
+    double nrm[3];   // adj(r,r) = face measure of an r-face
+    double invJ;     // 1/J
+    double aface;    // 1/(w_0 J)
+};
+
+struct KArgs {
+    Tables tab;
+    Geo geo;
+    Phys phys;
+    int blend, parabolic, ec, quirks;
+    int mode;             // 0: out = rhs; 1: out = y + dt rhs; 2: out = c0 u0 + c1 (y + dt rhs)
+    double dt, c0, c1;
+    const double *y;
+    const double *u0;
+    double *out;
+    const double *visc;   // [nv-1][NE][K]
+    const double *fvf;    // [NE][2 dim][nv-1][Nfp]
+    double *alpha;        // [NE]
+    double *eta;          // [NE]
+    unsigned long long *lam;
+};
+
+struct VArgs {
+    Tables tab;
+    Geo geo;
+    Phys phys;
+    int quirks;
+    const double *y;
+    double *visc;
+    double *fvf;
+    double *grad;         // optional [dim][nv][NE][K]
+};
+
+template <int DIM, int NP> struct Cfg {
+    static constexpr int NV = DIM + 2;
+    static constexpr int K = DIM == 2 ? NP * NP : NP * NP * NP;
+    static constexpr int NFP = DIM == 2 ? NP : NP * NP;
+    static constexpr int NPAIR = NP * (NP - 1) / 2;
+    static constexpr int BLOCK = K > 256 ? ((K + 63) / 64) * 64 : 256;
+    static constexpr int EPB = BLOCK / K;
+    static constexpr int JPL = NPAIR + 2 + (NP - 1);
+    static constexpr int lc_pick()
+    {
+        for (int lc = NFP; lc >= 1; lc--)
+            if (NFP % lc == 0 && EPB * lc * JPL * NV * 8 <= 48 * 1024) return lc;
+        return 1;
+    }
+    static constexpr int LC = lc_pick();
+    static constexpr int FXN = LC * JPL;
+};
+
+template <int DIM, int NP>
+__device__ __forceinline__ int line_node(int r, int l, int m)
+{
+    if (DIM == 2) return r == 0 ? l * NP + m : m * NP + l;
+    const int a = l % NP, b = l / NP;
+    if (r == 0) return (b * NP + a) * NP + m;
+    if (r == 1) return (b * NP + m) * NP + a;
+    return (m * NP + b) * NP + a;
+}
+
+template <int DIM, int NP>
+__device__ __forceinline__ int line_of(int r, int i, int j, int k)
+{
+    if (DIM == 2) return r == 0 ? j : i;
+    if (r == 0) return k * NP + j;
+    if (r == 1) return k * NP + i;
+    return j * NP + i;
+}
+
+template <int NP> __device__ __forceinline__ int pair_index(int a, int b)
+{
+    // a < b; pairs ordered (0,1),(0,2),...,(0,N),(1,2),...
+    return a * NP - (a * (a + 1)) / 2 + (b - a - 1);
+}
+
+template <int NP> __device__ __forceinline__ void pair_ab(int p, int &a, int &b)
+{
+    a = 0;
+    int rem = p;
+    while (rem >= NP - 1 - a) { rem -= NP - 1 - a; a++; }
+    b = a + 1 + rem;
+}
+
+__device__ __forceinline__ int xcd_remap(int bid, int nb)
+{
+    // blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of element groups so face neighbours share its L2
+    const int q = nb >> 3, rr = nb & 7;
+    const int x = bid & 7, k = bid >> 3;
+    return (x < rr) ? x * (q + 1) + k : rr * (q + 1) + (x - rr) * q + k;
+}
+
+__device__ __forceinline__ int neighbor(const Geo &g, int e, int r, int side)
+{
+    int idx[3] = {e % g.n[0], (e / g.n[0]) % g.n[1], e / (g.n[0] * g.n[1])};
+    idx[r] = side ? (idx[r] + 1 == g.n[r] ? 0 : idx[r] + 1) : (idx[r] == 0 ? g.n[r] - 1 : idx[r] - 1);
+    return (idx[2] * g.n[1] + idx[1]) * g.n[0] + idx[0];
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (v < o) ? o : v;
+    }
+    return v;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void block_lam_max(double lam, unsigned long long *out, double *sred)
+{
+    lam = wave_max(lam);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sred[w] = lam;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sred[0];
+        for (int i = 1; i < (BLOCK + 63) / 64; i++) m = (m < sred[i]) ? sred[i] : m;
+        if (m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_rhs
+// ---------------------------------------------------------------------------
+template <int DIM, int NP>
+__global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
+{
+    using C = Cfg<DIM, NP>;
+    constexpr int NV = C::NV, K = C::K, NFP = C::NFP, NPAIR = C::NPAIR;
+    constexpr int EPB = C::EPB, LC = C::LC, BLOCK = C::BLOCK, FXN = C::FXN;
+    constexpr int GK = DIM == 3 ? K : 1;
+
+    __shared__ double sDh2[NP * NP];
+    __shared__ double sV1i[NP * NP];
+    __shared__ double sFvs[NP];
+    __shared__ double sSub[DIM][NP];
+    __shared__ double sU[EPB][NV][K];
+    __shared__ double sQ[EPB][DIM + 3][K];
+    __shared__ double sF[EPB][NV][FXN];
+    __shared__ double sG[EPB][NV][GK];
+    __shared__ double sAl[EPB];
+    __shared__ double sRed[(BLOCK + 63) / 64];
+
+    const int t = threadIdx.x;
+    for (int i = t; i < NP * NP; i += BLOCK) {
+        sDh2[i] = A.tab.Dh2[(i / NP) * MAXNP + i % NP];
+        sV1i[i] = A.tab.V1i[(i / NP) * MAXNP + i % NP];
+    }
+    for (int i = t; i < NP; i += BLOCK) sFvs[i] = A.tab.fvs[i];
+    for (int i = t; i < DIM * NP; i += BLOCK) sSub[i / NP][i % NP] = A.tab.subn[i / NP][i % NP];
+
+    const Phys &P = A.phys;
+    const int NE = A.geo.NE;
+    const size_t NK = (size_t)NE * K;
+    const int ebase = xcd_remap(blockIdx.x, gridDim.x) * EPB;
+    const int el = t / K;
+    const int n = t - el * K;
+    const int e = ebase + el;
+    const bool act = (el < EPB) && (e < NE);
+    const bool quirkNS3 = (DIM == 3) && A.parabolic && A.quirks;
+    const bool quirk3d = A.quirks != 0;
+    const int ii[3] = {n % NP, (n / NP) % NP, DIM == 3 ? n / (NP * NP) : 0};
+
+    // ---- node phase: state and per-node flux quantities into LDS ----
+    double lam = 0.0;
+    double ys[NV];
+    if (act) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            ys[v] = A.y[v * NK + (size_t)e * K + n];
+            sU[el][v][n] = ys[v];
+        }
+        const NodeQ<DIM> q = node_quants<DIM>(ys, P);
+#pragma unroll
+        for (int d = 0; d < DIM; d++) sQ[el][d][n] = q.v[d];
+        sQ[el][DIM][n] = q.p;
+        sQ[el][DIM + 1][n] = q.beta;
+        sQ[el][DIM + 2][n] = q.lam;
+        lam = (q.lam == q.lam) ? q.lam : 0.0;
+        if (quirkNS3 && ii[1] == NP - 1)
+#pragma unroll
+            for (int v = 0; v < NV; v++) sG[el][v][n] = 0.0;
+    }
+    __syncthreads();
+
+    // ---- modal indicator and blending coefficient (element-local) ----
+    if (A.blend) {
+        double *sX = &sF[0][0][0];
+        double *sY = sX + EPB * K;
+        if (act) {
+            // rho * ComputePressure(state) with V /= rho (BasicOperations.cpp:31-37)
+            const double rr = 1.0 / ys[0];
+            double vv = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                const double v = ys[1 + d] * rr;
+                vv = (d == 0) ? v * v : vv + v * v;
+            }
+            sX[el * K + n] = ys[0] * (P.gm1 * (ys[DIM + 1] - 0.5 * ys[0] * vv));
+        }
+        __syncthreads();
+        // modes = V^-1 q, V^-1 = kron of the 1-D inverses, one direction at a time
+        double *src = sX, *dst = sY;
+#pragma unroll
+        for (int r = 0; r < DIM; r++) {
+            if (act) {
+                const int a = ii[r];
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < NP; i++) {
+                    int jj[3] = {ii[0], ii[1], ii[2]};
+                    jj[r] = i;
+                    const int m = DIM == 2 ? jj[1] * NP + jj[0] : (jj[2] * NP + jj[1]) * NP + jj[0];
+                    s = (i == 0) ? sV1i[a * NP + i] * src[el * K + m] : s + sV1i[a * NP + i] * src[el * K + m];
+                }
+                dst[el * K + n] = s;
+            }
+            __syncthreads();
+            double *tmp = src; src = dst; dst = tmp;
+        }
+        if (act && n == 0) {
+            // energies in the reference's mode order n = (dx*Np + dy)*Np + dz
+            // (ModalBasis.cpp:72-104), masks of PerssonPeraireIndicator.cpp:38-52
+            double mm = 0.0, m1 = 0.0, m2 = 0.0;
+            for (int nm = 0; nm < K; nm++) {
+                int dg[3];
+                if (DIM == 2) { dg[0] = nm / NP; dg[1] = nm % NP; dg[2] = 0; }
+                else { dg[0] = nm / (NP * NP); dg[1] = (nm / NP) % NP; dg[2] = nm % NP; }
+                const int pos = DIM == 2 ? dg[1] * NP + dg[0] : (dg[2] * NP + dg[1]) * NP + dg[0];
+                const double md = src[el * K + pos];
+                bool in1 = true, in2 = true;
+#pragma unroll
+                for (int d = 0; d < DIM; d++) {
+                    if (dg[d] > NP - 3) in2 = false;
+                    if (dg[d] > NP - 2) in1 = false;
+                }
+                mm += md * md;
+                if (in1) m1 += md * md;
+                if (in2) m2 += md * md;
+            }
+            double eta = 1.0 - m1 / mm;
+            const double eta2 = 1.0 - m2 / m1;
+            eta = (eta < eta2) ? eta2 : eta;
+            // DGSEMOperator.cpp:67-77, sharpness 9.21024 (DGSEMOperator.hpp:50)
+            double al = 1.0 / (1.0 + exp(-9.21024 * (eta - P.thresh) / P.thresh));
+            if (al < P.alpha_min) al = 0.0;
+            else if (al > 1.0 - P.alpha_min) al = 1.0;
+            al = (P.alpha_max < al) ? P.alpha_max : al;
+            sAl[el] = al;
+            A.alpha[e] = al;
+            A.eta[e] = eta;
+        }
+        __syncthreads();
+    }
+    const double alpha_e = (A.blend && act) ? sAl[el] : 0.0;
+
+    double accV[NV], accS[NV], accF[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) { accV[v] = 0.0; accS[v] = 0.0; accF[v] = 0.0; }
+
+    const int nfv = A.blend ? NP - 1 : 0;
+    for (int r = 0; r < DIM; r++) {
+        const double nr = A.geo.nrm[r];
+        for (int l0 = 0; l0 < NFP; l0 += LC) {
+            const int npj = EPB * LC * NPAIR, nfj = EPB * LC * 2, nvj = EPB * LC * nfv;
+            for (int j = t; j < npj + nfj + nvj; j += BLOCK) {
+                if (j < npj) {
+                    // ---- symmetric volume pair (a < b) on line l ----
+                    const int ej = j / (LC * NPAIR);
+                    const int rem = j - ej * (LC * NPAIR);
+                    const int lc = rem / NPAIR, p = rem - lc * NPAIR;
+                    if (ebase + ej >= NE) continue;
+                    int a, b;
+                    pair_ab<NP>(p, a, b);
+                    const int na = line_node<DIM, NP>(r, l0 + lc, a);
+                    const int nb = line_node<DIM, NP>(r, l0 + lc, b);
+                    NodeQ<DIM> qa, qb;
+                    qa.rho = sU[ej][0][na]; qb.rho = sU[ej][0][nb];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][na]; qb.v[d] = sQ[ej][d][nb]; }
+                    qa.beta = sQ[ej][DIM + 1][na]; qb.beta = sQ[ej][DIM + 1][nb];
+                    double F[NV];
+                    two_point_flux<DIM, false>(r, nr, qa, qb, nullptr, nullptr, P, false, F);
+#pragma unroll
+                    for (int v = 0; v < NV; v++) sF[ej][v][lc * NPAIR + p] = F[v];
+                    if (quirkNS3 && r == 1 && b == NP - 1)
+#pragma unroll
+                        for (int v = 0; v < NV; v++) sG[ej][v][na] = F[v];
+                } else if (j < npj + nfj) {
+                    // ---- surface point: side 0 = -r face, 1 = +r face ----
+                    const int jf = j - npj;
+                    const int ej = jf / (LC * 2);
+                    const int rem = jf - ej * (LC * 2);
+                    const int lc = rem >> 1, side = rem & 1;
+                    const int eo = ebase + ej;
+                    if (eo >= NE) continue;
+                    const int l = l0 + lc;
+                    const int no = line_node<DIM, NP>(r, l, side ? NP - 1 : 0);
+                    const int en = neighbor(A.geo, eo, r, side);
+                    const int nn = line_node<DIM, NP>(r, l, side ? 0 : NP - 1);
+                    double so[NV], sn[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        so[v] = sU[ej][v][no];
+                        sn[v] = A.y[v * NK + (size_t)en * K + nn];
+                    }
+                    // canonical orientation: elem1 = low side, normal +e_r
+                    double *s1 = side ? so : sn;
+                    double *s2 = side ? sn : so;
+                    double fvn[NV];
+                    fvn[0] = 0.0;
+                    if (A.parabolic) {
+                        const double *fo = A.fvf + (((size_t)eo * 2 * DIM + 2 * r + side) * (NV - 1)) * NFP;
+                        const double *fn = A.fvf + (((size_t)en * 2 * DIM + 2 * r + (1 - side)) * (NV - 1)) * NFP;
+#pragma unroll
+                        for (int v = 1; v < NV; v++)
+                            fvn[v] = ((fo[(v - 1) * NFP + l] + fn[(v - 1) * NFP + l]) * 0.5) * nr;
+                        if (A.quirks) {
+                            // ComputeViscousFlux left velocities in the momentum
+                            // slots of state1/state2 (DGSEMIntegrator.cpp:471-481)
+                            const double r1 = 1.0 / s1[0], r2 = 1.0 / s2[0];
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) { s1[1 + d] *= r1; s2[1 + d] *= r2; }
+                        }
+                    }
+                    const NodeQ<DIM> q1 = node_quants<DIM>(s1, P);
+                    const NodeQ<DIM> q2 = node_quants<DIM>(s2, P);
+                    double F[NV];
+                    if (A.ec) two_point_flux<DIM, false>(r, nr, q1, q2, nullptr, nullptr, P, false, F);
+                    else two_point_flux<DIM, true>(r, nr, q1, q2, s1, s2, P, quirk3d, F);
+                    const int slot = LC * NPAIR + lc * 2 + side;
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        double d;
+                        if (A.parabolic) d = side ? (-F[v] + fvn[v]) : (F[v] - fvn[v]);
+                        else d = side ? -F[v] : F[v];
+                        sF[ej][v][slot] = d;
+                    }
+                } else {
+                    // ---- subcell FV interface i+1/2 on line l ----
+                    const int jv = j - npj - nfj;
+                    const int ej = jv / (LC * nfv);
+                    const int rem = jv - ej * (LC * nfv);
+                    const int lc = rem / nfv, i = rem - lc * nfv;
+                    if (ebase + ej >= NE) continue;
+                    const int slot = LC * (NPAIR + 2) + lc * (NP - 1) + i;
+                    if (!(sAl[ej] > 0.0)) {
+#pragma unroll
+                        for (int v = 0; v < NV; v++) sF[ej][v][slot] = 0.0;
+                        continue;
+                    }
+                    const int na = line_node<DIM, NP>(r, l0 + lc, i);
+                    const int nb = line_node<DIM, NP>(r, l0 + lc, i + 1);
+                    double s1[NV], s2[NV];
+                    NodeQ<DIM> qa, qb;
+#pragma unroll
+                    for (int v = 0; v < NV; v++) { s1[v] = sU[ej][v][na]; s2[v] = sU[ej][v][nb]; }
+                    qa.rho = s1[0]; qb.rho = s2[0];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][na]; qb.v[d] = sQ[ej][d][nb]; }
+                    qa.p = sQ[ej][DIM][na]; qb.p = sQ[ej][DIM][nb];
+                    qa.beta = sQ[ej][DIM + 1][na]; qb.beta = sQ[ej][DIM + 1][nb];
+                    qa.lam = sQ[ej][DIM + 2][na]; qb.lam = sQ[ej][DIM + 2][nb];
+                    double F[NV];
+                    two_point_flux<DIM, true>(r, sSub[r][i], qa, qb, s1, s2, P, quirk3d, F);
+#pragma unroll
+                    for (int v = 0; v < NV; v++) sF[ej][v][slot] = F[v];
+                }
+            }
+            __syncthreads();
+            if (act) {
+                const int l = line_of<DIM, NP>(r, ii[0], ii[1], ii[2]);
+                if (l >= l0 && l < l0 + LC) {
+                    const int lc = l - l0;
+                    const int a = ii[r];
+#pragma unroll
+                    for (int m = 0; m < NP; m++) {
+                        if (m == a) continue;
+                        const double dm = sDh2[a * NP + m];
+                        if (quirkNS3 && r == 2 && m < a) {
+                            // H_inviscid(id2).SetCol(k, g) (DGSEMIntegrator.cpp:702)
+                            const int nm = line_node<DIM, NP>(r, l, m);
+#pragma unroll
+                            for (int v = 0; v < NV; v++) accV[v] += dm * sG[el][v][nm];
+                        } else {
+                            const int p = m < a ? pair_index<NP>(m, a) : pair_index<NP>(a, m);
+#pragma unroll
+                            for (int v = 0; v < NV; v++) accV[v] += dm * sF[el][v][lc * NPAIR + p];
+                        }
+                    }
+                    if (a == 0)
+#pragma unroll
+                        for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sF[el][v][LC * NPAIR + lc * 2 + 0];
+                    if (a == NP - 1)
+#pragma unroll
+                        for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sF[el][v][LC * NPAIR + lc * 2 + 1];
+                    if (A.blend && alpha_e > 0.0) {
+                        const int base = LC * (NPAIR + 2) + lc * (NP - 1);
+#pragma unroll
+                        for (int v = 0; v < NV; v++) {
+                            double val;
+                            if (a == 0) val = (0.0 - sF[el][v][base + 0]) * sFvs[0];
+                            else if (a == NP - 1) val = sF[el][v][base + NP - 2] * sFvs[NP - 1];
+                            else val = (sF[el][v][base + a - 1] - sF[el][v][base + a]) * sFvs[a];
+                            accF[v] = (r == 0) ? val : accF[v] + val;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- assemble, stage update, store ----
+    if (act) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            double vol = -accV[v];
+            if (A.blend) vol *= (1.0 - alpha_e);
+            if (A.parabolic && v > 0) vol = vol + A.visc[(v - 1) * NK + (size_t)e * K + n];
+            vol *= A.geo.invJ;
+            double tot = (A.blend && alpha_e > 0.0) ? accF[v] * alpha_e + vol : vol;
+            tot += accS[v];
+            const size_t gi = v * NK + (size_t)e * K + n;
+            double o;
+            if (A.mode == 0) o = tot;
+            else if (A.mode == 1) o = ys[v] + A.dt * tot;
+            else o = A.c0 * A.u0[gi] + A.c1 * (ys[v] + A.dt * tot);
+            A.out[gi] = o;
+        }
+    }
+    block_lam_max<BLOCK>(lam, A.lam, sRed);
+}
+
+// ---------------------------------------------------------------------------
+// k_visc: entropy variables, BR1 lifting, gradient conversion, viscous fluxes
+// ---------------------------------------------------------------------------
+template <int DIM, int NP>
+__global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_visc(const VArgs A)
+{
+    using C = Cfg<DIM, NP>;
+    constexpr int NV = C::NV, K = C::K, NFP = C::NFP, EPB = C::EPB, BLOCK = C::BLOCK;
+
+    __shared__ double sD[NP * NP];
+    __shared__ double sDh[NP * NP];
+    __shared__ double sU[EPB][NV][K];
+    __shared__ double sW[EPB][NV][K];
+    __shared__ double sWN[EPB][2 * DIM][NV][NFP];
+    __shared__ double sFV[EPB][DIM][NV - 1][K];
+
+    const int t = threadIdx.x;
+    for (int i = t; i < NP * NP; i += BLOCK) {
+        sD[i] = A.tab.D[(i / NP) * MAXNP + i % NP];
+        sDh[i] = A.tab.Dh[(i / NP) * MAXNP + i % NP];
+    }
+    const Phys &P = A.phys;
+    const int NE = A.geo.NE;
+    const size_t NK = (size_t)NE * K;
+    const int ebase = xcd_remap(blockIdx.x, gridDim.x) * EPB;
+    const int el = t / K;
+    const int n = t - el * K;
+    const int e = ebase + el;
+    const bool act = (el < EPB) && (e < NE);
+    const int ii[3] = {n % NP, (n / NP) % NP, DIM == 3 ? n / (NP * NP) : 0};
+
+    double ys[NV];
+    if (act) {
+        double w[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            ys[v] = A.y[v * NK + (size_t)e * K + n];
+            sU[el][v][n] = ys[v];
+        }
+        cons2entropy<DIM>(ys, P, w);
+#pragma unroll
+        for (int v = 0; v < NV; v++) sW[el][v][n] = w[v];
+    }
+    // neighbour traces of the entropy variables
+    for (int j = t; j < EPB * 2 * DIM * NFP; j += BLOCK) {
+        const int ej = j / (2 * DIM * NFP);
+        const int rem = j - ej * (2 * DIM * NFP);
+        const int f = rem / NFP, q = rem - f * NFP;
+        const int eo = ebase + ej;
+        if (eo >= NE) continue;
+        const int r = f >> 1, side = f & 1;
+        const int en = neighbor(A.geo, eo, r, side);
+        const int nn = line_node<DIM, NP>(r, q, side ? 0 : NP - 1);
+        double s[NV], w[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) s[v] = A.y[v * NK + (size_t)en * K + nn];
+        cons2entropy<DIM>(s, P, w);
+#pragma unroll
+        for (int v = 0; v < NV; v++) sWN[ej][f][v][q] = w[v];
+    }
+    __syncthreads();
+
+    if (act) {
+        // volume part: grad_r w = sum_l D_{i_r l} w_l, times adj / J
+        // (LiftingBR1.cpp:176-215)
+        double gp[DIM][NV];
+#pragma unroll
+        for (int c = 0; c < DIM; c++) {
+            const int a = ii[c];
+            const int l = line_of<DIM, NP>(c, ii[0], ii[1], ii[2]);
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                double s = 0.0;
+#pragma unroll
+                for (int m = 0; m < NP; m++)
+                    s = (m == 0) ? sW[el][v][line_node<DIM, NP>(c, l, m)] * sD[a * NP + m]
+                                 : s + sW[el][v][line_node<DIM, NP>(c, l, m)] * sD[a * NP + m];
+                gp[c][v] = (s * A.geo.nrm[c]) * A.geo.invJ;
+            }
+            // surface part: +1/2 (w2 - w1) n / (w0 J) on both sides (LiftingBR1.cpp:37-65)
+            if (a == 0 || a == NP - 1) {
+                const int side = (a == NP - 1);
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    const double wo = sW[el][v][n];
+                    const double wn = sWN[el][2 * c + side][v][l];
+                    const double f = side ? 0.5 * (wn - wo) : 0.5 * (wo - wn);
+                    gp[c][v] += A.geo.aface * (f * A.geo.nrm[c]);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < DIM; c++) entgrad2primgrad<DIM>(ys, P, A.quirks != 0, gp[c]);
+        if (A.grad)
+#pragma unroll
+            for (int c = 0; c < DIM; c++)
+#pragma unroll
+                for (int v = 0; v < NV; v++) A.grad[(c * NV + v) * NK + (size_t)e * K + n] = gp[c][v];
+
+        double fl[DIM + 1][DIM];
+        // volume viscous flux uses the stale node K-1 state (DGSEMIntegrator.cpp:620)
+        double sv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) sv[v] = A.quirks ? sU[el][v][K - 1] : ys[v];
+        viscous_flux<DIM>(sv, gp, P, fl);
+#pragma unroll
+        for (int r = 0; r < DIM; r++)
+#pragma unroll
+            for (int v = 1; v < NV; v++) sFV[el][r][v - 1][n] = A.geo.nrm[r] * fl[v - 1][r];
+        bool onface = false;
+#pragma unroll
+        for (int c = 0; c < DIM; c++) onface |= (ii[c] == 0 || ii[c] == NP - 1);
+        if (onface) {
+            if (A.quirks) viscous_flux<DIM>(ys, gp, P, fl);
+#pragma unroll
+            for (int c = 0; c < DIM; c++) {
+                if (ii[c] == 0 || ii[c] == NP - 1) {
+                    const int side = (ii[c] == NP - 1);
+                    const int l = line_of<DIM, NP>(c, ii[0], ii[1], ii[2]);
+                    double *dst = A.fvf + (((size_t)e * 2 * DIM + 2 * c + side) * (NV - 1)) * NFP;
+#pragma unroll
+                    for (int v = 1; v < NV; v++) dst[(v - 1) * NFP + l] = fl[v - 1][c];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (act) {
+        // sum_r sum_l Dhat_{i_r l} (adj Fv)^r_l  (DGSEMIntegrator.cpp:705-738)
+        double acc[NV - 1];
+#pragma unroll
+        for (int v = 0; v < NV - 1; v++) acc[v] = 0.0;
+#pragma unroll
+        for (int r = 0; r < DIM; r++) {
+            const int a = ii[r];
+            const int l = line_of<DIM, NP>(r, ii[0], ii[1], ii[2]);
+#pragma unroll
+            for (int m = 0; m < NP; m++) {
+                const int nm = line_node<DIM, NP>(r, l, m);
+                const double dm = sDh[a * NP + m];
+#pragma unroll
+                for (int v = 0; v < NV - 1; v++) acc[v] += sFV[el][r][v][nm] * dm;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV - 1; v++) A.visc[v * NK + (size_t)e * K + n] = acc[v];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// small utility kernels
+// ---------------------------------------------------------------------------
+__global__ void k_reset_u64(unsigned long long *p) { *p = 0ull; }
+__global__ void k_reset_int(int *p) { *p = 0; }
+
+template <int DIM>
+__global__ void k_check(const double *u, size_t NK, double gm1, int *flag)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NK) return;
+    double s[DIM + 2];
+    bool bad = false;
+    for (int v = 0; v < DIM + 2; v++) {
+        s[v] = u[v * NK + i];
+        bad |= !isfinite(s[v]);
+    }
+    double vv = 0.0;
+    for (int d = 0; d < DIM; d++) vv += (s[1 + d] / s[0]) * (s[1 + d] / s[0]);
+    const double p = gm1 * (s[DIM + 1] - 0.5 * s[0] * vv);
+    bad |= !(s[0] > 0.0) || !(p > 0.0);
+    if (bad) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace host {
+
+// LGL nodes/weights on [0,1] (the rule of DGSEMIntegrator.cpp:18-22):
+// Newton on P_N' in long double from Chebyshev-Lobatto guesses.
+void lgl(int N, std::vector<double> &x, std::vector<double> &w)
+{
+    const int Np = N + 1;
+    std::vector<long double> z(Np), wz(Np);
+    for (int i = 0; i < Np; i++) {
+        long double t = (i == 0) ? -1.0L : (i == N ? 1.0L : -std::cos((long double)M_PI * i / N));
+        for (int it = 0; it < 100 && i > 0 && i < N; it++) {
+            long double p0 = 1.0L, p1 = t, d0 = 0.0L, d1 = 1.0L;
+            for (int k = 1; k < N; k++) {
+                long double p2 = ((2 * k + 1) * t * p1 - k * p0) / (k + 1);
+                long double d2 = d0 + (2 * k + 1) * p1;
+                p0 = p1; p1 = p2; d0 = d1; d1 = d2;
+            }
+            const long double PN = (N == 1) ? t : p1, dPN = (N == 1) ? 1.0L : d1;
+            const long double dd = (2.0L * t * dPN - N * (N + 1) * PN) / (1.0L - t * t);
+            const long double step = dPN / dd;
+            t -= step;
+            if (std::fabs((double)step) < 1e-22) break;
+        }
+        long double p0 = 1.0L, p1 = t;
+        for (int k = 1; k < N; k++) {
+            long double p2 = ((2 * k + 1) * t * p1 - k * p0) / (k + 1);
+            p0 = p1; p1 = p2;
+        }
+        const long double PN = (N == 1) ? t : p1;
+        z[i] = t;
+        wz[i] = 2.0L / (N * (N + 1) * PN * PN);
+    }
+    x.resize(Np);
+    w.resize(Np);
+    for (int i = 0; i < Np; i++) {
+        x[i] = (double)(0.5L * (1.0L + 0.5L * (z[i] - z[N - i])));
+        w[i] = (double)(0.25L * (wz[i] + wz[N - i]));
+    }
+    x[0] = 0.0;
+    x[N] = 1.0;
+    if (Np % 2 == 1) x[N / 2] = 0.5;
+}
+
+bool invert(int n, const std::vector<double> &A, std::vector<double> &Ai)
+{
+    std::vector<long double> a(n * 2 * n, 0.0L);
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) a[i * 2 * n + j] = A[i * n + j];
+        a[i * 2 * n + n + i] = 1.0L;
+    }
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::fabs((double)a[r * 2 * n + c]) > std::fabs((double)a[piv * 2 * n + c])) piv = r;
+        if (a[piv * 2 * n + c] == 0.0L) return false;
+        if (piv != c)
+            for (int j = 0; j < 2 * n; j++) std::swap(a[c * 2 * n + j], a[piv * 2 * n + j]);
+        const long double d = a[c * 2 * n + c];
+        for (int j = 0; j < 2 * n; j++) a[c * 2 * n + j] /= d;
+        for (int r = 0; r < n; r++)
+            if (r != c) {
+                const long double f = a[r * 2 * n + c];
+                for (int j = 0; j < 2 * n; j++) a[r * 2 * n + j] -= f * a[c * 2 * n + j];
+            }
+    }
+    Ai.assign(n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) Ai[i * n + j] = (double)a[i * 2 * n + n + j];
+    return true;
+}
+
+} // namespace host
+} // namespace prandtl
+
+using namespace prandtl;
+
+struct prandtl_ctx {
+    int dim, order, Np, K, NE, nv, Nfp;
+    prandtl_params prm;
+    Tables tab;
+    Geo geo;
+    Phys phys;
+    double *d_visc = nullptr, *d_fvf = nullptr, *d_alpha = nullptr, *d_eta = nullptr;
+    double *d_ya = nullptr, *d_yb = nullptr;
+    unsigned long long *d_lam = nullptr;
+    int *d_flag = nullptr;
+    std::string err;
+};
+
+namespace {
+
+std::string g_create_err;
+
+int fail(prandtl_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+int hip_check(prandtl_ctx *c, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return PRANDTL_OK;
+    return fail(c, PRANDTL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
+{
+    using Cf = Cfg<DIM, NP>;
+    VArgs a;
+    a.tab = c->tab; a.geo = c->geo; a.phys = c->phys;
+    a.quirks = c->prm.quirks;
+    a.y = y; a.visc = c->d_visc; a.fvf = c->d_fvf; a.grad = grad;
+    const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+    hipLaunchKernelGGL((k_visc<DIM, NP>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    return hip_check(c, hipGetLastError(), "k_visc launch");
+}
+
+template <int DIM, int NP>
+int launch_rhs(prandtl_ctx *c, const double *y, const double *u0, double *out, int mode, double dt,
+               double c0, double c1, hipStream_t s)
+{
+    using Cf = Cfg<DIM, NP>;
+    KArgs a;
+    a.tab = c->tab; a.geo = c->geo; a.phys = c->phys;
+    a.blend = c->prm.blending; a.parabolic = c->prm.parabolic;
+    a.ec = c->prm.ec_surface; a.quirks = c->prm.quirks;
+    a.mode = mode; a.dt = dt; a.c0 = c0; a.c1 = c1;
+    a.y = y; a.u0 = u0; a.out = out;
+    a.visc = c->d_visc; a.fvf = c->d_fvf; a.alpha = c->d_alpha; a.eta = c->d_eta;
+    a.lam = c->d_lam;
+    const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+    hipLaunchKernelGGL((k_rhs<DIM, NP>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    return hip_check(c, hipGetLastError(), "k_rhs launch");
+}
+
+template <int DIM, int NP>
+int stage_t(prandtl_ctx *c, const double *y, const double *u0, double *out, int mode, double dt,
+            double c0, double c1, hipStream_t s)
+{
+    if (c->prm.parabolic) {
+        int rc = launch_visc<DIM, NP>(c, y, nullptr, s);
+        if (rc) return rc;
+    }
+    return launch_rhs<DIM, NP>(c, y, u0, out, mode, dt, c0, c1, s);
+}
+
+#define PRANDTL_DISPATCH(FN, ...)                                                   \
+    switch (c->dim * 16 + c->Np) {                                                  \
+    case 2 * 16 + 3: return FN<2, 3>(__VA_ARGS__);                                  \
+    case 2 * 16 + 4: return FN<2, 4>(__VA_ARGS__);                                  \
+    case 2 * 16 + 5: return FN<2, 5>(__VA_ARGS__);                                  \
+    case 2 * 16 + 6: return FN<2, 6>(__VA_ARGS__);                                  \
+    case 3 * 16 + 3: return FN<3, 3>(__VA_ARGS__);                                  \
+    case 3 * 16 + 4: return FN<3, 4>(__VA_ARGS__);                                  \
+    case 3 * 16 + 5: return FN<3, 5>(__VA_ARGS__);                                  \
+    case 3 * 16 + 6: return FN<3, 6>(__VA_ARGS__);                                  \
+    case 3 * 16 + 7: return FN<3, 7>(__VA_ARGS__);                                  \
+    case 3 * 16 + 8: return FN<3, 8>(__VA_ARGS__);                                  \
+    default: return fail(c, PRANDTL_ERR_UNSUPPORTED, "dim/order not built");        \
+    }
+
+int stage(prandtl_ctx *c, const double *y, const double *u0, double *out, int mode, double dt,
+          double c0, double c1, hipStream_t s)
+{
+    PRANDTL_DISPATCH(stage_t, c, y, u0, out, mode, dt, c0, c1, s)
+}
+
+int visc_only(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
+{
+    PRANDTL_DISPATCH(launch_visc, c, y, grad, s)
+}
+
+} // namespace
+
+extern "C" {
+
+int prandtl_lgl(int order, double *x, double *w)
+{
+    if (order < 1 || order > 16 || !x || !w) return PRANDTL_ERR_ARG;
+    std::vector<double> xv, wv;
+    host::lgl(order, xv, wv);
+    std::memcpy(x, xv.data(), xv.size() * sizeof(double));
+    std::memcpy(w, wv.data(), wv.size() * sizeof(double));
+    return PRANDTL_OK;
+}
+
+int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, prandtl_ctx **out)
+{
+    if (!mesh || !prm || !out) return fail(nullptr, PRANDTL_ERR_ARG, "null argument");
+    *out = nullptr;
+    const int dim = mesh->dim, N = mesh->order;
+    if (dim < 2 || dim > 3) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "dim must be 2 or 3");
+    if (N < 2 || N > (dim == 2 ? 5 : 7))
+        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "order not built for this dim");
+    if (prm->precision != PRANDTL_FP64)
+        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "only the FP64 form is built");
+    if (!(prm->gamma > 1.0)) return fail(nullptr, PRANDTL_ERR_ARG, "gamma must exceed 1");
+    if (prm->parabolic && !(prm->Pr > 0.0)) return fail(nullptr, PRANDTL_ERR_ARG, "Pr must be positive");
+    long long NE = 1;
+    for (int r = 0; r < dim; r++) {
+        if (mesh->nelem[r] < 2) return fail(nullptr, PRANDTL_ERR_ARG, "need >= 2 elements per direction");
+        if (!(mesh->hi[r] > mesh->lo[r])) return fail(nullptr, PRANDTL_ERR_ARG, "empty box");
+        NE *= mesh->nelem[r];
+    }
+    if (NE > (1LL << 30)) return fail(nullptr, PRANDTL_ERR_ARG, "mesh too large");
+    prandtl_ctx *c = new (std::nothrow) prandtl_ctx();
+    if (!c) return fail(nullptr, PRANDTL_ERR_NOMEM, "host allocation");
+    c->dim = dim; c->order = N; c->Np = N + 1; c->nv = dim + 2;
+    c->K = dim == 2 ? c->Np * c->Np : c->Np * c->Np * c->Np;
+    c->Nfp = dim == 2 ? c->Np : c->Np * c->Np;
+    c->NE = (int)NE;
+    c->prm = *prm;
+    const int Np = c->Np;
+
+    // 1-D operators (DGSEMIntegrator.cpp:45-80)
+    std::vector<double> x, w;
+    host::lgl(N, x, w);
+    std::vector<double> wb(Np, 1.0), D(Np * Np, 0.0);
+    for (int i = 1; i < Np; i++)
+        for (int j = 0; j < i; j++) {
+            wb[j] *= (x[j] - x[i]);
+            wb[i] *= (x[i] - x[j]);
+        }
+    for (int i = 0; i < Np; i++) wb[i] = 1.0 / wb[i];
+    for (int iL = 0; iL < Np; iL++)
+        for (int i = 0; i < Np; i++)
+            if (iL != i) {
+                D[i * Np + iL] = wb[iL] / wb[i] / (x[i] - x[iL]);
+                D[i * Np + i] -= D[i * Np + iL];
+            }
+    std::memset(&c->tab, 0, sizeof(Tables));
+    for (int i = 0; i < Np; i++)
+        for (int j = 0; j < Np; j++) {
+            c->tab.D[i * MAXNP + j] = D[i * Np + j];
+            c->tab.Dh[i * MAXNP + j] = D[i * Np + j];
+            c->tab.Dh2[i * MAXNP + j] = D[i * Np + j] * 2.0;
+        }
+    c->tab.Dh[0] += 1.0 / w[0];
+    c->tab.Dh[N * MAXNP + N] -= 1.0 / w[N];
+    c->tab.Dh2[0] += 1.0 / w[0];
+    c->tab.Dh2[N * MAXNP + N] -= 1.0 / w[N];
+    // modal basis: 1-D shifted Legendre Vandermonde (ModalBasis.cpp:110-135)
+    std::vector<double> V(Np * Np), Vi;
+    for (int i = 0; i < Np; i++) {
+        std::vector<double> L(Np);
+        L[0] = 1.0;
+        const double z = 2.0 * x[i] - 1.0;
+        if (N >= 1) L[1] = z;
+        for (int k = 1; k < N; k++) L[k + 1] = ((2 * k + 1) * z * L[k] - k * L[k - 1]) / (k + 1);
+        for (int a = 0; a < Np; a++) V[i * Np + a] = L[a];
+    }
+    if (!host::invert(Np, V, Vi)) {
+        delete c;
+        return fail(nullptr, PRANDTL_ERR_ARG, "singular Vandermonde");
+    }
+    for (int a = 0; a < Np; a++)
+        for (int i = 0; i < Np; i++) c->tab.V1i[a * MAXNP + i] = Vi[a * Np + i];
+
+    // Cartesian geometry: J = det diag(h), adj = diag of cofactors
+    double h[3] = {1, 1, 1};
+    for (int r = 0; r < dim; r++) {
+        h[r] = (mesh->hi[r] - mesh->lo[r]) / mesh->nelem[r];
+        c->geo.n[r] = mesh->nelem[r];
+    }
+    for (int r = dim; r < 3; r++) c->geo.n[r] = 1;
+    c->geo.NE = c->NE;
+    double J;
+    if (dim == 2) {
+        J = h[0] * h[1];
+        c->geo.nrm[0] = h[1]; c->geo.nrm[1] = h[0]; c->geo.nrm[2] = 0.0;
+    } else {
+        J = h[0] * (h[1] * h[2]);
+        c->geo.nrm[0] = h[1] * h[2]; c->geo.nrm[1] = h[0] * h[2]; c->geo.nrm[2] = h[0] * h[1];
+    }
+    c->geo.invJ = 1.0 / J;
+    c->geo.aface = 1.0 / (w[0] * J);
+    for (int i = 0; i < Np; i++) c->tab.fvs[i] = 1.0 / (J * w[i]);
+    // subcell normals n_{i+1/2} = Ja + sum_{l<=i} w_l sum_q D_lq Ja (lines 1212-1233)
+    for (int r = 0; r < dim; r++)
+        for (int i = 1; i < Np; i++) {
+            double nor = c->geo.nrm[r];
+            for (int l = 0; l < i; l++) {
+                double tmp = 0.0;
+                for (int q = 0; q < Np; q++) tmp += c->geo.nrm[r] * D[l * Np + q];
+                tmp *= w[l];
+                nor += tmp;
+            }
+            c->tab.subn[r][i - 1] = nor;
+        }
+
+    c->phys.gamma = prm->gamma;
+    c->phys.gm1 = prm->gamma - 1.0;
+    c->phys.gm1i = 1.0 / c->phys.gm1;
+    c->phys.mu = prm->mu;
+    c->phys.lam_h = prm->parabolic ? prm->mu * prm->gamma * (1.0 / prm->Pr) : 0.0;
+    c->phys.alpha_max = prm->alpha_max;
+    c->phys.alpha_min = prm->alpha_min;
+    c->phys.thresh = 0.5 * std::pow(10.0, -1.8 * std::pow((double)N, 0.25));
+
+    const size_t NK = (size_t)c->NE * c->K;
+    hipError_t e = hipSuccess;
+    if (prm->parabolic) {
+        if (e == hipSuccess) e = hipMalloc(&c->d_visc, sizeof(double) * (c->nv - 1) * NK);
+        if (e == hipSuccess)
+            e = hipMalloc(&c->d_fvf, sizeof(double) * (size_t)c->NE * 2 * dim * (c->nv - 1) * c->Nfp);
+    }
+    if (e == hipSuccess) e = hipMalloc(&c->d_alpha, sizeof(double) * c->NE);
+    if (e == hipSuccess) e = hipMalloc(&c->d_eta, sizeof(double) * c->NE);
+    if (e == hipSuccess) e = hipMalloc(&c->d_ya, sizeof(double) * c->nv * NK);
+    if (e == hipSuccess) e = hipMalloc(&c->d_yb, sizeof(double) * c->nv * NK);
+    if (e == hipSuccess) e = hipMalloc(&c->d_lam, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&c->d_flag, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(c->d_alpha, 0, sizeof(double) * c->NE);
+    if (e == hipSuccess) e = hipMemset(c->d_eta, 0, sizeof(double) * c->NE);
+    if (e == hipSuccess) e = hipMemset(c->d_lam, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        g_create_err = std::string("device allocation: ") + hipGetErrorString(e);
+        prandtl_destroy(c);
+        return PRANDTL_ERR_NOMEM;
+    }
+    *out = c;
+    return PRANDTL_OK;
+}
+
+void prandtl_destroy(prandtl_ctx *c)
+{
+    if (!c) return;
+    double *bufs[] = {c->d_visc, c->d_fvf, c->d_alpha, c->d_eta, c->d_ya, c->d_yb};
+    for (double *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->d_lam) (void)hipFree(c->d_lam);
+    if (c->d_flag) (void)hipFree(c->d_flag);
+    delete c;
+}
+
+int prandtl_sizes(const prandtl_ctx *c, int *NE, int *K, int *nv)
+{
+    if (!c || !NE || !K || !nv) return PRANDTL_ERR_ARG;
+    *NE = c->NE; *K = c->K; *nv = c->nv;
+    return PRANDTL_OK;
+}
+
+int prandtl_rhs(prandtl_ctx *c, const double *u, double *dudt, void *stream)
+{
+    if (!c || !u || !dudt) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    if (u == dudt) return fail(c, PRANDTL_ERR_ARG, "u and dudt alias");
+    return stage(c, u, nullptr, dudt, 0, 0.0, 0.0, 0.0, (hipStream_t)stream);
+}
+
+int prandtl_rk3_step(prandtl_ctx *c, double *u, double dt, void *stream)
+{
+    if (!c || !u) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    // y1 = u + dt L(u); y2 = 3/4 u + 1/4 (y1 + dt L(y1)); u = 1/3 u + 2/3 (y2 + dt L(y2))
+    int rc = stage(c, u, nullptr, c->d_ya, 1, dt, 0.0, 0.0, s);
+    if (!rc) rc = stage(c, c->d_ya, u, c->d_yb, 2, dt, 3.0 / 4, 1.0 / 4, s);
+    if (!rc) rc = stage(c, c->d_yb, u, u, 2, dt, 1.0 / 3, 2.0 / 3, s);
+    return rc;
+}
+
+int prandtl_max_wave_speed(prandtl_ctx *c, double *out, int reset, void *stream)
+{
+    if (!c || !out) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long bits = 0;
+    int rc = hip_check(c, hipMemcpyAsync(&bits, c->d_lam, sizeof(bits), hipMemcpyDeviceToHost, s), "copy lambda");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *out = v;
+    if (reset) {
+        hipLaunchKernelGGL(k_reset_u64, dim3(1), dim3(1), 0, s, c->d_lam);
+        rc = hip_check(c, hipGetLastError(), "reset lambda");
+    }
+    return rc;
+}
+
+int prandtl_alpha(prandtl_ctx *c, double *alpha, double *eta, void *stream)
+{
+    if (!c) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = PRANDTL_OK;
+    if (alpha) rc = hip_check(c, hipMemcpyAsync(alpha, c->d_alpha, sizeof(double) * c->NE, hipMemcpyDeviceToHost, s), "copy alpha");
+    if (!rc && eta) rc = hip_check(c, hipMemcpyAsync(eta, c->d_eta, sizeof(double) * c->NE, hipMemcpyDeviceToHost, s), "copy eta");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+    return rc;
+}
+
+int prandtl_check_state(prandtl_ctx *c, const double *u, void *stream)
+{
+    if (!c || !u) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t NK = (size_t)c->NE * c->K;
+    hipLaunchKernelGGL(k_reset_int, dim3(1), dim3(1), 0, s, c->d_flag);
+    const int nb = (int)((NK + 255) / 256);
+    if (c->dim == 2) hipLaunchKernelGGL(k_check<2>, dim3(nb), dim3(256), 0, s, u, NK, c->phys.gm1, c->d_flag);
+    else hipLaunchKernelGGL(k_check<3>, dim3(nb), dim3(256), 0, s, u, NK, c->phys.gm1, c->d_flag);
+    int flag = 0;
+    int rc = hip_check(c, hipGetLastError(), "k_check launch");
+    if (!rc) rc = hip_check(c, hipMemcpyAsync(&flag, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, s), "copy flag");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    return flag ? fail(c, PRANDTL_ERR_STATE, "non-finite or non-positive density/pressure") : PRANDTL_OK;
+}
+
+int prandtl_debug_gradients(prandtl_ctx *c, const double *u, double *grad, void *stream)
+{
+    if (!c || !u || !grad) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    if (!c->prm.parabolic) return fail(c, PRANDTL_ERR_UNSUPPORTED, "not a parabolic context");
+    return visc_only(c, u, grad, (hipStream_t)stream);
+}
+
+const char *prandtl_last_error(const prandtl_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+} // extern "C"

@@ -1,0 +1,202 @@
+// Point physics of the DGSEM hot path, device side (gfx950, FP64).
+//
+// Restates, in the reference's operation order, the two-point fluxes of
+// ChandrashekarFlux.cpp:11-161, the log mean of BasicOperations.cpp:6-12, the
+// entropy/primitive transforms of BasicOperations.cpp:31-186 and the viscous
+// flux of NavierStokesFlux.cpp:12-90.  Normals are axis-aligned on the
+// Cartesian meshes this library builds, so a normal is a direction r and a
+// positive magnitude; the zero components of the reference's dot products
+// are dropped (they add +0 exactly for finite data).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace prandtl {
+
+struct Phys {
+    double gamma, gm1, gm1i;
+    double mu, lam_h;      // mu and mu*gamma*(1/Pr) (NavierStokesFlux.cpp:38)
+    double alpha_max, alpha_min, thresh;
+};
+
+// Per-node quantities shared by every two-point flux touching the node.
+template <int DIM> struct NodeQ {
+    double rho, v[DIM], p, beta, lam;
+};
+
+// rho, v = m/rho, p = (gamma-1)(E - rho|v|^2/2), beta = rho/(2p),
+// lam = |v| + sqrt(gamma p/rho)  (ChandrashekarFlux.cpp:17-68)
+template <int DIM>
+__device__ __forceinline__ NodeQ<DIM> node_quants(const double *s, const Phys &P)
+{
+    NodeQ<DIM> q;
+    q.rho = s[0];
+    double vsq = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        q.v[d] = s[1 + d] / q.rho;
+        vsq = (d == 0) ? q.v[d] * q.v[d] : vsq + q.v[d] * q.v[d];
+    }
+    q.p = P.gm1 * (s[DIM + 1] - 0.5 * q.rho * vsq);
+    q.beta = 0.5 * q.rho / q.p;
+    q.lam = sqrt(vsq) + sqrt(P.gamma * q.p / q.rho);
+    return q;
+}
+
+// ComputeLogMean, BasicOperations.cpp:6-12 (eps = 1e-4)
+__device__ __forceinline__ double logmean(double x, double y)
+{
+    const double xi = y / x;
+    const double u = (xi * (xi - 2.0) + 1.0) / (xi * (xi + 2.0) + 1.0);
+    return (u < 1e-4) ? (x + y) * 52.5 / (105.0 + u * (35.0 + u * (21.0 + 15.0 * u)))
+                      : (y - x) / log(xi);
+}
+
+// Chandrashekar two-point flux along direction r with normal magnitude nr.
+// DISS = false: ComputeVolumeFlux (ChandrashekarFlux.cpp:11-83), metric mean
+// of two equal metrics.  DISS = true: ComputeFaceFlux (lines 85-161) with the
+// conserved states s1, s2 for the jumps; quirk3d keeps the 3-D energy
+// dissipation term of line 131.
+template <int DIM, bool DISS>
+__device__ __forceinline__ void two_point_flux(int r, double nr, const NodeQ<DIM> &a,
+                                               const NodeQ<DIM> &b, const double *s1,
+                                               const double *s2, const Phys &P, bool quirk3d,
+                                               double *F)
+{
+    const double rho_ln = logmean(a.rho, b.rho);
+    const double rho_mean = 0.5 * (a.rho + b.rho);
+    const double drho = b.rho - a.rho;
+    double vm[DIM];
+    double h_hat = 0.0, diss = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        vm[d] = 0.5 * (a.v[d] + b.v[d]);
+        const double hd = -0.25 * (a.v[d] * a.v[d] + b.v[d] * b.v[d]) + vm[d] * vm[d];
+        h_hat = (d == 0) ? hd : h_hat + hd;
+        if (DISS) {
+            const double dv = b.v[d] - a.v[d];
+            const double t0 = 0.5 * drho * a.v[d] * b.v[d];
+            const double dd = (DIM == 3 && d == 2 && quirk3d) ? t0 + rho_mean * dv + vm[d]
+                                                              : t0 + rho_mean * dv * vm[d];
+            diss = (d == 0) ? dd : diss + dd;
+        }
+    }
+    double vr = vm[0];
+#pragma unroll
+    for (int d = 1; d < DIM; d++)
+        if (d == r) vr = vm[d];
+    const double Vn = vr * nr;
+    const double beta_ln = logmean(a.beta, b.beta);
+    const double p_hat = 0.5 * (a.rho + b.rho) / (a.beta + b.beta);
+    h_hat += 0.5 / beta_ln * P.gm1i + p_hat / rho_ln;
+    const double rVn = rho_ln * Vn;
+    F[0] = rVn;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const double mom = rho_ln * vm[d];
+        F[1 + d] = (d == r) ? Vn * mom + p_hat * nr : Vn * mom;
+    }
+    F[DIM + 1] = rVn * h_hat;
+    if (DISS) {
+        diss += 0.5 * drho * P.gm1i / beta_ln + 0.5 * rho_mean * P.gm1i * (1.0 / b.beta - 1.0 / a.beta);
+        const double lam = (a.lam < b.lam) ? b.lam : a.lam;
+        const double hl = 0.5 * lam;
+        F[0] = F[0] - hl * (s2[0] - s1[0]) * nr;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) F[1 + d] = F[1 + d] - hl * (s2[1 + d] - s1[1 + d]) * nr;
+        F[DIM + 1] = F[DIM + 1] - hl * diss * nr;
+    }
+}
+
+// Conserv2Entropy, BasicOperations.cpp:81-104 (beta = rho/p)
+template <int DIM>
+__device__ __forceinline__ void cons2entropy(const double *s, const Phys &P, double *w)
+{
+    const double r = 1.0 / s[0];
+    double vel[DIM], vv = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        vel[d] = s[1 + d] * r;
+        vv = (d == 0) ? vel[d] * vel[d] : vv + vel[d] * vel[d];
+    }
+    const double v_sq = 0.5 * vv;
+    const double p = P.gm1 * (s[DIM + 1] - s[0] * v_sq);
+    const double beta = s[0] / p;
+    const double sent = log(p * pow(s[0], -P.gamma));
+    w[0] = (P.gamma - sent) * P.gm1i - beta * v_sq;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) w[1 + d] = beta * vel[d];
+    w[DIM + 1] = -beta;
+}
+
+// EntropyGrad2PrimGrad, BasicOperations.cpp:126-170, one direction in place.
+// quirks: the density term multiplies velocities (the aliased row, lines
+// 137-138 and 164) instead of momenta.
+template <int DIM>
+__device__ __forceinline__ void entgrad2primgrad(const double *s, const Phys &P, bool quirks,
+                                                 double *g)
+{
+    const double r = 1.0 / s[0];
+    double vel[DIM], vv = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        vel[d] = s[1 + d] * r;
+        vv = (d == 0) ? vel[d] * vel[d] : vv + vel[d] * vel[d];
+    }
+    const double v_sq = 0.5 * vv;
+    const double p = P.gm1 * (s[DIM + 1] - s[0] * v_sq);
+    const double KE = v_sq * s[0];
+    const double g0 = g[0], gL = g[DIM + 1];
+    double mdot = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        g[1 + d] = p / s[0] * (g[1 + d] + vel[d] * gL);
+        const double md = quirks ? vel[d] : s[1 + d];
+        mdot = (d == 0) ? md * g[1 + d] : mdot + md * g[1 + d];
+    }
+    g[0] = s[0] * g0 - gL * (KE - p * P.gm1i) + s[0] / p * mdot;
+    g[DIM + 1] = p / s[0] * (g[0] + p * gL);
+}
+
+// NavierStokesFlux::ComputeViscousFlux (NavierStokesFlux.cpp:12-90), constant
+// mu, mu_bulk = 2/3.  dq[c][v] = d(prim_v)/dx_c; out fl[v][c], v = 1..nv-1
+// (the mass row is zero and not stored: fl[v-1][c]).
+template <int DIM>
+__device__ __forceinline__ void viscous_flux(const double *s, const double (*dq)[DIM + 2],
+                                             const Phys &P, double (*fl)[DIM])
+{
+    constexpr int NV = DIM + 2;
+    const double mu_bulk = 2.0 / 3.0;
+    const double rho = s[0];
+    double vel[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) vel[d] = s[1 + d] / rho;
+    // ComputePressure: V /= rho multiplies by 1/rho (BasicOperations.cpp:33-36)
+    const double rr = 1.0 / rho;
+    double vv = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const double v = s[1 + d] * rr;
+        vv = (d == 0) ? v * v : vv + v * v;
+    }
+    const double p = P.gm1 * (s[NV - 1] - 0.5 * rho * vv);
+    double div = dq[0][1];
+#pragma unroll
+    for (int d = 1; d < DIM; d++) div += dq[d][1 + d];
+#pragma unroll
+    for (int c = 0; c < DIM; c++) {
+        const double cvdT = P.gm1i / rho * (dq[c][NV - 1] - p / rho * dq[c][0]);
+#pragma unroll
+        for (int a = 0; a < DIM; a++) {
+            double t;
+            if (a == c) t = P.mu * (2.0 * dq[a][1 + a] - mu_bulk * div);
+            else t = P.mu * (dq[c][1 + a] + dq[a][1 + c]);
+            fl[a][c] = t;
+        }
+        double e = vel[0] * fl[0][c];
+#pragma unroll
+        for (int a = 1; a < DIM; a++) e = e + vel[a] * fl[a][c];
+        fl[NV - 2][c] = e + P.lam_h * cvdT;
+    }
+}
+
+} // namespace prandtl

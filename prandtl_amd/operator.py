@@ -1,0 +1,174 @@
+"""Host-side mirror of the reference's operator interface, over the HIP library.
+
+`DGSEMOperator` mirrors Prandtl::DGSEMOperator (DGSEMOperator.hpp:15-141):
+`Mult(u, dudt)` overwrites dudt with the semi-discrete RHS,
+`GetMaxCharSpeed()` returns the running maximum wave speed.
+`RK3SSPSolver` mirrors mfem::RK3SSPSolver as used at Simulation.cpp:186-189
+and 1010 (`Init(op)`, `Step(x, t, dt)`), with the three stages fused into
+the RHS kernels.  `TimeLoop` restates the dt policy of Simulation::Run
+(Simulation.cpp:893-907, 1007-1022).
+
+State vectors are torch float64 tensors on the GPU in byNODES layout
+(var, element, node).  Every call goes to libprandtl_hip.so on the current
+torch stream; there is no CPU path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def lgl_nodes(order):
+    """LGL nodes and weights on [0,1] as the operator uses them (prandtl_lgl)."""
+    x = np.zeros(order + 1)
+    w = np.zeros(order + 1)
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(lib().prandtl_lgl(order, x.ctypes.data_as(dp), w.ctypes.data_as(dp)))
+    return x, w
+
+
+def _stream_ptr():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DGSEMOperator:
+    """One GPU's DGSEM semi-discrete operator on a periodic Cartesian box."""
+
+    def __init__(self, dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0,
+                 parabolic=False, blending=False, alpha_max=0.5, alpha_min=0.001,
+                 ec_surface=False, quirks=True, precision="fp64"):
+        m = _lib.MeshDesc()
+        m.dim, m.order = dim, order
+        for r in range(3):
+            m.nelem[r] = int(nelem[r]) if r < dim else 1
+            m.lo[r] = float(lo[r]) if r < dim else 0.0
+            m.hi[r] = float(hi[r]) if r < dim else 1.0
+        p = _lib.Params()
+        p.gamma, p.Pr, p.mu = float(gamma), float(Pr), float(mu)
+        p.alpha_max, p.alpha_min = float(alpha_max), float(alpha_min)
+        p.parabolic, p.blending = int(parabolic), int(blending)
+        p.ec_surface, p.quirks = int(ec_surface), int(quirks)
+        p.precision = {"fp64": _lib.PRANDTL_FP64, "mixed": _lib.PRANDTL_MIXED}[precision]
+        h = ctypes.c_void_p()
+        rc = lib().prandtl_create(ctypes.byref(m), ctypes.byref(p), ctypes.byref(h))
+        check(rc, None)
+        self._h = h
+        NE, K, nv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().prandtl_sizes(h, ctypes.byref(NE), ctypes.byref(K), ctypes.byref(nv)), h)
+        self.NE, self.K, self.nv = NE.value, K.value, nv.value
+        self.dim, self.order = dim, order
+        self.nelem = tuple(int(nelem[r]) for r in range(dim))
+        self.lo = tuple(float(v) for v in lo[:dim])
+        self.hi = tuple(float(v) for v in hi[:dim])
+        self.height = self.width = self.nv * self.NE * self.K
+        self.hmin = min((self.hi[r] - self.lo[r]) / self.nelem[r] for r in range(dim))
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().prandtl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- checks -------------------------------------------------------------
+    def _vec(self, x, name):
+        import torch
+        if not isinstance(x, torch.Tensor) or x.dtype != torch.float64 or not x.is_cuda:
+            raise TypeError("%s must be a float64 CUDA tensor" % name)
+        if not x.is_contiguous() or x.numel() != self.height:
+            raise ValueError("%s must be contiguous with %d entries" % (name, self.height))
+        return ctypes.c_void_p(x.data_ptr())
+
+    # -- Operator surface ---------------------------------------------------
+    def Mult(self, u, dudt):
+        """dudt <- L(u) (DGSEMOperator::Mult, DGSEMOperator.cpp:537-591)."""
+        check(lib().prandtl_rhs(self._h, self._vec(u, "u"), self._vec(dudt, "dudt"), _stream_ptr()), self._h)
+
+    def Step(self, u, dt):
+        """u <- one SSP-RK3 step of size dt, in place."""
+        check(lib().prandtl_rk3_step(self._h, self._vec(u, "u"), float(dt), _stream_ptr()), self._h)
+
+    def GetMaxCharSpeed(self, reset=False):
+        out = ctypes.c_double()
+        check(lib().prandtl_max_wave_speed(self._h, ctypes.byref(out), int(reset), _stream_ptr()), self._h)
+        return out.value
+
+    def alpha(self):
+        a = np.zeros(self.NE)
+        e = np.zeros(self.NE)
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(lib().prandtl_alpha(self._h, a.ctypes.data_as(dp), e.ctypes.data_as(dp), _stream_ptr()), self._h)
+        return a, e
+
+    def check_state(self, u):
+        """True if every node has finite values, rho > 0 and p > 0."""
+        rc = lib().prandtl_check_state(self._h, self._vec(u, "u"), _stream_ptr())
+        if rc == _lib.PRANDTL_ERR_STATE:
+            return False
+        check(rc, self._h)
+        return True
+
+    def gradients(self, u):
+        """Primitive-variable BR1 gradients [dim][nv][NE][K] (parabolic contexts)."""
+        import torch
+        g = torch.empty((self.dim, self.nv, self.NE, self.K), dtype=torch.float64, device=u.device)
+        check(lib().prandtl_debug_gradients(self._h, self._vec(u, "u"), ctypes.c_void_p(g.data_ptr()),
+                                            _stream_ptr()), self._h)
+        return g
+
+
+class RK3SSPSolver:
+    """mfem::RK3SSPSolver surface: Init(op) then Step(x, t, dt) -> (t, dt)."""
+
+    def Init(self, f):
+        self.f = f
+
+    def Step(self, x, t, dt):
+        self.f.Step(x, dt)
+        return t + dt, dt
+
+
+class TimeLoop:
+    """Simulation::Run's stepping and dt policy (Simulation.cpp:893-907, 1007-1022).
+
+    variable_dt: dt = cfl*hmin/(lambda*(N+1)^2) from the running maximum wave
+    speed, recomputed after every step; otherwise dt is fixed.  The last step
+    is clipped to t_final.
+    """
+
+    def __init__(self, op, t_final, cfl=None, dt=None, variable_dt=False):
+        self.op, self.t_final, self.cfl = op, float(t_final), cfl
+        self.variable_dt = variable_dt
+        self.dt = dt
+        self.ode = RK3SSPSolver()
+        self.ode.Init(op)
+
+    def _cfl_dt(self):
+        lam = self.op.GetMaxCharSpeed()
+        return self.cfl * self.op.hmin / (lam * (self.op.order + 1) ** 2)
+
+    def run(self, u, t=0.0, max_steps=None):
+        import torch
+        if self.variable_dt and self.cfl:
+            z = torch.empty_like(u)
+            self.op.Mult(u, z)
+            self.dt = self._cfl_dt()
+        done, steps = False, 0
+        while not done:
+            dt_real = min(self.dt, self.t_final - t)
+            t, _ = self.ode.Step(u, t, dt_real)
+            if self.variable_dt and self.cfl:
+                self.dt = self._cfl_dt()
+            steps += 1
+            done = t >= self.t_final - 1e-8 * self.dt
+            if max_steps is not None and steps >= max_steps:
+                break
+        return t, steps

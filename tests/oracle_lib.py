@@ -1,0 +1,217 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, and only as the checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_DIR = os.path.join(os.path.dirname(_HERE), "oracle")
+_lib = None
+
+_D = ctypes.c_double
+_P = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int), ("order", ctypes.c_int), ("nelem", ctypes.c_int * 3),
+        ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
+        ("gamma", ctypes.c_double), ("Pr", ctypes.c_double), ("mu", ctypes.c_double),
+        ("parabolic", ctypes.c_int), ("blending", ctypes.c_int),
+        ("alpha_max", ctypes.c_double), ("alpha_min", ctypes.c_double),
+        ("ec_surface", ctypes.c_int), ("quirks", ctypes.c_int), ("threads", ctypes.c_int),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(ORACLE_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        L.orc_logmean.restype = _D
+        L.orc_logmean.argtypes = [_D, _D]
+        L.orc_volume_flux.restype = _D
+        L.orc_volume_flux.argtypes = [ctypes.c_int, _D, _P, _P, _P, _P, _P]
+        L.orc_face_flux.restype = _D
+        L.orc_face_flux.argtypes = [ctypes.c_int, _D, ctypes.c_int, _P, _P, _P, _P]
+        L.orc_cons2entropy.argtypes = [ctypes.c_int, _D, _P, _P]
+        L.orc_entgrad2primgrad.argtypes = [ctypes.c_int, _D, ctypes.c_int, _P, _P]
+        L.orc_viscous_flux.argtypes = [ctypes.c_int, _D, _D, _D, _P, _P, _P, _P, _P]
+        L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(ctypes.c_void_p)]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_sizes.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3
+        for name in ("orc_coords", "orc_weights", "orc_alpha", "orc_eta", "orc_last_gradients"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, _P]
+        L.orc_rhs.argtypes = [ctypes.c_void_p, _P, _P]
+        L.orc_rk3_step.argtypes = [ctypes.c_void_p, _P, _D]
+        L.orc_max_wave_speed.restype = _D
+        L.orc_max_wave_speed.argtypes = [ctypes.c_void_p]
+        L.orc_reset_wave_speed.argtypes = [ctypes.c_void_p]
+        L.orc_hmin.restype = _D
+        L.orc_hmin.argtypes = [ctypes.c_void_p]
+        L.orc_count_rhs_flops.argtypes = [ctypes.c_void_p, _P, _P]
+        for name in ("orc_lgl", "orc_dmat", "orc_legendre_vdm"):
+            getattr(L, name).argtypes = [ctypes.c_int] + [_P] * (2 if name != "orc_dmat" else 3)
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_P)
+
+
+def lgl(order):
+    x = np.zeros(order + 1)
+    w = np.zeros(order + 1)
+    assert lib().orc_lgl(order, ptr(x), ptr(w)) == 0
+    return x, w
+
+
+def dmat(order):
+    n = order + 1
+    D, Dh, Dh2 = (np.zeros((n, n)) for _ in range(3))
+    assert lib().orc_dmat(order, ptr(D), ptr(Dh), ptr(Dh2)) == 0
+    return D, Dh, Dh2
+
+
+def legendre_vdm(order):
+    n = order + 1
+    V, Vi = np.zeros((n, n)), np.zeros((n, n))
+    assert lib().orc_legendre_vdm(order, ptr(V), ptr(Vi)) == 0
+    return V, Vi
+
+
+def logmean(x, y):
+    return lib().orc_logmean(float(x), float(y))
+
+
+def volume_flux(dim, gamma, s1, s2, m1, m2):
+    F = np.zeros(dim + 2)
+    lam = lib().orc_volume_flux(dim, gamma, ptr(np.ascontiguousarray(s1, float)),
+                                ptr(np.ascontiguousarray(s2, float)),
+                                ptr(np.ascontiguousarray(m1, float)),
+                                ptr(np.ascontiguousarray(m2, float)), ptr(F))
+    return F, lam
+
+
+def face_flux(dim, gamma, quirks, s1, s2, nor):
+    F = np.zeros(dim + 2)
+    lam = lib().orc_face_flux(dim, gamma, quirks, ptr(np.ascontiguousarray(s1, float)),
+                              ptr(np.ascontiguousarray(s2, float)),
+                              ptr(np.ascontiguousarray(nor, float)), ptr(F))
+    return F, lam
+
+
+def cons2entropy(dim, gamma, s):
+    w = np.zeros(dim + 2)
+    lib().orc_cons2entropy(dim, gamma, ptr(np.ascontiguousarray(s, float)), ptr(w))
+    return w
+
+
+def entgrad2primgrad(dim, gamma, quirks, s, g):
+    g = np.array(g, dtype=float)
+    lib().orc_entgrad2primgrad(dim, gamma, quirks, ptr(np.ascontiguousarray(s, float)), ptr(g))
+    return g
+
+
+def viscous_flux(dim, gamma, Pr, mu, s, dq):
+    fl = np.zeros((dim + 2) * dim)
+    dqs = [np.ascontiguousarray(d, float) for d in dq] + [np.zeros(dim + 2)] * (3 - dim)
+    lib().orc_viscous_flux(dim, gamma, Pr, mu, ptr(np.ascontiguousarray(s, float)),
+                           ptr(dqs[0]), ptr(dqs[1]), ptr(dqs[2]), ptr(fl))
+    return fl.reshape(dim + 2, dim)
+
+
+class Oracle:
+    """Operator-shaped handle on the CPU restatement (Mult / Step / GetMaxCharSpeed)."""
+
+    def __init__(self, dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0,
+                 parabolic=False, blending=False, alpha_max=0.5, alpha_min=0.001,
+                 ec_surface=False, quirks=True, threads=1):
+        p = OrcParams()
+        p.dim, p.order = dim, order
+        for r in range(3):
+            p.nelem[r] = int(nelem[r]) if r < dim else 1
+            p.lo[r] = float(lo[r]) if r < dim else 0.0
+            p.hi[r] = float(hi[r]) if r < dim else 1.0
+        p.gamma, p.Pr, p.mu = gamma, Pr, mu
+        p.parabolic, p.blending = int(parabolic), int(blending)
+        p.alpha_max, p.alpha_min = alpha_max, alpha_min
+        p.ec_surface, p.quirks, p.threads = int(ec_surface), int(quirks), int(threads)
+        self.params = p
+        h = ctypes.c_void_p()
+        rc = lib().orc_create(ctypes.byref(p), ctypes.byref(h))
+        if rc != 0:
+            raise ValueError("orc_create failed: %d" % rc)
+        self.h = h
+        NE, K, nv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().orc_sizes(h, ctypes.byref(NE), ctypes.byref(K), ctypes.byref(nv))
+        self.NE, self.K, self.nv, self.dim = NE.value, K.value, nv.value, dim
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    @property
+    def size(self):
+        return self.nv * self.NE * self.K
+
+    def coords(self):
+        X = np.zeros((self.dim, self.NE, self.K))
+        lib().orc_coords(self.h, ptr(X))
+        return X
+
+    def weights(self):
+        W = np.zeros((self.NE, self.K))
+        lib().orc_weights(self.h, ptr(W))
+        return W
+
+    def Mult(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        out = np.zeros_like(u)
+        assert lib().orc_rhs(self.h, ptr(u), ptr(out)) == 0
+        return out
+
+    def Step(self, u, dt):
+        u = np.array(u, dtype=np.float64).reshape(-1)
+        assert lib().orc_rk3_step(self.h, ptr(u), float(dt)) == 0
+        return u
+
+    def GetMaxCharSpeed(self):
+        return lib().orc_max_wave_speed(self.h)
+
+    def reset_wave_speed(self):
+        lib().orc_reset_wave_speed(self.h)
+
+    def hmin(self):
+        return lib().orc_hmin(self.h)
+
+    def alpha(self):
+        a = np.zeros(self.NE)
+        lib().orc_alpha(self.h, ptr(a))
+        return a
+
+    def eta(self):
+        a = np.zeros(self.NE)
+        lib().orc_eta(self.h, ptr(a))
+        return a
+
+    def gradients(self):
+        g = np.zeros((self.dim, self.nv, self.NE, self.K))
+        assert lib().orc_last_gradients(self.h, ptr(g)) == 0
+        return g
+
+    def count_flops(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        out = np.zeros(8)
+        lib().orc_count_rhs_flops(self.h, ptr(u), ptr(out))
+        keys = ["total", "indicator", "lifting", "viscous", "volume", "surface", "fv", "assembly"]
+        return dict(zip(keys, out.tolist()))

@@ -1,0 +1,204 @@
+"""GPU parity: libprandtl_hip.so against the CPU oracle on the same inputs.
+
+Bar: relative max-norm difference <= 1e-12 for the plain FP64 RHS
+(max|gpu - oracle| / max|oracle| over the whole state vector).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from prandtl_amd import DGSEMOperator, cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TWO_PI = 2.0 * math.pi
+
+
+def rel_max(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def to_dev(u, dev):
+    import torch
+    return torch.tensor(np.ascontiguousarray(u).reshape(-1), dtype=torch.float64, device=dev)
+
+
+def pair(cfg, threads=8):
+    kw = dict(cfg)
+    op = DGSEMOperator(**kw)
+    orc = O.Oracle(threads=threads, **kw)
+    return op, orc
+
+
+def gpu_rhs(op, u, dev):
+    import torch
+    du = to_dev(u, dev)
+    out = torch.empty_like(du)
+    op.Mult(du, out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def states(cfg, seed=1):
+    X = cases.node_coords(cfg["dim"], cfg["order"], cfg["nelem"], cfg["lo"], cfg["hi"])
+    L = [cfg["hi"][r] - cfg["lo"][r] for r in range(cfg["dim"])]
+    out = {"random": cases.random_smooth_state(X, cfg["dim"], seed, L=L)}
+    near = cases.random_smooth_state(X, cfg["dim"], seed + 7, amp=1e-4, L=L)
+    out["near_constant"] = near
+    return X, out
+
+
+IV = dict(dim=2, order=3, nelem=(16, 16), lo=(-5.0, -5.0), hi=(5.0, 5.0))
+KH = dict(dim=2, order=4, nelem=(64, 64), lo=(-1.0, -1.0), hi=(1.0, 1.0), blending=True, alpha_max=0.5)
+TGV = dict(dim=3, order=3, nelem=(16, 16, 16), lo=(0.0, 0.0, 0.0), hi=(TWO_PI,) * 3,
+           parabolic=True, mu=1.0 / 1600.0, Pr=0.71)
+
+
+def test_iv2d_rhs_parity(cuda):
+    op, orc = pair(IV)
+    X, st = states(IV)
+    st["ic"] = cases.isentropic_vortex(X)
+    for name, u in st.items():
+        r = rel_max(gpu_rhs(op, u, cuda), orc.Mult(u))
+        print("iv2d", name, r, flush=True)
+        assert r <= TOL, (name, r)
+    assert abs(op.GetMaxCharSpeed() - orc.GetMaxCharSpeed()) <= 1e-14 * orc.GetMaxCharSpeed()
+
+
+@pytest.mark.parametrize("quirks", [True, False])
+def test_tgv3d_ns_rhs_parity(cuda, quirks):
+    cfg = dict(TGV, quirks=quirks)
+    op, orc = pair(cfg)
+    X, st = states(cfg)
+    st["ic"] = cases.taylor_green(X)
+    for name, u in st.items():
+        r = rel_max(gpu_rhs(op, u, cuda), orc.Mult(u))
+        print("tgv3d quirks=%d" % quirks, name, r, flush=True)
+        assert r <= TOL, (name, r)
+
+
+@pytest.mark.parametrize("quirks", [True, False])
+def test_tgv3d_gradients(cuda, quirks):
+    cfg = dict(TGV, nelem=(6, 5, 4), quirks=quirks)
+    op, orc = pair(cfg)
+    X, st = states(cfg, seed=3)
+    u = st["random"]
+    orc.Mult(u)
+    g_ref = orc.gradients()
+    import torch
+    g = op.gradients(to_dev(u, cuda))
+    torch.cuda.synchronize()
+    r = rel_max(g.cpu().numpy(), g_ref)
+    print("grad quirks=%d" % quirks, r, flush=True)
+    assert r <= TOL
+
+
+def kh_with_jumps(X, gamma=1.4):
+    """KH shear layer plus density/pressure jumps inside elements, so the
+    indicator switches blending on in the elements the jumps cross."""
+    B = np.tanh(15.0 * X[1] + 7.5) - np.tanh(15.0 * X[1] - 7.5)
+    rho = (0.5 + 0.75 * B) * (1.0 + 0.5 * (X[0] > 0.31))
+    u = 0.5 * (B - 1.0)
+    v = 0.1 * np.sin(2.0 * np.pi * X[0])
+    p = 1.0 + 0.5 * (X[1] > 0.137)
+    return cases.prim_to_cons(rho, [u, v], p, gamma)
+
+
+def test_kh2d_blending_parity(cuda):
+    op, orc = pair(KH)
+    X = cases.node_coords(2, 4, KH["nelem"], KH["lo"], KH["hi"])
+    for name, u in (("ic", cases.kelvin_helmholtz(X)), ("jumps", kh_with_jumps(X))):
+        r = rel_max(gpu_rhs(op, u, cuda), orc.Mult(u))
+        a_gpu, e_gpu = op.alpha()
+        a_ref, e_ref = orc.alpha(), orc.eta()
+        print("kh2d", name, "rhs", r, "alpha>0:", int((a_ref > 0).sum()),
+              "max|d eta|", np.abs(e_gpu - e_ref).max(), "max|d alpha|", np.abs(a_gpu - a_ref).max(), flush=True)
+        if name == "jumps":
+            assert (a_ref > 0).sum() > 0
+        assert np.abs(e_gpu - e_ref).max() <= 1e-12
+        assert np.abs(a_gpu - a_ref).max() <= 1e-10
+        assert r <= TOL
+
+
+@pytest.mark.parametrize("dim,order", [(2, 2), (2, 3), (2, 4), (2, 5), (3, 2), (3, 3), (3, 4), (3, 5), (3, 6), (3, 7)])
+@pytest.mark.parametrize("physics", ["euler", "ns", "blend"])
+def test_orders_parity(cuda, dim, order, physics):
+    nel = (5, 4) if dim == 2 else (3, 3, 2)
+    if dim == 3 and order >= 6:
+        nel = (2, 2, 2)
+    cfg = dict(dim=dim, order=order, nelem=nel, lo=(0.0,) * dim, hi=(1.0, 1.3, 0.7)[:dim])
+    if physics == "ns":
+        cfg.update(parabolic=True, mu=0.01, Pr=0.71)
+    if physics == "blend":
+        cfg.update(blending=True, alpha_max=0.5)
+    op, orc = pair(cfg)
+    X, st = states(cfg, seed=11 + order)
+    u = st["random"]
+    r = rel_max(gpu_rhs(op, u, cuda), orc.Mult(u))
+    print(dim, order, physics, r, flush=True)
+    assert r <= TOL
+
+
+def test_rk3_step_parity(cuda):
+    import torch
+    op, orc = pair(IV)
+    X = cases.node_coords(2, 3, IV["nelem"], IV["lo"], IV["hi"])
+    u = cases.isentropic_vortex(X)
+    du = to_dev(u, cuda)
+    dt = 0.01
+    for _ in range(3):
+        op.Step(du, dt)
+    u_ref = u.reshape(-1).copy()
+    for _ in range(3):
+        u_ref = orc.Step(u_ref, dt)
+    torch.cuda.synchronize()
+    r = rel_max(du.cpu().numpy(), u_ref)
+    print("rk3 3 steps", r, flush=True)
+    assert r <= TOL
+
+
+def test_ns_rk3_step_parity(cuda):
+    import torch
+    cfg = dict(TGV, nelem=(8, 8, 8))
+    op, orc = pair(cfg)
+    X = cases.node_coords(3, 3, cfg["nelem"], cfg["lo"], cfg["hi"])
+    u = cases.taylor_green(X)
+    du = to_dev(u, cuda)
+    op.Step(du, 1e-3)
+    u_ref = orc.Step(u.reshape(-1).copy(), 1e-3)
+    torch.cuda.synchronize()
+    r = rel_max(du.cpu().numpy(), u_ref)
+    print("ns rk3", r, flush=True)
+    assert r <= TOL
+
+
+def test_ec_entropy_conservation_gpu(cuda):
+    """EC volume + EC surface flux on a periodic mesh: sum w.J.w(u).RHS = 0."""
+    cfg = dict(dim=3, order=3, nelem=(4, 4, 4), lo=(0.0,) * 3, hi=(1.0,) * 3, ec_surface=True, quirks=False)
+    op = DGSEMOperator(**cfg)
+    X, st = states(cfg, seed=5)
+    u = st["random"]
+    rhs = gpu_rhs(op, u, cuda).reshape(5, -1)
+    WJ = cases.quadrature_weights(3, 3, cfg["nelem"], cfg["lo"], cfg["hi"]).reshape(-1)
+    U = u.reshape(5, -1)
+    W = np.stack([O.cons2entropy(3, 1.4, U[:, i]) for i in range(U.shape[1])], axis=1)
+    prod = np.sum(WJ * np.sum(W * rhs, axis=0))
+    scale = np.sum(WJ * np.sum(np.abs(W * rhs), axis=0))
+    print("entropy production", prod, scale, flush=True)
+    assert abs(prod) <= 1e-13 * scale
+
+
+def test_check_state(cuda):
+    op = DGSEMOperator(**IV)
+    X = cases.node_coords(2, 3, IV["nelem"], IV["lo"], IV["hi"])
+    u = cases.isentropic_vortex(X)
+    assert op.check_state(to_dev(u, cuda))
+    bad = u.copy()
+    bad[3, 7, 3] = -1.0   # negative total energy -> negative pressure
+    assert not op.check_state(to_dev(bad, cuda))
+    bad = u.copy()
+    bad[0, 2, 1] = np.nan
+    assert not op.check_state(to_dev(bad, cuda))
