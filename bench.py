@@ -1,0 +1,185 @@
+"""Throughput of the DGSEM RHS + SSP-RK3 stage update on one MI355X.
+
+Default workload: 3-D Navier-Stokes Taylor-Green vortex, periodic [0,2pi]^3,
+16^3 hexahedra, N = 3 (64^3 nodes x 5 variables), Re = 1600, Ma = 0.1,
+Pr = 0.71, BR1 lifting, Chandrashekar volume flux, reference surface flux,
+plain FP64.  A "step" is one SSP-RK3 step = three RHS evaluations with the
+stage combinations fused in; DOF-updates/s = nodes x RHS evaluations / s.
+
+The state is resident in HBM before the timed region starts.  The JSON line
+also carries the roofline of the dominant kernel (k_rhs, timed with HIP
+events on its own stream inside the timed region) and a CPU baseline: the
+oracle (an FP64 CPU restatement of the reference RHS, not MFEM) timed on this
+host on a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12      # MI355X dense FP64 (vector = matrix), FLOP/s
+PEAK_HBM = 8.0e12        # B/s
+METRIC = "RHS DOF-updates/s, plain FP64 vs mixed, 3D TGV N=3; L2 err vs FP64 oracle"
+
+WORKLOADS = {
+    "tgv3d_n3": dict(dim=3, order=3, nelem=(16, 16, 16), lo=(0.0,) * 3, hi=(2 * math.pi,) * 3,
+                     parabolic=True, mu=1.0 / 1600.0, Pr=0.71, gamma=1.4),
+    "tgv3d_n7": dict(dim=3, order=7, nelem=(32, 32, 32), lo=(0.0,) * 3, hi=(2 * math.pi,) * 3,
+                     parabolic=True, mu=1.0 / 1600.0, Pr=0.71, gamma=1.4, blending=True, alpha_max=0.1),
+    "kh2d_n4": dict(dim=2, order=4, nelem=(64, 64), lo=(-1.0, -1.0), hi=(1.0, 1.0),
+                    blending=True, alpha_max=0.5, gamma=1.4),
+    "iv2d_n3": dict(dim=2, order=3, nelem=(16, 16), lo=(-5.0, -5.0), hi=(5.0, 5.0), gamma=1.4),
+}
+
+
+def initial_state(name, cfg):
+    from prandtl_amd import cases
+    X = cases.node_coords(cfg["dim"], cfg["order"], cfg["nelem"], cfg["lo"], cfg["hi"])
+    if name.startswith("tgv"):
+        return cases.taylor_green(X)
+    if name.startswith("kh"):
+        return cases.kelvin_helmholtz(X)
+    return cases.isentropic_vortex(X)
+
+
+def algorithmic_counts(name, quirks):
+    """Per-node algorithmic FP64 op counts of one RHS (oracle op-counting
+    instantiation on the initial state, committed as a fixture)."""
+    path = os.path.join(ROOT, "tests", "golden", "flop_counts.json")
+    with open(path) as f:
+        table = json.load(f)
+    return table["%s_%s" % (name, "faithful" if quirks else "corrected")]
+
+
+def cpu_baseline(name, cfg, quirks, u, budget_s):
+    """Oracle RHS on this host's cores, bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = min(os.cpu_count() or 1, 16)
+    orc = O.Oracle(threads=threads, quirks=quirks, **cfg)
+    nodes = orc.NE * orc.K
+    orc.Mult(u)                       # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        orc.Mult(u)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 50:
+            break
+    return {"value": nodes * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": "%d RHS evaluations of %s (%d nodes) in %.1f s, OpenMP over elements/faces"
+                      % (n, name, nodes, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="tgv3d_n3", choices=sorted(WORKLOADS))
+    ap.add_argument("--corrected", action="store_true", help="reference defects corrected")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the path is single-GPU in this build "
+              "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from prandtl_amd import DGSEMOperator
+
+    name = args.workload
+    cfg = dict(WORKLOADS[name])
+    quirks = not args.corrected
+    dev = torch.device("cuda:0")
+    op = DGSEMOperator(quirks=quirks, **cfg)
+    u_host = initial_state(name, cfg)
+    u = torch.tensor(u_host.reshape(-1), dtype=torch.float64, device=dev)
+    nodes = op.NE * op.K
+
+    # fixed dt from CFL 0.1 of the initial wave speed (TGV config policy)
+    z = torch.empty_like(u)
+    op.Mult(u, z)
+    lam = op.GetMaxCharSpeed()
+    dt = 0.1 * op.hmin / (lam * (cfg["order"] + 1) ** 2)
+
+    for _ in range(args.warmup):
+        op.Step(u, dt)
+    torch.cuda.synchronize()
+
+    op.profile_begin()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        op.Step(u, dt)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = op.profile_end()
+    ok = op.check_state(u)
+
+    rhs_evals = 3 * args.steps
+    value = nodes * rhs_evals / elapsed
+    ms_rhs, n_rhs = prof["rhs"]
+    ms_visc, n_visc = prof["visc"]
+    t_rhs = ms_rhs / max(n_rhs, 1) * 1e-3
+
+    counts = algorithmic_counts(name, quirks)
+    nv = op.nv
+    # k_rhs: inviscid volume pairs, surface fluxes, FV, indicator, assembly, RK stage
+    fl_rhs = counts["volume"] + counts["surface"] + counts["fv"] + counts["indicator"] + counts["assembly"] + 4 * nv
+    # compulsory bytes of k_rhs per node: read y, write out, read u0 on 2 of 3
+    # stages; viscous inputs (nv-1 volume + face traces) when parabolic
+    by_rhs = 8 * nv * (1 + 1 + 2.0 / 3.0)
+    if cfg.get("parabolic"):
+        Np = cfg["order"] + 1
+        by_rhs += 8 * (nv - 1) * (1 + 2 * cfg["dim"] * Np ** (cfg["dim"] - 1) / Np ** cfg["dim"])
+    flops_launch = fl_rhs * nodes
+    bytes_launch = by_rhs * nodes
+    ai = flops_launch / bytes_launch
+    ridge = PEAK_FP64 / PEAK_HBM
+    if ai >= ridge:
+        roof = {"bound": "mfma", "achieved": flops_launch / t_rhs / 1e12, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "pipe": "fp64 (vector and matrix peaks are equal on MI355X)"}
+    else:
+        roof = {"bound": "hbm", "achieved": bytes_launch / t_rhs / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = "k_rhs<%d,%d>" % (cfg["dim"], cfg["order"] + 1)
+    roof["avg_launch_us"] = t_rhs * 1e6
+    roof["algorithmic_flops_per_launch"] = flops_launch
+    roof["algorithmic_bytes_per_launch"] = bytes_launch
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic initial condition)",
+        "config": {"workload": name + (" (reference defects corrected)" if args.corrected else
+                                      " (reference defects reproduced)"),
+                   "elements": op.NE, "order": cfg["order"], "nodes": nodes, "vars": nv,
+                   "rhs_per_step": 3, "dt": dt, "state_ok": bool(ok),
+                   "kernel_ms": {"k_rhs": ms_rhs / max(n_rhs, 1), "k_visc": ms_visc / max(n_visc, 1)}},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(name, cfg, quirks, u_host, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

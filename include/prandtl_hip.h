@@ -98,6 +98,15 @@ int prandtl_lgl(int order, double *x, double *w);
  * (DGSEMOperator.cpp:96-159), written to a device buffer. */
 int prandtl_debug_gradients(prandtl_ctx *ctx, const double *u_dev, double *grad_dev, void *stream);
 
+/* Kernel timing, the per-phase timing the reference lacks (it has only a
+ * whole-run clock, Simulation.cpp:910-914,1162-1171): between begin and end
+ * every kernel launch of this context is bracketed by HIP events on its
+ * stream.  end synchronises and returns the summed durations in ms and the
+ * launch counts: [0] = RHS/stage kernel, [1] = parabolic (lifting/viscous)
+ * kernel. */
+int prandtl_profile_begin(prandtl_ctx *ctx);
+int prandtl_profile_end(prandtl_ctx *ctx, double *ms, int *count);
+
 /* Last error message of this context (or of create when ctx is NULL). */
 const char *prandtl_last_error(const prandtl_ctx *ctx);
 

@@ -62,6 +62,12 @@ inline bool operator!=(Cnt a, Cnt b) { return a.v != b.v; }
 
 inline double val(double x) { return x; }
 inline double val(Cnt x) { return x.v; }
+inline double val(long double x) { return (double)x; }
+inline long double mlog(long double x) { return std::log(x); }
+inline long double msqrt(long double x) { return std::sqrt(x); }
+inline long double mexp(long double x) { return std::exp(x); }
+inline long double mpow(long double x, double y) { return std::pow(x, (long double)y); }
+inline long double mabs(long double x) { return std::fabs(x); }
 inline double mlog(double x) { return std::log(x); }
 inline Cnt mlog(Cnt x) { ++g_ops[g_cat]; return Cnt(std::log(x.v)); }
 inline double msqrt(double x) { return std::sqrt(x); }
@@ -1127,6 +1133,17 @@ int orc_last_gradients(const orc_ctx *c, double *grad)
 {
     if (c->grad.empty()) return 1;
     std::memcpy(grad, c->grad.data(), c->grad.size() * sizeof(double));
+    return 0;
+}
+
+int orc_rhs_ld(orc_ctx *c, const double *u, double *dudt)
+{
+    const size_t n = (size_t)c->m.nv * c->m.NE * c->m.K;
+    std::vector<long double> U(n), dU(n);
+    for (size_t i = 0; i < n; i++) U[i] = u[i];
+    Engine<long double> E(c);
+    E.rhs(U.data(), dU.data(), c->p.threads);
+    for (size_t i = 0; i < n; i++) dudt[i] = (double)dU[i];
     return 0;
 }
 

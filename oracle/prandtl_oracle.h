@@ -57,6 +57,9 @@ int orc_sizes(const orc_ctx *c, int *NE, int *K, int *nv);
 int orc_coords(const orc_ctx *c, double *X /* [dim][NE][K] */);
 int orc_weights(const orc_ctx *c, double *WJ /* [NE][K] omega_i * J_i */);
 int orc_rhs(orc_ctx *c, const double *u, double *dudt);
+/* the same RHS evaluated in x87 extended precision (64-bit mantissa), as a
+ * higher-precision reference for rounding-error studies */
+int orc_rhs_ld(orc_ctx *c, const double *u, double *dudt);
 int orc_rk3_step(orc_ctx *c, double *u, double dt);
 double orc_max_wave_speed(const orc_ctx *c);
 void orc_reset_wave_speed(orc_ctx *c);

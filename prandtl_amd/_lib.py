@@ -21,7 +21,7 @@ PRANDTL_MIXED = 1
 EXPORTED = (
     "prandtl_create", "prandtl_destroy", "prandtl_sizes", "prandtl_rhs", "prandtl_rk3_step",
     "prandtl_max_wave_speed", "prandtl_alpha", "prandtl_check_state", "prandtl_lgl",
-    "prandtl_debug_gradients", "prandtl_last_error",
+    "prandtl_debug_gradients", "prandtl_profile_begin", "prandtl_profile_end", "prandtl_last_error",
 )
 
 
@@ -70,6 +70,8 @@ def lib():
     L.prandtl_check_state.argtypes = [vp, vp, vp]
     L.prandtl_lgl.argtypes = [ctypes.c_int, dp, dp]
     L.prandtl_debug_gradients.argtypes = [vp, vp, vp, vp]
+    L.prandtl_profile_begin.argtypes = [vp]
+    L.prandtl_profile_end.argtypes = [vp, dp, ip]
     L.prandtl_last_error.argtypes = [vp]
     L.prandtl_last_error.restype = ctypes.c_char_p
     _lib = L

@@ -752,6 +752,11 @@ struct prandtl_ctx {
     unsigned long long *d_lam = nullptr;
     int *d_flag = nullptr;
     std::string err;
+    // kernel timing (prandtl_profile_begin/end): events bracketing every launch
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kind;   // 0 = k_rhs, 1 = k_visc
+    size_t ev_used = 0;
 };
 
 namespace {
@@ -771,6 +776,31 @@ int hip_check(prandtl_ctx *c, hipError_t e, const char *what)
     return fail(c, PRANDTL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Event pair around one launch when profiling is on (start before, stop after).
+struct ProfScope {
+    prandtl_ctx *c;
+    hipStream_t s;
+    size_t i;
+    ProfScope(prandtl_ctx *cc, hipStream_t ss, int kind) : c(cc), s(ss), i(0)
+    {
+        if (!c->prof) return;
+        i = c->ev_used;
+        while (c->ev_pool.size() < i + 2) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) { c->prof = false; return; }
+            c->ev_pool.push_back(e);
+        }
+        c->ev_kind.resize((i + 2) / 2);
+        c->ev_kind[i / 2] = kind;
+        c->ev_used = i + 2;
+        (void)hipEventRecord(c->ev_pool[i], s);
+    }
+    ~ProfScope()
+    {
+        if (c->prof && c->ev_used >= i + 2) (void)hipEventRecord(c->ev_pool[i + 1], s);
+    }
+};
+
 template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
 {
     using Cf = Cfg<DIM, NP>;
@@ -779,6 +809,7 @@ template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, doub
     a.quirks = c->prm.quirks;
     a.y = y; a.visc = c->d_visc; a.fvf = c->d_fvf; a.grad = grad;
     const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+    ProfScope ps(c, s, 1);
     hipLaunchKernelGGL((k_visc<DIM, NP>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
     return hip_check(c, hipGetLastError(), "k_visc launch");
 }
@@ -797,6 +828,7 @@ int launch_rhs(prandtl_ctx *c, const double *y, const double *u0, double *out, i
     a.visc = c->d_visc; a.fvf = c->d_fvf; a.alpha = c->d_alpha; a.eta = c->d_eta;
     a.lam = c->d_lam;
     const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+    ProfScope ps(c, s, 0);
     hipLaunchKernelGGL((k_rhs<DIM, NP>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
     return hip_check(c, hipGetLastError(), "k_rhs launch");
 }
@@ -999,6 +1031,7 @@ void prandtl_destroy(prandtl_ctx *c)
         if (b) (void)hipFree(b);
     if (c->d_lam) (void)hipFree(c->d_lam);
     if (c->d_flag) (void)hipFree(c->d_flag);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     delete c;
 }
 
@@ -1078,6 +1111,35 @@ int prandtl_debug_gradients(prandtl_ctx *c, const double *u, double *grad, void 
     if (!c || !u || !grad) return fail(c, PRANDTL_ERR_ARG, "null argument");
     if (!c->prm.parabolic) return fail(c, PRANDTL_ERR_UNSUPPORTED, "not a parabolic context");
     return visc_only(c, u, grad, (hipStream_t)stream);
+}
+
+int prandtl_profile_begin(prandtl_ctx *c)
+{
+    if (!c) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    c->prof = true;
+    c->ev_used = 0;
+    c->ev_kind.clear();
+    return PRANDTL_OK;
+}
+
+int prandtl_profile_end(prandtl_ctx *c, double *ms, int *count)
+{
+    if (!c || !ms || !count) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    ms[0] = ms[1] = 0.0;
+    count[0] = count[1] = 0;
+    for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+        int rc = hip_check(c, hipEventSynchronize(c->ev_pool[k + 1]), "event sync");
+        if (rc) return rc;
+        float t = 0.0f;
+        rc = hip_check(c, hipEventElapsedTime(&t, c->ev_pool[k], c->ev_pool[k + 1]), "elapsed");
+        if (rc) return rc;
+        const int kind = c->ev_kind[k / 2];
+        ms[kind] += t;
+        count[kind] += 1;
+    }
+    c->prof = false;
+    c->ev_used = 0;
+    return PRANDTL_OK;
 }
 
 const char *prandtl_last_error(const prandtl_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }

@@ -116,6 +116,17 @@ class DGSEMOperator:
         check(rc, self._h)
         return True
 
+    def profile_begin(self):
+        """Bracket every following kernel launch with HIP events."""
+        check(lib().prandtl_profile_begin(self._h), self._h)
+
+    def profile_end(self):
+        """Synchronise; return {'rhs': (ms, launches), 'visc': (ms, launches)}."""
+        ms = (ctypes.c_double * 2)()
+        cnt = (ctypes.c_int * 2)()
+        check(lib().prandtl_profile_end(self._h, ms, cnt), self._h)
+        return {"rhs": (ms[0], cnt[0]), "visc": (ms[1], cnt[1])}
+
     def gradients(self, u):
         """Primitive-variable BR1 gradients [dim][nv][NE][K] (parabolic contexts)."""
         import torch

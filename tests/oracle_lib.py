@@ -49,6 +49,7 @@ def lib():
         for name in ("orc_coords", "orc_weights", "orc_alpha", "orc_eta", "orc_last_gradients"):
             getattr(L, name).argtypes = [ctypes.c_void_p, _P]
         L.orc_rhs.argtypes = [ctypes.c_void_p, _P, _P]
+        L.orc_rhs_ld.argtypes = [ctypes.c_void_p, _P, _P]
         L.orc_rk3_step.argtypes = [ctypes.c_void_p, _P, _D]
         L.orc_max_wave_speed.restype = _D
         L.orc_max_wave_speed.argtypes = [ctypes.c_void_p]
@@ -178,6 +179,13 @@ class Oracle:
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
         out = np.zeros_like(u)
         assert lib().orc_rhs(self.h, ptr(u), ptr(out)) == 0
+        return out
+
+    def Mult_extended(self, u):
+        """The RHS in long double (extended precision), rounded to double."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        out = np.zeros_like(u)
+        assert lib().orc_rhs_ld(self.h, ptr(u), ptr(out)) == 0
         return out
 
     def Step(self, u, dt):

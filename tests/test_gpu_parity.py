@@ -68,6 +68,24 @@ def test_iv2d_rhs_parity(cuda):
     assert abs(op.GetMaxCharSpeed() - orc.GetMaxCharSpeed()) <= 1e-14 * orc.GetMaxCharSpeed()
 
 
+def assert_parity(tag, gpu, orc, u):
+    """1e-12 relative max-norm against the FP64 oracle.  Where the state makes
+    that bar fall below FP64 rounding noise (the oracle itself differs from
+    an extended-precision evaluation by more than 1e-12, e.g. the corrected
+    Taylor-Green initial state, whose RHS is ~1e-3 of its summed fluxes), the
+    GPU must instead be at least as close to the extended-precision RHS as
+    the FP64 oracle is, within a factor 2."""
+    ref = orc.Mult(u)
+    r = rel_max(gpu, ref)
+    if r <= TOL:
+        print(tag, "rel", r, flush=True)
+        return
+    ext = orc.Mult_extended(u)
+    e_gpu, e_orc = rel_max(gpu, ext), rel_max(ref, ext)
+    print(tag, "rel", r, "| vs extended: gpu", e_gpu, "oracle", e_orc, flush=True)
+    assert e_orc > TOL and e_gpu <= 2.0 * e_orc, (tag, r, e_gpu, e_orc)
+
+
 @pytest.mark.parametrize("quirks", [True, False])
 def test_tgv3d_ns_rhs_parity(cuda, quirks):
     cfg = dict(TGV, quirks=quirks)
@@ -75,9 +93,7 @@ def test_tgv3d_ns_rhs_parity(cuda, quirks):
     X, st = states(cfg)
     st["ic"] = cases.taylor_green(X)
     for name, u in st.items():
-        r = rel_max(gpu_rhs(op, u, cuda), orc.Mult(u))
-        print("tgv3d quirks=%d" % quirks, name, r, flush=True)
-        assert r <= TOL, (name, r)
+        assert_parity("tgv3d quirks=%d %s" % (quirks, name), gpu_rhs(op, u, cuda), orc, u)
 
 
 @pytest.mark.parametrize("quirks", [True, False])
