@@ -82,10 +82,23 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="tgv3d_n3", choices=sorted(WORKLOADS))
-    ap.add_argument("--corrected", action="store_true", help="reference defects corrected")
+    ap.add_argument("--faithful", action="store_true",
+                    help="reproduce the reference's defects (its NS-3D path is unstable on this case)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": args.copy_calibration, "launches": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the path is single-GPU in this build "
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
@@ -97,8 +110,12 @@ def main():
 
     name = args.workload
     cfg = dict(WORKLOADS[name])
-    quirks = not args.corrected
+    quirks = bool(args.faithful)
     dev = torch.device("cuda:0")
+    # a non-default stream (default priority): the RK3 step is captured once
+    # as a hipGraph on it and replayed
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     op = DGSEMOperator(quirks=quirks, **cfg)
     u_host = initial_state(name, cfg)
     u = torch.tensor(u_host.reshape(-1), dtype=torch.float64, device=dev)
@@ -114,13 +131,21 @@ def main():
         op.Step(u, dt)
     torch.cuda.synchronize()
 
-    op.profile_begin()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         op.Step(u, dt)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+
+    # per-kernel durations: the same steps again, every launch bracketed by
+    # HIP events on the stream the kernels run on
+    psteps = min(args.steps, 50)
+    op.profile_begin()
+    tp = time.perf_counter()
+    for _ in range(psteps):
+        op.Step(u, dt)
+    torch.cuda.synchronize()
+    elapsed_prof = time.perf_counter() - tp
     prof = op.profile_end()
     ok = op.check_state(u)
 
@@ -169,10 +194,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (analytic initial condition)",
-        "config": {"workload": name + (" (reference defects corrected)" if args.corrected else
-                                      " (reference defects reproduced)"),
+        "config": {"workload": name + (" (reference defects reproduced)" if quirks else
+                                      " (reference defects corrected)"),
                    "elements": op.NE, "order": cfg["order"], "nodes": nodes, "vars": nv,
-                   "rhs_per_step": 3, "dt": dt, "state_ok": bool(ok),
+                   "rhs_per_step": 3, "dt": dt, "state_ok": bool(ok), "hip_graph": True,
+                   "ms_per_step_event_profiled": elapsed_prof / psteps * 1e3,
                    "kernel_ms": {"k_rhs": ms_rhs / max(n_rhs, 1), "k_visc": ms_visc / max(n_visc, 1)}},
         "roofline": roof,
     }

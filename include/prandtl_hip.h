@@ -75,6 +75,12 @@ int prandtl_rhs(prandtl_ctx *ctx, const double *u_dev, double *dudt_dev, void *s
  * kernel; u is advanced in place. */
 int prandtl_rk3_step(prandtl_ctx *ctx, double *u_dev, double dt, void *stream);
 
+/* On a non-null stream the three stages of prandtl_rk3_step are captured
+ * once as a hipGraph (keyed by u_dev and stream; dt is read from device
+ * memory) and replayed on later steps.  enable = 0 launches the kernels
+ * directly every step.  Default: enabled. */
+int prandtl_set_graph(prandtl_ctx *ctx, int enable);
+
 /* DGSEMOperator::GetMaxCharSpeed (DGSEMOperator.hpp:106-109): running
  * maximum of |v|+c over every state the operator has evaluated since
  * creation or the last reset (DGSEMIntegrator.cpp:39,188,270).  Synchronises

@@ -7,7 +7,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libprandtl_hip.so")
+LIB_PATH = os.environ.get("PRANDTL_HIP_LIB") or os.path.join(_HERE, "libprandtl_hip.so")
 
 PRANDTL_OK = 0
 PRANDTL_ERR_ARG = 1
@@ -20,6 +20,7 @@ PRANDTL_MIXED = 1
 
 EXPORTED = (
     "prandtl_create", "prandtl_destroy", "prandtl_sizes", "prandtl_rhs", "prandtl_rk3_step",
+    "prandtl_set_graph",
     "prandtl_max_wave_speed", "prandtl_alpha", "prandtl_check_state", "prandtl_lgl",
     "prandtl_debug_gradients", "prandtl_profile_begin", "prandtl_profile_end", "prandtl_last_error",
 )
@@ -65,6 +66,7 @@ def lib():
     L.prandtl_sizes.argtypes = [vp, ip, ip, ip]
     L.prandtl_rhs.argtypes = [vp, vp, vp, vp]
     L.prandtl_rk3_step.argtypes = [vp, vp, ctypes.c_double, vp]
+    L.prandtl_set_graph.argtypes = [vp, ctypes.c_int]
     L.prandtl_max_wave_speed.argtypes = [vp, dp, ctypes.c_int, vp]
     L.prandtl_alpha.argtypes = [vp, dp, dp, vp]
     L.prandtl_check_state.argtypes = [vp, vp, vp]

@@ -41,6 +41,10 @@ namespace prandtl {
 
 constexpr int MAXNP = 8;
 
+#ifndef PRANDTL_WAVES
+#define PRANDTL_WAVES 3
+#endif
+
 struct Tables {
     double D[MAXNP * MAXNP];    // D_ij (row-major), DGSEMIntegrator.cpp:58-69
     double Dh[MAXNP * MAXNP];   // D + E            (lines 71-74, un-transposed)
@@ -65,6 +69,7 @@ struct KArgs {
     int blend, parabolic, ec, quirks;
     int mode;             // 0: out = rhs; 1: out = y + dt rhs; 2: out = c0 u0 + c1 (y + dt rhs)
     double dt, c0, c1;
+    const double *dtp;    // if set, dt is read from device memory (graph replays)
     const double *y;
     const double *u0;
     double *out;
@@ -86,22 +91,25 @@ struct VArgs {
     double *grad;         // optional [dim][nv][NE][K]
 };
 
-template <int DIM, int NP> struct Cfg {
+template <int DIM, int NP, bool BLEND = false> struct Cfg {
     static constexpr int NV = DIM + 2;
     static constexpr int K = DIM == 2 ? NP * NP : NP * NP * NP;
     static constexpr int NFP = DIM == 2 ? NP : NP * NP;
     static constexpr int NPAIR = NP * (NP - 1) / 2;
     static constexpr int BLOCK = K > 256 ? ((K + 63) / 64) * 64 : 256;
     static constexpr int EPB = BLOCK / K;
-    static constexpr int JPL = NPAIR + 2 + (NP - 1);
+    // jobs per line: volume pairs, the two surface points, subcell interfaces
+    static constexpr int JPL = NPAIR + 2 + (BLEND ? NP - 1 : 0);
     static constexpr int lc_pick()
     {
         for (int lc = NFP; lc >= 1; lc--)
-            if (NFP % lc == 0 && EPB * lc * JPL * NV * 8 <= 48 * 1024) return lc;
+            if (NFP % lc == 0 && EPB * lc * JPL * NV * 8 <= 24 * 1024) return lc;
         return 1;
     }
     static constexpr int LC = lc_pick();
     static constexpr int FXN = LC * JPL;
+    // waves per SIMD to ask the register allocator for
+    static constexpr int WAVES = K > 256 ? 2 : PRANDTL_WAVES;
 };
 
 template <int DIM, int NP>
@@ -179,29 +187,65 @@ __device__ __forceinline__ void block_lam_max(double lam, unsigned long long *ou
 // ---------------------------------------------------------------------------
 // k_rhs
 // ---------------------------------------------------------------------------
-template <int DIM, int NP>
-__global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
+// Launch shape of k_rhs: per element the work items are K nodes, JF surface
+// points (all 2*DIM faces) and, per direction, JP volume pairs plus JV
+// subcell interfaces.  EPB elements per workgroup are chosen so that each
+// phase gives every thread about one item (3-D N=3: 2 elements, 192
+// threads: 96 surface points and 96 pairs per direction per element).
+template <int DIM, int NP, bool BLEND> struct RCfg {
+    static constexpr int NV = DIM + 2;
+    static constexpr int K = DIM == 2 ? NP * NP : NP * NP * NP;
+    static constexpr int NFP = DIM == 2 ? NP : NP * NP;
+    static constexpr int NPAIR = NP * (NP - 1) / 2;
+    static constexpr int NFV = BLEND ? NP - 1 : 0;
+    static constexpr int JP = NFP * (NPAIR + NFV);
+    static constexpr int JF = 2 * DIM * NFP;
+    static constexpr int mx(int a, int b) { return a > b ? a : b; }
+    static constexpr int WORK = mx(K, mx(JP, JF));
+    static constexpr int EPB = K > 128 ? 1 : mx(1, 256 / WORK);
+    static constexpr int r64(int x) { return ((x + 63) / 64) * 64; }
+    static constexpr int BLOCK = K > 256 ? r64(K) : (r64(EPB * WORK) > 512 ? 512 : r64(EPB * WORK));
+    static constexpr int lc_pick()
+    {
+        for (int lc = NFP; lc >= 1; lc--)
+            if (NFP % lc == 0 && EPB * lc * (NPAIR + NFV) * NV * 8 <= 32 * 1024) return lc;
+        return 1;
+    }
+    static constexpr int LC = lc_pick();
+    static constexpr int FXN = LC * (NPAIR + NFV);
+    static constexpr int WAVES = K > 256 ? 2 : PRANDTL_WAVES;
+    static_assert(EPB * K <= BLOCK, "one node per thread");
+    static_assert(!BLEND || 2 * K <= NV * FXN, "indicator scratch fits the flux buffer");
+};
+
+// BLEND: modal indicator + subcell FV blending compiled in.
+// QNS3: the reference's NS-3D zeta-partner defect (faithful NS in 3-D).
+template <int DIM, int NP, bool BLEND, bool QNS3>
+__global__ void __launch_bounds__((RCfg<DIM, NP, BLEND>::BLOCK), (RCfg<DIM, NP, BLEND>::WAVES))
+k_rhs(const KArgs A)
 {
-    using C = Cfg<DIM, NP>;
-    constexpr int NV = C::NV, K = C::K, NFP = C::NFP, NPAIR = C::NPAIR;
-    constexpr int EPB = C::EPB, LC = C::LC, BLOCK = C::BLOCK, FXN = C::FXN;
-    constexpr int GK = DIM == 3 ? K : 1;
+    using C = RCfg<DIM, NP, BLEND>;
+    constexpr int NV = C::NV, K = C::K, NFP = C::NFP, NPAIR = C::NPAIR, NFV = C::NFV;
+    constexpr int EPB = C::EPB, LC = C::LC, BLOCK = C::BLOCK, FXN = C::FXN, JF = C::JF;
+    constexpr int GK = QNS3 ? K : 1, GE = QNS3 ? EPB : 1;
+    constexpr int NQ = DIM + 1 + (BLEND ? 2 : 0);   // v[DIM], beta, (1/beta, lambda)
 
     __shared__ double sDh2[NP * NP];
-    __shared__ double sV1i[NP * NP];
+    __shared__ double sV1i[BLEND ? NP * NP : 1];
     __shared__ double sFvs[NP];
     __shared__ double sSub[DIM][NP];
     __shared__ double sU[EPB][NV][K];
-    __shared__ double sQ[EPB][DIM + 3][K];
-    __shared__ double sF[EPB][NV][FXN];
-    __shared__ double sG[EPB][NV][GK];
+    __shared__ double sQ[EPB][NQ][K];
+    __shared__ double sFc[EPB][NV][JF];    // signed surface contributions
+    __shared__ double sF[EPB][NV][FXN];    // pair and subcell fluxes of one line chunk
+    __shared__ double sG[GE][NV][GK];
     __shared__ double sAl[EPB];
     __shared__ double sRed[(BLOCK + 63) / 64];
 
     const int t = threadIdx.x;
     for (int i = t; i < NP * NP; i += BLOCK) {
         sDh2[i] = A.tab.Dh2[(i / NP) * MAXNP + i % NP];
-        sV1i[i] = A.tab.V1i[(i / NP) * MAXNP + i % NP];
+        if (BLEND) sV1i[i] = A.tab.V1i[(i / NP) * MAXNP + i % NP];
     }
     for (int i = t; i < NP; i += BLOCK) sFvs[i] = A.tab.fvs[i];
     for (int i = t; i < DIM * NP; i += BLOCK) sSub[i / NP][i % NP] = A.tab.subn[i / NP][i % NP];
@@ -214,34 +258,59 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
     const int n = t - el * K;
     const int e = ebase + el;
     const bool act = (el < EPB) && (e < NE);
-    const bool quirkNS3 = (DIM == 3) && A.parabolic && A.quirks;
     const bool quirk3d = A.quirks != 0;
     const int ii[3] = {n % NP, (n / NP) % NP, DIM == 3 ? n / (NP * NP) : 0};
 
+    // ---- prologue: own node state, and the neighbour data of this thread's
+    // first surface point, all loads in flight together ----
+    double ys[NV];
+    if (act)
+#pragma unroll
+        for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)e * K + n];
+    double pn[NV], pfo[NV], pfn[NV];
+    {
+        const int jf = t;
+        const int ej = jf / JF;
+        const int eo = ebase + ej;
+        if (jf < EPB * JF && eo < NE) {
+            const int rem = jf - ej * JF;
+            const int f = rem / NFP, l = rem - f * NFP;
+            const int r = f >> 1, side = f & 1;
+            const int en = neighbor(A.geo, eo, r, side);
+            const int nn = line_node<DIM, NP>(r, l, side ? 0 : NP - 1);
+#pragma unroll
+            for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)en * K + nn];
+            if (A.parabolic) {
+                const double *fo = A.fvf + (((size_t)eo * 2 * DIM + f) * (NV - 1)) * NFP;
+                const double *fn = A.fvf + (((size_t)en * 2 * DIM + (f ^ 1)) * (NV - 1)) * NFP;
+#pragma unroll
+                for (int v = 1; v < NV; v++) { pfo[v] = fo[(v - 1) * NFP + l]; pfn[v] = fn[(v - 1) * NFP + l]; }
+            }
+        }
+    }
+
     // ---- node phase: state and per-node flux quantities into LDS ----
     double lam = 0.0;
-    double ys[NV];
     if (act) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            ys[v] = A.y[v * NK + (size_t)e * K + n];
-            sU[el][v][n] = ys[v];
-        }
+        for (int v = 0; v < NV; v++) sU[el][v][n] = ys[v];
         const NodeQ<DIM> q = node_quants<DIM>(ys, P);
 #pragma unroll
         for (int d = 0; d < DIM; d++) sQ[el][d][n] = q.v[d];
-        sQ[el][DIM][n] = q.p;
-        sQ[el][DIM + 1][n] = q.beta;
-        sQ[el][DIM + 2][n] = q.lam;
+        sQ[el][DIM][n] = q.beta;
+        if (BLEND) {
+            sQ[el][DIM + 1][n] = q.ibeta;
+            sQ[el][DIM + 2][n] = q.lam;
+        }
         lam = (q.lam == q.lam) ? q.lam : 0.0;
-        if (quirkNS3 && ii[1] == NP - 1)
+        if (QNS3 && ii[1] == NP - 1)
 #pragma unroll
             for (int v = 0; v < NV; v++) sG[el][v][n] = 0.0;
     }
     __syncthreads();
 
     // ---- modal indicator and blending coefficient (element-local) ----
-    if (A.blend) {
+    if (BLEND) {
         double *sX = &sF[0][0][0];
         double *sY = sX + EPB * K;
         if (act) {
@@ -309,18 +378,82 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
         }
         __syncthreads();
     }
-    const double alpha_e = (A.blend && act) ? sAl[el] : 0.0;
+    const double alpha_e = (BLEND && act) ? sAl[el] : 0.0;
+
+    // ---- surface points of all faces (canonical orientation: elem1 = low
+    // side, normal +e_r), signed contribution for this element ----
+    for (int jf = t; jf < EPB * JF; jf += BLOCK) {
+        const int ej = jf / JF;
+        const int eo = ebase + ej;
+        if (eo >= NE) continue;
+        const int rem = jf - ej * JF;
+        const int f = rem / NFP, l = rem - f * NFP;
+        const int r = f >> 1, side = f & 1;
+        const double nr = A.geo.nrm[r];
+        if (jf != t) {
+            // more surface points than threads: load this one now
+            const int en = neighbor(A.geo, eo, r, side);
+            const int nn = line_node<DIM, NP>(r, l, side ? 0 : NP - 1);
+#pragma unroll
+            for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)en * K + nn];
+            if (A.parabolic) {
+                const double *fo = A.fvf + (((size_t)eo * 2 * DIM + f) * (NV - 1)) * NFP;
+                const double *fn = A.fvf + (((size_t)en * 2 * DIM + (f ^ 1)) * (NV - 1)) * NFP;
+#pragma unroll
+                for (int v = 1; v < NV; v++) { pfo[v] = fo[(v - 1) * NFP + l]; pfn[v] = fn[(v - 1) * NFP + l]; }
+            }
+        }
+        const int no = line_node<DIM, NP>(r, l, side ? NP - 1 : 0);
+        double so[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) so[v] = sU[ej][v][no];
+        double *s1 = side ? so : pn;
+        double *s2 = side ? pn : so;
+        double fvn[NV];
+        fvn[0] = 0.0;
+        if (A.parabolic) {
+#pragma unroll
+            for (int v = 1; v < NV; v++) fvn[v] = ((pfo[v] + pfn[v]) * 0.5) * nr;
+            if (A.quirks) {
+                // ComputeViscousFlux left velocities in the momentum slots of
+                // state1/state2 (DGSEMIntegrator.cpp:471-481)
+                const double r1 = 1.0 / s1[0], r2 = 1.0 / s2[0];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) { s1[1 + d] *= r1; s2[1 + d] *= r2; }
+            }
+        }
+        const NodeQ<DIM> q1 = node_quants<DIM>(s1, P);
+        const NodeQ<DIM> q2 = node_quants<DIM>(s2, P);
+        double F[NV];
+        if (A.ec) two_point_flux<DIM, false>(r, nr, q1, q2, nullptr, nullptr, P, false, F);
+        else two_point_flux<DIM, true>(r, nr, q1, q2, s1, s2, P, quirk3d, F);
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            double d;
+            if (A.parabolic) d = side ? (-F[v] + fvn[v]) : (F[v] - fvn[v]);
+            else d = side ? -F[v] : F[v];
+            sFc[ej][v][rem] = d;
+        }
+    }
 
     double accV[NV], accS[NV], accF[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) { accV[v] = 0.0; accS[v] = 0.0; accF[v] = 0.0; }
+    double u0v[NV], vis[NV];
 
-    const int nfv = A.blend ? NP - 1 : 0;
     for (int r = 0; r < DIM; r++) {
         const double nr = A.geo.nrm[r];
+        if (r == DIM - 1 && act) {
+            // inputs of the final assembly, in flight during the last direction
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                u0v[v] = (A.mode == 2) ? A.u0[v * NK + (size_t)e * K + n] : 0.0;
+                vis[v] = (A.parabolic && v > 0) ? A.visc[(v - 1) * NK + (size_t)e * K + n] : 0.0;
+            }
+        }
         for (int l0 = 0; l0 < NFP; l0 += LC) {
-            const int npj = EPB * LC * NPAIR, nfj = EPB * LC * 2, nvj = EPB * LC * nfv;
-            for (int j = t; j < npj + nfj + nvj; j += BLOCK) {
+            const int npj = EPB * LC * NPAIR, nvj = EPB * LC * NFV;
+            for (int j = t; j < npj + nvj; j += BLOCK) {
                 if (j < npj) {
                     // ---- symmetric volume pair (a < b) on line l ----
                     const int ej = j / (LC * NPAIR);
@@ -335,72 +468,22 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
                     qa.rho = sU[ej][0][na]; qb.rho = sU[ej][0][nb];
 #pragma unroll
                     for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][na]; qb.v[d] = sQ[ej][d][nb]; }
-                    qa.beta = sQ[ej][DIM + 1][na]; qb.beta = sQ[ej][DIM + 1][nb];
+                    qa.beta = sQ[ej][DIM][na]; qb.beta = sQ[ej][DIM][nb];
                     double F[NV];
                     two_point_flux<DIM, false>(r, nr, qa, qb, nullptr, nullptr, P, false, F);
 #pragma unroll
                     for (int v = 0; v < NV; v++) sF[ej][v][lc * NPAIR + p] = F[v];
-                    if (quirkNS3 && r == 1 && b == NP - 1)
+                    if (QNS3 && r == 1 && b == NP - 1)
 #pragma unroll
                         for (int v = 0; v < NV; v++) sG[ej][v][na] = F[v];
-                } else if (j < npj + nfj) {
-                    // ---- surface point: side 0 = -r face, 1 = +r face ----
-                    const int jf = j - npj;
-                    const int ej = jf / (LC * 2);
-                    const int rem = jf - ej * (LC * 2);
-                    const int lc = rem >> 1, side = rem & 1;
-                    const int eo = ebase + ej;
-                    if (eo >= NE) continue;
-                    const int l = l0 + lc;
-                    const int no = line_node<DIM, NP>(r, l, side ? NP - 1 : 0);
-                    const int en = neighbor(A.geo, eo, r, side);
-                    const int nn = line_node<DIM, NP>(r, l, side ? 0 : NP - 1);
-                    double so[NV], sn[NV];
-#pragma unroll
-                    for (int v = 0; v < NV; v++) {
-                        so[v] = sU[ej][v][no];
-                        sn[v] = A.y[v * NK + (size_t)en * K + nn];
-                    }
-                    // canonical orientation: elem1 = low side, normal +e_r
-                    double *s1 = side ? so : sn;
-                    double *s2 = side ? sn : so;
-                    double fvn[NV];
-                    fvn[0] = 0.0;
-                    if (A.parabolic) {
-                        const double *fo = A.fvf + (((size_t)eo * 2 * DIM + 2 * r + side) * (NV - 1)) * NFP;
-                        const double *fn = A.fvf + (((size_t)en * 2 * DIM + 2 * r + (1 - side)) * (NV - 1)) * NFP;
-#pragma unroll
-                        for (int v = 1; v < NV; v++)
-                            fvn[v] = ((fo[(v - 1) * NFP + l] + fn[(v - 1) * NFP + l]) * 0.5) * nr;
-                        if (A.quirks) {
-                            // ComputeViscousFlux left velocities in the momentum
-                            // slots of state1/state2 (DGSEMIntegrator.cpp:471-481)
-                            const double r1 = 1.0 / s1[0], r2 = 1.0 / s2[0];
-#pragma unroll
-                            for (int d = 0; d < DIM; d++) { s1[1 + d] *= r1; s2[1 + d] *= r2; }
-                        }
-                    }
-                    const NodeQ<DIM> q1 = node_quants<DIM>(s1, P);
-                    const NodeQ<DIM> q2 = node_quants<DIM>(s2, P);
-                    double F[NV];
-                    if (A.ec) two_point_flux<DIM, false>(r, nr, q1, q2, nullptr, nullptr, P, false, F);
-                    else two_point_flux<DIM, true>(r, nr, q1, q2, s1, s2, P, quirk3d, F);
-                    const int slot = LC * NPAIR + lc * 2 + side;
-#pragma unroll
-                    for (int v = 0; v < NV; v++) {
-                        double d;
-                        if (A.parabolic) d = side ? (-F[v] + fvn[v]) : (F[v] - fvn[v]);
-                        else d = side ? -F[v] : F[v];
-                        sF[ej][v][slot] = d;
-                    }
                 } else {
                     // ---- subcell FV interface i+1/2 on line l ----
-                    const int jv = j - npj - nfj;
-                    const int ej = jv / (LC * nfv);
-                    const int rem = jv - ej * (LC * nfv);
-                    const int lc = rem / nfv, i = rem - lc * nfv;
+                    const int jv = j - npj;
+                    const int ej = jv / (LC * NFV);
+                    const int rem = jv - ej * (LC * NFV);
+                    const int lc = rem / NFV, i = rem - lc * NFV;
                     if (ebase + ej >= NE) continue;
-                    const int slot = LC * (NPAIR + 2) + lc * (NP - 1) + i;
+                    const int slot = LC * NPAIR + lc * NFV + i;
                     if (!(sAl[ej] > 0.0)) {
 #pragma unroll
                         for (int v = 0; v < NV; v++) sF[ej][v][slot] = 0.0;
@@ -415,9 +498,9 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
                     qa.rho = s1[0]; qb.rho = s2[0];
 #pragma unroll
                     for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][na]; qb.v[d] = sQ[ej][d][nb]; }
-                    qa.p = sQ[ej][DIM][na]; qb.p = sQ[ej][DIM][nb];
-                    qa.beta = sQ[ej][DIM + 1][na]; qb.beta = sQ[ej][DIM + 1][nb];
-                    qa.lam = sQ[ej][DIM + 2][na]; qb.lam = sQ[ej][DIM + 2][nb];
+                    qa.beta = sQ[ej][DIM][na]; qb.beta = sQ[ej][DIM][nb];
+                    qa.ibeta = sQ[ej][NQ - 2][na]; qb.ibeta = sQ[ej][NQ - 2][nb];
+                    qa.lam = sQ[ej][NQ - 1][na]; qb.lam = sQ[ej][NQ - 1][nb];
                     double F[NV];
                     two_point_flux<DIM, true>(r, sSub[r][i], qa, qb, s1, s2, P, quirk3d, F);
 #pragma unroll
@@ -434,7 +517,7 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
                     for (int m = 0; m < NP; m++) {
                         if (m == a) continue;
                         const double dm = sDh2[a * NP + m];
-                        if (quirkNS3 && r == 2 && m < a) {
+                        if (QNS3 && r == 2 && m < a) {
                             // H_inviscid(id2).SetCol(k, g) (DGSEMIntegrator.cpp:702)
                             const int nm = line_node<DIM, NP>(r, l, m);
 #pragma unroll
@@ -447,12 +530,12 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
                     }
                     if (a == 0)
 #pragma unroll
-                        for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sF[el][v][LC * NPAIR + lc * 2 + 0];
+                        for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sFc[el][v][(2 * r) * NFP + l];
                     if (a == NP - 1)
 #pragma unroll
-                        for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sF[el][v][LC * NPAIR + lc * 2 + 1];
-                    if (A.blend && alpha_e > 0.0) {
-                        const int base = LC * (NPAIR + 2) + lc * (NP - 1);
+                        for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sFc[el][v][(2 * r + 1) * NFP + l];
+                    if (BLEND && alpha_e > 0.0) {
+                        const int base = LC * NPAIR + lc * NFV;
 #pragma unroll
                         for (int v = 0; v < NV; v++) {
                             double val;
@@ -469,20 +552,22 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
     }
 
     // ---- assemble, stage update, store ----
+    const double dt = A.dtp ? *A.dtp : A.dt;
     if (act) {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             double vol = -accV[v];
-            if (A.blend) vol *= (1.0 - alpha_e);
-            if (A.parabolic && v > 0) vol = vol + A.visc[(v - 1) * NK + (size_t)e * K + n];
+            if (BLEND) vol *= (1.0 - alpha_e);
+            if (A.parabolic && v > 0) vol = vol + vis[v];
             vol *= A.geo.invJ;
-            double tot = (A.blend && alpha_e > 0.0) ? accF[v] * alpha_e + vol : vol;
+            double tot = (BLEND && alpha_e > 0.0) ? accF[v] * alpha_e + vol : vol;
             tot += accS[v];
             const size_t gi = v * NK + (size_t)e * K + n;
+            const double yv = sU[el][v][n];
             double o;
             if (A.mode == 0) o = tot;
-            else if (A.mode == 1) o = ys[v] + A.dt * tot;
-            else o = A.c0 * A.u0[gi] + A.c1 * (ys[v] + A.dt * tot);
+            else if (A.mode == 1) o = yv + dt * tot;
+            else o = A.c0 * u0v[v] + A.c1 * (yv + dt * tot);
             A.out[gi] = o;
         }
     }
@@ -493,17 +578,23 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_rhs(const KArgs A)
 // k_visc: entropy variables, BR1 lifting, gradient conversion, viscous fluxes
 // ---------------------------------------------------------------------------
 template <int DIM, int NP>
-__global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_visc(const VArgs A)
+__global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) k_visc(const VArgs A)
 {
     using C = Cfg<DIM, NP>;
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, EPB = C::EPB, BLOCK = C::BLOCK;
+    // phase 1 (lifting) needs the entropy variables and neighbour traces,
+    // phase 2 (viscous divergence) the contravariant fluxes: one LDS region
+    constexpr int P1 = EPB * NV * K + EPB * 2 * DIM * NV * NFP;
+    constexpr int P2 = EPB * DIM * (NV - 1) * K;
+    constexpr int SCR = P1 > P2 ? P1 : P2;
 
     __shared__ double sD[NP * NP];
     __shared__ double sDh[NP * NP];
     __shared__ double sU[EPB][NV][K];
-    __shared__ double sW[EPB][NV][K];
-    __shared__ double sWN[EPB][2 * DIM][NV][NFP];
-    __shared__ double sFV[EPB][DIM][NV - 1][K];
+    __shared__ double sScr[SCR];
+    double (*sW)[NV][K] = reinterpret_cast<double (*)[NV][K]>(sScr);
+    double (*sWN)[2 * DIM][NV][NFP] = reinterpret_cast<double (*)[2 * DIM][NV][NFP]>(sScr + EPB * NV * K);
+    double (*sFV)[DIM][NV - 1][K] = reinterpret_cast<double (*)[DIM][NV - 1][K]>(sScr);
 
     const int t = threadIdx.x;
     for (int i = t; i < NP * NP; i += BLOCK) {
@@ -551,10 +642,10 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_visc(const VArgs A)
     }
     __syncthreads();
 
+    double gp[DIM][NV];
     if (act) {
         // volume part: grad_r w = sum_l D_{i_r l} w_l, times adj / J
         // (LiftingBR1.cpp:176-215)
-        double gp[DIM][NV];
 #pragma unroll
         for (int c = 0; c < DIM; c++) {
             const int a = ii[c];
@@ -587,7 +678,10 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_visc(const VArgs A)
             for (int c = 0; c < DIM; c++)
 #pragma unroll
                 for (int v = 0; v < NV; v++) A.grad[(c * NV + v) * NK + (size_t)e * K + n] = gp[c][v];
+    }
+    __syncthreads();   // sW/sWN are dead from here; sFV reuses their storage
 
+    if (act) {
         double fl[DIM + 1][DIM];
         // volume viscous flux uses the stale node K-1 state (DGSEMIntegrator.cpp:620)
         double sv[NV];
@@ -642,6 +736,7 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK)) k_visc(const VArgs A)
 // small utility kernels
 // ---------------------------------------------------------------------------
 __global__ void k_reset_u64(unsigned long long *p) { *p = 0ull; }
+__global__ void k_set_double(double *p, double v) { *p = v; }
 __global__ void k_reset_int(int *p) { *p = 0; }
 
 template <int DIM>
@@ -757,6 +852,14 @@ struct prandtl_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kind;   // 0 = k_rhs, 1 = k_visc
     size_t ev_used = 0;
+    // SSP-RK3 step captured as a hipGraph (dt read from d_dt), replayed per step
+    bool use_graph = true;
+    double *d_dt = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const double *graph_u = nullptr;
+    hipStream_t graph_stream = nullptr;
+    const double *dtp = nullptr;  // set while launching stages for a graph
 };
 
 namespace {
@@ -814,23 +917,33 @@ template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, doub
     return hip_check(c, hipGetLastError(), "k_visc launch");
 }
 
+template <int DIM, int NP, bool BLEND>
+int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
+{
+    using Cf = RCfg<DIM, NP, BLEND>;
+    const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+    ProfScope ps(c, s, 0);
+    const bool qns3 = DIM == 3 && c->prm.parabolic && c->prm.quirks;
+    if (qns3) hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, DIM == 3>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    return hip_check(c, hipGetLastError(), "k_rhs launch");
+}
+
 template <int DIM, int NP>
 int launch_rhs(prandtl_ctx *c, const double *y, const double *u0, double *out, int mode, double dt,
                double c0, double c1, hipStream_t s)
 {
-    using Cf = Cfg<DIM, NP>;
     KArgs a;
     a.tab = c->tab; a.geo = c->geo; a.phys = c->phys;
     a.blend = c->prm.blending; a.parabolic = c->prm.parabolic;
     a.ec = c->prm.ec_surface; a.quirks = c->prm.quirks;
     a.mode = mode; a.dt = dt; a.c0 = c0; a.c1 = c1;
+    a.dtp = c->dtp;
     a.y = y; a.u0 = u0; a.out = out;
     a.visc = c->d_visc; a.fvf = c->d_fvf; a.alpha = c->d_alpha; a.eta = c->d_eta;
     a.lam = c->d_lam;
-    const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
-    ProfScope ps(c, s, 0);
-    hipLaunchKernelGGL((k_rhs<DIM, NP>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
-    return hip_check(c, hipGetLastError(), "k_rhs launch");
+    if (c->prm.blending) return launch_rhs_t<DIM, NP, true>(c, a, s);
+    return launch_rhs_t<DIM, NP, false>(c, a, s);
 }
 
 template <int DIM, int NP>
@@ -1010,6 +1123,7 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     if (e == hipSuccess) e = hipMalloc(&c->d_yb, sizeof(double) * c->nv * NK);
     if (e == hipSuccess) e = hipMalloc(&c->d_lam, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&c->d_flag, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&c->d_dt, sizeof(double));
     if (e == hipSuccess) e = hipMemset(c->d_alpha, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_eta, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_lam, 0, sizeof(unsigned long long));
@@ -1031,6 +1145,9 @@ void prandtl_destroy(prandtl_ctx *c)
         if (b) (void)hipFree(b);
     if (c->d_lam) (void)hipFree(c->d_lam);
     if (c->d_flag) (void)hipFree(c->d_flag);
+    if (c->d_dt) (void)hipFree(c->d_dt);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     delete c;
 }
@@ -1049,15 +1166,50 @@ int prandtl_rhs(prandtl_ctx *c, const double *u, double *dudt, void *stream)
     return stage(c, u, nullptr, dudt, 0, 0.0, 0.0, 0.0, (hipStream_t)stream);
 }
 
-int prandtl_rk3_step(prandtl_ctx *c, double *u, double dt, void *stream)
+static int rk3_stages(prandtl_ctx *c, double *u, double dt, hipStream_t s)
 {
-    if (!c || !u) return fail(c, PRANDTL_ERR_ARG, "null argument");
-    hipStream_t s = (hipStream_t)stream;
     // y1 = u + dt L(u); y2 = 3/4 u + 1/4 (y1 + dt L(y1)); u = 1/3 u + 2/3 (y2 + dt L(y2))
     int rc = stage(c, u, nullptr, c->d_ya, 1, dt, 0.0, 0.0, s);
     if (!rc) rc = stage(c, c->d_ya, u, c->d_yb, 2, dt, 3.0 / 4, 1.0 / 4, s);
     if (!rc) rc = stage(c, c->d_yb, u, u, 2, dt, 1.0 / 3, 2.0 / 3, s);
     return rc;
+}
+
+int prandtl_rk3_step(prandtl_ctx *c, double *u, double dt, void *stream)
+{
+    if (!c || !u) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    // The null stream cannot be captured; profiling brackets single launches.
+    if (!c->use_graph || c->prof || s == nullptr) return rk3_stages(c, u, dt, s);
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(1), 0, s, c->d_dt, dt);
+    int rc = hip_check(c, hipGetLastError(), "set dt");
+    if (rc) return rc;
+    if (!c->graph_exec || c->graph_u != u || c->graph_stream != s) {
+        if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+        if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+        rc = hip_check(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        if (rc) return rc;
+        c->dtp = c->d_dt;
+        const int rs = rk3_stages(c, u, dt, s);
+        c->dtp = nullptr;
+        hipGraph_t g = nullptr;
+        rc = hip_check(c, hipStreamEndCapture(s, &g), "end capture");
+        if (rs) { if (g) (void)hipGraphDestroy(g); return rs; }
+        if (rc) return rc;
+        c->graph = g;
+        rc = hip_check(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0), "graph instantiate");
+        if (rc) return rc;
+        c->graph_u = u;
+        c->graph_stream = s;
+    }
+    return hip_check(c, hipGraphLaunch(c->graph_exec, s), "graph launch");
+}
+
+int prandtl_set_graph(prandtl_ctx *c, int enable)
+{
+    if (!c) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    c->use_graph = enable != 0;
+    return PRANDTL_OK;
 }
 
 int prandtl_max_wave_speed(prandtl_ctx *c, double *out, int reset, void *stream)

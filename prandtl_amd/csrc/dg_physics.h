@@ -20,42 +20,67 @@ struct Phys {
 
 // Per-node quantities shared by every two-point flux touching the node.
 template <int DIM> struct NodeQ {
-    double rho, v[DIM], p, beta, lam;
+    double rho, v[DIM], beta, ibeta, lam;
 };
 
-// rho, v = m/rho, p = (gamma-1)(E - rho|v|^2/2), beta = rho/(2p),
-// lam = |v| + sqrt(gamma p/rho)  (ChandrashekarFlux.cpp:17-68)
+// rho, v = m/rho, p = (gamma-1)(E - rho|v|^2/2), beta = rho/(2p), 1/beta,
+// lam = |v| + sqrt(gamma p/rho)  (ChandrashekarFlux.cpp:17-68).  One
+// reciprocal of rho replaces the reference's per-component divisions.
 template <int DIM>
 __device__ __forceinline__ NodeQ<DIM> node_quants(const double *s, const Phys &P)
 {
     NodeQ<DIM> q;
     q.rho = s[0];
+    const double ri = 1.0 / q.rho;
     double vsq = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        q.v[d] = s[1 + d] / q.rho;
+        q.v[d] = s[1 + d] * ri;
         vsq = (d == 0) ? q.v[d] * q.v[d] : vsq + q.v[d] * q.v[d];
     }
-    q.p = P.gm1 * (s[DIM + 1] - 0.5 * q.rho * vsq);
-    q.beta = 0.5 * q.rho / q.p;
-    q.lam = sqrt(vsq) + sqrt(P.gamma * q.p / q.rho);
+    const double p = P.gm1 * (s[DIM + 1] - 0.5 * q.rho * vsq);
+    q.beta = 0.5 * q.rho / p;
+    q.ibeta = 2.0 * p * ri;
+    q.lam = sqrt(vsq) + sqrt(P.gamma * p * ri);
     return q;
 }
 
-// ComputeLogMean, BasicOperations.cpp:6-12 (eps = 1e-4)
+// Logarithmic mean LM(x,y) and its reciprocal, ComputeLogMean of
+// BasicOperations.cpp:6-12 (eps = 1e-4).  With d = y-x, s = y+x the
+// reference's u = ((xi-1)/(xi+1))^2 with xi = y/x is d^2/s^2, so the branch
+// test u < eps is q < eps*t (q = d^2, t = s^2) and the series branch
+//   LM = 52.5 s / (105 + 35u + 21u^2 + 15u^3)
+//      = 52.5 s t^3 / (((15 q + 21 t) q + 35 t^2) q + 105 t^3)
+// costs one division instead of three.  The log branch keeps the reference's
+// (y-x)/log(y/x).  Either form differs from the reference's by rounding only
+// (t^3 stays in range for |x|,|y| in 1e-40..1e40).
+__device__ __forceinline__ double lm_series_den(double q, double t)
+{
+    return ((15.0 * q + 21.0 * t) * q + 35.0 * t * t) * q + 105.0 * t * t * t;
+}
+
 __device__ __forceinline__ double logmean(double x, double y)
 {
-    const double xi = y / x;
-    const double u = (xi * (xi - 2.0) + 1.0) / (xi * (xi + 2.0) + 1.0);
-    return (u < 1e-4) ? (x + y) * 52.5 / (105.0 + u * (35.0 + u * (21.0 + 15.0 * u)))
-                      : (y - x) / log(xi);
+    const double d = y - x, s = y + x;
+    const double q = d * d, t = s * s;
+    if (q < 1e-4 * t) return 52.5 * s * (t * t * t) / lm_series_den(q, t);
+    return d / log(y / x);
+}
+
+__device__ __forceinline__ double inv_logmean(double x, double y)
+{
+    const double d = y - x, s = y + x;
+    const double q = d * d, t = s * s;
+    if (q < 1e-4 * t) return lm_series_den(q, t) / (52.5 * s * (t * t * t));
+    return log(y / x) / d;
 }
 
 // Chandrashekar two-point flux along direction r with normal magnitude nr.
 // DISS = false: ComputeVolumeFlux (ChandrashekarFlux.cpp:11-83), metric mean
 // of two equal metrics.  DISS = true: ComputeFaceFlux (lines 85-161) with the
 // conserved states s1, s2 for the jumps; quirk3d keeps the 3-D energy
-// dissipation term of line 131.
+// dissipation term of line 131.  Only 1/beta_ln enters the fluxes, so the
+// beta log mean is formed as its reciprocal.
 template <int DIM, bool DISS>
 __device__ __forceinline__ void two_point_flux(int r, double nr, const NodeQ<DIM> &a,
                                                const NodeQ<DIM> &b, const double *s1,
@@ -85,9 +110,9 @@ __device__ __forceinline__ void two_point_flux(int r, double nr, const NodeQ<DIM
     for (int d = 1; d < DIM; d++)
         if (d == r) vr = vm[d];
     const double Vn = vr * nr;
-    const double beta_ln = logmean(a.beta, b.beta);
+    const double ibeta_ln = inv_logmean(a.beta, b.beta);
     const double p_hat = 0.5 * (a.rho + b.rho) / (a.beta + b.beta);
-    h_hat += 0.5 / beta_ln * P.gm1i + p_hat / rho_ln;
+    h_hat += 0.5 * ibeta_ln * P.gm1i + p_hat / rho_ln;
     const double rVn = rho_ln * Vn;
     F[0] = rVn;
 #pragma unroll
@@ -97,7 +122,7 @@ __device__ __forceinline__ void two_point_flux(int r, double nr, const NodeQ<DIM
     }
     F[DIM + 1] = rVn * h_hat;
     if (DISS) {
-        diss += 0.5 * drho * P.gm1i / beta_ln + 0.5 * rho_mean * P.gm1i * (1.0 / b.beta - 1.0 / a.beta);
+        diss += 0.5 * drho * P.gm1i * ibeta_ln + 0.5 * rho_mean * P.gm1i * (b.ibeta - a.ibeta);
         const double lam = (a.lam < b.lam) ? b.lam : a.lam;
         const double hl = 0.5 * lam;
         F[0] = F[0] - hl * (s2[0] - s1[0]) * nr;
@@ -121,7 +146,9 @@ __device__ __forceinline__ void cons2entropy(const double *s, const Phys &P, dou
     const double v_sq = 0.5 * vv;
     const double p = P.gm1 * (s[DIM + 1] - s[0] * v_sq);
     const double beta = s[0] / p;
-    const double sent = log(p * pow(s[0], -P.gamma));
+    // log(p rho^-gamma) as log p - gamma log rho: two logs instead of a log
+    // and a pow (rounding-level difference)
+    const double sent = log(p) - P.gamma * log(s[0]);
     w[0] = (P.gamma - sent) * P.gm1i - beta * v_sq;
 #pragma unroll
     for (int d = 0; d < DIM; d++) w[1 + d] = beta * vel[d];

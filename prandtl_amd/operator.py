@@ -96,6 +96,10 @@ class DGSEMOperator:
         """u <- one SSP-RK3 step of size dt, in place."""
         check(lib().prandtl_rk3_step(self._h, self._vec(u, "u"), float(dt), _stream_ptr()), self._h)
 
+    def set_graph(self, enable):
+        """Capture/replay the RK3 step as a hipGraph on non-default streams."""
+        check(lib().prandtl_set_graph(self._h, int(enable)), self._h)
+
     def GetMaxCharSpeed(self, reset=False):
         out = ctypes.c_double()
         check(lib().prandtl_max_wave_speed(self._h, ctypes.byref(out), int(reset), _stream_ptr()), self._h)
