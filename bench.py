@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--workload", default="tgv3d_n3", choices=sorted(WORKLOADS))
     ap.add_argument("--faithful", action="store_true",
                     help="reproduce the reference's defects (its NS-3D path is unstable on this case)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -116,7 +117,7 @@ def main():
     # as a hipGraph on it and replayed
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.set_stream(stream)
-    op = DGSEMOperator(quirks=quirks, **cfg)
+    op = DGSEMOperator(quirks=quirks, precision=args.precision, **cfg)
     u_host = initial_state(name, cfg)
     u = torch.tensor(u_host.reshape(-1), dtype=torch.float64, device=dev)
     nodes = op.NE * op.K
@@ -192,7 +193,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64 (FP32 BR1/viscous)",
         "data": "synthetic (analytic initial condition)",
         "config": {"workload": name + (" (reference defects reproduced)" if quirks else
                                       " (reference defects corrected)"),

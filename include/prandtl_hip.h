@@ -52,7 +52,11 @@ typedef struct prandtl_params {
     int blending;       /* modal indicator + subcell FV blending */
     int ec_surface;     /* 1: entropy-conservative two-point flux at faces */
     int quirks;         /* 1: reproduce the reference's defects, 0: corrected */
-    int precision;      /* PRANDTL_FP64 or PRANDTL_MIXED */
+    int precision;      /* PRANDTL_FP64, or PRANDTL_MIXED: BR1 gradients, viscous
+                           fluxes and their derivative sums in FP32 (entropy
+                           variables, two-point fluxes, indicator and RK
+                           accumulation stay FP64); Navier-Stokes with quirks = 0
+                           only, identical to FP64 for Euler */
 } prandtl_params;
 
 /* DGSEMOperator constructor (DGSEMOperator.hpp:87-99): builds LGL rule,

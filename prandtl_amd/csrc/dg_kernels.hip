@@ -32,6 +32,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/prandtl_hip.h"
@@ -73,8 +74,8 @@ struct KArgs {
     const double *y;
     const double *u0;
     double *out;
-    const double *visc;   // [nv-1][NE][K]
-    const double *fvf;    // [NE][2 dim][nv-1][Nfp]
+    const void *visc;     // [nv-1][NE][K]            (FP32 in the mixed form)
+    const void *fvf;      // [NE][2 dim][nv-1][Nfp]   (FP32 in the mixed form)
     double *alpha;        // [NE]
     double *eta;          // [NE]
     unsigned long long *lam;
@@ -86,8 +87,8 @@ struct VArgs {
     Phys phys;
     int quirks;
     const double *y;
-    double *visc;
-    double *fvf;
+    void *visc;
+    void *fvf;
     double *grad;         // optional [dim][nv][NE][K]
 };
 
@@ -220,7 +221,8 @@ template <int DIM, int NP, bool BLEND> struct RCfg {
 
 // BLEND: modal indicator + subcell FV blending compiled in.
 // QNS3: the reference's NS-3D zeta-partner defect (faithful NS in 3-D).
-template <int DIM, int NP, bool BLEND, bool QNS3>
+// VT: element type of the viscous inputs from k_visc (float in the mixed form).
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT>
 __global__ void __launch_bounds__((RCfg<DIM, NP, BLEND>::BLOCK), (RCfg<DIM, NP, BLEND>::WAVES))
 k_rhs(const KArgs A)
 {
@@ -281,8 +283,8 @@ k_rhs(const KArgs A)
 #pragma unroll
             for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)en * K + nn];
             if (A.parabolic) {
-                const double *fo = A.fvf + (((size_t)eo * 2 * DIM + f) * (NV - 1)) * NFP;
-                const double *fn = A.fvf + (((size_t)en * 2 * DIM + (f ^ 1)) * (NV - 1)) * NFP;
+                const VT *fo = static_cast<const VT *>(A.fvf) + (((size_t)eo * 2 * DIM + f) * (NV - 1)) * NFP;
+                const VT *fn = static_cast<const VT *>(A.fvf) + (((size_t)en * 2 * DIM + (f ^ 1)) * (NV - 1)) * NFP;
 #pragma unroll
                 for (int v = 1; v < NV; v++) { pfo[v] = fo[(v - 1) * NFP + l]; pfn[v] = fn[(v - 1) * NFP + l]; }
             }
@@ -298,7 +300,7 @@ k_rhs(const KArgs A)
 #pragma unroll
         for (int d = 0; d < DIM; d++) sQ[el][d][n] = q.v[d];
         sQ[el][DIM][n] = q.beta;
-        if (BLEND) {
+        if constexpr (BLEND) {
             sQ[el][DIM + 1][n] = q.ibeta;
             sQ[el][DIM + 2][n] = q.lam;
         }
@@ -397,8 +399,8 @@ k_rhs(const KArgs A)
 #pragma unroll
             for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)en * K + nn];
             if (A.parabolic) {
-                const double *fo = A.fvf + (((size_t)eo * 2 * DIM + f) * (NV - 1)) * NFP;
-                const double *fn = A.fvf + (((size_t)en * 2 * DIM + (f ^ 1)) * (NV - 1)) * NFP;
+                const VT *fo = static_cast<const VT *>(A.fvf) + (((size_t)eo * 2 * DIM + f) * (NV - 1)) * NFP;
+                const VT *fn = static_cast<const VT *>(A.fvf) + (((size_t)en * 2 * DIM + (f ^ 1)) * (NV - 1)) * NFP;
 #pragma unroll
                 for (int v = 1; v < NV; v++) { pfo[v] = fo[(v - 1) * NFP + l]; pfn[v] = fn[(v - 1) * NFP + l]; }
             }
@@ -448,7 +450,8 @@ k_rhs(const KArgs A)
 #pragma unroll
             for (int v = 0; v < NV; v++) {
                 u0v[v] = (A.mode == 2) ? A.u0[v * NK + (size_t)e * K + n] : 0.0;
-                vis[v] = (A.parabolic && v > 0) ? A.visc[(v - 1) * NK + (size_t)e * K + n] : 0.0;
+                vis[v] = (A.parabolic && v > 0)
+                             ? (double)static_cast<const VT *>(A.visc)[(v - 1) * NK + (size_t)e * K + n] : 0.0;
             }
         }
         for (int l0 = 0; l0 < NFP; l0 += LC) {
@@ -577,10 +580,17 @@ k_rhs(const KArgs A)
 // ---------------------------------------------------------------------------
 // k_visc: entropy variables, BR1 lifting, gradient conversion, viscous fluxes
 // ---------------------------------------------------------------------------
-template <int DIM, int NP>
+// MIXED: the mixed-precision form.  Entropy variables stay FP64; each
+// element's are shifted by their value at node 0 (in FP64) and the BR1
+// derivatives, gradient conversion, viscous flux and D^ sums then run in
+// FP32 on those offsets (D annihilates constants, so the shift leaves the
+// derivatives unchanged while keeping their significant digits).  The
+// outputs visc and fvf are FP32 in that form.
+template <int DIM, int NP, bool MIXED>
 __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) k_visc(const VArgs A)
 {
     using C = Cfg<DIM, NP>;
+    using R = typename std::conditional<MIXED, float, double>::type;
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, EPB = C::EPB, BLOCK = C::BLOCK;
     // phase 1 (lifting) needs the entropy variables and neighbour traces,
     // phase 2 (viscous divergence) the contravariant fluxes: one LDS region
@@ -588,18 +598,19 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
     constexpr int P2 = EPB * DIM * (NV - 1) * K;
     constexpr int SCR = P1 > P2 ? P1 : P2;
 
-    __shared__ double sD[NP * NP];
-    __shared__ double sDh[NP * NP];
+    __shared__ R sD[NP * NP];
+    __shared__ R sDh[NP * NP];
     __shared__ double sU[EPB][NV][K];
-    __shared__ double sScr[SCR];
-    double (*sW)[NV][K] = reinterpret_cast<double (*)[NV][K]>(sScr);
-    double (*sWN)[2 * DIM][NV][NFP] = reinterpret_cast<double (*)[2 * DIM][NV][NFP]>(sScr + EPB * NV * K);
-    double (*sFV)[DIM][NV - 1][K] = reinterpret_cast<double (*)[DIM][NV - 1][K]>(sScr);
+    __shared__ double sWref[EPB][NV];
+    __shared__ R sScr[SCR];
+    R (*sW)[NV][K] = reinterpret_cast<R (*)[NV][K]>(sScr);
+    R (*sWN)[2 * DIM][NV][NFP] = reinterpret_cast<R (*)[2 * DIM][NV][NFP]>(sScr + EPB * NV * K);
+    R (*sFV)[DIM][NV - 1][K] = reinterpret_cast<R (*)[DIM][NV - 1][K]>(sScr);
 
     const int t = threadIdx.x;
     for (int i = t; i < NP * NP; i += BLOCK) {
-        sD[i] = A.tab.D[(i / NP) * MAXNP + i % NP];
-        sDh[i] = A.tab.Dh[(i / NP) * MAXNP + i % NP];
+        sD[i] = (R)A.tab.D[(i / NP) * MAXNP + i % NP];
+        sDh[i] = (R)A.tab.Dh[(i / NP) * MAXNP + i % NP];
     }
     const Phys &P = A.phys;
     const int NE = A.geo.NE;
@@ -610,19 +621,26 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
     const int e = ebase + el;
     const bool act = (el < EPB) && (e < NE);
     const int ii[3] = {n % NP, (n / NP) % NP, DIM == 3 ? n / (NP * NP) : 0};
+    R *visc = reinterpret_cast<R *>(A.visc);
+    R *fvf = reinterpret_cast<R *>(A.fvf);
+    const R gm1 = (R)P.gm1, gm1i = (R)P.gm1i, mu = (R)P.mu, lam_h = (R)P.lam_h;
 
-    double ys[NV];
+    double ys[NV], wv[NV];
     if (act) {
-        double w[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             ys[v] = A.y[v * NK + (size_t)e * K + n];
             sU[el][v][n] = ys[v];
         }
-        cons2entropy<DIM>(ys, P, w);
+        cons2entropy<DIM>(ys, P, wv);
+        if (MIXED && n == 0)
 #pragma unroll
-        for (int v = 0; v < NV; v++) sW[el][v][n] = w[v];
+            for (int v = 0; v < NV; v++) sWref[el][v] = wv[v];
     }
+    if (MIXED) __syncthreads();
+    if (act)
+#pragma unroll
+        for (int v = 0; v < NV; v++) sW[el][v][n] = MIXED ? (R)(wv[v] - sWref[el][v]) : (R)wv[v];
     // neighbour traces of the entropy variables
     for (int j = t; j < EPB * 2 * DIM * NFP; j += BLOCK) {
         const int ej = j / (2 * DIM * NFP);
@@ -633,17 +651,22 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
         const int r = f >> 1, side = f & 1;
         const int en = neighbor(A.geo, eo, r, side);
         const int nn = line_node<DIM, NP>(r, q, side ? 0 : NP - 1);
-        double s[NV], w[NV];
+        double sv[NV], w[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) s[v] = A.y[v * NK + (size_t)en * K + nn];
-        cons2entropy<DIM>(s, P, w);
+        for (int v = 0; v < NV; v++) sv[v] = A.y[v * NK + (size_t)en * K + nn];
+        cons2entropy<DIM>(sv, P, w);
 #pragma unroll
-        for (int v = 0; v < NV; v++) sWN[ej][f][v][q] = w[v];
+        for (int v = 0; v < NV; v++) sWN[ej][f][v][q] = MIXED ? (R)(w[v] - sWref[ej][v]) : (R)w[v];
     }
     __syncthreads();
 
-    double gp[DIM][NV];
+    R gp[DIM][NV];
+    R yr[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) yr[v] = act ? (R)ys[v] : R(1);
     if (act) {
+        const R nrm[3] = {(R)A.geo.nrm[0], (R)A.geo.nrm[1], (R)A.geo.nrm[2]};
+        const R invJ = (R)A.geo.invJ, aface = (R)A.geo.aface, half = 0.5;
         // volume part: grad_r w = sum_l D_{i_r l} w_l, times adj / J
         // (LiftingBR1.cpp:176-215)
 #pragma unroll
@@ -652,57 +675,57 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
             const int l = line_of<DIM, NP>(c, ii[0], ii[1], ii[2]);
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                double s = 0.0;
+                R s = 0;
 #pragma unroll
                 for (int m = 0; m < NP; m++)
                     s = (m == 0) ? sW[el][v][line_node<DIM, NP>(c, l, m)] * sD[a * NP + m]
                                  : s + sW[el][v][line_node<DIM, NP>(c, l, m)] * sD[a * NP + m];
-                gp[c][v] = (s * A.geo.nrm[c]) * A.geo.invJ;
+                gp[c][v] = (s * nrm[c]) * invJ;
             }
             // surface part: +1/2 (w2 - w1) n / (w0 J) on both sides (LiftingBR1.cpp:37-65)
             if (a == 0 || a == NP - 1) {
                 const int side = (a == NP - 1);
 #pragma unroll
                 for (int v = 0; v < NV; v++) {
-                    const double wo = sW[el][v][n];
-                    const double wn = sWN[el][2 * c + side][v][l];
-                    const double f = side ? 0.5 * (wn - wo) : 0.5 * (wo - wn);
-                    gp[c][v] += A.geo.aface * (f * A.geo.nrm[c]);
+                    const R wo = sW[el][v][n];
+                    const R wn = sWN[el][2 * c + side][v][l];
+                    const R f = side ? half * (wn - wo) : half * (wo - wn);
+                    gp[c][v] += aface * (f * nrm[c]);
                 }
             }
         }
 #pragma unroll
-        for (int c = 0; c < DIM; c++) entgrad2primgrad<DIM>(ys, P, A.quirks != 0, gp[c]);
+        for (int c = 0; c < DIM; c++) entgrad2primgrad<DIM, R>(yr, gm1, gm1i, A.quirks != 0, gp[c]);
         if (A.grad)
 #pragma unroll
             for (int c = 0; c < DIM; c++)
 #pragma unroll
-                for (int v = 0; v < NV; v++) A.grad[(c * NV + v) * NK + (size_t)e * K + n] = gp[c][v];
+                for (int v = 0; v < NV; v++) A.grad[(c * NV + v) * NK + (size_t)e * K + n] = (double)gp[c][v];
     }
     __syncthreads();   // sW/sWN are dead from here; sFV reuses their storage
 
     if (act) {
-        double fl[DIM + 1][DIM];
+        R fl[DIM + 1][DIM];
         // volume viscous flux uses the stale node K-1 state (DGSEMIntegrator.cpp:620)
-        double sv[NV];
+        R sv[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) sv[v] = A.quirks ? sU[el][v][K - 1] : ys[v];
-        viscous_flux<DIM>(sv, gp, P, fl);
+        for (int v = 0; v < NV; v++) sv[v] = A.quirks ? (R)sU[el][v][K - 1] : yr[v];
+        viscous_flux<DIM, R>(sv, gp, gm1, gm1i, mu, lam_h, fl);
 #pragma unroll
         for (int r = 0; r < DIM; r++)
 #pragma unroll
-            for (int v = 1; v < NV; v++) sFV[el][r][v - 1][n] = A.geo.nrm[r] * fl[v - 1][r];
+            for (int v = 1; v < NV; v++) sFV[el][r][v - 1][n] = (R)A.geo.nrm[r] * fl[v - 1][r];
         bool onface = false;
 #pragma unroll
         for (int c = 0; c < DIM; c++) onface |= (ii[c] == 0 || ii[c] == NP - 1);
         if (onface) {
-            if (A.quirks) viscous_flux<DIM>(ys, gp, P, fl);
+            if (A.quirks) viscous_flux<DIM, R>(yr, gp, gm1, gm1i, mu, lam_h, fl);
 #pragma unroll
             for (int c = 0; c < DIM; c++) {
                 if (ii[c] == 0 || ii[c] == NP - 1) {
                     const int side = (ii[c] == NP - 1);
                     const int l = line_of<DIM, NP>(c, ii[0], ii[1], ii[2]);
-                    double *dst = A.fvf + (((size_t)e * 2 * DIM + 2 * c + side) * (NV - 1)) * NFP;
+                    R *dst = fvf + (((size_t)e * 2 * DIM + 2 * c + side) * (NV - 1)) * NFP;
 #pragma unroll
                     for (int v = 1; v < NV; v++) dst[(v - 1) * NFP + l] = fl[v - 1][c];
                 }
@@ -712,9 +735,9 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
     __syncthreads();
     if (act) {
         // sum_r sum_l Dhat_{i_r l} (adj Fv)^r_l  (DGSEMIntegrator.cpp:705-738)
-        double acc[NV - 1];
+        R acc[NV - 1];
 #pragma unroll
-        for (int v = 0; v < NV - 1; v++) acc[v] = 0.0;
+        for (int v = 0; v < NV - 1; v++) acc[v] = 0;
 #pragma unroll
         for (int r = 0; r < DIM; r++) {
             const int a = ii[r];
@@ -722,13 +745,13 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
 #pragma unroll
             for (int m = 0; m < NP; m++) {
                 const int nm = line_node<DIM, NP>(r, l, m);
-                const double dm = sDh[a * NP + m];
+                const R dm = sDh[a * NP + m];
 #pragma unroll
                 for (int v = 0; v < NV - 1; v++) acc[v] += sFV[el][r][v][nm] * dm;
             }
         }
 #pragma unroll
-        for (int v = 0; v < NV - 1; v++) A.visc[v * NK + (size_t)e * K + n] = acc[v];
+        for (int v = 0; v < NV - 1; v++) visc[v * NK + (size_t)e * K + n] = acc[v];
     }
 }
 
@@ -913,7 +936,9 @@ template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, doub
     a.y = y; a.visc = c->d_visc; a.fvf = c->d_fvf; a.grad = grad;
     const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
     ProfScope ps(c, s, 1);
-    hipLaunchKernelGGL((k_visc<DIM, NP>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    if (c->prm.precision == PRANDTL_MIXED)
+        hipLaunchKernelGGL((k_visc<DIM, NP, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_visc<DIM, NP, false>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
     return hip_check(c, hipGetLastError(), "k_visc launch");
 }
 
@@ -924,8 +949,10 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
     const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
     ProfScope ps(c, s, 0);
     const bool qns3 = DIM == 3 && c->prm.parabolic && c->prm.quirks;
-    if (qns3) hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, DIM == 3>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
-    else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    if (qns3) hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, DIM == 3, double>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    else if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
+        hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, float>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
     return hip_check(c, hipGetLastError(), "k_rhs launch");
 }
 
@@ -1005,8 +1032,10 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     if (dim < 2 || dim > 3) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "dim must be 2 or 3");
     if (N < 2 || N > (dim == 2 ? 5 : 7))
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "order not built for this dim");
-    if (prm->precision != PRANDTL_FP64)
-        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "only the FP64 form is built");
+    if (prm->precision != PRANDTL_FP64 && prm->precision != PRANDTL_MIXED)
+        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "precision must be PRANDTL_FP64 or PRANDTL_MIXED");
+    if (prm->precision == PRANDTL_MIXED && prm->parabolic && prm->quirks)
+        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "the mixed form is built for corrected mode (quirks = 0)");
     if (!(prm->gamma > 1.0)) return fail(nullptr, PRANDTL_ERR_ARG, "gamma must exceed 1");
     if (prm->parabolic && !(prm->Pr > 0.0)) return fail(nullptr, PRANDTL_ERR_ARG, "Pr must be positive");
     long long NE = 1;

@@ -155,74 +155,75 @@ __device__ __forceinline__ void cons2entropy(const double *s, const Phys &P, dou
     w[DIM + 1] = -beta;
 }
 
-// EntropyGrad2PrimGrad, BasicOperations.cpp:126-170, one direction in place.
-// quirks: the density term multiplies velocities (the aliased row, lines
-// 137-138 and 164) instead of momenta.
-template <int DIM>
-__device__ __forceinline__ void entgrad2primgrad(const double *s, const Phys &P, bool quirks,
-                                                 double *g)
+// EntropyGrad2PrimGrad, BasicOperations.cpp:126-170, one direction in place,
+// in arithmetic R (double, or float in the mixed form).  quirks: the density
+// term multiplies velocities (the aliased row, lines 137-138 and 164)
+// instead of momenta.
+template <int DIM, typename R>
+__device__ __forceinline__ void entgrad2primgrad(const R *s, R gm1, R gm1i, bool quirks, R *g)
 {
-    const double r = 1.0 / s[0];
-    double vel[DIM], vv = 0.0;
+    const R one = 1, half = 0.5;
+    const R r = one / s[0];
+    R vel[DIM], vv = 0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         vel[d] = s[1 + d] * r;
         vv = (d == 0) ? vel[d] * vel[d] : vv + vel[d] * vel[d];
     }
-    const double v_sq = 0.5 * vv;
-    const double p = P.gm1 * (s[DIM + 1] - s[0] * v_sq);
-    const double KE = v_sq * s[0];
-    const double g0 = g[0], gL = g[DIM + 1];
-    double mdot = 0.0;
+    const R v_sq = half * vv;
+    const R p = gm1 * (s[DIM + 1] - s[0] * v_sq);
+    const R KE = v_sq * s[0];
+    const R g0 = g[0], gL = g[DIM + 1];
+    R mdot = 0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         g[1 + d] = p / s[0] * (g[1 + d] + vel[d] * gL);
-        const double md = quirks ? vel[d] : s[1 + d];
+        const R md = quirks ? vel[d] : s[1 + d];
         mdot = (d == 0) ? md * g[1 + d] : mdot + md * g[1 + d];
     }
-    g[0] = s[0] * g0 - gL * (KE - p * P.gm1i) + s[0] / p * mdot;
+    g[0] = s[0] * g0 - gL * (KE - p * gm1i) + s[0] / p * mdot;
     g[DIM + 1] = p / s[0] * (g[0] + p * gL);
 }
 
 // NavierStokesFlux::ComputeViscousFlux (NavierStokesFlux.cpp:12-90), constant
-// mu, mu_bulk = 2/3.  dq[c][v] = d(prim_v)/dx_c; out fl[v][c], v = 1..nv-1
-// (the mass row is zero and not stored: fl[v-1][c]).
-template <int DIM>
-__device__ __forceinline__ void viscous_flux(const double *s, const double (*dq)[DIM + 2],
-                                             const Phys &P, double (*fl)[DIM])
+// mu, mu_bulk = 2/3, in arithmetic R.  dq[c][v] = d(prim_v)/dx_c; out
+// fl[v][c], v = 1..nv-1 (the mass row is zero and not stored: fl[v-1][c]).
+template <int DIM, typename R>
+__device__ __forceinline__ void viscous_flux(const R *s, const R (*dq)[DIM + 2], R gm1, R gm1i,
+                                             R mu, R lam_h, R (*fl)[DIM])
 {
     constexpr int NV = DIM + 2;
-    const double mu_bulk = 2.0 / 3.0;
-    const double rho = s[0];
-    double vel[DIM];
+    const R mu_bulk = R(2) / R(3), one = 1, two = 2, half = 0.5;
+    const R rho = s[0];
+    R vel[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; d++) vel[d] = s[1 + d] / rho;
     // ComputePressure: V /= rho multiplies by 1/rho (BasicOperations.cpp:33-36)
-    const double rr = 1.0 / rho;
-    double vv = 0.0;
+    const R rr = one / rho;
+    R vv = 0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        const double v = s[1 + d] * rr;
+        const R v = s[1 + d] * rr;
         vv = (d == 0) ? v * v : vv + v * v;
     }
-    const double p = P.gm1 * (s[NV - 1] - 0.5 * rho * vv);
-    double div = dq[0][1];
+    const R p = gm1 * (s[NV - 1] - half * rho * vv);
+    R div = dq[0][1];
 #pragma unroll
     for (int d = 1; d < DIM; d++) div += dq[d][1 + d];
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
-        const double cvdT = P.gm1i / rho * (dq[c][NV - 1] - p / rho * dq[c][0]);
+        const R cvdT = gm1i / rho * (dq[c][NV - 1] - p / rho * dq[c][0]);
 #pragma unroll
         for (int a = 0; a < DIM; a++) {
-            double t;
-            if (a == c) t = P.mu * (2.0 * dq[a][1 + a] - mu_bulk * div);
-            else t = P.mu * (dq[c][1 + a] + dq[a][1 + c]);
+            R t;
+            if (a == c) t = mu * (two * dq[a][1 + a] - mu_bulk * div);
+            else t = mu * (dq[c][1 + a] + dq[a][1 + c]);
             fl[a][c] = t;
         }
-        double e = vel[0] * fl[0][c];
+        R e = vel[0] * fl[0][c];
 #pragma unroll
         for (int a = 1; a < DIM; a++) e = e + vel[a] * fl[a][c];
-        fl[NV - 2][c] = e + P.lam_h * cvdT;
+        fl[NV - 2][c] = e + lam_h * cvdT;
     }
 }
 

@@ -1,0 +1,93 @@
+// C++ caller of the drop-in surface (include/prandtl_operator.hpp) on the GPU,
+// checked against the CPU oracle (oracle/liboracle.so) on the same input:
+// 2-D isentropic vortex, 16x16 quads, N = 3, RHS and three RK3 steps.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "prandtl_oracle.h"
+#include "prandtl_operator.hpp"
+
+static double rel_max(const std::vector<double> &a, const std::vector<double> &b)
+{
+    double num = 0.0, den = 0.0;
+    for (size_t i = 0; i < a.size(); i++) {
+        num = std::fmax(num, std::fabs(a[i] - b[i]));
+        den = std::fmax(den, std::fabs(b[i]));
+    }
+    return num / den;
+}
+
+int main()
+{
+    const int N = 3, Np = N + 1, nx = 16, ny = 16;
+    prandtl_mesh_desc mesh = {2, N, {nx, ny, 1}, {-5.0, -5.0, 0.0}, {5.0, 5.0, 1.0}};
+    prandtl_params prm = {1.4, 0.72, 0.0, 0.5, 0.001, 0, 0, 0, 1, PRANDTL_FP64};
+    hipStream_t stream;
+    if (hipStreamCreate(&stream) != hipSuccess) { std::printf("FAIL stream\n"); return 1; }
+    prandtl_amd::DGSEMOperator op(mesh, prm, stream);
+    prandtl_amd::RK3SSPSolver ode;
+    ode.Init(op);
+
+    // isentropic vortex (beta = 5, freestream rho = u = v = p = 1)
+    std::vector<double> x(Np), w(Np);
+    prandtl_lgl(N, x.data(), w.data());
+    const int NE = op.NumElements(), K = op.NodesPerElement();
+    const size_t NK = (size_t)NE * K;
+    std::vector<double> u(4 * NK);
+    const double h = 10.0 / nx, g = 1.4, beta = 5.0, pi = std::acos(-1.0);
+    for (int e = 0; e < NE; e++)
+        for (int i = 0; i < K; i++) {
+            const double X = -5.0 + (e % nx + x[i % Np]) * h, Y = -5.0 + (e / nx + x[i / Np]) * h;
+            const double r2 = X * X + Y * Y;
+            const double f = beta / (2 * pi) * std::exp(0.5 * (1 - r2));
+            const double T = 1 - (g - 1) * beta * beta / (8 * g * pi * pi) * std::exp(1 - r2);
+            const double rho = std::pow(T, 1 / (g - 1)), uu = 1 - f * Y, vv = 1 + f * X, p = rho * T;
+            u[0 * NK + e * K + i] = rho;
+            u[1 * NK + e * K + i] = rho * uu;
+            u[2 * NK + e * K + i] = rho * vv;
+            u[3 * NK + e * K + i] = p / (g - 1) + 0.5 * rho * (uu * uu + vv * vv);
+        }
+    double *du = nullptr, *dr = nullptr;
+    if (hipMalloc(&du, u.size() * 8) != hipSuccess || hipMalloc(&dr, u.size() * 8) != hipSuccess) {
+        std::printf("FAIL alloc\n");
+        return 1;
+    }
+    (void)hipMemcpy(du, u.data(), u.size() * 8, hipMemcpyHostToDevice);
+    op.Mult(du, dr);
+    std::vector<double> rhs(u.size()), step(u.size());
+    (void)hipStreamSynchronize(stream);
+    (void)hipMemcpy(rhs.data(), dr, rhs.size() * 8, hipMemcpyDeviceToHost);
+
+    orc_params op_ = {};
+    op_.dim = 2; op_.order = N; op_.nelem[0] = nx; op_.nelem[1] = ny; op_.nelem[2] = 1;
+    op_.lo[0] = op_.lo[1] = -5.0; op_.hi[0] = op_.hi[1] = 5.0; op_.hi[2] = 1.0;
+    op_.gamma = 1.4; op_.Pr = 0.72; op_.alpha_max = 0.5; op_.alpha_min = 0.001; op_.quirks = 1;
+    op_.threads = 4;
+    orc_ctx *orc = nullptr;
+    if (orc_create(&op_, &orc) != 0) { std::printf("FAIL oracle\n"); return 1; }
+    std::vector<double> ref(u.size());
+    orc_rhs(orc, u.data(), ref.data());
+    const double r1 = rel_max(rhs, ref);
+
+    double t = 0.0, dt = 0.01;
+    std::vector<double> uref = u;
+    for (int s = 0; s < 3; s++) {
+        ode.Step(du, t, dt);
+        orc_rk3_step(orc, uref.data(), dt);
+    }
+    (void)hipStreamSynchronize(stream);
+    (void)hipMemcpy(step.data(), du, step.size() * 8, hipMemcpyDeviceToHost);
+    const double r2 = rel_max(step, uref);
+    const double lam = op.GetMaxCharSpeed(), lam_ref = orc_max_wave_speed(orc);
+    const bool ok = r1 <= 1e-12 && r2 <= 1e-12 && std::fabs(lam - lam_ref) <= 1e-14 * lam_ref &&
+                    std::fabs(t - 0.03) < 1e-15 && op.StateIsPhysical(du);
+    std::printf("%s rhs_rel=%.3e step_rel=%.3e lambda=%.17g/%.17g\n", ok ? "OK" : "FAIL", r1, r2, lam, lam_ref);
+    orc_destroy(orc);
+    (void)hipFree(du);
+    (void)hipFree(dr);
+    (void)hipStreamDestroy(stream);
+    return ok ? 0 : 1;
+}

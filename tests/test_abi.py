@@ -48,6 +48,7 @@ def _create(mesh_kw, prm_kw):
     p.gamma, p.Pr, p.mu = prm_kw.get("gamma", 1.4), 0.72, 0.0
     p.alpha_max, p.alpha_min = 0.5, 0.001
     p.parabolic = prm_kw.get("parabolic", 0)
+    p.quirks = prm_kw.get("quirks", 1)
     p.precision = prm_kw.get("precision", 0)
     h = ctypes.c_void_p()
     rc = lib().prandtl_create(ctypes.byref(m), ctypes.byref(p), ctypes.byref(h))
@@ -57,6 +58,7 @@ def _create(mesh_kw, prm_kw):
 @pytest.mark.parametrize("mesh_kw,prm_kw,code", [
     (dict(dim=4), {}, _lib.PRANDTL_ERR_UNSUPPORTED),
     (dict(dim=2, order=6), {}, _lib.PRANDTL_ERR_UNSUPPORTED),
+    ({}, dict(parabolic=1, precision=1), _lib.PRANDTL_ERR_UNSUPPORTED),
     (dict(dim=3, order=1), {}, _lib.PRANDTL_ERR_UNSUPPORTED),
     (dict(nelem=(1, 4, 4)), {}, _lib.PRANDTL_ERR_ARG),
     (dict(hi=-1.0), {}, _lib.PRANDTL_ERR_ARG),
