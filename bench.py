@@ -35,6 +35,9 @@ WORKLOADS = {
                     blending=True, alpha_max=0.5, gamma=1.4),
     "iv2d_n3": dict(dim=2, order=3, nelem=(16, 16), lo=(-5.0, -5.0), hi=(5.0, 5.0), gamma=1.4),
 }
+# 64 independent 2-D Euler cases (isentropic vortices and blended
+# Kelvin-Helmholtz layers, meshes 8^2..64^2, N = 2..5) advanced together
+ENSEMBLE = "ensemble64"
 
 
 def initial_state(name, cfg):
@@ -48,11 +51,14 @@ def initial_state(name, cfg):
 
 
 def algorithmic_counts(name, quirks):
-    """Per-node algorithmic FP64 op counts of one RHS (oracle op-counting
-    instantiation on the initial state, committed as a fixture)."""
+    """Algorithmic FP64 op counts of one RHS (oracle op-counting instantiation
+    on the initial state, committed as a fixture): per node for a single
+    case, totals over all cases for the ensemble (quirks None)."""
     path = os.path.join(ROOT, "tests", "golden", "flop_counts.json")
     with open(path) as f:
         table = json.load(f)
+    if quirks is None:
+        return table[name]
     return table["%s_%s" % (name, "faithful" if quirks else "corrected")]
 
 
@@ -76,15 +82,148 @@ def cpu_baseline(name, cfg, quirks, u, budget_s):
                       % (n, name, nodes, el)}
 
 
+def cpu_baseline_ensemble(case_list, budget_s):
+    """Oracle RHS over the ensemble's cases in order, bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from prandtl_amd import cases
+    threads = min(os.cpu_count() or 1, 16)
+    nodes, n, el = 0, 0, 0.0
+    for c in case_list:
+        kw = cases.operator_kwargs(c)
+        kw.pop("precision")
+        orc = O.Oracle(threads=threads, **kw)
+        u = cases.ensemble_state(c)
+        t0 = time.perf_counter()
+        orc.Mult(u)
+        el += time.perf_counter() - t0
+        nodes += orc.NE * orc.K
+        n += 1
+        if el > budget_s:
+            break
+    return {"value": nodes / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": "one RHS evaluation of each of the first %d ensemble cases (%d nodes) in %.1f s, "
+                      "OpenMP over elements/faces" % (n, nodes, el)}
+
+
+def run_ensemble(args):
+    """Aggregate throughput of the ensemble launch vs sequential per-case launches."""
+    import torch
+    from prandtl_amd import cases
+    from prandtl_amd.operator import Ensemble
+
+    precision = args.precision or "mixed"
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+    case_list = cases.ensemble_cases(64, 20261202, precision=precision)
+    ens = Ensemble(case_list)
+    us, dts = [], []
+    for c, op in zip(case_list, ens.ops):
+        u = torch.tensor(cases.ensemble_state(c).reshape(-1), dtype=torch.float64, device=dev)
+        z = torch.empty_like(u)
+        op.Mult(u, z)
+        # fixed dt from CFL 0.5 of the initial wave speed
+        dts.append(0.5 * op.hmin / (op.GetMaxCharSpeed(reset=True) * (c["order"] + 1) ** 2))
+        us.append(u)
+    nodes = sum(op.NE * op.K for op in ens.ops)
+
+    def ens_steps(k):
+        for _ in range(k):
+            ens.Step(us, dts)
+
+    def seq_steps(k):
+        for _ in range(k):
+            for op, u, dt in zip(ens.ops, us, dts):
+                op.Step(u, dt)
+
+    ens_steps(args.warmup)
+    seq_steps(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens_steps(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    seq_steps(args.steps)
+    torch.cuda.synchronize()
+    elapsed_seq = time.perf_counter() - t0
+
+    # k_ens duration: each replay of the step graph is three k_ens launches
+    # back to back on this stream, bracketed by events
+    psteps = min(args.steps, 50)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(psteps)]
+    for a, b in evs:
+        a.record(stream)
+        ens.Step(us, dts)
+        b.record(stream)
+    torch.cuda.synchronize()
+    t_launch = sum(a.elapsed_time(b) for a, b in evs) / psteps / 3 * 1e-3
+    ok = all(op.check_state(u) for op, u in zip(ens.ops, us))
+
+    counts = algorithmic_counts(ENSEMBLE, None)
+    nv = 4
+    flops_launch = (counts["volume"] + counts["surface"] + counts["fv"] + counts["indicator"]
+                    + counts["assembly"] + 4 * nv * nodes)
+    bytes_launch = 8 * nv * (1 + 1 + 2.0 / 3.0) * nodes
+    roof = roofline(flops_launch, bytes_launch, t_launch, "k_ens (8 variants N=2..5 x blending)")
+    value = nodes * 3 * args.steps / elapsed
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == "fp64" else "f64 (mixed form; the Euler path has no FP32 stage)",
+        "data": "synthetic (analytic initial conditions, seeded case draws)",
+        "config": {"workload": ENSEMBLE + ": 64 2-D Euler cases, 32 isentropic vortices + 32 blended "
+                               "Kelvin-Helmholtz, meshes {8..64}^2, N 2..5, seed 20261202",
+                   "cases": len(case_list), "nodes": nodes, "vars": nv, "rhs_per_step": 3,
+                   "precision": precision, "state_ok": bool(ok), "hip_graph": True,
+                   "sequential_per_case": {"value": nodes * 3 * args.steps / elapsed_seq,
+                                           "ms_per_step": elapsed_seq / args.steps * 1e3,
+                                           "launches_per_step": 3 * len(case_list)},
+                   "speedup_vs_sequential": elapsed_seq / elapsed},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_ensemble(case_list, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    ens.close()
+
+
+def roofline(flops_launch, bytes_launch, t_launch, kernel):
+    """Roofline entry of one kernel from its algorithmic counts per launch."""
+    ai = flops_launch / bytes_launch
+    if ai >= PEAK_FP64 / PEAK_HBM:
+        roof = {"bound": "mfma", "achieved": flops_launch / t_launch / 1e12, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "pipe": "fp64 (vector and matrix peaks are equal on MI355X)"}
+    else:
+        roof = {"bound": "hbm", "achieved": bytes_launch / t_launch / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = kernel
+    roof["avg_launch_us"] = t_launch * 1e6
+    roof["algorithmic_flops_per_launch"] = flops_launch
+    roof["algorithmic_bytes_per_launch"] = bytes_launch
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="tgv3d_n3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="tgv3d_n3", choices=sorted(WORKLOADS) + [ENSEMBLE])
     ap.add_argument("--faithful", action="store_true",
                     help="reproduce the reference's defects (its NS-3D path is unstable on this case)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
+                    help="default: fp64 for single cases, mixed for the ensemble")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -105,10 +244,14 @@ def main():
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    if args.workload == ENSEMBLE:
+        run_ensemble(args)
+        return
+
     import torch
     from prandtl_amd import DGSEMOperator
 
+    args.precision = args.precision or "fp64"
     name = args.workload
     cfg = dict(WORKLOADS[name])
     quirks = bool(args.faithful)
@@ -166,21 +309,7 @@ def main():
     if cfg.get("parabolic"):
         Np = cfg["order"] + 1
         by_rhs += 8 * (nv - 1) * (1 + 2 * cfg["dim"] * Np ** (cfg["dim"] - 1) / Np ** cfg["dim"])
-    flops_launch = fl_rhs * nodes
-    bytes_launch = by_rhs * nodes
-    ai = flops_launch / bytes_launch
-    ridge = PEAK_FP64 / PEAK_HBM
-    if ai >= ridge:
-        roof = {"bound": "mfma", "achieved": flops_launch / t_rhs / 1e12, "peak": PEAK_FP64 / 1e12,
-                "unit": "TFLOP/s", "pipe": "fp64 (vector and matrix peaks are equal on MI355X)"}
-    else:
-        roof = {"bound": "hbm", "achieved": bytes_launch / t_rhs / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["kernel"] = "k_rhs<%d,%d>" % (cfg["dim"], cfg["order"] + 1)
-    roof["avg_launch_us"] = t_rhs * 1e6
-    roof["algorithmic_flops_per_launch"] = flops_launch
-    roof["algorithmic_bytes_per_launch"] = bytes_launch
+    roof = roofline(fl_rhs * nodes, by_rhs * nodes, t_rhs, "k_rhs<%d,%d>" % (cfg["dim"], cfg["order"] + 1))
 
     line = {
         "metric": METRIC,

@@ -120,6 +120,27 @@ int prandtl_profile_end(prandtl_ctx *ctx, double *ms, int *count);
 /* Last error message of this context (or of create when ctx is NULL). */
 const char *prandtl_last_error(const prandtl_ctx *ctx);
 
+/* ---- Batched ensemble (no counterpart in the reference, which runs one
+ * case per process: Simulation::Run, Simulation.cpp:882-1181) ----
+ * Independent 2-D Euler cases of orders 2..5, with or without blending, each
+ * with its own mesh and parameters.  One RK3 step advances every case: per
+ * stage a single launch covers the element tiles of all cases, one tile per
+ * workgroup, in a host-built order of decreasing cost that the workgroup
+ * dispatcher consumes as a work queue; on a non-null stream the three stages
+ * are captured once as a hipGraph.  Results are identical to stepping each
+ * case alone with prandtl_rk3_step.  At most 256 cases.  The case contexts
+ * belong to the ensemble; prandtl_ensemble_destroy frees them. */
+typedef struct prandtl_ensemble prandtl_ensemble;
+int prandtl_ensemble_create(int ncases, const prandtl_mesh_desc *meshes,
+                            const prandtl_params *params, prandtl_ensemble **out);
+void prandtl_ensemble_destroy(prandtl_ensemble *ens);
+/* the per-case context (sizes, wave speed, alpha) */
+int prandtl_ensemble_case(prandtl_ensemble *ens, int i, prandtl_ctx **ctx);
+/* u: ncases device state pointers, dt: ncases step sizes (host array) */
+int prandtl_ensemble_rk3_step(prandtl_ensemble *ens, double *const *u_dev, const double *dt,
+                              void *stream);
+const char *prandtl_ensemble_last_error(const prandtl_ensemble *ens);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,8 @@ EXPORTED = (
     "prandtl_set_graph",
     "prandtl_max_wave_speed", "prandtl_alpha", "prandtl_check_state", "prandtl_lgl",
     "prandtl_debug_gradients", "prandtl_profile_begin", "prandtl_profile_end", "prandtl_last_error",
+    "prandtl_ensemble_create", "prandtl_ensemble_destroy", "prandtl_ensemble_case",
+    "prandtl_ensemble_rk3_step", "prandtl_ensemble_last_error",
 )
 
 
@@ -76,6 +78,14 @@ def lib():
     L.prandtl_profile_end.argtypes = [vp, dp, ip]
     L.prandtl_last_error.argtypes = [vp]
     L.prandtl_last_error.restype = ctypes.c_char_p
+    L.prandtl_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(MeshDesc), ctypes.POINTER(Params),
+                                          ctypes.POINTER(vp)]
+    L.prandtl_ensemble_destroy.argtypes = [vp]
+    L.prandtl_ensemble_destroy.restype = None
+    L.prandtl_ensemble_case.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.prandtl_ensemble_rk3_step.argtypes = [vp, ctypes.POINTER(vp), dp, vp]
+    L.prandtl_ensemble_last_error.argtypes = [vp]
+    L.prandtl_ensemble_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -83,5 +93,12 @@ def lib():
 def check(rc, ctx=None):
     if rc != PRANDTL_OK:
         msg = lib().prandtl_last_error(ctx)
+        raise PrandtlError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def check_ensemble(rc, ens=None):
+    if rc != PRANDTL_OK:
+        msg = lib().prandtl_ensemble_last_error(ens)
         raise PrandtlError(rc, msg.decode() if msg else "")
     return rc

@@ -55,6 +55,45 @@ class MT19937_64:
         """(x >> 11) * 2^-53 in [0, 1)."""
         return (self() >> 11) * (1.0 / 9007199254740992.0)
 
+    def uniform_array(self, n):
+        """The next n values of uniform(), drawn 312 at a time with numpy.
+
+        The twist splits into three runs whose inputs are all known when the
+        run starts: i < 156 reads old words only, 156 <= i < 311 reads
+        mt[i-156] (updated by the first run), and i = 311 reads the new mt[0].
+        """
+        out = np.empty(n, dtype=np.uint64)
+        mt = np.array(self.mt, dtype=np.uint64)
+        upper, lower = np.uint64(0xFFFFFFFF80000000), np.uint64(0x7FFFFFFF)
+        matrix = np.uint64(0xB5026F5AA96619E9)
+        one = np.uint64(1)
+
+        def twist_run(i0, i1, nxt, far):
+            x = (mt[i0:i1] & upper) | (nxt & lower)
+            xa = (x >> one) ^ np.where((x & one) == one, matrix, np.uint64(0))
+            mt[i0:i1] = far ^ xa
+
+        k = 0
+        idx = self.idx
+        while k < n:
+            if idx >= 312:
+                twist_run(0, 156, mt[1:157], mt[156:312])
+                twist_run(156, 311, mt[157:312], mt[0:155])
+                twist_run(311, 312, mt[0:1], mt[155:156])
+                idx = 0
+            take = min(312 - idx, n - k)
+            y = mt[idx:idx + take].copy()
+            y ^= (y >> np.uint64(29)) & np.uint64(0x5555555555555555)
+            y ^= (y << np.uint64(17)) & np.uint64(0x71D67FFFEDA60000)
+            y ^= (y << np.uint64(37)) & np.uint64(0xFFF7EEE000000000)
+            y ^= y >> np.uint64(43)
+            out[k:k + take] = y
+            k += take
+            idx += take
+        self.mt = [int(v) for v in mt]
+        self.idx = idx
+        return (out >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
 
 def node_coords(dim, order, nelem, lo, hi):
     """Physical node coordinates X[dim][NE][K]."""
@@ -126,8 +165,7 @@ def kelvin_helmholtz(X, gamma=1.4, s=15.0, seed=20261202, noise=1e-3):
     B = np.tanh(s * y + 0.5 * s) - np.tanh(s * y - 0.5 * s)
     rho = 0.5 + 0.75 * B
     u = 0.5 * (B - 1.0)
-    g = MT19937_64(seed)
-    r = np.array([g.uniform() for _ in range(x.size)]).reshape(x.shape)
+    r = MT19937_64(seed).uniform_array(x.size).reshape(x.shape)
     v = 0.1 * np.sin(2.0 * np.pi * x) + noise * (2.0 * r - 1.0)
     p = np.ones_like(x)
     return prim_to_cons(rho, [u, v], p, gamma)
@@ -182,3 +220,46 @@ def quadrature_weights(dim, order, nelem, lo, hi):
         wt = wt * w[i // (Np * Np)]
     NE = int(np.prod([nelem[r] for r in range(dim)]))
     return np.broadcast_to(wt * J, (NE, Np ** dim)).copy()
+
+
+# Ensemble of independent 2-D Euler cases: square meshes n x n with n from
+# ENSEMBLE_MESHES and degree N from ENSEMBLE_ORDERS; the first half are
+# isentropic vortices on [-5,5]^2 with beta ~ U[3,6], the second half
+# Kelvin-Helmholtz shear layers on [-1,1]^2 with thickness s ~ U[10,20] and
+# blending (alpha_max 0.5).
+ENSEMBLE_MESHES = (8, 16, 24, 32, 48, 64)
+ENSEMBLE_ORDERS = (2, 3, 4, 5)
+
+
+def ensemble_cases(ncases=64, seed=20261202, precision="mixed"):
+    """Case list drawn from one std::mt19937_64: per case, in this order, a
+    mesh index, an order index (index = floor(r * n)) and the case parameter
+    (lo + r (hi - lo)).  Returns dicts with the DGSEMOperator keyword
+    arguments plus 'kind', 'param' and 'seed' (the KH noise seed)."""
+    g = MT19937_64(seed)
+    out = []
+    for i in range(ncases):
+        n = ENSEMBLE_MESHES[int(g.uniform() * len(ENSEMBLE_MESHES))]
+        N = ENSEMBLE_ORDERS[int(g.uniform() * len(ENSEMBLE_ORDERS))]
+        r = g.uniform()
+        c = dict(dim=2, order=N, nelem=(n, n), gamma=1.4, precision=precision)
+        if i < ncases // 2:
+            c.update(lo=(-5.0, -5.0), hi=(5.0, 5.0), kind="iv", param=3.0 + 3.0 * r, seed=0)
+        else:
+            c.update(lo=(-1.0, -1.0), hi=(1.0, 1.0), kind="kh", param=10.0 + 10.0 * r,
+                     seed=seed + 1 + i, blending=True, alpha_max=0.5)
+        out.append(c)
+    return out
+
+
+def ensemble_state(case):
+    """Initial state of one ensemble case, [nv][NE][K]."""
+    X = node_coords(2, case["order"], case["nelem"], case["lo"], case["hi"])
+    if case["kind"] == "iv":
+        return isentropic_vortex(X, case["gamma"], beta=case["param"])
+    return kelvin_helmholtz(X, case["gamma"], s=case["param"], seed=case["seed"])
+
+
+def operator_kwargs(case):
+    """The DGSEMOperator keyword arguments of an ensemble case."""
+    return {k: v for k, v in case.items() if k not in ("kind", "param", "seed")}

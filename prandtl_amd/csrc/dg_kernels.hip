@@ -29,6 +29,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -78,7 +79,7 @@ struct KArgs {
     const void *fvf;      // [NE][2 dim][nv-1][Nfp]   (FP32 in the mixed form)
     double *alpha;        // [NE]
     double *eta;          // [NE]
-    unsigned long long *lam;
+    double *lam;          // per-block running maxima [gridDim.x]
 };
 
 struct VArgs {
@@ -171,8 +172,12 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
+// Running maximum wave speed: each block keeps its own slot (one thread, a
+// plain read-modify-write, no contention); prandtl_max_wave_speed reduces
+// the slots when asked.  Non-finite speeds are ignored, as std::max(current,
+// NaN) does in the reference (DGSEMIntegrator.cpp:270).
 template <int BLOCK>
-__device__ __forceinline__ void block_lam_max(double lam, unsigned long long *out, double *sred)
+__device__ __forceinline__ void block_lam_max(double lam, double *slots, double *sred, int slot)
 {
     lam = wave_max(lam);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -181,7 +186,8 @@ __device__ __forceinline__ void block_lam_max(double lam, unsigned long long *ou
     if (threadIdx.x == 0) {
         double m = sred[0];
         for (int i = 1; i < (BLOCK + 63) / 64; i++) m = (m < sred[i]) ? sred[i] : m;
-        if (m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+        const double old = slots[slot];
+        if (old < m) slots[slot] = m;
     }
 }
 
@@ -193,7 +199,7 @@ __device__ __forceinline__ void block_lam_max(double lam, unsigned long long *ou
 // subcell interfaces.  EPB elements per workgroup are chosen so that each
 // phase gives every thread about one item (3-D N=3: 2 elements, 192
 // threads: 96 surface points and 96 pairs per direction per element).
-template <int DIM, int NP, bool BLEND> struct RCfg {
+template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
     static constexpr int NV = DIM + 2;
     static constexpr int K = DIM == 2 ? NP * NP : NP * NP * NP;
     static constexpr int NFP = DIM == 2 ? NP : NP * NP;
@@ -205,7 +211,7 @@ template <int DIM, int NP, bool BLEND> struct RCfg {
     static constexpr int WORK = mx(K, mx(JP, JF));
     static constexpr int EPB = K > 128 ? 1 : mx(1, 256 / WORK);
     static constexpr int r64(int x) { return ((x + 63) / 64) * 64; }
-    static constexpr int BLOCK = K > 256 ? r64(K) : (r64(EPB * WORK) > 512 ? 512 : r64(EPB * WORK));
+    static constexpr int BLOCK = FB ? FB : K > 256 ? r64(K) : (r64(EPB * WORK) > 512 ? 512 : r64(EPB * WORK));
     static constexpr int lc_pick()
     {
         for (int lc = NFP; lc >= 1; lc--)
@@ -219,30 +225,50 @@ template <int DIM, int NP, bool BLEND> struct RCfg {
     static_assert(!BLEND || 2 * K <= NV * FXN, "indicator scratch fits the flux buffer");
 };
 
+// LDS of one k_rhs tile (EPB elements).
+template <int DIM, int NP, bool BLEND, bool QNS3, int FB> struct RhsSmem {
+    using C = RCfg<DIM, NP, BLEND, FB>;
+    static constexpr int NQ = DIM + 1 + (BLEND ? 2 : 0);   // v[DIM], beta, (1/beta, lambda)
+    static constexpr int GK = QNS3 ? C::K : 1, GE = QNS3 ? C::EPB : 1;
+    double Dh2[NP * NP];
+    double V1i[BLEND ? NP * NP : 1];
+    double Fvs[NP];
+    double Sub[DIM][NP];
+    double U[C::EPB][C::NV][C::K];
+    double Q[C::EPB][NQ][C::K];
+    double Fc[C::EPB][C::NV][C::JF];    // signed surface contributions
+    double F[C::EPB][C::NV][C::FXN];    // pair and subcell fluxes of one line chunk
+    double G[GE][C::NV][GK];
+    double Al[C::EPB];
+    double Red[(C::BLOCK + 63) / 64];
+};
+
+// One tile of k_rhs: elements [tile*EPB, tile*EPB + EPB) of the case A
+// describes; slot indexes the tile's running wave-speed maximum.
 // BLEND: modal indicator + subcell FV blending compiled in.
 // QNS3: the reference's NS-3D zeta-partner defect (faithful NS in 3-D).
 // VT: element type of the viscous inputs from k_visc (float in the mixed form).
-template <int DIM, int NP, bool BLEND, bool QNS3, typename VT>
-__global__ void __launch_bounds__((RCfg<DIM, NP, BLEND>::BLOCK), (RCfg<DIM, NP, BLEND>::WAVES))
-k_rhs(const KArgs A)
+// FB: forced workgroup size (0: the natural one of RCfg).
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB>
+__device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const int slot,
+                                         RhsSmem<DIM, NP, BLEND, QNS3, FB> &S)
 {
-    using C = RCfg<DIM, NP, BLEND>;
+    using C = RCfg<DIM, NP, BLEND, FB>;
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, NPAIR = C::NPAIR, NFV = C::NFV;
     constexpr int EPB = C::EPB, LC = C::LC, BLOCK = C::BLOCK, FXN = C::FXN, JF = C::JF;
-    constexpr int GK = QNS3 ? K : 1, GE = QNS3 ? EPB : 1;
-    constexpr int NQ = DIM + 1 + (BLEND ? 2 : 0);   // v[DIM], beta, (1/beta, lambda)
-
-    __shared__ double sDh2[NP * NP];
-    __shared__ double sV1i[BLEND ? NP * NP : 1];
-    __shared__ double sFvs[NP];
-    __shared__ double sSub[DIM][NP];
-    __shared__ double sU[EPB][NV][K];
-    __shared__ double sQ[EPB][NQ][K];
-    __shared__ double sFc[EPB][NV][JF];    // signed surface contributions
-    __shared__ double sF[EPB][NV][FXN];    // pair and subcell fluxes of one line chunk
-    __shared__ double sG[GE][NV][GK];
-    __shared__ double sAl[EPB];
-    __shared__ double sRed[(BLOCK + 63) / 64];
+    constexpr int NQ = RhsSmem<DIM, NP, BLEND, QNS3, FB>::NQ;
+    (void)FXN;
+    auto &sDh2 = S.Dh2;
+    auto &sV1i = S.V1i;
+    auto &sFvs = S.Fvs;
+    auto &sSub = S.Sub;
+    auto &sU = S.U;
+    auto &sQ = S.Q;
+    auto &sFc = S.Fc;
+    auto &sF = S.F;
+    auto &sG = S.G;
+    auto &sAl = S.Al;
+    auto &sRed = S.Red;
 
     const int t = threadIdx.x;
     for (int i = t; i < NP * NP; i += BLOCK) {
@@ -255,7 +281,7 @@ k_rhs(const KArgs A)
     const Phys &P = A.phys;
     const int NE = A.geo.NE;
     const size_t NK = (size_t)NE * K;
-    const int ebase = xcd_remap(blockIdx.x, gridDim.x) * EPB;
+    const int ebase = tile * EPB;
     const int el = t / K;
     const int n = t - el * K;
     const int e = ebase + el;
@@ -479,7 +505,7 @@ k_rhs(const KArgs A)
                     if (QNS3 && r == 1 && b == NP - 1)
 #pragma unroll
                         for (int v = 0; v < NV; v++) sG[ej][v][na] = F[v];
-                } else {
+                } else if constexpr (BLEND) {
                     // ---- subcell FV interface i+1/2 on line l ----
                     const int jv = j - npj;
                     const int ej = jv / (LC * NFV);
@@ -574,7 +600,74 @@ k_rhs(const KArgs A)
             A.out[gi] = o;
         }
     }
-    block_lam_max<BLOCK>(lam, A.lam, sRed);
+    block_lam_max<BLOCK>(lam, A.lam, sRed, slot);
+}
+
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT>
+__global__ void __launch_bounds__((RCfg<DIM, NP, BLEND>::BLOCK), (RCfg<DIM, NP, BLEND>::WAVES))
+k_rhs(const KArgs A)
+{
+    __shared__ RhsSmem<DIM, NP, BLEND, QNS3, 0> S;
+    rhs_body<DIM, NP, BLEND, QNS3, VT, 0>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
+}
+
+// ---------------------------------------------------------------------------
+// k_ens: one launch per RK stage over the element tiles of an ensemble of
+// independent 2-D Euler cases with mixed polynomial degree and blending.
+//
+// Workgroup b takes tile b of a host-built list sorted by decreasing cost; the
+// workgroup dispatcher hands workgroups to CUs in index order as slots free,
+// so the list is the work queue and needs no device counter.  A persistent
+// form (a loop over tiles inside the kernel, from an atomic counter or a
+// grid stride) keeps the tile body inside a loop, and register allocation
+// then spills about 1.3 KB per lane at 168 VGPRs; this form stays at the
+// largest variant's own 107 VGPRs with no scratch.
+// ---------------------------------------------------------------------------
+struct EnsTile {
+    int c;   // case
+    int g;   // element group (tile) of that case
+    int v;   // variant: 2*(NP-3) + blend
+};
+
+constexpr int ENS_BLOCK = 256;
+
+template <int NP, bool BLEND> constexpr size_t ens_smem() { return sizeof(RhsSmem<2, NP, BLEND, false, ENS_BLOCK>); }
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+constexpr size_t ENS_SMEM = cmax(cmax(cmax(ens_smem<3, false>(), ens_smem<3, true>()), cmax(ens_smem<4, false>(), ens_smem<4, true>())),
+                                 cmax(cmax(ens_smem<5, false>(), ens_smem<5, true>()), cmax(ens_smem<6, false>(), ens_smem<6, true>())));
+
+template <int NP, bool BLEND>
+__device__ __forceinline__ void ens_tile(const KArgs &A, int g, double *smem)
+{
+    rhs_body<2, NP, BLEND, false, double, ENS_BLOCK>(A, g, g, *reinterpret_cast<RhsSmem<2, NP, BLEND, false, ENS_BLOCK> *>(smem));
+}
+
+// grid = number of tiles; every field of a tile is uniform over the workgroup
+__global__ void __launch_bounds__(ENS_BLOCK, 3)
+k_ens(const KArgs *__restrict__ cases, const EnsTile *__restrict__ tiles)
+{
+    __shared__ __align__(16) double smem[(ENS_SMEM + 7) / 8];
+    const EnsTile T = tiles[blockIdx.x];
+    const KArgs &A = cases[T.c];
+    switch (T.v) {
+    case 0: ens_tile<3, false>(A, T.g, smem); break;
+    case 1: ens_tile<3, true>(A, T.g, smem); break;
+    case 2: ens_tile<4, false>(A, T.g, smem); break;
+    case 3: ens_tile<4, true>(A, T.g, smem); break;
+    case 4: ens_tile<5, false>(A, T.g, smem); break;
+    case 5: ens_tile<5, true>(A, T.g, smem); break;
+    case 6: ens_tile<6, false>(A, T.g, smem); break;
+    case 7: ens_tile<6, true>(A, T.g, smem); break;
+    default: break;
+    }
+}
+
+constexpr int ENS_MAX_CASES = 256;
+struct DtArgs { double dt[ENS_MAX_CASES]; };
+__global__ void k_set_dts(double *dst, int n, const DtArgs a)
+{
+    const int i = threadIdx.x;
+    if (i < n) dst[i] = a.dt[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -758,7 +851,27 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
 // ---------------------------------------------------------------------------
 // small utility kernels
 // ---------------------------------------------------------------------------
-__global__ void k_reset_u64(unsigned long long *p) { *p = 0ull; }
+__global__ void k_fill(double *p, int n, double v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// max over the per-block slots into out[0] (one block)
+__global__ void k_max_reduce(const double *slots, int n, double *out)
+{
+    __shared__ double red[4];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) m = (m < slots[i]) ? slots[i] : m;
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int i = 1; i < 4; i++) r = (r < red[i]) ? red[i] : r;
+        out[0] = r;
+    }
+}
 __global__ void k_set_double(double *p, double v) { *p = v; }
 __global__ void k_reset_int(int *p) { *p = 0; }
 
@@ -867,7 +980,8 @@ struct prandtl_ctx {
     Phys phys;
     double *d_visc = nullptr, *d_fvf = nullptr, *d_alpha = nullptr, *d_eta = nullptr;
     double *d_ya = nullptr, *d_yb = nullptr;
-    unsigned long long *d_lam = nullptr;
+    double *d_lam = nullptr;       // per-block running maxima (NE slots)
+    double *d_lam_out = nullptr;
     int *d_flag = nullptr;
     std::string err;
     // kernel timing (prandtl_profile_begin/end): events bracketing every launch
@@ -956,19 +1070,26 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
     return hip_check(c, hipGetLastError(), "k_rhs launch");
 }
 
-template <int DIM, int NP>
-int launch_rhs(prandtl_ctx *c, const double *y, const double *u0, double *out, int mode, double dt,
-               double c0, double c1, hipStream_t s)
+KArgs make_kargs(const prandtl_ctx *c, const double *y, const double *u0, double *out, int mode,
+                 double dt, double c0, double c1, const double *dtp)
 {
     KArgs a;
     a.tab = c->tab; a.geo = c->geo; a.phys = c->phys;
     a.blend = c->prm.blending; a.parabolic = c->prm.parabolic;
     a.ec = c->prm.ec_surface; a.quirks = c->prm.quirks;
     a.mode = mode; a.dt = dt; a.c0 = c0; a.c1 = c1;
-    a.dtp = c->dtp;
+    a.dtp = dtp;
     a.y = y; a.u0 = u0; a.out = out;
     a.visc = c->d_visc; a.fvf = c->d_fvf; a.alpha = c->d_alpha; a.eta = c->d_eta;
     a.lam = c->d_lam;
+    return a;
+}
+
+template <int DIM, int NP>
+int launch_rhs(prandtl_ctx *c, const double *y, const double *u0, double *out, int mode, double dt,
+               double c0, double c1, hipStream_t s)
+{
+    const KArgs a = make_kargs(c, y, u0, out, mode, dt, c0, c1, c->dtp);
     if (c->prm.blending) return launch_rhs_t<DIM, NP, true>(c, a, s);
     return launch_rhs_t<DIM, NP, false>(c, a, s);
 }
@@ -1150,12 +1271,13 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     if (e == hipSuccess) e = hipMalloc(&c->d_eta, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMalloc(&c->d_ya, sizeof(double) * c->nv * NK);
     if (e == hipSuccess) e = hipMalloc(&c->d_yb, sizeof(double) * c->nv * NK);
-    if (e == hipSuccess) e = hipMalloc(&c->d_lam, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&c->d_lam, sizeof(double) * c->NE);
+    if (e == hipSuccess) e = hipMalloc(&c->d_lam_out, sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_flag, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&c->d_dt, sizeof(double));
     if (e == hipSuccess) e = hipMemset(c->d_alpha, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_eta, 0, sizeof(double) * c->NE);
-    if (e == hipSuccess) e = hipMemset(c->d_lam, 0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->d_lam, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         g_create_err = std::string("device allocation: ") + hipGetErrorString(e);
@@ -1173,6 +1295,7 @@ void prandtl_destroy(prandtl_ctx *c)
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (c->d_lam) (void)hipFree(c->d_lam);
+    if (c->d_lam_out) (void)hipFree(c->d_lam_out);
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_dt) (void)hipFree(c->d_dt);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -1245,15 +1368,15 @@ int prandtl_max_wave_speed(prandtl_ctx *c, double *out, int reset, void *stream)
 {
     if (!c || !out) return fail(c, PRANDTL_ERR_ARG, "null argument");
     hipStream_t s = (hipStream_t)stream;
-    unsigned long long bits = 0;
-    int rc = hip_check(c, hipMemcpyAsync(&bits, c->d_lam, sizeof(bits), hipMemcpyDeviceToHost, s), "copy lambda");
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, s, c->d_lam, c->NE, c->d_lam_out);
+    double v = 0.0;
+    int rc = hip_check(c, hipGetLastError(), "reduce lambda");
+    if (!rc) rc = hip_check(c, hipMemcpyAsync(&v, c->d_lam_out, sizeof(v), hipMemcpyDeviceToHost, s), "copy lambda");
     if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
     if (rc) return rc;
-    double v;
-    std::memcpy(&v, &bits, sizeof(v));
     *out = v;
     if (reset) {
-        hipLaunchKernelGGL(k_reset_u64, dim3(1), dim3(1), 0, s, c->d_lam);
+        hipLaunchKernelGGL(k_fill, dim3((c->NE + 255) / 256), dim3(256), 0, s, c->d_lam, c->NE, 0.0);
         rc = hip_check(c, hipGetLastError(), "reset lambda");
     }
     return rc;
@@ -1324,5 +1447,206 @@ int prandtl_profile_end(prandtl_ctx *c, double *ms, int *count)
 }
 
 const char *prandtl_last_error(const prandtl_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+} // extern "C"
+
+struct prandtl_ensemble {
+    int n = 0;
+    std::vector<prandtl_ctx *> ctx;
+    std::vector<double *> u;               // caller state pointers the arguments were built for
+    std::vector<double> dt;                // step sizes last copied to d_dts
+    KArgs *d_args[3] = {nullptr, nullptr, nullptr};
+    EnsTile *d_tiles = nullptr;
+    int ntiles = 0;
+    double *d_dts = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int ens_epb(int NP, int blend)
+{
+    switch (NP * 2 + blend) {
+    case 6: return RCfg<2, 3, false, ENS_BLOCK>::EPB;
+    case 7: return RCfg<2, 3, true, ENS_BLOCK>::EPB;
+    case 8: return RCfg<2, 4, false, ENS_BLOCK>::EPB;
+    case 9: return RCfg<2, 4, true, ENS_BLOCK>::EPB;
+    case 10: return RCfg<2, 5, false, ENS_BLOCK>::EPB;
+    case 11: return RCfg<2, 5, true, ENS_BLOCK>::EPB;
+    case 12: return RCfg<2, 6, false, ENS_BLOCK>::EPB;
+    case 13: return RCfg<2, 6, true, ENS_BLOCK>::EPB;
+    default: return 0;
+    }
+}
+
+// relative cost of one tile of variant v = 2*(NP-3) + blend: nodes per tile
+// times the per-node pair work (NP-1 partners per direction) plus the
+// indicator and subcell work of a blended tile
+double ens_cost(int v)
+{
+    const int NP = 3 + v / 2, blend = v & 1;
+    const int epb = ens_epb(NP, blend);
+    return (double)epb * NP * NP * ((NP - 1) + 4.0 + (blend ? 3.0 : 0.0));
+}
+
+int ens_fail(prandtl_ensemble *E, int code, const std::string &msg)
+{
+    if (E) E->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+int ens_hip(prandtl_ensemble *E, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return PRANDTL_OK;
+    return ens_fail(E, PRANDTL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// per-stage argument arrays for the current state pointers
+int ens_bind(prandtl_ensemble *E, double *const *u, hipStream_t s)
+{
+    std::vector<KArgs> a[3];
+    for (int i = 0; i < E->n; i++) {
+        prandtl_ctx *c = E->ctx[i];
+        const double *dtp = E->d_dts + i;
+        a[0].push_back(make_kargs(c, u[i], nullptr, c->d_ya, 1, 0.0, 0.0, 0.0, dtp));
+        a[1].push_back(make_kargs(c, c->d_ya, u[i], c->d_yb, 2, 0.0, 3.0 / 4, 1.0 / 4, dtp));
+        a[2].push_back(make_kargs(c, c->d_yb, u[i], u[i], 2, 0.0, 1.0 / 3, 2.0 / 3, dtp));
+    }
+    for (int k = 0; k < 3; k++) {
+        int rc = ens_hip(E, hipMemcpyAsync(E->d_args[k], a[k].data(), sizeof(KArgs) * E->n, hipMemcpyHostToDevice, s), "copy args");
+        if (rc) return rc;
+    }
+    int rc = ens_hip(E, hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    E->u.assign(u, u + E->n);
+    return PRANDTL_OK;
+}
+
+int ens_stages(prandtl_ensemble *E, hipStream_t s)
+{
+    for (int k = 0; k < 3; k++)
+        hipLaunchKernelGGL(k_ens, dim3(E->ntiles), dim3(ENS_BLOCK), 0, s, E->d_args[k], E->d_tiles);
+    return ens_hip(E, hipGetLastError(), "k_ens launch");
+}
+
+} // namespace
+
+extern "C" {
+
+int prandtl_ensemble_create(int ncases, const prandtl_mesh_desc *meshes, const prandtl_params *params,
+                            prandtl_ensemble **out)
+{
+    if (!meshes || !params || !out || ncases < 1) return ens_fail(nullptr, PRANDTL_ERR_ARG, "bad arguments");
+    if (ncases > ENS_MAX_CASES) return ens_fail(nullptr, PRANDTL_ERR_ARG, "too many cases");
+    *out = nullptr;
+    for (int i = 0; i < ncases; i++) {
+        if (meshes[i].dim != 2 || meshes[i].order < 2 || meshes[i].order > 5)
+            return ens_fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "ensemble cases are 2-D with order 2..5");
+        if (params[i].parabolic || params[i].ec_surface)
+            return ens_fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "ensemble cases are Euler with the reference surface flux");
+    }
+    prandtl_ensemble *E = new (std::nothrow) prandtl_ensemble();
+    if (!E) return ens_fail(nullptr, PRANDTL_ERR_NOMEM, "host allocation");
+    E->n = ncases;
+    std::vector<EnsTile> tiles;
+    for (int i = 0; i < ncases; i++) {
+        prandtl_ctx *c = nullptr;
+        const int rc = prandtl_create(&meshes[i], &params[i], &c);
+        if (rc) {
+            prandtl_ensemble_destroy(E);
+            return ens_fail(nullptr, rc, "case " + std::to_string(i) + ": " + g_create_err);
+        }
+        E->ctx.push_back(c);
+        const int blend = params[i].blending ? 1 : 0;
+        const int epb = ens_epb(c->Np, blend);
+        const int groups = (c->NE + epb - 1) / epb;
+        for (int g = 0; g < groups; g++) tiles.push_back({i, g, 2 * (c->Np - 3) + blend});
+    }
+    // most expensive tiles first, so the tail of the dispatch order is made of
+    // cheap tiles (longest-processing-time-first list scheduling)
+    std::stable_sort(tiles.begin(), tiles.end(),
+                     [](const EnsTile &a, const EnsTile &b) { return ens_cost(a.v) > ens_cost(b.v); });
+    E->ntiles = (int)tiles.size();
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 3 && e == hipSuccess; k++) e = hipMalloc(&E->d_args[k], sizeof(KArgs) * ncases);
+    if (e == hipSuccess) e = hipMalloc(&E->d_tiles, sizeof(EnsTile) * E->ntiles);
+    if (e == hipSuccess) e = hipMalloc(&E->d_dts, sizeof(double) * ncases);
+    if (e == hipSuccess) e = hipMemcpy(E->d_tiles, tiles.data(), sizeof(EnsTile) * E->ntiles, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        g_create_err = std::string("ensemble device allocation: ") + hipGetErrorString(e);
+        prandtl_ensemble_destroy(E);
+        return PRANDTL_ERR_NOMEM;
+    }
+    *out = E;
+    return PRANDTL_OK;
+}
+
+void prandtl_ensemble_destroy(prandtl_ensemble *E)
+{
+    if (!E) return;
+    if (E->exec) (void)hipGraphExecDestroy(E->exec);
+    if (E->graph) (void)hipGraphDestroy(E->graph);
+    for (int k = 0; k < 3; k++)
+        if (E->d_args[k]) (void)hipFree(E->d_args[k]);
+    if (E->d_tiles) (void)hipFree(E->d_tiles);
+    if (E->d_dts) (void)hipFree(E->d_dts);
+    for (prandtl_ctx *c : E->ctx) prandtl_destroy(c);
+    delete E;
+}
+
+int prandtl_ensemble_case(prandtl_ensemble *E, int i, prandtl_ctx **ctx)
+{
+    if (!E || !ctx || i < 0 || i >= E->n) return ens_fail(E, PRANDTL_ERR_ARG, "bad arguments");
+    *ctx = E->ctx[i];
+    return PRANDTL_OK;
+}
+
+int prandtl_ensemble_rk3_step(prandtl_ensemble *E, double *const *u, const double *dt, void *stream)
+{
+    if (!E || !u || !dt) return ens_fail(E, PRANDTL_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    bool rebind = E->u.size() != (size_t)E->n;
+    for (int i = 0; !rebind && i < E->n; i++) rebind = E->u[i] != u[i];
+    if (rebind) {
+        const int rc = ens_bind(E, u, s);
+        if (rc) return rc;
+        if (E->exec) { (void)hipGraphExecDestroy(E->exec); E->exec = nullptr; }
+        if (E->graph) { (void)hipGraphDestroy(E->graph); E->graph = nullptr; }
+    }
+    // step sizes live in device memory, read by the stage kernels: copy them
+    // (by a kernel, so the graph is unaffected) only when they change
+    int rc = PRANDTL_OK;
+    if (rebind || E->dt.size() != (size_t)E->n || !std::equal(E->dt.begin(), E->dt.end(), dt)) {
+        DtArgs da;
+        for (int i = 0; i < E->n; i++) da.dt[i] = dt[i];
+        hipLaunchKernelGGL(k_set_dts, dim3(1), dim3(ENS_MAX_CASES), 0, s, E->d_dts, E->n, da);
+        rc = ens_hip(E, hipGetLastError(), "set dt");
+        if (rc) return rc;
+        E->dt.assign(dt, dt + E->n);
+    }
+    if (s == nullptr) return ens_stages(E, s);
+    if (!E->exec || E->graph_stream != s) {
+        if (E->exec) { (void)hipGraphExecDestroy(E->exec); E->exec = nullptr; }
+        if (E->graph) { (void)hipGraphDestroy(E->graph); E->graph = nullptr; }
+        rc = ens_hip(E, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        if (rc) return rc;
+        const int rs = ens_stages(E, s);
+        hipGraph_t g = nullptr;
+        rc = ens_hip(E, hipStreamEndCapture(s, &g), "end capture");
+        if (rs) { if (g) (void)hipGraphDestroy(g); return rs; }
+        if (rc) return rc;
+        E->graph = g;
+        rc = ens_hip(E, hipGraphInstantiate(&E->exec, g, nullptr, nullptr, 0), "graph instantiate");
+        if (rc) return rc;
+        E->graph_stream = s;
+    }
+    return ens_hip(E, hipGraphLaunch(E->exec, s), "graph launch");
+}
+
+const char *prandtl_ensemble_last_error(const prandtl_ensemble *E) { return E ? E->err.c_str() : g_create_err.c_str(); }
 
 } // extern "C"

@@ -29,6 +29,24 @@ def lgl_nodes(order):
     return x, w
 
 
+def _descs(dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0, parabolic=False, blending=False,
+           alpha_max=0.5, alpha_min=0.001, ec_surface=False, quirks=True, precision="fp64"):
+    """prandtl_mesh_desc and prandtl_params for one case."""
+    m = _lib.MeshDesc()
+    m.dim, m.order = dim, order
+    for r in range(3):
+        m.nelem[r] = int(nelem[r]) if r < dim else 1
+        m.lo[r] = float(lo[r]) if r < dim else 0.0
+        m.hi[r] = float(hi[r]) if r < dim else 1.0
+    p = _lib.Params()
+    p.gamma, p.Pr, p.mu = float(gamma), float(Pr), float(mu)
+    p.alpha_max, p.alpha_min = float(alpha_max), float(alpha_min)
+    p.parabolic, p.blending = int(parabolic), int(blending)
+    p.ec_surface, p.quirks = int(ec_surface), int(quirks)
+    p.precision = {"fp64": _lib.PRANDTL_FP64, "mixed": _lib.PRANDTL_MIXED}[precision]
+    return m, p
+
+
 def _stream_ptr():
     import torch
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -40,22 +58,26 @@ class DGSEMOperator:
     def __init__(self, dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0,
                  parabolic=False, blending=False, alpha_max=0.5, alpha_min=0.001,
                  ec_surface=False, quirks=True, precision="fp64"):
-        m = _lib.MeshDesc()
-        m.dim, m.order = dim, order
-        for r in range(3):
-            m.nelem[r] = int(nelem[r]) if r < dim else 1
-            m.lo[r] = float(lo[r]) if r < dim else 0.0
-            m.hi[r] = float(hi[r]) if r < dim else 1.0
-        p = _lib.Params()
-        p.gamma, p.Pr, p.mu = float(gamma), float(Pr), float(mu)
-        p.alpha_max, p.alpha_min = float(alpha_max), float(alpha_min)
-        p.parabolic, p.blending = int(parabolic), int(blending)
-        p.ec_surface, p.quirks = int(ec_surface), int(quirks)
-        p.precision = {"fp64": _lib.PRANDTL_FP64, "mixed": _lib.PRANDTL_MIXED}[precision]
+        m, p = _descs(dim, order, nelem, lo, hi, gamma, Pr, mu, parabolic, blending, alpha_max,
+                      alpha_min, ec_surface, quirks, precision)
         h = ctypes.c_void_p()
         rc = lib().prandtl_create(ctypes.byref(m), ctypes.byref(p), ctypes.byref(h))
         check(rc, None)
         self._h = h
+        self._owner = True
+        self._describe(dim, order, nelem, lo, hi)
+
+    @classmethod
+    def _view(cls, handle, dim, order, nelem, lo, hi):
+        """A non-owning operator over a context that another object frees."""
+        op = cls.__new__(cls)
+        op._h = handle
+        op._owner = False
+        op._describe(dim, order, nelem, lo, hi)
+        return op
+
+    def _describe(self, dim, order, nelem, lo, hi):
+        h = self._h
         NE, K, nv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().prandtl_sizes(h, ctypes.byref(NE), ctypes.byref(K), ctypes.byref(nv)), h)
         self.NE, self.K, self.nv = NE.value, K.value, nv.value
@@ -69,7 +91,8 @@ class DGSEMOperator:
     # -- lifetime ----------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
-            lib().prandtl_destroy(self._h)
+            if self._owner:
+                lib().prandtl_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -138,6 +161,54 @@ class DGSEMOperator:
         check(lib().prandtl_debug_gradients(self._h, self._vec(u, "u"), ctypes.c_void_p(g.data_ptr()),
                                             _stream_ptr()), self._h)
         return g
+
+
+class Ensemble:
+    """Independent 2-D Euler cases advanced together (prandtl_ensemble_*).
+
+    `cases` is a sequence of dicts with the DGSEMOperator keyword arguments
+    (dim, order, nelem, lo, hi, gamma, blending, alpha_max, ...).  `Step(us,
+    dts)` advances case i's state us[i] by dts[i], one SSP-RK3 step each, in
+    one launch per RK stage; `ops[i]` is a non-owning DGSEMOperator over case
+    i's context (sizes, wave speed, alpha).
+    """
+
+    def __init__(self, cases):
+        n = len(cases)
+        meshes = (_lib.MeshDesc * n)()
+        params = (_lib.Params * n)()
+        for i, c in enumerate(cases):
+            meshes[i], params[i] = _descs(**c)
+        h = ctypes.c_void_p()
+        rc = lib().prandtl_ensemble_create(n, meshes, params, ctypes.byref(h))
+        _lib.check_ensemble(rc, None)
+        self._h = h
+        self.n = n
+        self.ops = []
+        for i, c in enumerate(cases):
+            ch = ctypes.c_void_p()
+            _lib.check_ensemble(lib().prandtl_ensemble_case(h, i, ctypes.byref(ch)), h)
+            self.ops.append(DGSEMOperator._view(ch, c["dim"], c["order"], c["nelem"], c["lo"], c["hi"]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for op in self.ops:
+                op.close()
+            lib().prandtl_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def Step(self, us, dts):
+        if len(us) != self.n or len(dts) != self.n:
+            raise ValueError("need %d states and %d step sizes" % (self.n, self.n))
+        ptrs = (ctypes.c_void_p * self.n)(*[op._vec(u, "u").value for op, u in zip(self.ops, us)])
+        dt = (ctypes.c_double * self.n)(*[float(v) for v in dts])
+        _lib.check_ensemble(lib().prandtl_ensemble_rk3_step(self._h, ptrs, dt, _stream_ptr()), self._h)
 
 
 class RK3SSPSolver:

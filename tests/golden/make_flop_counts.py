@@ -6,6 +6,9 @@ The N=7 Taylor-Green count is taken on an 8^3-element mesh of the same box
 (the 32^3 mesh is too large for the counting scalar); per-node counts differ
 only through log-mean branch choices.
 
+"ensemble64" holds, for the 64-case ensemble of 2-D Euler cases (seed
+20261202), the per-RHS totals summed over all cases and the node count.
+
 Usage: python tests/golden/make_flop_counts.py
 """
 import json
@@ -18,6 +21,7 @@ sys.path[:0] = [ROOT, os.path.dirname(HERE)]
 
 import oracle_lib as O  # noqa: E402
 import bench  # noqa: E402
+from prandtl_amd import cases  # noqa: E402
 
 
 def main():
@@ -33,6 +37,15 @@ def main():
             nodes = orc.NE * orc.K
             out["%s_%s" % (name, "faithful" if quirks else "corrected")] = {
                 k: v / nodes for k, v in c.items()}
+    tot, nodes = {}, 0
+    for c in cases.ensemble_cases(64, 20261202):
+        kw = cases.operator_kwargs(c)
+        kw.pop("precision")
+        orc = O.Oracle(**kw)
+        for k, v in orc.count_flops(cases.ensemble_state(c)).items():
+            tot[k] = tot.get(k, 0.0) + v
+        nodes += orc.NE * orc.K
+    out["ensemble64"] = dict(tot, nodes=nodes)
     with open(os.path.join(HERE, "flop_counts.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print(json.dumps(out, indent=1, sort_keys=True))

@@ -79,3 +79,47 @@ def test_null_arguments():
     assert L.prandtl_sizes(None, None, None, None) == _lib.PRANDTL_ERR_ARG
     assert L.prandtl_lgl(0, None, None) == _lib.PRANDTL_ERR_ARG
     L.prandtl_destroy(None)
+
+
+def _ensemble_create(case_kw):
+    n = len(case_kw)
+    meshes = (_lib.MeshDesc * n)()
+    params = (_lib.Params * n)()
+    for i, kw in enumerate(case_kw):
+        m, p = meshes[i], params[i]
+        m.dim, m.order = kw.get("dim", 2), kw.get("order", 3)
+        for r in range(3):
+            m.nelem[r] = 4
+            m.lo[r], m.hi[r] = 0.0, 1.0
+        p.gamma, p.Pr, p.mu = 1.4, 0.72, 0.0
+        p.alpha_max, p.alpha_min = 0.5, 0.001
+        p.parabolic = kw.get("parabolic", 0)
+        p.ec_surface = kw.get("ec_surface", 0)
+        p.quirks = 1
+    h = ctypes.c_void_p()
+    rc = lib().prandtl_ensemble_create(n, meshes, params, ctypes.byref(h))
+    return rc, h
+
+
+@pytest.mark.parametrize("case_kw,code", [
+    ([{}, dict(dim=3)], _lib.PRANDTL_ERR_UNSUPPORTED),
+    ([dict(order=6)], _lib.PRANDTL_ERR_UNSUPPORTED),
+    ([dict(order=1)], _lib.PRANDTL_ERR_UNSUPPORTED),
+    ([{}, dict(parabolic=1)], _lib.PRANDTL_ERR_UNSUPPORTED),
+    ([dict(ec_surface=1)], _lib.PRANDTL_ERR_UNSUPPORTED),
+    ([{}] * 257, _lib.PRANDTL_ERR_ARG),
+])
+def test_ensemble_create_rejects(case_kw, code):
+    rc, h = _ensemble_create(case_kw)
+    assert rc == code
+    assert not h.value
+    assert lib().prandtl_ensemble_last_error(None)
+
+
+def test_ensemble_null_arguments():
+    L = lib()
+    h = ctypes.c_void_p()
+    assert L.prandtl_ensemble_create(0, None, None, ctypes.byref(h)) == _lib.PRANDTL_ERR_ARG
+    assert L.prandtl_ensemble_rk3_step(None, None, None, None) == _lib.PRANDTL_ERR_ARG
+    assert L.prandtl_ensemble_case(None, 0, ctypes.byref(h)) == _lib.PRANDTL_ERR_ARG
+    L.prandtl_ensemble_destroy(None)
