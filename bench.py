@@ -166,7 +166,8 @@ def run_ensemble(args):
     flops_launch = (counts["volume"] + counts["surface"] + counts["fv"] + counts["indicator"]
                     + counts["assembly"] + 4 * nv * nodes)
     bytes_launch = 8 * nv * (1 + 1 + 2.0 / 3.0) * nodes
-    roof = roofline(flops_launch, bytes_launch, t_launch, "k_ens (8 variants N=2..5 x blending)")
+    roof = roofline(flops_launch, bytes_launch, t_launch, "k_ens (8 variants N=2..5 x blending)",
+                    measured_traffic(ENSEMBLE, "k_ens"))
     value = nodes * 3 * args.steps / elapsed
     line = {
         "metric": METRIC,
@@ -197,7 +198,22 @@ def run_ensemble(args):
     ens.close()
 
 
-def roofline(flops_launch, bytes_launch, t_launch, kernel):
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel, from the committed FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/hbm_traffic_<workload>.txt,
+    calibrated against a copy kernel); None when there is none."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == "traffic" and parts[1] == workload and parts[2] == kernel:
+                return float(parts[3])
+    return None
+
+
+def roofline(flops_launch, bytes_launch, t_launch, kernel, traffic=None):
     """Roofline entry of one kernel from its algorithmic counts per launch."""
     ai = flops_launch / bytes_launch
     if ai >= PEAK_FP64 / PEAK_HBM:
@@ -206,7 +222,7 @@ def roofline(flops_launch, bytes_launch, t_launch, kernel):
     else:
         roof = {"bound": "hbm", "achieved": bytes_launch / t_launch / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = traffic
     roof["kernel"] = kernel
     roof["avg_launch_us"] = t_launch * 1e6
     roof["algorithmic_flops_per_launch"] = flops_launch
@@ -309,7 +325,9 @@ def main():
     if cfg.get("parabolic"):
         Np = cfg["order"] + 1
         by_rhs += 8 * (nv - 1) * (1 + 2 * cfg["dim"] * Np ** (cfg["dim"] - 1) / Np ** cfg["dim"])
-    roof = roofline(fl_rhs * nodes, by_rhs * nodes, t_rhs, "k_rhs<%d,%d>" % (cfg["dim"], cfg["order"] + 1))
+    kname = "k_rhs<%d,%d>" % (cfg["dim"], cfg["order"] + 1)
+    traffic = measured_traffic(name, kname) if (args.precision == "fp64" and not quirks) else None
+    roof = roofline(fl_rhs * nodes, by_rhs * nodes, t_rhs, kname, traffic)
 
     line = {
         "metric": METRIC,

@@ -13,6 +13,7 @@ State vectors are torch float64 tensors on the GPU in byNODES layout
 torch stream; there is no CPU path.
 """
 import ctypes
+import inspect
 
 import numpy as np
 
@@ -45,6 +46,9 @@ def _descs(dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0, parabolic=Fals
     p.ec_surface, p.quirks = int(ec_surface), int(quirks)
     p.precision = {"fp64": _lib.PRANDTL_FP64, "mixed": _lib.PRANDTL_MIXED}[precision]
     return m, p
+
+
+_DESC_KEYS = frozenset(inspect.signature(_descs).parameters)
 
 
 def _stream_ptr():
@@ -167,7 +171,8 @@ class Ensemble:
     """Independent 2-D Euler cases advanced together (prandtl_ensemble_*).
 
     `cases` is a sequence of dicts with the DGSEMOperator keyword arguments
-    (dim, order, nelem, lo, hi, gamma, blending, alpha_max, ...).  `Step(us,
+    (dim, order, nelem, lo, hi, gamma, blending, alpha_max, ...); other keys
+    are ignored.  `Step(us,
     dts)` advances case i's state us[i] by dts[i], one SSP-RK3 step each, in
     one launch per RK stage; `ops[i]` is a non-owning DGSEMOperator over case
     i's context (sizes, wave speed, alpha).
@@ -178,7 +183,7 @@ class Ensemble:
         meshes = (_lib.MeshDesc * n)()
         params = (_lib.Params * n)()
         for i, c in enumerate(cases):
-            meshes[i], params[i] = _descs(**c)
+            meshes[i], params[i] = _descs(**{k: v for k, v in c.items() if k in _DESC_KEYS})
         h = ctypes.c_void_p()
         rc = lib().prandtl_ensemble_create(n, meshes, params, ctypes.byref(h))
         _lib.check_ensemble(rc, None)
