@@ -199,6 +199,39 @@ __device__ __forceinline__ void block_lam_max(double lam, double *slots, double 
 // subcell interfaces.  EPB elements per workgroup are chosen so that each
 // phase gives every thread about one item (3-D N=3: 2 elements, 192
 // threads: 96 surface points and 96 pairs per direction per element).
+//
+// FB (forced block size, 0 = natural) gives the other launch shapes the
+// autotuner tries; rshape() computes a shape without instantiating RCfg, so
+// the host can test which shapes exist.
+struct RShape {
+    int K, WORK, EPB, BLOCK;
+};
+constexpr int cmx(int a, int b) { return a > b ? a : b; }
+constexpr int r64(int x) { return ((x + 63) / 64) * 64; }
+constexpr RShape rshape(int DIM, int NP, bool BLEND, int FB)
+{
+    const int K = DIM == 2 ? NP * NP : NP * NP * NP;
+    const int NFP = DIM == 2 ? NP : NP * NP;
+    const int JP = NFP * (NP * (NP - 1) / 2 + (BLEND ? NP - 1 : 0));
+    const int WORK = cmx(K, cmx(JP, 2 * DIM * NFP));
+    const int EPB = K > 128 ? 1 : cmx(1, (FB ? FB : 256) / WORK);
+    const int BLOCK = FB ? FB : K > 256 ? r64(K) : (r64(EPB * WORK) > 512 ? 512 : r64(EPB * WORK));
+    return RShape{K, WORK, EPB, BLOCK};
+}
+// Launch shapes of k_rhs: 0 natural, 1..3 blocks of 64, 128, 256 threads.
+// A forced shape exists when it holds one node per thread and differs from
+// the natural one.
+constexpr int RHS_SHAPES = PRANDTL_RHS_SHAPES;
+constexpr int rhs_shape_fb(int shape) { return shape == 1 ? 64 : shape == 2 ? 128 : shape == 3 ? 256 : 0; }
+constexpr bool rhs_shape_ok(int DIM, int NP, bool BLEND, int shape)
+{
+    if (shape == 0) return true;
+    if (shape < 0 || shape >= RHS_SHAPES) return false;
+    const RShape n = rshape(DIM, NP, BLEND, 0);
+    const RShape f = rshape(DIM, NP, BLEND, rhs_shape_fb(shape));
+    return f.EPB * f.K <= f.BLOCK && !(f.EPB == n.EPB && f.BLOCK == n.BLOCK);
+}
+
 template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
     static constexpr int NV = DIM + 2;
     static constexpr int K = DIM == 2 ? NP * NP : NP * NP * NP;
@@ -207,11 +240,9 @@ template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
     static constexpr int NFV = BLEND ? NP - 1 : 0;
     static constexpr int JP = NFP * (NPAIR + NFV);
     static constexpr int JF = 2 * DIM * NFP;
-    static constexpr int mx(int a, int b) { return a > b ? a : b; }
-    static constexpr int WORK = mx(K, mx(JP, JF));
-    static constexpr int EPB = K > 128 ? 1 : mx(1, 256 / WORK);
-    static constexpr int r64(int x) { return ((x + 63) / 64) * 64; }
-    static constexpr int BLOCK = FB ? FB : K > 256 ? r64(K) : (r64(EPB * WORK) > 512 ? 512 : r64(EPB * WORK));
+    static constexpr int WORK = rshape(DIM, NP, BLEND, FB).WORK;
+    static constexpr int EPB = rshape(DIM, NP, BLEND, FB).EPB;
+    static constexpr int BLOCK = rshape(DIM, NP, BLEND, FB).BLOCK;
     static constexpr int lc_pick()
     {
         for (int lc = NFP; lc >= 1; lc--)
@@ -220,7 +251,8 @@ template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
     }
     static constexpr int LC = lc_pick();
     static constexpr int FXN = LC * (NPAIR + NFV);
-    static constexpr int WAVES = K > 256 ? 2 : PRANDTL_WAVES;
+    // one- and two-wave blocks: cap registers at 128 so 4 waves fit a SIMD
+    static constexpr int WAVES = K > 256 ? 2 : BLOCK <= 128 ? 4 : PRANDTL_WAVES;
     static_assert(EPB * K <= BLOCK, "one node per thread");
     static_assert(!BLEND || 2 * K <= NV * FXN, "indicator scratch fits the flux buffer");
 };
@@ -236,8 +268,10 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB> struct RhsSmem {
     double Sub[DIM][NP];
     double U[C::EPB][C::NV][C::K];
     double Q[C::EPB][NQ][C::K];
-    double Fc[C::EPB][C::NV][C::JF];    // signed surface contributions
-    double F[C::EPB][C::NV][C::FXN];    // pair and subcell fluxes of one line chunk
+    // signed surface contributions (surface phase), then the pair and
+    // subcell fluxes of one line chunk (direction phases): one buffer
+    static constexpr int FF = C::JF > C::FXN ? C::JF : C::FXN;
+    double F[C::EPB][C::NV][FF];
     double G[GE][C::NV][GK];
     double Al[C::EPB];
     double Red[(C::BLOCK + 63) / 64];
@@ -264,8 +298,8 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     auto &sSub = S.Sub;
     auto &sU = S.U;
     auto &sQ = S.Q;
-    auto &sFc = S.Fc;
     auto &sF = S.F;
+    auto &sFc = S.F;
     auto &sG = S.G;
     auto &sAl = S.Al;
     auto &sRed = S.Red;
@@ -469,6 +503,22 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     for (int v = 0; v < NV; v++) { accV[v] = 0.0; accS[v] = 0.0; accF[v] = 0.0; }
     double u0v[NV], vis[NV];
 
+    // this node's surface terms, faces in direction order, low side first
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < DIM; r++) {
+            const int l = line_of<DIM, NP>(r, ii[0], ii[1], ii[2]);
+            if (ii[r] == 0)
+#pragma unroll
+                for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sFc[el][v][(2 * r) * NFP + l];
+            if (ii[r] == NP - 1)
+#pragma unroll
+                for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sFc[el][v][(2 * r + 1) * NFP + l];
+        }
+    }
+    __syncthreads();
+
     for (int r = 0; r < DIM; r++) {
         const double nr = A.geo.nrm[r];
         if (r == DIM - 1 && act) {
@@ -557,12 +607,6 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                             for (int v = 0; v < NV; v++) accV[v] += dm * sF[el][v][lc * NPAIR + p];
                         }
                     }
-                    if (a == 0)
-#pragma unroll
-                        for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sFc[el][v][(2 * r) * NFP + l];
-                    if (a == NP - 1)
-#pragma unroll
-                        for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sFc[el][v][(2 * r + 1) * NFP + l];
                     if (BLEND && alpha_e > 0.0) {
                         const int base = LC * NPAIR + lc * NFV;
 #pragma unroll
@@ -603,12 +647,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     block_lam_max<BLOCK>(lam, A.lam, sRed, slot);
 }
 
-template <int DIM, int NP, bool BLEND, bool QNS3, typename VT>
-__global__ void __launch_bounds__((RCfg<DIM, NP, BLEND>::BLOCK), (RCfg<DIM, NP, BLEND>::WAVES))
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB>
+__global__ void __launch_bounds__((RCfg<DIM, NP, BLEND, FB>::BLOCK), (RCfg<DIM, NP, BLEND, FB>::WAVES))
 k_rhs(const KArgs A)
 {
-    __shared__ RhsSmem<DIM, NP, BLEND, QNS3, 0> S;
-    rhs_body<DIM, NP, BLEND, QNS3, VT, 0>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
+    __shared__ RhsSmem<DIM, NP, BLEND, QNS3, FB> S;
+    rhs_body<DIM, NP, BLEND, QNS3, VT, FB>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -997,6 +1041,9 @@ struct prandtl_ctx {
     const double *graph_u = nullptr;
     hipStream_t graph_stream = nullptr;
     const double *dtp = nullptr;  // set while launching stages for a graph
+    double graph_dt = 0.0;        // value last copied to d_dt
+    bool graph_dt_set = false;
+    int rhs_shape = 0;            // k_rhs launch shape (only shapes rhs_shape_ok accepts are set)
 };
 
 namespace {
@@ -1056,17 +1103,42 @@ template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, doub
     return hip_check(c, hipGetLastError(), "k_visc launch");
 }
 
+template <int DIM, int NP, bool BLEND, int FB>
+void launch_rhs_fb(prandtl_ctx *c, const KArgs &a, hipStream_t s)
+{
+    using Cf = RCfg<DIM, NP, BLEND, FB>;
+    const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+    if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
+        hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, float, FB>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, FB>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+}
+
 template <int DIM, int NP, bool BLEND>
 int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
 {
-    using Cf = RCfg<DIM, NP, BLEND>;
-    const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
     ProfScope ps(c, s, 0);
     const bool qns3 = DIM == 3 && c->prm.parabolic && c->prm.quirks;
-    if (qns3) hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, DIM == 3, double>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
-    else if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
-        hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, float>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
-    else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    if (qns3) {
+        // the faithful NS-3D form runs in the natural shape only
+        using Cf = RCfg<DIM, NP, BLEND>;
+        const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+        hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, DIM == 3, double, 0>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+        return hip_check(c, hipGetLastError(), "k_rhs launch");
+    }
+    switch (c->rhs_shape) {
+    case 1:
+        if constexpr (rhs_shape_ok(DIM, NP, BLEND, 1)) launch_rhs_fb<DIM, NP, BLEND, 64>(c, a, s);
+        break;
+    case 2:
+        if constexpr (rhs_shape_ok(DIM, NP, BLEND, 2)) launch_rhs_fb<DIM, NP, BLEND, 128>(c, a, s);
+        break;
+    case 3:
+        if constexpr (rhs_shape_ok(DIM, NP, BLEND, 3)) launch_rhs_fb<DIM, NP, BLEND, 256>(c, a, s);
+        break;
+    default:
+        launch_rhs_fb<DIM, NP, BLEND, 0>(c, a, s);
+        break;
+    }
     return hip_check(c, hipGetLastError(), "k_rhs launch");
 }
 
@@ -1129,6 +1201,26 @@ int stage(prandtl_ctx *c, const double *y, const double *u0, double *out, int mo
 int visc_only(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
 {
     PRANDTL_DISPATCH(launch_visc, c, y, grad, s)
+}
+
+// k_rhs alone (mode 0, RHS into out); k_visc's outputs must be current
+int rhs_only(prandtl_ctx *c, const double *y, double *out, hipStream_t s)
+{
+    PRANDTL_DISPATCH(launch_rhs, c, y, nullptr, out, 0, 0.0, 0.0, 0.0, s)
+}
+
+bool shape_ok(const prandtl_ctx *c, int shape)
+{
+    if (shape == 0) return true;
+    if (c->dim == 3 && c->prm.parabolic && c->prm.quirks) return false;
+    return rhs_shape_ok(c->dim, c->Np, c->prm.blending != 0, shape);
+}
+
+void drop_graph(prandtl_ctx *c)
+{
+    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    c->graph_u = nullptr;
 }
 
 } // namespace
@@ -1333,9 +1425,15 @@ int prandtl_rk3_step(prandtl_ctx *c, double *u, double dt, void *stream)
     hipStream_t s = (hipStream_t)stream;
     // The null stream cannot be captured; profiling brackets single launches.
     if (!c->use_graph || c->prof || s == nullptr) return rk3_stages(c, u, dt, s);
-    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(1), 0, s, c->d_dt, dt);
-    int rc = hip_check(c, hipGetLastError(), "set dt");
-    if (rc) return rc;
+    int rc = PRANDTL_OK;
+    // dt lives in device memory for the graph; copy it only when it changes
+    if (!c->graph_dt_set || c->graph_dt != dt || c->graph_stream != s) {
+        hipLaunchKernelGGL(k_set_double, dim3(1), dim3(1), 0, s, c->d_dt, dt);
+        rc = hip_check(c, hipGetLastError(), "set dt");
+        if (rc) return rc;
+        c->graph_dt = dt;
+        c->graph_dt_set = true;
+    }
     if (!c->graph_exec || c->graph_u != u || c->graph_stream != s) {
         if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
         if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -1355,6 +1453,53 @@ int prandtl_rk3_step(prandtl_ctx *c, double *u, double dt, void *stream)
         c->graph_stream = s;
     }
     return hip_check(c, hipGraphLaunch(c->graph_exec, s), "graph launch");
+}
+
+int prandtl_set_rhs_shape(prandtl_ctx *c, int shape)
+{
+    if (!c) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    if (!shape_ok(c, shape)) return fail(c, PRANDTL_ERR_UNSUPPORTED, "launch shape not built for this case");
+    if (shape != c->rhs_shape) drop_graph(c);
+    c->rhs_shape = shape;
+    return PRANDTL_OK;
+}
+
+int prandtl_autotune(prandtl_ctx *c, const double *u, int *shape_out, double *us_out, void *stream)
+{
+    if (!c || !u) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = PRANDTL_OK;
+    if (c->prm.parabolic) rc = visc_only(c, u, nullptr, s);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = hip_check(c, hipEventCreate(&e0), "event");
+    if (!rc) rc = hip_check(c, hipEventCreate(&e1), "event");
+    const int prev = c->rhs_shape;
+    int best = prev;
+    double best_us = 0.0;
+    for (int sh = 0; sh < PRANDTL_RHS_SHAPES && !rc; sh++) {
+        if (us_out) us_out[sh] = -1.0;
+        if (!shape_ok(c, sh)) continue;
+        c->rhs_shape = sh;
+        constexpr int warm = 2, reps = 10;
+        for (int i = 0; i < warm && !rc; i++) rc = rhs_only(c, u, c->d_yb, s);
+        if (!rc) rc = hip_check(c, hipEventRecord(e0, s), "event record");
+        for (int i = 0; i < reps && !rc; i++) rc = rhs_only(c, u, c->d_yb, s);
+        if (!rc) rc = hip_check(c, hipEventRecord(e1, s), "event record");
+        if (!rc) rc = hip_check(c, hipEventSynchronize(e1), "event sync");
+        float ms = 0.0f;
+        if (!rc) rc = hip_check(c, hipEventElapsedTime(&ms, e0, e1), "event time");
+        if (rc) break;
+        const double us = 1e3 * ms / reps;
+        if (us_out) us_out[sh] = us;
+        if (sh == 0 || us < best_us) { best = sh; best_us = us; }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c->rhs_shape = rc ? prev : best;
+    if (!rc && best != prev) drop_graph(c);
+    if (!rc && shape_out) *shape_out = best;
+    return rc;
 }
 
 int prandtl_set_graph(prandtl_ctx *c, int enable)

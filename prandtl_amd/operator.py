@@ -127,6 +127,22 @@ class DGSEMOperator:
         """Capture/replay the RK3 step as a hipGraph on non-default streams."""
         check(lib().prandtl_set_graph(self._h, int(enable)), self._h)
 
+    def set_rhs_shape(self, shape):
+        """Force a launch shape of the RHS kernel (0 = default)."""
+        check(lib().prandtl_set_rhs_shape(self._h, int(shape)), self._h)
+
+    def autotune(self, u):
+        """Time every launch shape on state u and keep the fastest.
+
+        Returns (shape, times): times[s] is the mean RHS-kernel time in
+        microseconds of shape s, or None for shapes this case does not have.
+        """
+        shape = ctypes.c_int()
+        us = (ctypes.c_double * _lib.PRANDTL_RHS_SHAPES)()
+        check(lib().prandtl_autotune(self._h, self._vec(u, "u"), ctypes.byref(shape), us, _stream_ptr()),
+              self._h)
+        return shape.value, [v if v >= 0 else None for v in us]
+
     def GetMaxCharSpeed(self, reset=False):
         out = ctypes.c_double()
         check(lib().prandtl_max_wave_speed(self._h, ctypes.byref(out), int(reset), _stream_ptr()), self._h)
