@@ -289,8 +289,8 @@ def main():
     lam = op.GetMaxCharSpeed()
     dt = 0.1 * op.hmin / (lam * (cfg["order"] + 1) ** 2)
 
-    # launch shape of the RHS kernel: timed on this state, fastest kept
-    shape, shape_us = (0, None) if args.no_autotune else op.autotune(u)
+    # launch shapes of both kernels: timed on this state, fastest kept
+    shape, shape_us = ({"rhs": 0, "visc": 0}, None) if args.no_autotune else op.autotune(u)
 
     for _ in range(args.warmup):
         op.Step(u, dt)
@@ -351,7 +351,7 @@ def main():
                                       " (reference defects corrected)"),
                    "elements": op.NE, "order": cfg["order"], "nodes": nodes, "vars": nv,
                    "rhs_per_step": 3, "dt": dt, "state_ok": bool(ok), "hip_graph": True,
-                   "rhs_shape": shape, "rhs_shape_us": shape_us,
+                   "launch_shapes": shape, "launch_shape_us": shape_us,
                    "ms_per_step_event_profiled": elapsed_prof / psteps * 1e3,
                    "kernel_ms": {"k_rhs": ms_rhs / max(n_rhs, 1), "k_visc": ms_visc / max(n_visc, 1)}},
         "roofline": roof,

@@ -85,21 +85,26 @@ int prandtl_rk3_step(prandtl_ctx *ctx, double *u_dev, double dt, void *stream);
  * directly every step.  Default: enabled. */
 int prandtl_set_graph(prandtl_ctx *ctx, int enable);
 
-/* Launch shapes of the RHS kernel (no counterpart in the reference): how
- * many elements one workgroup takes and how many threads it has.  Shape 0
- * is the default; shapes 1..3 force 64, 128 or 256 threads per workgroup
- * where that still gives every node its own thread.  Every shape gives
- * bitwise the same results.  PRANDTL_ERR_UNSUPPORTED for a shape this
- * case does not have. */
-#define PRANDTL_RHS_SHAPES 4
-int prandtl_set_rhs_shape(prandtl_ctx *ctx, int shape);
+/* Launch shapes (no counterpart in the reference): how many elements one
+ * workgroup of a kernel takes and how many threads it has.  kernel is
+ * PRANDTL_KERNEL_RHS (the flux-differencing kernel) or PRANDTL_KERNEL_VISC
+ * (the BR1 lifting and viscous-flux kernel, Navier-Stokes only).  Shape 0 is
+ * the default; shapes 1..3 force 64, 128 or 256 threads per workgroup where
+ * that still gives every node its own thread.  Every shape gives bitwise the
+ * same results.  PRANDTL_ERR_UNSUPPORTED for a shape this case does not
+ * have. */
+#define PRANDTL_SHAPES 4
+#define PRANDTL_KERNEL_RHS 0
+#define PRANDTL_KERNEL_VISC 1
+int prandtl_set_shape(prandtl_ctx *ctx, int kernel, int shape);
 
-/* Autotuner: times every shape this case has on state u_dev (2 + 10 RHS
- * evaluations each, into context scratch; the running wave speed sees
- * u_dev) and keeps the fastest.  shape_out (may be NULL) receives it;
- * us_out (may be NULL) receives PRANDTL_RHS_SHAPES mean microseconds per
- * RHS kernel, -1 for shapes this case does not have. */
-int prandtl_autotune(prandtl_ctx *ctx, const double *u_dev, int *shape_out, double *us_out,
+/* Autotuner: times every shape this case has, of each kernel, on state
+ * u_dev (2 + 10 launches each, into context scratch; the running wave speed
+ * sees u_dev) and keeps the fastest of each.  shapes_out (may be NULL)
+ * receives the two chosen shapes [rhs, visc]; us_out (may be NULL) receives
+ * 2 * PRANDTL_SHAPES mean microseconds per launch, rhs shapes first, -1 for
+ * shapes a case does not have. */
+int prandtl_autotune(prandtl_ctx *ctx, const double *u_dev, int *shapes_out, double *us_out,
                      void *stream);
 
 /* DGSEMOperator::GetMaxCharSpeed (DGSEMOperator.hpp:106-109): running

@@ -17,11 +17,13 @@ PRANDTL_ERR_STATE = 4
 PRANDTL_ERR_NOMEM = 5
 PRANDTL_FP64 = 0
 PRANDTL_MIXED = 1
-PRANDTL_RHS_SHAPES = 4
+PRANDTL_SHAPES = 4
+PRANDTL_KERNEL_RHS = 0
+PRANDTL_KERNEL_VISC = 1
 
 EXPORTED = (
     "prandtl_create", "prandtl_destroy", "prandtl_sizes", "prandtl_rhs", "prandtl_rk3_step",
-    "prandtl_set_graph", "prandtl_set_rhs_shape", "prandtl_autotune",
+    "prandtl_set_graph", "prandtl_set_shape", "prandtl_autotune",
     "prandtl_max_wave_speed", "prandtl_alpha", "prandtl_check_state", "prandtl_lgl",
     "prandtl_debug_gradients", "prandtl_profile_begin", "prandtl_profile_end", "prandtl_last_error",
     "prandtl_ensemble_create", "prandtl_ensemble_destroy", "prandtl_ensemble_case",
@@ -70,7 +72,7 @@ def lib():
     L.prandtl_rhs.argtypes = [vp, vp, vp, vp]
     L.prandtl_rk3_step.argtypes = [vp, vp, ctypes.c_double, vp]
     L.prandtl_set_graph.argtypes = [vp, ctypes.c_int]
-    L.prandtl_set_rhs_shape.argtypes = [vp, ctypes.c_int]
+    L.prandtl_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.prandtl_autotune.argtypes = [vp, vp, ip, dp, vp]
     L.prandtl_max_wave_speed.argtypes = [vp, dp, ctypes.c_int, vp]
     L.prandtl_alpha.argtypes = [vp, dp, dp, vp]

@@ -93,25 +93,38 @@ struct VArgs {
     double *grad;         // optional [dim][nv][NE][K]
 };
 
-template <int DIM, int NP, bool BLEND = false> struct Cfg {
+constexpr int cmx(int a, int b) { return a > b ? a : b; }
+constexpr int r64(int x) { return ((x + 63) / 64) * 64; }
+
+// Launch shapes of both kernels: 0 natural, 1..3 forced blocks of 64, 128
+// and 256 threads (autotuned per case).
+constexpr int NSHAPES = PRANDTL_SHAPES;
+constexpr int shape_fb(int shape) { return shape == 1 ? 64 : shape == 2 ? 128 : shape == 3 ? 256 : 0; }
+
+// Launch shape of k_visc: one thread per node, BLOCK / K elements per
+// workgroup.
+constexpr int visc_block(int DIM, int NP, int FB)
+{
+    const int K = DIM == 2 ? NP * NP : NP * NP * NP;
+    return FB ? FB : K > 256 ? r64(K) : 256;
+}
+constexpr bool visc_shape_ok(int DIM, int NP, int shape)
+{
+    if (shape == 0) return true;
+    if (shape < 0 || shape >= NSHAPES) return false;
+    const int K = DIM == 2 ? NP * NP : NP * NP * NP;
+    const int fb = shape_fb(shape);
+    return K <= fb && fb != visc_block(DIM, NP, 0);
+}
+template <int DIM, int NP, int FB = 0> struct VCfg {
     static constexpr int NV = DIM + 2;
     static constexpr int K = DIM == 2 ? NP * NP : NP * NP * NP;
     static constexpr int NFP = DIM == 2 ? NP : NP * NP;
-    static constexpr int NPAIR = NP * (NP - 1) / 2;
-    static constexpr int BLOCK = K > 256 ? ((K + 63) / 64) * 64 : 256;
+    static constexpr int BLOCK = visc_block(DIM, NP, FB);
     static constexpr int EPB = BLOCK / K;
-    // jobs per line: volume pairs, the two surface points, subcell interfaces
-    static constexpr int JPL = NPAIR + 2 + (BLEND ? NP - 1 : 0);
-    static constexpr int lc_pick()
-    {
-        for (int lc = NFP; lc >= 1; lc--)
-            if (NFP % lc == 0 && EPB * lc * JPL * NV * 8 <= 24 * 1024) return lc;
-        return 1;
-    }
-    static constexpr int LC = lc_pick();
-    static constexpr int FXN = LC * JPL;
     // waves per SIMD to ask the register allocator for
-    static constexpr int WAVES = K > 256 ? 2 : PRANDTL_WAVES;
+    static constexpr int WAVES = K > 256 ? 2 : BLOCK <= 128 ? 4 : PRANDTL_WAVES;
+    static_assert(EPB >= 1, "one node per thread");
 };
 
 template <int DIM, int NP>
@@ -206,8 +219,6 @@ __device__ __forceinline__ void block_lam_max(double lam, double *slots, double 
 struct RShape {
     int K, WORK, EPB, BLOCK;
 };
-constexpr int cmx(int a, int b) { return a > b ? a : b; }
-constexpr int r64(int x) { return ((x + 63) / 64) * 64; }
 constexpr RShape rshape(int DIM, int NP, bool BLEND, int FB)
 {
     const int K = DIM == 2 ? NP * NP : NP * NP * NP;
@@ -218,17 +229,14 @@ constexpr RShape rshape(int DIM, int NP, bool BLEND, int FB)
     const int BLOCK = FB ? FB : K > 256 ? r64(K) : (r64(EPB * WORK) > 512 ? 512 : r64(EPB * WORK));
     return RShape{K, WORK, EPB, BLOCK};
 }
-// Launch shapes of k_rhs: 0 natural, 1..3 blocks of 64, 128, 256 threads.
-// A forced shape exists when it holds one node per thread and differs from
-// the natural one.
-constexpr int RHS_SHAPES = PRANDTL_RHS_SHAPES;
-constexpr int rhs_shape_fb(int shape) { return shape == 1 ? 64 : shape == 2 ? 128 : shape == 3 ? 256 : 0; }
+// A forced k_rhs shape exists when it holds one node per thread and differs
+// from the natural one.
 constexpr bool rhs_shape_ok(int DIM, int NP, bool BLEND, int shape)
 {
     if (shape == 0) return true;
-    if (shape < 0 || shape >= RHS_SHAPES) return false;
+    if (shape < 0 || shape >= NSHAPES) return false;
     const RShape n = rshape(DIM, NP, BLEND, 0);
-    const RShape f = rshape(DIM, NP, BLEND, rhs_shape_fb(shape));
+    const RShape f = rshape(DIM, NP, BLEND, shape_fb(shape));
     return f.EPB * f.K <= f.BLOCK && !(f.EPB == n.EPB && f.BLOCK == n.BLOCK);
 }
 
@@ -723,10 +731,10 @@ __global__ void k_set_dts(double *dst, int n, const DtArgs a)
 // FP32 on those offsets (D annihilates constants, so the shift leaves the
 // derivatives unchanged while keeping their significant digits).  The
 // outputs visc and fvf are FP32 in that form.
-template <int DIM, int NP, bool MIXED>
-__global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) k_visc(const VArgs A)
+template <int DIM, int NP, bool MIXED, int FB>
+__global__ void __launch_bounds__((VCfg<DIM, NP, FB>::BLOCK), (VCfg<DIM, NP, FB>::WAVES)) k_visc(const VArgs A)
 {
-    using C = Cfg<DIM, NP>;
+    using C = VCfg<DIM, NP, FB>;
     using R = typename std::conditional<MIXED, float, double>::type;
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, EPB = C::EPB, BLOCK = C::BLOCK;
     // phase 1 (lifting) needs the entropy variables and neighbour traces,
@@ -737,7 +745,7 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
 
     __shared__ R sD[NP * NP];
     __shared__ R sDh[NP * NP];
-    __shared__ double sU[EPB][NV][K];
+    __shared__ double sUl[EPB][NV];   // state of node K-1 (the faithful stale state)
     __shared__ double sWref[EPB][NV];
     __shared__ R sScr[SCR];
     R (*sW)[NV][K] = reinterpret_cast<R (*)[NV][K]>(sScr);
@@ -762,13 +770,34 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
     R *fvf = reinterpret_cast<R *>(A.fvf);
     const R gm1 = (R)P.gm1, gm1i = (R)P.gm1i, mu = (R)P.mu, lam_h = (R)P.lam_h;
 
+    // every global load of this thread in flight together: its own node and
+    // the neighbour trace nodes of its face jobs
+    constexpr int NJ = EPB * 2 * DIM * NFP;
+    constexpr int JT = (NJ + BLOCK - 1) / BLOCK;
     double ys[NV], wv[NV];
-    if (act) {
+    double tv[JT][NV];
+    if (act)
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            ys[v] = A.y[v * NK + (size_t)e * K + n];
-            sU[el][v][n] = ys[v];
+        for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)e * K + n];
+#pragma unroll
+    for (int k = 0; k < JT; k++) {
+        const int j = t + k * BLOCK;
+        const int ej = j / (2 * DIM * NFP);
+        const int eo = ebase + ej;
+        if (j < NJ && eo < NE) {
+            const int rem = j - ej * (2 * DIM * NFP);
+            const int f = rem / NFP, q = rem - f * NFP;
+            const int r = f >> 1, side = f & 1;
+            const int en = neighbor(A.geo, eo, r, side);
+            const int nn = line_node<DIM, NP>(r, q, side ? 0 : NP - 1);
+#pragma unroll
+            for (int v = 0; v < NV; v++) tv[k][v] = A.y[v * NK + (size_t)en * K + nn];
         }
+    }
+    if (act) {
+        if (n == K - 1)
+#pragma unroll
+            for (int v = 0; v < NV; v++) sUl[el][v] = ys[v];
         cons2entropy<DIM>(ys, P, wv);
         if (MIXED && n == 0)
 #pragma unroll
@@ -779,21 +808,19 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
 #pragma unroll
         for (int v = 0; v < NV; v++) sW[el][v][n] = MIXED ? (R)(wv[v] - sWref[el][v]) : (R)wv[v];
     // neighbour traces of the entropy variables
-    for (int j = t; j < EPB * 2 * DIM * NFP; j += BLOCK) {
+#pragma unroll
+    for (int k = 0; k < JT; k++) {
+        const int j = t + k * BLOCK;
         const int ej = j / (2 * DIM * NFP);
-        const int rem = j - ej * (2 * DIM * NFP);
-        const int f = rem / NFP, q = rem - f * NFP;
         const int eo = ebase + ej;
-        if (eo >= NE) continue;
-        const int r = f >> 1, side = f & 1;
-        const int en = neighbor(A.geo, eo, r, side);
-        const int nn = line_node<DIM, NP>(r, q, side ? 0 : NP - 1);
-        double sv[NV], w[NV];
+        if (j < NJ && eo < NE) {
+            const int rem = j - ej * (2 * DIM * NFP);
+            const int f = rem / NFP, q = rem - f * NFP;
+            double w[NV];
+            cons2entropy<DIM>(tv[k], P, w);
 #pragma unroll
-        for (int v = 0; v < NV; v++) sv[v] = A.y[v * NK + (size_t)en * K + nn];
-        cons2entropy<DIM>(sv, P, w);
-#pragma unroll
-        for (int v = 0; v < NV; v++) sWN[ej][f][v][q] = MIXED ? (R)(w[v] - sWref[ej][v]) : (R)w[v];
+            for (int v = 0; v < NV; v++) sWN[ej][f][v][q] = MIXED ? (R)(w[v] - sWref[ej][v]) : (R)w[v];
+        }
     }
     __syncthreads();
 
@@ -846,7 +873,7 @@ __global__ void __launch_bounds__((Cfg<DIM, NP>::BLOCK), (Cfg<DIM, NP>::WAVES)) 
         // volume viscous flux uses the stale node K-1 state (DGSEMIntegrator.cpp:620)
         R sv[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) sv[v] = A.quirks ? (R)sU[el][v][K - 1] : yr[v];
+        for (int v = 0; v < NV; v++) sv[v] = A.quirks ? (R)sUl[el][v] : yr[v];
         viscous_flux<DIM, R>(sv, gp, gm1, gm1i, mu, lam_h, fl);
 #pragma unroll
         for (int r = 0; r < DIM; r++)
@@ -1044,6 +1071,7 @@ struct prandtl_ctx {
     double graph_dt = 0.0;        // value last copied to d_dt
     bool graph_dt_set = false;
     int rhs_shape = 0;            // k_rhs launch shape (only shapes rhs_shape_ok accepts are set)
+    int visc_shape = 0;           // k_visc launch shape (visc_shape_ok)
 };
 
 namespace {
@@ -1088,18 +1116,36 @@ struct ProfScope {
     }
 };
 
+template <int DIM, int NP, int FB> void launch_visc_fb(prandtl_ctx *c, const VArgs &a, hipStream_t s)
+{
+    using Cf = VCfg<DIM, NP, FB>;
+    const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+    if (c->prm.precision == PRANDTL_MIXED)
+        hipLaunchKernelGGL((k_visc<DIM, NP, true, FB>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_visc<DIM, NP, false, FB>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+}
+
 template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
 {
-    using Cf = Cfg<DIM, NP>;
     VArgs a;
     a.tab = c->tab; a.geo = c->geo; a.phys = c->phys;
     a.quirks = c->prm.quirks;
     a.y = y; a.visc = c->d_visc; a.fvf = c->d_fvf; a.grad = grad;
-    const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
     ProfScope ps(c, s, 1);
-    if (c->prm.precision == PRANDTL_MIXED)
-        hipLaunchKernelGGL((k_visc<DIM, NP, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
-    else hipLaunchKernelGGL((k_visc<DIM, NP, false>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+    switch (c->visc_shape) {
+    case 1:
+        if constexpr (visc_shape_ok(DIM, NP, 1)) launch_visc_fb<DIM, NP, 64>(c, a, s);
+        break;
+    case 2:
+        if constexpr (visc_shape_ok(DIM, NP, 2)) launch_visc_fb<DIM, NP, 128>(c, a, s);
+        break;
+    case 3:
+        if constexpr (visc_shape_ok(DIM, NP, 3)) launch_visc_fb<DIM, NP, 256>(c, a, s);
+        break;
+    default:
+        launch_visc_fb<DIM, NP, 0>(c, a, s);
+        break;
+    }
     return hip_check(c, hipGetLastError(), "k_visc launch");
 }
 
@@ -1209,8 +1255,10 @@ int rhs_only(prandtl_ctx *c, const double *y, double *out, hipStream_t s)
     PRANDTL_DISPATCH(launch_rhs, c, y, nullptr, out, 0, 0.0, 0.0, 0.0, s)
 }
 
-bool shape_ok(const prandtl_ctx *c, int shape)
+bool shape_ok(const prandtl_ctx *c, int kernel, int shape)
 {
+    if (kernel == PRANDTL_KERNEL_VISC) return c->prm.parabolic && visc_shape_ok(c->dim, c->Np, shape);
+    if (kernel != PRANDTL_KERNEL_RHS) return false;
     if (shape == 0) return true;
     if (c->dim == 3 && c->prm.parabolic && c->prm.quirks) return false;
     return rhs_shape_ok(c->dim, c->Np, c->prm.blending != 0, shape);
@@ -1455,51 +1503,64 @@ int prandtl_rk3_step(prandtl_ctx *c, double *u, double dt, void *stream)
     return hip_check(c, hipGraphLaunch(c->graph_exec, s), "graph launch");
 }
 
-int prandtl_set_rhs_shape(prandtl_ctx *c, int shape)
+int prandtl_set_shape(prandtl_ctx *c, int kernel, int shape)
 {
     if (!c) return fail(c, PRANDTL_ERR_ARG, "null argument");
-    if (!shape_ok(c, shape)) return fail(c, PRANDTL_ERR_UNSUPPORTED, "launch shape not built for this case");
-    if (shape != c->rhs_shape) drop_graph(c);
-    c->rhs_shape = shape;
+    if (!shape_ok(c, kernel, shape)) return fail(c, PRANDTL_ERR_UNSUPPORTED, "launch shape not built for this case");
+    int &cur = kernel == PRANDTL_KERNEL_RHS ? c->rhs_shape : c->visc_shape;
+    if (shape != cur) drop_graph(c);
+    cur = shape;
     return PRANDTL_OK;
 }
 
-int prandtl_autotune(prandtl_ctx *c, const double *u, int *shape_out, double *us_out, void *stream)
+int prandtl_autotune(prandtl_ctx *c, const double *u, int *shapes_out, double *us_out, void *stream)
 {
     if (!c || !u) return fail(c, PRANDTL_ERR_ARG, "null argument");
     hipStream_t s = (hipStream_t)stream;
-    int rc = PRANDTL_OK;
-    if (c->prm.parabolic) rc = visc_only(c, u, nullptr, s);
-    if (rc) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    rc = hip_check(c, hipEventCreate(&e0), "event");
+    int rc = hip_check(c, hipEventCreate(&e0), "event");
     if (!rc) rc = hip_check(c, hipEventCreate(&e1), "event");
-    const int prev = c->rhs_shape;
-    int best = prev;
-    double best_us = 0.0;
-    for (int sh = 0; sh < PRANDTL_RHS_SHAPES && !rc; sh++) {
-        if (us_out) us_out[sh] = -1.0;
-        if (!shape_ok(c, sh)) continue;
-        c->rhs_shape = sh;
-        constexpr int warm = 2, reps = 10;
-        for (int i = 0; i < warm && !rc; i++) rc = rhs_only(c, u, c->d_yb, s);
-        if (!rc) rc = hip_check(c, hipEventRecord(e0, s), "event record");
-        for (int i = 0; i < reps && !rc; i++) rc = rhs_only(c, u, c->d_yb, s);
-        if (!rc) rc = hip_check(c, hipEventRecord(e1, s), "event record");
-        if (!rc) rc = hip_check(c, hipEventSynchronize(e1), "event sync");
-        float ms = 0.0f;
-        if (!rc) rc = hip_check(c, hipEventElapsedTime(&ms, e0, e1), "event time");
-        if (rc) break;
-        const double us = 1e3 * ms / reps;
-        if (us_out) us_out[sh] = us;
-        if (sh == 0 || us < best_us) { best = sh; best_us = us; }
+    const int prev[2] = {c->rhs_shape, c->visc_shape};
+    int best[2] = {prev[0], prev[1]};
+    for (int i = 0; us_out && i < 2 * PRANDTL_SHAPES; i++) us_out[i] = -1.0;
+    // k_visc first (k_rhs reads its outputs), then k_rhs
+    for (int kern = PRANDTL_KERNEL_VISC; kern >= PRANDTL_KERNEL_RHS && !rc; kern--) {
+        int &cur = kern == PRANDTL_KERNEL_RHS ? c->rhs_shape : c->visc_shape;
+        double best_us = 0.0;
+        bool any = false;
+        for (int sh = 0; sh < PRANDTL_SHAPES && !rc; sh++) {
+            if (!shape_ok(c, kern, sh)) continue;
+            cur = sh;
+            auto run = [&]() {
+                return kern == PRANDTL_KERNEL_RHS ? rhs_only(c, u, c->d_yb, s) : visc_only(c, u, nullptr, s);
+            };
+            constexpr int warm = 2, reps = 10;
+            for (int i = 0; i < warm && !rc; i++) rc = run();
+            if (!rc) rc = hip_check(c, hipEventRecord(e0, s), "event record");
+            for (int i = 0; i < reps && !rc; i++) rc = run();
+            if (!rc) rc = hip_check(c, hipEventRecord(e1, s), "event record");
+            if (!rc) rc = hip_check(c, hipEventSynchronize(e1), "event sync");
+            float ms = 0.0f;
+            if (!rc) rc = hip_check(c, hipEventElapsedTime(&ms, e0, e1), "event time");
+            if (rc) break;
+            const double us = 1e3 * ms / reps;
+            if (us_out) us_out[kern * PRANDTL_SHAPES + sh] = us;
+            if (!any || us < best_us) { best[kern] = sh; best_us = us; any = true; }
+        }
+        // the winner stays in place, so k_rhs is timed after the best k_visc
+        if (!rc) cur = best[kern];
+        if (!rc && kern == PRANDTL_KERNEL_VISC && c->prm.parabolic) rc = visc_only(c, u, nullptr, s);
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    c->rhs_shape = rc ? prev : best;
-    if (!rc && best != prev) drop_graph(c);
-    if (!rc && shape_out) *shape_out = best;
-    return rc;
+    if (rc) {
+        c->rhs_shape = prev[0];
+        c->visc_shape = prev[1];
+        return rc;
+    }
+    if (c->rhs_shape != prev[0] || c->visc_shape != prev[1]) drop_graph(c);
+    if (shapes_out) { shapes_out[0] = c->rhs_shape; shapes_out[1] = c->visc_shape; }
+    return PRANDTL_OK;
 }
 
 int prandtl_set_graph(prandtl_ctx *c, int enable)

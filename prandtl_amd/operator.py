@@ -127,21 +127,25 @@ class DGSEMOperator:
         """Capture/replay the RK3 step as a hipGraph on non-default streams."""
         check(lib().prandtl_set_graph(self._h, int(enable)), self._h)
 
-    def set_rhs_shape(self, shape):
-        """Force a launch shape of the RHS kernel (0 = default)."""
-        check(lib().prandtl_set_rhs_shape(self._h, int(shape)), self._h)
+    def set_shape(self, kernel, shape):
+        """Force a launch shape (0 = default) of kernel "rhs" or "visc"."""
+        k = {"rhs": _lib.PRANDTL_KERNEL_RHS, "visc": _lib.PRANDTL_KERNEL_VISC}[kernel]
+        check(lib().prandtl_set_shape(self._h, k, int(shape)), self._h)
 
     def autotune(self, u):
-        """Time every launch shape on state u and keep the fastest.
+        """Time every launch shape of both kernels on state u; keep the fastest.
 
-        Returns (shape, times): times[s] is the mean RHS-kernel time in
-        microseconds of shape s, or None for shapes this case does not have.
+        Returns ({"rhs": shape, "visc": shape}, {"rhs": times, "visc": times}),
+        times[s] the mean microseconds per launch of shape s, None for shapes
+        this case does not have.
         """
-        shape = ctypes.c_int()
-        us = (ctypes.c_double * _lib.PRANDTL_RHS_SHAPES)()
-        check(lib().prandtl_autotune(self._h, self._vec(u, "u"), ctypes.byref(shape), us, _stream_ptr()),
-              self._h)
-        return shape.value, [v if v >= 0 else None for v in us]
+        shapes = (ctypes.c_int * 2)()
+        us = (ctypes.c_double * (2 * _lib.PRANDTL_SHAPES))()
+        check(lib().prandtl_autotune(self._h, self._vec(u, "u"), shapes, us, _stream_ptr()), self._h)
+        n = _lib.PRANDTL_SHAPES
+        times = [v if v >= 0 else None for v in us]
+        return ({"rhs": shapes[0], "visc": shapes[1]},
+                {"rhs": times[:n], "visc": times[n:]})
 
     def GetMaxCharSpeed(self, reset=False):
         out = ctypes.c_double()

@@ -45,17 +45,32 @@ def state(cfg, dev):
     return torch.tensor(u.reshape(-1), dtype=torch.float64, device=dev)
 
 
-def shapes_of(op):
+def shapes_of(op, kernel="rhs"):
     out = []
-    for s in range(_lib.PRANDTL_RHS_SHAPES):
+    for s in range(_lib.PRANDTL_SHAPES):
         try:
-            op.set_rhs_shape(s)
+            op.set_shape(kernel, s)
         except PrandtlError as e:
             assert e.code == _lib.PRANDTL_ERR_UNSUPPORTED
             continue
         out.append(s)
-    op.set_rhs_shape(0)
+    if out:
+        op.set_shape(kernel, 0)
     return out
+
+
+def combos(op, cfg):
+    """(rhs shape, visc shape) pairs: every rhs shape with the default visc
+    shape, and every visc shape with the default rhs shape."""
+    rs = shapes_of(op, "rhs")
+    vs = shapes_of(op, "visc") if cfg.get("parabolic") else []
+    return [(r, 0) for r in rs] + [(0, v) for v in vs[1:]]
+
+
+def set_combo(op, cfg, combo):
+    op.set_shape("rhs", combo[0])
+    if cfg.get("parabolic"):
+        op.set_shape("visc", combo[1])
 
 
 @pytest.mark.parametrize("name", sorted(CONFIGS))
@@ -66,15 +81,16 @@ def test_shapes_bitwise_equal(cuda, name):
     u = state(cfg, cuda)
     ref = torch.empty_like(u)
     op.Mult(u, ref)
-    shapes = shapes_of(op)
     if cfg.get("quirks") and cfg.get("parabolic"):
-        assert shapes == [0]
-    for s in shapes[1:]:
-        op.set_rhs_shape(s)
+        assert shapes_of(op, "rhs") == [0]
+    if not cfg.get("parabolic"):
+        assert shapes_of(op, "visc") == []
+    for combo in combos(op, cfg)[1:]:
+        set_combo(op, cfg, combo)
         out = torch.empty_like(u)
         op.Mult(u, out)
         torch.cuda.synchronize()
-        assert torch.equal(out, ref), "%s: shape %d differs from shape 0" % (name, s)
+        assert torch.equal(out, ref), "%s: shapes %s differ from the default" % (name, combo)
 
 
 @pytest.mark.parametrize("name", ["2d_n4_blend", "3d_n3_ns", "3d_n3_ns_mixed"])
@@ -83,19 +99,19 @@ def test_shapes_graph_steps_equal(cuda, name):
     cfg = CONFIGS[name]
     op = DGSEMOperator(**cfg)
     u0 = state(cfg, cuda)
-    shapes = shapes_of(op)
+    cs = combos(op, cfg)
     results = []
     stream = torch.cuda.Stream(device=cuda)
     with torch.cuda.stream(stream):
-        for s in shapes:
-            op.set_rhs_shape(s)
+        for combo in cs:
+            set_combo(op, cfg, combo)
             u = u0.clone()
             for _ in range(3):
                 op.Step(u, 1e-3)
             results.append(u)
     torch.cuda.synchronize()
-    for s, r in zip(shapes[1:], results[1:]):
-        assert torch.equal(r, results[0]), "%s: shape %d steps differ" % (name, s)
+    for combo, r in zip(cs[1:], results[1:]):
+        assert torch.equal(r, results[0]), "%s: shapes %s steps differ" % (name, combo)
 
 
 def test_autotune(cuda):
@@ -105,13 +121,16 @@ def test_autotune(cuda):
     u = state(cfg, cuda)
     ref = torch.empty_like(u)
     op.Mult(u, ref)
-    shape, times = op.autotune(u)
-    valid = shapes_of(op)
-    assert shape in valid
-    assert len(times) == _lib.PRANDTL_RHS_SHAPES
-    assert all((times[s] is not None) == (s in valid) for s in range(_lib.PRANDTL_RHS_SHAPES))
-    assert times[shape] == min(t for t in times if t is not None)
-    op.set_rhs_shape(shape)
+    shapes, times = op.autotune(u)
+    for kernel in ("rhs", "visc"):
+        valid = shapes_of(op, kernel)
+        t = times[kernel]
+        assert shapes[kernel] in valid
+        assert len(t) == _lib.PRANDTL_SHAPES
+        assert all((t[s] is not None) == (s in valid) for s in range(_lib.PRANDTL_SHAPES))
+        assert t[shapes[kernel]] == min(v for v in t if v is not None)
+    op.set_shape("rhs", shapes["rhs"])
+    op.set_shape("visc", shapes["visc"])
     out = torch.empty_like(u)
     op.Mult(u, out)
     torch.cuda.synchronize()
