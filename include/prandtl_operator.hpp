@@ -6,6 +6,7 @@
 //     void Mult(const Vector &u, Vector &dudt) const     -> Mult(const double *u, double *dudt) const
 //     real_t GetMaxCharSpeed()                           -> GetMaxCharSpeed()
 //   mfem::RK3SSPSolver::Step(x, t, dt)                   -> RK3SSPSolver::Step(double *x, double &t, double &dt)
+//   (no counterpart)                                     -> DGSEMOperator::Autotune / SetShape, Ensemble
 //
 // Pointers are device pointers in the reference's byNODES layout
 // (var*NE*K + elem*K + node).  Errors throw prandtl_amd::Error on the C++
@@ -16,6 +17,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "prandtl_hip.h"
 
@@ -79,6 +81,18 @@ class DGSEMOperator {
         return true;
     }
 
+    // launch shapes of the two kernels, timed on state u (prandtl_autotune)
+    struct Shapes {
+        int rhs = 0, visc = 0;
+    };
+    Shapes Autotune(const double *u)
+    {
+        int sh[2] = {0, 0};
+        check(prandtl_autotune(ctx_, u, sh, nullptr, stream_), ctx_);
+        return Shapes{sh[0], sh[1]};
+    }
+    void SetShape(int kernel, int shape) { check(prandtl_set_shape(ctx_, kernel, shape), ctx_); }
+
     prandtl_ctx *Handle() const { return ctx_; }
 
   private:
@@ -90,6 +104,41 @@ class DGSEMOperator {
     void *stream_ = nullptr;
     int NE_ = 0, K_ = 0, nv_ = 0, order_ = 0;
     double hmin_ = 0.0;
+};
+
+// Independent 2-D Euler cases stepped together, one launch per RK stage
+// (prandtl_ensemble_*).  Case contexts belong to the ensemble.
+class Ensemble {
+  public:
+    Ensemble(const std::vector<prandtl_mesh_desc> &meshes, const std::vector<prandtl_params> &params,
+             void *stream = nullptr)
+        : stream_(stream)
+    {
+        if (meshes.size() != params.size()) throw Error(PRANDTL_ERR_ARG, "meshes and params differ in length");
+        const int rc = prandtl_ensemble_create((int)meshes.size(), meshes.data(), params.data(), &ens_);
+        if (rc != PRANDTL_OK) throw Error(rc, prandtl_ensemble_last_error(nullptr));
+    }
+    ~Ensemble() { prandtl_ensemble_destroy(ens_); }
+    Ensemble(const Ensemble &) = delete;
+    Ensemble &operator=(const Ensemble &) = delete;
+
+    // one SSP-RK3 step of case i's state u[i] by dt[i], for every case
+    void Step(double *const *u, const double *dt)
+    {
+        const int rc = prandtl_ensemble_rk3_step(ens_, u, dt, stream_);
+        if (rc != PRANDTL_OK) throw Error(rc, prandtl_ensemble_last_error(ens_));
+    }
+    prandtl_ctx *Case(int i)
+    {
+        prandtl_ctx *c = nullptr;
+        const int rc = prandtl_ensemble_case(ens_, i, &c);
+        if (rc != PRANDTL_OK) throw Error(rc, prandtl_ensemble_last_error(ens_));
+        return c;
+    }
+
+  private:
+    prandtl_ensemble *ens_ = nullptr;
+    void *stream_ = nullptr;
 };
 
 // mfem::RK3SSPSolver surface (Simulation.cpp:186-189, 1010)

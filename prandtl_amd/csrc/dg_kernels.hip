@@ -187,10 +187,11 @@ __device__ __forceinline__ double wave_max(double v)
 
 // Running maximum wave speed: each block keeps its own slot (one thread, a
 // plain read-modify-write, no contention); prandtl_max_wave_speed reduces
-// the slots when asked.  Non-finite speeds are ignored, as std::max(current,
+// the slots when asked.  old: the slot's value, read by thread 0 when the
+// block started (only this block writes the slot during a launch).  Non-finite speeds are ignored, as std::max(current,
 // NaN) does in the reference (DGSEMIntegrator.cpp:270).
 template <int BLOCK>
-__device__ __forceinline__ void block_lam_max(double lam, double *slots, double *sred, int slot)
+__device__ __forceinline__ void block_lam_max(double lam, double old, double *slots, double *sred, int slot)
 {
     lam = wave_max(lam);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -199,7 +200,6 @@ __device__ __forceinline__ void block_lam_max(double lam, double *slots, double 
     if (threadIdx.x == 0) {
         double m = sred[0];
         for (int i = 1; i < (BLOCK + 63) / 64; i++) m = (m < sred[i]) ? sred[i] : m;
-        const double old = slots[slot];
         if (old < m) slots[slot] = m;
     }
 }
@@ -331,8 +331,11 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     const bool quirk3d = A.quirks != 0;
     const int ii[3] = {n % NP, (n / NP) % NP, DIM == 3 ? n / (NP * NP) : 0};
 
-    // ---- prologue: own node state, and the neighbour data of this thread's
-    // first surface point, all loads in flight together ----
+    // ---- prologue: own node state, the neighbour data of this thread's
+    // first surface point, the step size and the tile's wave-speed slot, all
+    // loads in flight together ----
+    const double dt = A.dtp ? *A.dtp : A.dt;
+    const double lam_old = (t == 0) ? A.lam[slot] : 0.0;
     double ys[NV];
     if (act)
 #pragma unroll
@@ -633,7 +636,6 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     }
 
     // ---- assemble, stage update, store ----
-    const double dt = A.dtp ? *A.dtp : A.dt;
     if (act) {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -652,12 +654,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             A.out[gi] = o;
         }
     }
-    block_lam_max<BLOCK>(lam, A.lam, sRed, slot);
+    block_lam_max<BLOCK>(lam, lam_old, A.lam, sRed, slot);
 }
 
 template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB>
-__global__ void __launch_bounds__((RCfg<DIM, NP, BLEND, FB>::BLOCK), (RCfg<DIM, NP, BLEND, FB>::WAVES))
-k_rhs(const KArgs A)
+__global__ void __launch_bounds__((RCfg<DIM, NP, BLEND, FB>::BLOCK))
+__attribute__((amdgpu_waves_per_eu(RCfg<DIM, NP, BLEND, FB>::WAVES))) k_rhs(const KArgs A)
 {
     __shared__ RhsSmem<DIM, NP, BLEND, QNS3, FB> S;
     rhs_body<DIM, NP, BLEND, QNS3, VT, FB>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
@@ -695,7 +697,7 @@ __device__ __forceinline__ void ens_tile(const KArgs &A, int g, double *smem)
 }
 
 // grid = number of tiles; every field of a tile is uniform over the workgroup
-__global__ void __launch_bounds__(ENS_BLOCK, 3)
+__global__ void __launch_bounds__(ENS_BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
 k_ens(const KArgs *__restrict__ cases, const EnsTile *__restrict__ tiles)
 {
     __shared__ __align__(16) double smem[(ENS_SMEM + 7) / 8];
@@ -732,7 +734,8 @@ __global__ void k_set_dts(double *dst, int n, const DtArgs a)
 // derivatives unchanged while keeping their significant digits).  The
 // outputs visc and fvf are FP32 in that form.
 template <int DIM, int NP, bool MIXED, int FB>
-__global__ void __launch_bounds__((VCfg<DIM, NP, FB>::BLOCK), (VCfg<DIM, NP, FB>::WAVES)) k_visc(const VArgs A)
+__global__ void __launch_bounds__((VCfg<DIM, NP, FB>::BLOCK))
+__attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArgs A)
 {
     using C = VCfg<DIM, NP, FB>;
     using R = typename std::conditional<MIXED, float, double>::type;

@@ -1,6 +1,8 @@
 // C++ caller of the drop-in surface (include/prandtl_operator.hpp) on the GPU,
 // checked against the CPU oracle (oracle/liboracle.so) on the same input:
-// 2-D isentropic vortex, 16x16 quads, N = 3, RHS and three RK3 steps.
+// 2-D isentropic vortex, 16x16 quads, N = 3, RHS and three RK3 steps; then
+// the autotuned launch shape and a two-case ensemble, each bitwise against
+// the default path.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -82,9 +84,57 @@ int main()
     (void)hipMemcpy(step.data(), du, step.size() * 8, hipMemcpyDeviceToHost);
     const double r2 = rel_max(step, uref);
     const double lam = op.GetMaxCharSpeed(), lam_ref = orc_max_wave_speed(orc);
-    const bool ok = r1 <= 1e-12 && r2 <= 1e-12 && std::fabs(lam - lam_ref) <= 1e-14 * lam_ref &&
-                    std::fabs(t - 0.03) < 1e-15 && op.StateIsPhysical(du);
-    std::printf("%s rhs_rel=%.3e step_rel=%.3e lambda=%.17g/%.17g\n", ok ? "OK" : "FAIL", r1, r2, lam, lam_ref);
+    bool ok = r1 <= 1e-12 && r2 <= 1e-12 && std::fabs(lam - lam_ref) <= 1e-14 * lam_ref &&
+              std::fabs(t - 0.03) < 1e-15 && op.StateIsPhysical(du);
+
+    // the autotuned launch shape gives bitwise the same RHS
+    (void)hipMemcpy(du, u.data(), u.size() * 8, hipMemcpyHostToDevice);
+    const auto shapes = op.Autotune(du);
+    op.Mult(du, dr);
+    std::vector<double> rhs2(u.size());
+    (void)hipStreamSynchronize(stream);
+    (void)hipMemcpy(rhs2.data(), dr, rhs2.size() * 8, hipMemcpyDeviceToHost);
+    const bool tuned_same = rhs2 == rhs;
+
+    // an ensemble of this case and a coarser N=2 one: bitwise the same as
+    // stepping each alone
+    prandtl_mesh_desc mesh2 = {2, 2, {8, 6, 1}, {-5.0, -5.0, 0.0}, {5.0, 5.0, 1.0}};
+    prandtl_amd::Ensemble ens({mesh, mesh2}, {prm, prm}, stream);
+    prandtl_amd::DGSEMOperator op2(mesh2, prm, stream);
+    const size_t n2 = 4 * (size_t)op2.NumElements() * op2.NodesPerElement();
+    std::vector<double> u2(n2);
+    for (size_t i = 0; i < n2; i++) u2[i] = (i < n2 / 4) ? 1.0 + 0.1 * std::sin(0.01 * i) : 0.5 + 0.01 * (i % 7);
+    double *ea = nullptr, *eb = nullptr, *sa = nullptr, *sb = nullptr;
+    if (hipMalloc(&ea, u.size() * 8) != hipSuccess || hipMalloc(&eb, n2 * 8) != hipSuccess ||
+        hipMalloc(&sa, u.size() * 8) != hipSuccess || hipMalloc(&sb, n2 * 8) != hipSuccess) {
+        std::printf("FAIL alloc\n");
+        return 1;
+    }
+    (void)hipMemcpy(ea, u.data(), u.size() * 8, hipMemcpyHostToDevice);
+    (void)hipMemcpy(sa, u.data(), u.size() * 8, hipMemcpyHostToDevice);
+    (void)hipMemcpy(eb, u2.data(), n2 * 8, hipMemcpyHostToDevice);
+    (void)hipMemcpy(sb, u2.data(), n2 * 8, hipMemcpyHostToDevice);
+    double *ptrs[2] = {ea, eb};
+    const double dts[2] = {0.01, 0.02};
+    for (int s = 0; s < 2; s++) {
+        ens.Step(ptrs, dts);
+        op.Step(sa, dts[0]);
+        op2.Step(sb, dts[1]);
+    }
+    (void)hipStreamSynchronize(stream);
+    std::vector<double> va(u.size()), vb(u.size()), wa(n2), wb(n2);
+    (void)hipMemcpy(va.data(), ea, u.size() * 8, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(vb.data(), sa, u.size() * 8, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(wa.data(), eb, n2 * 8, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(wb.data(), sb, n2 * 8, hipMemcpyDeviceToHost);
+    const bool ens_same = va == vb && wa == wb;
+    ok = ok && tuned_same && ens_same;
+    std::printf("%s rhs_rel=%.3e step_rel=%.3e lambda=%.17g/%.17g shapes=%d/%d tuned_same=%d ensemble_same=%d\n",
+                ok ? "OK" : "FAIL", r1, r2, lam, lam_ref, shapes.rhs, shapes.visc, (int)tuned_same, (int)ens_same);
+    (void)hipFree(ea);
+    (void)hipFree(eb);
+    (void)hipFree(sa);
+    (void)hipFree(sb);
     orc_destroy(orc);
     (void)hipFree(du);
     (void)hipFree(dr);
