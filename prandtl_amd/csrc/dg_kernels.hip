@@ -94,6 +94,7 @@ struct VArgs {
 };
 
 constexpr int cmx(int a, int b) { return a > b ? a : b; }
+constexpr int pow2_at_least(int x) { return x <= 1 ? 1 : 2 * pow2_at_least((x + 1) / 2); }
 constexpr int r64(int x) { return ((x + 63) / 64) * 64; }
 
 // Launch shapes of both kernels: 0 natural, 1..3 forced blocks of 64, 128
@@ -259,10 +260,13 @@ template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
     }
     static constexpr int LC = lc_pick();
     static constexpr int FXN = LC * (NPAIR + NFV);
-    // one- and two-wave blocks: cap registers at 128 so 4 waves fit a SIMD
-    static constexpr int WAVES = K > 256 ? 2 : BLOCK <= 128 ? 4 : PRANDTL_WAVES;
+    // Registers for 4 waves per SIMD (128) where a workgroup has at most two
+    // waves or more than 256 nodes: there, two workgroups per CU beat the
+    // spills the cap causes (3-D N=7 with blending: 3.4 ms per launch at
+    // 128 registers against 4.3 ms at 180).  Otherwise PRANDTL_WAVES.
+    static constexpr int WAVES = (K > 256 || BLOCK <= 128) ? 4 : PRANDTL_WAVES;
     static_assert(EPB * K <= BLOCK, "one node per thread");
-    static_assert(!BLEND || 2 * K <= NV * FXN, "indicator scratch fits the flux buffer");
+    static_assert(!BLEND || 4 * K <= NV * (JF > FXN ? JF : FXN), "indicator scratch fits the flux buffer");
 };
 
 // LDS of one k_rhs tile (EPB elements).
@@ -417,26 +421,40 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             __syncthreads();
             double *tmp = src; src = dst; dst = tmp;
         }
-        if (act && n == 0) {
-            // energies in the reference's mode order n = (dx*Np + dy)*Np + dz
-            // (ModalBasis.cpp:72-104), masks of PerssonPeraireIndicator.cpp:38-52
-            double mm = 0.0, m1 = 0.0, m2 = 0.0;
-            for (int nm = 0; nm < K; nm++) {
-                int dg[3];
-                if (DIM == 2) { dg[0] = nm / NP; dg[1] = nm % NP; dg[2] = 0; }
-                else { dg[0] = nm / (NP * NP); dg[1] = (nm / NP) % NP; dg[2] = nm % NP; }
-                const int pos = DIM == 2 ? dg[1] * NP + dg[0] : (dg[2] * NP + dg[1]) * NP + dg[0];
-                const double md = src[el * K + pos];
-                bool in1 = true, in2 = true;
+        // modal energies: all modes, modes of degree <= N-1 and <= N-2 in
+        // every direction (masks of PerssonPeraireIndicator.cpp:38-52).  Node
+        // n holds the mode of degrees ii[] (mode order of ModalBasis.cpp:72-104
+        // maps to this position); the three sums are a binary tree over the
+        // element's K entries, the same for every launch shape.
+        double *sE0 = dst, *sE1 = sX + 2 * EPB * K, *sE2 = sX + 3 * EPB * K;
+        if (act) {
+            const double md = src[el * K + n];
+            bool in1 = true, in2 = true;
 #pragma unroll
-                for (int d = 0; d < DIM; d++) {
-                    if (dg[d] > NP - 3) in2 = false;
-                    if (dg[d] > NP - 2) in1 = false;
-                }
-                mm += md * md;
-                if (in1) m1 += md * md;
-                if (in2) m2 += md * md;
+            for (int d = 0; d < DIM; d++) {
+                if (ii[d] > NP - 3) in2 = false;
+                if (ii[d] > NP - 2) in1 = false;
             }
+            const double e2 = md * md;
+            sE0[el * K + n] = e2;
+            sE1[el * K + n] = in1 ? e2 : 0.0;
+            sE2[el * K + n] = in2 ? e2 : 0.0;
+        }
+        __syncthreads();
+        constexpr int HALF = pow2_at_least(K) / 2;
+#pragma unroll
+        for (int h = HALF; h >= 1; h >>= 1) {
+            if (act && n < h && n + h < K) {
+                const int i0 = el * K + n;
+                sE0[i0] += sE0[i0 + h];
+                sE1[i0] += sE1[i0 + h];
+                sE2[i0] += sE2[i0 + h];
+            }
+            __syncthreads();
+        }
+        double mm = 0.0, m1 = 0.0, m2 = 0.0;
+        if (act && n == 0) { mm = sE0[el * K]; m1 = sE1[el * K]; m2 = sE2[el * K]; }
+        if (act && n == 0) {
             double eta = 1.0 - m1 / mm;
             const double eta2 = 1.0 - m2 / m1;
             eta = (eta < eta2) ? eta2 : eta;

@@ -266,7 +266,9 @@ class TimeLoop:
         lam = self.op.GetMaxCharSpeed()
         return self.cfl * self.op.hmin / (lam * (self.op.order + 1) ** 2)
 
-    def run(self, u, t=0.0, max_steps=None):
+    def run(self, u, t=0.0, max_steps=None, callback=None):
+        """Advance u to t_final; callback(t, steps, done), if given, runs after
+        every step."""
         import torch
         if self.variable_dt and self.cfl:
             z = torch.empty_like(u)
@@ -281,5 +283,7 @@ class TimeLoop:
             steps += 1
             done = t >= self.t_final - 1e-8 * self.dt
             if max_steps is not None and steps >= max_steps:
-                break
+                done = True
+            if callback is not None:
+                callback(t, steps, done)
         return t, steps
