@@ -344,8 +344,13 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     if (act)
 #pragma unroll
         for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)e * K + n];
+    // Blended kernels carry the subcell-flux work and its accumulator; they
+    // do not hold loads in flight across phases (the surface data is loaded
+    // in the surface phase, the assembly inputs at the assembly), which keeps
+    // their register count down.
+    constexpr bool PREFETCH = !BLEND;
     double pn[NV], pfo[NV], pfn[NV];
-    {
+    if constexpr (PREFETCH) {
         const int jf = t;
         const int ej = jf / JF;
         const int eo = ebase + ej;
@@ -481,8 +486,8 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         const int f = rem / NFP, l = rem - f * NFP;
         const int r = f >> 1, side = f & 1;
         const double nr = A.geo.nrm[r];
-        if (jf != t) {
-            // more surface points than threads: load this one now
+        if (!PREFETCH || jf != t) {
+            // not prefetched, or more surface points than threads: load now
             const int en = neighbor(A.geo, eo, r, side);
             const int nn = line_node<DIM, NP>(r, l, side ? 0 : NP - 1);
 #pragma unroll
@@ -550,7 +555,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
 
     for (int r = 0; r < DIM; r++) {
         const double nr = A.geo.nrm[r];
-        if (r == DIM - 1 && act) {
+        if (PREFETCH && r == DIM - 1 && act) {
             // inputs of the final assembly, in flight during the last direction
 #pragma unroll
             for (int v = 0; v < NV; v++) {
@@ -654,6 +659,14 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     }
 
     // ---- assemble, stage update, store ----
+    if (!PREFETCH && act) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            u0v[v] = (A.mode == 2) ? A.u0[v * NK + (size_t)e * K + n] : 0.0;
+            vis[v] = (A.parabolic && v > 0)
+                         ? (double)static_cast<const VT *>(A.visc)[(v - 1) * NK + (size_t)e * K + n] : 0.0;
+        }
+    }
     if (act) {
 #pragma unroll
         for (int v = 0; v < NV; v++) {
