@@ -213,6 +213,44 @@ def measured_traffic(workload, kernel):
     return None
 
 
+def mixed_comparison(name, cfg, u_host, dt, steps, warmup, autotune, dev):
+    """The mixed form on the same workload, in the same process: its
+    throughput, and its RHS and kinetic-energy rate against the plain FP64
+    GPU RHS at the initial state (the FP64 RHS matches the oracle to 1e-12
+    relative max-norm, tests/test_gpu_parity.py)."""
+    import numpy as np
+    import torch
+    from prandtl_amd import DGSEMOperator, cases
+    opd = DGSEMOperator(precision="fp64", quirks=False, **cfg)
+    opm = DGSEMOperator(precision="mixed", quirks=False, **cfg)
+    u0 = torch.tensor(u_host.reshape(-1), dtype=torch.float64, device=dev)
+    zd, zm = torch.empty_like(u0), torch.empty_like(u0)
+    opd.Mult(u0, zd)
+    opm.Mult(u0, zm)
+    torch.cuda.synchronize()
+    rd, rm = zd.cpu().numpy(), zm.cpu().numpy()
+    WJ = cases.quadrature_weights(cfg["dim"], cfg["order"], cfg["nelem"], cfg["lo"], cfg["hi"])
+    vol = float(np.prod([cfg["hi"][r] - cfg["lo"][r] for r in range(cfg["dim"])]))
+    ek_d = cases.kinetic_energy_rate(u_host, rd.reshape(u_host.shape), WJ, vol)
+    ek_m = cases.kinetic_energy_rate(u_host, rm.reshape(u_host.shape), WJ, vol)
+    shapes = opm.autotune(u0)[0] if autotune else None
+    u = u0.clone()
+    for _ in range(warmup):
+        opm.Step(u, dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        opm.Step(u, dt)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nodes = opm.NE * opm.K
+    return {"value": nodes * 3 * steps / el, "ms_per_step": el / steps * 1e3, "launch_shapes": shapes,
+            "state_ok": bool(opm.check_state(u)),
+            "rhs_rel_l2_vs_fp64": float(np.linalg.norm(rm - rd) / np.linalg.norm(rd)),
+            "dEk_dt_fp64": ek_d, "dEk_dt_mixed": ek_m,
+            "dEk_dt_rel_diff": abs(ek_m - ek_d) / abs(ek_d)}
+
+
 def roofline(flops_launch, bytes_launch, t_launch, kernel, traffic=None):
     """Roofline entry of one kernel from its algorithmic counts per launch."""
     ai = flops_launch / bytes_launch
@@ -243,7 +281,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default launch shape of the RHS kernel")
+                    help="keep the default launch shapes of both kernels")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-form comparison of a plain FP64 Navier-Stokes run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -356,6 +396,10 @@ def main():
                    "kernel_ms": {"k_rhs": ms_rhs / max(n_rhs, 1), "k_visc": ms_visc / max(n_visc, 1)}},
         "roofline": roof,
     }
+    if args.precision == "fp64" and cfg.get("parabolic") and not quirks and not args.no_mixed:
+        m = mixed_comparison(name, cfg, u_host, dt, args.steps, args.warmup, not args.no_autotune, dev)
+        m["speedup_vs_fp64"] = m["value"] / value
+        line["config"]["mixed"] = m
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(name, cfg, quirks, u_host, args.cpu_seconds)
     print(json.dumps(line), flush=True)
