@@ -79,6 +79,20 @@ int prandtl_rhs(prandtl_ctx *ctx, const double *u_dev, double *dudt_dev, void *s
  * kernel; u is advanced in place. */
 int prandtl_rk3_step(prandtl_ctx *ctx, double *u_dev, double dt, void *stream);
 
+/* Simulation::Run's stepping (Simulation.cpp:893-907, 1007-1022) on the
+ * device: SSP-RK3 steps of u_dev from *t_io until t_final, each step
+ * clipped to t_final - t.  cfl > 0: variable dt = cfl*hmin/(lambda*(N+1)^2)
+ * from the running maximum wave speed, set after an initial RHS evaluation
+ * and after every step; otherwise the fixed step dt.  Stops when
+ * t >= t_final - 1e-8 dt, or after max_steps steps if max_steps > 0.  No host
+ * synchronisation per step: batches of 16 steps run from one graph (on a
+ * non-null stream) and the host reads the loop state once per batch; steps
+ * after the end do nothing.  *t_io receives the final time, *steps_out
+ * (may be NULL) the number of steps.  Bitwise the same steps as the host
+ * loop of prandtl_rk3_step and prandtl_max_wave_speed. */
+int prandtl_run(prandtl_ctx *ctx, double *u_dev, double *t_io, double t_final, double cfl,
+                double dt, int max_steps, int *steps_out, void *stream);
+
 /* On a non-null stream the three stages of prandtl_rk3_step are captured
  * once as a hipGraph (keyed by u_dev and stream; dt is read from device
  * memory) and replayed on later steps.  enable = 0 launches the kernels

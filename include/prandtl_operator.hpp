@@ -6,6 +6,7 @@
 //     void Mult(const Vector &u, Vector &dudt) const     -> Mult(const double *u, double *dudt) const
 //     real_t GetMaxCharSpeed()                           -> GetMaxCharSpeed()
 //   mfem::RK3SSPSolver::Step(x, t, dt)                   -> RK3SSPSolver::Step(double *x, double &t, double &dt)
+//   Simulation::Run time loop (Simulation.cpp:1007-1022)  -> DGSEMOperator::Run (on the device)
 //   (no counterpart)                                     -> DGSEMOperator::Autotune / SetShape, Ensemble
 //
 // Pointers are device pointers in the reference's byNODES layout
@@ -64,6 +65,15 @@ class DGSEMOperator {
 
     // one SSP-RK3 step in place (the three stages fused into the RHS kernels)
     void Step(double *u, double dt) const { check(prandtl_rk3_step(ctx_, u, dt, stream_), ctx_); }
+
+    // Simulation::Run's time loop on the device (prandtl_run): u from t to
+    // t_final with variable dt from cfl (> 0) or the fixed dt; returns steps
+    int Run(double *u, double &t, double t_final, double cfl, double dt = 0.0, int max_steps = 0) const
+    {
+        int steps = 0;
+        check(prandtl_run(ctx_, u, &t, t_final, cfl, dt, max_steps, &steps, stream_), ctx_);
+        return steps;
+    }
 
     // DGSEMOperator::GetMaxCharSpeed: running maximum of |v| + c
     double GetMaxCharSpeed(bool reset = false) const

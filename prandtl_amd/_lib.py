@@ -23,7 +23,7 @@ PRANDTL_KERNEL_VISC = 1
 
 EXPORTED = (
     "prandtl_create", "prandtl_destroy", "prandtl_sizes", "prandtl_rhs", "prandtl_rk3_step",
-    "prandtl_set_graph", "prandtl_set_shape", "prandtl_autotune",
+    "prandtl_set_graph", "prandtl_set_shape", "prandtl_autotune", "prandtl_run",
     "prandtl_max_wave_speed", "prandtl_alpha", "prandtl_check_state", "prandtl_lgl",
     "prandtl_debug_gradients", "prandtl_profile_begin", "prandtl_profile_end", "prandtl_last_error",
     "prandtl_ensemble_create", "prandtl_ensemble_destroy", "prandtl_ensemble_case",
@@ -74,6 +74,7 @@ def lib():
     L.prandtl_set_graph.argtypes = [vp, ctypes.c_int]
     L.prandtl_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.prandtl_autotune.argtypes = [vp, vp, ip, dp, vp]
+    L.prandtl_run.argtypes = [vp, vp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ip, vp]
     L.prandtl_max_wave_speed.argtypes = [vp, dp, ctypes.c_int, vp]
     L.prandtl_alpha.argtypes = [vp, dp, dp, vp]
     L.prandtl_check_state.argtypes = [vp, vp, vp]

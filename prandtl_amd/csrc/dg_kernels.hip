@@ -25,6 +25,11 @@
 // order.  Face jobs are computed by both neighbouring elements ("owner
 // computes") with the face's canonical orientation, so each face point's flux
 // is bitwise the same on both sides and no atomics or scatter are needed.
+//
+// Both kernels are also built for forced workgroup sizes (64, 128 or 256
+// threads, "launch shapes"); prandtl_autotune picks one per context by
+// timing them on the caller's state.  k_ens runs k_rhs's tile body over the
+// element tiles of many independent 2-D cases in one launch per RK stage.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -80,6 +85,7 @@ struct KArgs {
     double *alpha;        // [NE]
     double *eta;          // [NE]
     double *lam;          // per-block running maxima [gridDim.x]
+    const int *skip;      // if set and nonzero, the launch does nothing (a finished device loop)
 };
 
 struct VArgs {
@@ -91,6 +97,7 @@ struct VArgs {
     void *visc;
     void *fvf;
     double *grad;         // optional [dim][nv][NE][K]
+    const int *skip;      // as KArgs::skip
 };
 
 constexpr int cmx(int a, int b) { return a > b ? a : b; }
@@ -316,6 +323,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     auto &sAl = S.Al;
     auto &sRed = S.Red;
 
+    if (A.skip && *A.skip) return;   // uniform: before any barrier
     const int t = threadIdx.x;
     for (int i = t; i < NP * NP; i += BLOCK) {
         sDh2[i] = A.tab.Dh2[(i / NP) * MAXNP + i % NP];
@@ -770,6 +778,7 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
 {
     using C = VCfg<DIM, NP, FB>;
     using R = typename std::conditional<MIXED, float, double>::type;
+    if (A.skip && *A.skip) return;   // uniform: before any barrier
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, EPB = C::EPB, BLOCK = C::BLOCK;
     // phase 1 (lifting) needs the entropy variables and neighbour traces,
     // phase 2 (viscous divergence) the contravariant fluxes: one LDS region
@@ -980,6 +989,71 @@ __global__ void k_max_reduce(const double *slots, int n, double *out)
 __global__ void k_set_double(double *p, double v) { *p = v; }
 __global__ void k_reset_int(int *p) { *p = 0; }
 
+// State of a time loop run on the device (Simulation::Run's stepping,
+// Simulation.cpp:893-907, 1007-1022): time, step size, the clipped step of
+// the current step, done flag and step count.
+struct LoopState {
+    double t, dt, dt_real;
+    int done, steps;
+};
+
+// maximum of slots[0..n) over a block of LOOP_BLOCK threads (every thread
+// gets it)
+constexpr int LOOP_BLOCK = 1024;
+__device__ __forceinline__ double block_max_slots(const double *slots, int n)
+{
+    __shared__ double red[LOOP_BLOCK / 64];
+    double m = 0.0;
+#pragma unroll 4
+    for (int i = threadIdx.x; i < n; i += LOOP_BLOCK) m = (m < slots[i]) ? slots[i] : m;
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    double r = red[0];
+    for (int i = 1; i < LOOP_BLOCK / 64; i++) r = (r < red[i]) ? red[i] : r;
+    return r;
+}
+
+// the step actually taken: min(dt, t_final - t)
+__device__ __forceinline__ double clipped(double dt, double t, double t_final)
+{
+    const double r = t_final - t;
+    return (dt < r) ? dt : r;
+}
+
+// first step of a variable-dt loop: dt = (cfl * hmin) / (lambda * (N+1)^2)
+// from the running maximum wave speed, and its clipped step
+__global__ void __launch_bounds__(LOOP_BLOCK) k_loop_dt(LoopState *L, const double *slots, int n, double cfl_hmin, double np1sq,
+                          double t_final)
+{
+    const double lam = block_max_slots(slots, n);
+    if (threadIdx.x == 0) {
+        L->dt = cfl_hmin / (lam * np1sq);
+        L->dt_real = clipped(L->dt, L->t, t_final);
+    }
+}
+
+// after a step: t += dt_real; variable dt: dt from the running maximum;
+// done when t >= t_final - 1e-8 dt or max_steps (if > 0) steps were taken;
+// otherwise the next step's clipped dt
+__global__ void __launch_bounds__(LOOP_BLOCK) k_loop_end(LoopState *L, const double *slots, int n, double cfl_hmin, double np1sq,
+                           double t_final, int variable, int max_steps)
+{
+    if (L->done) return;   // uniform
+    const double lam = variable ? block_max_slots(slots, n) : 0.0;
+    if (threadIdx.x == 0) {
+        const double t = L->t + L->dt_real;
+        const int steps = L->steps + 1;
+        const double dt = variable ? cfl_hmin / (lam * np1sq) : L->dt;
+        const int done = (t >= t_final - 1e-8 * dt) || (max_steps > 0 && steps >= max_steps);
+        L->t = t;
+        L->steps = steps;
+        L->dt = dt;
+        if (!done) L->dt_real = clipped(dt, t, t_final);
+        L->done = done;
+    }
+}
+
 template <int DIM>
 __global__ void k_check(const double *u, size_t NK, double gm1, int *flag)
 {
@@ -1079,6 +1153,7 @@ using namespace prandtl;
 
 struct prandtl_ctx {
     int dim, order, Np, K, NE, nv, Nfp;
+    prandtl_mesh_desc mesh;
     prandtl_params prm;
     Tables tab;
     Geo geo;
@@ -1106,6 +1181,9 @@ struct prandtl_ctx {
     bool graph_dt_set = false;
     int rhs_shape = 0;            // k_rhs launch shape (only shapes rhs_shape_ok accepts are set)
     int visc_shape = 0;           // k_visc launch shape (visc_shape_ok)
+    // device time loop (prandtl_run)
+    LoopState *d_loop = nullptr;
+    const int *skip = nullptr;    // set while launching the loop's stages
 };
 
 namespace {
@@ -1165,6 +1243,7 @@ template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, doub
     a.tab = c->tab; a.geo = c->geo; a.phys = c->phys;
     a.quirks = c->prm.quirks;
     a.y = y; a.visc = c->d_visc; a.fvf = c->d_fvf; a.grad = grad;
+    a.skip = c->skip;
     ProfScope ps(c, s, 1);
     switch (c->visc_shape) {
     case 1:
@@ -1234,6 +1313,7 @@ KArgs make_kargs(const prandtl_ctx *c, const double *y, const double *u0, double
     a.y = y; a.u0 = u0; a.out = out;
     a.visc = c->d_visc; a.fvf = c->d_fvf; a.alpha = c->d_alpha; a.eta = c->d_eta;
     a.lam = c->d_lam;
+    a.skip = c->skip;
     return a;
 }
 
@@ -1343,6 +1423,7 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     prandtl_ctx *c = new (std::nothrow) prandtl_ctx();
     if (!c) return fail(nullptr, PRANDTL_ERR_NOMEM, "host allocation");
     c->dim = dim; c->order = N; c->Np = N + 1; c->nv = dim + 2;
+    c->mesh = *mesh;
     c->K = dim == 2 ? c->Np * c->Np : c->Np * c->Np * c->Np;
     c->Nfp = dim == 2 ? c->Np : c->Np * c->Np;
     c->NE = (int)NE;
@@ -1469,6 +1550,7 @@ void prandtl_destroy(prandtl_ctx *c)
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (c->d_lam) (void)hipFree(c->d_lam);
+    if (c->d_loop) (void)hipFree(c->d_loop);
     if (c->d_lam_out) (void)hipFree(c->d_lam_out);
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_dt) (void)hipFree(c->d_dt);
@@ -1594,6 +1676,80 @@ int prandtl_autotune(prandtl_ctx *c, const double *u, int *shapes_out, double *u
     }
     if (c->rhs_shape != prev[0] || c->visc_shape != prev[1]) drop_graph(c);
     if (shapes_out) { shapes_out[0] = c->rhs_shape; shapes_out[1] = c->visc_shape; }
+    return PRANDTL_OK;
+}
+
+int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double cfl, double dt, int max_steps,
+                int *steps_out, void *stream)
+{
+    if (!c || !u || !t_io) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    const bool variable = cfl > 0.0;
+    if (!variable && !(dt > 0.0)) return fail(c, PRANDTL_ERR_ARG, "need cfl > 0 or dt > 0");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = PRANDTL_OK;
+    if (!c->d_loop) rc = hip_check(c, hipMalloc(&c->d_loop, sizeof(LoopState)), "loop state");
+    if (rc) return rc;
+    // hmin and (N+1)^2 as the host loop forms them
+    double hmin = (c->mesh.hi[0] - c->mesh.lo[0]) / c->mesh.nelem[0];
+    for (int r = 1; r < c->dim; r++) {
+        const double h = (c->mesh.hi[r] - c->mesh.lo[r]) / c->mesh.nelem[r];
+        hmin = h < hmin ? h : hmin;
+    }
+    const double cfl_hmin = cfl * hmin, np1sq = (double)(c->Np * c->Np);
+    LoopState L0 = {*t_io, variable ? 0.0 : dt, 0.0, 0, 0};
+    if (!variable) L0.dt_real = (dt < t_final - *t_io) ? dt : t_final - *t_io;
+    rc = hip_check(c, hipMemcpyAsync(c->d_loop, &L0, sizeof(L0), hipMemcpyHostToDevice, s), "loop init");
+    if (!rc && variable) {
+        // the initial RHS evaluation that seeds the wave speed (Simulation.cpp:903)
+        rc = stage(c, u, nullptr, c->d_yb, 0, 0.0, 0.0, 0.0, s);
+        if (!rc) {
+            hipLaunchKernelGGL(k_loop_dt, dim3(1), dim3(LOOP_BLOCK), 0, s, c->d_loop, c->d_lam, c->NE, cfl_hmin, np1sq,
+                               t_final);
+            rc = hip_check(c, hipGetLastError(), "loop dt");
+        }
+    }
+    if (rc) return rc;
+    // a batch of steps; steps after the loop is done do nothing
+    constexpr int BATCH = 16;
+    auto batch = [&]() -> int {
+        c->skip = &c->d_loop->done;
+        c->dtp = &c->d_loop->dt_real;
+        int r2 = PRANDTL_OK;
+        for (int k = 0; k < BATCH && !r2; k++) {
+            r2 = rk3_stages(c, u, 0.0, s);
+            if (!r2) {
+                hipLaunchKernelGGL(k_loop_end, dim3(1), dim3(LOOP_BLOCK), 0, s, c->d_loop, c->d_lam, c->NE, cfl_hmin,
+                                   np1sq, t_final, variable ? 1 : 0, max_steps);
+                r2 = hip_check(c, hipGetLastError(), "loop end");
+            }
+        }
+        c->skip = nullptr;
+        c->dtp = nullptr;
+        return r2;
+    };
+    const bool use_graph = c->use_graph && !c->prof && s != nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (use_graph) {
+        rc = hip_check(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        if (rc) return rc;
+        const int rb = batch();
+        rc = hip_check(c, hipStreamEndCapture(s, &g), "end capture");
+        if (rb) { if (g) (void)hipGraphDestroy(g); return rb; }
+        if (!rc) rc = hip_check(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "graph instantiate");
+    }
+    LoopState L = L0;
+    while (!rc) {
+        rc = use_graph ? hip_check(c, hipGraphLaunch(ge, s), "graph launch") : batch();
+        if (!rc) rc = hip_check(c, hipMemcpyAsync(&L, c->d_loop, sizeof(L), hipMemcpyDeviceToHost, s), "loop state");
+        if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+        if (rc || L.done) break;
+    }
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    *t_io = L.t;
+    if (steps_out) *steps_out = L.steps;
     return PRANDTL_OK;
 }
 

@@ -123,6 +123,16 @@ class DGSEMOperator:
         """u <- one SSP-RK3 step of size dt, in place."""
         check(lib().prandtl_rk3_step(self._h, self._vec(u, "u"), float(dt), _stream_ptr()), self._h)
 
+    def Run(self, u, t, t_final, cfl=None, dt=None, max_steps=0):
+        """The time loop on the device (prandtl_run): steps u from t to
+        t_final with variable dt from cfl, or fixed dt; returns (t, steps)."""
+        tt = ctypes.c_double(float(t))
+        steps = ctypes.c_int()
+        check(lib().prandtl_run(self._h, self._vec(u, "u"), ctypes.byref(tt), float(t_final),
+                                float(cfl) if cfl else 0.0, float(dt) if dt else 0.0, int(max_steps),
+                                ctypes.byref(steps), _stream_ptr()), self._h)
+        return tt.value, steps.value
+
     def set_graph(self, enable):
         """Capture/replay the RK3 step as a hipGraph on non-default streams."""
         check(lib().prandtl_set_graph(self._h, int(enable)), self._h)

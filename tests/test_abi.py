@@ -123,3 +123,9 @@ def test_ensemble_null_arguments():
     assert L.prandtl_ensemble_rk3_step(None, None, None, None) == _lib.PRANDTL_ERR_ARG
     assert L.prandtl_ensemble_case(None, 0, ctypes.byref(h)) == _lib.PRANDTL_ERR_ARG
     L.prandtl_ensemble_destroy(None)
+
+
+def test_run_null_arguments():
+    L = lib()
+    t = ctypes.c_double(0.0)
+    assert L.prandtl_run(None, None, ctypes.byref(t), 1.0, 0.5, 0.0, 0, None, None) == _lib.PRANDTL_ERR_ARG

@@ -1,0 +1,59 @@
+"""The time loop on the device (prandtl_run) against the host loop of
+single steps (TimeLoop over prandtl_rk3_step and prandtl_max_wave_speed).
+
+Both form dt from the same running maximum wave speed with the same
+arithmetic, so the bar is the same step count and time, and bitwise equal
+states."""
+import math
+
+import numpy as np
+import pytest
+
+from prandtl_amd import DGSEMOperator, cases
+from prandtl_amd.operator import TimeLoop
+
+pytestmark = pytest.mark.gpu
+
+IV = dict(dim=2, order=3, nelem=(16, 16), lo=(-5.0, -5.0), hi=(5.0, 5.0))
+TGV = dict(dim=3, order=3, nelem=(4, 4, 4), lo=(0.0,) * 3, hi=(2 * math.pi,) * 3,
+           parabolic=True, mu=1.0 / 1600.0, Pr=0.71, quirks=False)
+
+
+def state(cfg, dev):
+    import torch
+    X = cases.node_coords(cfg["dim"], cfg["order"], cfg["nelem"], cfg["lo"], cfg["hi"])
+    u = cases.isentropic_vortex(X) if cfg["dim"] == 2 else cases.taylor_green(X)
+    return torch.tensor(u.reshape(-1), dtype=torch.float64, device=dev)
+
+
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+def test_device_loop_variable_dt_matches_host_loop(cuda, graph):
+    import torch
+    s = torch.cuda.Stream(device=cuda) if graph else torch.cuda.default_stream(cuda)
+    with torch.cuda.stream(s):
+        a, b = DGSEMOperator(**IV), DGSEMOperator(**IV)
+        ua, ub = state(IV, cuda), state(IV, cuda)
+        t_host, n_host = TimeLoop(a, 1.0, cfl=0.5, variable_dt=True).run(ua)
+        t_dev, n_dev = b.Run(ub, 0.0, 1.0, cfl=0.5)
+    torch.cuda.synchronize()
+    print("host %d steps t=%.17g, device %d steps t=%.17g" % (n_host, t_host, n_dev, t_dev), flush=True)
+    assert n_dev == n_host and t_dev == t_host
+    assert torch.equal(ua, ub)
+    assert b.GetMaxCharSpeed() == a.GetMaxCharSpeed()
+
+
+def test_device_loop_fixed_dt_and_max_steps(cuda):
+    import torch
+    s = torch.cuda.Stream(device=cuda)
+    with torch.cuda.stream(s):
+        a, b = DGSEMOperator(**TGV), DGSEMOperator(**TGV)
+        ua, ub = state(TGV, cuda), state(TGV, cuda)
+        dt = 1e-3
+        t_host, n_host = TimeLoop(a, 0.05, dt=dt).run(ua)
+        t_dev, n_dev = b.Run(ub, 0.0, 0.05, dt=dt)
+        uc = state(TGV, cuda)
+        t5, n5 = DGSEMOperator(**TGV).Run(uc, 0.0, 1.0, dt=dt, max_steps=5)
+    torch.cuda.synchronize()
+    assert n_dev == n_host == 50 and t_dev == t_host
+    assert torch.equal(ua, ub)
+    assert n5 == 5 and t5 == sum([dt] * 5)
