@@ -89,9 +89,12 @@ int prandtl_rk3_step(prandtl_ctx *ctx, double *u_dev, double dt, void *stream);
  * non-null stream) and the host reads the loop state once per batch; steps
  * after the end do nothing.  *t_io receives the final time, *steps_out
  * (may be NULL) the number of steps.  Bitwise the same steps as the host
- * loop of prandtl_rk3_step and prandtl_max_wave_speed. */
+ * loop of prandtl_rk3_step and prandtl_max_wave_speed.  check_every > 0:
+ * after every step whose count is a multiple of it, the NaN/Inf check of
+ * Simulation.cpp:1025-1035 (extended to rho <= 0 and p <= 0) stops the loop
+ * with PRANDTL_ERR_STATE; *t_io and *steps_out then give where it stopped. */
 int prandtl_run(prandtl_ctx *ctx, double *u_dev, double *t_io, double t_final, double cfl,
-                double dt, int max_steps, int *steps_out, void *stream);
+                double dt, int max_steps, int check_every, int *steps_out, void *stream);
 
 /* On a non-null stream the three stages of prandtl_rk3_step are captured
  * once as a hipGraph (keyed by u_dev and stream; dt is read from device

@@ -68,10 +68,11 @@ class DGSEMOperator {
 
     // Simulation::Run's time loop on the device (prandtl_run): u from t to
     // t_final with variable dt from cfl (> 0) or the fixed dt; returns steps
-    int Run(double *u, double &t, double t_final, double cfl, double dt = 0.0, int max_steps = 0) const
+    int Run(double *u, double &t, double t_final, double cfl, double dt = 0.0, int max_steps = 0,
+            int check_every = 0) const
     {
         int steps = 0;
-        check(prandtl_run(ctx_, u, &t, t_final, cfl, dt, max_steps, &steps, stream_), ctx_);
+        check(prandtl_run(ctx_, u, &t, t_final, cfl, dt, max_steps, check_every, &steps, stream_), ctx_);
         return steps;
     }
 

@@ -74,7 +74,8 @@ def lib():
     L.prandtl_set_graph.argtypes = [vp, ctypes.c_int]
     L.prandtl_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.prandtl_autotune.argtypes = [vp, vp, ip, dp, vp]
-    L.prandtl_run.argtypes = [vp, vp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ip, vp]
+    L.prandtl_run.argtypes = [vp, vp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                              ctypes.c_int, ip, vp]
     L.prandtl_max_wave_speed.argtypes = [vp, dp, ctypes.c_int, vp]
     L.prandtl_alpha.argtypes = [vp, dp, dp, vp]
     L.prandtl_check_state.argtypes = [vp, vp, vp]

@@ -994,7 +994,7 @@ __global__ void k_reset_int(int *p) { *p = 0; }
 // the current step, done flag and step count.
 struct LoopState {
     double t, dt, dt_real;
-    int done, steps;
+    int done, steps, bad;
 };
 
 // maximum of slots[0..n) over a block of LOOP_BLOCK threads (every thread
@@ -1070,6 +1070,32 @@ __global__ void k_check(const double *u, size_t NK, double gm1, int *flag)
     const double p = gm1 * (s[DIM + 1] - 0.5 * s[0] * vv);
     bad |= !(s[0] > 0.0) || !(p > 0.0);
     if (bad) atomicOr(flag, 1);
+}
+
+// The NaN/Inf check of Simulation.cpp:1025-1035 inside a device loop, after
+// every step whose count is a multiple of `every`: a non-finite or
+// non-positive density or pressure stops the loop (bad = done = 1) before
+// the next step.  Every writer stores the same values.
+template <int DIM>
+__global__ void k_loop_check(LoopState *L, const double *u, size_t NK, double gm1, int every)
+{
+    if (L->done || L->steps % every) return;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NK) return;
+    double s[DIM + 2];
+    bool bad = false;
+    for (int v = 0; v < DIM + 2; v++) {
+        s[v] = u[v * NK + i];
+        bad |= !isfinite(s[v]);
+    }
+    double vv = 0.0;
+    for (int d = 0; d < DIM; d++) vv += (s[1 + d] / s[0]) * (s[1 + d] / s[0]);
+    const double p = gm1 * (s[DIM + 1] - 0.5 * s[0] * vv);
+    bad |= !(s[0] > 0.0) || !(p > 0.0);
+    if (bad) {
+        L->bad = 1;
+        L->done = 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1680,7 +1706,7 @@ int prandtl_autotune(prandtl_ctx *c, const double *u, int *shapes_out, double *u
 }
 
 int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double cfl, double dt, int max_steps,
-                int *steps_out, void *stream)
+                int check_every, int *steps_out, void *stream)
 {
     if (!c || !u || !t_io) return fail(c, PRANDTL_ERR_ARG, "null argument");
     const bool variable = cfl > 0.0;
@@ -1696,7 +1722,7 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
         hmin = h < hmin ? h : hmin;
     }
     const double cfl_hmin = cfl * hmin, np1sq = (double)(c->Np * c->Np);
-    LoopState L0 = {*t_io, variable ? 0.0 : dt, 0.0, 0, 0};
+    LoopState L0 = {*t_io, variable ? 0.0 : dt, 0.0, 0, 0, 0};
     if (!variable) L0.dt_real = (dt < t_final - *t_io) ? dt : t_final - *t_io;
     rc = hip_check(c, hipMemcpyAsync(c->d_loop, &L0, sizeof(L0), hipMemcpyHostToDevice, s), "loop init");
     if (!rc && variable) {
@@ -1721,6 +1747,15 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
                 hipLaunchKernelGGL(k_loop_end, dim3(1), dim3(LOOP_BLOCK), 0, s, c->d_loop, c->d_lam, c->NE, cfl_hmin,
                                    np1sq, t_final, variable ? 1 : 0, max_steps);
                 r2 = hip_check(c, hipGetLastError(), "loop end");
+            }
+            if (!r2 && check_every > 0) {
+                const size_t NK = (size_t)c->NE * c->K;
+                const int nb = (int)((NK + 255) / 256);
+                if (c->dim == 2)
+                    hipLaunchKernelGGL(k_loop_check<2>, dim3(nb), dim3(256), 0, s, c->d_loop, u, NK, c->phys.gm1, check_every);
+                else
+                    hipLaunchKernelGGL(k_loop_check<3>, dim3(nb), dim3(256), 0, s, c->d_loop, u, NK, c->phys.gm1, check_every);
+                r2 = hip_check(c, hipGetLastError(), "loop check");
             }
         }
         c->skip = nullptr;
@@ -1750,6 +1785,9 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
     if (rc) return rc;
     *t_io = L.t;
     if (steps_out) *steps_out = L.steps;
+    if (L.bad)
+        return fail(c, PRANDTL_ERR_STATE,
+                    "non-finite or non-positive density/pressure after step " + std::to_string(L.steps));
     return PRANDTL_OK;
 }
 

@@ -123,14 +123,22 @@ class DGSEMOperator:
         """u <- one SSP-RK3 step of size dt, in place."""
         check(lib().prandtl_rk3_step(self._h, self._vec(u, "u"), float(dt), _stream_ptr()), self._h)
 
-    def Run(self, u, t, t_final, cfl=None, dt=None, max_steps=0):
+    def Run(self, u, t, t_final, cfl=None, dt=None, max_steps=0, check_every=0):
         """The time loop on the device (prandtl_run): steps u from t to
-        t_final with variable dt from cfl, or fixed dt; returns (t, steps)."""
+        t_final with variable dt from cfl, or fixed dt; returns (t, steps).
+        check_every > 0: state check after every such number of steps; a bad
+        state raises PrandtlError (code PRANDTL_ERR_STATE) whose .t and
+        .steps say where the loop stopped."""
         tt = ctypes.c_double(float(t))
         steps = ctypes.c_int()
-        check(lib().prandtl_run(self._h, self._vec(u, "u"), ctypes.byref(tt), float(t_final),
-                                float(cfl) if cfl else 0.0, float(dt) if dt else 0.0, int(max_steps),
-                                ctypes.byref(steps), _stream_ptr()), self._h)
+        rc = lib().prandtl_run(self._h, self._vec(u, "u"), ctypes.byref(tt), float(t_final),
+                               float(cfl) if cfl else 0.0, float(dt) if dt else 0.0, int(max_steps),
+                               int(check_every), ctypes.byref(steps), _stream_ptr())
+        try:
+            check(rc, self._h)
+        except _lib.PrandtlError as e:
+            e.t, e.steps = tt.value, steps.value
+            raise
         return tt.value, steps.value
 
     def set_graph(self, enable):

@@ -128,4 +128,4 @@ def test_ensemble_null_arguments():
 def test_run_null_arguments():
     L = lib()
     t = ctypes.c_double(0.0)
-    assert L.prandtl_run(None, None, ctypes.byref(t), 1.0, 0.5, 0.0, 0, None, None) == _lib.PRANDTL_ERR_ARG
+    assert L.prandtl_run(None, None, ctypes.byref(t), 1.0, 0.5, 0.0, 0, 0, None, None) == _lib.PRANDTL_ERR_ARG

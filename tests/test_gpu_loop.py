@@ -57,3 +57,22 @@ def test_device_loop_fixed_dt_and_max_steps(cuda):
     assert n_dev == n_host == 50 and t_dev == t_host
     assert torch.equal(ua, ub)
     assert n5 == 5 and t5 == sum([dt] * 5)
+
+
+@pytest.mark.parametrize("every", [1, 5])
+def test_device_loop_state_check_stops(cuda, every):
+    """A state with one non-physical node (negative density, set on the
+    host) stops the loop at the first check: step `every`."""
+    import torch
+    from prandtl_amd import PrandtlError, _lib
+    op = DGSEMOperator(**IV)
+    u = state(IV, cuda)
+    u[7] = -1.0                       # density of one node
+    with pytest.raises(PrandtlError) as ei:
+        op.Run(u, 0.0, 1.0, dt=1e-3, check_every=every)
+    torch.cuda.synchronize()
+    assert ei.value.code == _lib.PRANDTL_ERR_STATE
+    assert ei.value.steps == every and ei.value.t == sum([1e-3] * every)
+    good = DGSEMOperator(**IV)
+    t, n = good.Run(state(IV, cuda), 0.0, 0.02, dt=1e-3, check_every=1)
+    assert n == 20
