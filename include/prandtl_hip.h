@@ -24,7 +24,8 @@ enum {
     PRANDTL_ERR_UNSUPPORTED = 2,  /* configuration not built */
     PRANDTL_ERR_HIP = 3,          /* HIP runtime error (see prandtl_last_error) */
     PRANDTL_ERR_STATE = 4,        /* non-finite or non-positive density/pressure */
-    PRANDTL_ERR_NOMEM = 5
+    PRANDTL_ERR_NOMEM = 5,
+    PRANDTL_ERR_IO = 6            /* checkpoint file missing, truncated, corrupt or for another case */
 };
 
 enum { PRANDTL_FP64 = 0, PRANDTL_MIXED = 1 };
@@ -158,6 +159,26 @@ int prandtl_profile_end(prandtl_ctx *ctx, double *ms, int *count);
 
 /* Last error message of this context (or of create when ctx is NULL). */
 const char *prandtl_last_error(const prandtl_ctx *ctx);
+
+/* ---- Checkpoints (Simulation.cpp:1084-1116 save, 350-388 load) ----
+ * The folder layout of the reference: dir/Cycle<cycle>/checkpoint_cycle_<cycle>.json
+ * holding {"time", "cycle"}, and .../checkpoint_cycle_<cycle>.00000000.chk.
+ * The .chk is this library's format, not MFEM's GridFunction text: a header
+ * (magic, version, dim, order, elements per direction, variables, value
+ * count), the state as raw FP64 in byNODES order, and a 64-bit FNV-1a
+ * checksum of the state bytes.  The time is written at full precision.
+ * Reading checks the header against the expected mesh and variables, the
+ * length and the checksum; any mismatch is PRANDTL_ERR_IO with the reason in
+ * prandtl_last_error.  _write/_read work on host arrays and need no GPU;
+ * _save/_load copy a context's device state (u_dev) through them. */
+int prandtl_checkpoint_write(const char *dir, int cycle, double t, const prandtl_mesh_desc *mesh, int nv,
+                             const double *u_host);
+int prandtl_checkpoint_read(const char *dir, int cycle, const prandtl_mesh_desc *mesh, int nv,
+                            double *u_host, double *t_out);
+int prandtl_checkpoint_save(prandtl_ctx *ctx, const double *u_dev, const char *dir, int cycle, double t,
+                            void *stream);
+int prandtl_checkpoint_load(prandtl_ctx *ctx, double *u_dev, const char *dir, int cycle, double *t_out,
+                            void *stream);
 
 /* ---- Batched ensemble (no counterpart in the reference, which runs one
  * case per process: Simulation::Run, Simulation.cpp:882-1181) ----

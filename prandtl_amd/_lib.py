@@ -15,6 +15,7 @@ PRANDTL_ERR_UNSUPPORTED = 2
 PRANDTL_ERR_HIP = 3
 PRANDTL_ERR_STATE = 4
 PRANDTL_ERR_NOMEM = 5
+PRANDTL_ERR_IO = 6
 PRANDTL_FP64 = 0
 PRANDTL_MIXED = 1
 PRANDTL_SHAPES = 4
@@ -28,6 +29,7 @@ EXPORTED = (
     "prandtl_debug_gradients", "prandtl_profile_begin", "prandtl_profile_end", "prandtl_last_error",
     "prandtl_ensemble_create", "prandtl_ensemble_destroy", "prandtl_ensemble_case",
     "prandtl_ensemble_rk3_step", "prandtl_ensemble_last_error",
+    "prandtl_checkpoint_write", "prandtl_checkpoint_read", "prandtl_checkpoint_save", "prandtl_checkpoint_load",
 )
 
 
@@ -85,6 +87,11 @@ def lib():
     L.prandtl_profile_end.argtypes = [vp, dp, ip]
     L.prandtl_last_error.argtypes = [vp]
     L.prandtl_last_error.restype = ctypes.c_char_p
+    cp = ctypes.c_char_p
+    L.prandtl_checkpoint_write.argtypes = [cp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(MeshDesc), ctypes.c_int, dp]
+    L.prandtl_checkpoint_read.argtypes = [cp, ctypes.c_int, ctypes.POINTER(MeshDesc), ctypes.c_int, dp, dp]
+    L.prandtl_checkpoint_save.argtypes = [vp, vp, cp, ctypes.c_int, ctypes.c_double, vp]
+    L.prandtl_checkpoint_load.argtypes = [vp, vp, cp, ctypes.c_int, dp, vp]
     L.prandtl_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(MeshDesc), ctypes.POINTER(Params),
                                           ctypes.POINTER(vp)]
     L.prandtl_ensemble_destroy.argtypes = [vp]

@@ -33,9 +33,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -1881,6 +1884,181 @@ int prandtl_profile_end(prandtl_ctx *c, double *ms, int *count)
 }
 
 const char *prandtl_last_error(const prandtl_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// checkpoints: the layout of Simulation.cpp:1084-1116 (Cycle<n>/ folder,
+// checkpoint_cycle_<n>.<rank8>.chk and a JSON with time and cycle), with the
+// state as raw FP64 (byNODES) behind a header that must match the mesh and
+// a 64-bit FNV-1a checksum at the end, and the time at full precision
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ChkHeader {
+    char magic[8];
+    int32_t version;
+    int32_t dim, order, nelem[3], nv;
+    int64_t count;
+};
+constexpr char CHK_MAGIC[8] = {'P', 'R', 'D', 'T', 'L', 'C', 'H', 'K'};
+constexpr int32_t CHK_VERSION = 1;
+
+uint64_t fnv1a64(const void *data, size_t n)
+{
+    const unsigned char *p = static_cast<const unsigned char *>(data);
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; i++) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::string chk_dir(const char *dir, int cycle) { return std::string(dir) + "/Cycle" + std::to_string(cycle); }
+std::string chk_state(const char *dir, int cycle)
+{
+    return chk_dir(dir, cycle) + "/checkpoint_cycle_" + std::to_string(cycle) + ".00000000.chk";
+}
+std::string chk_meta(const char *dir, int cycle)
+{
+    return chk_dir(dir, cycle) + "/checkpoint_cycle_" + std::to_string(cycle) + ".json";
+}
+
+// value of "key": <number> in a small JSON object
+bool json_number(const std::string &txt, const char *key, double &out)
+{
+    const std::string k = std::string("\"") + key + "\"";
+    const size_t at = txt.find(k);
+    if (at == std::string::npos) return false;
+    const size_t colon = txt.find(':', at + k.size());
+    if (colon == std::string::npos) return false;
+    const char *b = txt.c_str() + colon + 1;
+    char *e = nullptr;
+    out = std::strtod(b, &e);
+    return e != b;
+}
+
+int64_t mesh_count(const prandtl_mesh_desc *m, int nv)
+{
+    int64_t NE = 1, K = 1;
+    for (int r = 0; r < m->dim; r++) {
+        NE *= m->nelem[r];
+        K *= m->order + 1;
+    }
+    return NE * K * nv;
+}
+
+int io_fail(const std::string &msg)
+{
+    g_create_err = msg;
+    return PRANDTL_ERR_IO;
+}
+
+} // namespace
+
+extern "C" {
+
+int prandtl_checkpoint_write(const char *dir, int cycle, double t, const prandtl_mesh_desc *mesh, int nv,
+                             const double *u)
+{
+    if (!dir || !mesh || !u || cycle < 0 || nv < 1 || mesh->dim < 1 || mesh->dim > 3)
+        return fail(nullptr, PRANDTL_ERR_ARG, "bad arguments");
+    const int64_t count = mesh_count(mesh, nv);
+    std::error_code ec;
+    std::filesystem::create_directories(chk_dir(dir, cycle), ec);
+    if (ec) return io_fail("cannot create " + chk_dir(dir, cycle) + ": " + ec.message());
+    ChkHeader h{};
+    std::memcpy(h.magic, CHK_MAGIC, 8);
+    h.version = CHK_VERSION;
+    h.dim = mesh->dim;
+    h.order = mesh->order;
+    for (int r = 0; r < 3; r++) h.nelem[r] = r < mesh->dim ? mesh->nelem[r] : 1;
+    h.nv = nv;
+    h.count = count;
+    const uint64_t sum = fnv1a64(u, sizeof(double) * count);
+    const std::string path = chk_state(dir, cycle), tmp = path + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+        if (!f) return io_fail("cannot write " + tmp);
+        f.write(reinterpret_cast<const char *>(&h), sizeof(h));
+        f.write(reinterpret_cast<const char *>(u), sizeof(double) * count);
+        f.write(reinterpret_cast<const char *>(&sum), sizeof(sum));
+        if (!f) return io_fail("write failed: " + tmp);
+    }
+    std::filesystem::rename(tmp, path, ec);
+    if (ec) return io_fail("cannot rename " + tmp + ": " + ec.message());
+    std::ofstream m(chk_meta(dir, cycle), std::ios::trunc);
+    if (!m) return io_fail("cannot write " + chk_meta(dir, cycle));
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.17g", t);
+    m << "{\n \"time\": " << buf << ",\n \"cycle\": " << cycle << "\n}\n";
+    if (!m) return io_fail("write failed: " + chk_meta(dir, cycle));
+    return PRANDTL_OK;
+}
+
+int prandtl_checkpoint_read(const char *dir, int cycle, const prandtl_mesh_desc *mesh, int nv, double *u,
+                            double *t_out)
+{
+    if (!dir || !mesh || !u || !t_out || cycle < 0 || nv < 1 || mesh->dim < 1 || mesh->dim > 3)
+        return fail(nullptr, PRANDTL_ERR_ARG, "bad arguments");
+    std::ifstream m(chk_meta(dir, cycle));
+    if (!m) return io_fail("cannot open " + chk_meta(dir, cycle));
+    const std::string txt((std::istreambuf_iterator<char>(m)), std::istreambuf_iterator<char>());
+    double t = 0.0, cyc = -1.0;
+    if (!json_number(txt, "time", t) || !json_number(txt, "cycle", cyc))
+        return io_fail("no time or cycle in " + chk_meta(dir, cycle));
+    if (cyc != (double)cycle) return io_fail("cycle in " + chk_meta(dir, cycle) + " is not " + std::to_string(cycle));
+    const std::string path = chk_state(dir, cycle);
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return io_fail("cannot open " + path);
+    ChkHeader h{};
+    if (!f.read(reinterpret_cast<char *>(&h), sizeof(h))) return io_fail("truncated header: " + path);
+    if (std::memcmp(h.magic, CHK_MAGIC, 8) != 0 || h.version != CHK_VERSION) return io_fail("not a checkpoint: " + path);
+    const int64_t count = mesh_count(mesh, nv);
+    bool same = h.dim == mesh->dim && h.order == mesh->order && h.nv == nv && h.count == count;
+    for (int r = 0; r < mesh->dim; r++) same = same && h.nelem[r] == mesh->nelem[r];
+    if (!same) return io_fail("checkpoint mesh or variables differ from this case: " + path);
+    std::vector<double> buf((size_t)count);
+    if (!f.read(reinterpret_cast<char *>(buf.data()), sizeof(double) * count)) return io_fail("truncated state: " + path);
+    uint64_t sum = 0;
+    if (!f.read(reinterpret_cast<char *>(&sum), sizeof(sum))) return io_fail("truncated checksum: " + path);
+    if (f.peek() != std::char_traits<char>::eof()) return io_fail("trailing data: " + path);
+    if (sum != fnv1a64(buf.data(), sizeof(double) * count)) return io_fail("checksum mismatch: " + path);
+    std::memcpy(u, buf.data(), sizeof(double) * count);
+    *t_out = t;
+    return PRANDTL_OK;
+}
+
+int prandtl_checkpoint_save(prandtl_ctx *c, const double *u, const char *dir, int cycle, double t, void *stream)
+{
+    if (!c || !u || !dir) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    std::vector<double> h((size_t)c->nv * c->NE * c->K);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = hip_check(c, hipMemcpyAsync(h.data(), u, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s), "copy state");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    rc = prandtl_checkpoint_write(dir, cycle, t, &c->mesh, c->nv, h.data());
+    if (rc) c->err = g_create_err;
+    return rc;
+}
+
+int prandtl_checkpoint_load(prandtl_ctx *c, double *u, const char *dir, int cycle, double *t_out, void *stream)
+{
+    if (!c || !u || !dir || !t_out) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    std::vector<double> h((size_t)c->nv * c->NE * c->K);
+    double t = 0.0;
+    int rc = prandtl_checkpoint_read(dir, cycle, &c->mesh, c->nv, h.data(), &t);
+    if (rc) {
+        c->err = g_create_err;
+        return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    rc = hip_check(c, hipMemcpyAsync(u, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s), "copy state");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+    if (!rc) *t_out = t;
+    return rc;
+}
 
 } // extern "C"
 

@@ -51,6 +51,32 @@ def _descs(dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0, parabolic=Fals
 _DESC_KEYS = frozenset(inspect.signature(_descs).parameters)
 
 
+def checkpoint_write(directory, cycle, t, u, dim, order, nelem, lo, hi):
+    """Write a host state (numpy float64, byNODES) as checkpoint `cycle`
+    under `directory` (prandtl_checkpoint_write; no GPU needed)."""
+    m, _ = _descs(dim, order, nelem, lo, hi)
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+    dp = ctypes.POINTER(ctypes.c_double)
+    nv = dim + 2
+    check(lib().prandtl_checkpoint_write(str(directory).encode(), int(cycle), float(t), ctypes.byref(m), nv,
+                                         u.ctypes.data_as(dp)))
+
+
+def checkpoint_read(directory, cycle, dim, order, nelem, lo, hi):
+    """Read checkpoint `cycle` for this mesh into a new host array; returns
+    (u, t).  Raises PrandtlError (PRANDTL_ERR_IO) for a missing, truncated,
+    corrupt or mismatched checkpoint."""
+    m, _ = _descs(dim, order, nelem, lo, hi)
+    nv = dim + 2
+    NE = int(np.prod([int(nelem[r]) for r in range(dim)]))
+    u = np.empty(nv * NE * (order + 1) ** dim)
+    t = ctypes.c_double()
+    dp = ctypes.POINTER(ctypes.c_double)
+    check(lib().prandtl_checkpoint_read(str(directory).encode(), int(cycle), ctypes.byref(m), nv,
+                                        u.ctypes.data_as(dp), ctypes.byref(t)))
+    return u, t.value
+
+
 def _stream_ptr():
     import torch
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -140,6 +166,18 @@ class DGSEMOperator:
             e.t, e.steps = tt.value, steps.value
             raise
         return tt.value, steps.value
+
+    def save_checkpoint(self, u, directory, cycle, t):
+        """Checkpoint `cycle` of device state u at time t (prandtl_checkpoint_save)."""
+        check(lib().prandtl_checkpoint_save(self._h, self._vec(u, "u"), str(directory).encode(), int(cycle),
+                                            float(t), _stream_ptr()), self._h)
+
+    def load_checkpoint(self, u, directory, cycle):
+        """Load checkpoint `cycle` into device state u; returns its time."""
+        t = ctypes.c_double()
+        check(lib().prandtl_checkpoint_load(self._h, self._vec(u, "u"), str(directory).encode(), int(cycle),
+                                            ctypes.byref(t), _stream_ptr()), self._h)
+        return t.value
 
     def set_graph(self, enable):
         """Capture/replay the RK3 step as a hipGraph on non-default streams."""

@@ -76,3 +76,24 @@ def test_device_loop_state_check_stops(cuda, every):
     good = DGSEMOperator(**IV)
     t, n = good.Run(state(IV, cuda), 0.0, 0.02, dt=1e-3, check_every=1)
     assert n == 20
+
+
+def test_checkpoint_resume_bitwise(cuda, tmp_path):
+    """10 steps, checkpoint, 10 more; a new operator that loads the
+    checkpoint and takes the same 10 steps ends bitwise equal (fixed dt:
+    a restart starts a new running wave speed, as the reference's does)."""
+    import torch
+    s = torch.cuda.Stream(device=cuda)
+    with torch.cuda.stream(s):
+        a = DGSEMOperator(**TGV)
+        u = state(TGV, cuda)
+        t, n = a.Run(u, 0.0, 1.0, dt=1e-3, max_steps=10)
+        a.save_checkpoint(u, tmp_path, n, t)
+        t_end, _ = a.Run(u, t, 1.0, dt=1e-3, max_steps=10)
+        b = DGSEMOperator(**TGV)
+        v = torch.empty_like(u)
+        t_load = b.load_checkpoint(v, tmp_path, 10)
+        t_end2, _ = b.Run(v, t_load, 1.0, dt=1e-3, max_steps=10)
+    torch.cuda.synchronize()
+    assert t_load == t and t_end2 == t_end
+    assert torch.equal(u, v)
