@@ -160,6 +160,10 @@ int prandtl_profile_end(prandtl_ctx *ctx, double *ms, int *count);
 /* Last error message of this context (or of create when ctx is NULL). */
 const char *prandtl_last_error(const prandtl_ctx *ctx);
 
+/* Primitive output fields (Simulation.cpp:1046-1061): prim_dev receives
+ * rho, the velocity components and p, [dim+2][NE][K] like the state. */
+int prandtl_primitives(prandtl_ctx *ctx, const double *u_dev, double *prim_dev, void *stream);
+
 /* ---- Checkpoints (Simulation.cpp:1084-1116 save, 350-388 load) ----
  * The folder layout of the reference: dir/Cycle<cycle>/checkpoint_cycle_<cycle>.json
  * holding {"time", "cycle"}, and .../checkpoint_cycle_<cycle>.00000000.chk.

@@ -26,7 +26,7 @@ EXPORTED = (
     "prandtl_create", "prandtl_destroy", "prandtl_sizes", "prandtl_rhs", "prandtl_rk3_step",
     "prandtl_set_graph", "prandtl_set_shape", "prandtl_autotune", "prandtl_run",
     "prandtl_max_wave_speed", "prandtl_alpha", "prandtl_check_state", "prandtl_lgl",
-    "prandtl_debug_gradients", "prandtl_profile_begin", "prandtl_profile_end", "prandtl_last_error",
+    "prandtl_debug_gradients", "prandtl_primitives", "prandtl_profile_begin", "prandtl_profile_end", "prandtl_last_error",
     "prandtl_ensemble_create", "prandtl_ensemble_destroy", "prandtl_ensemble_case",
     "prandtl_ensemble_rk3_step", "prandtl_ensemble_last_error",
     "prandtl_checkpoint_write", "prandtl_checkpoint_read", "prandtl_checkpoint_save", "prandtl_checkpoint_load",
@@ -83,6 +83,7 @@ def lib():
     L.prandtl_check_state.argtypes = [vp, vp, vp]
     L.prandtl_lgl.argtypes = [ctypes.c_int, dp, dp]
     L.prandtl_debug_gradients.argtypes = [vp, vp, vp, vp]
+    L.prandtl_primitives.argtypes = [vp, vp, vp, vp]
     L.prandtl_profile_begin.argtypes = [vp]
     L.prandtl_profile_end.argtypes = [vp, dp, ip]
     L.prandtl_last_error.argtypes = [vp]

@@ -1075,6 +1075,25 @@ __global__ void k_check(const double *u, size_t NK, double gm1, int *flag)
     if (bad) atomicOr(flag, 1);
 }
 
+// Primitive output fields of Simulation.cpp:1046-1061: rho, velocity
+// components mom_d/rho, and p = (gamma-1)(E - rho |v|^2 / 2), with |v|^2
+// summed in the reference's order from the divided velocities.
+template <int DIM>
+__global__ void k_primitives(const double *u, double *prim, size_t NK, double gm1)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NK) return;
+    const double rho = u[i];
+    double vsq = 0.0;
+    prim[i] = rho;
+    for (int d = 0; d < DIM; d++) {
+        const double v = u[(1 + d) * NK + i] / rho;
+        prim[(1 + d) * NK + i] = v;
+        vsq = (d == 0) ? v * v : vsq + v * v;
+    }
+    prim[(DIM + 1) * NK + i] = gm1 * (u[(DIM + 1) * NK + i] - 0.5 * rho * vsq);
+}
+
 // The NaN/Inf check of Simulation.cpp:1025-1035 inside a device loop, after
 // every step whose count is a multiple of `every`: a non-finite or
 // non-positive density or pressure stops the loop (bad = done = 1) before
@@ -1845,6 +1864,17 @@ int prandtl_check_state(prandtl_ctx *c, const double *u, void *stream)
     if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
     if (rc) return rc;
     return flag ? fail(c, PRANDTL_ERR_STATE, "non-finite or non-positive density/pressure") : PRANDTL_OK;
+}
+
+int prandtl_primitives(prandtl_ctx *c, const double *u, double *prim, void *stream)
+{
+    if (!c || !u || !prim) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t NK = (size_t)c->NE * c->K;
+    const int nb = (int)((NK + 255) / 256);
+    if (c->dim == 2) hipLaunchKernelGGL(k_primitives<2>, dim3(nb), dim3(256), 0, s, u, prim, NK, c->phys.gm1);
+    else hipLaunchKernelGGL(k_primitives<3>, dim3(nb), dim3(256), 0, s, u, prim, NK, c->phys.gm1);
+    return hip_check(c, hipGetLastError(), "k_primitives launch");
 }
 
 int prandtl_debug_gradients(prandtl_ctx *c, const double *u, double *grad, void *stream)

@@ -234,6 +234,15 @@ class DGSEMOperator:
         check(lib().prandtl_profile_end(self._h, ms, cnt), self._h)
         return {"rhs": (ms[0], cnt[0]), "visc": (ms[1], cnt[1])}
 
+    def primitives(self, u):
+        """rho, velocity components and p of state u, [nv][NE][K]
+        (the reference's output fields, Simulation.cpp:1046-1061)."""
+        import torch
+        prim = torch.empty((self.nv, self.NE, self.K), dtype=torch.float64, device=u.device)
+        check(lib().prandtl_primitives(self._h, self._vec(u, "u"), ctypes.c_void_p(prim.data_ptr()),
+                                       _stream_ptr()), self._h)
+        return prim
+
     def gradients(self, u):
         """Primitive-variable BR1 gradients [dim][nv][NE][K] (parabolic contexts)."""
         import torch

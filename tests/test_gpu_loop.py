@@ -97,3 +97,24 @@ def test_checkpoint_resume_bitwise(cuda, tmp_path):
     torch.cuda.synchronize()
     assert t_load == t and t_end2 == t_end
     assert torch.equal(u, v)
+
+
+@pytest.mark.parametrize("cfg", [IV, TGV], ids=["2d", "3d"])
+def test_primitives(cuda, cfg):
+    import torch
+    op = DGSEMOperator(**cfg)
+    u = state(cfg, cuda)
+    prim = op.primitives(u)
+    torch.cuda.synchronize()
+    U = u.cpu().numpy().reshape(op.nv, op.NE, op.K)
+    P = prim.cpu().numpy()
+    dim = cfg["dim"]
+    vel = [U[1 + d] / U[0] for d in range(dim)]
+    vsq = vel[0] * vel[0]
+    for d in range(1, dim):
+        vsq = vsq + vel[d] * vel[d]
+    p = 0.4 * (U[dim + 1] - 0.5 * U[0] * vsq)
+    assert np.array_equal(P[0], U[0])
+    for d in range(dim):
+        assert np.array_equal(P[1 + d], vel[d])
+    assert np.abs(P[dim + 1] - p).max() <= 1e-15 * np.abs(p).max()
