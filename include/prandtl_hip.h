@@ -58,6 +58,11 @@ typedef struct prandtl_params {
                            variables, two-point fluxes, indicator and RK
                            accumulation stay FP64); Navier-Stokes with quirks = 0
                            only, identical to FP64 for Euler */
+    int sutherland;     /* SUTHERLAND build (NavierStokesFlux.hpp:32-36): mu(T) by
+                           Sutherland's law with mu0 = 1.716e-5, T0 = 273.15,
+                           Ts = 110.4 (Physics.hpp:35-37), T = p / (rho R_gas);
+                           mu is then unused */
+    double R_gas;       /* gas constant for that temperature (> 0 with sutherland) */
 } prandtl_params;
 
 /* DGSEMOperator constructor (DGSEMOperator.hpp:87-99): builds LGL rule,

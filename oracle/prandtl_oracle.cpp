@@ -393,10 +393,22 @@ template <class T> void entgrad2primgrad(int dim, double gamma, int quirks, cons
     g[L] = p / s[0] * (g[0] + p * gs[L]);
 }
 
-/* NavierStokesFlux::ComputeViscousFlux, NavierStokesFlux.cpp:12-90 (constant
- * mu, mu_bulk = 2/3 from Physics.hpp:34).  flux is nv x dim, row 0 = 0. */
+/* NavierStokesFlux::ComputeViscosity (NavierStokesFlux.hpp:32-36): Sutherland's
+ * law with the constants of Physics.hpp:35-37 and T0pTs = T0 + Ts formed as
+ * in the constructor (NavierStokesFlux.hpp:22). */
 template <class T>
-void viscous_flux(int dim, double gamma, double Pr, double mu, const T *s,
+T sutherland_mu(T rho, T p, double R_gas)
+{
+    const double mu0 = 1.716e-5, T0 = 273.15, Ts = 110.4, T0pTs = T0 + Ts;
+    const T temp = p / (rho * T(R_gas));
+    return T(mu0) * T(T0pTs) / (temp + T(Ts)) * (temp / T(T0)) * msqrt(temp / T(T0));
+}
+
+/* NavierStokesFlux::ComputeViscousFlux, NavierStokesFlux.cpp:12-90 (mu_bulk =
+ * 2/3 from Physics.hpp:34).  R_gas > 0: the SUTHERLAND build's per-point mu
+ * (lines 34-36), else the constant mu.  flux is nv x dim, row 0 = 0. */
+template <class T>
+void viscous_flux(int dim, double gamma, double Pr, double mu_const, double R_gas, const T *s,
                   const T *const *dq, T *flux)
 {
     const double gm1 = gamma - 1.0, gm1i = 1.0 / gm1;
@@ -406,6 +418,7 @@ void viscous_flux(int dim, double gamma, double Pr, double mu, const T *s,
     prim[0] = s[0];
     for (int d = 0; d < dim; d++) prim[1 + d] = s[1 + d] / s[0];
     prim[nv - 1] = pressure(dim, gamma, s);
+    const T mu = R_gas > 0.0 ? sutherland_mu<T>(prim[0], prim[nv - 1], R_gas) : T(mu_const);
     T lambda = mu * gamma * (1.0 / Pr);
     T div = dq[0][1];
     for (int d = 1; d < dim; d++) div += dq[d][1 + d];
@@ -743,7 +756,7 @@ template <class T> struct Engine {
                     for (int v = 0; v < nv; v++) dq[cc][v] = G[(cc * nv + v) * NK + (size_t)e * K + i];
                     dqp[cc] = dq[cc];
                 }
-                viscous_flux(dim, gamma, c->p.Pr, c->p.mu, st, dqp, fl);
+                viscous_flux(dim, gamma, c->p.Pr, c->p.mu, c->p.sutherland ? c->p.R_gas : 0.0, st, dqp, fl);
                 const double *adj = &m.adj[((size_t)e * K + i) * dim * dim];
                 for (int r = 0; r < dim; r++)
                     for (int v = 0; v < nv; v++) {
@@ -855,10 +868,10 @@ template <class T> struct Engine {
                         for (int v = 0; v < nv; v++) dq[cc][v] = G[(cc * nv + v) * NK + (size_t)e1 * K + n1];
                         dqp[cc] = dq[cc];
                     }
-                    viscous_flux(dim, gamma, c->p.Pr, c->p.mu, s1, dqp, fl1);
+                    viscous_flux(dim, gamma, c->p.Pr, c->p.mu, c->p.sutherland ? c->p.R_gas : 0.0, s1, dqp, fl1);
                     for (int cc = 0; cc < dim; cc++)
                         for (int v = 0; v < nv; v++) dq[cc][v] = G[(cc * nv + v) * NK + (size_t)e2 * K + n2];
-                    viscous_flux(dim, gamma, c->p.Pr, c->p.mu, s2, dqp, fl2);
+                    viscous_flux(dim, gamma, c->p.Pr, c->p.mu, c->p.sutherland ? c->p.R_gas : 0.0, s2, dqp, fl2);
                     for (int v = 0; v < nv; v++) {
                         fn[v] = 0.0;
                         for (int cc = 0; cc < dim; cc++) {
@@ -1013,11 +1026,11 @@ void orc_entgrad2primgrad(int dim, double gamma, int quirks, const double *s, do
     entgrad2primgrad<double>(dim, gamma, quirks, s, g);
 }
 
-void orc_viscous_flux(int dim, double gamma, double Pr, double mu, const double *s,
+void orc_viscous_flux(int dim, double gamma, double Pr, double mu, double R_gas, const double *s,
                       const double *dqdx, const double *dqdy, const double *dqdz, double *flux)
 {
     const double *dq[3] = {dqdx, dqdy, dqdz};
-    viscous_flux<double>(dim, gamma, Pr, mu, s, dq, flux);
+    viscous_flux<double>(dim, gamma, Pr, mu, R_gas, s, dq, flux);
 }
 
 int orc_create(const orc_params *p, orc_ctx **out)
@@ -1026,6 +1039,7 @@ int orc_create(const orc_params *p, orc_ctx **out)
     if (p->dim < 2 || p->dim > 3 || p->order < 1 || p->order > 12) return 2;
     for (int r = 0; r < p->dim; r++)
         if (p->nelem[r] < 1 || !(p->hi[r] > p->lo[r])) return 3;
+    if (p->sutherland && !(p->R_gas > 0.0)) return 5;
     orc_ctx *c = new (std::nothrow) orc_ctx();
     if (!c) return 4;
     c->p = *p;

@@ -31,6 +31,8 @@ typedef struct orc_params {
     int ec_surface;     /* 1: entropy-conservative F# at faces (entropy test) */
     int quirks;         /* 1: reproduce the reference's defects, 0: corrected */
     int threads;        /* OpenMP threads for the double instantiation (<=0: 1) */
+    int sutherland;     /* SUTHERLAND build: mu(T) by Sutherland's law, T = p / (rho R_gas) */
+    double R_gas;       /* gas constant of that temperature (> 0 with sutherland) */
 } orc_params;
 
 /* 1-D rules and operators (setup arithmetic of DGSEMIntegrator.cpp:8-82) */
@@ -46,7 +48,8 @@ double orc_face_flux(int dim, double gamma, int quirks, const double *s1, const 
                      const double *nor, double *F);
 void orc_cons2entropy(int dim, double gamma, const double *s, double *w);
 void orc_entgrad2primgrad(int dim, double gamma, int quirks, const double *s, double *g);
-void orc_viscous_flux(int dim, double gamma, double Pr, double mu, const double *s,
+/* R_gas > 0: Sutherland viscosity (mu is then unused) */
+void orc_viscous_flux(int dim, double gamma, double Pr, double mu, double R_gas, const double *s,
                       const double *dqdx, const double *dqdy, const double *dqdz,
                       double *flux /* nv x dim row-major */);
 

@@ -42,7 +42,8 @@ class Params(ctypes.Structure):
     _fields_ = [("gamma", ctypes.c_double), ("Pr", ctypes.c_double), ("mu", ctypes.c_double),
                 ("alpha_max", ctypes.c_double), ("alpha_min", ctypes.c_double),
                 ("parabolic", ctypes.c_int), ("blending", ctypes.c_int),
-                ("ec_surface", ctypes.c_int), ("quirks", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("ec_surface", ctypes.c_int), ("quirks", ctypes.c_int), ("precision", ctypes.c_int),
+                ("sutherland", ctypes.c_int), ("R_gas", ctypes.c_double)]
 
 
 class PrandtlError(RuntimeError):

@@ -814,7 +814,8 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
     const int ii[3] = {n % NP, (n / NP) % NP, DIM == 3 ? n / (NP * NP) : 0};
     R *visc = reinterpret_cast<R *>(A.visc);
     R *fvf = reinterpret_cast<R *>(A.fvf);
-    const R gm1 = (R)P.gm1, gm1i = (R)P.gm1i, mu = (R)P.mu, lam_h = (R)P.lam_h;
+    const R gm1 = (R)P.gm1, gm1i = (R)P.gm1i;
+    const ViscCoef<R> vcoef{(R)P.mu, (R)P.lam_h, P.sutherland != 0, (R)P.R_gas, (R)P.gamma, (R)P.PrInv};
 
     // every global load of this thread in flight together: its own node and
     // the neighbour trace nodes of its face jobs
@@ -920,7 +921,7 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
         R sv[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) sv[v] = A.quirks ? (R)sUl[el][v] : yr[v];
-        viscous_flux<DIM, R>(sv, gp, gm1, gm1i, mu, lam_h, fl);
+        viscous_flux<DIM, R>(sv, gp, gm1, gm1i, vcoef, fl);
 #pragma unroll
         for (int r = 0; r < DIM; r++)
 #pragma unroll
@@ -929,7 +930,7 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
 #pragma unroll
         for (int c = 0; c < DIM; c++) onface |= (ii[c] == 0 || ii[c] == NP - 1);
         if (onface) {
-            if (A.quirks) viscous_flux<DIM, R>(yr, gp, gm1, gm1i, mu, lam_h, fl);
+            if (A.quirks) viscous_flux<DIM, R>(yr, gp, gm1, gm1i, vcoef, fl);
 #pragma unroll
             for (int c = 0; c < DIM; c++) {
                 if (ii[c] == 0 || ii[c] == NP - 1) {
@@ -1461,6 +1462,8 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "the mixed form is built for corrected mode (quirks = 0)");
     if (!(prm->gamma > 1.0)) return fail(nullptr, PRANDTL_ERR_ARG, "gamma must exceed 1");
     if (prm->parabolic && !(prm->Pr > 0.0)) return fail(nullptr, PRANDTL_ERR_ARG, "Pr must be positive");
+    if (prm->parabolic && prm->sutherland && !(prm->R_gas > 0.0))
+        return fail(nullptr, PRANDTL_ERR_ARG, "R_gas must be positive with Sutherland viscosity");
     long long NE = 1;
     for (int r = 0; r < dim; r++) {
         if (mesh->nelem[r] < 2) return fail(nullptr, PRANDTL_ERR_ARG, "need >= 2 elements per direction");
@@ -1562,6 +1565,9 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     c->phys.alpha_max = prm->alpha_max;
     c->phys.alpha_min = prm->alpha_min;
     c->phys.thresh = 0.5 * std::pow(10.0, -1.8 * std::pow((double)N, 0.25));
+    c->phys.sutherland = prm->parabolic && prm->sutherland;
+    c->phys.R_gas = prm->R_gas;
+    c->phys.PrInv = 1.0 / prm->Pr;
 
     const size_t NK = (size_t)c->NE * c->K;
     hipError_t e = hipSuccess;

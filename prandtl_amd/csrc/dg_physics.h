@@ -16,7 +16,27 @@ struct Phys {
     double gamma, gm1, gm1i;
     double mu, lam_h;      // mu and mu*gamma*(1/Pr) (NavierStokesFlux.cpp:38)
     double alpha_max, alpha_min, thresh;
+    int sutherland;        // mu(T) per point (NavierStokesFlux.hpp:32-36)
+    double R_gas, PrInv;
 };
+
+// Viscosity coefficients of one viscous-flux evaluation, in arithmetic R:
+// constant mu and lam = mu*gamma*(1/Pr), or Sutherland's law per point.
+template <typename R> struct ViscCoef {
+    R mu, lam_h;
+    bool suth;
+    R R_gas, gamma, PrInv;
+};
+
+// NavierStokesFlux::ComputeViscosity (NavierStokesFlux.hpp:32-36) with the
+// constants of Physics.hpp:35-37; T0pTs = T0 + Ts as in the constructor.
+template <typename R>
+__device__ __forceinline__ R sutherland_mu(R rho, R p, R R_gas)
+{
+    const R mu0 = R(1.716e-5), T0 = R(273.15), Ts = R(110.4), T0pTs = R(273.15 + 110.4);
+    const R T = p / (rho * R_gas);
+    return mu0 * T0pTs / (T + Ts) * (T / T0) * sqrt(T / T0);
+}
 
 // Per-node quantities shared by every two-point flux touching the node.
 template <int DIM> struct NodeQ {
@@ -185,12 +205,12 @@ __device__ __forceinline__ void entgrad2primgrad(const R *s, R gm1, R gm1i, bool
     g[DIM + 1] = p / s[0] * (g[0] + p * gL);
 }
 
-// NavierStokesFlux::ComputeViscousFlux (NavierStokesFlux.cpp:12-90), constant
-// mu, mu_bulk = 2/3, in arithmetic R.  dq[c][v] = d(prim_v)/dx_c; out
+// NavierStokesFlux::ComputeViscousFlux (NavierStokesFlux.cpp:12-90), mu_bulk =
+// 2/3, in arithmetic R; constant mu or Sutherland's law (ViscCoef).  dq[c][v] = d(prim_v)/dx_c; out
 // fl[v][c], v = 1..nv-1 (the mass row is zero and not stored: fl[v-1][c]).
 template <int DIM, typename R>
 __device__ __forceinline__ void viscous_flux(const R *s, const R (*dq)[DIM + 2], R gm1, R gm1i,
-                                             R mu, R lam_h, R (*fl)[DIM])
+                                             const ViscCoef<R> &vc, R (*fl)[DIM])
 {
     constexpr int NV = DIM + 2;
     const R mu_bulk = R(2) / R(3), one = 1, two = 2, half = 0.5;
@@ -207,6 +227,12 @@ __device__ __forceinline__ void viscous_flux(const R *s, const R (*dq)[DIM + 2],
         vv = (d == 0) ? v * v : vv + v * v;
     }
     const R p = gm1 * (s[NV - 1] - half * rho * vv);
+    R mu = vc.mu, lam_h = vc.lam_h;
+    if (vc.suth) {
+        // SUTHERLAND build: mu from Conserv2Prim's rho and p, lambda = mu*gamma*(1/Pr)
+        mu = sutherland_mu<R>(rho, p, vc.R_gas);
+        lam_h = mu * vc.gamma * vc.PrInv;
+    }
     R div = dq[0][1];
 #pragma unroll
     for (int d = 1; d < DIM; d++) div += dq[d][1 + d];

@@ -31,7 +31,8 @@ def lgl_nodes(order):
 
 
 def _descs(dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0, parabolic=False, blending=False,
-           alpha_max=0.5, alpha_min=0.001, ec_surface=False, quirks=True, precision="fp64"):
+           alpha_max=0.5, alpha_min=0.001, ec_surface=False, quirks=True, precision="fp64",
+           sutherland=False, R_gas=287.05):
     """prandtl_mesh_desc and prandtl_params for one case."""
     m = _lib.MeshDesc()
     m.dim, m.order = dim, order
@@ -45,6 +46,7 @@ def _descs(dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0, parabolic=Fals
     p.parabolic, p.blending = int(parabolic), int(blending)
     p.ec_surface, p.quirks = int(ec_surface), int(quirks)
     p.precision = {"fp64": _lib.PRANDTL_FP64, "mixed": _lib.PRANDTL_MIXED}[precision]
+    p.sutherland, p.R_gas = int(sutherland), float(R_gas)
     return m, p
 
 
@@ -87,9 +89,9 @@ class DGSEMOperator:
 
     def __init__(self, dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0,
                  parabolic=False, blending=False, alpha_max=0.5, alpha_min=0.001,
-                 ec_surface=False, quirks=True, precision="fp64"):
+                 ec_surface=False, quirks=True, precision="fp64", sutherland=False, R_gas=287.05):
         m, p = _descs(dim, order, nelem, lo, hi, gamma, Pr, mu, parabolic, blending, alpha_max,
-                      alpha_min, ec_surface, quirks, precision)
+                      alpha_min, ec_surface, quirks, precision, sutherland, R_gas)
         h = ctypes.c_void_p()
         rc = lib().prandtl_create(ctypes.byref(m), ctypes.byref(p), ctypes.byref(h))
         check(rc, None)

@@ -24,6 +24,7 @@ class OrcParams(ctypes.Structure):
         ("parabolic", ctypes.c_int), ("blending", ctypes.c_int),
         ("alpha_max", ctypes.c_double), ("alpha_min", ctypes.c_double),
         ("ec_surface", ctypes.c_int), ("quirks", ctypes.c_int), ("threads", ctypes.c_int),
+        ("sutherland", ctypes.c_int), ("R_gas", ctypes.c_double),
     ]
 
 
@@ -42,7 +43,7 @@ def lib():
         L.orc_face_flux.argtypes = [ctypes.c_int, _D, ctypes.c_int, _P, _P, _P, _P]
         L.orc_cons2entropy.argtypes = [ctypes.c_int, _D, _P, _P]
         L.orc_entgrad2primgrad.argtypes = [ctypes.c_int, _D, ctypes.c_int, _P, _P]
-        L.orc_viscous_flux.argtypes = [ctypes.c_int, _D, _D, _D, _P, _P, _P, _P, _P]
+        L.orc_viscous_flux.argtypes = [ctypes.c_int, _D, _D, _D, _D, _P, _P, _P, _P, _P]
         L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(ctypes.c_void_p)]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_sizes.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3
@@ -122,10 +123,11 @@ def entgrad2primgrad(dim, gamma, quirks, s, g):
     return g
 
 
-def viscous_flux(dim, gamma, Pr, mu, s, dq):
+def viscous_flux(dim, gamma, Pr, mu, s, dq, R_gas=0.0):
+    """R_gas > 0: Sutherland viscosity."""
     fl = np.zeros((dim + 2) * dim)
     dqs = [np.ascontiguousarray(d, float) for d in dq] + [np.zeros(dim + 2)] * (3 - dim)
-    lib().orc_viscous_flux(dim, gamma, Pr, mu, ptr(np.ascontiguousarray(s, float)),
+    lib().orc_viscous_flux(dim, gamma, Pr, mu, R_gas, ptr(np.ascontiguousarray(s, float)),
                            ptr(dqs[0]), ptr(dqs[1]), ptr(dqs[2]), ptr(fl))
     return fl.reshape(dim + 2, dim)
 
@@ -135,7 +137,7 @@ class Oracle:
 
     def __init__(self, dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0,
                  parabolic=False, blending=False, alpha_max=0.5, alpha_min=0.001,
-                 ec_surface=False, quirks=True, threads=1):
+                 ec_surface=False, quirks=True, threads=1, sutherland=False, R_gas=287.05):
         p = OrcParams()
         p.dim, p.order = dim, order
         for r in range(3):
@@ -146,6 +148,7 @@ class Oracle:
         p.parabolic, p.blending = int(parabolic), int(blending)
         p.alpha_max, p.alpha_min = alpha_max, alpha_min
         p.ec_surface, p.quirks, p.threads = int(ec_surface), int(quirks), int(threads)
+        p.sutherland, p.R_gas = int(sutherland), float(R_gas)
         self.params = p
         h = ctypes.c_void_p()
         rc = lib().orc_create(ctypes.byref(p), ctypes.byref(h))

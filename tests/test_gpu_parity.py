@@ -243,3 +243,25 @@ def test_graph_replays_match_eager(cuda):
             d = float((ug - ue).abs().max() / ue.abs().max())
             print("graph step", step, d, flush=True)
             assert d == 0.0
+
+
+@pytest.mark.parametrize("dim,order,quirks,precision", [(2, 3, False, "fp64"), (3, 3, False, "fp64"),
+                                                        (3, 3, True, "fp64"), (3, 4, False, "mixed")])
+def test_sutherland_parity(cuda, dim, order, quirks, precision):
+    """Sutherland viscosity (NavierStokesFlux.hpp:32-36) on states at air-like
+    temperatures (rho ~ 1.2, p ~ 1e5, T ~ 290 K), where mu(T) varies over the
+    state: RHS against the oracle (the mixed form: its FP32 viscous part
+    against the FP64 oracle to 1e-6)."""
+    cfg = dict(dim=dim, order=order, nelem=(5, 4) if dim == 2 else (3, 3, 2), lo=(0.0,) * dim,
+               hi=(1e-3, 1.3e-3, 0.7e-3)[:dim], parabolic=True, Pr=0.71, quirks=quirks,
+               sutherland=True, R_gas=287.05)
+    op = DGSEMOperator(precision=precision, **cfg)
+    orc = O.Oracle(threads=8, **cfg)
+    X = cases.node_coords(dim, order, cfg["nelem"], cfg["lo"], cfg["hi"])
+    L = [cfg["hi"][r] - cfg["lo"][r] for r in range(dim)]
+    u = cases.random_smooth_state(X, dim, 5 + order, amp=0.2, rho0=1.2, p0=1e5, L=L)
+    u[1:dim + 1] *= 30.0                  # velocities of tens of m/s
+    u[dim + 1] += 0.5 * (900.0 - 1.0) * sum(u[1 + d] ** 2 / 900.0 for d in range(dim)) / u[0]
+    r = rel_max(gpu_rhs(op, u, cuda), orc.Mult(u))
+    print("sutherland", dim, order, quirks, precision, r, flush=True)
+    assert r <= (TOL if precision == "fp64" else 1e-6)

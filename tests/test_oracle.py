@@ -487,3 +487,47 @@ def test_flop_counter_counts_every_pair():
     assert c1["volume"] > 0 and c1["surface"] > 0
     # per element costs are the same up to log-mean branch choices
     assert abs(c2["volume"] / c1["volume"] - 2.0) < 0.05
+
+
+def _shear_state(T, rho=1.2, R_gas=287.05, gamma=1.4):
+    """2-D state at rest with temperature T (p = rho R T), and a pure shear
+    gradient du/dy = 1 plus a pressure gradient dp/dx = 1 (primitive
+    gradients, rows x and y)."""
+    p = rho * R_gas * T
+    s = np.array([rho, 0.0, 0.0, p / (gamma - 1.0)])
+    dqdx = np.array([0.0, 0.0, 0.0, 1.0])
+    dqdy = np.array([0.0, 1.0, 0.0, 0.0])
+    return s, [dqdx, dqdy]
+
+
+def test_sutherland_viscosity_pins():
+    """Sutherland's law (NavierStokesFlux.hpp:32-36, constants of
+    Physics.hpp:35-37): mu(T0) = mu0, and air at 300 K has mu = 1.846e-5
+    Pa s (the textbook value of the law with these constants).  The shear
+    entry (u-momentum, y) of the viscous flux is mu du/dy."""
+    s, dq = _shear_state(273.15)
+    fl = O.viscous_flux(2, 1.4, 0.72, 123.0, s, dq, R_gas=287.05)
+    assert abs(fl[1, 1] - 1.716e-5) <= 1e-15 * 1.716e-5
+    s, dq = _shear_state(300.0)
+    fl = O.viscous_flux(2, 1.4, 0.72, 123.0, s, dq, R_gas=287.05)
+    assert abs(fl[1, 1] - 1.846e-5) < 0.0005e-5
+    # viscosity grows with temperature, as T^1.5 / (T + Ts)
+    mus = []
+    for T in (200.0, 400.0, 800.0):
+        s, dq = _shear_state(T)
+        mus.append(O.viscous_flux(2, 1.4, 0.72, 0.0, s, dq, R_gas=287.05)[1, 1])
+    r = (400.0 / 200.0) ** 1.5 * (200.0 + 110.4) / (400.0 + 110.4)
+    assert mus[0] < mus[1] < mus[2] and abs(mus[1] / mus[0] - r) < 1e-13
+
+
+def test_sutherland_heat_flux_uses_the_same_mu():
+    """The Fourier term lambda cv dT/dx takes lambda = mu(T) gamma / Pr: at
+    rest with only dp/dx the energy flux is (gamma/Pr) mu(T) / ((gamma-1) rho)
+    dp/dx, and with constant viscosity it scales with the constant instead."""
+    s, dq = _shear_state(300.0)
+    fl = O.viscous_flux(2, 1.4, 0.72, 0.0, s, dq, R_gas=287.05)
+    mu = fl[1, 1]
+    q = fl[3, 0]
+    assert abs(q - 1.4 / 0.72 * mu / (0.4 * 1.2)) <= 1e-12 * abs(q)
+    fl_c = O.viscous_flux(2, 1.4, 0.72, 2.0 * mu, s, dq)
+    assert abs(fl_c[3, 0] - 2.0 * q) <= 1e-12 * abs(q) and abs(fl_c[1, 1] - 2.0 * mu) <= 1e-15 * mu
