@@ -41,7 +41,28 @@ typedef struct prandtl_mesh_desc {
     int order;          /* polynomial degree N (Np = N+1): 2..5 in 2-D, 2..7 in 3-D */
     int nelem[3];       /* elements per direction */
     double lo[3], hi[3];
+    /* Boundary condition of each box side: bc[r][0] at lo[r], bc[r][1] at
+     * hi[r], PRANDTL_BC_* (zero: periodic).  A direction is periodic on both
+     * sides or on neither.  The boundary integrators the reference attaches
+     * by boundary name (Simulation.cpp:490-735); Euler only. */
+    int bc[3][2];
+    /* conservative outer state (dim + 2 values) of supersonic-inflow and
+     * specified-state sides: the constant-state constructors of
+     * SupersonicInflowBdrFaceIntegrator.cpp:12-17 and
+     * SpecifiedStateBdrFaceIntegrator.cpp:12-16 */
+    double bc_state[3][2][5];
 } prandtl_mesh_desc;
+
+/* Boundary types, by the reference's configuration names ("type" of
+ * runTime.conditions.boundary_conditions, Simulation.cpp:518-735) */
+enum {
+    PRANDTL_BC_PERIODIC = 0,
+    PRANDTL_BC_SLIP = 1,               /* "slip": SlipWallBdrFaceIntegrator.cpp:15-60 */
+    PRANDTL_BC_SUPERSONIC_INFLOW = 2,  /* "supersonic-inflow", constant state */
+    PRANDTL_BC_SUPERSONIC_OUTFLOW = 3, /* "supersonic-outflow": outer state = interior */
+    PRANDTL_BC_SYMMETRY = 4,           /* "symmetry": mirrored normal momentum */
+    PRANDTL_BC_SPECIFIED_STATE = 5     /* "specified-state", constant state */
+};
 
 /* Physics and scheme switches: PhysicsConstants (Physics.hpp:10-37), the
  * compile-time PARABOLIC / SUBCELL_FV_BLENDING flags (CMakeLists.txt:36-50)

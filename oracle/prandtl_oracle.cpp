@@ -334,6 +334,131 @@ T face_flux(int dim, double gamma, int quirks, const T *s1, const T *s2, const T
     return lam;
 }
 
+/* Normalize (BasicOperations.hpp:39-42): vec /= Norml2(vec), MFEM's
+ * operator/= multiplying by the reciprocal */
+template <class T> void normalize(int dim, const T *nor, T *unit)
+{
+    T m = 1.0 / norml2(dim, nor);
+    for (int d = 0; d < dim; d++) unit[d] = nor[d] * m;
+}
+
+/* MFEM Vector dot product: prod = 0, prod += a_i b_i in index order */
+template <class T> T dot(int n, const T *a, const T *b)
+{
+    T prod = 0.0;
+    for (int i = 0; i < n; i++) prod += a[i] * b[i];
+    return prod;
+}
+
+/* RotateState (BasicOperations.cpp:238-259) with Normal and Cross
+ * (BasicOperations.cpp:211-236): momentum components in the (n, t1, t2)
+ * frame, t1 = Normal(n), t2 = n x t1 */
+template <class T> void rotate_state(int dim, T *state, const T *nor)
+{
+    T tan1[3];
+    if (dim == 2) { tan1[0] = -nor[1]; tan1[1] = nor[0]; }
+    else {
+        if (mabs(nor[0]) > T(1e-10)) {
+            T q = nor[1] / nor[0];
+            T den = msqrt(1.0 + mpow(q, 2.0));
+            tan1[0] = -(nor[1] / nor[0]) / den;
+            tan1[1] = 1.0 / den;
+        } else {
+            tan1[0] = -1.0;
+            tan1[1] = 0.0;
+        }
+        tan1[2] = 0.0;
+    }
+    T *rhoV = state + 1;
+    T rho_u = dot(dim, rhoV, nor);
+    T rho_v = dot(dim, rhoV, tan1);
+    if (dim == 3) {
+        T tan2[3];
+        tan2[0] = nor[1] * tan1[2] - nor[2] * tan1[1];
+        tan2[1] = nor[2] * tan1[0] - nor[0] * tan1[2];
+        tan2[2] = nor[0] * tan1[1] - nor[1] * tan1[0];
+        rhoV[2] = dot(dim, rhoV, tan2);
+    }
+    rhoV[0] = rho_u;
+    rhoV[1] = rho_v;
+}
+
+/* SlipWallBdrFaceIntegrator::ComputeBdrFaceInviscidFlux
+ * (SlipWallBdrFaceIntegrator.cpp:15-60): the wall pressure p* of the
+ * state rotated into the face frame (shock branch for u_n > 0, rarefaction
+ * branch clipped at 1e-4 otherwise), flux (0, p* nor, 0); returns the
+ * rotated-frame |v| + a. */
+template <class T> T slip_wall_flux(int dim, double gamma, const T *s1, const T *nor, T *F)
+{
+    const double gm1 = gamma - 1.0, gm1i = 1.0 / gm1;
+    const double gp1 = 1.0 + gamma, gp1i = 1.0 / gp1;
+    const int nv = dim + 2;
+    T unit[3], st[5], prim[5];
+    normalize(dim, nor, unit);
+    for (int v = 0; v < nv; v++) st[v] = s1[v];
+    rotate_state(dim, st, unit);
+    /* Conserv2Prim (BasicOperations.cpp:172-186): velocities by division,
+     * pressure by ComputePressure */
+    prim[0] = st[0];
+    for (int d = 0; d < dim; d++) prim[1 + d] = st[1 + d] / st[0];
+    {
+        T r = 1.0 / st[0];
+        T V[3];
+        for (int d = 0; d < dim; d++) V[d] = st[1 + d] * r;
+        prim[nv - 1] = gm1 * (st[nv - 1] - 0.5 * st[0] * dot(dim, V, V));
+    }
+    T rho = prim[0], un = prim[1], p = prim[nv - 1];
+    T an = sound_speed(p, rho, gamma);
+    T p_star;
+    if (un > T(0.0))
+        p_star = p + 0.25 * un * gp1 * rho *
+                 (un + msqrt(mpow(un, 2.0) + 8.0 * gp1i / rho * p * (gm1 * gp1i + 1.0)));
+    else
+        p_star = p * mpow(mmax(1.0 + 0.5 * gm1 * un / an, T(0.0001)), 2.0 * gamma * gm1i);
+    for (int v = 0; v < nv; v++) F[v] = 0.0;
+    for (int d = 0; d < dim; d++) F[1 + d] = p_star * nor[d];
+    return msqrt(dot(dim, prim + 1, prim + 1)) + an;
+}
+
+/* ComputeBdrFaceInviscidFlux of each boundary type: the outer state of
+ * ComputeOuterInviscidState and ChandrashekarFlux::ComputeFaceFlux with
+ * state1 = interior (BdrFaceIntegrator.cpp:252-258); slip walls replace the
+ * whole flux.  Symmetry (SymmetryBdrFaceIntegrator.cpp): outer momentum
+ * m - 2 (m.n) n with the unit normal, a zero normal gives a zero flux. */
+template <class T>
+T bc_flux(int dim, double gamma, int quirks, int type, const T *s1, const T *outer, const T *nor, T *F)
+{
+    const int nv = dim + 2;
+    T s2[5];
+    switch (type) {
+    case ORC_BC_SLIP:
+        return slip_wall_flux(dim, gamma, s1, nor, F);
+    case ORC_BC_SUPERSONIC_INFLOW:
+    case ORC_BC_SPECIFIED_STATE:
+        for (int v = 0; v < nv; v++) s2[v] = outer[v];
+        break;
+    case ORC_BC_SUPERSONIC_OUTFLOW:
+        for (int v = 0; v < nv; v++) s2[v] = s1[v];
+        break;
+    case ORC_BC_SYMMETRY: {
+        if (!(norml2(dim, nor) != T(0.0))) {
+            for (int v = 0; v < nv; v++) F[v] = 0.0;
+            return 0.0;
+        }
+        T unit[3];
+        normalize(dim, nor, unit);
+        for (int v = 0; v < nv; v++) s2[v] = s1[v];
+        T mn = dot(dim, s2 + 1, unit);
+        for (int d = 0; d < dim; d++) s2[1 + d] += -2.0 * mn * unit[d];
+        break;
+    }
+    default:
+        for (int v = 0; v < nv; v++) F[v] = 0.0;
+        return 0.0;
+    }
+    return face_flux(dim, gamma, quirks, s1, s2, nor, F);
+}
+
 /* ComputePressure, BasicOperations.cpp:31-37 (V /= rho multiplies by 1/rho) */
 template <class T> T pressure(int dim, double gamma, const T *s)
 {
@@ -454,6 +579,10 @@ struct Mesh {
     std::vector<double> fnor;        /* [nf*Nfp*dim] */
     std::vector<double> sub;         /* [NE][dim][Nfp][N][dim] interior subcell normals */
     double hmin;
+    /* boundary faces: elements at the low (side 0) / high (side 1) end of
+     * each non-periodic direction r; their faces' elem1 is that element */
+    std::vector<int> bel[3][2];
+    double ad[3];                    /* face measure of an r-face */
 };
 
 inline int line_node(int dim, int Np, int r, int l, int m)
@@ -511,8 +640,11 @@ void build_mesh(orc_ctx *c)
     m.adj.assign((size_t)NE * K * dim * dim, 0.0);
     for (size_t q = 0; q < (size_t)NE * K; q++)
         for (int r = 0; r < dim; r++) m.adj[q * dim * dim + r * dim + r] = ad[r];
+    for (int r = 0; r < 3; r++) m.ad[r] = ad[r];
     /* interior faces: every element is elem1 of its +r face, elem2 is its
-     * periodic +r neighbour, normal = +|face| e_r */
+     * periodic +r neighbour, normal = +|face| e_r; in a non-periodic
+     * direction the +r face of the last element is a boundary face
+     * (fe2 = -1, skipped by the interior loops) */
     m.nf = dim * NE;
     m.fe1.resize(m.nf); m.fe2.resize(m.nf);
     m.fn1.resize((size_t)m.nf * Nfp); m.fn2.resize((size_t)m.nf * Nfp);
@@ -522,9 +654,13 @@ void build_mesh(orc_ctx *c)
         for (int e = 0; e < NE; e++, f++) {
             int ex = e % m.n[0], ey = (e / m.n[0]) % m.n[1], ez = e / (m.n[0] * m.n[1]);
             int idx[3] = {ex, ey, ez};
+            const bool open = p.bc[r][0] != ORC_BC_PERIODIC;
+            if (open && idx[r] == 0) m.bel[r][0].push_back(e);
+            if (open && idx[r] == m.n[r] - 1) m.bel[r][1].push_back(e);
+            const bool last = idx[r] + 1 == m.n[r];
             idx[r] = (idx[r] + 1) % m.n[r];
             int e2 = (idx[2] * m.n[1] + idx[1]) * m.n[0] + idx[0];
-            m.fe1[f] = e; m.fe2[f] = e2;
+            m.fe1[f] = e; m.fe2[f] = (open && last) ? -1 : e2;
             for (int q = 0; q < Nfp; q++) {
                 m.fn1[(size_t)f * Nfp + q] = line_node(dim, Np, r, q, N);
                 m.fn2[(size_t)f * Nfp + q] = line_node(dim, Np, r, q, 0);
@@ -666,6 +802,7 @@ template <class T> struct Engine {
         for (int f = 0; f < m.nf; f++)
             for (int q = 0; q < Nfp; q++) {
                 int e1 = m.fe1[f], e2 = m.fe2[f];
+                if (e2 < 0) continue;
                 int n1 = m.fn1[(size_t)f * Nfp + q], n2 = m.fn2[(size_t)f * Nfp + q];
                 const double *nor = &m.fnor[((size_t)f * Nfp + q) * dim];
                 T a1 = 1.0 / (w0 * T(m.J[(size_t)e1 * K + n1]));
@@ -853,6 +990,7 @@ template <class T> struct Engine {
         for (int f = f0; f < f1; f++)
             for (int q = 0; q < Nfp; q++) {
                 int e1 = m.fe1[f], e2 = m.fe2[f];
+                if (e2 < 0) continue;
                 int n1 = m.fn1[(size_t)f * Nfp + q], n2 = m.fn2[(size_t)f * Nfp + q];
                 T s1[5], s2[5], nor[3], F[5];
                 state(u, e1, n1, s1);
@@ -905,6 +1043,35 @@ template <class T> struct Engine {
         return lam;
     }
 
+    /* boundary faces of side `side` of direction r, elements [b0, b1) of
+     * that side's list: BdrFaceIntegrator::AssembleFaceVector
+     * (BdrFaceIntegrator.cpp:61-123) adds -F_b / (w_0 J) at the face node of
+     * the interior element, F_b the boundary type's flux for the outward
+     * normal (CalcOrtho of the face Jacobian: -/+ |face| e_r) */
+    T bfaces(const T *u, int r, int side, int b0, int b1, T *dudt, T lam)
+    {
+        const int type = c->p.bc[r][side];
+        const double w0 = c->w[0];
+        T outer[5], nor[3];
+        for (int v = 0; v < nv; v++) outer[v] = c->p.bc_state[r][side][v];
+        for (int d = 0; d < dim; d++) nor[d] = 0.0;
+        nor[r] = side ? m.ad[r] : -m.ad[r];
+        for (int b = b0; b < b1; b++) {
+            const int e = m.bel[r][side][b];
+            for (int q = 0; q < Nfp; q++) {
+                const int n1 = line_node(dim, Np, r, q, side ? N : 0);
+                T s1[5], F[5];
+                state(u, e, n1, s1);
+                g_cat = CAT_FACE;
+                lam = mmax(lam, bc_flux(dim, gamma, c->p.quirks, type, s1, outer, nor, F));
+                g_cat = CAT_ASM;
+                T a1 = 1.0 / (w0 * T(m.J[(size_t)e * K + n1]));
+                for (int v = 0; v < nv; v++) dudt[v * NK + (size_t)e * K + n1] += a1 * (-F[v]);
+            }
+        }
+        return lam;
+    }
+
     /* DGSEMOperator::Mult (DGSEMOperator.cpp:537-591) with
      * DGSEMNonlinearForm::Mult (DGSEMNonlinearForm.cpp:647-849, 1302-1545):
      * dudt is overwritten. */
@@ -932,6 +1099,16 @@ template <class T> struct Engine {
 #endif
                 for (int b = 0; b < NE; b++) l = faces(u, r * NE + b, r * NE + b + 1, dudt, l);
             }
+            /* boundary faces after the interior ones (DGSEMNonlinearForm.cpp:776-826);
+             * the faces of one side touch disjoint nodes */
+            for (int r = 0; r < dim; r++)
+                for (int side = 0; side < 2; side++) {
+                    const int nb = (int)m.bel[r][side].size();
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+                    for (int b = 0; b < nb; b++) l = bfaces(u, r, side, b, b + 1, dudt, l);
+                }
             lam = std::max(lam, val(l));
         }
         (void)nth;
@@ -1033,6 +1210,12 @@ void orc_viscous_flux(int dim, double gamma, double Pr, double mu, double R_gas,
     viscous_flux<double>(dim, gamma, Pr, mu, R_gas, s, dq, flux);
 }
 
+double orc_bc_flux(int dim, double gamma, int quirks, int type, const double *s1, const double *outer,
+                   const double *nor, double *F)
+{
+    return bc_flux<double>(dim, gamma, quirks, type, s1, outer, nor, F);
+}
+
 int orc_create(const orc_params *p, orc_ctx **out)
 {
     if (!p || !out) return 1;
@@ -1040,6 +1223,14 @@ int orc_create(const orc_params *p, orc_ctx **out)
     for (int r = 0; r < p->dim; r++)
         if (p->nelem[r] < 1 || !(p->hi[r] > p->lo[r])) return 3;
     if (p->sutherland && !(p->R_gas > 0.0)) return 5;
+    for (int r = 0; r < p->dim; r++) {
+        for (int side = 0; side < 2; side++)
+            if (p->bc[r][side] < ORC_BC_PERIODIC || p->bc[r][side] > ORC_BC_SPECIFIED_STATE) return 6;
+        if ((p->bc[r][0] == ORC_BC_PERIODIC) != (p->bc[r][1] == ORC_BC_PERIODIC)) return 6;
+        /* Navier-Stokes boundaries need the boundary lifting and viscous
+         * fluxes, not restated */
+        if (p->parabolic && p->bc[r][0] != ORC_BC_PERIODIC) return 7;
+    }
     orc_ctx *c = new (std::nothrow) orc_ctx();
     if (!c) return 4;
     c->p = *p;

@@ -33,7 +33,22 @@ typedef struct orc_params {
     int threads;        /* OpenMP threads for the double instantiation (<=0: 1) */
     int sutherland;     /* SUTHERLAND build: mu(T) by Sutherland's law, T = p / (rho R_gas) */
     double R_gas;       /* gas constant of that temperature (> 0 with sutherland) */
+    /* boundary condition of each box side, bc[r][0] at lo[r], bc[r][1] at
+     * hi[r]: ORC_BC_* below; a direction is periodic on both sides or on
+     * neither.  bc_state: the constant conservative outer state of the
+     * supersonic-inflow / specified-state sides.  Euler only. */
+    int bc[3][2];
+    double bc_state[3][2][5];
 } orc_params;
+
+enum {
+    ORC_BC_PERIODIC = 0,
+    ORC_BC_SLIP = 1,               /* SlipWallBdrFaceIntegrator.cpp:15-60 */
+    ORC_BC_SUPERSONIC_INFLOW = 2,  /* SupersonicInflowBdrFaceIntegrator.cpp (constant state) */
+    ORC_BC_SUPERSONIC_OUTFLOW = 3, /* SupersonicOutflowBdrFaceIntegrator.cpp */
+    ORC_BC_SYMMETRY = 4,           /* SymmetryBdrFaceIntegrator.cpp */
+    ORC_BC_SPECIFIED_STATE = 5     /* SpecifiedStateBdrFaceIntegrator.cpp (constant state) */
+};
 
 /* 1-D rules and operators (setup arithmetic of DGSEMIntegrator.cpp:8-82) */
 int orc_lgl(int order, double *x, double *w);
@@ -52,6 +67,12 @@ void orc_entgrad2primgrad(int dim, double gamma, int quirks, const double *s, do
 void orc_viscous_flux(int dim, double gamma, double Pr, double mu, double R_gas, const double *s,
                       const double *dqdx, const double *dqdy, const double *dqdz,
                       double *flux /* nv x dim row-major */);
+
+/* ComputeBdrFaceInviscidFlux of boundary type `type` for the interior state
+ * s1, outer state `outer` (inflow / specified state only) and the outward
+ * face normal nor (face-measure scaled); returns the wave speed */
+double orc_bc_flux(int dim, double gamma, int quirks, int type, const double *s1, const double *outer,
+                   const double *nor, double *F);
 
 /* operator */
 int orc_create(const orc_params *p, orc_ctx **out);

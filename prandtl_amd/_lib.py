@@ -35,7 +35,13 @@ EXPORTED = (
 
 class MeshDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("nelem", ctypes.c_int * 3),
-                ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3)]
+                ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
+                ("bc", (ctypes.c_int * 2) * 3), ("bc_state", ((ctypes.c_double * 5) * 2) * 3)]
+
+
+# boundary types (PRANDTL_BC_*) by the reference's configuration names
+BC_TYPES = {"periodic": 0, "slip": 1, "supersonic-inflow": 2, "supersonic-outflow": 3,
+            "symmetry": 4, "specified-state": 5}
 
 
 class Params(ctypes.Structure):

@@ -171,6 +171,28 @@ def kelvin_helmholtz(X, gamma=1.4, s=15.0, seed=20261202, noise=1e-3):
     return prim_to_cons(rho, [u, v], p, gamma)
 
 
+def triple_point(X, gamma=1.4):
+    """TriplePointShockInteractionIC (TestCases/Euler/2D/TriplePointShockInteraction/
+    TriplePointShockInteraction.hpp:9-39), gas at rest: rho = 1, p = 1 for
+    x <= 1; rho = 1, p = 0.1 for x <= 7 and y <= 1.5; rho = 0.125, p = 0.1
+    elsewhere."""
+    x, y = X[0], X[1]
+    left = x <= 1.0
+    low = (x <= 7.0) & (y <= 1.5)
+    rho = np.where(left, 1.0, np.where(low, 1.0, 0.125))
+    p = np.where(left, 1.0, 0.1)
+    zero = np.zeros_like(x)
+    return prim_to_cons(rho, [zero, zero], p, gamma)
+
+
+# The triple-point problem of the reference's test cases (config.json there:
+# order 3, SUBCELL_FV_BLENDING, slip walls on all four sides, cfl 1.0, final
+# time 4.0) on the box [0,7] x [0,3]; its Gmsh mesh is not shipped, so the box
+# is meshed uniformly (h = 1/16 here).
+TRIPLE_POINT = dict(dim=2, order=3, nelem=(112, 48), lo=(0.0, 0.0), hi=(7.0, 3.0), gamma=1.4,
+                    blending=True, alpha_max=0.5, bc=[("slip", "slip"), ("slip", "slip")])
+
+
 def random_smooth_state(X, dim, seed, amp=0.2, gamma=1.4, rho0=1.0, p0=1.0, L=None):
     """Smooth positive periodic state: a few random Fourier modes on each primitive."""
     rng = np.random.default_rng(seed)

@@ -70,6 +70,8 @@ struct Geo {
     double nrm[3];   // adj(r,r) = face measure of an r-face
     double invJ;     // 1/J
     double aface;    // 1/(w_0 J)
+    int bcmask;      // boundary type (BC_*) of side (r, s) in bits 3(2r+s)..+2; 0: periodic box
+    const double *bcs;    // device [3][2][5]: outer states of inflow / specified-state sides
 };
 
 struct KArgs {
@@ -185,6 +187,16 @@ __device__ __forceinline__ int neighbor(const Geo &g, int e, int r, int side)
     int idx[3] = {e % g.n[0], (e / g.n[0]) % g.n[1], e / (g.n[0] * g.n[1])};
     idx[r] = side ? (idx[r] + 1 == g.n[r] ? 0 : idx[r] + 1) : (idx[r] == 0 ? g.n[r] - 1 : idx[r] - 1);
     return (idx[2] * g.n[1] + idx[1]) * g.n[0] + idx[0];
+}
+
+// Boundary type of element e's face (r, side), BC_PERIODIC for a face with
+// a neighbour.
+__device__ __forceinline__ int boundary_of(const Geo &g, int e, int r, int side)
+{
+    const int idx = r == 0 ? e % g.n[0] : r == 1 ? (e / g.n[0]) % g.n[1] : e / (g.n[0] * g.n[1]);
+    const int nr = r == 0 ? g.n[0] : r == 1 ? g.n[1] : g.n[2];
+    const bool at = side ? idx == nr - 1 : idx == 0;
+    return at ? (g.bcmask >> (3 * (2 * r + side))) & 7 : BC_PERIODIC;
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -305,7 +317,7 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB> struct RhsSmem {
 // QNS3: the reference's NS-3D zeta-partner defect (faithful NS in 3-D).
 // VT: element type of the viscous inputs from k_visc (float in the mixed form).
 // FB: forced workgroup size (0: the natural one of RCfg).
-template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB>
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false>
 __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const int slot,
                                          RhsSmem<DIM, NP, BLEND, QNS3, FB> &S)
 {
@@ -514,6 +526,22 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         double so[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) so[v] = sU[ej][v][no];
+        if constexpr (OPEN) {
+            // boundary point: -F_b for the outward normal at the interior
+            // node (BdrFaceIntegrator.cpp:61-123, applied after the interior
+            // faces by DGSEMNonlinearForm.cpp:776-826); its wave speed joins
+            // the running maximum (DGSEMOperator.cpp:585-589)
+            const int bt = boundary_of(A.geo, eo, r, side);
+            if (bt != BC_PERIODIC) {
+                double F[NV];
+                const double lb = boundary_flux<DIM>(bt, r, side ? nr : -nr, nr, so,
+                                                     A.geo.bcs + (2 * r + side) * 5, P, quirk3d, F);
+                lam = (lam < lb) ? lb : lam;
+#pragma unroll
+                for (int v = 0; v < NV; v++) sFc[ej][v][rem] = -F[v];
+                continue;
+            }
+        }
         double *s1 = side ? so : pn;
         double *s2 = side ? pn : so;
         double fvn[NV];
@@ -532,8 +560,8 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         const NodeQ<DIM> q1 = node_quants<DIM>(s1, P);
         const NodeQ<DIM> q2 = node_quants<DIM>(s2, P);
         double F[NV];
-        if (A.ec) two_point_flux<DIM, false>(r, nr, q1, q2, nullptr, nullptr, P, false, F);
-        else two_point_flux<DIM, true>(r, nr, q1, q2, s1, s2, P, quirk3d, F);
+        if (A.ec) two_point_flux<DIM, false>(r, nr, nr, q1, q2, nullptr, nullptr, P, false, F);
+        else two_point_flux<DIM, true>(r, nr, nr, q1, q2, s1, s2, P, quirk3d, F);
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             double d;
@@ -594,7 +622,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][na]; qb.v[d] = sQ[ej][d][nb]; }
                     qa.beta = sQ[ej][DIM][na]; qb.beta = sQ[ej][DIM][nb];
                     double F[NV];
-                    two_point_flux<DIM, false>(r, nr, qa, qb, nullptr, nullptr, P, false, F);
+                    two_point_flux<DIM, false>(r, nr, nr, qa, qb, nullptr, nullptr, P, false, F);
 #pragma unroll
                     for (int v = 0; v < NV; v++) sF[ej][v][lc * NPAIR + p] = F[v];
                     if (QNS3 && r == 1 && b == NP - 1)
@@ -626,7 +654,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     qa.ibeta = sQ[ej][NQ - 2][na]; qb.ibeta = sQ[ej][NQ - 2][nb];
                     qa.lam = sQ[ej][NQ - 1][na]; qb.lam = sQ[ej][NQ - 1][nb];
                     double F[NV];
-                    two_point_flux<DIM, true>(r, sSub[r][i], qa, qb, s1, s2, P, quirk3d, F);
+                    two_point_flux<DIM, true>(r, sSub[r][i], sSub[r][i], qa, qb, s1, s2, P, quirk3d, F);
 #pragma unroll
                     for (int v = 0; v < NV; v++) sF[ej][v][slot] = F[v];
                 }
@@ -699,12 +727,14 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     block_lam_max<BLOCK>(lam, lam_old, A.lam, sRed, slot);
 }
 
-template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB>
+// OPEN: the box has non-periodic sides (Euler, natural shape); the periodic
+// kernels carry no boundary code.
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false>
 __global__ void __launch_bounds__((RCfg<DIM, NP, BLEND, FB>::BLOCK))
 __attribute__((amdgpu_waves_per_eu(RCfg<DIM, NP, BLEND, FB>::WAVES))) k_rhs(const KArgs A)
 {
     __shared__ RhsSmem<DIM, NP, BLEND, QNS3, FB> S;
-    rhs_body<DIM, NP, BLEND, QNS3, VT, FB>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
+    rhs_body<DIM, NP, BLEND, QNS3, VT, FB, OPEN>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -1221,6 +1251,7 @@ struct prandtl_ctx {
     // SSP-RK3 step captured as a hipGraph (dt read from d_dt), replayed per step
     bool use_graph = true;
     double *d_dt = nullptr;
+    double *d_bcs = nullptr;       // [3][2][5] outer boundary states (open boxes only)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     const double *graph_u = nullptr;
@@ -1326,6 +1357,13 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
 {
     ProfScope ps(c, s, 0);
     const bool qns3 = DIM == 3 && c->prm.parabolic && c->prm.quirks;
+    if (c->geo.bcmask) {
+        // boundary conditions (Euler only), natural shape
+        using Cf = RCfg<DIM, NP, BLEND>;
+        const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+        hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+        return hip_check(c, hipGetLastError(), "k_rhs launch");
+    }
     if (qns3) {
         // the faithful NS-3D form runs in the natural shape only
         using Cf = RCfg<DIM, NP, BLEND>;
@@ -1424,6 +1462,7 @@ bool shape_ok(const prandtl_ctx *c, int kernel, int shape)
     if (kernel != PRANDTL_KERNEL_RHS) return false;
     if (shape == 0) return true;
     if (c->dim == 3 && c->prm.parabolic && c->prm.quirks) return false;
+    if (c->geo.bcmask) return false;
     return rhs_shape_ok(c->dim, c->Np, c->prm.blending != 0, shape);
 }
 
@@ -1471,6 +1510,27 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
         NE *= mesh->nelem[r];
     }
     if (NE > (1LL << 30)) return fail(nullptr, PRANDTL_ERR_ARG, "mesh too large");
+    bool open = false;
+    for (int r = 0; r < dim; r++) {
+        for (int side = 0; side < 2; side++) {
+            const int t = mesh->bc[r][side];
+            if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_SPECIFIED_STATE)
+                return fail(nullptr, PRANDTL_ERR_ARG, "unknown boundary type");
+            if (t == PRANDTL_BC_SUPERSONIC_INFLOW || t == PRANDTL_BC_SPECIFIED_STATE) {
+                const double *g = mesh->bc_state[r][side];
+                double ke = 0.0;
+                for (int d = 0; d < dim; d++) ke += g[1 + d] * g[1 + d];
+                const double pg = (prm->gamma - 1.0) * (g[dim + 1] - 0.5 * ke / g[0]);
+                if (!(g[0] > 0.0) || !(pg > 0.0) || !std::isfinite(pg))
+                    return fail(nullptr, PRANDTL_ERR_ARG, "boundary state needs positive density and pressure");
+            }
+        }
+        if ((mesh->bc[r][0] == PRANDTL_BC_PERIODIC) != (mesh->bc[r][1] == PRANDTL_BC_PERIODIC))
+            return fail(nullptr, PRANDTL_ERR_ARG, "a direction is periodic on both sides or on neither");
+        open = open || mesh->bc[r][0] != PRANDTL_BC_PERIODIC;
+    }
+    if (open && prm->parabolic)
+        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "boundary conditions are built for Euler (parabolic = 0)");
     prandtl_ctx *c = new (std::nothrow) prandtl_ctx();
     if (!c) return fail(nullptr, PRANDTL_ERR_NOMEM, "host allocation");
     c->dim = dim; c->order = N; c->Np = N + 1; c->nv = dim + 2;
@@ -1543,6 +1603,10 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     }
     c->geo.invJ = 1.0 / J;
     c->geo.aface = 1.0 / (w[0] * J);
+    c->geo.bcmask = 0;
+    c->geo.bcs = nullptr;
+    for (int r = 0; r < dim; r++)
+        for (int side = 0; side < 2; side++) c->geo.bcmask |= mesh->bc[r][side] << (3 * (2 * r + side));
     for (int i = 0; i < Np; i++) c->tab.fvs[i] = 1.0 / (J * w[i]);
     // subcell normals n_{i+1/2} = Ja + sum_{l<=i} w_l sum_q D_lq Ja (lines 1212-1233)
     for (int r = 0; r < dim; r++)
@@ -1584,6 +1648,13 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     if (e == hipSuccess) e = hipMalloc(&c->d_lam_out, sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_flag, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&c->d_dt, sizeof(double));
+    if (e == hipSuccess && c->geo.bcmask) {
+        double hb[3][2][5];
+        std::memcpy(hb, mesh->bc_state, sizeof(hb));
+        e = hipMalloc(&c->d_bcs, sizeof(hb));
+        if (e == hipSuccess) e = hipMemcpy(c->d_bcs, hb, sizeof(hb), hipMemcpyHostToDevice);
+        c->geo.bcs = c->d_bcs;
+    }
     if (e == hipSuccess) e = hipMemset(c->d_alpha, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_eta, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_lam, 0, sizeof(double) * c->NE);
@@ -1608,6 +1679,7 @@ void prandtl_destroy(prandtl_ctx *c)
     if (c->d_lam_out) (void)hipFree(c->d_lam_out);
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_dt) (void)hipFree(c->d_dt);
+    if (c->d_bcs) (void)hipFree(c->d_bcs);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -2196,6 +2268,9 @@ int prandtl_ensemble_create(int ncases, const prandtl_mesh_desc *meshes, const p
             return ens_fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "ensemble cases are 2-D with order 2..5");
         if (params[i].parabolic || params[i].ec_surface)
             return ens_fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "ensemble cases are Euler with the reference surface flux");
+        for (int r = 0; r < 2; r++)
+            if (meshes[i].bc[r][0] != PRANDTL_BC_PERIODIC || meshes[i].bc[r][1] != PRANDTL_BC_PERIODIC)
+                return ens_fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "ensemble cases are periodic boxes");
     }
     prandtl_ensemble *E = new (std::nothrow) prandtl_ensemble();
     if (!E) return ens_fail(nullptr, PRANDTL_ERR_NOMEM, "host allocation");

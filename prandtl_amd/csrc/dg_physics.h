@@ -95,14 +95,16 @@ __device__ __forceinline__ double inv_logmean(double x, double y)
     return log(y / x) / d;
 }
 
-// Chandrashekar two-point flux along direction r with normal magnitude nr.
-// DISS = false: ComputeVolumeFlux (ChandrashekarFlux.cpp:11-83), metric mean
-// of two equal metrics.  DISS = true: ComputeFaceFlux (lines 85-161) with the
-// conserved states s1, s2 for the jumps; quirk3d keeps the 3-D energy
-// dissipation term of line 131.  Only 1/beta_ln enters the fluxes, so the
-// beta log mean is formed as its reciprocal.
+// Chandrashekar two-point flux for the normal nr e_r (nr signed) whose
+// dissipation is scaled by nd = |nr| (the face normal's Norml2,
+// ChandrashekarFlux.cpp:88).  DISS = false: ComputeVolumeFlux
+// (ChandrashekarFlux.cpp:11-83), metric mean of two equal metrics.
+// DISS = true: ComputeFaceFlux (lines 85-161) with the conserved states s1,
+// s2 for the jumps; quirk3d keeps the 3-D energy dissipation term of line
+// 131.  Only 1/beta_ln enters the fluxes, so the beta log mean is formed as
+// its reciprocal.
 template <int DIM, bool DISS>
-__device__ __forceinline__ void two_point_flux(int r, double nr, const NodeQ<DIM> &a,
+__device__ __forceinline__ void two_point_flux(int r, double nr, double nd, const NodeQ<DIM> &a,
                                                const NodeQ<DIM> &b, const double *s1,
                                                const double *s2, const Phys &P, bool quirk3d,
                                                double *F)
@@ -145,11 +147,99 @@ __device__ __forceinline__ void two_point_flux(int r, double nr, const NodeQ<DIM
         diss += 0.5 * drho * P.gm1i * ibeta_ln + 0.5 * rho_mean * P.gm1i * (b.ibeta - a.ibeta);
         const double lam = (a.lam < b.lam) ? b.lam : a.lam;
         const double hl = 0.5 * lam;
-        F[0] = F[0] - hl * (s2[0] - s1[0]) * nr;
+        F[0] = F[0] - hl * (s2[0] - s1[0]) * nd;
 #pragma unroll
-        for (int d = 0; d < DIM; d++) F[1 + d] = F[1 + d] - hl * (s2[1 + d] - s1[1 + d]) * nr;
-        F[DIM + 1] = F[DIM + 1] - hl * diss * nr;
+        for (int d = 0; d < DIM; d++) F[1 + d] = F[1 + d] - hl * (s2[1 + d] - s1[1 + d]) * nd;
+        F[DIM + 1] = F[DIM + 1] - hl * diss * nd;
     }
+}
+
+// Boundary types (PRANDTL_BC_*)
+enum { BC_PERIODIC = 0, BC_SLIP = 1, BC_INFLOW = 2, BC_OUTFLOW = 3, BC_SYMMETRY = 4, BC_SPECIFIED = 5 };
+
+// SlipWallBdrFaceIntegrator::ComputeBdrFaceInviscidFlux
+// (SlipWallBdrFaceIntegrator.cpp:15-60) for the outward normal ns e_r.  The
+// unit normal is ns * (1/|ns|) (Normalize, BasicOperations.hpp:39-42);
+// RotateState (BasicOperations.cpp:238-259) of an axis-aligned normal
+// reduces to the component permutations below: its tangents Normal(n) and
+// n x Normal(n) have unit entries, whose products are exact.  p* is the
+// wall pressure of the shock (u_n > 0) or rarefaction (u_n <= 0, base
+// clipped at 1e-4) branch; F = (0, p* ns e_r, 0).  Returns the face-frame
+// |v| + a.
+template <int DIM>
+__device__ __forceinline__ double slip_wall_flux(int r, double ns, const double *s, const Phys &P,
+                                                 double *F)
+{
+    const double u = ns * (1.0 / fabs(ns));
+    double m[DIM];
+    if (DIM == 2) {
+        if (r == 0) { m[0] = s[1] * u; m[1] = s[2] * u; }
+        else { m[0] = s[2] * u; m[1] = -(s[1] * u); }
+    } else {
+        if (r == 0) { m[0] = s[1] * u; m[1] = s[2]; m[DIM - 1] = s[3] * u; }
+        else if (r == 1) { m[0] = s[2] * u; m[1] = -s[1]; m[DIM - 1] = s[3] * u; }
+        else { m[0] = s[3] * u; m[1] = -s[1]; m[DIM - 1] = -(s[2] * u); }
+    }
+    // Conserv2Prim (BasicOperations.cpp:172-186): velocities by division,
+    // the pressure's by the reciprocal (ComputePressure, lines 31-37)
+    const double rho = s[0];
+    const double ri = 1.0 / rho;
+    double vel[DIM], vv = 0.0, VV = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        vel[d] = m[d] / rho;
+        const double V = m[d] * ri;
+        VV = (d == 0) ? V * V : VV + V * V;
+        vv = (d == 0) ? vel[d] * vel[d] : vv + vel[d] * vel[d];
+    }
+    const double p = P.gm1 * (s[DIM + 1] - 0.5 * rho * VV);
+    const double an = sqrt(P.gamma * p / rho);
+    const double un = vel[0];
+    const double gp1 = 1.0 + P.gamma, gp1i = 1.0 / gp1;
+    double p_star;
+    if (un > 0.0)
+        p_star = p + 0.25 * un * gp1 * rho * (un + sqrt(un * un + 8.0 * gp1i / rho * p * (P.gm1 * gp1i + 1.0)));
+    else {
+        double base = 1.0 + 0.5 * P.gm1 * un / an;
+        base = (base < 0.0001) ? 0.0001 : base;
+        p_star = p * pow(base, 2.0 * P.gamma * P.gm1i);
+    }
+#pragma unroll
+    for (int v = 0; v < DIM + 2; v++) F[v] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++)
+        if (d == r) F[1 + d] = p_star * ns;
+    return sqrt(vv) + an;
+}
+
+// ComputeBdrFaceInviscidFlux of a boundary point (BdrFaceIntegrator.cpp:
+// 252-258): the outer state of ComputeOuterInviscidState and the face flux
+// with state1 = interior s, normal ns e_r (ns signed outward, nd = |ns|).
+// Symmetry (SymmetryBdrFaceIntegrator.cpp) mirrors the normal momentum with
+// the unit normal u: m_r - 2 (m_r u) u.  Slip walls replace the flux.
+// Returns the point's wave speed.
+template <int DIM>
+__device__ __forceinline__ double boundary_flux(int type, int r, double ns, double nd, const double *s,
+                                                const double *outer, const Phys &P, bool quirk3d,
+                                                double *F)
+{
+    if (type == BC_SLIP) return slip_wall_flux<DIM>(r, ns, s, P, F);
+    double g[DIM + 2];
+#pragma unroll
+    for (int v = 0; v < DIM + 2; v++) g[v] = (type == BC_INFLOW || type == BC_SPECIFIED) ? outer[v] : s[v];
+    if (type == BC_SYMMETRY) {
+        const double u = ns * (1.0 / nd);
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            if (d == r) {
+                const double mn = s[1 + d] * u;
+                g[1 + d] = s[1 + d] + -2.0 * mn * u;
+            }
+    }
+    const NodeQ<DIM> q1 = node_quants<DIM>(s, P);
+    const NodeQ<DIM> q2 = node_quants<DIM>(g, P);
+    two_point_flux<DIM, true>(r, ns, nd, q1, q2, s, g, P, quirk3d, F);
+    return (q1.lam < q2.lam) ? q2.lam : q1.lam;
 }
 
 // Conserv2Entropy, BasicOperations.cpp:81-104 (beta = rho/p)

@@ -30,16 +30,33 @@ def lgl_nodes(order):
     return x, w
 
 
+def _bc_code(kind):
+    if isinstance(kind, str):
+        if kind not in _lib.BC_TYPES:
+            raise ValueError("unknown boundary type %r (one of %s)" % (kind, ", ".join(_lib.BC_TYPES)))
+        return _lib.BC_TYPES[kind]
+    return int(kind)
+
+
 def _descs(dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0, parabolic=False, blending=False,
            alpha_max=0.5, alpha_min=0.001, ec_surface=False, quirks=True, precision="fp64",
-           sutherland=False, R_gas=287.05):
-    """prandtl_mesh_desc and prandtl_params for one case."""
+           sutherland=False, R_gas=287.05, bc=None, bc_state=None):
+    """prandtl_mesh_desc and prandtl_params for one case.  bc: per direction
+    a (low, high) pair of boundary types by name ("slip", "symmetry",
+    "supersonic-inflow", "supersonic-outflow", "specified-state",
+    "periodic"), None for a periodic box; bc_state: per direction a
+    (low, high) pair of conservative outer states (None where unused)."""
     m = _lib.MeshDesc()
     m.dim, m.order = dim, order
     for r in range(3):
         m.nelem[r] = int(nelem[r]) if r < dim else 1
         m.lo[r] = float(lo[r]) if r < dim else 0.0
         m.hi[r] = float(hi[r]) if r < dim else 1.0
+        for side in range(2):
+            m.bc[r][side] = _bc_code(bc[r][side]) if (bc is not None and r < dim) else 0
+            st = bc_state[r][side] if (bc_state is not None and r < dim) else None
+            for v in range(5):
+                m.bc_state[r][side][v] = float(st[v]) if (st is not None and v < dim + 2) else 0.0
     p = _lib.Params()
     p.gamma, p.Pr, p.mu = float(gamma), float(Pr), float(mu)
     p.alpha_max, p.alpha_min = float(alpha_max), float(alpha_min)
@@ -85,13 +102,15 @@ def _stream_ptr():
 
 
 class DGSEMOperator:
-    """One GPU's DGSEM semi-discrete operator on a periodic Cartesian box."""
+    """One GPU's DGSEM semi-discrete operator on a Cartesian box, periodic or
+    with boundary conditions on its sides (bc, bc_state: see _descs)."""
 
     def __init__(self, dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0,
                  parabolic=False, blending=False, alpha_max=0.5, alpha_min=0.001,
-                 ec_surface=False, quirks=True, precision="fp64", sutherland=False, R_gas=287.05):
+                 ec_surface=False, quirks=True, precision="fp64", sutherland=False, R_gas=287.05,
+                 bc=None, bc_state=None):
         m, p = _descs(dim, order, nelem, lo, hi, gamma, Pr, mu, parabolic, blending, alpha_max,
-                      alpha_min, ec_surface, quirks, precision, sutherland, R_gas)
+                      alpha_min, ec_surface, quirks, precision, sutherland, R_gas, bc, bc_state)
         h = ctypes.c_void_p()
         rc = lib().prandtl_create(ctypes.byref(m), ctypes.byref(p), ctypes.byref(h))
         check(rc, None)

@@ -25,7 +25,30 @@ class OrcParams(ctypes.Structure):
         ("alpha_max", ctypes.c_double), ("alpha_min", ctypes.c_double),
         ("ec_surface", ctypes.c_int), ("quirks", ctypes.c_int), ("threads", ctypes.c_int),
         ("sutherland", ctypes.c_int), ("R_gas", ctypes.c_double),
+        ("bc", (ctypes.c_int * 2) * 3), ("bc_state", ((ctypes.c_double * 5) * 2) * 3),
     ]
+
+
+# boundary types by the names of the reference's configuration files
+# (Simulation.cpp:518-735); the values are the oracle's ORC_BC_* codes
+BC_CODES = {"periodic": 0, "slip": 1, "supersonic-inflow": 2, "supersonic-outflow": 3,
+            "symmetry": 4, "specified-state": 5}
+
+
+def bc_code(x):
+    return BC_CODES[x] if isinstance(x, str) else int(x)
+
+
+def set_bcs(p, dim, bc, bc_state):
+    """bc: per direction a (low, high) pair of boundary types (None: all
+    periodic); bc_state: per direction a (low, high) pair of outer states
+    (None where unused)."""
+    for r in range(3):
+        for side in range(2):
+            p.bc[r][side] = bc_code(bc[r][side]) if (bc is not None and r < dim) else 0
+            st = bc_state[r][side] if (bc_state is not None and r < dim) else None
+            for v in range(5):
+                p.bc_state[r][side][v] = float(st[v]) if (st is not None and v < dim + 2) else 0.0
 
 
 def lib():
@@ -41,6 +64,8 @@ def lib():
         L.orc_volume_flux.argtypes = [ctypes.c_int, _D, _P, _P, _P, _P, _P]
         L.orc_face_flux.restype = _D
         L.orc_face_flux.argtypes = [ctypes.c_int, _D, ctypes.c_int, _P, _P, _P, _P]
+        L.orc_bc_flux.restype = _D
+        L.orc_bc_flux.argtypes = [ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
         L.orc_cons2entropy.argtypes = [ctypes.c_int, _D, _P, _P]
         L.orc_entgrad2primgrad.argtypes = [ctypes.c_int, _D, ctypes.c_int, _P, _P]
         L.orc_viscous_flux.argtypes = [ctypes.c_int, _D, _D, _D, _D, _P, _P, _P, _P, _P]
@@ -111,6 +136,16 @@ def face_flux(dim, gamma, quirks, s1, s2, nor):
     return F, lam
 
 
+def bc_flux(dim, gamma, quirks, kind, s1, nor, outer=None):
+    """Boundary flux of type `kind` (name or code) for the interior state s1
+    and the outward face normal nor; returns (F, wave speed)."""
+    F = np.zeros(dim + 2)
+    o = np.zeros(dim + 2) if outer is None else np.ascontiguousarray(outer, float)
+    lam = lib().orc_bc_flux(dim, gamma, quirks, bc_code(kind), ptr(np.ascontiguousarray(s1, float)),
+                            ptr(o), ptr(np.ascontiguousarray(nor, float)), ptr(F))
+    return F, lam
+
+
 def cons2entropy(dim, gamma, s):
     w = np.zeros(dim + 2)
     lib().orc_cons2entropy(dim, gamma, ptr(np.ascontiguousarray(s, float)), ptr(w))
@@ -137,7 +172,8 @@ class Oracle:
 
     def __init__(self, dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0,
                  parabolic=False, blending=False, alpha_max=0.5, alpha_min=0.001,
-                 ec_surface=False, quirks=True, threads=1, sutherland=False, R_gas=287.05):
+                 ec_surface=False, quirks=True, threads=1, sutherland=False, R_gas=287.05,
+                 bc=None, bc_state=None):
         p = OrcParams()
         p.dim, p.order = dim, order
         for r in range(3):
@@ -149,6 +185,7 @@ class Oracle:
         p.alpha_max, p.alpha_min = alpha_max, alpha_min
         p.ec_surface, p.quirks, p.threads = int(ec_surface), int(quirks), int(threads)
         p.sutherland, p.R_gas = int(sutherland), float(R_gas)
+        set_bcs(p, dim, bc, bc_state)
         self.params = p
         h = ctypes.c_void_p()
         rc = lib().orc_create(ctypes.byref(p), ctypes.byref(h))

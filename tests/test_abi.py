@@ -44,6 +44,11 @@ def _create(mesh_kw, prm_kw):
     for r in range(3):
         m.nelem[r] = mesh_kw.get("nelem", (4, 4, 4))[r]
         m.lo[r], m.hi[r] = 0.0, mesh_kw.get("hi", 1.0)
+        for side in range(2):
+            m.bc[r][side] = mesh_kw.get("bc", ((0, 0),) * 3)[r][side]
+            st = mesh_kw.get("bc_state", (1.0, 0.0, 0.0, 2.5, 0.0))
+            for v in range(5):
+                m.bc_state[r][side][v] = st[v]
     p = _lib.Params()
     p.gamma, p.Pr, p.mu = prm_kw.get("gamma", 1.4), 0.72, 0.0
     p.alpha_max, p.alpha_min = 0.5, 0.001
@@ -64,6 +69,13 @@ def _create(mesh_kw, prm_kw):
     (dict(hi=-1.0), {}, _lib.PRANDTL_ERR_ARG),
     ({}, dict(gamma=1.0), _lib.PRANDTL_ERR_ARG),
     ({}, dict(precision=7), _lib.PRANDTL_ERR_UNSUPPORTED),
+    # boundary tables: one periodic side, unknown type, Navier-Stokes walls,
+    # an outer state with negative pressure or density
+    (dict(bc=((1, 0), (0, 0), (0, 0))), {}, _lib.PRANDTL_ERR_ARG),
+    (dict(bc=((7, 7), (0, 0), (0, 0))), {}, _lib.PRANDTL_ERR_ARG),
+    (dict(bc=((1, 1), (0, 0), (0, 0))), dict(parabolic=1), _lib.PRANDTL_ERR_UNSUPPORTED),
+    (dict(bc=((2, 3), (0, 0), (0, 0)), bc_state=(1.0, 3.0, 0.0, 1.0, 0.0)), {}, _lib.PRANDTL_ERR_ARG),
+    (dict(bc=((1, 1), (5, 5), (0, 0)), bc_state=(-1.0, 0.0, 0.0, 2.5, 0.0)), {}, _lib.PRANDTL_ERR_ARG),
 ])
 def test_create_rejects(mesh_kw, prm_kw, code):
     rc, h = _create(mesh_kw, prm_kw)
@@ -91,6 +103,8 @@ def _ensemble_create(case_kw):
         for r in range(3):
             m.nelem[r] = 4
             m.lo[r], m.hi[r] = 0.0, 1.0
+            for side in range(2):
+                m.bc[r][side] = kw.get("bc", 0) if r == 0 else 0
         p.gamma, p.Pr, p.mu = 1.4, 0.72, 0.0
         p.alpha_max, p.alpha_min = 0.5, 0.001
         p.parabolic = kw.get("parabolic", 0)
@@ -107,6 +121,7 @@ def _ensemble_create(case_kw):
     ([dict(order=1)], _lib.PRANDTL_ERR_UNSUPPORTED),
     ([{}, dict(parabolic=1)], _lib.PRANDTL_ERR_UNSUPPORTED),
     ([dict(ec_surface=1)], _lib.PRANDTL_ERR_UNSUPPORTED),
+    ([{}, dict(bc=1)], _lib.PRANDTL_ERR_UNSUPPORTED),
     ([{}] * 257, _lib.PRANDTL_ERR_ARG),
 ])
 def test_ensemble_create_rejects(case_kw, code):
