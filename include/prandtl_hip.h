@@ -93,8 +93,66 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *params,
                    prandtl_ctx **out);
 void prandtl_destroy(prandtl_ctx *ctx);
 
+/* ---- Unstructured 2-D quadrilateral meshes ----
+ * The mesh the reference reads with mfem::Mesh(mesh_file) (Simulation.cpp:
+ * 271-299) and its geometry: per-node Jacobians and adjugates of the
+ * bilinear element map (used at DGSEMIntegrator.cpp:255-267), face normals
+ * CalcOrtho(face Jacobian) outward of the face's first element
+ * (DGSEMIntegrator.cpp:186, BdrFaceIntegrator.cpp:94), and the two
+ * elements' face-point correspondence.  Elements are straight-sided quads
+ * with vertices counterclockwise; element e's nodes are lexicographic in its
+ * own frame (xi from vertex 0 to 1, eta from vertex 0 to 3). */
+typedef struct prandtl_quad_mesh {
+    int nvert;
+    const double *vert;         /* [nvert][2] vertex coordinates */
+    int nelem;
+    const int *elem;            /* [nelem][4] vertex indices, counterclockwise */
+    const double *elem_coords;  /* NULL, or [nelem][4][2] corner coordinates per
+                                   element (periodic meshes: MFEM's L2 mesh nodes;
+                                   topology then comes from elem) */
+    int nbdr;
+    const int *bdr;             /* [nbdr][2] boundary segments (vertex indices) */
+    const int *bdr_attr;        /* [nbdr] boundary attribute, 1..nattr */
+    int nattr;
+    const int *attr_bc;         /* [nattr] PRANDTL_BC_* of each attribute */
+    const double *attr_state;   /* [nattr][4] conservative outer states of the
+                                   inflow / specified-state attributes (may be NULL
+                                   if none uses one) */
+} prandtl_quad_mesh;
+
+/* DGSEMOperator constructor on a quadrilateral mesh (orders 2..5, Euler,
+ * with or without blending).  Every element edge must be shared by two
+ * elements or be a boundary segment whose attribute has a non-periodic
+ * type.  Uses the current device. */
+int prandtl_create_quad(const prandtl_quad_mesh *mesh, int order, const prandtl_params *params,
+                        prandtl_ctx **out);
+
+/* Mesh files: mfem::Mesh's readers for "MFEM mesh v1.0" (vertices, or L2_T1
+ * first-order nodes for periodic meshes) and Gmsh 2.2 ASCII ($PhysicalNames,
+ * $Nodes, $Elements with 4-node quadrilaterals and 2-node boundary lines;
+ * attributes are the physical tags).  Elements keep file order.  Elements
+ * with clockwise vertices are reoriented.  _refine splits every element into
+ * four (children of element i at 4i..4i+3, corners in the parent's frame
+ * (0,0), (1,0), (1,1), (0,1)), as ser_ref_levels does with
+ * Mesh::UniformRefinement (Simulation.cpp:293-300).  _view fills the
+ * geometry fields of a prandtl_quad_mesh (pointers into the file object;
+ * the caller sets nattr / attr_bc / attr_state).  _name returns the name of
+ * physical group i (0-based) and its attribute, NULL past the end. */
+typedef struct prandtl_mesh_file prandtl_mesh_file;
+int prandtl_mesh_read(const char *path, prandtl_mesh_file **out);
+int prandtl_mesh_refine(prandtl_mesh_file *m);
+int prandtl_mesh_view(const prandtl_mesh_file *m, prandtl_quad_mesh *view);
+const char *prandtl_mesh_name(const prandtl_mesh_file *m, int i, int *attr);
+void prandtl_mesh_free(prandtl_mesh_file *m);
+const char *prandtl_mesh_last_error(void);
+
 /* NE, K (nodes/element), nv (variables) */
 int prandtl_sizes(const prandtl_ctx *ctx, int *NE, int *K, int *nv);
+
+/* the smallest element size h_min of the dt policy: min over elements of
+ * GetElementSize(e, 1), the Jacobian's smaller singular value at the element
+ * centre (Simulation.cpp:896-901); min_r (hi-lo)/nelem on a box */
+int prandtl_hmin(const prandtl_ctx *ctx, double *hmin);
 
 /* DGSEMOperator::Mult(const Vector &u, Vector &dudt) const
  * (DGSEMOperator.hpp:105, DGSEMOperator.cpp:537-591): dudt is overwritten

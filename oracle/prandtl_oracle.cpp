@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <new>
 #include <vector>
 #ifdef _OPENMP
@@ -430,6 +431,7 @@ T bc_flux(int dim, double gamma, int quirks, int type, const T *s1, const T *out
 {
     const int nv = dim + 2;
     T s2[5];
+    for (int v = 0; v < nv; v++) s2[v] = s1[v];
     switch (type) {
     case ORC_BC_SLIP:
         return slip_wall_flux(dim, gamma, s1, nor, F);
@@ -583,6 +585,13 @@ struct Mesh {
      * each non-periodic direction r; their faces' elem1 is that element */
     std::vector<int> bel[3][2];
     double ad[3];                    /* face measure of an r-face */
+    /* quadrilateral meshes: node coordinates, and the boundary faces
+     * (element, type, attribute, face nodes and outward normals per point) */
+    bool quad = false;
+    std::vector<double> X;           /* [2][NE*K] */
+    std::vector<int> bq_e, bq_type, bq_attr, bq_n;
+    std::vector<double> bq_nor;
+    std::vector<double> attr_state; /* [nattr][5] */
 };
 
 inline int line_node(int dim, int Np, int r, int l, int m)
@@ -610,6 +619,8 @@ struct orc_ctx {
 };
 
 namespace {
+
+void build_subcells(orc_ctx *c);
 
 void build_mesh(orc_ctx *c)
 {
@@ -667,8 +678,15 @@ void build_mesh(orc_ctx *c)
                 m.fnor[((size_t)f * Nfp + q) * dim + r] = ad[r];
             }
         }
-    /* subcell normals, DGSEMIntegrator.cpp:1199-1304:
-     * n_{m+1/2} = Ja^r(line start) + sum_{l<=m} w_l sum_q D_{lq} Ja^r_q */
+    build_subcells(c);
+}
+
+/* subcell normals, DGSEMIntegrator.cpp:1199-1304:
+ * n_{m+1/2} = Ja^r(line start) + sum_{l<=m} w_l sum_q D_{lq} Ja^r_q */
+void build_subcells(orc_ctx *c)
+{
+    Mesh &m = c->m;
+    const int dim = m.dim, K = m.K, NE = m.NE, Np = m.Np, N = m.N, Nfp = m.Nfp;
     if (c->p.blending) {
         m.sub.assign((size_t)NE * dim * Nfp * N * dim, 0.0);
         for (int e = 0; e < NE; e++)
@@ -693,6 +711,144 @@ void build_mesh(orc_ctx *c)
                     }
                 }
     }
+}
+
+/* Quadrilateral mesh geometry, restating what mfem::Mesh and its element
+ * transformations give the reference (DGSEMIntegrator.cpp:251-267, 186;
+ * BdrFaceIntegrator.cpp:79-95):
+ *  - node (i, j) of element e sits at (xi_i, eta_j) of the bilinear map of
+ *    its four corners; the Jacobian there is the sum over corners of corner
+ *    times the bilinear shape derivative (BiLinear2DFiniteElement), its
+ *    adjugate rows are the contravariant metrics, its determinant J;
+ *  - an edge shared by two elements is one face, its first element the one
+ *    with the lower index (faces are created in element order); the normal
+ *    at every point is CalcOrtho of the first element's edge Jacobian,
+ *    (dy, -dx) along its counterclockwise traversal, outward of it;
+ *  - an edge that no other element shares is a boundary face of the
+ *    segment with the same two vertices.
+ * Returns 0, or 8 for a mesh that does not close. */
+int build_mesh_quad(orc_ctx *c, const orc_quad_mesh *q)
+{
+    Mesh &m = c->m;
+    m.quad = true;
+    m.dim = 2; m.N = c->p.order; m.Np = m.N + 1; m.nv = 4;
+    m.K = m.Np * m.Np; m.Nfp = m.Np;
+    m.NE = q->nelem;
+    m.n[0] = m.NE; m.n[1] = m.n[2] = 1;
+    const int K = m.K, NE = m.NE, Np = m.Np, N = m.N, Nfp = m.Nfp;
+    const size_t NK = (size_t)NE * K;
+    auto cornerx = [&](int e, int k, int i) {
+        return q->elem_coords ? q->elem_coords[(size_t)e * 8 + 2 * k + i]
+                              : q->vert[2 * (size_t)q->elem[(size_t)e * 4 + k] + i];
+    };
+    m.J.assign(NK, 0.0);
+    m.adj.assign(NK * 4, 0.0);
+    m.X.assign(2 * NK, 0.0);
+    m.hmin = 1e300;
+    for (int e = 0; e < NE; e++) {
+        for (int j = 0; j < Np; j++)
+            for (int i = 0; i < Np; i++) {
+                const double xi = c->x[i], eta = c->x[j];
+                /* shape N_k and derivatives (dN/dxi, dN/deta), corners in vertex order */
+                const double Nk[4] = {(1 - xi) * (1 - eta), xi * (1 - eta), xi * eta, (1 - xi) * eta};
+                const double dN[4][2] = {{eta - 1, xi - 1}, {1 - eta, -xi}, {eta, xi}, {-eta, 1 - xi}};
+                double Jm[2][2];
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++) {
+                        double sum = 0.0;
+                        for (int k = 0; k < 4; k++) sum += dN[k][b] * cornerx(e, k, a);
+                        Jm[a][b] = sum;
+                    }
+                const size_t nd = (size_t)e * K + j * Np + i;
+                for (int a = 0; a < 2; a++) {
+                    double x = 0.0;
+                    for (int k = 0; k < 4; k++) x += Nk[k] * cornerx(e, k, a);
+                    m.X[a * NK + nd] = x;
+                }
+                /* adjugate: row 0 = (J11, -J01), row 1 = (-J10, J00) */
+                double *ad = &m.adj[nd * 4];
+                ad[0] = Jm[1][1]; ad[1] = -Jm[0][1];
+                ad[2] = -Jm[1][0]; ad[3] = Jm[0][0];
+                m.J[nd] = Jm[0][0] * Jm[1][1] - Jm[1][0] * Jm[0][1];
+                if (!(m.J[nd] > 0.0)) return 8;
+            }
+        /* GetElementSize(e, 1): smaller singular value of J at the centre */
+        double Jc[2][2];
+        const double dN[4][2] = {{-0.5, -0.5}, {0.5, -0.5}, {0.5, 0.5}, {-0.5, 0.5}};
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                double sum = 0.0;
+                for (int k = 0; k < 4; k++) sum += dN[k][b] * cornerx(e, k, a);
+                Jc[a][b] = sum;
+            }
+        const double S = Jc[0][0] * Jc[0][0] + Jc[0][1] * Jc[0][1] + Jc[1][0] * Jc[1][0] + Jc[1][1] * Jc[1][1];
+        const double det = std::fabs(Jc[0][0] * Jc[1][1] - Jc[1][0] * Jc[0][1]);
+        const double smin = std::sqrt(0.5 * (S - std::sqrt(std::max(S * S - 4 * det * det, 0.0))));
+        m.hmin = std::min(m.hmin, smin);
+    }
+    /* edges of local face f = 2r + side in increasing-l order, and the
+     * counterclockwise traversal of the same edge */
+    const int fv[4][2] = {{0, 3}, {1, 2}, {0, 1}, {3, 2}};
+    const int ccw[4][2] = {{3, 0}, {1, 2}, {0, 1}, {2, 3}};
+    auto vtx = [&](int e, int k) { return q->elem[(size_t)e * 4 + k]; };
+    std::map<std::pair<int, int>, std::vector<std::pair<int, int>>> edges;
+    for (int e = 0; e < NE; e++)
+        for (int f = 0; f < 4; f++) {
+            int a = vtx(e, fv[f][0]), b = vtx(e, fv[f][1]);
+            edges[{std::min(a, b), std::max(a, b)}].push_back({e, f});
+        }
+    std::map<std::pair<int, int>, int> seg;
+    for (int i = 0; i < q->nbdr; i++) {
+        int a = q->bdr[2 * i], b = q->bdr[2 * i + 1];
+        seg[{std::min(a, b), std::max(a, b)}] = q->bdr_attr[i];
+    }
+    m.fe1.clear(); m.fe2.clear(); m.fn1.clear(); m.fn2.clear(); m.fnor.clear();
+    for (auto &kv : edges) {
+        const auto &lst = kv.second;
+        if (lst.size() > 2) return 8;
+        int e1 = lst[0].first, f1 = lst[0].second;
+        if (lst.size() == 2 && lst[1].first < e1) { e1 = lst[1].first; f1 = lst[1].second; }
+        const double px = cornerx(e1, ccw[f1][0], 0), py = cornerx(e1, ccw[f1][0], 1);
+        const double qx = cornerx(e1, ccw[f1][1], 0), qy = cornerx(e1, ccw[f1][1], 1);
+        const double nx = qy - py, ny = -(qx - px);
+        const int r1 = f1 >> 1, s1 = f1 & 1;
+        if (lst.size() == 2) {
+            const int e2 = (e1 == lst[0].first && f1 == lst[0].second) ? lst[1].first : lst[0].first;
+            const int f2 = (e1 == lst[0].first && f1 == lst[0].second) ? lst[1].second : lst[0].second;
+            const int r2 = f2 >> 1, s2 = f2 & 1;
+            const bool same = vtx(e1, fv[f1][0]) == vtx(e2, fv[f2][0]);
+            m.fe1.push_back(e1);
+            m.fe2.push_back(e2);
+            for (int l = 0; l < Nfp; l++) {
+                m.fn1.push_back(line_node(2, Np, r1, l, s1 ? N : 0));
+                m.fn2.push_back(line_node(2, Np, r2, same ? l : N - l, s2 ? N : 0));
+                m.fnor.push_back(nx);
+                m.fnor.push_back(ny);
+            }
+        } else {
+            auto it = seg.find(kv.first);
+            if (it == seg.end()) return 8;
+            const int attr = it->second;
+            if (attr < 1 || attr > q->nattr) return 8;
+            const int type = q->attr_bc[attr - 1];
+            if (type < ORC_BC_SLIP || type > ORC_BC_SPECIFIED_STATE) return 8;
+            m.bq_e.push_back(e1);
+            m.bq_type.push_back(type);
+            m.bq_attr.push_back(attr - 1);
+            for (int l = 0; l < Nfp; l++) {
+                m.bq_n.push_back(line_node(2, Np, r1, l, s1 ? N : 0));
+                m.bq_nor.push_back(nx);
+                m.bq_nor.push_back(ny);
+            }
+        }
+    }
+    m.nf = (int)m.fe1.size();
+    m.attr_state.assign(5 * (size_t)std::max(q->nattr, 1), 0.0);
+    for (int a = 0; a < q->nattr; a++)
+        if (q->attr_state)
+            for (int v = 0; v < 4; v++) m.attr_state[5 * (size_t)a + v] = q->attr_state[4 * (size_t)a + v];
+    build_subcells(c);
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1072,6 +1228,31 @@ template <class T> struct Engine {
         return lam;
     }
 
+    /* boundary faces of a quadrilateral mesh, [b0, b1) of its list: as
+     * bfaces, with each face's own normal and attribute */
+    T bfaces_quad(const T *u, int b0, int b1, T *dudt, T lam)
+    {
+        const double w0 = c->w[0];
+        for (int b = b0; b < b1; b++) {
+            const int e = m.bq_e[b];
+            T outer[5];
+            for (int v = 0; v < nv; v++) outer[v] = m.attr_state[5 * (size_t)m.bq_attr[b] + v];
+            for (int q = 0; q < Nfp; q++) {
+                const int n1 = m.bq_n[(size_t)b * Nfp + q];
+                T s1[5], F[5], nor[2];
+                state(u, e, n1, s1);
+                nor[0] = m.bq_nor[((size_t)b * Nfp + q) * 2];
+                nor[1] = m.bq_nor[((size_t)b * Nfp + q) * 2 + 1];
+                g_cat = CAT_FACE;
+                lam = mmax(lam, bc_flux(dim, gamma, c->p.quirks, m.bq_type[b], s1, outer, nor, F));
+                g_cat = CAT_ASM;
+                T a1 = 1.0 / (w0 * T(m.J[(size_t)e * K + n1]));
+                for (int v = 0; v < nv; v++) dudt[v * NK + (size_t)e * K + n1] += a1 * (-F[v]);
+            }
+        }
+        return lam;
+    }
+
     /* DGSEMOperator::Mult (DGSEMOperator.cpp:537-591) with
      * DGSEMNonlinearForm::Mult (DGSEMNonlinearForm.cpp:647-849, 1302-1545):
      * dudt is overwritten. */
@@ -1092,8 +1273,18 @@ template <class T> struct Engine {
 #pragma omp for schedule(static)
 #endif
             for (int e = 0; e < NE; e++) l = element(u, e, dudt, l, Fi, Fv, el);
+            if (m.quad) {
+                /* faces of a general mesh share corner nodes: one thread */
+#ifdef _OPENMP
+#pragma omp single
+#endif
+                {
+                    l = faces(u, 0, m.nf, dudt, l);
+                    l = bfaces_quad(u, 0, (int)m.bq_e.size(), dudt, l);
+                }
+            }
             /* faces of one direction touch disjoint nodes of each element */
-            for (int r = 0; r < dim; r++) {
+            for (int r = 0; r < dim && !m.quad; r++) {
 #ifdef _OPENMP
 #pragma omp for schedule(static)
 #endif
@@ -1101,7 +1292,7 @@ template <class T> struct Engine {
             }
             /* boundary faces after the interior ones (DGSEMNonlinearForm.cpp:776-826);
              * the faces of one side touch disjoint nodes */
-            for (int r = 0; r < dim; r++)
+            for (int r = 0; r < dim && !m.quad; r++)
                 for (int side = 0; side < 2; side++) {
                     const int nb = (int)m.bel[r][side].size();
 #ifdef _OPENMP
@@ -1216,14 +1407,24 @@ double orc_bc_flux(int dim, double gamma, int quirks, int type, const double *s1
     return bc_flux<double>(dim, gamma, quirks, type, s1, outer, nor, F);
 }
 
-int orc_create(const orc_params *p, orc_ctx **out)
+static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **out);
+
+int orc_create(const orc_params *p, orc_ctx **out) { return create_impl(p, nullptr, out); }
+
+int orc_create_quad(const orc_params *p, const orc_quad_mesh *m, orc_ctx **out)
+{
+    if (!p || !m || !out || p->dim != 2 || p->parabolic) return 1;
+    return create_impl(p, m, out);
+}
+
+static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **out)
 {
     if (!p || !out) return 1;
     if (p->dim < 2 || p->dim > 3 || p->order < 1 || p->order > 12) return 2;
-    for (int r = 0; r < p->dim; r++)
+    for (int r = 0; r < p->dim && !qm; r++)
         if (p->nelem[r] < 1 || !(p->hi[r] > p->lo[r])) return 3;
     if (p->sutherland && !(p->R_gas > 0.0)) return 5;
-    for (int r = 0; r < p->dim; r++) {
+    for (int r = 0; r < p->dim && !qm; r++) {
         for (int side = 0; side < 2; side++)
             if (p->bc[r][side] < ORC_BC_PERIODIC || p->bc[r][side] > ORC_BC_SPECIFIED_STATE) return 6;
         if ((p->bc[r][0] == ORC_BC_PERIODIC) != (p->bc[r][1] == ORC_BC_PERIODIC)) return 6;
@@ -1236,7 +1437,14 @@ int orc_create(const orc_params *p, orc_ctx **out)
     c->p = *p;
     lgl_rule(p->order, c->x, c->w);
     dmats(p->order, c->x, c->w, c->D, c->Dh, c->Dh2);
-    build_mesh(c);
+    if (qm) {
+        if (build_mesh_quad(c, qm) != 0) {
+            delete c;
+            return 8;
+        }
+    } else {
+        build_mesh(c);
+    }
     const Mesh &m = c->m;
     c->alpha.assign(m.NE, 0.0);
     c->eta.assign(m.NE, 0.0);
@@ -1287,6 +1495,10 @@ int orc_coords(const orc_ctx *c, double *X)
 {
     const Mesh &m = c->m;
     const size_t NK = (size_t)m.NE * m.K;
+    if (m.quad) {
+        std::memcpy(X, m.X.data(), m.X.size() * sizeof(double));
+        return 0;
+    }
     for (int e = 0; e < m.NE; e++) {
         int idx[3] = {e % m.n[0], (e / m.n[0]) % m.n[1], e / (m.n[0] * m.n[1])};
         for (int i = 0; i < m.K; i++) {

@@ -76,6 +76,28 @@ double orc_bc_flux(int dim, double gamma, int quirks, int type, const double *s1
 
 /* operator */
 int orc_create(const orc_params *p, orc_ctx **out);
+
+/* 2-D quadrilateral mesh: straight-sided elements, vertices counterclockwise,
+ * element e's nodes lexicographic in its own frame (xi from vertex 0 to 1,
+ * eta from vertex 0 to 3).  elem_coords (optional) gives per-element corner
+ * coordinates for periodic meshes, topology then from elem.  Boundary
+ * segments carry attributes 1..nattr, whose ORC_BC_* type and outer state
+ * (4 values) are attr_bc / attr_state. */
+typedef struct orc_quad_mesh {
+    int nvert;
+    const double *vert;
+    int nelem;
+    const int *elem;
+    const double *elem_coords;
+    int nbdr;
+    const int *bdr;
+    const int *bdr_attr;
+    int nattr;
+    const int *attr_bc;
+    const double *attr_state;
+} orc_quad_mesh;
+/* p->dim must be 2; p->nelem / lo / hi / bc are ignored */
+int orc_create_quad(const orc_params *p, const orc_quad_mesh *m, orc_ctx **out);
 void orc_destroy(orc_ctx *c);
 int orc_sizes(const orc_ctx *c, int *NE, int *K, int *nv);
 int orc_coords(const orc_ctx *c, double *X /* [dim][NE][K] */);

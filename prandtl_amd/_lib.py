@@ -30,6 +30,8 @@ EXPORTED = (
     "prandtl_ensemble_create", "prandtl_ensemble_destroy", "prandtl_ensemble_case",
     "prandtl_ensemble_rk3_step", "prandtl_ensemble_last_error",
     "prandtl_checkpoint_write", "prandtl_checkpoint_read", "prandtl_checkpoint_save", "prandtl_checkpoint_load",
+    "prandtl_create_quad", "prandtl_hmin", "prandtl_mesh_read", "prandtl_mesh_refine", "prandtl_mesh_view",
+    "prandtl_mesh_name", "prandtl_mesh_free", "prandtl_mesh_last_error",
 )
 
 
@@ -37,6 +39,16 @@ class MeshDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("nelem", ctypes.c_int * 3),
                 ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
                 ("bc", (ctypes.c_int * 2) * 3), ("bc_state", ((ctypes.c_double * 5) * 2) * 3)]
+
+
+class QuadMesh(ctypes.Structure):
+    _fields_ = [("nvert", ctypes.c_int), ("vert", ctypes.POINTER(ctypes.c_double)),
+                ("nelem", ctypes.c_int), ("elem", ctypes.POINTER(ctypes.c_int)),
+                ("elem_coords", ctypes.POINTER(ctypes.c_double)),
+                ("nbdr", ctypes.c_int), ("bdr", ctypes.POINTER(ctypes.c_int)),
+                ("bdr_attr", ctypes.POINTER(ctypes.c_int)),
+                ("nattr", ctypes.c_int), ("attr_bc", ctypes.POINTER(ctypes.c_int)),
+                ("attr_state", ctypes.POINTER(ctypes.c_double))]
 
 
 # boundary types (PRANDTL_BC_*) by the reference's configuration names
@@ -108,6 +120,17 @@ def lib():
     L.prandtl_ensemble_rk3_step.argtypes = [vp, ctypes.POINTER(vp), dp, vp]
     L.prandtl_ensemble_last_error.argtypes = [vp]
     L.prandtl_ensemble_last_error.restype = ctypes.c_char_p
+    L.prandtl_create_quad.argtypes = [ctypes.POINTER(QuadMesh), ctypes.c_int, ctypes.POINTER(Params),
+                                      ctypes.POINTER(vp)]
+    L.prandtl_hmin.argtypes = [vp, dp]
+    L.prandtl_mesh_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.prandtl_mesh_refine.argtypes = [vp]
+    L.prandtl_mesh_view.argtypes = [vp, ctypes.POINTER(QuadMesh)]
+    L.prandtl_mesh_name.argtypes = [vp, ctypes.c_int, ip]
+    L.prandtl_mesh_name.restype = ctypes.c_char_p
+    L.prandtl_mesh_free.argtypes = [vp]
+    L.prandtl_mesh_last_error.argtypes = []
+    L.prandtl_mesh_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 

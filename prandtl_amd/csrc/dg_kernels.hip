@@ -46,6 +46,7 @@
 
 #include "../../include/prandtl_hip.h"
 #include "dg_physics.h"
+#include "mesh.h"
 
 namespace prandtl {
 
@@ -61,6 +62,7 @@ struct Tables {
     double Dh2[MAXNP * MAXNP];  // 2D + E           (lines 76-80, un-transposed)
     double V1i[MAXNP * MAXNP];  // inverse 1-D shifted-Legendre Vandermonde
     double fvs[MAXNP];          // 1/(J w_i)            (line 1046)
+    double w[MAXNP];            // LGL weights on [0,1]
     double subn[3][MAXNP];      // subcell normal magnitude per interface (1199-1304)
 };
 
@@ -71,7 +73,15 @@ struct Geo {
     double invJ;     // 1/J
     double aface;    // 1/(w_0 J)
     int bcmask;      // boundary type (BC_*) of side (r, s) in bits 3(2r+s)..+2; 0: periodic box
-    const double *bcs;    // device [3][2][5]: outer states of inflow / specified-state sides
+    const double *bcs;    // device outer states of inflow / specified-state boundaries, 5 per
+                          // box side [3][2] (Cartesian) or per attribute (quadrilateral meshes)
+    // general 2-D quadrilateral geometry (null on Cartesian boxes), local faces f = 2r + side
+    const double *met;    // [4][NE*K] adjugate row r, component d at 2r+d
+    const double *jac;    // [NE*K] Jacobian determinant
+    const int *fface;     // [NE*4] bit 0: first element of the face; bits 1-3 boundary type; 4+: attribute-1
+    const int *fnb;       // [NE*4*NFP] neighbour node e2*K + n2 of each interior face point
+    const double *fnor;   // [NE*4*NFP*2] normal outward of the face's first element
+    const double *subn;   // [NE][2][NFP][NP-1][2] subcell interface normals
 };
 
 struct KArgs {
@@ -292,7 +302,7 @@ template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
 };
 
 // LDS of one k_rhs tile (EPB elements).
-template <int DIM, int NP, bool BLEND, bool QNS3, int FB> struct RhsSmem {
+template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false> struct RhsSmem {
     using C = RCfg<DIM, NP, BLEND, FB>;
     static constexpr int NQ = DIM + 1 + (BLEND ? 2 : 0);   // v[DIM], beta, (1/beta, lambda)
     static constexpr int GK = QNS3 ? C::K : 1, GE = QNS3 ? C::EPB : 1;
@@ -309,6 +319,7 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB> struct RhsSmem {
     double G[GE][C::NV][GK];
     double Al[C::EPB];
     double Red[(C::BLOCK + 63) / 64];
+    double M[GEN ? C::EPB : 1][4][GEN ? C::K : 1];   // per-node adjugate rows (general geometry)
 };
 
 // One tile of k_rhs: elements [tile*EPB, tile*EPB + EPB) of the case A
@@ -317,14 +328,19 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB> struct RhsSmem {
 // QNS3: the reference's NS-3D zeta-partner defect (faithful NS in 3-D).
 // VT: element type of the viscous inputs from k_visc (float in the mixed form).
 // FB: forced workgroup size (0: the natural one of RCfg).
-template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false>
+// OPEN: boundary sides on a Cartesian box.  GEN: general 2-D quadrilateral
+// geometry (per-node metrics, face tables; boundaries by attribute).
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false, bool GEN = false>
 __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const int slot,
-                                         RhsSmem<DIM, NP, BLEND, QNS3, FB> &S)
+                                         RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN> &S)
 {
     using C = RCfg<DIM, NP, BLEND, FB>;
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, NPAIR = C::NPAIR, NFV = C::NFV;
     constexpr int EPB = C::EPB, LC = C::LC, BLOCK = C::BLOCK, FXN = C::FXN, JF = C::JF;
-    constexpr int NQ = RhsSmem<DIM, NP, BLEND, QNS3, FB>::NQ;
+    constexpr int NQ = RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN>::NQ;
+    static_assert(!GEN || DIM == 2, "general geometry is 2-D");
+    auto &sM = S.M;
+    (void)sM;
     (void)FXN;
     auto &sDh2 = S.Dh2;
     auto &sV1i = S.V1i;
@@ -367,11 +383,18 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     if (act)
 #pragma unroll
         for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)e * K + n];
+    // general geometry: this node's adjugate rows and Jacobian
+    double jn = 1.0, mown[4] = {0.0, 0.0, 0.0, 0.0};
+    if (GEN && act) {
+        jn = A.geo.jac[(size_t)e * K + n];
+#pragma unroll
+        for (int c = 0; c < 4; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
+    }
     // Blended kernels carry the subcell-flux work and its accumulator; they
     // do not hold loads in flight across phases (the surface data is loaded
     // in the surface phase, the assembly inputs at the assembly), which keeps
     // their register count down.
-    constexpr bool PREFETCH = !BLEND;
+    constexpr bool PREFETCH = !BLEND && !GEN;
     double pn[NV], pfo[NV], pfn[NV];
     if constexpr (PREFETCH) {
         const int jf = t;
@@ -399,6 +422,9 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     if (act) {
 #pragma unroll
         for (int v = 0; v < NV; v++) sU[el][v][n] = ys[v];
+        if constexpr (GEN)
+#pragma unroll
+            for (int c = 0; c < 4; c++) sM[el][c][n] = mown[c];
         const NodeQ<DIM> q = node_quants<DIM>(ys, P);
 #pragma unroll
         for (int d = 0; d < DIM; d++) sQ[el][d][n] = q.v[d];
@@ -509,7 +535,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         const int f = rem / NFP, l = rem - f * NFP;
         const int r = f >> 1, side = f & 1;
         const double nr = A.geo.nrm[r];
-        if (!PREFETCH || jf != t) {
+        if (!GEN && (!PREFETCH || jf != t)) {
             // not prefetched, or more surface points than threads: load now
             const int en = neighbor(A.geo, eo, r, side);
             const int nn = line_node<DIM, NP>(r, l, side ? 0 : NP - 1);
@@ -526,6 +552,35 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         double so[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) so[v] = sU[ej][v][no];
+        if constexpr (GEN) {
+            // general geometry: the face's normal and first element from the
+            // face tables (owner computes, as on the box), boundary points by
+            // attribute
+            const size_t efo = (size_t)eo * 4 + f;
+            const int code = A.geo.fface[efo];
+            const int bt = (code >> 1) & 7;
+            const double nvec[2] = {A.geo.fnor[(efo * NFP + l) * 2], A.geo.fnor[(efo * NFP + l) * 2 + 1]};
+            double F[NV];
+            if (bt != BC_PERIODIC) {
+                const double lb = boundary_flux_v2(bt, nvec, so, A.geo.bcs + (code >> 4) * 5, P, F);
+                lam = (lam < lb) ? lb : lam;
+#pragma unroll
+                for (int v = 0; v < NV; v++) sFc[ej][v][rem] = -F[v];
+                continue;
+            }
+            const int gn = A.geo.fnb[efo * NFP + l];
+#pragma unroll
+            for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)gn];
+            const bool own = code & 1;
+            double *g1 = own ? so : pn;
+            double *g2 = own ? pn : so;
+            const NodeQ<DIM> q1 = node_quants<DIM>(g1, P);
+            const NodeQ<DIM> q2 = node_quants<DIM>(g2, P);
+            two_point_flux_v<DIM, true>(nvec, norml2_2(nvec), q1, q2, g1, g2, P, false, F);
+#pragma unroll
+            for (int v = 0; v < NV; v++) sFc[ej][v][rem] = own ? -F[v] : F[v];
+            continue;
+        }
         if constexpr (OPEN) {
             // boundary point: -F_b for the outward normal at the interior
             // node (BdrFaceIntegrator.cpp:61-123, applied after the interior
@@ -578,16 +633,17 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
 
     // this node's surface terms, faces in direction order, low side first
     __syncthreads();
+    const double aface = GEN ? 1.0 / (A.tab.w[0] * jn) : A.geo.aface;
     if (act) {
 #pragma unroll
         for (int r = 0; r < DIM; r++) {
             const int l = line_of<DIM, NP>(r, ii[0], ii[1], ii[2]);
             if (ii[r] == 0)
 #pragma unroll
-                for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sFc[el][v][(2 * r) * NFP + l];
+                for (int v = 0; v < NV; v++) accS[v] += aface * sFc[el][v][(2 * r) * NFP + l];
             if (ii[r] == NP - 1)
 #pragma unroll
-                for (int v = 0; v < NV; v++) accS[v] += A.geo.aface * sFc[el][v][(2 * r + 1) * NFP + l];
+                for (int v = 0; v < NV; v++) accS[v] += aface * sFc[el][v][(2 * r + 1) * NFP + l];
         }
     }
     __syncthreads();
@@ -622,7 +678,14 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][na]; qb.v[d] = sQ[ej][d][nb]; }
                     qa.beta = sQ[ej][DIM][na]; qb.beta = sQ[ej][DIM][nb];
                     double F[NV];
-                    two_point_flux<DIM, false>(r, nr, nr, qa, qb, nullptr, nullptr, P, false, F);
+                    if constexpr (GEN) {
+                        // ComputeVolumeFlux with the mean of the two adjugate rows
+                        const double mm[2] = {(sM[ej][2 * r][na] + sM[ej][2 * r][nb]) * 0.5,
+                                              (sM[ej][2 * r + 1][na] + sM[ej][2 * r + 1][nb]) * 0.5};
+                        two_point_flux_v<DIM, false>(mm, 0.0, qa, qb, nullptr, nullptr, P, false, F);
+                    } else {
+                        two_point_flux<DIM, false>(r, nr, nr, qa, qb, nullptr, nullptr, P, false, F);
+                    }
 #pragma unroll
                     for (int v = 0; v < NV; v++) sF[ej][v][lc * NPAIR + p] = F[v];
                     if (QNS3 && r == 1 && b == NP - 1)
@@ -654,7 +717,13 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     qa.ibeta = sQ[ej][NQ - 2][na]; qb.ibeta = sQ[ej][NQ - 2][nb];
                     qa.lam = sQ[ej][NQ - 1][na]; qb.lam = sQ[ej][NQ - 1][nb];
                     double F[NV];
-                    two_point_flux<DIM, true>(r, sSub[r][i], sSub[r][i], qa, qb, s1, s2, P, quirk3d, F);
+                    if constexpr (GEN) {
+                        const double *sn = A.geo.subn + (((((size_t)(ebase + ej) * 2 + r) * NFP + l0 + lc) * (NP - 1) + i) * 2);
+                        const double nv2[2] = {sn[0], sn[1]};
+                        two_point_flux_v<DIM, true>(nv2, norml2_2(nv2), qa, qb, s1, s2, P, quirk3d, F);
+                    } else {
+                        two_point_flux<DIM, true>(r, sSub[r][i], sSub[r][i], qa, qb, s1, s2, P, quirk3d, F);
+                    }
 #pragma unroll
                     for (int v = 0; v < NV; v++) sF[ej][v][slot] = F[v];
                 }
@@ -682,12 +751,14 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     }
                     if (BLEND && alpha_e > 0.0) {
                         const int base = LC * NPAIR + lc * NFV;
+                        // 1/(J_i w_i) (DGSEMIntegrator.cpp:1046) with this node's J
+                        const double fvs = GEN ? 1.0 / (jn * A.tab.w[a]) : sFvs[a];
 #pragma unroll
                         for (int v = 0; v < NV; v++) {
                             double val;
-                            if (a == 0) val = (0.0 - sF[el][v][base + 0]) * sFvs[0];
-                            else if (a == NP - 1) val = sF[el][v][base + NP - 2] * sFvs[NP - 1];
-                            else val = (sF[el][v][base + a - 1] - sF[el][v][base + a]) * sFvs[a];
+                            if (a == 0) val = (0.0 - sF[el][v][base + 0]) * fvs;
+                            else if (a == NP - 1) val = sF[el][v][base + NP - 2] * fvs;
+                            else val = (sF[el][v][base + a - 1] - sF[el][v][base + a]) * fvs;
                             accF[v] = (r == 0) ? val : accF[v] + val;
                         }
                     }
@@ -712,7 +783,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             double vol = -accV[v];
             if (BLEND) vol *= (1.0 - alpha_e);
             if (A.parabolic && v > 0) vol = vol + vis[v];
-            vol *= A.geo.invJ;
+            vol *= GEN ? 1.0 / jn : A.geo.invJ;
             double tot = (BLEND && alpha_e > 0.0) ? accF[v] * alpha_e + vol : vol;
             tot += accS[v];
             const size_t gi = v * NK + (size_t)e * K + n;
@@ -728,13 +799,13 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
 }
 
 // OPEN: the box has non-periodic sides (Euler, natural shape); the periodic
-// kernels carry no boundary code.
-template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false>
+// kernels carry no boundary code.  GEN: a 2-D quadrilateral mesh.
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false, bool GEN = false>
 __global__ void __launch_bounds__((RCfg<DIM, NP, BLEND, FB>::BLOCK))
 __attribute__((amdgpu_waves_per_eu(RCfg<DIM, NP, BLEND, FB>::WAVES))) k_rhs(const KArgs A)
 {
-    __shared__ RhsSmem<DIM, NP, BLEND, QNS3, FB> S;
-    rhs_body<DIM, NP, BLEND, QNS3, VT, FB, OPEN>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
+    __shared__ RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN> S;
+    rhs_body<DIM, NP, BLEND, QNS3, VT, FB, OPEN, GEN>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -1252,6 +1323,9 @@ struct prandtl_ctx {
     bool use_graph = true;
     double *d_dt = nullptr;
     double *d_bcs = nullptr;       // [3][2][5] outer boundary states (open boxes only)
+    void *d_geo = nullptr;         // general quadrilateral geometry tables
+    bool gen = false;              // quadrilateral mesh (k_rhs general-geometry variant)
+    double hmin = 0.0;             // smallest element size (the dt policy's h_min)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     const double *graph_u = nullptr;
@@ -1357,6 +1431,15 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
 {
     ProfScope ps(c, s, 0);
     const bool qns3 = DIM == 3 && c->prm.parabolic && c->prm.quirks;
+    if constexpr (DIM == 2 && NP <= 6) {
+        if (c->gen) {
+            // quadrilateral mesh (Euler), natural shape
+            using Cf = RCfg<DIM, NP, BLEND>;
+            const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+            hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+            return hip_check(c, hipGetLastError(), "k_rhs launch");
+        }
+    }
     if (c->geo.bcmask) {
         // boundary conditions (Euler only), natural shape
         using Cf = RCfg<DIM, NP, BLEND>;
@@ -1462,7 +1545,7 @@ bool shape_ok(const prandtl_ctx *c, int kernel, int shape)
     if (kernel != PRANDTL_KERNEL_RHS) return false;
     if (shape == 0) return true;
     if (c->dim == 3 && c->prm.parabolic && c->prm.quirks) return false;
-    if (c->geo.bcmask) return false;
+    if (c->geo.bcmask || c->gen) return false;
     return rhs_shape_ok(c->dim, c->Np, c->prm.blending != 0, shape);
 }
 
@@ -1487,7 +1570,14 @@ int prandtl_lgl(int order, double *x, double *w)
     return PRANDTL_OK;
 }
 
-int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, prandtl_ctx **out)
+} // extern "C"
+
+namespace {
+
+// prandtl_create and prandtl_create_quad: qm null for a Cartesian box (mesh),
+// else a quadrilateral mesh (mesh then only carries dim = 2 and the order).
+int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, const prandtl_params *prm,
+                prandtl_ctx **out)
 {
     if (!mesh || !prm || !out) return fail(nullptr, PRANDTL_ERR_ARG, "null argument");
     *out = nullptr;
@@ -1504,14 +1594,20 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     if (prm->parabolic && prm->sutherland && !(prm->R_gas > 0.0))
         return fail(nullptr, PRANDTL_ERR_ARG, "R_gas must be positive with Sutherland viscosity");
     long long NE = 1;
-    for (int r = 0; r < dim; r++) {
+    if (qm) {
+        if (prm->parabolic)
+            return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "quadrilateral meshes are built for Euler (parabolic = 0)");
+        if (qm->nelem < 1) return fail(nullptr, PRANDTL_ERR_ARG, "empty mesh");
+        NE = qm->nelem;
+    }
+    for (int r = 0; r < dim && !qm; r++) {
         if (mesh->nelem[r] < 2) return fail(nullptr, PRANDTL_ERR_ARG, "need >= 2 elements per direction");
         if (!(mesh->hi[r] > mesh->lo[r])) return fail(nullptr, PRANDTL_ERR_ARG, "empty box");
         NE *= mesh->nelem[r];
     }
     if (NE > (1LL << 30)) return fail(nullptr, PRANDTL_ERR_ARG, "mesh too large");
     bool open = false;
-    for (int r = 0; r < dim; r++) {
+    for (int r = 0; r < dim && !qm; r++) {
         for (int side = 0; side < 2; side++) {
             const int t = mesh->bc[r][side];
             if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_SPECIFIED_STATE)
@@ -1531,6 +1627,24 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     }
     if (open && prm->parabolic)
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "boundary conditions are built for Euler (parabolic = 0)");
+    std::vector<double> qstates;   // quadrilateral meshes: outer states per attribute
+    if (qm) {
+        if (qm->nattr < 0 || (qm->nattr > 0 && !qm->attr_bc)) return fail(nullptr, PRANDTL_ERR_ARG, "bad attribute table");
+        qstates.assign(5 * (size_t)std::max(qm->nattr, 1), 0.0);
+        for (int a = 0; a < qm->nattr; a++) {
+            const int t = qm->attr_bc[a];
+            if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_SPECIFIED_STATE)
+                return fail(nullptr, PRANDTL_ERR_ARG, "unknown boundary type");
+            if (t == PRANDTL_BC_SUPERSONIC_INFLOW || t == PRANDTL_BC_SPECIFIED_STATE) {
+                if (!qm->attr_state) return fail(nullptr, PRANDTL_ERR_ARG, "boundary state missing");
+                const double *g = qm->attr_state + 4 * (size_t)a;
+                const double pg = (prm->gamma - 1.0) * (g[3] - 0.5 * (g[1] * g[1] + g[2] * g[2]) / g[0]);
+                if (!(g[0] > 0.0) || !(pg > 0.0) || !std::isfinite(pg))
+                    return fail(nullptr, PRANDTL_ERR_ARG, "boundary state needs positive density and pressure");
+                for (int v = 0; v < 4; v++) qstates[5 * (size_t)a + v] = g[v];
+            }
+        }
+    }
     prandtl_ctx *c = new (std::nothrow) prandtl_ctx();
     if (!c) return fail(nullptr, PRANDTL_ERR_NOMEM, "host allocation");
     c->dim = dim; c->order = N; c->Np = N + 1; c->nv = dim + 2;
@@ -1585,11 +1699,26 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     for (int a = 0; a < Np; a++)
         for (int i = 0; i < Np; i++) c->tab.V1i[a * MAXNP + i] = Vi[a * Np + i];
 
+    for (int i = 0; i < Np; i++) c->tab.w[i] = w[i];
+    quad::Geometry qg;
+    if (qm) {
+        std::string err;
+        if (!quad::build(qm, N, x, w, D, qg, err)) {
+            delete c;
+            return fail(nullptr, PRANDTL_ERR_ARG, err);
+        }
+        c->gen = true;
+        c->hmin = qg.hmin;
+    }
     // Cartesian geometry: J = det diag(h), adj = diag of cofactors
     double h[3] = {1, 1, 1};
     for (int r = 0; r < dim; r++) {
-        h[r] = (mesh->hi[r] - mesh->lo[r]) / mesh->nelem[r];
-        c->geo.n[r] = mesh->nelem[r];
+        h[r] = qm ? 1.0 : (mesh->hi[r] - mesh->lo[r]) / mesh->nelem[r];
+        c->geo.n[r] = qm ? (r == 0 ? c->NE : 1) : mesh->nelem[r];
+    }
+    if (!qm) {
+        c->hmin = h[0];
+        for (int r = 1; r < dim; r++) c->hmin = h[r] < c->hmin ? h[r] : c->hmin;
     }
     for (int r = dim; r < 3; r++) c->geo.n[r] = 1;
     c->geo.NE = c->NE;
@@ -1605,7 +1734,7 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
     c->geo.aface = 1.0 / (w[0] * J);
     c->geo.bcmask = 0;
     c->geo.bcs = nullptr;
-    for (int r = 0; r < dim; r++)
+    for (int r = 0; r < dim && !qm; r++)
         for (int side = 0; side < 2; side++) c->geo.bcmask |= mesh->bc[r][side] << (3 * (2 * r + side));
     for (int i = 0; i < Np; i++) c->tab.fvs[i] = 1.0 / (J * w[i]);
     // subcell normals n_{i+1/2} = Ja + sum_{l<=i} w_l sum_q D_lq Ja (lines 1212-1233)
@@ -1655,6 +1784,27 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
         if (e == hipSuccess) e = hipMemcpy(c->d_bcs, hb, sizeof(hb), hipMemcpyHostToDevice);
         c->geo.bcs = c->d_bcs;
     }
+    if (e == hipSuccess && qm) {
+        // general geometry tables, one allocation
+        auto bytes = [](const auto &v) { return v.size() * sizeof(v[0]); };
+        const size_t parts[7] = {bytes(qg.met), bytes(qg.jac), bytes(qg.fnor), bytes(qg.subn), bytes(qstates),
+                                 bytes(qg.fface), bytes(qg.fnb)};
+        size_t off[8] = {0};
+        for (int i = 0; i < 7; i++) off[i + 1] = off[i] + ((parts[i] + 255) / 256) * 256;
+        e = hipMalloc(&c->d_geo, off[7]);
+        const void *src[7] = {qg.met.data(), qg.jac.data(), qg.fnor.data(), qg.subn.data(), qstates.data(),
+                              qg.fface.data(), qg.fnb.data()};
+        for (int i = 0; i < 7 && e == hipSuccess; i++)
+            e = hipMemcpy(static_cast<char *>(c->d_geo) + off[i], src[i], parts[i], hipMemcpyHostToDevice);
+        char *b = static_cast<char *>(c->d_geo);
+        c->geo.met = reinterpret_cast<const double *>(b + off[0]);
+        c->geo.jac = reinterpret_cast<const double *>(b + off[1]);
+        c->geo.fnor = reinterpret_cast<const double *>(b + off[2]);
+        c->geo.subn = reinterpret_cast<const double *>(b + off[3]);
+        c->geo.bcs = reinterpret_cast<const double *>(b + off[4]);
+        c->geo.fface = reinterpret_cast<const int *>(b + off[5]);
+        c->geo.fnb = reinterpret_cast<const int *>(b + off[6]);
+    }
     if (e == hipSuccess) e = hipMemset(c->d_alpha, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_eta, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_lam, 0, sizeof(double) * c->NE);
@@ -1665,6 +1815,33 @@ int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, pra
         return PRANDTL_ERR_NOMEM;
     }
     *out = c;
+    return PRANDTL_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int prandtl_create(const prandtl_mesh_desc *mesh, const prandtl_params *prm, prandtl_ctx **out)
+{
+    return create_impl(mesh, nullptr, prm, out);
+}
+
+int prandtl_create_quad(const prandtl_quad_mesh *mesh, int order, const prandtl_params *params, prandtl_ctx **out)
+{
+    if (!mesh) return fail(nullptr, PRANDTL_ERR_ARG, "null argument");
+    prandtl_mesh_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.dim = 2;
+    d.order = order;
+    d.nelem[0] = mesh->nelem; d.nelem[1] = 1; d.nelem[2] = 1;
+    return create_impl(&d, mesh, params, out);
+}
+
+int prandtl_hmin(const prandtl_ctx *c, double *hmin)
+{
+    if (!c || !hmin) return PRANDTL_ERR_ARG;
+    *hmin = c->hmin;
     return PRANDTL_OK;
 }
 
@@ -1680,6 +1857,7 @@ void prandtl_destroy(prandtl_ctx *c)
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_dt) (void)hipFree(c->d_dt);
     if (c->d_bcs) (void)hipFree(c->d_bcs);
+    if (c->d_geo) (void)hipFree(c->d_geo);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -1816,11 +1994,7 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
     if (!c->d_loop) rc = hip_check(c, hipMalloc(&c->d_loop, sizeof(LoopState)), "loop state");
     if (rc) return rc;
     // hmin and (N+1)^2 as the host loop forms them
-    double hmin = (c->mesh.hi[0] - c->mesh.lo[0]) / c->mesh.nelem[0];
-    for (int r = 1; r < c->dim; r++) {
-        const double h = (c->mesh.hi[r] - c->mesh.lo[r]) / c->mesh.nelem[r];
-        hmin = h < hmin ? h : hmin;
-    }
+    const double hmin = c->hmin;
     const double cfl_hmin = cfl * hmin, np1sq = (double)(c->Np * c->Np);
     LoopState L0 = {*t_io, variable ? 0.0 : dt, 0.0, 0, 0, 0};
     if (!variable) L0.dt_real = (dt < t_final - *t_io) ? dt : t_final - *t_io;

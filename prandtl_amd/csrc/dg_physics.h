@@ -154,6 +154,75 @@ __device__ __forceinline__ void two_point_flux(int r, double nr, double nd, cons
     }
 }
 
+// MFEM Vector::Norml2 of a 2-vector (scaled, overflow-safe), the face
+// normal's magnitude in ChandrashekarFlux::ComputeFaceFlux (line 88).
+__device__ __forceinline__ double norml2_2(const double *n)
+{
+    double scale = 0.0, sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        if (n[i] != 0.0) {
+            const double a = fabs(n[i]);
+            if (scale <= a) {
+                const double q = scale / a;
+                sum = 1.0 + sum * (q * q);
+                scale = a;
+            } else {
+                const double q = a / scale;
+                sum += q * q;
+            }
+        }
+    }
+    return scale * sqrt(sum);
+}
+
+// two_point_flux for a general metric / normal vector nv (volume pairs:
+// the mean of the two nodes' adjugate rows; faces: the face normal), the
+// dissipation scaled by nd = |nv|.
+template <int DIM, bool DISS>
+__device__ __forceinline__ void two_point_flux_v(const double *nv, double nd, const NodeQ<DIM> &a,
+                                                 const NodeQ<DIM> &b, const double *s1,
+                                                 const double *s2, const Phys &P, bool quirk3d,
+                                                 double *F)
+{
+    const double rho_ln = logmean(a.rho, b.rho);
+    const double rho_mean = 0.5 * (a.rho + b.rho);
+    const double drho = b.rho - a.rho;
+    double vm[DIM];
+    double h_hat = 0.0, diss = 0.0, Vn = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        vm[d] = 0.5 * (a.v[d] + b.v[d]);
+        Vn = (d == 0) ? vm[d] * nv[d] : Vn + vm[d] * nv[d];
+        const double hd = -0.25 * (a.v[d] * a.v[d] + b.v[d] * b.v[d]) + vm[d] * vm[d];
+        h_hat = (d == 0) ? hd : h_hat + hd;
+        if (DISS) {
+            const double dv = b.v[d] - a.v[d];
+            const double t0 = 0.5 * drho * a.v[d] * b.v[d];
+            const double dd = (DIM == 3 && d == 2 && quirk3d) ? t0 + rho_mean * dv + vm[d]
+                                                              : t0 + rho_mean * dv * vm[d];
+            diss = (d == 0) ? dd : diss + dd;
+        }
+    }
+    const double ibeta_ln = inv_logmean(a.beta, b.beta);
+    const double p_hat = 0.5 * (a.rho + b.rho) / (a.beta + b.beta);
+    h_hat += 0.5 * ibeta_ln * P.gm1i + p_hat / rho_ln;
+    const double rVn = rho_ln * Vn;
+    F[0] = rVn;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) F[1 + d] = Vn * (rho_ln * vm[d]) + p_hat * nv[d];
+    F[DIM + 1] = rVn * h_hat;
+    if (DISS) {
+        diss += 0.5 * drho * P.gm1i * ibeta_ln + 0.5 * rho_mean * P.gm1i * (b.ibeta - a.ibeta);
+        const double lam = (a.lam < b.lam) ? b.lam : a.lam;
+        const double hl = 0.5 * lam;
+        F[0] = F[0] - hl * (s2[0] - s1[0]) * nd;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) F[1 + d] = F[1 + d] - hl * (s2[1 + d] - s1[1 + d]) * nd;
+        F[DIM + 1] = F[DIM + 1] - hl * diss * nd;
+    }
+}
+
 // Boundary types (PRANDTL_BC_*)
 enum { BC_PERIODIC = 0, BC_SLIP = 1, BC_INFLOW = 2, BC_OUTFLOW = 3, BC_SYMMETRY = 4, BC_SPECIFIED = 5 };
 
@@ -210,6 +279,65 @@ __device__ __forceinline__ double slip_wall_flux(int r, double ns, const double 
     for (int d = 0; d < DIM; d++)
         if (d == r) F[1 + d] = p_star * ns;
     return sqrt(vv) + an;
+}
+
+// slip_wall_flux for a general 2-D outward normal n: Normalize, then
+// RotateState with Normal(n) = (-n1, n0) (BasicOperations.cpp:211-259).
+__device__ __forceinline__ double slip_wall_flux_v2(const double *n, const double *s, const Phys &P, double *F)
+{
+    const double inv = 1.0 / norml2_2(n);
+    const double u0 = n[0] * inv, u1 = n[1] * inv;
+    const double m[2] = {s[1] * u0 + s[2] * u1, s[1] * -u1 + s[2] * u0};
+    const double rho = s[0];
+    const double ri = 1.0 / rho;
+    double vel[2], vv = 0.0, VV = 0.0;
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        vel[d] = m[d] / rho;
+        const double V = m[d] * ri;
+        VV = (d == 0) ? V * V : VV + V * V;
+        vv = (d == 0) ? vel[d] * vel[d] : vv + vel[d] * vel[d];
+    }
+    const double p = P.gm1 * (s[3] - 0.5 * rho * VV);
+    const double an = sqrt(P.gamma * p / rho);
+    const double un = vel[0];
+    const double gp1 = 1.0 + P.gamma, gp1i = 1.0 / gp1;
+    double p_star;
+    if (un > 0.0)
+        p_star = p + 0.25 * un * gp1 * rho * (un + sqrt(un * un + 8.0 * gp1i / rho * p * (P.gm1 * gp1i + 1.0)));
+    else {
+        double base = 1.0 + 0.5 * P.gm1 * un / an;
+        base = (base < 0.0001) ? 0.0001 : base;
+        p_star = p * pow(base, 2.0 * P.gamma * P.gm1i);
+    }
+    F[0] = 0.0;
+    F[1] = p_star * n[0];
+    F[2] = p_star * n[1];
+    F[3] = 0.0;
+    return sqrt(vv) + an;
+}
+
+// boundary_flux for a general 2-D outward normal n (general quadrilateral
+// meshes); symmetry mirrors m - 2 (m.u) u with the unit normal u.
+__device__ __forceinline__ double boundary_flux_v2(int type, const double *n, const double *s, const double *outer,
+                                                   const Phys &P, double *F)
+{
+    if (type == BC_SLIP) return slip_wall_flux_v2(n, s, P, F);
+    const double nd = norml2_2(n);
+    double g[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) g[v] = (type == BC_INFLOW || type == BC_SPECIFIED) ? outer[v] : s[v];
+    if (type == BC_SYMMETRY) {
+        const double inv = 1.0 / nd;
+        const double u0 = n[0] * inv, u1 = n[1] * inv;
+        const double mn = s[1] * u0 + s[2] * u1;
+        g[1] = s[1] + -2.0 * mn * u0;
+        g[2] = s[2] + -2.0 * mn * u1;
+    }
+    const NodeQ<2> q1 = node_quants<2>(s, P);
+    const NodeQ<2> q2 = node_quants<2>(g, P);
+    two_point_flux_v<2, true>(n, nd, q1, q2, s, g, P, false, F);
+    return (q1.lam < q2.lam) ? q2.lam : q1.lam;
 }
 
 // ComputeBdrFaceInviscidFlux of a boundary point (BdrFaceIntegrator.cpp:

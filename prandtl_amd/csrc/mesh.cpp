@@ -1,0 +1,562 @@
+// Quadrilateral mesh files (MFEM mesh v1.0, Gmsh 2.2 ASCII), uniform
+// refinement, and the per-node / per-face geometry the general-geometry
+// k_rhs reads.  Host code only.
+#include "mesh.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <new>
+#include <sstream>
+#include <unordered_map>
+#include <utility>
+
+struct prandtl_mesh_file {
+    std::vector<double> vert;      // [nv][2]
+    std::vector<int> elem;         // [ne][4]
+    std::vector<double> ecoords;   // empty, or [ne][4][2]
+    std::vector<int> bdr;          // [nb][2]
+    std::vector<int> battr;        // [nb]
+    std::vector<std::pair<std::string, int>> names;   // boundary physical groups
+};
+
+namespace {
+
+thread_local std::string g_mesh_err;
+
+int mesh_fail(const std::string &msg)
+{
+    g_mesh_err = msg;
+    return PRANDTL_ERR_IO;
+}
+
+long long edge_key(int a, int b)
+{
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return ((long long)lo << 32) | (unsigned int)hi;
+}
+
+// corner k (vertex order) of element e
+inline void corner(const prandtl_mesh_file &m, int e, int k, double &x, double &y)
+{
+    if (!m.ecoords.empty()) {
+        x = m.ecoords[(size_t)e * 8 + 2 * k];
+        y = m.ecoords[(size_t)e * 8 + 2 * k + 1];
+    } else {
+        const int v = m.elem[(size_t)e * 4 + k];
+        x = m.vert[2 * (size_t)v];
+        y = m.vert[2 * (size_t)v + 1];
+    }
+}
+
+// Elements with clockwise corners get the order (v0, v3, v2, v1), as
+// Mesh::Finalize's orientation fix does for quadrilaterals.
+void orient(prandtl_mesh_file &m)
+{
+    const int ne = (int)(m.elem.size() / 4);
+    for (int e = 0; e < ne; e++) {
+        double c[4][2];
+        for (int k = 0; k < 4; k++) corner(m, e, k, c[k][0], c[k][1]);
+        double a = 0.0;
+        for (int k = 0; k < 4; k++) {
+            const int k1 = (k + 1) & 3;
+            a += c[k][0] * c[k1][1] - c[k1][0] * c[k][1];
+        }
+        if (a < 0.0) {
+            std::swap(m.elem[(size_t)e * 4 + 1], m.elem[(size_t)e * 4 + 3]);
+            if (!m.ecoords.empty())
+                for (int i = 0; i < 2; i++)
+                    std::swap(m.ecoords[(size_t)e * 8 + 2 + i], m.ecoords[(size_t)e * 8 + 6 + i]);
+        }
+    }
+}
+
+std::string next_line(std::istream &in)
+{
+    std::string s;
+    while (std::getline(in, s)) {
+        const size_t a = s.find_first_not_of(" \t\r");
+        if (a == std::string::npos) continue;
+        if (s[a] == '#') continue;
+        const size_t b = s.find_last_not_of(" \t\r");
+        return s.substr(a, b - a + 1);
+    }
+    return std::string();
+}
+
+int read_gmsh(std::istream &in, prandtl_mesh_file &m)
+{
+    std::string line = next_line(in);   // the version line after $MeshFormat
+    {
+        std::istringstream ls(line);
+        double ver = 0.0;
+        int ftype = -1;
+        ls >> ver >> ftype;
+        if (ver < 2.0 || ver >= 3.0 || ftype != 0) return mesh_fail("only Gmsh 2.x ASCII files are read");
+    }
+    std::unordered_map<long long, int> node_of;
+    bool have_nodes = false, have_elems = false;
+    while (in) {
+        line = next_line(in);
+        if (line.empty()) break;
+        if (line == "$PhysicalNames") {
+            const int n = std::atoi(next_line(in).c_str());
+            for (int i = 0; i < n; i++) {
+                std::istringstream ls(next_line(in));
+                int dim = 0, tag = 0;
+                std::string name;
+                ls >> dim >> tag;
+                std::getline(ls, name);
+                const size_t a = name.find('"'), b = name.rfind('"');
+                if (a != std::string::npos && b > a) name = name.substr(a + 1, b - a - 1);
+                if (dim == 1) m.names.emplace_back(name, tag);
+            }
+        } else if (line == "$Nodes") {
+            const int n = std::atoi(next_line(in).c_str());
+            if (n <= 0) return mesh_fail("empty $Nodes");
+            m.vert.resize(2 * (size_t)n);
+            for (int i = 0; i < n; i++) {
+                std::istringstream ls(next_line(in));
+                long long id = 0;
+                double x = 0, y = 0, z = 0;
+                if (!(ls >> id >> x >> y >> z)) return mesh_fail("bad $Nodes record");
+                node_of[id] = i;
+                m.vert[2 * (size_t)i] = x;
+                m.vert[2 * (size_t)i + 1] = y;
+            }
+            have_nodes = true;
+        } else if (line == "$Elements") {
+            if (!have_nodes) return mesh_fail("$Elements before $Nodes");
+            const int n = std::atoi(next_line(in).c_str());
+            for (int i = 0; i < n; i++) {
+                std::istringstream ls(next_line(in));
+                long long id = 0;
+                int type = 0, ntags = 0;
+                if (!(ls >> id >> type >> ntags)) return mesh_fail("bad $Elements record");
+                std::vector<long long> tags(ntags);
+                for (int t = 0; t < ntags; t++) ls >> tags[t];
+                const int nn = type == 1 ? 2 : type == 3 ? 4 : type == 15 ? 1 : -1;
+                if (nn < 0) return mesh_fail("element type " + std::to_string(type) + ": only quadrilaterals and boundary lines are read");
+                int v[4];
+                for (int k = 0; k < nn; k++) {
+                    long long nid = 0;
+                    if (!(ls >> nid)) return mesh_fail("short $Elements record");
+                    auto it = node_of.find(nid);
+                    if (it == node_of.end()) return mesh_fail("element references an unknown node");
+                    v[k] = it->second;
+                }
+                const int attr = ntags > 0 ? (int)tags[0] : 1;
+                if (type == 3)
+                    for (int k = 0; k < 4; k++) m.elem.push_back(v[k]);
+                else if (type == 1) {
+                    m.bdr.push_back(v[0]);
+                    m.bdr.push_back(v[1]);
+                    m.battr.push_back(attr);
+                }
+            }
+            have_elems = true;
+        } else if (line[0] == '$' && line.compare(0, 4, "$End") != 0) {
+            // skip an unknown section
+            const std::string end = "$End" + line.substr(1);
+            while (in && next_line(in) != end) {}
+        }
+    }
+    if (!have_elems || m.elem.empty()) return mesh_fail("no quadrilateral elements");
+    return PRANDTL_OK;
+}
+
+int read_mfem(std::istream &in, prandtl_mesh_file &m)
+{
+    int dim = 0, nv = -1;
+    std::string line;
+    while (in) {
+        line = next_line(in);
+        if (line.empty()) break;
+        if (line == "dimension") {
+            dim = std::atoi(next_line(in).c_str());
+            if (dim != 2) return mesh_fail("only 2-D MFEM meshes are read");
+        } else if (line == "elements") {
+            const int n = std::atoi(next_line(in).c_str());
+            for (int i = 0; i < n; i++) {
+                std::istringstream ls(next_line(in));
+                int attr = 0, geom = 0, v[4];
+                ls >> attr >> geom;
+                if (geom != 3) return mesh_fail("only quadrilateral elements (geometry 3) are read");
+                for (int k = 0; k < 4; k++)
+                    if (!(ls >> v[k])) return mesh_fail("short element record");
+                for (int k = 0; k < 4; k++) m.elem.push_back(v[k]);
+            }
+        } else if (line == "boundary") {
+            const int n = std::atoi(next_line(in).c_str());
+            for (int i = 0; i < n; i++) {
+                std::istringstream ls(next_line(in));
+                int attr = 0, geom = 0, a = 0, b = 0;
+                if (!(ls >> attr >> geom >> a >> b) || geom != 1) return mesh_fail("bad boundary record");
+                m.bdr.push_back(a);
+                m.bdr.push_back(b);
+                m.battr.push_back(attr);
+            }
+        } else if (line == "vertices") {
+            nv = std::atoi(next_line(in).c_str());
+            const std::string nxt = next_line(in);
+            if (nxt == "nodes") {
+                // first-order L2 nodes with Gauss-Lobatto points: per element
+                // the corners in lexicographic order (0,0), (1,0), (0,1), (1,1)
+                std::string fes = next_line(in), fec = next_line(in), vd = next_line(in), ord = next_line(in);
+                if (fec.find("L2_T1_2D_P1") == std::string::npos)
+                    return mesh_fail("mesh nodes: only the L2_T1_2D_P1 collection is read");
+                const int ordering = std::atoi(ord.substr(ord.find(':') + 1).c_str());
+                const int ne = (int)(m.elem.size() / 4);
+                std::vector<double> vals;
+                vals.reserve((size_t)ne * 8);
+                double val;
+                while ((int)vals.size() < ne * 8 && in >> val) vals.push_back(val);
+                if ((int)vals.size() < ne * 8) return mesh_fail("short nodes section");
+                m.ecoords.assign((size_t)ne * 8, 0.0);
+                static const int lex_to_vertex[4] = {0, 1, 3, 2};
+                for (int e = 0; e < ne; e++)
+                    for (int q = 0; q < 4; q++)
+                        for (int c = 0; c < 2; c++) {
+                            const double x = ordering == 1 ? vals[((size_t)e * 4 + q) * 2 + c]
+                                                           : vals[(size_t)c * ne * 4 + (size_t)e * 4 + q];
+                            m.ecoords[(size_t)e * 8 + 2 * lex_to_vertex[q] + c] = x;
+                        }
+                m.vert.assign(2 * (size_t)std::max(nv, 0), 0.0);
+                for (int e = 0; e < ne; e++)
+                    for (int k = 0; k < 4; k++) {
+                        const int v = m.elem[(size_t)e * 4 + k];
+                        if (v >= 0 && v < nv) {
+                            m.vert[2 * (size_t)v] = m.ecoords[(size_t)e * 8 + 2 * k];
+                            m.vert[2 * (size_t)v + 1] = m.ecoords[(size_t)e * 8 + 2 * k + 1];
+                        }
+                    }
+            } else {
+                const int vdim = std::atoi(nxt.c_str());
+                if (vdim != 2) return mesh_fail("vertex dimension must be 2");
+                m.vert.resize(2 * (size_t)nv);
+                for (int i = 0; i < nv; i++) {
+                    std::istringstream ls(next_line(in));
+                    if (!(ls >> m.vert[2 * (size_t)i] >> m.vert[2 * (size_t)i + 1])) return mesh_fail("bad vertex record");
+                }
+            }
+        } else if (line == "mfem_mesh_end") {
+            break;
+        }
+    }
+    if (m.elem.empty() || nv <= 0) return mesh_fail("no elements or vertices");
+    for (int v : m.elem)
+        if (v < 0 || v >= nv) return mesh_fail("element vertex out of range");
+    for (int v : m.bdr)
+        if (v < 0 || v >= nv) return mesh_fail("boundary vertex out of range");
+    return PRANDTL_OK;
+}
+
+} // namespace
+
+namespace prandtl {
+namespace quad {
+
+namespace {
+
+// Jacobian of the bilinear map at (xi, eta): J(i,j) = sum_k c_k[i] dN_k/dxi_j,
+// summed over the corners in order (IsoparametricTransformation::EvalJacobian
+// with BiLinear2DFiniteElement's derivatives).
+void jacobian(const double c[4][2], double xi, double eta, double J[2][2])
+{
+    const double ds[4][2] = {{eta - 1.0, xi - 1.0}, {1.0 - eta, -xi}, {eta, xi}, {-eta, 1.0 - xi}};
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            double a = 0.0;
+            for (int k = 0; k < 4; k++) a += ds[k][j] * c[k][i];
+            J[i][j] = a;
+        }
+}
+
+// face f's end points in l-increasing order (vertex positions 0..3)
+const int FACE_VERT[4][2] = {{0, 3}, {1, 2}, {0, 1}, {3, 2}};
+// face f's end points in counterclockwise traversal (the outward normal's side)
+const int FACE_CCW[4][2] = {{3, 0}, {1, 2}, {0, 1}, {2, 3}};
+
+inline int line_node2(int NP, int r, int l, int m) { return r == 0 ? l * NP + m : m * NP + l; }
+
+} // namespace
+
+bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
+           const std::vector<double> &w, const std::vector<double> &D, Geometry &g, std::string &err)
+{
+    const int NE = m->nelem, NP = order + 1, K = NP * NP, NFP = NP, N = order;
+    if (NE < 1 || !m->elem || (!m->vert && !m->elem_coords)) { err = "empty mesh"; return false; }
+    g.NE = NE; g.NP = NP; g.K = K; g.NFP = NFP;
+    auto corners = [&](int e, double c[4][2]) {
+        for (int k = 0; k < 4; k++)
+            for (int i = 0; i < 2; i++)
+                c[k][i] = m->elem_coords ? m->elem_coords[(size_t)e * 8 + 2 * k + i]
+                                         : m->vert[2 * (size_t)m->elem[(size_t)e * 4 + k] + i];
+    };
+    for (size_t i = 0; i < (size_t)NE * 4; i++)
+        if (m->elem[i] < 0 || (m->vert && !m->elem_coords && m->elem[i] >= m->nvert)) {
+            err = "element vertex out of range";
+            return false;
+        }
+    // per-node metrics
+    const size_t NK = (size_t)NE * K;
+    g.met.assign(4 * NK, 0.0);
+    g.jac.assign(NK, 0.0);
+    g.hmin = INFINITY;
+    for (int e = 0; e < NE; e++) {
+        double c[4][2];
+        corners(e, c);
+        for (int j = 0; j < NP; j++)
+            for (int i = 0; i < NP; i++) {
+                double J[2][2];
+                jacobian(c, x[i], x[j], J);
+                const size_t q = (size_t)e * K + j * NP + i;
+                // CalcAdjugate: rows (J11, -J01) and (-J10, J00)
+                g.met[0 * NK + q] = J[1][1];
+                g.met[1 * NK + q] = -J[0][1];
+                g.met[2 * NK + q] = -J[1][0];
+                g.met[3 * NK + q] = J[0][0];
+                const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+                if (!(det > 0.0)) {
+                    err = "element " + std::to_string(e) + " has a non-positive Jacobian";
+                    return false;
+                }
+                g.jac[q] = det;
+            }
+        // GetElementSize(e, 1): the smallest singular value of J at the centre
+        double J[2][2];
+        jacobian(c, 0.5, 0.5, J);
+        const double S = J[0][0] * J[0][0] + J[0][1] * J[0][1] + J[1][0] * J[1][0] + J[1][1] * J[1][1];
+        const double det = std::fabs(J[0][0] * J[1][1] - J[1][0] * J[0][1]);
+        const double smax = std::sqrt(0.5 * (S + std::sqrt(std::max(S * S - 4.0 * det * det, 0.0))));
+        g.hmin = std::min(g.hmin, det / smax);
+    }
+    // faces: element edges keyed by their vertex pair
+    std::unordered_map<long long, std::pair<int, int>> first;   // key -> (e, f)
+    first.reserve((size_t)NE * 4);
+    std::vector<int> nbr_e((size_t)NE * 4, -1), nbr_f((size_t)NE * 4, -1);
+    for (int e = 0; e < NE; e++)
+        for (int f = 0; f < 4; f++) {
+            const int a = m->elem[(size_t)e * 4 + FACE_VERT[f][0]], b = m->elem[(size_t)e * 4 + FACE_VERT[f][1]];
+            if (a == b) { err = "degenerate element edge"; return false; }
+            const long long key = edge_key(a, b);
+            auto it = first.find(key);
+            if (it == first.end()) {
+                first.emplace(key, std::make_pair(e, f));
+            } else {
+                const int e2 = it->second.first, f2 = it->second.second;
+                if (nbr_e[(size_t)e2 * 4 + f2] >= 0) { err = "an edge is shared by more than two elements"; return false; }
+                nbr_e[(size_t)e2 * 4 + f2] = e; nbr_f[(size_t)e2 * 4 + f2] = f;
+                nbr_e[(size_t)e * 4 + f] = e2; nbr_f[(size_t)e * 4 + f] = f2;
+            }
+        }
+    std::unordered_map<long long, int> battr;
+    for (int i = 0; i < m->nbdr; i++) battr[edge_key(m->bdr[2 * i], m->bdr[2 * i + 1])] = m->bdr_attr[i];
+    g.fface.assign((size_t)NE * 4, 0);
+    g.fnb.assign((size_t)NE * 4 * NFP, -1);
+    g.fnor.assign((size_t)NE * 4 * NFP * 2, 0.0);
+    for (int e = 0; e < NE; e++)
+        for (int f = 0; f < 4; f++) {
+            const size_t ef = (size_t)e * 4 + f;
+            const int e2 = nbr_e[ef];
+            int e1 = e, f1 = f;   // the face's first element and its local face
+            int code = 0;
+            if (e2 >= 0) {
+                if (e2 < e) { e1 = e2; f1 = nbr_f[ef]; }
+                else code = FACE_OWN;
+                const int f2 = nbr_f[ef];
+                const int s_me = m->elem[(size_t)e * 4 + FACE_VERT[f][0]];
+                const int s_nb = m->elem[(size_t)e2 * 4 + FACE_VERT[f2][0]];
+                const bool same = s_me == s_nb;
+                const int r2 = f2 >> 1, side2 = f2 & 1;
+                for (int l = 0; l < NFP; l++) {
+                    const int l2 = same ? l : N - l;
+                    g.fnb[ef * NFP + l] = e2 * K + line_node2(NP, r2, l2, side2 ? N : 0);
+                }
+            } else {
+                const int a = m->elem[(size_t)e * 4 + FACE_VERT[f][0]], b = m->elem[(size_t)e * 4 + FACE_VERT[f][1]];
+                auto it = battr.find(edge_key(a, b));
+                if (it == battr.end()) {
+                    err = "edge of element " + std::to_string(e) + " has no neighbour and no boundary segment";
+                    return false;
+                }
+                const int attr = it->second;
+                if (attr < 1 || attr > m->nattr || !m->attr_bc) {
+                    err = "boundary attribute " + std::to_string(attr) + " has no boundary condition";
+                    return false;
+                }
+                const int t = m->attr_bc[attr - 1];
+                if (t < PRANDTL_BC_SLIP || t > PRANDTL_BC_SPECIFIED_STATE) {
+                    err = "boundary attribute " + std::to_string(attr) + " needs a non-periodic boundary type";
+                    return false;
+                }
+                code = FACE_OWN | (t << 1) | ((attr - 1) << 4);
+            }
+            g.fface[ef] = code;
+            // CalcOrtho of the first element's edge, counterclockwise: (dy, -dx)
+            double c[4][2];
+            corners(e1, c);
+            const int P = FACE_CCW[f1][0], Q = FACE_CCW[f1][1];
+            const double dx = c[Q][0] - c[P][0], dy = c[Q][1] - c[P][1];
+            for (int l = 0; l < NFP; l++) {
+                g.fnor[(ef * NFP + l) * 2 + 0] = dy;
+                g.fnor[(ef * NFP + l) * 2 + 1] = -dx;
+            }
+        }
+    // subcell interface normals n_{i+1/2} = Ja^r(line start) + sum_{l<=i} w_l sum_q D_lq Ja^r_q
+    g.subn.assign((size_t)NE * 2 * NFP * N * 2, 0.0);
+    for (int e = 0; e < NE; e++)
+        for (int r = 0; r < 2; r++)
+            for (int l = 0; l < NFP; l++) {
+                const int n0 = line_node2(NP, r, l, 0);
+                for (int i = 1; i < NP; i++) {
+                    double nor[2];
+                    for (int d = 0; d < 2; d++) nor[d] = g.met[(2 * r + d) * NK + (size_t)e * K + n0];
+                    for (int ll = 0; ll < i; ll++) {
+                        double tmp[2] = {0.0, 0.0};
+                        for (int q = 0; q < NP; q++) {
+                            const int nq = line_node2(NP, r, l, q);
+                            for (int d = 0; d < 2; d++) tmp[d] += g.met[(2 * r + d) * NK + (size_t)e * K + nq] * D[ll * NP + q];
+                        }
+                        for (int d = 0; d < 2; d++) { tmp[d] *= w[ll]; nor[d] += tmp[d]; }
+                    }
+                    for (int d = 0; d < 2; d++)
+                        g.subn[((((size_t)e * 2 + r) * NFP + l) * N + (i - 1)) * 2 + d] = nor[d];
+                }
+            }
+    return true;
+}
+
+} // namespace quad
+} // namespace prandtl
+
+extern "C" {
+
+int prandtl_mesh_read(const char *path, prandtl_mesh_file **out)
+{
+    if (!path || !out) {
+        g_mesh_err = "null argument";
+        return PRANDTL_ERR_ARG;
+    }
+    *out = nullptr;
+    std::ifstream in(path);
+    if (!in) return mesh_fail(std::string("cannot open ") + path);
+    std::string head;
+    std::getline(in, head);
+    auto *m = new (std::nothrow) prandtl_mesh_file();
+    if (!m) {
+        g_mesh_err = "host allocation";
+        return PRANDTL_ERR_NOMEM;
+    }
+    int rc;
+    if (head.compare(0, 11, "$MeshFormat") == 0) rc = read_gmsh(in, *m);
+    else if (head.compare(0, 9, "MFEM mesh") == 0) rc = read_mfem(in, *m);
+    else rc = mesh_fail("unknown mesh format (MFEM mesh v1.0 or Gmsh 2.2 ASCII)");
+    if (rc != PRANDTL_OK) {
+        delete m;
+        return rc;
+    }
+    orient(*m);
+    *out = m;
+    return PRANDTL_OK;
+}
+
+int prandtl_mesh_refine(prandtl_mesh_file *m)
+{
+    if (!m) return PRANDTL_ERR_ARG;
+    const int ne = (int)(m->elem.size() / 4);
+    int nv = (int)(m->vert.size() / 2);
+    std::unordered_map<long long, int> mid;
+    std::vector<double> vert = m->vert;
+    auto midpoint = [&](int a, int b) {
+        const long long key = edge_key(a, b);
+        auto it = mid.find(key);
+        if (it != mid.end()) return it->second;
+        vert.push_back(0.5 * (m->vert[2 * (size_t)a] + m->vert[2 * (size_t)b]));
+        vert.push_back(0.5 * (m->vert[2 * (size_t)a + 1] + m->vert[2 * (size_t)b + 1]));
+        mid.emplace(key, nv);
+        return nv++;
+    };
+    std::vector<int> elem((size_t)ne * 16);
+    std::vector<double> ecoords(m->ecoords.empty() ? 0 : (size_t)ne * 32);
+    for (int e = 0; e < ne; e++) {
+        const int *v = &m->elem[(size_t)e * 4];
+        const int m01 = midpoint(v[0], v[1]), m12 = midpoint(v[1], v[2]);
+        const int m23 = midpoint(v[2], v[3]), m30 = midpoint(v[3], v[0]);
+        const int ctr = nv++;
+        double c[4][2];
+        corner(*m, e, 0, c[0][0], c[0][1]);
+        corner(*m, e, 1, c[1][0], c[1][1]);
+        corner(*m, e, 2, c[2][0], c[2][1]);
+        corner(*m, e, 3, c[3][0], c[3][1]);
+        vert.push_back(0.25 * (((c[0][0] + c[1][0]) + c[2][0]) + c[3][0]));
+        vert.push_back(0.25 * (((c[0][1] + c[1][1]) + c[2][1]) + c[3][1]));
+        const int child[4][4] = {{v[0], m01, ctr, m30}, {m01, v[1], m12, ctr},
+                                 {ctr, m12, v[2], m23}, {m30, ctr, m23, v[3]}};
+        // child corners in the parent's (xi, eta)
+        const double pc[4][4][2] = {{{0, 0}, {0.5, 0}, {0.5, 0.5}, {0, 0.5}},
+                                    {{0.5, 0}, {1, 0}, {1, 0.5}, {0.5, 0.5}},
+                                    {{0.5, 0.5}, {1, 0.5}, {1, 1}, {0.5, 1}},
+                                    {{0, 0.5}, {0.5, 0.5}, {0.5, 1}, {0, 1}}};
+        for (int k = 0; k < 4; k++)
+            for (int q = 0; q < 4; q++) {
+                elem[((size_t)e * 4 + k) * 4 + q] = child[k][q];
+                if (!ecoords.empty()) {
+                    const double xi = pc[k][q][0], eta = pc[k][q][1];
+                    for (int i = 0; i < 2; i++)
+                        ecoords[(((size_t)e * 4 + k) * 4 + q) * 2 + i] =
+                            (1 - xi) * (1 - eta) * c[0][i] + xi * (1 - eta) * c[1][i] + xi * eta * c[2][i] +
+                            (1 - xi) * eta * c[3][i];
+                }
+            }
+    }
+    std::vector<int> bdr, battr;
+    const int nb = (int)m->battr.size();
+    for (int i = 0; i < nb; i++) {
+        const int a = m->bdr[2 * i], b = m->bdr[2 * i + 1];
+        auto it = mid.find(edge_key(a, b));
+        if (it == mid.end()) {
+            // a segment that bounds no element stays whole
+            bdr.push_back(a); bdr.push_back(b); battr.push_back(m->battr[i]);
+            continue;
+        }
+        bdr.push_back(a); bdr.push_back(it->second); battr.push_back(m->battr[i]);
+        bdr.push_back(it->second); bdr.push_back(b); battr.push_back(m->battr[i]);
+    }
+    m->vert.swap(vert);
+    m->elem.swap(elem);
+    m->ecoords.swap(ecoords);
+    m->bdr.swap(bdr);
+    m->battr.swap(battr);
+    return PRANDTL_OK;
+}
+
+int prandtl_mesh_view(const prandtl_mesh_file *m, prandtl_quad_mesh *v)
+{
+    if (!m || !v) return PRANDTL_ERR_ARG;
+    v->nvert = (int)(m->vert.size() / 2);
+    v->vert = m->vert.data();
+    v->nelem = (int)(m->elem.size() / 4);
+    v->elem = m->elem.data();
+    v->elem_coords = m->ecoords.empty() ? nullptr : m->ecoords.data();
+    v->nbdr = (int)m->battr.size();
+    v->bdr = m->bdr.data();
+    v->bdr_attr = m->battr.data();
+    return PRANDTL_OK;
+}
+
+const char *prandtl_mesh_name(const prandtl_mesh_file *m, int i, int *attr)
+{
+    if (!m || i < 0 || i >= (int)m->names.size()) return nullptr;
+    if (attr) *attr = m->names[i].second;
+    return m->names[i].first.c_str();
+}
+
+void prandtl_mesh_free(prandtl_mesh_file *m) { delete m; }
+
+const char *prandtl_mesh_last_error(void) { return g_mesh_err.c_str(); }
+
+} // extern "C"

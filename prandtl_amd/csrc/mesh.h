@@ -1,0 +1,42 @@
+// Host-side geometry of unstructured 2-D quadrilateral meshes for k_rhs's
+// general-geometry variant.
+#ifndef PRANDTL_MESH_H
+#define PRANDTL_MESH_H
+
+#include <string>
+#include <vector>
+
+#include "../../include/prandtl_hip.h"
+
+namespace prandtl {
+namespace quad {
+
+// Per-face code (fface): bit 0 set when the element is the face's first
+// element (lower index: the owner of the normal's orientation), bits 1-3 the
+// boundary type (0 for an interior face), bits 4 and up the attribute - 1.
+constexpr int FACE_OWN = 1;
+inline int face_type(int code) { return (code >> 1) & 7; }
+inline int face_attr(int code) { return code >> 4; }
+
+// Local faces f = 2 r + side: r = 0 the xi = 0 / xi = 1 faces, r = 1 the
+// eta = 0 / eta = 1 faces; face point l runs along the other coordinate.
+struct Geometry {
+    int NE = 0, NP = 0, K = 0, NFP = 0;
+    std::vector<double> met;    // [4][NE*K]: adjugate row r, component d at 2r+d
+    std::vector<double> jac;    // [NE*K]: det of the Jacobian
+    std::vector<int> fface;     // [NE*4]
+    std::vector<int> fnb;       // [NE*4*NFP]: neighbour node e2*K + n2 (interior faces)
+    std::vector<double> fnor;   // [NE*4*NFP*2]: normal outward of the face's first element
+    std::vector<double> subn;   // [NE][2][NFP][NP-1][2]: subcell interface normals
+    double hmin = 0.0;
+};
+
+// x, w: LGL nodes and weights on [0,1]; D: derivative matrix, row-major
+// NP x NP.  Returns false with err set for a mesh it cannot use.
+bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
+           const std::vector<double> &w, const std::vector<double> &D, Geometry &g, std::string &err);
+
+} // namespace quad
+} // namespace prandtl
+
+#endif

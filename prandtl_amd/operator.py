@@ -101,9 +101,89 @@ def _stream_ptr():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+class QuadMesh:
+    """A 2-D quadrilateral mesh: vert [nv,2], elem [ne,4] (counterclockwise),
+    optional elem_coords [ne,4,2] (per-element corners of periodic meshes),
+    bdr [nb,2] boundary segments with bdr_attr [nb], names {name: attribute}."""
+
+    def __init__(self, vert, elem, bdr=None, bdr_attr=None, elem_coords=None, names=None):
+        self.vert = np.ascontiguousarray(vert, dtype=np.float64).reshape(-1, 2)
+        self.elem = np.ascontiguousarray(elem, dtype=np.int32).reshape(-1, 4)
+        self.elem_coords = None if elem_coords is None else \
+            np.ascontiguousarray(elem_coords, dtype=np.float64).reshape(-1, 4, 2)
+        self.bdr = np.zeros((0, 2), np.int32) if bdr is None else np.ascontiguousarray(bdr, dtype=np.int32).reshape(-1, 2)
+        self.bdr_attr = np.zeros(0, np.int32) if bdr_attr is None else np.ascontiguousarray(bdr_attr, dtype=np.int32)
+        self.names = dict(names or {})
+
+    @property
+    def nelem(self):
+        return len(self.elem)
+
+    @classmethod
+    def read(cls, path, refine=0):
+        """Read an MFEM v1.0 or Gmsh 2.2 file (prandtl_mesh_read) and refine it
+        `refine` times (ser_ref_levels)."""
+        h = ctypes.c_void_p()
+        rc = lib().prandtl_mesh_read(str(path).encode(), ctypes.byref(h))
+        if rc != _lib.PRANDTL_OK:
+            raise _lib.PrandtlError(rc, lib().prandtl_mesh_last_error().decode())
+        try:
+            for _ in range(refine):
+                check(lib().prandtl_mesh_refine(h))
+            v = _lib.QuadMesh()
+            check(lib().prandtl_mesh_view(h, ctypes.byref(v)))
+            vert = np.ctypeslib.as_array(v.vert, shape=(v.nvert * 2,)).copy() if v.nvert else np.zeros(0)
+            elem = np.ctypeslib.as_array(v.elem, shape=(v.nelem * 4,)).copy()
+            ec = np.ctypeslib.as_array(v.elem_coords, shape=(v.nelem * 8,)).copy() if v.elem_coords else None
+            bdr = np.ctypeslib.as_array(v.bdr, shape=(v.nbdr * 2,)).copy() if v.nbdr else None
+            battr = np.ctypeslib.as_array(v.bdr_attr, shape=(v.nbdr,)).copy() if v.nbdr else None
+            names, i = {}, 0
+            while True:
+                a = ctypes.c_int()
+                nm = lib().prandtl_mesh_name(h, i, ctypes.byref(a))
+                if nm is None:
+                    break
+                names[nm.decode()] = a.value
+                i += 1
+        finally:
+            lib().prandtl_mesh_free(h)
+        return cls(vert, elem, bdr, battr, ec, names)
+
+    def corners(self):
+        """[ne,4,2] corner coordinates."""
+        return self.elem_coords if self.elem_coords is not None else self.vert[self.elem]
+
+    def node_coords(self, order):
+        """Node coordinates X[2][NE][K] of the bilinear element maps."""
+        x, _ = lgl_nodes(order)
+        Np = order + 1
+        i = np.arange(Np * Np)
+        xi, eta = x[i % Np], x[i // Np]
+        N = np.stack([(1 - xi) * (1 - eta), xi * (1 - eta), xi * eta, (1 - xi) * eta])   # [4][K]
+        c = self.corners()
+        return np.einsum("ek d,kn->den".replace(" ", ""), c, N)
+
+    def attribute_tables(self, bc):
+        """bc: {name or attribute: type or (type, state)} -> (attr_bc, attr_state)."""
+        nattr = int(self.bdr_attr.max()) if len(self.bdr_attr) else 0
+        for key in bc or {}:
+            a = self.names[key] if isinstance(key, str) else int(key)
+            nattr = max(nattr, a)
+        attr_bc = np.zeros(max(nattr, 1), np.int32)
+        attr_state = np.zeros((max(nattr, 1), 4))
+        for key, val in (bc or {}).items():
+            a = self.names[key] if isinstance(key, str) else int(key)
+            kind, st = (val, None) if isinstance(val, (str, int)) else (val[0], val[1])
+            attr_bc[a - 1] = _bc_code(kind)
+            if st is not None:
+                attr_state[a - 1] = np.asarray(st, float)[:4]
+        return nattr, attr_bc, attr_state
+
+
 class DGSEMOperator:
     """One GPU's DGSEM semi-discrete operator on a Cartesian box, periodic or
-    with boundary conditions on its sides (bc, bc_state: see _descs)."""
+    with boundary conditions on its sides (bc, bc_state: see _descs), or on a
+    2-D quadrilateral mesh (DGSEMOperator.quad)."""
 
     def __init__(self, dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0,
                  parabolic=False, blending=False, alpha_max=0.5, alpha_min=0.001,
@@ -117,6 +197,36 @@ class DGSEMOperator:
         self._h = h
         self._owner = True
         self._describe(dim, order, nelem, lo, hi)
+
+    @classmethod
+    def quad(cls, mesh, order, bc=None, gamma=1.4, blending=False, alpha_max=0.5, alpha_min=0.001,
+             quirks=True, precision="fp64"):
+        """The operator on QuadMesh `mesh` (prandtl_create_quad), Euler;
+        bc maps boundary names or attributes to a type or (type, outer state)."""
+        _, p = _descs(2, order, (1, 1), (0.0, 0.0), (1.0, 1.0), gamma=gamma, blending=blending,
+                      alpha_max=alpha_max, alpha_min=alpha_min, quirks=quirks, precision=precision)
+        nattr, attr_bc, attr_state = mesh.attribute_tables(bc)
+        dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        q = _lib.QuadMesh()
+        q.nvert = len(mesh.vert)
+        q.vert = mesh.vert.ctypes.data_as(dp)
+        q.nelem = len(mesh.elem)
+        q.elem = mesh.elem.ctypes.data_as(ip)
+        q.elem_coords = mesh.elem_coords.ctypes.data_as(dp) if mesh.elem_coords is not None else dp()
+        q.nbdr = len(mesh.bdr_attr)
+        q.bdr = mesh.bdr.ctypes.data_as(ip)
+        q.bdr_attr = mesh.bdr_attr.ctypes.data_as(ip)
+        q.nattr = nattr
+        q.attr_bc = attr_bc.ctypes.data_as(ip)
+        q.attr_state = np.ascontiguousarray(attr_state).ctypes.data_as(dp)
+        h = ctypes.c_void_p()
+        check(lib().prandtl_create_quad(ctypes.byref(q), int(order), ctypes.byref(p), ctypes.byref(h)), None)
+        op = cls.__new__(cls)
+        op._h = h
+        op._owner = True
+        op._describe(2, order, (len(mesh.elem), 1), (0.0, 0.0), (1.0, 1.0))
+        op.mesh = mesh
+        return op
 
     @classmethod
     def _view(cls, handle, dim, order, nelem, lo, hi):
@@ -137,7 +247,9 @@ class DGSEMOperator:
         self.lo = tuple(float(v) for v in lo[:dim])
         self.hi = tuple(float(v) for v in hi[:dim])
         self.height = self.width = self.nv * self.NE * self.K
-        self.hmin = min((self.hi[r] - self.lo[r]) / self.nelem[r] for r in range(dim))
+        hm = ctypes.c_double()
+        check(lib().prandtl_hmin(h, ctypes.byref(hm)), h)
+        self.hmin = hm.value
 
     # -- lifetime ----------------------------------------------------------
     def close(self):

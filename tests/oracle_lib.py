@@ -51,6 +51,16 @@ def set_bcs(p, dim, bc, bc_state):
                 p.bc_state[r][side][v] = float(st[v]) if (st is not None and v < dim + 2) else 0.0
 
 
+class OrcQuadMesh(ctypes.Structure):
+    _fields_ = [("nvert", ctypes.c_int), ("vert", ctypes.POINTER(ctypes.c_double)),
+                ("nelem", ctypes.c_int), ("elem", ctypes.POINTER(ctypes.c_int)),
+                ("elem_coords", ctypes.POINTER(ctypes.c_double)),
+                ("nbdr", ctypes.c_int), ("bdr", ctypes.POINTER(ctypes.c_int)),
+                ("bdr_attr", ctypes.POINTER(ctypes.c_int)),
+                ("nattr", ctypes.c_int), ("attr_bc", ctypes.POINTER(ctypes.c_int)),
+                ("attr_state", ctypes.POINTER(ctypes.c_double))]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -70,6 +80,8 @@ def lib():
         L.orc_entgrad2primgrad.argtypes = [ctypes.c_int, _D, ctypes.c_int, _P, _P]
         L.orc_viscous_flux.argtypes = [ctypes.c_int, _D, _D, _D, _D, _P, _P, _P, _P, _P]
         L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(ctypes.c_void_p)]
+        L.orc_create_quad.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcQuadMesh),
+                                      ctypes.POINTER(ctypes.c_void_p)]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_sizes.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3
         for name in ("orc_coords", "orc_weights", "orc_alpha", "orc_eta", "orc_last_gradients"):
@@ -195,6 +207,56 @@ class Oracle:
         NE, K, nv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         lib().orc_sizes(h, ctypes.byref(NE), ctypes.byref(K), ctypes.byref(nv))
         self.NE, self.K, self.nv, self.dim = NE.value, K.value, nv.value, dim
+
+    @classmethod
+    def quad(cls, mesh, order, gamma=1.4, blending=False, alpha_max=0.5, alpha_min=0.001,
+             quirks=True, threads=1, ec_surface=False):
+        """The oracle on a quadrilateral mesh: `mesh` a dict with vert [nv,2],
+        elem [ne,4], optional elem_coords [ne,4,2], bdr [nb,2], bdr_attr [nb],
+        attr_bc [na] (names or codes), attr_state [na,4]."""
+        self = cls.__new__(cls)
+        p = OrcParams()
+        p.dim, p.order = 2, order
+        for r in range(3):
+            p.nelem[r], p.lo[r], p.hi[r] = 1, 0.0, 1.0
+        p.gamma, p.Pr, p.mu = gamma, 0.72, 0.0
+        p.parabolic, p.blending = 0, int(blending)
+        p.alpha_max, p.alpha_min = alpha_max, alpha_min
+        p.ec_surface, p.quirks, p.threads = int(ec_surface), int(quirks), int(threads)
+        p.sutherland, p.R_gas = 0, 287.05
+        set_bcs(p, 2, None, None)
+        keep = {}
+
+        def arr(name, dtype, ctype):
+            if mesh.get(name) is None:
+                return ctypes.POINTER(ctype)()
+            a = np.ascontiguousarray(mesh[name], dtype=dtype).reshape(-1)
+            keep[name] = a
+            return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+        q = OrcQuadMesh()
+        q.nvert = len(mesh["vert"]) if mesh.get("vert") is not None else 0
+        q.vert = arr("vert", np.float64, ctypes.c_double)
+        q.nelem = len(mesh["elem"])
+        q.elem = arr("elem", np.int32, ctypes.c_int)
+        q.elem_coords = arr("elem_coords", np.float64, ctypes.c_double)
+        q.nbdr = len(mesh["bdr_attr"]) if mesh.get("bdr_attr") is not None else 0
+        q.bdr = arr("bdr", np.int32, ctypes.c_int)
+        q.bdr_attr = arr("bdr_attr", np.int32, ctypes.c_int)
+        codes = [bc_code(t) for t in mesh.get("attr_bc", [])]
+        q.nattr = len(codes)
+        mesh = dict(mesh, attr_bc=codes if codes else None)
+        q.attr_bc = arr("attr_bc", np.int32, ctypes.c_int)
+        q.attr_state = arr("attr_state", np.float64, ctypes.c_double)
+        h = ctypes.c_void_p()
+        rc = lib().orc_create_quad(ctypes.byref(p), ctypes.byref(q), ctypes.byref(h))
+        if rc != 0:
+            raise ValueError("orc_create_quad failed: %d" % rc)
+        self.params, self.h = p, h
+        NE, K, nv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().orc_sizes(h, ctypes.byref(NE), ctypes.byref(K), ctypes.byref(nv))
+        self.NE, self.K, self.nv, self.dim = NE.value, K.value, nv.value, 2
+        return self
 
     def __del__(self):
         if getattr(self, "h", None):

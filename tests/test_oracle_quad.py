@@ -1,0 +1,178 @@
+"""Pins of the oracle on 2-D quadrilateral meshes (per-node metrics, face
+matching, normals, boundary faces by attribute).
+
+- A uniform grid given as a quad mesh reproduces the Cartesian operator
+  (periodic, and with every boundary type), so the general path restates
+  the pinned Cartesian one where they overlap.
+- Distorted meshes (bilinear elements with displaced vertices): a uniform
+  state is steady (the discrete metric identities hold for bilinear maps),
+  and sum_i w_i J_i RHS_i = 0 on a periodic mesh.
+- Rotating mesh and state together rotates the RHS (a transposed adjugate
+  or a wrong normal sign breaks this).
+- Renumbering elements and turning each element's local frame leaves the
+  RHS at every physical node unchanged (pins the face-point pairing for
+  both relative orientations).
+- With boundaries: sum_i w_i J_i RHS_i = -sum over boundary points of
+  w_face F_b with F_b from the point flux and the edge normal.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import quad_meshes as Q
+from prandtl_amd import cases
+
+G = 1.4
+
+
+def cons(rho, v, p):
+    v = np.asarray(v, float)
+    return np.array([rho, *(rho * v), p / (G - 1) + 0.5 * rho * np.dot(v, v)])
+
+
+def smooth(X, L=(1.0, 1.0), seed=4):
+    return cases.random_smooth_state(X, 2, seed, L=list(L))
+
+
+def jumpy(X):
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * X[0]) + 0.5 * (X[0] > 0.43)
+    u = 0.3 * np.cos(2 * np.pi * X[1])
+    v = 0.2 * np.sin(2 * np.pi * X[0] + 1)
+    p = 1.0 + 0.3 * (X[1] > 0.61) + 0.1 * np.cos(2 * np.pi * X[0])
+    return cases.prim_to_cons(rho, [u, v], p, G)
+
+
+@pytest.mark.parametrize("order,blend", [(2, False), (3, True), (5, False)])
+def test_uniform_periodic_grid_equals_cartesian(order, blend):
+    n = (5, 4)
+    box = O.Oracle(2, order, n, (0.0, 0.0), (1.0, 0.8), blending=blend)
+    quad = O.Oracle.quad(Q.grid(*n, hi=(1.0, 0.8)), order, blending=blend)
+    u = jumpy(box.coords())
+    assert np.allclose(quad.coords(), box.coords(), rtol=0, atol=1e-15)
+    rb, rq = box.Mult(u), quad.Mult(u)
+    assert np.abs(rb - rq).max() <= 1e-13 * np.abs(rb).max()
+    assert abs(box.GetMaxCharSpeed() - quad.GetMaxCharSpeed()) <= 1e-14
+
+
+@pytest.mark.parametrize("kinds", [("slip", "supersonic-outflow", "symmetry", "specified-state"),
+                                   ("supersonic-inflow", "slip", "slip", "symmetry")])
+def test_uniform_bounded_grid_equals_cartesian(kinds):
+    n, order = (4, 5), 3
+    g = cons(1.2, [0.7, -0.1], 1.1)
+    box = O.Oracle(2, order, n, (0.0, 0.0), (1.0, 1.0), bc=[kinds[:2], kinds[2:]],
+                   bc_state=[(g, g), (g, g)])
+    mesh = Q.grid(*n, periodic=(False, False))
+    mesh.update(attr_bc=list(kinds), attr_state=[g[:4]] * 4)
+    quad = O.Oracle.quad(mesh, order)
+    u = smooth(box.coords())
+    rb, rq = box.Mult(u), quad.Mult(u)
+    assert np.abs(rb - rq).max() <= 1e-13 * np.abs(rb).max()
+    assert abs(box.GetMaxCharSpeed() - quad.GetMaxCharSpeed()) <= 1e-14
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+def test_free_stream_on_distorted_mesh(order):
+    mesh = Q.grid(6, 5, perturb=0.25, seed=order)
+    for blend in (False, True):
+        orc = O.Oracle.quad(mesh, order, blending=blend)
+        u = np.repeat(cons(1.3, [0.4, -0.7], 2.0)[:, None], orc.NE * orc.K, axis=1)
+        assert np.abs(orc.Mult(u)).max() < 1e-11   # rounding; a metric-identity slip gives O(1e-3)
+
+
+@pytest.mark.parametrize("order,blend", [(3, False), (4, True)])
+def test_conservation_on_distorted_periodic_mesh(order, blend):
+    orc = O.Oracle.quad(Q.grid(6, 6, perturb=0.3, seed=11), order, blending=blend)
+    r = orc.Mult(jumpy(orc.coords())).reshape(4, orc.NE, orc.K)
+    WJ = orc.weights()
+    for v in range(4):
+        assert abs(np.sum(WJ * r[v])) < 1e-13 * np.sum(WJ * np.abs(r[v]))
+    if blend:
+        assert (orc.alpha() > 0).any()
+
+
+@pytest.mark.parametrize("order,blend", [(3, False), (4, True)])
+def test_rotation_equivariance(order, blend):
+    theta = 0.7
+    c, s = np.cos(theta), np.sin(theta)
+    R = np.array([[c, -s], [s, c]])
+    a = O.Oracle.quad(Q.grid(5, 4, perturb=0.2, seed=3), order, blending=blend)
+    b = O.Oracle.quad(Q.grid(5, 4, perturb=0.2, seed=3, rotate=theta), order, blending=blend)
+    Xa = a.coords()
+    assert np.allclose(np.einsum("ij,jek->iek", R, Xa), b.coords(), atol=1e-14)
+    ua = jumpy(Xa)
+    ub = ua.copy()
+    ub[1:3] = np.einsum("ij,jek->iek", R, ua[1:3])
+    ra = a.Mult(ua).reshape(4, a.NE, a.K)
+    rb = b.Mult(ub).reshape(4, b.NE, b.K)
+    ra_rot = ra.copy()
+    ra_rot[1:3] = np.einsum("ij,jek->iek", R, ra[1:3])
+    assert np.abs(ra_rot - rb).max() <= 1e-12 * np.abs(ra).max()
+    assert abs(a.GetMaxCharSpeed() - b.GetMaxCharSpeed()) <= 1e-13
+
+
+@pytest.mark.parametrize("periodic", [(True, True), (False, False)])
+def test_relabelled_elements_and_turned_frames(periodic):
+    order = 3
+    mesh = Q.grid(4, 4, periodic=periodic, perturb=0.2, seed=5)
+    g = cons(1.2, [0.5, 0.2], 1.0)
+    if not all(periodic):
+        mesh.update(attr_bc=["slip", "supersonic-outflow", "specified-state", "symmetry"], attr_state=[g] * 4)
+    mesh2, perm = Q.relabel(mesh, seed=9)
+    a = O.Oracle.quad(mesh, order, blending=True)
+    b = O.Oracle.quad(mesh2, order, blending=True)
+    Xa, Xb = a.coords(), b.coords()
+    K = a.K
+    # element k of b is element perm[k] of a: match its nodes by position
+    ua = jumpy(Xa)
+    idx = np.zeros((b.NE, K), dtype=np.int64)
+    for k in range(b.NE):
+        ea = perm[k]
+        for n in range(K):
+            d = np.abs(Xa[:, ea, :] - Xb[:, k, n][:, None]).sum(axis=0)
+            m = int(np.argmin(d))
+            assert d[m] < 1e-12
+            idx[k, n] = ea * K + m
+    ub = ua.reshape(4, -1)[:, idx.reshape(-1)].reshape(4, b.NE, K)
+    ra = a.Mult(ua).reshape(4, -1)
+    rb = b.Mult(ub).reshape(4, -1)
+    assert np.abs(ra[:, idx.reshape(-1)] - rb).max() <= 1e-12 * np.abs(ra).max()
+    assert np.array_equal(a.alpha()[perm], b.alpha())
+
+
+def test_boundary_balance_on_distorted_mesh():
+    order = 3
+    mesh = Q.grid(5, 4, periodic=(False, False), perturb=0.25, seed=8)
+    g = cons(1.2, [0.6, -0.3], 1.1)
+    kinds = ["slip", "supersonic-outflow", "symmetry", "specified-state"]
+    mesh.update(attr_bc=kinds, attr_state=[g] * 4)
+    orc = O.Oracle.quad(mesh, order)
+    X = orc.coords()
+    u = cases.prim_to_cons(1.0 + 0.2 * np.sin(3 * X[0] + 1), [0.3 * np.cos(2 * X[1]), 0.2 * np.sin(X[0])],
+                           1.0 + 0.2 * np.cos(3 * X[1]), G)
+    r = orc.Mult(u).reshape(4, orc.NE, orc.K)
+    WJ = orc.weights()
+    tot = np.array([np.sum(WJ * r[v]) for v in range(4)])
+    # expected: -sum over boundary points of w_q F_b, normal (dy, -dx) of the
+    # element's counterclockwise edge
+    x, w = O.lgl(order)
+    Np = order + 1
+    c = mesh["vert"][mesh["elem"]]
+    fv = {0: (0, 3), 1: (1, 2), 2: (0, 1), 3: (3, 2)}
+    ccw = {0: (3, 0), 1: (1, 2), 2: (0, 1), 3: (2, 3)}
+    seg = {tuple(sorted(s)): a for s, a in zip(mesh["bdr"].tolist(), mesh["bdr_attr"].tolist())}
+    U = u.reshape(4, orc.NE, orc.K)
+    expect = np.zeros(4)
+    for e in range(orc.NE):
+        for f in range(4):
+            key = tuple(sorted((mesh["elem"][e][fv[f][0]], mesh["elem"][e][fv[f][1]])))
+            if key not in seg:
+                continue
+            attr = seg[key]
+            P, Qp = c[e, ccw[f][0]], c[e, ccw[f][1]]
+            nor = np.array([Qp[1] - P[1], -(Qp[0] - P[0])])
+            r_, side = f >> 1, f & 1
+            for l in range(Np):
+                node = l * Np + (order if side else 0) if r_ == 0 else (order if side else 0) * Np + l
+                F, _ = O.bc_flux(2, G, 1, kinds[attr - 1], U[:, e, node], nor, outer=g)
+                expect -= w[l] * F
+    assert np.abs(tot - expect).max() < 1e-13 * max(np.abs(expect).max(), 1.0)
