@@ -34,19 +34,72 @@ WORKLOADS = {
     "kh2d_n4": dict(dim=2, order=4, nelem=(64, 64), lo=(-1.0, -1.0), hi=(1.0, 1.0),
                     blending=True, alpha_max=0.5, gamma=1.4),
     "iv2d_n3": dict(dim=2, order=3, nelem=(16, 16), lo=(-5.0, -5.0), hi=(5.0, 5.0), gamma=1.4),
+    # the reference's triple-point case: slip walls on all sides, blending
+    "triple2d_n3": dict(dim=2, order=3, nelem=(112, 48), lo=(0.0, 0.0), hi=(7.0, 3.0), gamma=1.4,
+                        blending=True, alpha_max=0.5, bc=[("slip", "slip"), ("slip", "slip")]),
+    # the reference's forward-facing step on its own mesh refined once
+    # (13600 quadrilaterals), Mach-3 inflow, outflow, slip walls, blending;
+    # the shock is formed by spin-up steps before the timed region
+    "ffs2d_n3": dict(dim=2, order=3, mesh="ForwardFacingStep.msh", refine=1, gamma=1.4, blending=True,
+                     alpha_max=0.5, spinup=300,
+                     bc={"left": ("supersonic-inflow", (1.4, 4.2, 0.0, 1.0 / 0.4 + 0.5 * 1.4 * 9.0)),
+                         "right": "supersonic-outflow", "top": "slip", "bottom": "slip"}),
 }
+MESH_DIR = os.path.join(ROOT, "tests", "golden", "meshes")
 # 64 independent 2-D Euler cases (isentropic vortices and blended
 # Kelvin-Helmholtz layers, meshes 8^2..64^2, N = 2..5) advanced together
 ENSEMBLE = "ensemble64"
 
 
+def quad_mesh(cfg):
+    from prandtl_amd.operator import QuadMesh
+    return QuadMesh.read(os.path.join(MESH_DIR, cfg["mesh"]), refine=cfg.get("refine", 0))
+
+
+def make_operator(cfg, quirks, precision):
+    from prandtl_amd import DGSEMOperator
+    if "mesh" in cfg:
+        return DGSEMOperator.quad(quad_mesh(cfg), cfg["order"], bc=cfg["bc"], gamma=cfg["gamma"],
+                                  blending=cfg.get("blending", False), alpha_max=cfg.get("alpha_max", 0.5),
+                                  quirks=quirks, precision=precision)
+    return DGSEMOperator(quirks=quirks, precision=precision, **cfg)
+
+
+def make_oracle(cfg, quirks, threads):
+    """The oracle on the same workload (test infrastructure, CPU baseline and op counts)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    if "mesh" in cfg:
+        m = quad_mesh(cfg)
+        nattr = int(m.bdr_attr.max())
+        kinds, states = ["slip"] * nattr, [[1.0, 0.0, 0.0, 2.5]] * nattr
+        for key, v in cfg["bc"].items():
+            a = m.names[key]
+            kinds[a - 1] = v if isinstance(v, str) else v[0]
+            if not isinstance(v, str):
+                states[a - 1] = list(v[1])
+        d = dict(vert=m.vert, elem=m.elem, elem_coords=m.elem_coords, bdr=m.bdr, bdr_attr=m.bdr_attr,
+                 attr_bc=kinds, attr_state=states)
+        return O.Oracle.quad(d, cfg["order"], gamma=cfg["gamma"], blending=cfg.get("blending", False),
+                             alpha_max=cfg.get("alpha_max", 0.5), quirks=quirks, threads=threads)
+    return O.Oracle(threads=threads, quirks=quirks, **cfg)
+
+
 def initial_state(name, cfg):
+    import numpy as np
     from prandtl_amd import cases
+    if "mesh" in cfg:
+        # the reference's ForwardFacingStepIC: the inflow state everywhere
+        m = quad_mesh(cfg)
+        K = (cfg["order"] + 1) ** 2
+        return np.repeat(np.asarray(cfg["bc"]["left"][1], float)[:, None], m.nelem * K, axis=1)
     X = cases.node_coords(cfg["dim"], cfg["order"], cfg["nelem"], cfg["lo"], cfg["hi"])
     if name.startswith("tgv"):
         return cases.taylor_green(X)
     if name.startswith("kh"):
         return cases.kelvin_helmholtz(X)
+    if name.startswith("triple"):
+        return cases.triple_point(X)
     return cases.isentropic_vortex(X)
 
 
@@ -64,10 +117,8 @@ def algorithmic_counts(name, quirks):
 
 def cpu_baseline(name, cfg, quirks, u, budget_s):
     """Oracle RHS on this host's cores, bounded sample of the same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
     threads = min(os.cpu_count() or 1, 16)
-    orc = O.Oracle(threads=threads, quirks=quirks, **cfg)
+    orc = make_oracle(cfg, quirks, threads)
     nodes = orc.NE * orc.K
     orc.Mult(u)                       # warm-up
     n, t0 = 0, time.perf_counter()
@@ -318,10 +369,14 @@ def main():
     # as a hipGraph on it and replayed
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.set_stream(stream)
-    op = DGSEMOperator(quirks=quirks, precision=args.precision, **cfg)
+    op = make_operator(cfg, quirks, args.precision)
     u_host = initial_state(name, cfg)
     u = torch.tensor(u_host.reshape(-1), dtype=torch.float64, device=dev)
     nodes = op.NE * op.K
+    if cfg.get("spinup"):
+        # untimed: develop the flow (shock and blending) from the uniform start
+        op.Run(u, 0.0, 1e30, cfl=1.0, max_steps=cfg["spinup"])
+        torch.cuda.synchronize()
 
     # fixed dt from CFL 0.1 of the initial wave speed (TGV config policy)
     z = torch.empty_like(u)
@@ -370,6 +425,15 @@ def main():
     if cfg.get("parabolic"):
         Np = cfg["order"] + 1
         by_rhs += 8 * (nv - 1) * (1 + 2 * cfg["dim"] * Np ** (cfg["dim"] - 1) / Np ** cfg["dim"])
+    if "mesh" in cfg:
+        # general geometry: adjugate rows and J per node, per face point the
+        # neighbour node (4 B) and normal (16 B), a code per face, and the
+        # subcell normals of blended elements
+        Np = cfg["order"] + 1
+        K = Np * Np
+        by_rhs += 8 * 5 + (4 * Np * 20 + 4 * 4) / K
+        if cfg.get("blending"):
+            by_rhs += 2 * Np * (Np - 1) * 2 * 8 / K
     kname = "k_rhs<%d,%d>" % (cfg["dim"], cfg["order"] + 1)
     traffic = measured_traffic(name, kname) if (args.precision == "fp64" and not quirks) else None
     roof = roofline(fl_rhs * nodes, by_rhs * nodes, t_rhs, kname, traffic)

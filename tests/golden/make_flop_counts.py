@@ -31,7 +31,7 @@ def main():
         if name == "tgv3d_n7":
             cfg["nelem"] = (8, 8, 8)
         for quirks in (True, False):
-            orc = O.Oracle(quirks=quirks, **cfg)
+            orc = bench.make_oracle(cfg, quirks, 1)
             u = bench.initial_state(name, cfg)
             c = orc.count_flops(u)
             nodes = orc.NE * orc.K
