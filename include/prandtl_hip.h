@@ -49,7 +49,11 @@ typedef struct prandtl_mesh_desc {
     /* conservative outer state (dim + 2 values) of supersonic-inflow and
      * specified-state sides: the constant-state constructors of
      * SupersonicInflowBdrFaceIntegrator.cpp:12-17 and
-     * SpecifiedStateBdrFaceIntegrator.cpp:12-16 */
+     * SpecifiedStateBdrFaceIntegrator.cpp:12-16; wall velocity and heat
+     * flux of no-slip sides (NoSlipAdiabWallBdrFaceIntegrator.cpp:10-15).
+     * Navier-Stokes boxes take supersonic inflow / outflow, specified state
+     * and no-slip walls (the reference has no viscous or lifting flux for
+     * slip and symmetry sides); Euler boxes every type but no-slip walls. */
     double bc_state[3][2][5];
 } prandtl_mesh_desc;
 
@@ -61,7 +65,10 @@ enum {
     PRANDTL_BC_SUPERSONIC_INFLOW = 2,  /* "supersonic-inflow", constant state */
     PRANDTL_BC_SUPERSONIC_OUTFLOW = 3, /* "supersonic-outflow": outer state = interior */
     PRANDTL_BC_SYMMETRY = 4,           /* "symmetry": mirrored normal momentum */
-    PRANDTL_BC_SPECIFIED_STATE = 5     /* "specified-state", constant state */
+    PRANDTL_BC_SPECIFIED_STATE = 5,    /* "specified-state", constant state */
+    PRANDTL_BC_NO_SLIP_ADIABATIC = 6   /* "no-slip-adiabatic" (Navier-Stokes): wall velocity in
+                                          bc_state[0..dim-1], normal heat flux in bc_state[3];
+                                          NoSlipAdiabWallBdrFaceIntegrator.cpp */
 };
 
 /* Physics and scheme switches: PhysicsConstants (Physics.hpp:10-37), the

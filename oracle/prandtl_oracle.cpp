@@ -434,6 +434,7 @@ T bc_flux(int dim, double gamma, int quirks, int type, const T *s1, const T *out
     for (int v = 0; v < nv; v++) s2[v] = s1[v];
     switch (type) {
     case ORC_BC_SLIP:
+    case ORC_BC_NO_SLIP_ADIABATIC:   /* inherits SlipWall's inviscid flux */
         return slip_wall_flux(dim, gamma, s1, nor, F);
     case ORC_BC_SUPERSONIC_INFLOW:
     case ORC_BC_SPECIFIED_STATE:
@@ -459,6 +460,51 @@ T bc_flux(int dim, double gamma, int quirks, int type, const T *s1, const T *out
         return 0.0;
     }
     return face_flux(dim, gamma, quirks, s1, s2, nor, F);
+}
+
+/* Entropy2Conserv, BasicOperations.cpp:106-122.  `Vector vel(ent_state.
+ * GetData() + 1, dim); vel /= beta` scales the caller's entropy vector in
+ * place (operator/= multiplies by 1/beta): on return w[1..dim] hold the
+ * velocities, which the faithful boundary lifting flux then subtracts. */
+template <class T> void entropy2conserv(int dim, double gamma, T *w, T *s)
+{
+    const double gm1 = gamma - 1.0, gm1i = 1.0 / gm1;
+    T beta = -w[dim + 1];
+    T ib = 1.0 / beta;
+    for (int d = 0; d < dim; d++) w[1 + d] = w[1 + d] * ib;
+    T vv = 0.0;
+    for (int d = 0; d < dim; d++) vv += w[1 + d] * w[1 + d];
+    T sent = gamma - (w[0] + 0.5 * beta * vv) * gm1;
+    s[0] = mpow(mexp(-sent) / beta, gm1i);
+    for (int d = 0; d < dim; d++) s[1 + d] = s[0] * w[1 + d];
+    s[dim + 1] = s[0] * (1.0 / beta * gm1i + 0.5 * vv);
+}
+
+template <class T> void cons2entropy(int dim, double gamma, const T *s, T *w);
+
+/* ComputeBdrFaceLiftingFlux.  Base (BdrFaceIntegrator.cpp:276-282):
+ * 0.5 (w(u_out) - w), u_out the outer state of the conservative interior
+ * state; quirks=1 keeps the velocities Entropy2Conserv leaves in w.  No-slip
+ * adiabatic wall (NoSlipAdiabWallBdrFaceIntegrator.cpp): the wall's entropy
+ * variables (w0, beta V, -beta) minus w, beta = -w_last, not halved. */
+template <class T>
+void bc_lift_flux(int dim, double gamma, int quirks, int type, const T *w, const T *outer, T *f)
+{
+    const int nv = dim + 2;
+    if (type == ORC_BC_NO_SLIP_ADIABATIC) {
+        T v = -w[nv - 1];
+        f[0] = w[0];
+        for (int d = 0; d < dim; d++) f[1 + d] = T(outer[d]) * v;
+        f[nv - 1] = -v;
+        for (int k = 0; k < nv; k++) f[k] -= w[k];
+        return;
+    }
+    T wi[5], cs[5], g[5];
+    for (int k = 0; k < nv; k++) wi[k] = w[k];
+    entropy2conserv(dim, gamma, wi, cs);
+    const T *ghost = (type == ORC_BC_SUPERSONIC_OUTFLOW) ? cs : outer;
+    cons2entropy(dim, gamma, ghost, g);
+    for (int k = 0; k < nv; k++) f[k] = (g[k] - (quirks ? wi[k] : w[k])) * 0.5;
 }
 
 /* ComputePressure, BasicOperations.cpp:31-37 (V /= rho multiplies by 1/rho) */
@@ -972,6 +1018,29 @@ template <class T> struct Engine {
                     }
                 }
             }
+        /* boundary faces, after the interior ones: LiftingBR1::
+         * AssembleLiftingBdrFaceVector (LiftingBR1.cpp:311-407) adds
+         * f n_c / (w_0 J) with the boundary type's lifting flux f */
+        for (int r = 0; r < dim; r++)
+            for (int side = 0; side < 2; side++) {
+                const int type = c->p.bc[r][side];
+                T outer[5];
+                for (int v = 0; v < nv; v++) outer[v] = c->p.bc_state[r][side][v];
+                for (int e : m.bel[r][side])
+                    for (int q = 0; q < Nfp; q++) {
+                        const int n1 = line_node(dim, Np, r, q, side ? N : 0);
+                        T wv[5], f[5];
+                        for (int v = 0; v < nv; v++) wv[v] = W[v * NK + (size_t)e * K + n1];
+                        bc_lift_flux(dim, gamma, c->p.quirks, type, wv, outer, f);
+                        T a1 = 1.0 / (w0 * T(m.J[(size_t)e * K + n1]));
+                        const double nr = side ? m.ad[r] : -m.ad[r];
+                        for (int v = 0; v < nv; v++)
+                            for (int cc = 0; cc < dim; cc++) {
+                                T fc = f[v] * (cc == r ? nr : 0.0);
+                                G[(cc * nv + v) * NK + (size_t)e * K + n1] += a1 * fc;
+                            }
+                    }
+            }
         for (int e = 0; e < NE; e++)
             for (int i = 0; i < K; i++) {
                 T st[5];
@@ -1220,9 +1289,43 @@ template <class T> struct Engine {
                 state(u, e, n1, s1);
                 g_cat = CAT_FACE;
                 lam = mmax(lam, bc_flux(dim, gamma, c->p.quirks, type, s1, outer, nor, F));
+                T d[5];
+                for (int v = 0; v < nv; v++) d[v] = -F[v];
+                if (c->p.parabolic) {
+                    /* ComputeBdrFaceViscousFlux (BdrFaceIntegrator.cpp:141-175):
+                     * -F* + Fv.n; Fv of the interior state and gradients
+                     * (outflow, wall) or of the outer state with zero
+                     * gradients (inflow, specified state); the adiabatic wall
+                     * replaces the energy term by V.(Fv.n)_mom + qn |n| */
+                    g_cat = CAT_VISC;
+                    T fl[15], dq[3][5], zero[5], fn[5];
+                    const T *dqp[3];
+                    const bool ext = type == ORC_BC_SUPERSONIC_INFLOW || type == ORC_BC_SPECIFIED_STATE;
+                    for (int v = 0; v < nv; v++) zero[v] = 0.0;
+                    for (int cc = 0; cc < dim; cc++) {
+                        for (int v = 0; v < nv; v++) dq[cc][v] = G[(cc * nv + v) * NK + (size_t)e * K + n1];
+                        dqp[cc] = ext ? zero : dq[cc];
+                    }
+                    T sv[5];
+                    for (int v = 0; v < nv; v++) sv[v] = ext ? outer[v] : s1[v];
+                    viscous_flux(dim, gamma, c->p.Pr, c->p.mu, c->p.sutherland ? c->p.R_gas : 0.0, sv, dqp, fl);
+                    for (int v = 0; v < nv; v++) {
+                        fn[v] = 0.0;
+                        for (int cc = 0; cc < dim; cc++) fn[v] += fl[v * dim + cc] * nor[cc];
+                    }
+                    if (type == ORC_BC_NO_SLIP_ADIABATIC) {
+                        T nn = 0.0;
+                        for (int cc = 0; cc < dim; cc++) nn += nor[cc] * nor[cc];
+                        T qn = T(c->p.bc_state[r][side][3]) * msqrt(nn);
+                        T en = T(c->p.bc_state[r][side][0]) * fn[1];
+                        for (int cc = 1; cc < dim; cc++) en = en + T(c->p.bc_state[r][side][cc]) * fn[1 + cc];
+                        fn[nv - 1] = en + qn;
+                    }
+                    for (int v = 0; v < nv; v++) d[v] += fn[v];
+                }
                 g_cat = CAT_ASM;
                 T a1 = 1.0 / (w0 * T(m.J[(size_t)e * K + n1]));
-                for (int v = 0; v < nv; v++) dudt[v * NK + (size_t)e * K + n1] += a1 * (-F[v]);
+                for (int v = 0; v < nv; v++) dudt[v * NK + (size_t)e * K + n1] += a1 * d[v];
             }
         }
         return lam;
@@ -1407,6 +1510,12 @@ double orc_bc_flux(int dim, double gamma, int quirks, int type, const double *s1
     return bc_flux<double>(dim, gamma, quirks, type, s1, outer, nor, F);
 }
 
+void orc_bc_lift_flux(int dim, double gamma, int quirks, int type, const double *w, const double *outer,
+                      double *f)
+{
+    bc_lift_flux<double>(dim, gamma, quirks, type, w, outer, f);
+}
+
 static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **out);
 
 int orc_create(const orc_params *p, orc_ctx **out) { return create_impl(p, nullptr, out); }
@@ -1426,11 +1535,19 @@ static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **o
     if (p->sutherland && !(p->R_gas > 0.0)) return 5;
     for (int r = 0; r < p->dim && !qm; r++) {
         for (int side = 0; side < 2; side++)
-            if (p->bc[r][side] < ORC_BC_PERIODIC || p->bc[r][side] > ORC_BC_SPECIFIED_STATE) return 6;
+            if (p->bc[r][side] < ORC_BC_PERIODIC || p->bc[r][side] > ORC_BC_NO_SLIP_ADIABATIC) return 6;
         if ((p->bc[r][0] == ORC_BC_PERIODIC) != (p->bc[r][1] == ORC_BC_PERIODIC)) return 6;
-        /* Navier-Stokes boundaries need the boundary lifting and viscous
-         * fluxes, not restated */
-        if (p->parabolic && p->bc[r][0] != ORC_BC_PERIODIC) return 7;
+        for (int side = 0; side < 2; side++) {
+            const int t = p->bc[r][side];
+            /* the reference has no viscous or lifting flux for slip and
+             * symmetry sides (BdrFaceIntegrator.cpp:247-270 raise), and
+             * no-slip walls without viscous terms are not used */
+            if (p->parabolic && (t == ORC_BC_SLIP || t == ORC_BC_SYMMETRY)) return 7;
+            if (!p->parabolic && t == ORC_BC_NO_SLIP_ADIABATIC) return 7;
+            /* NoSlipAdiabWallBdrFaceIntegrator.cpp:27 scales its member qn by
+             * |n| on every call: only qn = 0 is reproducible */
+            if (p->quirks && t == ORC_BC_NO_SLIP_ADIABATIC && p->bc_state[r][side][3] != 0.0) return 7;
+        }
     }
     orc_ctx *c = new (std::nothrow) orc_ctx();
     if (!c) return 4;

@@ -47,7 +47,9 @@ enum {
     ORC_BC_SUPERSONIC_INFLOW = 2,  /* SupersonicInflowBdrFaceIntegrator.cpp (constant state) */
     ORC_BC_SUPERSONIC_OUTFLOW = 3, /* SupersonicOutflowBdrFaceIntegrator.cpp */
     ORC_BC_SYMMETRY = 4,           /* SymmetryBdrFaceIntegrator.cpp */
-    ORC_BC_SPECIFIED_STATE = 5     /* SpecifiedStateBdrFaceIntegrator.cpp (constant state) */
+    ORC_BC_SPECIFIED_STATE = 5,    /* SpecifiedStateBdrFaceIntegrator.cpp (constant state) */
+    ORC_BC_NO_SLIP_ADIABATIC = 6   /* NoSlipAdiabWallBdrFaceIntegrator.cpp (constant wall velocity
+                                      bc_state[0..dim-1] and heat flux bc_state[3]) */
 };
 
 /* 1-D rules and operators (setup arithmetic of DGSEMIntegrator.cpp:8-82) */
@@ -73,6 +75,10 @@ void orc_viscous_flux(int dim, double gamma, double Pr, double mu, double R_gas,
  * face normal nor (face-measure scaled); returns the wave speed */
 double orc_bc_flux(int dim, double gamma, int quirks, int type, const double *s1, const double *outer,
                    const double *nor, double *F);
+/* ComputeBdrFaceLiftingFlux of boundary type `type` for the interior entropy
+ * variables w (BdrFaceIntegrator.cpp:276-282, NoSlipAdiabWallBdrFaceIntegrator.cpp) */
+void orc_bc_lift_flux(int dim, double gamma, int quirks, int type, const double *w, const double *outer,
+                      double *f);
 
 /* operator */
 int orc_create(const orc_params *p, orc_ctx **out);

@@ -53,7 +53,7 @@ class QuadMesh(ctypes.Structure):
 
 # boundary types (PRANDTL_BC_*) by the reference's configuration names
 BC_TYPES = {"periodic": 0, "slip": 1, "supersonic-inflow": 2, "supersonic-outflow": 3,
-            "symmetry": 4, "specified-state": 5}
+            "symmetry": 4, "specified-state": 5, "no-slip-adiabatic": 6}
 
 
 class Params(ctypes.Structure):

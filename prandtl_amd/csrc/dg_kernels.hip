@@ -589,11 +589,33 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             const int bt = boundary_of(A.geo, eo, r, side);
             if (bt != BC_PERIODIC) {
                 double F[NV];
-                const double lb = boundary_flux<DIM>(bt, r, side ? nr : -nr, nr, so,
-                                                     A.geo.bcs + (2 * r + side) * 5, P, quirk3d, F);
+                const double ns = side ? nr : -nr;
+                const double *bs = A.geo.bcs + (2 * r + side) * 5;
+                const double lb = boundary_flux<DIM>(bt, r, ns, nr, so, bs, P, quirk3d, F);
                 lam = (lam < lb) ? lb : lam;
+                double d[NV];
 #pragma unroll
-                for (int v = 0; v < NV; v++) sFc[ej][v][rem] = -F[v];
+                for (int v = 0; v < NV; v++) d[v] = -F[v];
+                if (A.parabolic && (bt == BC_OUTFLOW || bt == BC_NOSLIP)) {
+                    // + Fv.n of the interior state and gradients (this
+                    // element's trace from k_visc); inflow and specified
+                    // states have zero gradients, so no viscous flux
+                    // (BdrFaceIntegrator.cpp:141-175); the no-slip wall's energy
+                    // term is V.(Fv.n)_mom + qn |n|
+                    double fn[NV];
+#pragma unroll
+                    for (int v = 1; v < NV; v++) fn[v] = pfo[v] * ns;
+                    if (bt == BC_NOSLIP) {
+                        double en = bs[0] * fn[1];
+#pragma unroll
+                        for (int dd = 1; dd < DIM; dd++) en = en + bs[dd] * fn[1 + dd];
+                        fn[NV - 1] = en + bs[3] * sqrt(ns * ns);
+                    }
+#pragma unroll
+                    for (int v = 1; v < NV; v++) d[v] += fn[v];
+                }
+#pragma unroll
+                for (int v = 0; v < NV; v++) sFc[ej][v][rem] = d[v];
                 continue;
             }
         }
@@ -876,7 +898,9 @@ __global__ void k_set_dts(double *dst, int n, const DtArgs a)
 // FP32 on those offsets (D annihilates constants, so the shift leaves the
 // derivatives unchanged while keeping their significant digits).  The
 // outputs visc and fvf are FP32 in that form.
-template <int DIM, int NP, bool MIXED, int FB>
+// OPEN: a box with boundary sides; their face points take the boundary
+// type's lifting flux (LiftingBR1.cpp:311-407) instead of the neighbour jump.
+template <int DIM, int NP, bool MIXED, int FB, bool OPEN = false>
 __global__ void __launch_bounds__((VCfg<DIM, NP, FB>::BLOCK))
 __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArgs A)
 {
@@ -936,8 +960,13 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
             const int rem = j - ej * (2 * DIM * NFP);
             const int f = rem / NFP, q = rem - f * NFP;
             const int r = f >> 1, side = f & 1;
-            const int en = neighbor(A.geo, eo, r, side);
-            const int nn = line_node<DIM, NP>(r, q, side ? 0 : NP - 1);
+            int en = neighbor(A.geo, eo, r, side);
+            int nn = line_node<DIM, NP>(r, q, side ? 0 : NP - 1);
+            if (OPEN && boundary_of(A.geo, eo, r, side) != BC_PERIODIC) {
+                // boundary point: this element's own face node
+                en = eo;
+                nn = line_node<DIM, NP>(r, q, side ? NP - 1 : 0);
+            }
 #pragma unroll
             for (int v = 0; v < NV; v++) tv[k][v] = A.y[v * NK + (size_t)en * K + nn];
         }
@@ -966,8 +995,17 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
             const int f = rem / NFP, q = rem - f * NFP;
             double w[NV];
             cons2entropy<DIM>(tv[k], P, w);
+            const int bt = OPEN ? boundary_of(A.geo, eo, f >> 1, f & 1) : BC_PERIODIC;
+            if (OPEN && bt != BC_PERIODIC) {
+                // the lifting flux itself (not a trace) for the surface part
+                double fb[NV];
+                lift_flux<DIM>(bt, w, A.geo.bcs + f * 5, P, A.quirks != 0, fb);
 #pragma unroll
-            for (int v = 0; v < NV; v++) sWN[ej][f][v][q] = MIXED ? (R)(w[v] - sWref[ej][v]) : (R)w[v];
+                for (int v = 0; v < NV; v++) sWN[ej][f][v][q] = (R)fb[v];
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; v++) sWN[ej][f][v][q] = MIXED ? (R)(w[v] - sWref[ej][v]) : (R)w[v];
+            }
         }
     }
     __syncthreads();
@@ -997,12 +1035,19 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
             // surface part: +1/2 (w2 - w1) n / (w0 J) on both sides (LiftingBR1.cpp:37-65)
             if (a == 0 || a == NP - 1) {
                 const int side = (a == NP - 1);
+                if (OPEN && boundary_of(A.geo, e, c, side) != BC_PERIODIC) {
+                    // f n_c / (w0 J) with the outward normal (LiftingBR1.cpp:311-407)
+                    const R nc = side ? nrm[c] : -nrm[c];
 #pragma unroll
-                for (int v = 0; v < NV; v++) {
-                    const R wo = sW[el][v][n];
-                    const R wn = sWN[el][2 * c + side][v][l];
-                    const R f = side ? half * (wn - wo) : half * (wo - wn);
-                    gp[c][v] += aface * (f * nrm[c]);
+                    for (int v = 0; v < NV; v++) gp[c][v] += aface * (sWN[el][2 * c + side][v][l] * nc);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < NV; v++) {
+                        const R wo = sW[el][v][n];
+                        const R wn = sWN[el][2 * c + side][v][l];
+                        const R f = side ? half * (wn - wo) : half * (wo - wn);
+                        gp[c][v] += aface * (f * nrm[c]);
+                    }
                 }
             }
         }
@@ -1399,6 +1444,15 @@ template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, doub
     a.y = y; a.visc = c->d_visc; a.fvf = c->d_fvf; a.grad = grad;
     a.skip = c->skip;
     ProfScope ps(c, s, 1);
+    if (c->geo.bcmask) {
+        // boundary sides: the open variant, natural shape
+        using Cf = VCfg<DIM, NP, 0>;
+        const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+        if (c->prm.precision == PRANDTL_MIXED)
+            hipLaunchKernelGGL((k_visc<DIM, NP, true, 0, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_visc<DIM, NP, false, 0, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+        return hip_check(c, hipGetLastError(), "k_visc launch");
+    }
     switch (c->visc_shape) {
     case 1:
         if constexpr (visc_shape_ok(DIM, NP, 1)) launch_visc_fb<DIM, NP, 64>(c, a, s);
@@ -1441,10 +1495,12 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
         }
     }
     if (c->geo.bcmask) {
-        // boundary conditions (Euler only), natural shape
+        // boundary sides, natural shape (not with the faithful NS-3D form)
         using Cf = RCfg<DIM, NP, BLEND>;
         const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
-        hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+        if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
+            hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, float, 0, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
         return hip_check(c, hipGetLastError(), "k_rhs launch");
     }
     if (qns3) {
@@ -1541,7 +1597,8 @@ int rhs_only(prandtl_ctx *c, const double *y, double *out, hipStream_t s)
 
 bool shape_ok(const prandtl_ctx *c, int kernel, int shape)
 {
-    if (kernel == PRANDTL_KERNEL_VISC) return c->prm.parabolic && visc_shape_ok(c->dim, c->Np, shape);
+    if (kernel == PRANDTL_KERNEL_VISC)
+        return c->prm.parabolic && (shape == 0 || (!c->geo.bcmask && visc_shape_ok(c->dim, c->Np, shape)));
     if (kernel != PRANDTL_KERNEL_RHS) return false;
     if (shape == 0) return true;
     if (c->dim == 3 && c->prm.parabolic && c->prm.quirks) return false;
@@ -1610,8 +1667,16 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     for (int r = 0; r < dim && !qm; r++) {
         for (int side = 0; side < 2; side++) {
             const int t = mesh->bc[r][side];
-            if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_SPECIFIED_STATE)
+            if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_NO_SLIP_ADIABATIC)
                 return fail(nullptr, PRANDTL_ERR_ARG, "unknown boundary type");
+            if (prm->parabolic && (t == PRANDTL_BC_SLIP || t == PRANDTL_BC_SYMMETRY))
+                return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
+                            "slip and symmetry sides have no viscous flux (Navier-Stokes: no-slip walls)");
+            if (!prm->parabolic && t == PRANDTL_BC_NO_SLIP_ADIABATIC)
+                return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "no-slip walls need parabolic = 1");
+            if (prm->quirks && t == PRANDTL_BC_NO_SLIP_ADIABATIC && mesh->bc_state[r][side][3] != 0.0)
+                return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
+                            "faithful mode reproduces no-slip walls with zero heat flux only");
             if (t == PRANDTL_BC_SUPERSONIC_INFLOW || t == PRANDTL_BC_SPECIFIED_STATE) {
                 const double *g = mesh->bc_state[r][side];
                 double ke = 0.0;
@@ -1625,8 +1690,8 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
             return fail(nullptr, PRANDTL_ERR_ARG, "a direction is periodic on both sides or on neither");
         open = open || mesh->bc[r][0] != PRANDTL_BC_PERIODIC;
     }
-    if (open && prm->parabolic)
-        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "boundary conditions are built for Euler (parabolic = 0)");
+    if (open && prm->parabolic && dim == 3 && prm->quirks)
+        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "boundary sides are not built with the faithful NS-3D form");
     std::vector<double> qstates;   // quadrilateral meshes: outer states per attribute
     if (qm) {
         if (qm->nattr < 0 || (qm->nattr > 0 && !qm->attr_bc)) return fail(nullptr, PRANDTL_ERR_ARG, "bad attribute table");

@@ -224,7 +224,8 @@ __device__ __forceinline__ void two_point_flux_v(const double *nv, double nd, co
 }
 
 // Boundary types (PRANDTL_BC_*)
-enum { BC_PERIODIC = 0, BC_SLIP = 1, BC_INFLOW = 2, BC_OUTFLOW = 3, BC_SYMMETRY = 4, BC_SPECIFIED = 5 };
+enum { BC_PERIODIC = 0, BC_SLIP = 1, BC_INFLOW = 2, BC_OUTFLOW = 3, BC_SYMMETRY = 4, BC_SPECIFIED = 5,
+       BC_NOSLIP = 6 };
 
 // SlipWallBdrFaceIntegrator::ComputeBdrFaceInviscidFlux
 // (SlipWallBdrFaceIntegrator.cpp:15-60) for the outward normal ns e_r.  The
@@ -351,7 +352,8 @@ __device__ __forceinline__ double boundary_flux(int type, int r, double ns, doub
                                                 const double *outer, const Phys &P, bool quirk3d,
                                                 double *F)
 {
-    if (type == BC_SLIP) return slip_wall_flux<DIM>(r, ns, s, P, F);
+    // no-slip walls inherit the slip wall's inviscid flux
+    if (type == BC_SLIP || type == BC_NOSLIP) return slip_wall_flux<DIM>(r, ns, s, P, F);
     double g[DIM + 2];
 #pragma unroll
     for (int v = 0; v < DIM + 2; v++) g[v] = (type == BC_INFLOW || type == BC_SPECIFIED) ? outer[v] : s[v];
@@ -391,6 +393,51 @@ __device__ __forceinline__ void cons2entropy(const double *s, const Phys &P, dou
 #pragma unroll
     for (int d = 0; d < DIM; d++) w[1 + d] = beta * vel[d];
     w[DIM + 1] = -beta;
+}
+
+// ComputeBdrFaceLiftingFlux of a boundary point for the interior entropy
+// variables w.  No-slip wall (NoSlipAdiabWallBdrFaceIntegrator.cpp): the
+// wall's (w0, beta V, -beta) minus w, beta = -w_last, V = outer[0..DIM-1].
+// Base (BdrFaceIntegrator.cpp:276-282): Entropy2Conserv (BasicOperations.cpp:
+// 106-122) of w, the outer state (the state itself for outflow, outer for
+// inflow / specified state), Conserv2Entropy, minus w, halved.  quirks:
+// Entropy2Conserv's in-place `vel /= beta` leaves velocities in w's momentum
+// slots, and those are what is subtracted.
+template <int DIM>
+__device__ __forceinline__ void lift_flux(int type, const double *w, const double *outer, const Phys &P,
+                                          bool quirks, double *f)
+{
+    constexpr int NV = DIM + 2;
+    if (type == BC_NOSLIP) {
+        const double v = -w[NV - 1];
+        f[0] = w[0] - w[0];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) f[1 + d] = outer[d] * v - w[1 + d];
+        f[NV - 1] = -v - w[NV - 1];
+        return;
+    }
+    double wi[NV], cs[NV], g[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) wi[k] = w[k];
+    const double beta = -wi[NV - 1];
+    const double ib = 1.0 / beta;
+    double vv = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        wi[1 + d] = wi[1 + d] * ib;
+        vv = (d == 0) ? wi[1] * wi[1] : vv + wi[1 + d] * wi[1 + d];
+    }
+    const double sent = P.gamma - (wi[0] + 0.5 * beta * vv) * P.gm1;
+    cs[0] = pow(exp(-sent) / beta, P.gm1i);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) cs[1 + d] = cs[0] * wi[1 + d];
+    cs[NV - 1] = cs[0] * (1.0 / beta * P.gm1i + 0.5 * vv);
+    double gs[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) gs[k] = (type == BC_OUTFLOW) ? cs[k] : outer[k];
+    cons2entropy<DIM>(gs, P, g);
+#pragma unroll
+    for (int k = 0; k < NV; k++) f[k] = (g[k] - (quirks ? wi[k] : w[k])) * 0.5;
 }
 
 // EntropyGrad2PrimGrad, BasicOperations.cpp:126-170, one direction in place,

@@ -32,7 +32,7 @@ class OrcParams(ctypes.Structure):
 # boundary types by the names of the reference's configuration files
 # (Simulation.cpp:518-735); the values are the oracle's ORC_BC_* codes
 BC_CODES = {"periodic": 0, "slip": 1, "supersonic-inflow": 2, "supersonic-outflow": 3,
-            "symmetry": 4, "specified-state": 5}
+            "symmetry": 4, "specified-state": 5, "no-slip-adiabatic": 6}
 
 
 def bc_code(x):
@@ -76,6 +76,7 @@ def lib():
         L.orc_face_flux.argtypes = [ctypes.c_int, _D, ctypes.c_int, _P, _P, _P, _P]
         L.orc_bc_flux.restype = _D
         L.orc_bc_flux.argtypes = [ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P]
+        L.orc_bc_lift_flux.argtypes = [ctypes.c_int, _D, ctypes.c_int, ctypes.c_int, _P, _P, _P]
         L.orc_cons2entropy.argtypes = [ctypes.c_int, _D, _P, _P]
         L.orc_entgrad2primgrad.argtypes = [ctypes.c_int, _D, ctypes.c_int, _P, _P]
         L.orc_viscous_flux.argtypes = [ctypes.c_int, _D, _D, _D, _D, _P, _P, _P, _P, _P]
@@ -156,6 +157,14 @@ def bc_flux(dim, gamma, quirks, kind, s1, nor, outer=None):
     lam = lib().orc_bc_flux(dim, gamma, quirks, bc_code(kind), ptr(np.ascontiguousarray(s1, float)),
                             ptr(o), ptr(np.ascontiguousarray(nor, float)), ptr(F))
     return F, lam
+
+
+def bc_lift_flux(dim, gamma, quirks, kind, w, outer=None):
+    """Boundary lifting flux of type `kind` for interior entropy variables w."""
+    f = np.zeros(dim + 2)
+    o = np.zeros(5) if outer is None else np.ascontiguousarray(outer, float)
+    lib().orc_bc_lift_flux(dim, gamma, quirks, bc_code(kind), ptr(np.ascontiguousarray(w, float)), ptr(o), ptr(f))
+    return f
 
 
 def cons2entropy(dim, gamma, s):
