@@ -877,7 +877,7 @@ int build_mesh_quad(orc_ctx *c, const orc_quad_mesh *q)
             const int attr = it->second;
             if (attr < 1 || attr > q->nattr) return 8;
             const int type = q->attr_bc[attr - 1];
-            if (type < ORC_BC_SLIP || type > ORC_BC_SPECIFIED_STATE) return 8;
+            if (type < ORC_BC_SLIP || type > ORC_BC_NO_SLIP_ADIABATIC) return 8;
             m.bq_e.push_back(e1);
             m.bq_type.push_back(type);
             m.bq_attr.push_back(attr - 1);
@@ -1041,6 +1041,24 @@ template <class T> struct Engine {
                             }
                     }
             }
+        /* the same on quadrilateral meshes, each point with its own normal */
+        for (size_t b = 0; b < m.bq_e.size(); b++) {
+            const int e = m.bq_e[b];
+            T outer[5];
+            for (int v = 0; v < nv; v++) outer[v] = m.attr_state[5 * (size_t)m.bq_attr[b] + v];
+            for (int q = 0; q < Nfp; q++) {
+                const int n1 = m.bq_n[b * Nfp + q];
+                T wv[5], f[5];
+                for (int v = 0; v < nv; v++) wv[v] = W[v * NK + (size_t)e * K + n1];
+                bc_lift_flux(dim, gamma, c->p.quirks, m.bq_type[b], wv, outer, f);
+                T a1 = 1.0 / (w0 * T(m.J[(size_t)e * K + n1]));
+                for (int v = 0; v < nv; v++)
+                    for (int cc = 0; cc < dim; cc++) {
+                        T fc = f[v] * m.bq_nor[(b * Nfp + q) * 2 + cc];
+                        G[(cc * nv + v) * NK + (size_t)e * K + n1] += a1 * fc;
+                    }
+            }
+        }
         for (int e = 0; e < NE; e++)
             for (int i = 0; i < K; i++) {
                 T st[5];
@@ -1347,10 +1365,39 @@ template <class T> struct Engine {
                 nor[0] = m.bq_nor[((size_t)b * Nfp + q) * 2];
                 nor[1] = m.bq_nor[((size_t)b * Nfp + q) * 2 + 1];
                 g_cat = CAT_FACE;
-                lam = mmax(lam, bc_flux(dim, gamma, c->p.quirks, m.bq_type[b], s1, outer, nor, F));
+                const int type = m.bq_type[b];
+                lam = mmax(lam, bc_flux(dim, gamma, c->p.quirks, type, s1, outer, nor, F));
+                T d[5];
+                for (int v = 0; v < nv; v++) d[v] = -F[v];
+                if (c->p.parabolic) {
+                    /* as bfaces: -F* + Fv.n */
+                    g_cat = CAT_VISC;
+                    T fl[15], dq[3][5], zero[5], fn[5], sv[5];
+                    const T *dqp[3];
+                    const bool ext = type == ORC_BC_SUPERSONIC_INFLOW || type == ORC_BC_SPECIFIED_STATE;
+                    for (int v = 0; v < nv; v++) { zero[v] = 0.0; sv[v] = ext ? outer[v] : s1[v]; }
+                    for (int cc = 0; cc < dim; cc++) {
+                        for (int v = 0; v < nv; v++) dq[cc][v] = G[(cc * nv + v) * NK + (size_t)e * K + n1];
+                        dqp[cc] = ext ? zero : dq[cc];
+                    }
+                    viscous_flux(dim, gamma, c->p.Pr, c->p.mu, c->p.sutherland ? c->p.R_gas : 0.0, sv, dqp, fl);
+                    for (int v = 0; v < nv; v++) {
+                        fn[v] = 0.0;
+                        for (int cc = 0; cc < dim; cc++) fn[v] += fl[v * dim + cc] * nor[cc];
+                    }
+                    if (type == ORC_BC_NO_SLIP_ADIABATIC) {
+                        T nn = 0.0;
+                        for (int cc = 0; cc < dim; cc++) nn += nor[cc] * nor[cc];
+                        T qn = outer[3] * msqrt(nn);
+                        T en = outer[0] * fn[1];
+                        for (int cc = 1; cc < dim; cc++) en = en + outer[cc] * fn[1 + cc];
+                        fn[nv - 1] = en + qn;
+                    }
+                    for (int v = 0; v < nv; v++) d[v] += fn[v];
+                }
                 g_cat = CAT_ASM;
                 T a1 = 1.0 / (w0 * T(m.J[(size_t)e * K + n1]));
-                for (int v = 0; v < nv; v++) dudt[v * NK + (size_t)e * K + n1] += a1 * (-F[v]);
+                for (int v = 0; v < nv; v++) dudt[v * NK + (size_t)e * K + n1] += a1 * d[v];
             }
         }
         return lam;
@@ -1522,7 +1569,14 @@ int orc_create(const orc_params *p, orc_ctx **out) { return create_impl(p, nullp
 
 int orc_create_quad(const orc_params *p, const orc_quad_mesh *m, orc_ctx **out)
 {
-    if (!p || !m || !out || p->dim != 2 || p->parabolic) return 1;
+    if (!p || !m || !out || p->dim != 2) return 1;
+    for (int a = 0; a < m->nattr; a++) {
+        const int t = m->attr_bc[a];
+        if (p->parabolic && (t == ORC_BC_SLIP || t == ORC_BC_SYMMETRY)) return 7;
+        if (!p->parabolic && t == ORC_BC_NO_SLIP_ADIABATIC) return 7;
+        if (p->quirks && t == ORC_BC_NO_SLIP_ADIABATIC && m->attr_state && m->attr_state[4 * (size_t)a + 3] != 0.0)
+            return 7;
+    }
     return create_impl(p, m, out);
 }
 

@@ -80,6 +80,7 @@ struct Geo {
     const double *jac;    // [NE*K] Jacobian determinant
     const int *fface;     // [NE*4] bit 0: first element of the face; bits 1-3 boundary type; 4+: attribute-1
     const int *fnb;       // [NE*4*NFP] neighbour node e2*K + n2 of each interior face point
+    const int *fnf;       // [NE*4] 2 (e2*4 + f2) + reversed: the neighbour's face of each interior face
     const double *fnor;   // [NE*4*NFP*2] normal outward of the face's first element
     const double *subn;   // [NE][2][NFP][NP-1][2] subcell interface normals
 };
@@ -560,12 +561,28 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             const int code = A.geo.fface[efo];
             const int bt = (code >> 1) & 7;
             const double nvec[2] = {A.geo.fnor[(efo * NFP + l) * 2], A.geo.fnor[(efo * NFP + l) * 2 + 1]};
+            // viscous traces Fv.n (the face's normal) of this side, and of the
+            // neighbour's matching point
+            const VT *fvfp = static_cast<const VT *>(A.fvf);
             double F[NV];
             if (bt != BC_PERIODIC) {
-                const double lb = boundary_flux_v2(bt, nvec, so, A.geo.bcs + (code >> 4) * 5, P, F);
+                const double *bs = A.geo.bcs + (code >> 4) * 5;
+                const double lb = boundary_flux_v2(bt, nvec, so, bs, P, F);
                 lam = (lam < lb) ? lb : lam;
+                double d[NV];
 #pragma unroll
-                for (int v = 0; v < NV; v++) sFc[ej][v][rem] = -F[v];
+                for (int v = 0; v < NV; v++) d[v] = -F[v];
+                if (A.parabolic && (bt == BC_OUTFLOW || bt == BC_NOSLIP)) {
+                    double fn[NV];
+#pragma unroll
+                    for (int v = 1; v < NV; v++) fn[v] = (double)fvfp[(efo * (NV - 1) + (v - 1)) * NFP + l];
+                    if (bt == BC_NOSLIP)
+                        fn[NV - 1] = (bs[0] * fn[1] + bs[1] * fn[2]) + bs[3] * sqrt(nvec[0] * nvec[0] + nvec[1] * nvec[1]);
+#pragma unroll
+                    for (int v = 1; v < NV; v++) d[v] += fn[v];
+                }
+#pragma unroll
+                for (int v = 0; v < NV; v++) sFc[ej][v][rem] = d[v];
                 continue;
             }
             const int gn = A.geo.fnb[efo * NFP + l];
@@ -574,11 +591,34 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             const bool own = code & 1;
             double *g1 = own ? so : pn;
             double *g2 = own ? pn : so;
+            double fvn[NV];
+            fvn[0] = 0.0;
+            if (A.parabolic) {
+                const int nf = A.geo.fnf[efo];
+                const size_t fid = (size_t)(nf >> 1);
+                const int l2 = (nf & 1) ? NP - 1 - l : l;
+#pragma unroll
+                for (int v = 1; v < NV; v++)
+                    fvn[v] = ((double)fvfp[(efo * (NV - 1) + (v - 1)) * NFP + l] +
+                              (double)fvfp[(fid * (NV - 1) + (v - 1)) * NFP + l2]) * 0.5;
+                if (A.quirks) {
+                    // ComputeViscousFlux left velocities in the momentum slots
+                    // (DGSEMIntegrator.cpp:471-481)
+                    const double r1 = 1.0 / g1[0], r2 = 1.0 / g2[0];
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) { g1[1 + d] *= r1; g2[1 + d] *= r2; }
+                }
+            }
             const NodeQ<DIM> q1 = node_quants<DIM>(g1, P);
             const NodeQ<DIM> q2 = node_quants<DIM>(g2, P);
             two_point_flux_v<DIM, true>(nvec, norml2_2(nvec), q1, q2, g1, g2, P, false, F);
 #pragma unroll
-            for (int v = 0; v < NV; v++) sFc[ej][v][rem] = own ? -F[v] : F[v];
+            for (int v = 0; v < NV; v++) {
+                double d;
+                if (A.parabolic) d = own ? (-F[v] + fvn[v]) : (F[v] - fvn[v]);
+                else d = own ? -F[v] : F[v];
+                sFc[ej][v][rem] = d;
+            }
             continue;
         }
         if constexpr (OPEN) {
@@ -900,10 +940,13 @@ __global__ void k_set_dts(double *dst, int n, const DtArgs a)
 // outputs visc and fvf are FP32 in that form.
 // OPEN: a box with boundary sides; their face points take the boundary
 // type's lifting flux (LiftingBR1.cpp:311-407) instead of the neighbour jump.
-template <int DIM, int NP, bool MIXED, int FB, bool OPEN = false>
+// GEN: a 2-D quadrilateral mesh (per-node metrics, face tables; traces are
+// Fv.n with the face's normal).
+template <int DIM, int NP, bool MIXED, int FB, bool OPEN = false, bool GEN = false>
 __global__ void __launch_bounds__((VCfg<DIM, NP, FB>::BLOCK))
 __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArgs A)
 {
+    static_assert(!GEN || DIM == 2, "general geometry is 2-D");
     using C = VCfg<DIM, NP, FB>;
     using R = typename std::conditional<MIXED, float, double>::type;
     if (A.skip && *A.skip) return;   // uniform: before any barrier
@@ -951,6 +994,12 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
     if (act)
 #pragma unroll
         for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)e * K + n];
+    double jn = 1.0, mown[4] = {0.0, 0.0, 0.0, 0.0};
+    if (GEN && act) {
+        jn = A.geo.jac[(size_t)e * K + n];
+#pragma unroll
+        for (int c = 0; c < 4; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
+    }
 #pragma unroll
     for (int k = 0; k < JT; k++) {
         const int j = t + k * BLOCK;
@@ -966,6 +1015,17 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
                 // boundary point: this element's own face node
                 en = eo;
                 nn = line_node<DIM, NP>(r, q, side ? NP - 1 : 0);
+            }
+            if constexpr (GEN) {
+                const size_t efo = (size_t)eo * 4 + f;
+                if ((A.geo.fface[efo] >> 1) & 7) {
+                    en = eo;
+                    nn = line_node<DIM, NP>(r, q, side ? NP - 1 : 0);
+                } else {
+                    const int gn = A.geo.fnb[efo * NFP + q];
+                    en = gn / K;
+                    nn = gn - en * K;
+                }
             }
 #pragma unroll
             for (int v = 0; v < NV; v++) tv[k][v] = A.y[v * NK + (size_t)en * K + nn];
@@ -995,11 +1055,12 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
             const int f = rem / NFP, q = rem - f * NFP;
             double w[NV];
             cons2entropy<DIM>(tv[k], P, w);
-            const int bt = OPEN ? boundary_of(A.geo, eo, f >> 1, f & 1) : BC_PERIODIC;
-            if (OPEN && bt != BC_PERIODIC) {
+            const int code = GEN ? A.geo.fface[(size_t)eo * 4 + f] : 0;
+            const int bt = GEN ? ((code >> 1) & 7) : OPEN ? boundary_of(A.geo, eo, f >> 1, f & 1) : BC_PERIODIC;
+            if ((OPEN || GEN) && bt != BC_PERIODIC) {
                 // the lifting flux itself (not a trace) for the surface part
                 double fb[NV];
-                lift_flux<DIM>(bt, w, A.geo.bcs + f * 5, P, A.quirks != 0, fb);
+                lift_flux<DIM>(bt, w, A.geo.bcs + (GEN ? (code >> 4) : f) * 5, P, A.quirks != 0, fb);
 #pragma unroll
                 for (int v = 0; v < NV; v++) sWN[ej][f][v][q] = (R)fb[v];
             } else {
@@ -1014,7 +1075,63 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
     R yr[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) yr[v] = act ? (R)ys[v] : R(1);
-    if (act) {
+    if constexpr (GEN) if (act) {
+        // grad w = (sum_r (D-sum along r) adj row r) / J, then the face terms
+        // f n / (w0 J) with each face point's normal (LiftingBR1.cpp:10-272)
+        const R invJ = (R)(1.0 / jn), aface = (R)(1.0 / (A.tab.w[0] * jn)), half = 0.5;
+        R sd[DIM][NV];
+#pragma unroll
+        for (int c = 0; c < DIM; c++) {
+            const int a = ii[c];
+            const int l = line_of<DIM, NP>(c, ii[0], ii[1], ii[2]);
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                R s = 0;
+#pragma unroll
+                for (int m = 0; m < NP; m++)
+                    s = (m == 0) ? sW[el][v][line_node<DIM, NP>(c, l, m)] * sD[a * NP + m]
+                                 : s + sW[el][v][line_node<DIM, NP>(c, l, m)] * sD[a * NP + m];
+                sd[c][v] = s;
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < DIM; cc++)
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                gp[cc][v] = (sd[0][v] * (R)mown[cc] + sd[1][v] * (R)mown[2 + cc]) * invJ;
+#pragma unroll
+        for (int c = 0; c < DIM; c++) {
+            const int a = ii[c];
+            if (a == 0 || a == NP - 1) {
+                const int side = (a == NP - 1), f = 2 * c + side;
+                const int l = line_of<DIM, NP>(c, ii[0], ii[1], ii[2]);
+                const size_t ef = (size_t)e * 4 + f;
+                const int code = A.geo.fface[ef];
+                const R nv2[2] = {(R)A.geo.fnor[(ef * NFP + l) * 2], (R)A.geo.fnor[(ef * NFP + l) * 2 + 1]};
+                const bool bnd = ((code >> 1) & 7) != BC_PERIODIC;
+                const bool own = code & 1;
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    R fv;
+                    if (bnd) fv = sWN[el][f][v][l];
+                    else {
+                        const R wo = sW[el][v][n], wn = sWN[el][f][v][l];
+                        fv = own ? half * (wn - wo) : half * (wo - wn);
+                    }
+#pragma unroll
+                    for (int cc = 0; cc < DIM; cc++) gp[cc][v] += aface * (fv * nv2[cc]);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < DIM; c++) entgrad2primgrad<DIM, R>(yr, gm1, gm1i, A.quirks != 0, gp[c]);
+        if (A.grad)
+#pragma unroll
+            for (int c = 0; c < DIM; c++)
+#pragma unroll
+                for (int v = 0; v < NV; v++) A.grad[(c * NV + v) * NK + (size_t)e * K + n] = (double)gp[c][v];
+    }
+    if (!GEN && act) {
         const R nrm[3] = {(R)A.geo.nrm[0], (R)A.geo.nrm[1], (R)A.geo.nrm[2]};
         const R invJ = (R)A.geo.invJ, aface = (R)A.geo.aface, half = 0.5;
         // volume part: grad_r w = sum_l D_{i_r l} w_l, times adj / J
@@ -1071,7 +1188,17 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
 #pragma unroll
         for (int r = 0; r < DIM; r++)
 #pragma unroll
-            for (int v = 1; v < NV; v++) sFV[el][r][v - 1][n] = (R)A.geo.nrm[r] * fl[v - 1][r];
+            for (int v = 1; v < NV; v++) {
+                if constexpr (GEN) {
+                    // contravariant: adj row r times the physical flux
+                    R a = (R)mown[2 * r] * fl[v - 1][0];
+#pragma unroll
+                    for (int c = 1; c < DIM; c++) a += (R)mown[2 * r + c] * fl[v - 1][c];
+                    sFV[el][r][v - 1][n] = a;
+                } else {
+                    sFV[el][r][v - 1][n] = (R)A.geo.nrm[r] * fl[v - 1][r];
+                }
+            }
         bool onface = false;
 #pragma unroll
         for (int c = 0; c < DIM; c++) onface |= (ii[c] == 0 || ii[c] == NP - 1);
@@ -1083,8 +1210,16 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
                     const int side = (ii[c] == NP - 1);
                     const int l = line_of<DIM, NP>(c, ii[0], ii[1], ii[2]);
                     R *dst = fvf + (((size_t)e * 2 * DIM + 2 * c + side) * (NV - 1)) * NFP;
+                    if constexpr (GEN) {
+                        // Fv.n with the face's normal (outward of its first element)
+                        const size_t ef = (size_t)e * 4 + 2 * c + side;
+                        const R n0 = (R)A.geo.fnor[(ef * NFP + l) * 2], n1 = (R)A.geo.fnor[(ef * NFP + l) * 2 + 1];
 #pragma unroll
-                    for (int v = 1; v < NV; v++) dst[(v - 1) * NFP + l] = fl[v - 1][c];
+                        for (int v = 1; v < NV; v++) dst[(v - 1) * NFP + l] = fl[v - 1][0] * n0 + fl[v - 1][1] * n1;
+                    } else {
+#pragma unroll
+                        for (int v = 1; v < NV; v++) dst[(v - 1) * NFP + l] = fl[v - 1][c];
+                    }
                 }
             }
         }
@@ -1444,6 +1579,17 @@ template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, doub
     a.y = y; a.visc = c->d_visc; a.fvf = c->d_fvf; a.grad = grad;
     a.skip = c->skip;
     ProfScope ps(c, s, 1);
+    if constexpr (DIM == 2 && NP <= 6) {
+        if (c->gen) {
+            // quadrilateral mesh, natural shape
+            using Cf = VCfg<DIM, NP, 0>;
+            const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+            if (c->prm.precision == PRANDTL_MIXED)
+                hipLaunchKernelGGL((k_visc<DIM, NP, true, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+            else hipLaunchKernelGGL((k_visc<DIM, NP, false, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+            return hip_check(c, hipGetLastError(), "k_visc launch");
+        }
+    }
     if (c->geo.bcmask) {
         // boundary sides: the open variant, natural shape
         using Cf = VCfg<DIM, NP, 0>;
@@ -1487,10 +1633,12 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
     const bool qns3 = DIM == 3 && c->prm.parabolic && c->prm.quirks;
     if constexpr (DIM == 2 && NP <= 6) {
         if (c->gen) {
-            // quadrilateral mesh (Euler), natural shape
+            // quadrilateral mesh, natural shape
             using Cf = RCfg<DIM, NP, BLEND>;
             const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
-            hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+            if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
+                hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, float, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+            else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
             return hip_check(c, hipGetLastError(), "k_rhs launch");
         }
     }
@@ -1598,7 +1746,7 @@ int rhs_only(prandtl_ctx *c, const double *y, double *out, hipStream_t s)
 bool shape_ok(const prandtl_ctx *c, int kernel, int shape)
 {
     if (kernel == PRANDTL_KERNEL_VISC)
-        return c->prm.parabolic && (shape == 0 || (!c->geo.bcmask && visc_shape_ok(c->dim, c->Np, shape)));
+        return c->prm.parabolic && (shape == 0 || (!c->geo.bcmask && !c->gen && visc_shape_ok(c->dim, c->Np, shape)));
     if (kernel != PRANDTL_KERNEL_RHS) return false;
     if (shape == 0) return true;
     if (c->dim == 3 && c->prm.parabolic && c->prm.quirks) return false;
@@ -1652,8 +1800,6 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         return fail(nullptr, PRANDTL_ERR_ARG, "R_gas must be positive with Sutherland viscosity");
     long long NE = 1;
     if (qm) {
-        if (prm->parabolic)
-            return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "quadrilateral meshes are built for Euler (parabolic = 0)");
         if (qm->nelem < 1) return fail(nullptr, PRANDTL_ERR_ARG, "empty mesh");
         NE = qm->nelem;
     }
@@ -1698,8 +1844,21 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         qstates.assign(5 * (size_t)std::max(qm->nattr, 1), 0.0);
         for (int a = 0; a < qm->nattr; a++) {
             const int t = qm->attr_bc[a];
-            if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_SPECIFIED_STATE)
+            if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_NO_SLIP_ADIABATIC)
                 return fail(nullptr, PRANDTL_ERR_ARG, "unknown boundary type");
+            if (prm->parabolic && (t == PRANDTL_BC_SLIP || t == PRANDTL_BC_SYMMETRY))
+                return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
+                            "slip and symmetry sides have no viscous flux (Navier-Stokes: no-slip walls)");
+            if (!prm->parabolic && t == PRANDTL_BC_NO_SLIP_ADIABATIC)
+                return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "no-slip walls need parabolic = 1");
+            if (t == PRANDTL_BC_NO_SLIP_ADIABATIC) {
+                if (!qm->attr_state) return fail(nullptr, PRANDTL_ERR_ARG, "wall state missing");
+                const double *g = qm->attr_state + 4 * (size_t)a;
+                if (prm->quirks && g[3] != 0.0)
+                    return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
+                                "faithful mode reproduces no-slip walls with zero heat flux only");
+                for (int v = 0; v < 4; v++) qstates[5 * (size_t)a + v] = g[v];
+            }
             if (t == PRANDTL_BC_SUPERSONIC_INFLOW || t == PRANDTL_BC_SPECIFIED_STATE) {
                 if (!qm->attr_state) return fail(nullptr, PRANDTL_ERR_ARG, "boundary state missing");
                 const double *g = qm->attr_state + 4 * (size_t)a;
@@ -1852,14 +2011,14 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (e == hipSuccess && qm) {
         // general geometry tables, one allocation
         auto bytes = [](const auto &v) { return v.size() * sizeof(v[0]); };
-        const size_t parts[7] = {bytes(qg.met), bytes(qg.jac), bytes(qg.fnor), bytes(qg.subn), bytes(qstates),
-                                 bytes(qg.fface), bytes(qg.fnb)};
-        size_t off[8] = {0};
-        for (int i = 0; i < 7; i++) off[i + 1] = off[i] + ((parts[i] + 255) / 256) * 256;
-        e = hipMalloc(&c->d_geo, off[7]);
-        const void *src[7] = {qg.met.data(), qg.jac.data(), qg.fnor.data(), qg.subn.data(), qstates.data(),
-                              qg.fface.data(), qg.fnb.data()};
-        for (int i = 0; i < 7 && e == hipSuccess; i++)
+        const size_t parts[8] = {bytes(qg.met), bytes(qg.jac), bytes(qg.fnor), bytes(qg.subn), bytes(qstates),
+                                 bytes(qg.fface), bytes(qg.fnb), bytes(qg.fnf)};
+        size_t off[9] = {0};
+        for (int i = 0; i < 8; i++) off[i + 1] = off[i] + ((parts[i] + 255) / 256) * 256;
+        e = hipMalloc(&c->d_geo, off[8]);
+        const void *src[8] = {qg.met.data(), qg.jac.data(), qg.fnor.data(), qg.subn.data(), qstates.data(),
+                              qg.fface.data(), qg.fnb.data(), qg.fnf.data()};
+        for (int i = 0; i < 8 && e == hipSuccess; i++)
             e = hipMemcpy(static_cast<char *>(c->d_geo) + off[i], src[i], parts[i], hipMemcpyHostToDevice);
         char *b = static_cast<char *>(c->d_geo);
         c->geo.met = reinterpret_cast<const double *>(b + off[0]);
@@ -1869,6 +2028,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         c->geo.bcs = reinterpret_cast<const double *>(b + off[4]);
         c->geo.fface = reinterpret_cast<const int *>(b + off[5]);
         c->geo.fnb = reinterpret_cast<const int *>(b + off[6]);
+        c->geo.fnf = reinterpret_cast<const int *>(b + off[7]);
     }
     if (e == hipSuccess) e = hipMemset(c->d_alpha, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_eta, 0, sizeof(double) * c->NE);

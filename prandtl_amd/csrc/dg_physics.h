@@ -323,7 +323,7 @@ __device__ __forceinline__ double slip_wall_flux_v2(const double *n, const doubl
 __device__ __forceinline__ double boundary_flux_v2(int type, const double *n, const double *s, const double *outer,
                                                    const Phys &P, double *F)
 {
-    if (type == BC_SLIP) return slip_wall_flux_v2(n, s, P, F);
+    if (type == BC_SLIP || type == BC_NOSLIP) return slip_wall_flux_v2(n, s, P, F);
     const double nd = norml2_2(n);
     double g[4];
 #pragma unroll

@@ -357,6 +357,7 @@ bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
     for (int i = 0; i < m->nbdr; i++) battr[edge_key(m->bdr[2 * i], m->bdr[2 * i + 1])] = m->bdr_attr[i];
     g.fface.assign((size_t)NE * 4, 0);
     g.fnb.assign((size_t)NE * 4 * NFP, -1);
+    g.fnf.assign((size_t)NE * 4, -1);
     g.fnor.assign((size_t)NE * 4 * NFP * 2, 0.0);
     for (int e = 0; e < NE; e++)
         for (int f = 0; f < 4; f++) {
@@ -372,6 +373,7 @@ bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
                 const int s_nb = m->elem[(size_t)e2 * 4 + FACE_VERT[f2][0]];
                 const bool same = s_me == s_nb;
                 const int r2 = f2 >> 1, side2 = f2 & 1;
+                g.fnf[ef] = 2 * (e2 * 4 + f2) + (same ? 0 : 1);
                 for (int l = 0; l < NFP; l++) {
                     const int l2 = same ? l : N - l;
                     g.fnb[ef * NFP + l] = e2 * K + line_node2(NP, r2, l2, side2 ? N : 0);
@@ -389,7 +391,7 @@ bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
                     return false;
                 }
                 const int t = m->attr_bc[attr - 1];
-                if (t < PRANDTL_BC_SLIP || t > PRANDTL_BC_SPECIFIED_STATE) {
+                if (t < PRANDTL_BC_SLIP || t > PRANDTL_BC_NO_SLIP_ADIABATIC) {
                     err = "boundary attribute " + std::to_string(attr) + " needs a non-periodic boundary type";
                     return false;
                 }

@@ -26,6 +26,7 @@ struct Geometry {
     std::vector<double> jac;    // [NE*K]: det of the Jacobian
     std::vector<int> fface;     // [NE*4]
     std::vector<int> fnb;       // [NE*4*NFP]: neighbour node e2*K + n2 (interior faces)
+    std::vector<int> fnf;       // [NE*4]: 2 (e2*4 + f2) + reversed, the neighbour's face (interior faces)
     std::vector<double> fnor;   // [NE*4*NFP*2]: normal outward of the face's first element
     std::vector<double> subn;   // [NE][2][NFP][NP-1][2]: subcell interface normals
     double hmin = 0.0;

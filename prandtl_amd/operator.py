@@ -200,11 +200,15 @@ class DGSEMOperator:
 
     @classmethod
     def quad(cls, mesh, order, bc=None, gamma=1.4, blending=False, alpha_max=0.5, alpha_min=0.001,
-             quirks=True, precision="fp64"):
-        """The operator on QuadMesh `mesh` (prandtl_create_quad), Euler;
-        bc maps boundary names or attributes to a type or (type, outer state)."""
+             quirks=True, precision="fp64", parabolic=False, mu=0.0, Pr=0.72, sutherland=False,
+             R_gas=287.05):
+        """The operator on QuadMesh `mesh` (prandtl_create_quad); bc maps
+        boundary names or attributes to a type or (type, state): the outer
+        conservative state of inflow / specified-state boundaries, or
+        (V0, V1, 0, qn) of a no-slip wall."""
         _, p = _descs(2, order, (1, 1), (0.0, 0.0), (1.0, 1.0), gamma=gamma, blending=blending,
-                      alpha_max=alpha_max, alpha_min=alpha_min, quirks=quirks, precision=precision)
+                      alpha_max=alpha_max, alpha_min=alpha_min, quirks=quirks, precision=precision,
+                      parabolic=parabolic, mu=mu, Pr=Pr, sutherland=sutherland, R_gas=R_gas)
         nattr, attr_bc, attr_state = mesh.attribute_tables(bc)
         dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
         q = _lib.QuadMesh()

@@ -219,7 +219,8 @@ class Oracle:
 
     @classmethod
     def quad(cls, mesh, order, gamma=1.4, blending=False, alpha_max=0.5, alpha_min=0.001,
-             quirks=True, threads=1, ec_surface=False):
+             quirks=True, threads=1, ec_surface=False, parabolic=False, mu=0.0, Pr=0.72,
+             sutherland=False, R_gas=287.05):
         """The oracle on a quadrilateral mesh: `mesh` a dict with vert [nv,2],
         elem [ne,4], optional elem_coords [ne,4,2], bdr [nb,2], bdr_attr [nb],
         attr_bc [na] (names or codes), attr_state [na,4]."""
@@ -228,11 +229,11 @@ class Oracle:
         p.dim, p.order = 2, order
         for r in range(3):
             p.nelem[r], p.lo[r], p.hi[r] = 1, 0.0, 1.0
-        p.gamma, p.Pr, p.mu = gamma, 0.72, 0.0
-        p.parabolic, p.blending = 0, int(blending)
+        p.gamma, p.Pr, p.mu = gamma, Pr, mu
+        p.parabolic, p.blending = int(parabolic), int(blending)
         p.alpha_max, p.alpha_min = alpha_max, alpha_min
         p.ec_surface, p.quirks, p.threads = int(ec_surface), int(quirks), int(threads)
-        p.sutherland, p.R_gas = 0, 287.05
+        p.sutherland, p.R_gas = int(sutherland), float(R_gas)
         set_bcs(p, 2, None, None)
         keep = {}
 

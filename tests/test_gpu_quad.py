@@ -187,3 +187,122 @@ def test_forward_facing_step_reference_mesh(cuda):
     r = rel_max(out.cpu().numpy(), orc.Mult(u.cpu().numpy()))
     print("ffs rhs rel", r, flush=True)
     assert r <= TOL
+
+
+# ---------------------------------------------------------------- Navier-Stokes
+NS = dict(parabolic=True, mu=0.02, Pr=0.71)
+WALL = (0.0, 0.0, 0.0, 0.0)
+LID = (0.4, 0.0, 0.0, 0.0)
+
+NS_CASES = {
+    "ns_periodic_n3_blend": (dict(periodic=(True, True), perturb=0.25, seed=31), 3, True, False, None),
+    "ns_periodic_n4_faithful": (dict(periodic=(True, True), perturb=0.2, seed=32), 4, False, True, None),
+    "ns_cavity_n3": (dict(periodic=(False, False), perturb=0.2, seed=33), 3, False, False,
+                     (["no-slip-adiabatic"] * 4, [WALL, WALL, WALL, LID])),
+    "ns_channel_n5_faithful": (dict(periodic=(False, True), perturb=0.15, seed=34), 5, True, True,
+                               (["supersonic-inflow", "supersonic-outflow", "no-slip-adiabatic", "no-slip-adiabatic"],
+                                [IN, IN, WALL, WALL])),
+}
+
+
+def ns_build(name):
+    kw, order, blend, quirks, bcs = NS_CASES[name]
+    d = Q.grid(5, 4, **kw)
+    if bcs:
+        d.update(attr_bc=bcs[0], attr_state=bcs[1])
+    return d, order, blend, quirks
+
+
+@pytest.mark.parametrize("name", list(NS_CASES))
+def test_quad_ns_parity(cuda, name):
+    import torch
+    d, order, blend, quirks = ns_build(name)
+    orc = O.Oracle.quad(d, order, blending=blend, quirks=quirks, threads=8, **NS)
+    op = DGSEMOperator.quad(qmesh(d), order, bc=bc_dict(d), blending=blend, quirks=quirks, **NS)
+    u = state(orc)
+    du = to_dev(u, cuda)
+    out = torch.empty_like(du)
+    op.Mult(du, out)
+    g = op.gradients(du)
+    torch.cuda.synchronize()
+    r = rel_max(out.cpu().numpy(), orc.Mult(u))
+    rg = rel_max(g.cpu().numpy(), orc.gradients())
+    print(name, "rhs rel", r, "grad rel", rg, flush=True)
+    assert rg <= TOL and r <= TOL
+
+
+def test_quad_ns_mixed_and_graph(cuda):
+    import torch
+    d, order, blend, quirks = ns_build("ns_cavity_n3")
+    bc = bc_dict(d)
+    fp = DGSEMOperator.quad(qmesh(d), order, bc=bc, quirks=False, **NS)
+    mx = DGSEMOperator.quad(qmesh(d), order, bc=bc, quirks=False, precision="mixed", **NS)
+    orc = O.Oracle.quad(d, order, quirks=False, **NS)
+    u = to_dev(state(orc), cuda)
+    a, b = torch.empty_like(u), torch.empty_like(u)
+    fp.Mult(u, a)
+    mx.Mult(u, b)
+    torch.cuda.synchronize()
+    rel = float(torch.linalg.norm(b - a) / torch.linalg.norm(a))
+    print("mixed vs fp64", rel, flush=True)
+    assert rel < 1e-6
+    ua, ub = u.clone(), u.clone()
+    s = torch.cuda.Stream(device=cuda)
+    for _ in range(3):
+        with torch.cuda.stream(s):
+            fp.Step(ua, 1e-4)
+        mx.Step(ub, 1e-4)
+    torch.cuda.synchronize()
+    ue = u.clone()
+    fe = DGSEMOperator.quad(qmesh(d), order, bc=bc, quirks=False, **NS)
+    for _ in range(3):
+        fe.Step(ue, 1e-4)
+    torch.cuda.synchronize()
+    assert torch.equal(ua, ue)
+
+
+def test_lid_driven_cavity_reference_mesh(cuda):
+    """The reference's LidDrivenCavity case on its own mesh (256 quadrilaterals,
+    N = 3, blending, mu = 0.02, Pr = 0.72, Ma = 0.1 initial state at rest,
+    fixed dt = 1e-4, no-slip adiabatic walls, lid velocity (1, 0)): 300
+    steps stay physical, the walls keep the mass constant, the lid drives
+    the flow, and the RHS and gradients on the evolved state match the
+    oracle."""
+    import torch
+    mesh = QuadMesh.read(os.path.join(HERE, "golden", "meshes", "LidDrivenCavity.msh"))
+    wall, lid = (0.0, 0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0)
+    bc = {"bottom": ("no-slip-adiabatic", wall), "right": ("no-slip-adiabatic", wall),
+          "left": ("no-slip-adiabatic", wall), "lid": ("no-slip-adiabatic", lid)}
+    kw = dict(blending=True, alpha_max=0.5, parabolic=True, mu=0.02, Pr=0.72, quirks=False)
+    op = DGSEMOperator.quad(mesh, 3, bc=bc, **kw)
+    p0 = 1.0 / (0.1 * 0.1 * 1.4)
+    u0 = np.repeat(np.array([1.0, 0.0, 0.0, p0 / 0.4])[:, None], op.NE * op.K, axis=1)
+    u = to_dev(u0, cuda)
+    t, n = op.Run(u, 0.0, 3.0, dt=1e-4, max_steps=300)
+    torch.cuda.synchronize()
+    U = u.cpu().numpy().reshape(4, op.NE, op.K)
+    x, w = cases.lgl_nodes(3)
+    i = np.arange(16)
+    c = mesh.corners()
+    # w_i w_j J with J of each (affine) element
+    area = 0.5 * np.abs((c[:, 2, 0] - c[:, 0, 0]) * (c[:, 3, 1] - c[:, 1, 1]) -
+                        (c[:, 3, 0] - c[:, 1, 0]) * (c[:, 2, 1] - c[:, 0, 1]))
+    WJ = (w[i % 4] * w[i // 4])[None, :] * area[:, None]
+    m0, m1 = np.sum(WJ * u0.reshape(4, op.NE, op.K)[0]), np.sum(WJ * U[0])
+    print("ldc: %d steps, mass %.15g -> %.15g, max |u| %.4f" % (n, m0, m1, np.abs(U[1] / U[0]).max()), flush=True)
+    assert n == 300 and np.isfinite(U).all() and (U[0] > 0).all()
+    assert abs(m1 - m0) <= 1e-13 * m0
+    assert np.abs(U[1] / U[0]).max() > 0.1
+    kinds = ["no-slip-adiabatic"] * 4
+    d = dict(vert=mesh.vert, elem=mesh.elem, bdr=mesh.bdr, bdr_attr=mesh.bdr_attr, attr_bc=kinds,
+             attr_state=[wall, wall, lid, wall])
+    assert mesh.names["lid"] == 3
+    orc = O.Oracle.quad(d, 3, threads=8, **kw)
+    out = torch.empty_like(u)
+    op.Mult(u, out)
+    g = op.gradients(u)
+    torch.cuda.synchronize()
+    r = rel_max(out.cpu().numpy(), orc.Mult(u.cpu().numpy()))
+    rg = rel_max(g.cpu().numpy(), orc.gradients())
+    print("ldc rhs rel", r, "grad rel", rg, flush=True)
+    assert r <= TOL and rg <= TOL

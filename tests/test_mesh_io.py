@@ -107,10 +107,19 @@ def test_create_quad_rejects_open_meshes():
         DGSEMOperator.quad(m, 3, bc={"bottom": ("supersonic-inflow", [1.0, 3.0, 0.0, 1.0]),
                                      "right": "slip", "lid": "slip", "left": "slip"})
     assert ei.value.code == _lib.PRANDTL_ERR_ARG
-    # Navier-Stokes on a quadrilateral mesh is refused
-    _, prm = _descs(2, 3, (1, 1), (0, 0), (1, 1), parabolic=True, mu=0.01)
+    # Navier-Stokes with slip walls (no viscous flux in the reference), and
+    # no-slip walls without viscosity, are refused
+    with pytest.raises(PrandtlError) as ei:
+        DGSEMOperator.quad(m, 3, parabolic=True, mu=0.01,
+                           bc={"bottom": "slip", "right": "slip", "lid": "slip", "left": "slip"})
+    assert ei.value.code == _lib.PRANDTL_ERR_UNSUPPORTED
+    wall = ("no-slip-adiabatic", (0.0, 0.0, 0.0, 0.0))
+    with pytest.raises(PrandtlError) as ei:
+        DGSEMOperator.quad(m, 3, bc={"bottom": wall, "right": wall, "lid": wall, "left": wall})
+    assert ei.value.code == _lib.PRANDTL_ERR_UNSUPPORTED
+    # an empty mesh
+    _, prm = _descs(2, 3, (1, 1), (0, 0), (1, 1))
     q = _lib.QuadMesh()
-    q.nelem = m.nelem
     h = ctypes.c_void_p()
     rc = lib().prandtl_create_quad(ctypes.byref(q), 3, ctypes.byref(prm), ctypes.byref(h))
-    assert rc == _lib.PRANDTL_ERR_UNSUPPORTED and not h.value
+    assert rc == _lib.PRANDTL_ERR_ARG and not h.value

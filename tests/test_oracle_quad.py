@@ -176,3 +176,79 @@ def test_boundary_balance_on_distorted_mesh():
                 F, _ = O.bc_flux(2, G, 1, kinds[attr - 1], U[:, e, node], nor, outer=g)
                 expect -= w[l] * F
     assert np.abs(tot - expect).max() < 1e-13 * max(np.abs(expect).max(), 1.0)
+
+
+# ---------------------------------------------------------------- Navier-Stokes
+NS = dict(parabolic=True, mu=0.02, Pr=0.71)
+WALL = (0.0, 0.0, 0.0, 0.0)
+LID = (0.4, 0.0, 0.0, 0.0)
+
+
+@pytest.mark.parametrize("quirks", [False, True])
+def test_ns_uniform_grids_equal_cartesian(quirks):
+    """NS on quad meshes: periodic, and channels between no-slip walls (the
+    top one moving), equal the box operator."""
+    order, n = 3, (4, 5)
+    box = O.Oracle(2, order, n, (0.0, 0.0), (1.0, 0.8), quirks=quirks, **NS)
+    quad = O.Oracle.quad(Q.grid(*n, hi=(1.0, 0.8)), order, quirks=quirks, **NS)
+    u = smooth(box.coords())
+    rb, rq = box.Mult(u), quad.Mult(u)
+    assert np.abs(rb - rq).max() <= 1e-13 * np.abs(rb).max()
+    assert np.abs(box.gradients() - quad.gradients()).max() <= 1e-13 * np.abs(box.gradients()).max()
+    g = cons(1.1, [1.8, 0.1], 1.0)
+    box = O.Oracle(2, order, n, (0.0, 0.0), (1.0, 1.0), quirks=quirks,
+                   bc=[("supersonic-inflow", "supersonic-outflow"), ("no-slip-adiabatic", "no-slip-adiabatic")],
+                   bc_state=[(g, None), (WALL + (0,), LID + (0,))], **NS)
+    mesh = Q.grid(*n, periodic=(False, False))
+    mesh.update(attr_bc=["supersonic-inflow", "supersonic-outflow", "no-slip-adiabatic", "no-slip-adiabatic"],
+                attr_state=[g, g, WALL, LID])
+    quad = O.Oracle.quad(mesh, order, quirks=quirks, **NS)
+    u = smooth(box.coords())
+    rb, rq = box.Mult(u), quad.Mult(u)
+    assert np.abs(rb - rq).max() <= 1e-13 * np.abs(rb).max()
+
+
+def test_ns_free_stream_and_conservation_on_distorted_mesh():
+    mesh = Q.grid(6, 5, perturb=0.25, seed=21)
+    orc = O.Oracle.quad(mesh, 4, quirks=False, blending=True, **NS)
+    u = np.repeat(cons(1.3, [0.4, -0.7], 2.0)[:, None], orc.NE * orc.K, axis=1)
+    assert np.abs(orc.Mult(u)).max() < 1e-11
+    r = orc.Mult(jumpy(orc.coords())).reshape(4, orc.NE, orc.K)
+    WJ = orc.weights()
+    for v in range(4):
+        assert abs(np.sum(WJ * r[v])) < 1e-13 * np.sum(WJ * np.abs(r[v]))
+
+
+def test_ns_rotation_equivariance():
+    theta = 1.1
+    c, s = np.cos(theta), np.sin(theta)
+    R = np.array([[c, -s], [s, c]])
+    a = O.Oracle.quad(Q.grid(4, 4, perturb=0.2, seed=13), 3, quirks=False, **NS)
+    b = O.Oracle.quad(Q.grid(4, 4, perturb=0.2, seed=13, rotate=theta), 3, quirks=False, **NS)
+    ua = jumpy(a.coords())
+    ub = ua.copy()
+    ub[1:3] = np.einsum("ij,jek->iek", R, ua[1:3])
+    ra = a.Mult(ua).reshape(4, a.NE, a.K)
+    rb = b.Mult(ub).reshape(4, b.NE, b.K)
+    ra[1:3] = np.einsum("ij,jek->iek", R, ra[1:3])
+    assert np.abs(ra - rb).max() <= 1e-12 * np.abs(rb).max()
+
+
+def test_ns_walls_relabelled():
+    mesh = Q.grid(4, 4, periodic=(False, False), perturb=0.2, seed=17)
+    mesh.update(attr_bc=["no-slip-adiabatic"] * 4, attr_state=[WALL, WALL, WALL, LID])
+    mesh2, perm = Q.relabel(mesh, seed=3)
+    a = O.Oracle.quad(mesh, 3, quirks=False, **NS)
+    b = O.Oracle.quad(mesh2, 3, quirks=False, **NS)
+    Xa, Xb = a.coords(), b.coords()
+    K = a.K
+    idx = np.zeros((b.NE, K), dtype=np.int64)
+    for k in range(b.NE):
+        for n in range(K):
+            d = np.abs(Xa[:, perm[k], :] - Xb[:, k, n][:, None]).sum(axis=0)
+            idx[k, n] = perm[k] * K + int(np.argmin(d))
+    ua = smooth(Xa)
+    ub = ua.reshape(4, -1)[:, idx.reshape(-1)]
+    ra = a.Mult(ua).reshape(4, -1)
+    rb = b.Mult(ub).reshape(4, -1)
+    assert np.abs(ra[:, idx.reshape(-1)] - rb).max() <= 1e-12 * np.abs(ra).max()
