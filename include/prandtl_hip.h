@@ -66,9 +66,12 @@ enum {
     PRANDTL_BC_SUPERSONIC_OUTFLOW = 3, /* "supersonic-outflow": outer state = interior */
     PRANDTL_BC_SYMMETRY = 4,           /* "symmetry": mirrored normal momentum */
     PRANDTL_BC_SPECIFIED_STATE = 5,    /* "specified-state", constant state */
-    PRANDTL_BC_NO_SLIP_ADIABATIC = 6   /* "no-slip-adiabatic" (Navier-Stokes): wall velocity in
+    PRANDTL_BC_NO_SLIP_ADIABATIC = 6,  /* "no-slip-adiabatic" (Navier-Stokes): wall velocity in
                                           bc_state[0..dim-1], normal heat flux in bc_state[3];
                                           NoSlipAdiabWallBdrFaceIntegrator.cpp */
+    PRANDTL_BC_AXIS = 7                /* "axis" (quadrilateral meshes): the symmetry flux, and in
+                                          axisymmetric mode the marker of the axis nodes
+                                          (Simulation.cpp:520-531) */
 };
 
 /* Physics and scheme switches: PhysicsConstants (Physics.hpp:10-37), the
@@ -91,6 +94,16 @@ typedef struct prandtl_params {
                            Ts = 110.4 (Physics.hpp:35-37), T = p / (rho R_gas);
                            mu is then unused */
     double R_gas;       /* gas constant for that temperature (> 0 with sutherland) */
+    int axisymmetric;   /* AXISYMMETRIC build (quadrilateral meshes, Euler): the state
+                           is r U with r = y; U is recovered from it before each RHS
+                           (DGSEMOperator.cpp:161-424, with an "axis" boundary's
+                           low-order axis reconstruction), fluxes are weighted by r
+                           (DGSEMIntegrator.cpp:190-205,290,337,1030,1110,
+                           BdrFaceIntegrator.cpp:109), the radial momentum gains the
+                           pressure source (DGSEMIntegrator.cpp:411-426) and its RHS is
+                           zeroed on the axis (DGSEMOperator.cpp:521-533,573-583) */
+    double axis_rho_floor; /* density floor of the axis reconstruction (rho_floor_abs,
+                           DGSEMOperator.hpp:44,134-138); <= 0: 1e-12 */
 } prandtl_params;
 
 /* DGSEMOperator constructor (DGSEMOperator.hpp:87-99): builds LGL rule,
@@ -254,6 +267,13 @@ const char *prandtl_last_error(const prandtl_ctx *ctx);
 /* Primitive output fields (Simulation.cpp:1046-1061): prim_dev receives
  * rho, the velocity components and p, [dim+2][NE][K] like the state. */
 int prandtl_primitives(prandtl_ctx *ctx, const double *u_dev, double *prim_dev, void *stream);
+
+/* DGSEMOperator::RecoverStateFromWeighted (DGSEMOperator.cpp:161-424): the
+ * conservative state U_dev of the weighted state rU_dev of an axisymmetric
+ * context (both [nv][NE][K] device buffers); prandtl_primitives of such a
+ * context applies it first, as the reference's output does
+ * (Simulation.cpp:1041-1044). */
+int prandtl_axis_recover(prandtl_ctx *ctx, const double *rU_dev, double *U_dev, void *stream);
 
 /* ---- Checkpoints (Simulation.cpp:1084-1116 save, 350-388 load) ----
  * The folder layout of the reference: dir/Cycle<cycle>/checkpoint_cycle_<cycle>.json

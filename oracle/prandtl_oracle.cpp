@@ -638,6 +638,12 @@ struct Mesh {
     std::vector<int> bq_e, bq_type, bq_attr, bq_n;
     std::vector<double> bq_nor;
     std::vector<double> attr_state; /* [nattr][5] */
+    /* axisymmetric quadrilateral meshes: element corners [NE][4][2], the
+     * nodes in axis_idx (BuildAxisIndexFromMarker, DGSEMOperator.cpp:
+     * 446-485) and whether an axis boundary exists */
+    std::vector<double> cq;
+    std::vector<char> axis_idx;
+    bool axis_bc = false;
 };
 
 inline int line_node(int dim, int Np, int r, int l, int m)
@@ -790,6 +796,10 @@ int build_mesh_quad(orc_ctx *c, const orc_quad_mesh *q)
     m.J.assign(NK, 0.0);
     m.adj.assign(NK * 4, 0.0);
     m.X.assign(2 * NK, 0.0);
+    m.cq.assign((size_t)NE * 8, 0.0);
+    for (int e = 0; e < NE; e++)
+        for (int k = 0; k < 4; k++)
+            for (int i = 0; i < 2; i++) m.cq[(size_t)e * 8 + 2 * k + i] = cornerx(e, k, i);
     m.hmin = 1e300;
     for (int e = 0; e < NE; e++) {
         for (int j = 0; j < Np; j++)
@@ -877,9 +887,17 @@ int build_mesh_quad(orc_ctx *c, const orc_quad_mesh *q)
             const int attr = it->second;
             if (attr < 1 || attr > q->nattr) return 8;
             const int type = q->attr_bc[attr - 1];
-            if (type < ORC_BC_SLIP || type > ORC_BC_NO_SLIP_ADIABATIC) return 8;
+            if (type < ORC_BC_SLIP || type > ORC_BC_AXIS) return 8;
             m.bq_e.push_back(e1);
-            m.bq_type.push_back(type);
+            /* "axis" adds the symmetry integrator (Simulation.cpp:520-524) */
+            m.bq_type.push_back(type == ORC_BC_AXIS ? ORC_BC_SYMMETRY : type);
+            if (type == ORC_BC_AXIS) {
+                /* every node of the adjacent element with |r| <= 1e-14 */
+                m.axis_bc = true;
+                if (m.axis_idx.empty()) m.axis_idx.assign(NK, 0);
+                for (int i = 0; i < K; i++)
+                    if (std::fabs(m.X[NK + (size_t)e1 * K + i]) <= 1e-14) m.axis_idx[(size_t)e1 * K + i] = 1;
+            }
             m.bq_attr.push_back(attr - 1);
             for (int l = 0; l < Nfp; l++) {
                 m.bq_n.push_back(line_node(2, Np, r1, l, s1 ? N : 0));
@@ -889,12 +907,100 @@ int build_mesh_quad(orc_ctx *c, const orc_quad_mesh *q)
         }
     }
     m.nf = (int)m.fe1.size();
+    if (m.axis_idx.empty()) m.axis_idx.assign(NK, 0);
     m.attr_state.assign(5 * (size_t)std::max(q->nattr, 1), 0.0);
     for (int a = 0; a < q->nattr; a++)
         if (q->attr_state)
             for (int v = 0; v < 4; v++) m.attr_state[5 * (size_t)a + v] = q->attr_state[4 * (size_t)a + v];
     build_subcells(c);
     return 0;
+}
+
+/* DGSEMOperator::RecoverStateFromWeighted (DGSEMOperator.cpp:161-424) with
+ * low_order_axis set, as an "axis" boundary sets it (Simulation.cpp:
+ * 527-531), which makes every axis node "troubled" (line 238) and skips the
+ * shape-function derivative.  Off the axis U = rU / r (lines 248-256).  On
+ * it (in axis_idx, or r <= 0) each of rho, rho u_z, rho E is d(rU)/dr from
+ * the two nearest off-axis nodes at the node's z (else the two nearest
+ * overall): the least-squares slope through the origin, limited by minmod
+ * against 1.5 f_j / r_j and capped at 10 |f_j1| / r_j1 (lines 302-378); one
+ * such node gives its f / r; the radial momentum is 0.  A density below the
+ * floor or a non-finite one or energy takes the nearest off-axis node's
+ * state instead (lines 384-410). */
+void axis_recover(const orc_ctx *c, const double *rU, double *U)
+{
+    const Mesh &m = c->m;
+    const int K = m.K, NE = m.NE, nv = m.nv;
+    const size_t NK = (size_t)NE * K;
+    const double tiny = 1e-14, cap_mult = 10.0, z_tol = 1e-12, theta_lim = 1.5;
+    const double rho_floor = c->p.axis_rho_floor > 0.0 ? c->p.axis_rho_floor : 1e-12;
+    const double inf = std::numeric_limits<double>::infinity();
+    auto sign = [](double a) { return (double)((a > 0) - (a < 0)); };
+    auto minmod = [&](double a, double b) {
+        if (a * b <= 0.0) return 0.0;
+        return sign(a) * std::min(std::fabs(a), std::fabs(b));
+    };
+    auto clamp = [&](double x, double mx) {
+        if (!std::isfinite(x)) return 0.0;
+        return std::fabs(x) <= mx ? x : sign(x) * mx;
+    };
+    for (int e = 0; e < NE; e++) {
+        const double *re = &m.X[NK + (size_t)e * K], *ze = &m.X[(size_t)e * K];
+        for (int ld = 0; ld < K; ld++) {
+            const size_t nd = (size_t)e * K + ld;
+            const double r = re[ld];
+            if (!m.axis_idx[nd] && r > 0.0) {
+                for (int v = 0; v < nv; v++) U[v * NK + nd] = rU[v * NK + nd] / r;
+                continue;
+            }
+            const double z_axis = ze[ld];
+            auto d_dr = [&](int comp) -> double {
+                auto f = [&](int j) { return rU[comp * NK + (size_t)e * K + j]; };
+                int j1 = -1, j2 = -1;
+                double r1 = inf, r2 = inf;
+                for (int j = 0; j < K; j++) {
+                    const double rj = re[j];
+                    if (rj <= 0.0) continue;
+                    if (std::fabs(ze[j] - z_axis) <= z_tol) {
+                        if (rj < r1) { r2 = r1; j2 = j1; r1 = rj; j1 = j; }
+                        else if (rj < r2) { r2 = rj; j2 = j; }
+                    }
+                }
+                if (j1 < 0)
+                    for (int j = 0; j < K; j++) {
+                        const double rj = re[j];
+                        if (rj > 0.0 && rj < r1) { r2 = r1; j2 = j1; r1 = rj; j1 = j; }
+                        else if (rj > 0.0 && rj < r2) { r2 = rj; j2 = j; }
+                    }
+                const bool have_cap = j1 >= 0;
+                const double f_ref = have_cap ? std::fabs(f(j1)) : 0.0;
+                const double cap = (have_cap && !(f_ref < 1e-12)) ? cap_mult * f_ref / std::max(r1, tiny) : inf;
+                if (j1 >= 0 && j2 >= 0 && std::isfinite(r1) && std::isfinite(r2) && r1 > 0.0 && r2 > 0.0) {
+                    const double f1 = f(j1), f2 = f(j2);
+                    const double mm = (r1 * f1 + r2 * f2) / std::max(r1 * r1 + r2 * r2, tiny);
+                    const double s1 = f1 / r1, s2 = f2 / r2;
+                    return clamp(minmod(mm, minmod(theta_lim * s1, theta_lim * s2)), cap);
+                }
+                if (j1 >= 0 && std::isfinite(r1) && r1 > 0.0) return clamp(f(j1) / r1, cap);
+                return 0.0;
+            };
+            const double rho = d_dr(0), mz = d_dr(1), E = d_dr(3);
+            if (!std::isfinite(rho) || rho < rho_floor || !std::isfinite(E)) {
+                int nb = -1;
+                double rn = -1.0;
+                for (int j = 0; j < K; j++)
+                    if (re[j] > 0.0 && (nb == -1 || re[j] < rn)) { rn = re[j]; nb = j; }
+                /* create_impl refuses elements without an off-axis node */
+                for (int v = 0; v < nv; v++) U[v * NK + nd] = rU[v * NK + (size_t)e * K + nb] / rn;
+                U[2 * NK + nd] = 0.0;
+            } else {
+                U[0 * NK + nd] = rho;
+                U[1 * NK + nd] = mz;
+                U[2 * NK + nd] = 0.0;
+                U[3 * NK + nd] = E;
+            }
+        }
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -907,13 +1013,36 @@ template <class T> struct Engine {
     size_t NK;
     double gamma;
     std::vector<T> U, W, G;  /* state copy (counted only), entropy vars, gradients */
+    bool axi;
+    std::vector<T> Urec;     /* axisymmetric: the state recovered from r U */
 
     explicit Engine(orc_ctx *cc) : c(cc), m(cc->m)
     {
         dim = m.dim; N = m.N; Np = m.Np; K = m.K; NE = m.NE; nv = m.nv; Nfp = m.Nfp;
         NK = (size_t)NE * K;
         gamma = c->p.gamma;
+        axi = c->p.axisymmetric != 0;
     }
+
+    /* r = y of the element map at reference point (xi, eta): the bilinear
+     * map of the corners, as Tr.Transform (DGSEMIntegrator.cpp:272-276) */
+    double rmap(int e, double xi, double eta) const
+    {
+        const double Nk[4] = {(1 - xi) * (1 - eta), xi * (1 - eta), xi * eta, (1 - xi) * eta};
+        double y = 0.0;
+        for (int k = 0; k < 4; k++) y += Nk[k] * m.cq[(size_t)e * 8 + 2 * k + 1];
+        return y;
+    }
+    /* r at the reference midpoint of nodes a and b (r_hat of lines 272-276
+     * and 318-322, mid_r of ComputeFVFluxes lines 976-999) */
+    double rmid(int e, int a, int b) const
+    {
+        const double *x = c->x.data();
+        return rmap(e, 0.5 * (x[a % Np] + x[b % Np]), 0.5 * (x[a / Np] + x[b / Np]));
+    }
+    /* r at a face point: the node's coordinate (the face transformation's
+     * point, lines 190-193 and BdrFaceIntegrator.cpp:81-83) */
+    double rnode(int e, int n) const { return m.X[NK + (size_t)e * K + n]; }
 
     inline void state(const T *u, int e, int i, T *s) const
     {
@@ -1092,6 +1221,11 @@ template <class T> struct Engine {
                     const double *sn = &m.sub[((((size_t)e * dim + r) * Nfp + l) * N + i) * dim];
                     for (int d = 0; d < dim; d++) nor[d] = sn[d];
                     lam = mmax(lam, face_flux(dim, gamma, q, s1, s2, nor, F));
+                    if (axi) {
+                        /* flux_num *= mid_r (lines 1030-1032, 1110-1112) */
+                        const T rh = rmid(e, id1, id2);
+                        for (int v = 0; v < nv; v++) F[v] *= rh;
+                    }
                     T sc = 1.0 / (T(m.J[(size_t)e * K + id1]) * c->w[i]);
                     for (int v = 0; v < nv; v++) {
                         dU[v] -= F[v];
@@ -1175,6 +1309,11 @@ template <class T> struct Engine {
                             const double *adj2 = &m.adj[((size_t)e * K + id2) * dim * dim];
                             for (int d = 0; d < dim; d++) m2[d] = adj2[r * dim + d];
                             lam = mmax(lam, volume_flux(dim, gamma, s1, s2, m1, m2, f));
+                            if (axi) {
+                                /* f *= r_hat (lines 290-292, 337-339) */
+                                const T rh = rmid(e, id1, id2);
+                                for (int v = 0; v < nv; v++) f[v] *= rh;
+                            }
                             T *a = FI(r, id1, mm), *b = FI(r, id2, ii[r]);
                             for (int v = 0; v < nv; v++) {
                                 a[v] = f[v];
@@ -1209,6 +1348,13 @@ template <class T> struct Engine {
                         }
                     }
                     T invJ = 1.0 / T(m.J[(size_t)e * K + id1]);
+                    if (axi) {
+                        /* the radial momentum's pressure source, before the
+                         * volume term (lines 411-426, PressureFromConservative
+                         * lines 1306-1315) */
+                        const T ke = 0.5 * ((s1[1] * s1[1] + s1[2] * s1[2]) / s1[0]);
+                        el[id1 * nv + 2] += (gamma - 1.0) * (s1[3] - ke);
+                    }
                     for (int v = 0; v < nv; v++) {
                         T a = -dU[v];
                         if (blend) a *= (1.0 - alpha_e);
@@ -1272,6 +1418,11 @@ template <class T> struct Engine {
                 g_cat = CAT_FACE;
                 if (ec) lam = mmax(lam, volume_flux(dim, gamma, s1, s2, nor, nor, F));
                 else lam = mmax(lam, face_flux(dim, gamma, quirks, s1, s2, nor, F));
+                if (axi) {
+                    /* flux_num *= r (lines 190-205) */
+                    const T rf = rnode(e1, n1);
+                    for (int v = 0; v < nv; v++) F[v] *= rf;
+                }
                 for (int v = 0; v < nv; v++) { d1[v] = -F[v]; d2[v] = F[v]; }
                 if (par)
                     for (int v = 0; v < nv; v++) { d1[v] += fn[v]; d2[v] -= fn[v]; }
@@ -1367,6 +1518,11 @@ template <class T> struct Engine {
                 g_cat = CAT_FACE;
                 const int type = m.bq_type[b];
                 lam = mmax(lam, bc_flux(dim, gamma, c->p.quirks, type, s1, outer, nor, F));
+                if (axi) {
+                    /* dU_face1 *= r (BdrFaceIntegrator.cpp:109-111) */
+                    const T rf = rnode(e, n1);
+                    for (int v = 0; v < nv; v++) F[v] *= rf;
+                }
                 T d[5];
                 for (int v = 0; v < nv; v++) d[v] = -F[v];
                 if (c->p.parabolic) {
@@ -1408,6 +1564,15 @@ template <class T> struct Engine {
      * dudt is overwritten. */
     void rhs(const T *u, T *dudt, int threads)
     {
+        if (axi) {
+            /* RecoverStateFromWeighted(u, U) first (DGSEMOperator.cpp:539-541) */
+            std::vector<double> ud(nv * NK), Ud(nv * NK);
+            for (size_t i = 0; i < nv * NK; i++) ud[i] = val(u[i]);
+            axis_recover(c, ud.data(), Ud.data());
+            Urec.resize(nv * NK);
+            for (size_t i = 0; i < nv * NK; i++) Urec[i] = T(Ud[i]);
+            u = Urec.data();
+        }
         if (c->p.blending) blending(u);
         if (c->p.parabolic) lifting(u);
         for (size_t i = 0; i < nv * NK; i++) dudt[i] = 0.0;
@@ -1454,6 +1619,10 @@ template <class T> struct Engine {
         }
         (void)nth;
         c->lam = lam;
+        if (axi)
+            /* ZeroAxisRadialMom(dudt) (DGSEMOperator.cpp:573-583, 521-533) */
+            for (size_t i = 0; i < NK; i++)
+                if (m.axis_idx[i]) dudt[2 * NK + i] = 0.0;
     }
 };
 
@@ -1587,6 +1756,9 @@ static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **o
     for (int r = 0; r < p->dim && !qm; r++)
         if (p->nelem[r] < 1 || !(p->hi[r] > p->lo[r])) return 3;
     if (p->sutherland && !(p->R_gas > 0.0)) return 5;
+    /* axisymmetric: 2-D quadrilateral meshes, Euler (the reference's
+     * Navier-Stokes element and face routines carry no r weighting) */
+    if (p->axisymmetric && (!qm || p->parabolic)) return 9;
     for (int r = 0; r < p->dim && !qm; r++) {
         for (int side = 0; side < 2; side++)
             if (p->bc[r][side] < ORC_BC_PERIODIC || p->bc[r][side] > ORC_BC_NO_SLIP_ADIABATIC) return 6;
@@ -1617,6 +1789,22 @@ static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **o
         build_mesh(c);
     }
     const Mesh &m = c->m;
+    if (p->axisymmetric) {
+        /* nodes on the axis need an axis boundary (low_order_axis; the
+         * shape-derivative path depends on the previous call's alpha), none
+         * may lie below it, and every element keeps an off-axis node */
+        const size_t NK = (size_t)m.NE * m.K;
+        for (int e = 0; e < m.NE; e++) {
+            bool off = false, on = false;
+            for (int i = 0; i < m.K; i++) {
+                const double r = m.X[NK + (size_t)e * m.K + i];
+                if (r < -1e-14) { delete c; return 9; }
+                if (r > 0.0) off = true;
+                if (!(r > 0.0) || m.axis_idx[(size_t)e * m.K + i]) on = true;
+            }
+            if (!off || (on && !m.axis_bc)) { delete c; return 9; }
+        }
+    }
     c->alpha.assign(m.NE, 0.0);
     c->eta.assign(m.NE, 0.0);
     c->lam = -1.0; /* DGSEMIntegrator.cpp:39 */
@@ -1714,6 +1902,13 @@ int orc_alpha(const orc_ctx *c, double *alpha)
 int orc_eta(const orc_ctx *c, double *eta)
 {
     std::memcpy(eta, c->eta.data(), c->eta.size() * sizeof(double));
+    return 0;
+}
+
+int orc_axis_recover(const orc_ctx *c, const double *rU, double *U)
+{
+    if (!c->p.axisymmetric) return 1;
+    axis_recover(c, rU, U);
     return 0;
 }
 

@@ -39,6 +39,13 @@ typedef struct orc_params {
      * supersonic-inflow / specified-state sides.  Euler only. */
     int bc[3][2];
     double bc_state[3][2][5];
+    /* AXISYMMETRIC build (quadrilateral meshes, Euler): the state is r U with
+     * r = y, U is recovered from it before each RHS (DGSEMOperator.cpp:
+     * 161-424), fluxes are weighted by r and the radial momentum gains the
+     * pressure source.  axis_rho_floor: the density floor of the axis
+     * reconstruction (rho_floor_abs, DGSEMOperator.hpp:44; <= 0: 1e-12) */
+    int axisymmetric;
+    double axis_rho_floor;
 } orc_params;
 
 enum {
@@ -48,8 +55,10 @@ enum {
     ORC_BC_SUPERSONIC_OUTFLOW = 3, /* SupersonicOutflowBdrFaceIntegrator.cpp */
     ORC_BC_SYMMETRY = 4,           /* SymmetryBdrFaceIntegrator.cpp */
     ORC_BC_SPECIFIED_STATE = 5,    /* SpecifiedStateBdrFaceIntegrator.cpp (constant state) */
-    ORC_BC_NO_SLIP_ADIABATIC = 6   /* NoSlipAdiabWallBdrFaceIntegrator.cpp (constant wall velocity
+    ORC_BC_NO_SLIP_ADIABATIC = 6,  /* NoSlipAdiabWallBdrFaceIntegrator.cpp (constant wall velocity
                                       bc_state[0..dim-1] and heat flux bc_state[3]) */
+    ORC_BC_AXIS = 7                /* "axis": the symmetry flux, and in axisymmetric mode the
+                                      marker of the axis nodes (Simulation.cpp:520-531) */
 };
 
 /* 1-D rules and operators (setup arithmetic of DGSEMIntegrator.cpp:8-82) */
@@ -118,6 +127,9 @@ void orc_reset_wave_speed(orc_ctx *c);
 int orc_alpha(const orc_ctx *c, double *alpha /* NE */);
 int orc_eta(const orc_ctx *c, double *eta /* NE */);
 double orc_hmin(const orc_ctx *c);
+/* DGSEMOperator::RecoverStateFromWeighted (DGSEMOperator.cpp:161-424): the
+ * conservative state U of the weighted state rU (axisymmetric contexts) */
+int orc_axis_recover(const orc_ctx *c, const double *rU, double *U);
 /* BR1 primitive gradients of the last parabolic RHS, [dim][nv][NE][K] */
 int orc_last_gradients(const orc_ctx *c, double *grad);
 /* algorithmic op count of one RHS on state u (counted instantiation):

@@ -31,7 +31,7 @@ EXPORTED = (
     "prandtl_ensemble_rk3_step", "prandtl_ensemble_last_error",
     "prandtl_checkpoint_write", "prandtl_checkpoint_read", "prandtl_checkpoint_save", "prandtl_checkpoint_load",
     "prandtl_create_quad", "prandtl_hmin", "prandtl_mesh_read", "prandtl_mesh_refine", "prandtl_mesh_view",
-    "prandtl_mesh_name", "prandtl_mesh_free", "prandtl_mesh_last_error",
+    "prandtl_mesh_name", "prandtl_mesh_free", "prandtl_mesh_last_error", "prandtl_axis_recover",
 )
 
 
@@ -53,7 +53,7 @@ class QuadMesh(ctypes.Structure):
 
 # boundary types (PRANDTL_BC_*) by the reference's configuration names
 BC_TYPES = {"periodic": 0, "slip": 1, "supersonic-inflow": 2, "supersonic-outflow": 3,
-            "symmetry": 4, "specified-state": 5, "no-slip-adiabatic": 6}
+            "symmetry": 4, "specified-state": 5, "no-slip-adiabatic": 6, "axis": 7}
 
 
 class Params(ctypes.Structure):
@@ -61,7 +61,8 @@ class Params(ctypes.Structure):
                 ("alpha_max", ctypes.c_double), ("alpha_min", ctypes.c_double),
                 ("parabolic", ctypes.c_int), ("blending", ctypes.c_int),
                 ("ec_surface", ctypes.c_int), ("quirks", ctypes.c_int), ("precision", ctypes.c_int),
-                ("sutherland", ctypes.c_int), ("R_gas", ctypes.c_double)]
+                ("sutherland", ctypes.c_int), ("R_gas", ctypes.c_double),
+                ("axisymmetric", ctypes.c_int), ("axis_rho_floor", ctypes.c_double)]
 
 
 class PrandtlError(RuntimeError):
@@ -103,6 +104,7 @@ def lib():
     L.prandtl_lgl.argtypes = [ctypes.c_int, dp, dp]
     L.prandtl_debug_gradients.argtypes = [vp, vp, vp, vp]
     L.prandtl_primitives.argtypes = [vp, vp, vp, vp]
+    L.prandtl_axis_recover.argtypes = [vp, vp, vp, vp]
     L.prandtl_profile_begin.argtypes = [vp]
     L.prandtl_profile_end.argtypes = [vp, dp, ip]
     L.prandtl_last_error.argtypes = [vp]

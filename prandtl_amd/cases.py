@@ -193,6 +193,46 @@ TRIPLE_POINT = dict(dim=2, order=3, nelem=(112, 48), lo=(0.0, 0.0), hi=(7.0, 3.0
                     blending=True, alpha_max=0.5, bc=[("slip", "slip"), ("slip", "slip")])
 
 
+# The reference's axisymmetric Euler cases (AXISYMMETRIC build, order 1,
+# SUBCELL_FV_BLENDING with alpha_max 0.5, fixed dt), on their Gmsh meshes:
+# the 15-degree ramp (TestCases/Euler/2D/Ramp: RampIC and RampInletBCVector
+# of Ramp.hpp:9-29, Mach 2 free stream rho = 1, a = 1) and Nagashima's ramjet
+# (TestCases/Euler/2D/Nagashima_Ramjet/TR0: NagashimaIC and
+# NagashimaInletBCVector of Nagashima_Ramjet.hpp:9-42, Mach 2 from p0 =
+# 303975 Pa, a0 = 360.63 m/s).  The axis density floor is the one
+# Simulation.cpp:925-930 sets from the initial state, 1e-8 rho.
+def _nagashima_free_stream(gamma=1.4, R_gas=287.05, Ma=2.0, p0=303975.0, a0=360.63):
+    C = 1 + 0.5 * (gamma - 1) * Ma * Ma
+    a = a0 / np.sqrt(C)
+    ua = Ma * a
+    pa = p0 / C ** (gamma / (gamma - 1))
+    Ta = (a * a) / (gamma * R_gas)
+    rhoa = pa / (R_gas * Ta)
+    return np.array([rhoa, rhoa * ua, 0.0, pa / (gamma - 1.0) + 0.5 * rhoa * ua * ua])
+
+
+RAMP_STATE = np.array([1.0, 2.0, 0.0, (1.0 / 1.4) / (1.4 - 1.0) + 0.5 * 1.0 * 2.0 * 2.0])
+RAMP = dict(mesh="Ramp15deg_v2.msh", order=1, blending=True, alpha_max=0.5, dt=1e-4, t_final=3.0,
+            state=RAMP_STATE, axis_rho_floor=1e-8 * RAMP_STATE[0],
+            bc={"inlet": ("supersonic-inflow", RAMP_STATE), "axis": "axis", "ramp": "slip",
+                "outlet": "supersonic-outflow", "top": ("supersonic-inflow", RAMP_STATE)})
+NAGASHIMA_STATE = _nagashima_free_stream()
+NAGASHIMA = dict(mesh="Nagashima_TR0_10kQuad_v2.msh", order=1, blending=True, alpha_max=0.5, dt=1e-7,
+                 t_final=0.02, state=NAGASHIMA_STATE, axis_rho_floor=1e-8 * NAGASHIMA_STATE[0],
+                 bc={"inlet": ("supersonic-inflow", NAGASHIMA_STATE), "outlet": "supersonic-outflow",
+                     "axis": "axis", "center-body": "slip", "cowl-inner": "slip", "cowl-outer": "slip",
+                     "valve": "slip"})
+
+
+def axisymmetric_state(U, X):
+    """The weighted state r U (r = X[1]) of a state U: [4] (uniform) or [4][NE][K]."""
+    r = X[1]
+    U = np.asarray(U, float)
+    if U.ndim == 1:
+        U = U[:, None, None] * np.ones_like(r)[None]
+    return r[None] * U
+
+
 def random_smooth_state(X, dim, seed, amp=0.2, gamma=1.4, rho0=1.0, p0=1.0, L=None):
     """Smooth positive periodic state: a few random Fourier modes on each primitive."""
     rng = np.random.default_rng(seed)

@@ -83,6 +83,11 @@ struct Geo {
     const int *fnf;       // [NE*4] 2 (e2*4 + f2) + reversed: the neighbour's face of each interior face
     const double *fnor;   // [NE*4*NFP*2] normal outward of the face's first element
     const double *subn;   // [NE][2][NFP][NP-1][2] subcell interface normals
+    // axisymmetric quadrilateral meshes (null otherwise)
+    const double *rnode;  // [NE*K] r = y of each node
+    const int *axm;       // [NE*K] -1 off the axis, else 2 d + (1: radial RHS zeroed), d the descriptor
+    const int *axj;       // [nax][3] element-local nodes j1, j2 (ray fit), jn (floor fallback); -1 none
+    const double *axr;    // [nax][3] their r
 };
 
 struct KArgs {
@@ -94,6 +99,7 @@ struct KArgs {
     double dt, c0, c1;
     const double *dtp;    // if set, dt is read from device memory (graph replays)
     const double *y;
+    const double *yw;     // axisymmetric: the weighted state r U of the stage update (y holds U)
     const double *u0;
     double *out;
     const void *visc;     // [nv-1][NE][K]            (FP32 in the mixed form)
@@ -303,7 +309,7 @@ template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
 };
 
 // LDS of one k_rhs tile (EPB elements).
-template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false> struct RhsSmem {
+template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false, bool AXI = false> struct RhsSmem {
     using C = RCfg<DIM, NP, BLEND, FB>;
     static constexpr int NQ = DIM + 1 + (BLEND ? 2 : 0);   // v[DIM], beta, (1/beta, lambda)
     static constexpr int GK = QNS3 ? C::K : 1, GE = QNS3 ? C::EPB : 1;
@@ -321,6 +327,7 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false> stru
     double Al[C::EPB];
     double Red[(C::BLOCK + 63) / 64];
     double M[GEN ? C::EPB : 1][4][GEN ? C::K : 1];   // per-node adjugate rows (general geometry)
+    double R[AXI ? C::EPB : 1][AXI ? C::K : 1];      // per-node r (axisymmetric)
 };
 
 // One tile of k_rhs: elements [tile*EPB, tile*EPB + EPB) of the case A
@@ -330,18 +337,25 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false> stru
 // VT: element type of the viscous inputs from k_visc (float in the mixed form).
 // FB: forced workgroup size (0: the natural one of RCfg).
 // OPEN: boundary sides on a Cartesian box.  GEN: general 2-D quadrilateral
-// geometry (per-node metrics, face tables; boundaries by attribute).
-template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false, bool GEN = false>
+// geometry (per-node metrics, face tables; boundaries by attribute).  AXI
+// (with GEN, Euler): axisymmetric, A.y the recovered state U, A.yw the
+// weighted state r U; two-point, subcell and face fluxes weighted by r, the
+// radial momentum's pressure source, its RHS zeroed on the axis.
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false, bool GEN = false,
+          bool AXI = false>
 __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const int slot,
-                                         RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN> &S)
+                                         RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN, AXI> &S)
 {
     using C = RCfg<DIM, NP, BLEND, FB>;
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, NPAIR = C::NPAIR, NFV = C::NFV;
     constexpr int EPB = C::EPB, LC = C::LC, BLOCK = C::BLOCK, FXN = C::FXN, JF = C::JF;
-    constexpr int NQ = RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN>::NQ;
+    constexpr int NQ = RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN, AXI>::NQ;
     static_assert(!GEN || DIM == 2, "general geometry is 2-D");
+    static_assert(!AXI || GEN, "axisymmetric meshes are quadrilateral meshes");
     auto &sM = S.M;
     (void)sM;
+    auto &sR = S.R;
+    (void)sR;
     (void)FXN;
     auto &sDh2 = S.Dh2;
     auto &sV1i = S.V1i;
@@ -384,8 +398,13 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     if (act)
 #pragma unroll
         for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)e * K + n];
-    // general geometry: this node's adjugate rows and Jacobian
-    double jn = 1.0, mown[4] = {0.0, 0.0, 0.0, 0.0};
+    // general geometry: this node's adjugate rows and Jacobian (and r)
+    double jn = 1.0, mown[4] = {0.0, 0.0, 0.0, 0.0}, rown = 0.0;
+    int axown = -1;
+    if (AXI && act) {
+        rown = A.geo.rnode[(size_t)e * K + n];
+        axown = A.geo.axm[(size_t)e * K + n];
+    }
     if (GEN && act) {
         jn = A.geo.jac[(size_t)e * K + n];
 #pragma unroll
@@ -426,6 +445,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         if constexpr (GEN)
 #pragma unroll
             for (int c = 0; c < 4; c++) sM[el][c][n] = mown[c];
+        if constexpr (AXI) sR[el][n] = rown;
         const NodeQ<DIM> q = node_quants<DIM>(ys, P);
 #pragma unroll
         for (int d = 0; d < DIM; d++) sQ[el][d][n] = q.v[d];
@@ -569,6 +589,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                 const double *bs = A.geo.bcs + (code >> 4) * 5;
                 const double lb = boundary_flux_v2(bt, nvec, so, bs, P, F);
                 lam = (lam < lb) ? lb : lam;
+                if constexpr (AXI) {
+                    // dU_face1 *= r (BdrFaceIntegrator.cpp:109-111)
+                    const double rf = sR[ej][no];
+#pragma unroll
+                    for (int v = 0; v < NV; v++) F[v] *= rf;
+                }
                 double d[NV];
 #pragma unroll
                 for (int v = 0; v < NV; v++) d[v] = -F[v];
@@ -612,6 +638,13 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             const NodeQ<DIM> q1 = node_quants<DIM>(g1, P);
             const NodeQ<DIM> q2 = node_quants<DIM>(g2, P);
             two_point_flux_v<DIM, true>(nvec, norml2_2(nvec), q1, q2, g1, g2, P, false, F);
+            if constexpr (AXI) {
+                // flux_num *= r (DGSEMIntegrator.cpp:190-205), the r of the
+                // first element's node on both sides
+                const double rf = own ? sR[ej][no] : A.geo.rnode[gn];
+#pragma unroll
+                for (int v = 0; v < NV; v++) F[v] *= rf;
+            }
 #pragma unroll
             for (int v = 0; v < NV; v++) {
                 double d;
@@ -745,6 +778,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                         const double mm[2] = {(sM[ej][2 * r][na] + sM[ej][2 * r][nb]) * 0.5,
                                               (sM[ej][2 * r + 1][na] + sM[ej][2 * r + 1][nb]) * 0.5};
                         two_point_flux_v<DIM, false>(mm, 0.0, qa, qb, nullptr, nullptr, P, false, F);
+                        if constexpr (AXI) {
+                            // f *= r_hat, r at the pair's midpoint (DGSEMIntegrator.cpp:272-292)
+                            const double rh = (sR[ej][na] + sR[ej][nb]) * 0.5;
+#pragma unroll
+                            for (int v = 0; v < NV; v++) F[v] *= rh;
+                        }
                     } else {
                         two_point_flux<DIM, false>(r, nr, nr, qa, qb, nullptr, nullptr, P, false, F);
                     }
@@ -783,6 +822,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                         const double *sn = A.geo.subn + (((((size_t)(ebase + ej) * 2 + r) * NFP + l0 + lc) * (NP - 1) + i) * 2);
                         const double nv2[2] = {sn[0], sn[1]};
                         two_point_flux_v<DIM, true>(nv2, norml2_2(nv2), qa, qb, s1, s2, P, quirk3d, F);
+                        if constexpr (AXI) {
+                            // flux_num *= mid_r (DGSEMIntegrator.cpp:1030-1032, 1110-1112)
+                            const double rh = (sR[ej][na] + sR[ej][nb]) * 0.5;
+#pragma unroll
+                            for (int v = 0; v < NV; v++) F[v] *= rh;
+                        }
                     } else {
                         two_point_flux<DIM, true>(r, sSub[r][i], sSub[r][i], qa, qb, s1, s2, P, quirk3d, F);
                     }
@@ -840,6 +885,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         }
     }
     if (act) {
+        double src = 0.0;
+        if constexpr (AXI) {
+            // PressureFromConservative (DGSEMIntegrator.cpp:1306-1315) of this node
+            const double ke = 0.5 * ((sU[el][1][n] * sU[el][1][n] + sU[el][2][n] * sU[el][2][n]) / sU[el][0][n]);
+            src = P.gm1 * (sU[el][3][n] - ke);
+        }
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             double vol = -accV[v];
@@ -847,9 +898,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             if (A.parabolic && v > 0) vol = vol + vis[v];
             vol *= GEN ? 1.0 / jn : A.geo.invJ;
             double tot = (BLEND && alpha_e > 0.0) ? accF[v] * alpha_e + vol : vol;
+            if (AXI && v == 2) tot += src;   // DGSEMIntegrator.cpp:411-426
             tot += accS[v];
+            // ZeroAxisRadialMom (DGSEMOperator.cpp:521-533)
+            if (AXI && v == 2 && axown >= 0 && (axown & 1)) tot = 0.0;
             const size_t gi = v * NK + (size_t)e * K + n;
-            const double yv = sU[el][v][n];
+            const double yv = AXI ? A.yw[gi] : sU[el][v][n];
             double o;
             if (A.mode == 0) o = tot;
             else if (A.mode == 1) o = yv + dt * tot;
@@ -862,12 +916,13 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
 
 // OPEN: the box has non-periodic sides (Euler, natural shape); the periodic
 // kernels carry no boundary code.  GEN: a 2-D quadrilateral mesh.
-template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false, bool GEN = false>
+template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false, bool GEN = false,
+          bool AXI = false>
 __global__ void __launch_bounds__((RCfg<DIM, NP, BLEND, FB>::BLOCK))
 __attribute__((amdgpu_waves_per_eu(RCfg<DIM, NP, BLEND, FB>::WAVES))) k_rhs(const KArgs A)
 {
-    __shared__ RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN> S;
-    rhs_body<DIM, NP, BLEND, QNS3, VT, FB, OPEN, GEN>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
+    __shared__ RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN, AXI> S;
+    rhs_body<DIM, NP, BLEND, QNS3, VT, FB, OPEN, GEN, AXI>(A, xcd_remap(blockIdx.x, gridDim.x), blockIdx.x, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -1376,6 +1431,71 @@ __global__ void k_primitives(const double *u, double *prim, size_t NK, double gm
     prim[(DIM + 1) * NK + i] = gm1 * (u[(DIM + 1) * NK + i] - 0.5 * rho * vsq);
 }
 
+// DGSEMOperator::RecoverStateFromWeighted (DGSEMOperator.cpp:161-424), one
+// thread per node: U = rU / r off the axis; on it (low-order axis
+// reconstruction, as an "axis" boundary selects) each of rho, rho u_z, rho E
+// from the ray fit through the origin over the nodes j1, j2 of the node's
+// descriptor, minmod-limited against 1.5 f_j / r_j and capped at
+// 10 |f_j1| / r_j1 (one node: its f / r), radial momentum 0; a density below
+// the floor, or a non-finite one or energy, takes node jn's state instead.
+// The nodes are chosen on the host from the geometry.  skip as KArgs::skip.
+__device__ __forceinline__ double ax_minmod(double a, double b)
+{
+    if (a * b <= 0.0) return 0.0;
+    const double m = fmin(fabs(a), fabs(b));
+    return a > 0.0 ? m : -m;
+}
+
+__device__ __forceinline__ double ax_clamp(double x, double m)
+{
+    if (!isfinite(x)) return 0.0;
+    if (fabs(x) <= m) return x;
+    return x > 0.0 ? m : (x < 0.0 ? -m : 0.0);
+}
+
+__global__ void __launch_bounds__(256) k_axis(const double *rU, double *U, Geo g, int K, double rho_floor,
+                                              const int *skip)
+{
+    if (skip && *skip) return;
+    const size_t NK = (size_t)g.NE * K;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= NK) return;
+    const int code = g.axm[q];
+    if (code < 0) {
+        const double r = g.rnode[q];
+#pragma unroll
+        for (int v = 0; v < 4; v++) U[v * NK + q] = rU[v * NK + q] / r;
+        return;
+    }
+    const int d = code >> 1;
+    const size_t eb = q - q % (size_t)K;
+    const int j1 = g.axj[3 * d], j2 = g.axj[3 * d + 1], jn = g.axj[3 * d + 2];
+    const double r1 = g.axr[3 * d], r2 = g.axr[3 * d + 1], rn = g.axr[3 * d + 2];
+    auto fit = [&](int comp) -> double {
+        if (j1 < 0) return 0.0;
+        const double f1 = rU[comp * NK + eb + j1];
+        const double fref = fabs(f1);
+        const double cap = (fref < 1e-12) ? INFINITY : 10.0 * fref / fmax(r1, 1e-14);
+        if (j2 >= 0) {
+            const double f2 = rU[comp * NK + eb + j2];
+            const double m = (r1 * f1 + r2 * f2) / fmax(r1 * r1 + r2 * r2, 1e-14);
+            return ax_clamp(ax_minmod(m, ax_minmod(1.5 * (f1 / r1), 1.5 * (f2 / r2))), cap);
+        }
+        return ax_clamp(f1 / r1, cap);
+    };
+    const double rho = fit(0), mz = fit(1), E = fit(3);
+    if (!isfinite(rho) || rho < rho_floor || !isfinite(E)) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) U[v * NK + q] = rU[v * NK + eb + jn] / rn;
+        U[2 * NK + q] = 0.0;
+    } else {
+        U[q] = rho;
+        U[NK + q] = mz;
+        U[2 * NK + q] = 0.0;
+        U[3 * NK + q] = E;
+    }
+}
+
 // The NaN/Inf check of Simulation.cpp:1025-1035 inside a device loop, after
 // every step whose count is a multiple of `every`: a non-finite or
 // non-positive density or pressure stops the loop (bad = done = 1) before
@@ -1505,6 +1625,9 @@ struct prandtl_ctx {
     double *d_bcs = nullptr;       // [3][2][5] outer boundary states (open boxes only)
     void *d_geo = nullptr;         // general quadrilateral geometry tables
     bool gen = false;              // quadrilateral mesh (k_rhs general-geometry variant)
+    bool axi = false;              // axisymmetric (state r U)
+    double *d_axU = nullptr;       // [nv][NE][K] the state recovered from r U
+    double axis_floor = 1e-12;     // density floor of the axis reconstruction
     double hmin = 0.0;             // smallest element size (the dt policy's h_min)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -1636,6 +1759,15 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
             // quadrilateral mesh, natural shape
             using Cf = RCfg<DIM, NP, BLEND>;
             const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+            if (c->axi) {
+                // recover U from r U, then the axisymmetric RHS (Euler)
+                const size_t NK = (size_t)c->NE * c->K;
+                hipLaunchKernelGGL(k_axis, dim3((unsigned)((NK + 255) / 256)), dim3(256), 0, s, a.yw, c->d_axU,
+                                   c->geo, c->K, c->axis_floor, c->skip);
+                hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, false, true, true>), dim3(nb),
+                                   dim3(Cf::BLOCK), 0, s, a);
+                return hip_check(c, hipGetLastError(), "k_rhs launch");
+            }
             if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
                 hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, float, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
             else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
@@ -1684,7 +1816,8 @@ KArgs make_kargs(const prandtl_ctx *c, const double *y, const double *u0, double
     a.ec = c->prm.ec_surface; a.quirks = c->prm.quirks;
     a.mode = mode; a.dt = dt; a.c0 = c0; a.c1 = c1;
     a.dtp = dtp;
-    a.y = y; a.u0 = u0; a.out = out;
+    // axisymmetric: k_rhs reads the recovered state, the update the weighted one
+    a.y = c->axi ? c->d_axU : y; a.yw = y; a.u0 = u0; a.out = out;
     a.visc = c->d_visc; a.fvf = c->d_fvf; a.alpha = c->d_alpha; a.eta = c->d_eta;
     a.lam = c->d_lam;
     a.skip = c->skip;
@@ -1713,6 +1846,7 @@ int stage_t(prandtl_ctx *c, const double *y, const double *u0, double *out, int 
 
 #define PRANDTL_DISPATCH(FN, ...)                                                   \
     switch (c->dim * 16 + c->Np) {                                                  \
+    case 2 * 16 + 2: return FN<2, 2>(__VA_ARGS__);                                  \
     case 2 * 16 + 3: return FN<2, 3>(__VA_ARGS__);                                  \
     case 2 * 16 + 4: return FN<2, 4>(__VA_ARGS__);                                  \
     case 2 * 16 + 5: return FN<2, 5>(__VA_ARGS__);                                  \
@@ -1788,7 +1922,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     *out = nullptr;
     const int dim = mesh->dim, N = mesh->order;
     if (dim < 2 || dim > 3) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "dim must be 2 or 3");
-    if (N < 2 || N > (dim == 2 ? 5 : 7))
+    if (N < (dim == 2 ? 1 : 2) || N > (dim == 2 ? 5 : 7))
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "order not built for this dim");
     if (prm->precision != PRANDTL_FP64 && prm->precision != PRANDTL_MIXED)
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "precision must be PRANDTL_FP64 or PRANDTL_MIXED");
@@ -1838,13 +1972,18 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     }
     if (open && prm->parabolic && dim == 3 && prm->quirks)
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "boundary sides are not built with the faithful NS-3D form");
+    if (prm->axisymmetric && !qm)
+        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "axisymmetric mode is built for quadrilateral meshes");
+    if (prm->axisymmetric && prm->parabolic)
+        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
+                    "axisymmetric mode is built for Euler (the reference weights only the inviscid terms by r)");
     std::vector<double> qstates;   // quadrilateral meshes: outer states per attribute
     if (qm) {
         if (qm->nattr < 0 || (qm->nattr > 0 && !qm->attr_bc)) return fail(nullptr, PRANDTL_ERR_ARG, "bad attribute table");
         qstates.assign(5 * (size_t)std::max(qm->nattr, 1), 0.0);
         for (int a = 0; a < qm->nattr; a++) {
             const int t = qm->attr_bc[a];
-            if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_NO_SLIP_ADIABATIC)
+            if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_AXIS)
                 return fail(nullptr, PRANDTL_ERR_ARG, "unknown boundary type");
             if (prm->parabolic && (t == PRANDTL_BC_SLIP || t == PRANDTL_BC_SYMMETRY))
                 return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
@@ -1927,11 +2066,13 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     quad::Geometry qg;
     if (qm) {
         std::string err;
-        if (!quad::build(qm, N, x, w, D, qg, err)) {
+        if (!quad::build(qm, N, x, w, D, prm->axisymmetric != 0, qg, err)) {
             delete c;
             return fail(nullptr, PRANDTL_ERR_ARG, err);
         }
         c->gen = true;
+        c->axi = prm->axisymmetric != 0;
+        if (prm->axis_rho_floor > 0.0) c->axis_floor = prm->axis_rho_floor;
         c->hmin = qg.hmin;
     }
     // Cartesian geometry: J = det diag(h), adj = diag of cofactors
@@ -2011,15 +2152,19 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (e == hipSuccess && qm) {
         // general geometry tables, one allocation
         auto bytes = [](const auto &v) { return v.size() * sizeof(v[0]); };
-        const size_t parts[8] = {bytes(qg.met), bytes(qg.jac), bytes(qg.fnor), bytes(qg.subn), bytes(qstates),
-                                 bytes(qg.fface), bytes(qg.fnb), bytes(qg.fnf)};
-        size_t off[9] = {0};
-        for (int i = 0; i < 8; i++) off[i + 1] = off[i] + ((parts[i] + 255) / 256) * 256;
-        e = hipMalloc(&c->d_geo, off[8]);
-        const void *src[8] = {qg.met.data(), qg.jac.data(), qg.fnor.data(), qg.subn.data(), qstates.data(),
-                              qg.fface.data(), qg.fnb.data(), qg.fnf.data()};
-        for (int i = 0; i < 8 && e == hipSuccess; i++)
-            e = hipMemcpy(static_cast<char *>(c->d_geo) + off[i], src[i], parts[i], hipMemcpyHostToDevice);
+        constexpr int NPART = 12;
+        const size_t parts[NPART] = {bytes(qg.met), bytes(qg.jac), bytes(qg.fnor), bytes(qg.subn), bytes(qstates),
+                                     bytes(qg.fface), bytes(qg.fnb), bytes(qg.fnf), bytes(qg.rnode),
+                                     bytes(qg.axm), bytes(qg.axj), bytes(qg.axr)};
+        size_t off[NPART + 1] = {0};
+        for (int i = 0; i < NPART; i++) off[i + 1] = off[i] + ((parts[i] + 255) / 256) * 256;
+        e = hipMalloc(&c->d_geo, off[NPART]);
+        const void *src[NPART] = {qg.met.data(), qg.jac.data(), qg.fnor.data(), qg.subn.data(), qstates.data(),
+                                  qg.fface.data(), qg.fnb.data(), qg.fnf.data(), qg.rnode.data(),
+                                  qg.axm.data(), qg.axj.data(), qg.axr.data()};
+        for (int i = 0; i < NPART && e == hipSuccess; i++)
+            if (parts[i])
+                e = hipMemcpy(static_cast<char *>(c->d_geo) + off[i], src[i], parts[i], hipMemcpyHostToDevice);
         char *b = static_cast<char *>(c->d_geo);
         c->geo.met = reinterpret_cast<const double *>(b + off[0]);
         c->geo.jac = reinterpret_cast<const double *>(b + off[1]);
@@ -2029,7 +2174,14 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         c->geo.fface = reinterpret_cast<const int *>(b + off[5]);
         c->geo.fnb = reinterpret_cast<const int *>(b + off[6]);
         c->geo.fnf = reinterpret_cast<const int *>(b + off[7]);
+        if (c->axi) {
+            c->geo.rnode = reinterpret_cast<const double *>(b + off[8]);
+            c->geo.axm = reinterpret_cast<const int *>(b + off[9]);
+            c->geo.axj = reinterpret_cast<const int *>(b + off[10]);
+            c->geo.axr = reinterpret_cast<const double *>(b + off[11]);
+        }
     }
+    if (e == hipSuccess && c->axi) e = hipMalloc(&c->d_axU, sizeof(double) * c->nv * NK);
     if (e == hipSuccess) e = hipMemset(c->d_alpha, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_eta, 0, sizeof(double) * c->NE);
     if (e == hipSuccess) e = hipMemset(c->d_lam, 0, sizeof(double) * c->NE);
@@ -2073,7 +2225,7 @@ int prandtl_hmin(const prandtl_ctx *c, double *hmin)
 void prandtl_destroy(prandtl_ctx *c)
 {
     if (!c) return;
-    double *bufs[] = {c->d_visc, c->d_fvf, c->d_alpha, c->d_eta, c->d_ya, c->d_yb};
+    double *bufs[] = {c->d_visc, c->d_fvf, c->d_alpha, c->d_eta, c->d_ya, c->d_yb, c->d_axU};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (c->d_lam) (void)hipFree(c->d_lam);
@@ -2250,10 +2402,17 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
             if (!r2 && check_every > 0) {
                 const size_t NK = (size_t)c->NE * c->K;
                 const int nb = (int)((NK + 255) / 256);
+                const double *uc = u;
+                if (c->axi) {
+                    // the recovered state is checked
+                    hipLaunchKernelGGL(k_axis, dim3(nb), dim3(256), 0, s, u, c->d_axU, c->geo, c->K, c->axis_floor,
+                                       c->skip);
+                    uc = c->d_axU;
+                }
                 if (c->dim == 2)
-                    hipLaunchKernelGGL(k_loop_check<2>, dim3(nb), dim3(256), 0, s, c->d_loop, u, NK, c->phys.gm1, check_every);
+                    hipLaunchKernelGGL(k_loop_check<2>, dim3(nb), dim3(256), 0, s, c->d_loop, uc, NK, c->phys.gm1, check_every);
                 else
-                    hipLaunchKernelGGL(k_loop_check<3>, dim3(nb), dim3(256), 0, s, c->d_loop, u, NK, c->phys.gm1, check_every);
+                    hipLaunchKernelGGL(k_loop_check<3>, dim3(nb), dim3(256), 0, s, c->d_loop, uc, NK, c->phys.gm1, check_every);
                 r2 = hip_check(c, hipGetLastError(), "loop check");
             }
         }
@@ -2333,6 +2492,11 @@ int prandtl_check_state(prandtl_ctx *c, const double *u, void *stream)
     const size_t NK = (size_t)c->NE * c->K;
     hipLaunchKernelGGL(k_reset_int, dim3(1), dim3(1), 0, s, c->d_flag);
     const int nb = (int)((NK + 255) / 256);
+    if (c->axi) {
+        // axisymmetric: the recovered state
+        hipLaunchKernelGGL(k_axis, dim3(nb), dim3(256), 0, s, u, c->d_axU, c->geo, c->K, c->axis_floor, nullptr);
+        u = c->d_axU;
+    }
     if (c->dim == 2) hipLaunchKernelGGL(k_check<2>, dim3(nb), dim3(256), 0, s, u, NK, c->phys.gm1, c->d_flag);
     else hipLaunchKernelGGL(k_check<3>, dim3(nb), dim3(256), 0, s, u, NK, c->phys.gm1, c->d_flag);
     int flag = 0;
@@ -2343,12 +2507,28 @@ int prandtl_check_state(prandtl_ctx *c, const double *u, void *stream)
     return flag ? fail(c, PRANDTL_ERR_STATE, "non-finite or non-positive density/pressure") : PRANDTL_OK;
 }
 
+int prandtl_axis_recover(prandtl_ctx *c, const double *rU, double *U, void *stream)
+{
+    if (!c || !rU || !U) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    if (!c->axi) return fail(c, PRANDTL_ERR_UNSUPPORTED, "not an axisymmetric context");
+    const size_t NK = (size_t)c->NE * c->K;
+    hipLaunchKernelGGL(k_axis, dim3((unsigned)((NK + 255) / 256)), dim3(256), 0, (hipStream_t)stream, rU, U,
+                       c->geo, c->K, c->axis_floor, nullptr);
+    return hip_check(c, hipGetLastError(), "k_axis launch");
+}
+
 int prandtl_primitives(prandtl_ctx *c, const double *u, double *prim, void *stream)
 {
     if (!c || !u || !prim) return fail(c, PRANDTL_ERR_ARG, "null argument");
     hipStream_t s = (hipStream_t)stream;
     const size_t NK = (size_t)c->NE * c->K;
     const int nb = (int)((NK + 255) / 256);
+    if (c->axi) {
+        // the output of the recovered state (Simulation.cpp:1041-1044)
+        const int rc = prandtl_axis_recover(c, u, c->d_axU, stream);
+        if (rc) return rc;
+        u = c->d_axU;
+    }
     if (c->dim == 2) hipLaunchKernelGGL(k_primitives<2>, dim3(nb), dim3(256), 0, s, u, prim, NK, c->phys.gm1);
     else hipLaunchKernelGGL(k_primitives<3>, dim3(nb), dim3(256), 0, s, u, prim, NK, c->phys.gm1);
     return hip_check(c, hipGetLastError(), "k_primitives launch");

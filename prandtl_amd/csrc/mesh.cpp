@@ -284,8 +284,89 @@ inline int line_node2(int NP, int r, int l, int m) { return r == 0 ? l * NP + m 
 
 } // namespace
 
+// The axis tables of an axisymmetric mesh.  The state recovery
+// (DGSEMOperator.cpp:161-424) picks, for a node on the axis, the two
+// nearest off-axis nodes of its element at its z (within 1e-12), else the
+// two nearest overall, and for the density-floor fallback the nearest
+// off-axis node; these depend on the geometry only and are found here once.
+// The axis index holds the nodes with |r| <= 1e-14 of elements with an axis
+// boundary face (DGSEMOperator.cpp:446-485); nodes with r <= 0 outside it
+// are recovered the same way.
+bool axis_tables(const prandtl_quad_mesh *m, int NP, const std::vector<double> &x,
+                 const std::vector<char> &axis_el, Geometry &g, std::string &err)
+{
+    const int NE = g.NE, K = NP * NP;
+    const size_t NK = (size_t)NE * K;
+    bool axis_bc = false;
+    for (int e = 0; e < NE; e++) axis_bc = axis_bc || axis_el[e];
+    g.rnode.assign(NK, 0.0);
+    std::vector<double> z(NK, 0.0);
+    for (int e = 0; e < NE; e++) {
+        double c[4][2];
+        for (int k = 0; k < 4; k++)
+            for (int i = 0; i < 2; i++)
+                c[k][i] = m->elem_coords ? m->elem_coords[(size_t)e * 8 + 2 * k + i]
+                                         : m->vert[2 * (size_t)m->elem[(size_t)e * 4 + k] + i];
+        for (int j = 0; j < NP; j++)
+            for (int i = 0; i < NP; i++) {
+                // the bilinear map, corners in vertex order (Tr.Transform)
+                const double xi = x[i], eta = x[j];
+                const double Nk[4] = {(1 - xi) * (1 - eta), xi * (1 - eta), xi * eta, (1 - xi) * eta};
+                double zz = 0.0, rr = 0.0;
+                for (int k = 0; k < 4; k++) {
+                    zz += Nk[k] * c[k][0];
+                    rr += Nk[k] * c[k][1];
+                }
+                const size_t q = (size_t)e * K + j * NP + i;
+                z[q] = zz;
+                g.rnode[q] = rr;
+                if (rr < -1e-14) {
+                    err = "element " + std::to_string(e) + " has a node below the axis (r < 0)";
+                    return false;
+                }
+            }
+    }
+    g.axm.assign(NK, -1);
+    g.axj.clear();
+    g.axr.clear();
+    for (int e = 0; e < NE; e++) {
+        const double *re = &g.rnode[(size_t)e * K], *ze = &z[(size_t)e * K];
+        for (int ld = 0; ld < K; ld++) {
+            const bool in_idx = axis_el[e] && std::fabs(re[ld]) <= 1e-14;
+            if (!in_idx && re[ld] > 0.0) continue;
+            if (!axis_bc) {
+                err = "nodes on the axis (r = 0) need an \"axis\" boundary";
+                return false;
+            }
+            int j1 = -1, j2 = -1, jn = -1;
+            double r1 = INFINITY, r2 = INFINITY, rn = -1.0;
+            auto take = [&](int j) {
+                if (re[j] < r1) { r2 = r1; j2 = j1; r1 = re[j]; j1 = j; }
+                else if (re[j] < r2) { r2 = re[j]; j2 = j; }
+            };
+            for (int j = 0; j < K; j++)
+                if (re[j] > 0.0 && std::fabs(ze[j] - ze[ld]) <= 1e-12) take(j);
+            if (j1 < 0)
+                for (int j = 0; j < K; j++)
+                    if (re[j] > 0.0) take(j);
+            for (int j = 0; j < K; j++)
+                if (re[j] > 0.0 && (jn < 0 || re[j] < rn)) { rn = re[j]; jn = j; }
+            if (jn < 0) {
+                err = "element " + std::to_string(e) + " has no node off the axis";
+                return false;
+            }
+            const int d = (int)(g.axj.size() / 3);
+            g.axj.insert(g.axj.end(), {j1, j2, jn});
+            g.axr.insert(g.axr.end(), {j1 >= 0 ? r1 : 0.0, j2 >= 0 ? r2 : 0.0, rn});
+            g.axm[(size_t)e * K + ld] = 2 * d + (in_idx ? 1 : 0);
+        }
+    }
+    return true;
+}
+
 bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
-           const std::vector<double> &w, const std::vector<double> &D, Geometry &g, std::string &err)
+           const std::vector<double> &w, const std::vector<double> &D, bool axisymmetric, Geometry &g,
+           std::string &err)
 {
     const int NE = m->nelem, NP = order + 1, K = NP * NP, NFP = NP, N = order;
     if (NE < 1 || !m->elem || (!m->vert && !m->elem_coords)) { err = "empty mesh"; return false; }
@@ -359,6 +440,7 @@ bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
     g.fnb.assign((size_t)NE * 4 * NFP, -1);
     g.fnf.assign((size_t)NE * 4, -1);
     g.fnor.assign((size_t)NE * 4 * NFP * 2, 0.0);
+    std::vector<char> axis_el(NE, 0);   // elements with an axis boundary face
     for (int e = 0; e < NE; e++)
         for (int f = 0; f < 4; f++) {
             const size_t ef = (size_t)e * 4 + f;
@@ -390,10 +472,16 @@ bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
                     err = "boundary attribute " + std::to_string(attr) + " has no boundary condition";
                     return false;
                 }
-                const int t = m->attr_bc[attr - 1];
-                if (t < PRANDTL_BC_SLIP || t > PRANDTL_BC_NO_SLIP_ADIABATIC) {
+                int t = m->attr_bc[attr - 1];
+                if (t < PRANDTL_BC_SLIP || t > PRANDTL_BC_AXIS) {
                     err = "boundary attribute " + std::to_string(attr) + " needs a non-periodic boundary type";
                     return false;
+                }
+                if (t == PRANDTL_BC_AXIS) {
+                    // the symmetry flux (Simulation.cpp:520-524); the axis
+                    // nodes are this element's
+                    axis_el[e] = 1;
+                    t = PRANDTL_BC_SYMMETRY;
                 }
                 code = FACE_OWN | (t << 1) | ((attr - 1) << 4);
             }
@@ -429,6 +517,7 @@ bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
                         g.subn[((((size_t)e * 2 + r) * NFP + l) * N + (i - 1)) * 2 + d] = nor[d];
                 }
             }
+    if (axisymmetric && !axis_tables(m, NP, x, axis_el, g, err)) return false;
     return true;
 }
 

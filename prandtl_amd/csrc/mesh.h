@@ -30,12 +30,23 @@ struct Geometry {
     std::vector<double> fnor;   // [NE*4*NFP*2]: normal outward of the face's first element
     std::vector<double> subn;   // [NE][2][NFP][NP-1][2]: subcell interface normals
     double hmin = 0.0;
+    // axisymmetric meshes (r = y): r per node; per node -1 off the axis, else
+    // 2 d + z with d the node's axis descriptor and z = 1 when the node is in
+    // the axis index (its radial-momentum RHS is zeroed); per descriptor the
+    // element-local nodes j1, j2 of the ray fit and jn of the density-floor
+    // fallback (-1: none) and their r
+    std::vector<double> rnode;  // [NE*K]
+    std::vector<int> axm;       // [NE*K]
+    std::vector<int> axj;       // [nax][3]
+    std::vector<double> axr;    // [nax][3]
 };
 
 // x, w: LGL nodes and weights on [0,1]; D: derivative matrix, row-major
-// NP x NP.  Returns false with err set for a mesh it cannot use.
+// NP x NP; axisymmetric: also the axis tables.  Returns false with err set
+// for a mesh it cannot use.
 bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
-           const std::vector<double> &w, const std::vector<double> &D, Geometry &g, std::string &err);
+           const std::vector<double> &w, const std::vector<double> &D, bool axisymmetric, Geometry &g,
+           std::string &err);
 
 } // namespace quad
 } // namespace prandtl

@@ -40,7 +40,7 @@ def _bc_code(kind):
 
 def _descs(dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0, parabolic=False, blending=False,
            alpha_max=0.5, alpha_min=0.001, ec_surface=False, quirks=True, precision="fp64",
-           sutherland=False, R_gas=287.05, bc=None, bc_state=None):
+           sutherland=False, R_gas=287.05, bc=None, bc_state=None, axisymmetric=False, axis_rho_floor=1e-12):
     """prandtl_mesh_desc and prandtl_params for one case.  bc: per direction
     a (low, high) pair of boundary types by name ("slip", "symmetry",
     "supersonic-inflow", "supersonic-outflow", "specified-state",
@@ -64,6 +64,7 @@ def _descs(dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0, parabolic=Fals
     p.ec_surface, p.quirks = int(ec_surface), int(quirks)
     p.precision = {"fp64": _lib.PRANDTL_FP64, "mixed": _lib.PRANDTL_MIXED}[precision]
     p.sutherland, p.R_gas = int(sutherland), float(R_gas)
+    p.axisymmetric, p.axis_rho_floor = int(axisymmetric), float(axis_rho_floor)
     return m, p
 
 
@@ -201,14 +202,16 @@ class DGSEMOperator:
     @classmethod
     def quad(cls, mesh, order, bc=None, gamma=1.4, blending=False, alpha_max=0.5, alpha_min=0.001,
              quirks=True, precision="fp64", parabolic=False, mu=0.0, Pr=0.72, sutherland=False,
-             R_gas=287.05):
+             R_gas=287.05, axisymmetric=False, axis_rho_floor=1e-12):
         """The operator on QuadMesh `mesh` (prandtl_create_quad); bc maps
         boundary names or attributes to a type or (type, state): the outer
         conservative state of inflow / specified-state boundaries, or
-        (V0, V1, 0, qn) of a no-slip wall."""
+        (V0, V1, 0, qn) of a no-slip wall.  axisymmetric: the state is r U
+        with r = y (an "axis" boundary marks the axis)."""
         _, p = _descs(2, order, (1, 1), (0.0, 0.0), (1.0, 1.0), gamma=gamma, blending=blending,
                       alpha_max=alpha_max, alpha_min=alpha_min, quirks=quirks, precision=precision,
-                      parabolic=parabolic, mu=mu, Pr=Pr, sutherland=sutherland, R_gas=R_gas)
+                      parabolic=parabolic, mu=mu, Pr=Pr, sutherland=sutherland, R_gas=R_gas,
+                      axisymmetric=axisymmetric, axis_rho_floor=axis_rho_floor)
         nattr, attr_bc, attr_state = mesh.attribute_tables(bc)
         dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
         q = _lib.QuadMesh()
@@ -379,6 +382,15 @@ class DGSEMOperator:
         check(lib().prandtl_primitives(self._h, self._vec(u, "u"), ctypes.c_void_p(prim.data_ptr()),
                                        _stream_ptr()), self._h)
         return prim
+
+    def recover(self, ru):
+        """The state U of the weighted state r U (axisymmetric operators,
+        prandtl_axis_recover), [nv][NE][K]."""
+        import torch
+        U = torch.empty((self.nv, self.NE, self.K), dtype=torch.float64, device=ru.device)
+        check(lib().prandtl_axis_recover(self._h, self._vec(ru, "u"), ctypes.c_void_p(U.data_ptr()),
+                                         _stream_ptr()), self._h)
+        return U
 
     def gradients(self, u):
         """Primitive-variable BR1 gradients [dim][nv][NE][K] (parabolic contexts)."""

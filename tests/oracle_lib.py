@@ -26,13 +26,14 @@ class OrcParams(ctypes.Structure):
         ("ec_surface", ctypes.c_int), ("quirks", ctypes.c_int), ("threads", ctypes.c_int),
         ("sutherland", ctypes.c_int), ("R_gas", ctypes.c_double),
         ("bc", (ctypes.c_int * 2) * 3), ("bc_state", ((ctypes.c_double * 5) * 2) * 3),
+        ("axisymmetric", ctypes.c_int), ("axis_rho_floor", ctypes.c_double),
     ]
 
 
 # boundary types by the names of the reference's configuration files
 # (Simulation.cpp:518-735); the values are the oracle's ORC_BC_* codes
 BC_CODES = {"periodic": 0, "slip": 1, "supersonic-inflow": 2, "supersonic-outflow": 3,
-            "symmetry": 4, "specified-state": 5, "no-slip-adiabatic": 6}
+            "symmetry": 4, "specified-state": 5, "no-slip-adiabatic": 6, "axis": 7}
 
 
 def bc_code(x):
@@ -85,6 +86,7 @@ def lib():
                                       ctypes.POINTER(ctypes.c_void_p)]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_sizes.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3
+        L.orc_axis_recover.argtypes = [ctypes.c_void_p, _P, _P]
         for name in ("orc_coords", "orc_weights", "orc_alpha", "orc_eta", "orc_last_gradients"):
             getattr(L, name).argtypes = [ctypes.c_void_p, _P]
         L.orc_rhs.argtypes = [ctypes.c_void_p, _P, _P]
@@ -220,10 +222,11 @@ class Oracle:
     @classmethod
     def quad(cls, mesh, order, gamma=1.4, blending=False, alpha_max=0.5, alpha_min=0.001,
              quirks=True, threads=1, ec_surface=False, parabolic=False, mu=0.0, Pr=0.72,
-             sutherland=False, R_gas=287.05):
+             sutherland=False, R_gas=287.05, axisymmetric=False, axis_rho_floor=1e-12):
         """The oracle on a quadrilateral mesh: `mesh` a dict with vert [nv,2],
         elem [ne,4], optional elem_coords [ne,4,2], bdr [nb,2], bdr_attr [nb],
-        attr_bc [na] (names or codes), attr_state [na,4]."""
+        attr_bc [na] (names or codes), attr_state [na,4].  axisymmetric: the
+        state is r U (r = y)."""
         self = cls.__new__(cls)
         p = OrcParams()
         p.dim, p.order = 2, order
@@ -234,6 +237,7 @@ class Oracle:
         p.alpha_max, p.alpha_min = alpha_max, alpha_min
         p.ec_surface, p.quirks, p.threads = int(ec_surface), int(quirks), int(threads)
         p.sutherland, p.R_gas = int(sutherland), float(R_gas)
+        p.axisymmetric, p.axis_rho_floor = int(axisymmetric), float(axis_rho_floor)
         set_bcs(p, 2, None, None)
         keep = {}
 
@@ -323,6 +327,13 @@ class Oracle:
         a = np.zeros(self.NE)
         lib().orc_eta(self.h, ptr(a))
         return a
+
+    def recover(self, rU):
+        """The state U of the weighted state r U (axisymmetric oracles)."""
+        rU = np.ascontiguousarray(rU, dtype=np.float64).reshape(-1)
+        U = np.zeros_like(rU)
+        assert lib().orc_axis_recover(self.h, ptr(rU), ptr(U)) == 0
+        return U
 
     def gradients(self):
         g = np.zeros((self.dim, self.nv, self.NE, self.K))
