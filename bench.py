@@ -45,10 +45,26 @@ WORKLOADS = {
                      bc={"left": ("supersonic-inflow", (1.4, 4.2, 0.0, 1.0 / 0.4 + 0.5 * 1.4 * 9.0)),
                          "right": "supersonic-outflow", "top": "slip", "bottom": "slip"}),
 }
+# the reference's axisymmetric cases on their own meshes (order 1, blending,
+# fixed dt; the flow is developed by spin-up steps of that dt)
+for _name, _case in (("ramp2d_n1", "RAMP"), ("nagashima2d_n1", "NAGASHIMA")):
+    WORKLOADS[_name] = dict(dim=2, order=1, gamma=1.4, axisymmetric=True, spinup=300, case=_case)
 MESH_DIR = os.path.join(ROOT, "tests", "golden", "meshes")
 # 64 independent 2-D Euler cases (isentropic vortices and blended
 # Kelvin-Helmholtz layers, meshes 8^2..64^2, N = 2..5) advanced together
 ENSEMBLE = "ensemble64"
+
+
+def resolve(cfg):
+    """A workload's settings, those of an axisymmetric case from prandtl_amd.cases."""
+    if "case" not in cfg:
+        return cfg
+    from prandtl_amd import cases
+    c = getattr(cases, cfg["case"])
+    out = dict(cfg, mesh=c["mesh"], order=c["order"], blending=c["blending"], alpha_max=c["alpha_max"],
+               bc=c["bc"], dt=c["dt"], state=c["state"], axis_rho_floor=c["axis_rho_floor"])
+    del out["case"]
+    return out
 
 
 def quad_mesh(cfg):
@@ -56,12 +72,16 @@ def quad_mesh(cfg):
     return QuadMesh.read(os.path.join(MESH_DIR, cfg["mesh"]), refine=cfg.get("refine", 0))
 
 
+def axi_kwargs(cfg):
+    return dict(axisymmetric=True, axis_rho_floor=cfg["axis_rho_floor"]) if cfg.get("axisymmetric") else {}
+
+
 def make_operator(cfg, quirks, precision):
     from prandtl_amd import DGSEMOperator
     if "mesh" in cfg:
         return DGSEMOperator.quad(quad_mesh(cfg), cfg["order"], bc=cfg["bc"], gamma=cfg["gamma"],
                                   blending=cfg.get("blending", False), alpha_max=cfg.get("alpha_max", 0.5),
-                                  quirks=quirks, precision=precision)
+                                  quirks=quirks, precision=precision, **axi_kwargs(cfg))
     return DGSEMOperator(quirks=quirks, precision=precision, **cfg)
 
 
@@ -81,7 +101,8 @@ def make_oracle(cfg, quirks, threads):
         d = dict(vert=m.vert, elem=m.elem, elem_coords=m.elem_coords, bdr=m.bdr, bdr_attr=m.bdr_attr,
                  attr_bc=kinds, attr_state=states)
         return O.Oracle.quad(d, cfg["order"], gamma=cfg["gamma"], blending=cfg.get("blending", False),
-                             alpha_max=cfg.get("alpha_max", 0.5), quirks=quirks, threads=threads)
+                             alpha_max=cfg.get("alpha_max", 0.5), quirks=quirks, threads=threads,
+                             **axi_kwargs(cfg))
     return O.Oracle(threads=threads, quirks=quirks, **cfg)
 
 
@@ -89,10 +110,14 @@ def initial_state(name, cfg):
     import numpy as np
     from prandtl_amd import cases
     if "mesh" in cfg:
-        # the reference's ForwardFacingStepIC: the inflow state everywhere
+        # the reference's ForwardFacingStepIC, RampIC, NagashimaIC: the free
+        # stream everywhere (r U in axisymmetric mode)
         m = quad_mesh(cfg)
         K = (cfg["order"] + 1) ** 2
-        return np.repeat(np.asarray(cfg["bc"]["left"][1], float)[:, None], m.nelem * K, axis=1)
+        U = np.asarray(cfg["state"] if "state" in cfg else cfg["bc"]["left"][1], float)
+        if cfg.get("axisymmetric"):
+            return cases.axisymmetric_state(U, m.node_coords(cfg["order"])).reshape(4, -1)
+        return np.repeat(U[:, None], m.nelem * K, axis=1)
     X = cases.node_coords(cfg["dim"], cfg["order"], cfg["nelem"], cfg["lo"], cfg["hi"])
     if name.startswith("tgv"):
         return cases.taylor_green(X)
@@ -362,7 +387,7 @@ def main():
 
     args.precision = args.precision or "fp64"
     name = args.workload
-    cfg = dict(WORKLOADS[name])
+    cfg = resolve(dict(WORKLOADS[name]))
     quirks = bool(args.faithful)
     dev = torch.device("cuda:0")
     # a non-default stream (default priority): the RK3 step is captured once
@@ -375,14 +400,18 @@ def main():
     nodes = op.NE * op.K
     if cfg.get("spinup"):
         # untimed: develop the flow (shock and blending) from the uniform start
-        op.Run(u, 0.0, 1e30, cfl=1.0, max_steps=cfg["spinup"])
+        if "dt" in cfg:
+            op.Run(u, 0.0, 1e30, dt=cfg["dt"], max_steps=cfg["spinup"])
+        else:
+            op.Run(u, 0.0, 1e30, cfl=1.0, max_steps=cfg["spinup"])
         torch.cuda.synchronize()
 
     # fixed dt from CFL 0.1 of the initial wave speed (TGV config policy)
     z = torch.empty_like(u)
     op.Mult(u, z)
     lam = op.GetMaxCharSpeed()
-    dt = 0.1 * op.hmin / (lam * (cfg["order"] + 1) ** 2)
+    # (the axisymmetric cases: their configured fixed dt)
+    dt = cfg["dt"] if "dt" in cfg else 0.1 * op.hmin / (lam * (cfg["order"] + 1) ** 2)
 
     # launch shapes of both kernels: timed on this state, fastest kept
     shape, shape_us = ({"rhs": 0, "visc": 0}, None) if args.no_autotune else op.autotune(u)
@@ -434,6 +463,9 @@ def main():
         by_rhs += 8 * 5 + (4 * Np * 20 + 4 * 4) / K
         if cfg.get("blending"):
             by_rhs += 2 * Np * (Np - 1) * 2 * 8 / K
+        if cfg.get("axisymmetric"):
+            # r and the axis code per node (U is recovered from r U in registers)
+            by_rhs += 12
     kname = "k_rhs<%d,%d>" % (cfg["dim"], cfg["order"] + 1)
     traffic = measured_traffic(name, kname) if (args.precision == "fp64" and not quirks) else None
     roof = roofline(fl_rhs * nodes, by_rhs * nodes, t_rhs, kname, traffic)

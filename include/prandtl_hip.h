@@ -38,7 +38,7 @@ typedef struct prandtl_ctx prandtl_ctx;
  * Simulation.cpp:338-346).  Element index = (ez*ny + ey)*nx + ex. */
 typedef struct prandtl_mesh_desc {
     int dim;            /* 2 or 3 */
-    int order;          /* polynomial degree N (Np = N+1): 2..5 in 2-D, 2..7 in 3-D */
+    int order;          /* polynomial degree N (Np = N+1): 1..5 in 2-D, 2..7 in 3-D */
     int nelem[3];       /* elements per direction */
     double lo[3], hi[3];
     /* Boundary condition of each box side: bc[r][0] at lo[r], bc[r][1] at
@@ -140,10 +140,11 @@ typedef struct prandtl_quad_mesh {
                                    if none uses one) */
 } prandtl_quad_mesh;
 
-/* DGSEMOperator constructor on a quadrilateral mesh (orders 2..5, Euler,
- * with or without blending).  Every element edge must be shared by two
- * elements or be a boundary segment whose attribute has a non-periodic
- * type.  Uses the current device. */
+/* DGSEMOperator constructor on a quadrilateral mesh (orders 1..5; Euler or
+ * Navier-Stokes, with or without blending; axisymmetric Euler with
+ * params->axisymmetric).  Every element edge must be shared by two elements
+ * or be a boundary segment whose attribute has a non-periodic type.  Uses
+ * the current device. */
 int prandtl_create_quad(const prandtl_quad_mesh *mesh, int order, const prandtl_params *params,
                         prandtl_ctx **out);
 

@@ -88,6 +88,7 @@ struct Geo {
     const int *axm;       // [NE*K] -1 off the axis, else 2 d + (1: radial RHS zeroed), d the descriptor
     const int *axj;       // [nax][3] element-local nodes j1, j2 (ray fit), jn (floor fallback); -1 none
     const double *axr;    // [nax][3] their r
+    double axis_floor;    // density floor of the axis reconstruction
 };
 
 struct KArgs {
@@ -98,8 +99,7 @@ struct KArgs {
     int mode;             // 0: out = rhs; 1: out = y + dt rhs; 2: out = c0 u0 + c1 (y + dt rhs)
     double dt, c0, c1;
     const double *dtp;    // if set, dt is read from device memory (graph replays)
-    const double *y;
-    const double *yw;     // axisymmetric: the weighted state r U of the stage update (y holds U)
+    const double *y;      // the state (axisymmetric: the weighted state r U)
     const double *u0;
     double *out;
     const void *visc;     // [nv-1][NE][K]            (FP32 in the mixed form)
@@ -330,6 +330,66 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false, bool
     double R[AXI ? C::EPB : 1][AXI ? C::K : 1];      // per-node r (axisymmetric)
 };
 
+// DGSEMOperator::RecoverStateFromWeighted (DGSEMOperator.cpp:161-424) for one
+// node q: U = rU / r off the axis; on it (low-order axis reconstruction, as
+// an "axis" boundary selects) each of rho, rho u_z, rho E from the ray fit
+// through the origin over the nodes j1, j2 of the node's descriptor,
+// minmod-limited against 1.5 f_j / r_j and capped at 10 |f_j1| / r_j1 (one
+// node: its f / r), radial momentum 0; a density below the floor, or a
+// non-finite one or energy, takes node jn's state instead.  The nodes are
+// chosen on the host from the geometry.
+__device__ __forceinline__ double ax_minmod(double a, double b)
+{
+    if (a * b <= 0.0) return 0.0;
+    const double m = fmin(fabs(a), fabs(b));
+    return a > 0.0 ? m : -m;
+}
+
+__device__ __forceinline__ double ax_clamp(double x, double m)
+{
+    if (!isfinite(x)) return 0.0;
+    if (fabs(x) <= m) return x;
+    return x > 0.0 ? m : (x < 0.0 ? -m : 0.0);
+}
+
+__device__ __forceinline__ void axis_node(const double *rU, const Geo &g, int K, size_t NK, size_t q, double U[4])
+{
+    const int code = g.axm[q];
+    if (code < 0) {
+        const double r = g.rnode[q];
+#pragma unroll
+        for (int v = 0; v < 4; v++) U[v] = rU[v * NK + q] / r;
+        return;
+    }
+    const int d = code >> 1;
+    const size_t eb = q - q % (size_t)K;
+    const int j1 = g.axj[3 * d], j2 = g.axj[3 * d + 1], jn = g.axj[3 * d + 2];
+    const double r1 = g.axr[3 * d], r2 = g.axr[3 * d + 1], rn = g.axr[3 * d + 2];
+    auto fit = [&](int comp) -> double {
+        if (j1 < 0) return 0.0;
+        const double f1 = rU[comp * NK + eb + j1];
+        const double fref = fabs(f1);
+        const double cap = (fref < 1e-12) ? INFINITY : 10.0 * fref / fmax(r1, 1e-14);
+        if (j2 >= 0) {
+            const double f2 = rU[comp * NK + eb + j2];
+            const double m = (r1 * f1 + r2 * f2) / fmax(r1 * r1 + r2 * r2, 1e-14);
+            return ax_clamp(ax_minmod(m, ax_minmod(1.5 * (f1 / r1), 1.5 * (f2 / r2))), cap);
+        }
+        return ax_clamp(f1 / r1, cap);
+    };
+    const double rho = fit(0), mz = fit(1), E = fit(3);
+    if (!isfinite(rho) || rho < g.axis_floor || !isfinite(E)) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) U[v] = rU[v * NK + eb + jn] / rn;
+        U[2] = 0.0;
+    } else {
+        U[0] = rho;
+        U[1] = mz;
+        U[2] = 0.0;
+        U[3] = E;
+    }
+}
+
 // One tile of k_rhs: elements [tile*EPB, tile*EPB + EPB) of the case A
 // describes; slot indexes the tile's running wave-speed maximum.
 // BLEND: modal indicator + subcell FV blending compiled in.
@@ -338,9 +398,10 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false, bool
 // FB: forced workgroup size (0: the natural one of RCfg).
 // OPEN: boundary sides on a Cartesian box.  GEN: general 2-D quadrilateral
 // geometry (per-node metrics, face tables; boundaries by attribute).  AXI
-// (with GEN, Euler): axisymmetric, A.y the recovered state U, A.yw the
-// weighted state r U; two-point, subcell and face fluxes weighted by r, the
-// radial momentum's pressure source, its RHS zeroed on the axis.
+// (with GEN, Euler): axisymmetric, A.y the weighted state r U, recovered
+// node by node as it is loaded (this element's nodes and the neighbours'
+// face points); two-point, subcell and face fluxes weighted by r, the radial
+// momentum's pressure source, its RHS zeroed on the axis.
 template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false, bool GEN = false,
           bool AXI = false>
 __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const int slot,
@@ -401,9 +462,13 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     // general geometry: this node's adjugate rows and Jacobian (and r)
     double jn = 1.0, mown[4] = {0.0, 0.0, 0.0, 0.0}, rown = 0.0;
     int axown = -1;
+    double yw[AXI ? NV : 1];   // axisymmetric: the weighted state of the update
     if (AXI && act) {
         rown = A.geo.rnode[(size_t)e * K + n];
         axown = A.geo.axm[(size_t)e * K + n];
+#pragma unroll
+        for (int v = 0; v < NV; v++) yw[v] = ys[v];
+        axis_node(A.y, A.geo, K, NK, (size_t)e * K + n, ys);
     }
     if (GEN && act) {
         jn = A.geo.jac[(size_t)e * K + n];
@@ -612,8 +677,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                 continue;
             }
             const int gn = A.geo.fnb[efo * NFP + l];
+            if constexpr (AXI) {
+                axis_node(A.y, A.geo, K, NK, (size_t)gn, pn);
+            } else {
 #pragma unroll
-            for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)gn];
+                for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)gn];
+            }
             const bool own = code & 1;
             double *g1 = own ? so : pn;
             double *g2 = own ? pn : so;
@@ -903,7 +972,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             // ZeroAxisRadialMom (DGSEMOperator.cpp:521-533)
             if (AXI && v == 2 && axown >= 0 && (axown & 1)) tot = 0.0;
             const size_t gi = v * NK + (size_t)e * K + n;
-            const double yv = AXI ? A.yw[gi] : sU[el][v][n];
+            const double yv = AXI ? yw[v] : sU[el][v][n];
             double o;
             if (A.mode == 0) o = tot;
             else if (A.mode == 1) o = yv + dt * tot;
@@ -1431,69 +1500,18 @@ __global__ void k_primitives(const double *u, double *prim, size_t NK, double gm
     prim[(DIM + 1) * NK + i] = gm1 * (u[(DIM + 1) * NK + i] - 0.5 * rho * vsq);
 }
 
-// DGSEMOperator::RecoverStateFromWeighted (DGSEMOperator.cpp:161-424), one
-// thread per node: U = rU / r off the axis; on it (low-order axis
-// reconstruction, as an "axis" boundary selects) each of rho, rho u_z, rho E
-// from the ray fit through the origin over the nodes j1, j2 of the node's
-// descriptor, minmod-limited against 1.5 f_j / r_j and capped at
-// 10 |f_j1| / r_j1 (one node: its f / r), radial momentum 0; a density below
-// the floor, or a non-finite one or energy, takes node jn's state instead.
-// The nodes are chosen on the host from the geometry.  skip as KArgs::skip.
-__device__ __forceinline__ double ax_minmod(double a, double b)
-{
-    if (a * b <= 0.0) return 0.0;
-    const double m = fmin(fabs(a), fabs(b));
-    return a > 0.0 ? m : -m;
-}
-
-__device__ __forceinline__ double ax_clamp(double x, double m)
-{
-    if (!isfinite(x)) return 0.0;
-    if (fabs(x) <= m) return x;
-    return x > 0.0 ? m : (x < 0.0 ? -m : 0.0);
-}
-
-__global__ void __launch_bounds__(256) k_axis(const double *rU, double *U, Geo g, int K, double rho_floor,
-                                              const int *skip)
+// RecoverStateFromWeighted for every node (prandtl_axis_recover, state checks
+// and output of axisymmetric contexts).  skip as KArgs::skip.
+__global__ void __launch_bounds__(256) k_axis(const double *rU, double *U, Geo g, int K, const int *skip)
 {
     if (skip && *skip) return;
     const size_t NK = (size_t)g.NE * K;
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= NK) return;
-    const int code = g.axm[q];
-    if (code < 0) {
-        const double r = g.rnode[q];
+    double u[4];
+    axis_node(rU, g, K, NK, q, u);
 #pragma unroll
-        for (int v = 0; v < 4; v++) U[v * NK + q] = rU[v * NK + q] / r;
-        return;
-    }
-    const int d = code >> 1;
-    const size_t eb = q - q % (size_t)K;
-    const int j1 = g.axj[3 * d], j2 = g.axj[3 * d + 1], jn = g.axj[3 * d + 2];
-    const double r1 = g.axr[3 * d], r2 = g.axr[3 * d + 1], rn = g.axr[3 * d + 2];
-    auto fit = [&](int comp) -> double {
-        if (j1 < 0) return 0.0;
-        const double f1 = rU[comp * NK + eb + j1];
-        const double fref = fabs(f1);
-        const double cap = (fref < 1e-12) ? INFINITY : 10.0 * fref / fmax(r1, 1e-14);
-        if (j2 >= 0) {
-            const double f2 = rU[comp * NK + eb + j2];
-            const double m = (r1 * f1 + r2 * f2) / fmax(r1 * r1 + r2 * r2, 1e-14);
-            return ax_clamp(ax_minmod(m, ax_minmod(1.5 * (f1 / r1), 1.5 * (f2 / r2))), cap);
-        }
-        return ax_clamp(f1 / r1, cap);
-    };
-    const double rho = fit(0), mz = fit(1), E = fit(3);
-    if (!isfinite(rho) || rho < rho_floor || !isfinite(E)) {
-#pragma unroll
-        for (int v = 0; v < 4; v++) U[v * NK + q] = rU[v * NK + eb + jn] / rn;
-        U[2 * NK + q] = 0.0;
-    } else {
-        U[q] = rho;
-        U[NK + q] = mz;
-        U[2 * NK + q] = 0.0;
-        U[3 * NK + q] = E;
-    }
+    for (int v = 0; v < 4; v++) U[v * NK + q] = u[v];
 }
 
 // The NaN/Inf check of Simulation.cpp:1025-1035 inside a device loop, after
@@ -1626,8 +1644,7 @@ struct prandtl_ctx {
     void *d_geo = nullptr;         // general quadrilateral geometry tables
     bool gen = false;              // quadrilateral mesh (k_rhs general-geometry variant)
     bool axi = false;              // axisymmetric (state r U)
-    double *d_axU = nullptr;       // [nv][NE][K] the state recovered from r U
-    double axis_floor = 1e-12;     // density floor of the axis reconstruction
+    double *d_axU = nullptr;       // [nv][NE][K] the state recovered from r U (checks, output)
     double hmin = 0.0;             // smallest element size (the dt policy's h_min)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -1760,10 +1777,7 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
             using Cf = RCfg<DIM, NP, BLEND>;
             const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
             if (c->axi) {
-                // recover U from r U, then the axisymmetric RHS (Euler)
-                const size_t NK = (size_t)c->NE * c->K;
-                hipLaunchKernelGGL(k_axis, dim3((unsigned)((NK + 255) / 256)), dim3(256), 0, s, a.yw, c->d_axU,
-                                   c->geo, c->K, c->axis_floor, c->skip);
+                // the axisymmetric RHS (Euler), recovering U from r U as it loads
                 hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, false, true, true>), dim3(nb),
                                    dim3(Cf::BLOCK), 0, s, a);
                 return hip_check(c, hipGetLastError(), "k_rhs launch");
@@ -1816,8 +1830,7 @@ KArgs make_kargs(const prandtl_ctx *c, const double *y, const double *u0, double
     a.ec = c->prm.ec_surface; a.quirks = c->prm.quirks;
     a.mode = mode; a.dt = dt; a.c0 = c0; a.c1 = c1;
     a.dtp = dtp;
-    // axisymmetric: k_rhs reads the recovered state, the update the weighted one
-    a.y = c->axi ? c->d_axU : y; a.yw = y; a.u0 = u0; a.out = out;
+    a.y = y; a.u0 = u0; a.out = out;
     a.visc = c->d_visc; a.fvf = c->d_fvf; a.alpha = c->d_alpha; a.eta = c->d_eta;
     a.lam = c->d_lam;
     a.skip = c->skip;
@@ -2072,7 +2085,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         }
         c->gen = true;
         c->axi = prm->axisymmetric != 0;
-        if (prm->axis_rho_floor > 0.0) c->axis_floor = prm->axis_rho_floor;
+        c->geo.axis_floor = prm->axis_rho_floor > 0.0 ? prm->axis_rho_floor : 1e-12;
         c->hmin = qg.hmin;
     }
     // Cartesian geometry: J = det diag(h), adj = diag of cofactors
@@ -2405,8 +2418,7 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
                 const double *uc = u;
                 if (c->axi) {
                     // the recovered state is checked
-                    hipLaunchKernelGGL(k_axis, dim3(nb), dim3(256), 0, s, u, c->d_axU, c->geo, c->K, c->axis_floor,
-                                       c->skip);
+                    hipLaunchKernelGGL(k_axis, dim3(nb), dim3(256), 0, s, u, c->d_axU, c->geo, c->K, c->skip);
                     uc = c->d_axU;
                 }
                 if (c->dim == 2)
@@ -2494,7 +2506,7 @@ int prandtl_check_state(prandtl_ctx *c, const double *u, void *stream)
     const int nb = (int)((NK + 255) / 256);
     if (c->axi) {
         // axisymmetric: the recovered state
-        hipLaunchKernelGGL(k_axis, dim3(nb), dim3(256), 0, s, u, c->d_axU, c->geo, c->K, c->axis_floor, nullptr);
+        hipLaunchKernelGGL(k_axis, dim3(nb), dim3(256), 0, s, u, c->d_axU, c->geo, c->K, nullptr);
         u = c->d_axU;
     }
     if (c->dim == 2) hipLaunchKernelGGL(k_check<2>, dim3(nb), dim3(256), 0, s, u, NK, c->phys.gm1, c->d_flag);
@@ -2513,7 +2525,7 @@ int prandtl_axis_recover(prandtl_ctx *c, const double *rU, double *U, void *stre
     if (!c->axi) return fail(c, PRANDTL_ERR_UNSUPPORTED, "not an axisymmetric context");
     const size_t NK = (size_t)c->NE * c->K;
     hipLaunchKernelGGL(k_axis, dim3((unsigned)((NK + 255) / 256)), dim3(256), 0, (hipStream_t)stream, rU, U,
-                       c->geo, c->K, c->axis_floor, nullptr);
+                       c->geo, c->K, nullptr);
     return hip_check(c, hipGetLastError(), "k_axis launch");
 }
 

@@ -27,7 +27,7 @@ from prandtl_amd import cases  # noqa: E402
 def main():
     out = {}
     for name, cfg in bench.WORKLOADS.items():
-        cfg = dict(cfg)
+        cfg = bench.resolve(dict(cfg))
         if name == "tgv3d_n7":
             cfg["nelem"] = (8, 8, 8)
         for quirks in (True, False):
