@@ -24,6 +24,7 @@
 #include "prandtl_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -916,6 +917,211 @@ int build_mesh_quad(orc_ctx *c, const orc_quad_mesh *q)
     return 0;
 }
 
+/* Hexahedral mesh geometry, restating what mfem::Mesh and its element
+ * transformations give the reference for 3-D meshes (DGSEMIntegrator.cpp:
+ * 251-267, 186; BdrFaceIntegrator.cpp:79-95):
+ *  - node (i, j, k) of element e sits at (xi_i, eta_j, zeta_k) of the
+ *    trilinear map of its eight corners (MFEM's vertex order: the zeta = 0
+ *    face counterclockwise, then zeta = 1); the Jacobian there is the sum
+ *    over corners of corner times the trilinear shape derivative, its
+ *    adjugate rows the contravariant metrics, its determinant J;
+ *  - a quadrilateral face shared by two elements is one face, its first
+ *    element the lower index; its points pair up through the four shared
+ *    vertices (any of the 8 relative orientations); the normal at each
+ *    point is CalcOrtho of the face Jacobian, the cross product of the two
+ *    tangents, outward of the first element: the first element's adjugate
+ *    row r at that node, with the sign of its side;
+ *  - a face no other element shares is a boundary face of the boundary
+ *    quadrilateral with the same four vertices.
+ * Returns 0, or 8 for a mesh that does not close or has J <= 0. */
+int build_mesh_hex(orc_ctx *c, const orc_hex_mesh *q)
+{
+    Mesh &m = c->m;
+    m.quad = true;
+    m.dim = 3; m.N = c->p.order; m.Np = m.N + 1; m.nv = 5;
+    m.K = m.Np * m.Np * m.Np; m.Nfp = m.Np * m.Np;
+    m.NE = q->nelem;
+    m.n[0] = m.NE; m.n[1] = m.n[2] = 1;
+    const int K = m.K, NE = m.NE, Np = m.Np, N = m.N, Nfp = m.Nfp;
+    const size_t NK = (size_t)NE * K;
+    auto cornerx = [&](int e, int k, int i) {
+        return q->elem_coords ? q->elem_coords[(size_t)e * 24 + 3 * k + i]
+                              : q->vert[3 * (size_t)q->elem[(size_t)e * 8 + k] + i];
+    };
+    /* reference-cube corners of MFEM's vertex order */
+    const int cv[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    auto jac = [&](int e, const double xi[3], double Jm[3][3], double X[3]) {
+        for (int a = 0; a < 3; a++) {
+            X[a] = 0.0;
+            for (int b = 0; b < 3; b++) Jm[a][b] = 0.0;
+        }
+        for (int k = 0; k < 8; k++) {
+            double f[3], df[3];
+            for (int d = 0; d < 3; d++) {
+                f[d] = cv[k][d] ? xi[d] : 1.0 - xi[d];
+                df[d] = cv[k][d] ? 1.0 : -1.0;
+            }
+            const double Nk = f[0] * f[1] * f[2];
+            const double dN[3] = {df[0] * f[1] * f[2], f[0] * df[1] * f[2], f[0] * f[1] * df[2]};
+            for (int a = 0; a < 3; a++) {
+                const double ca = cornerx(e, k, a);
+                X[a] += Nk * ca;
+                for (int b = 0; b < 3; b++) Jm[a][b] += dN[b] * ca;
+            }
+        }
+    };
+    m.J.assign(NK, 0.0);
+    m.adj.assign(NK * 9, 0.0);
+    m.X.assign(3 * NK, 0.0);
+    m.hmin = 1e300;
+    for (int e = 0; e < NE; e++) {
+        for (int kk = 0; kk < Np; kk++)
+            for (int j = 0; j < Np; j++)
+                for (int i = 0; i < Np; i++) {
+                    const double xi[3] = {c->x[i], c->x[j], c->x[kk]};
+                    double Jm[3][3], X[3];
+                    jac(e, xi, Jm, X);
+                    const size_t nd = (size_t)e * K + (kk * Np + j) * Np + i;
+                    for (int a = 0; a < 3; a++) m.X[a * NK + nd] = X[a];
+                    /* adj(J): row r is the contravariant metric Ja^r */
+                    double *ad = &m.adj[nd * 9];
+                    ad[0] = Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1];
+                    ad[1] = Jm[0][2] * Jm[2][1] - Jm[0][1] * Jm[2][2];
+                    ad[2] = Jm[0][1] * Jm[1][2] - Jm[0][2] * Jm[1][1];
+                    ad[3] = Jm[1][2] * Jm[2][0] - Jm[1][0] * Jm[2][2];
+                    ad[4] = Jm[0][0] * Jm[2][2] - Jm[0][2] * Jm[2][0];
+                    ad[5] = Jm[0][2] * Jm[1][0] - Jm[0][0] * Jm[1][2];
+                    ad[6] = Jm[1][0] * Jm[2][1] - Jm[1][1] * Jm[2][0];
+                    ad[7] = Jm[0][1] * Jm[2][0] - Jm[0][0] * Jm[2][1];
+                    ad[8] = Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0];
+                    m.J[nd] = Jm[0][0] * ad[0] + Jm[0][1] * ad[3] + Jm[0][2] * ad[6];
+                    if (!(m.J[nd] > 0.0)) return 8;
+                }
+        /* GetElementSize(e, 1): the smallest singular value of J at the
+         * centre, from the eigenvalues of J^T J by cyclic Jacobi rotations */
+        const double xc[3] = {0.5, 0.5, 0.5};
+        double Jm[3][3], X[3], A[3][3];
+        jac(e, xc, Jm, X);
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                A[a][b] = 0.0;
+                for (int k = 0; k < 3; k++) A[a][b] += Jm[k][a] * Jm[k][b];
+            }
+        for (int sweep = 0; sweep < 60; sweep++) {
+            const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+            if (off <= 1e-34 * (A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2])) break;
+            for (int p = 0; p < 2; p++)
+                for (int r = p + 1; r < 3; r++) {
+                    if (A[p][r] == 0.0) continue;
+                    const double th = 0.5 * (A[r][r] - A[p][p]) / A[p][r];
+                    const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                    const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+                    for (int k = 0; k < 3; k++) {   /* A <- G^T A G */
+                        const double akp = A[k][p], akr = A[k][r];
+                        A[k][p] = cs * akp - sn * akr;
+                        A[k][r] = sn * akp + cs * akr;
+                    }
+                    for (int k = 0; k < 3; k++) {
+                        const double apk = A[p][k], ark = A[r][k];
+                        A[p][k] = cs * apk - sn * ark;
+                        A[r][k] = sn * apk + cs * ark;
+                    }
+                }
+        }
+        const double lmin = std::min(A[0][0], std::min(A[1][1], A[2][2]));
+        m.hmin = std::min(m.hmin, std::sqrt(std::max(lmin, 0.0)));
+    }
+    /* local face f = 2r + side: its corners at face coordinates (a, b) in
+     * the order (0,0), (1,0), (0,1), (1,1), a and b the other two directions
+     * in increasing order (face point l = a + Np b) */
+    const int fc[6][4] = {{0, 3, 4, 7}, {1, 2, 5, 6}, {0, 1, 4, 5}, {3, 2, 7, 6}, {0, 1, 3, 2}, {4, 5, 7, 6}};
+    auto vtx = [&](int e, int k) { return q->elem[(size_t)e * 8 + k]; };
+    auto key = [&](int a, int b, int cc, int d) {
+        std::array<int, 4> k4 = {a, b, cc, d};
+        std::sort(k4.begin(), k4.end());
+        return k4;
+    };
+    std::map<std::array<int, 4>, std::vector<std::pair<int, int>>> faces;
+    for (int e = 0; e < NE; e++)
+        for (int f = 0; f < 6; f++)
+            faces[key(vtx(e, fc[f][0]), vtx(e, fc[f][1]), vtx(e, fc[f][2]), vtx(e, fc[f][3]))].push_back({e, f});
+    std::map<std::array<int, 4>, int> bq;
+    for (int i = 0; i < q->nbdr; i++) {
+        const int *b = &q->bdr[4 * (size_t)i];
+        bq[key(b[0], b[1], b[2], b[3])] = q->bdr_attr[i];
+    }
+    m.fe1.clear(); m.fe2.clear(); m.fn1.clear(); m.fn2.clear(); m.fnor.clear();
+    auto normal = [&](int e, int f, int node, double *nor) {
+        const int r = f >> 1;
+        const double sg = (f & 1) ? 1.0 : -1.0;
+        for (int d = 0; d < 3; d++) nor[d] = sg * m.adj[((size_t)e * K + node) * 9 + r * 3 + d];
+    };
+    for (auto &kv : faces) {
+        const auto &lst = kv.second;
+        if (lst.size() > 2) return 8;
+        int e1 = lst[0].first, f1 = lst[0].second;
+        if (lst.size() == 2 && lst[1].first < e1) { e1 = lst[1].first; f1 = lst[1].second; }
+        const int r1 = f1 >> 1, s1 = f1 & 1;
+        if (lst.size() == 2) {
+            const bool first0 = (e1 == lst[0].first && f1 == lst[0].second);
+            const int e2 = first0 ? lst[1].first : lst[0].first;
+            const int f2 = first0 ? lst[1].second : lst[0].second;
+            const int r2 = f2 >> 1, s2 = f2 & 1;
+            /* where the first element's corners (0,0), (1,0), (0,1) sit on
+             * the second element's face */
+            int ca[3], cb[3];
+            const int pick[3] = {0, 1, 2};
+            for (int t = 0; t < 3; t++) {
+                const int g = vtx(e1, fc[f1][pick[t]]);
+                int hit = -1;
+                for (int u = 0; u < 4; u++)
+                    if (vtx(e2, fc[f2][u]) == g) hit = u;
+                if (hit < 0) return 8;
+                ca[t] = hit & 1;
+                cb[t] = hit >> 1;
+            }
+            m.fe1.push_back(e1);
+            m.fe2.push_back(e2);
+            for (int l = 0; l < Nfp; l++) {
+                const int a = l % Np, b = l / Np;
+                const int a2 = N * ca[0] + (ca[1] - ca[0]) * a + (ca[2] - ca[0]) * b;
+                const int b2 = N * cb[0] + (cb[1] - cb[0]) * a + (cb[2] - cb[0]) * b;
+                const int n1 = line_node(3, Np, r1, l, s1 ? N : 0);
+                m.fn1.push_back(n1);
+                m.fn2.push_back(line_node(3, Np, r2, a2 + Np * b2, s2 ? N : 0));
+                double nor[3];
+                normal(e1, f1, n1, nor);
+                for (int d = 0; d < 3; d++) m.fnor.push_back(nor[d]);
+            }
+        } else {
+            auto it = bq.find(kv.first);
+            if (it == bq.end()) return 8;
+            const int attr = it->second;
+            if (attr < 1 || attr > q->nattr) return 8;
+            const int type = q->attr_bc[attr - 1];
+            if (type < ORC_BC_SLIP || type > ORC_BC_NO_SLIP_ADIABATIC) return 8;
+            m.bq_e.push_back(e1);
+            m.bq_type.push_back(type);
+            m.bq_attr.push_back(attr - 1);
+            for (int l = 0; l < Nfp; l++) {
+                const int n1 = line_node(3, Np, r1, l, s1 ? N : 0);
+                m.bq_n.push_back(n1);
+                double nor[3];
+                normal(e1, f1, n1, nor);
+                for (int d = 0; d < 3; d++) m.bq_nor.push_back(nor[d]);
+            }
+        }
+    }
+    m.nf = (int)m.fe1.size();
+    m.axis_idx.assign(NK, 0);
+    m.attr_state.assign(5 * (size_t)std::max(q->nattr, 1), 0.0);
+    for (int a = 0; a < q->nattr; a++)
+        if (q->attr_state)
+            for (int v = 0; v < 5; v++) m.attr_state[5 * (size_t)a + v] = q->attr_state[5 * (size_t)a + v];
+    build_subcells(c);
+    return 0;
+}
+
 /* DGSEMOperator::RecoverStateFromWeighted (DGSEMOperator.cpp:161-424) with
  * low_order_axis set, as an "axis" boundary sets it (Simulation.cpp:
  * 527-531), which makes every axis node "troubled" (line 238) and skips the
@@ -1183,7 +1389,7 @@ template <class T> struct Engine {
                 T a1 = 1.0 / (w0 * T(m.J[(size_t)e * K + n1]));
                 for (int v = 0; v < nv; v++)
                     for (int cc = 0; cc < dim; cc++) {
-                        T fc = f[v] * m.bq_nor[(b * Nfp + q) * 2 + cc];
+                        T fc = f[v] * m.bq_nor[(b * Nfp + q) * dim + cc];
                         G[(cc * nv + v) * NK + (size_t)e * K + n1] += a1 * fc;
                     }
             }
@@ -1511,10 +1717,9 @@ template <class T> struct Engine {
             for (int v = 0; v < nv; v++) outer[v] = m.attr_state[5 * (size_t)m.bq_attr[b] + v];
             for (int q = 0; q < Nfp; q++) {
                 const int n1 = m.bq_n[(size_t)b * Nfp + q];
-                T s1[5], F[5], nor[2];
+                T s1[5], F[5], nor[3];
                 state(u, e, n1, s1);
-                nor[0] = m.bq_nor[((size_t)b * Nfp + q) * 2];
-                nor[1] = m.bq_nor[((size_t)b * Nfp + q) * 2 + 1];
+                for (int d = 0; d < dim; d++) nor[d] = m.bq_nor[((size_t)b * Nfp + q) * dim + d];
                 g_cat = CAT_FACE;
                 const int type = m.bq_type[b];
                 lam = mmax(lam, bc_flux(dim, gamma, c->p.quirks, type, s1, outer, nor, F));
@@ -1732,9 +1937,17 @@ void orc_bc_lift_flux(int dim, double gamma, int quirks, int type, const double 
     bc_lift_flux<double>(dim, gamma, quirks, type, w, outer, f);
 }
 
-static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **out);
+static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **out,
+                       const orc_hex_mesh *hm = nullptr);
 
 int orc_create(const orc_params *p, orc_ctx **out) { return create_impl(p, nullptr, out); }
+
+int orc_create_hex(const orc_params *p, const orc_hex_mesh *m, orc_ctx **out)
+{
+    if (!p || !m || p->dim != 3) return 1;
+    if (m->nelem < 1 || !m->elem || (!m->vert && !m->elem_coords)) return 3;
+    return create_impl(p, nullptr, out, m);
+}
 
 int orc_create_quad(const orc_params *p, const orc_quad_mesh *m, orc_ctx **out)
 {
@@ -1749,17 +1962,18 @@ int orc_create_quad(const orc_params *p, const orc_quad_mesh *m, orc_ctx **out)
     return create_impl(p, m, out);
 }
 
-static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **out)
+static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **out, const orc_hex_mesh *hm)
 {
     if (!p || !out) return 1;
     if (p->dim < 2 || p->dim > 3 || p->order < 1 || p->order > 12) return 2;
-    for (int r = 0; r < p->dim && !qm; r++)
+    const bool mesh = qm || hm;
+    for (int r = 0; r < p->dim && !mesh; r++)
         if (p->nelem[r] < 1 || !(p->hi[r] > p->lo[r])) return 3;
     if (p->sutherland && !(p->R_gas > 0.0)) return 5;
     /* axisymmetric: 2-D quadrilateral meshes, Euler (the reference's
      * Navier-Stokes element and face routines carry no r weighting) */
     if (p->axisymmetric && (!qm || p->parabolic)) return 9;
-    for (int r = 0; r < p->dim && !qm; r++) {
+    for (int r = 0; r < p->dim && !mesh; r++) {
         for (int side = 0; side < 2; side++)
             if (p->bc[r][side] < ORC_BC_PERIODIC || p->bc[r][side] > ORC_BC_NO_SLIP_ADIABATIC) return 6;
         if ((p->bc[r][0] == ORC_BC_PERIODIC) != (p->bc[r][1] == ORC_BC_PERIODIC)) return 6;
@@ -1780,7 +1994,12 @@ static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **o
     c->p = *p;
     lgl_rule(p->order, c->x, c->w);
     dmats(p->order, c->x, c->w, c->D, c->Dh, c->Dh2);
-    if (qm) {
+    if (hm) {
+        if (build_mesh_hex(c, hm) != 0) {
+            delete c;
+            return 8;
+        }
+    } else if (qm) {
         if (build_mesh_quad(c, qm) != 0) {
             delete c;
             return 8;

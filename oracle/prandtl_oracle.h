@@ -113,6 +113,27 @@ typedef struct orc_quad_mesh {
 } orc_quad_mesh;
 /* p->dim must be 2; p->nelem / lo / hi / bc are ignored */
 int orc_create_quad(const orc_params *p, const orc_quad_mesh *m, orc_ctx **out);
+
+/* 3-D hexahedral mesh: trilinear elements, vertices in MFEM's order (the
+ * zeta = 0 face counterclockwise seen from +zeta, then zeta = 1); element
+ * e's nodes lexicographic in its own frame.  elem_coords (optional) gives
+ * per-element corner coordinates [nelem][8][3] for periodic meshes.
+ * Boundary quadrilaterals (4 vertices each) carry attributes 1..nattr;
+ * attr_state holds 5 values per attribute.  p->dim must be 3. */
+typedef struct orc_hex_mesh {
+    int nvert;
+    const double *vert;
+    int nelem;
+    const int *elem;
+    const double *elem_coords;
+    int nbdr;
+    const int *bdr;
+    const int *bdr_attr;
+    int nattr;
+    const int *attr_bc;
+    const double *attr_state;
+} orc_hex_mesh;
+int orc_create_hex(const orc_params *p, const orc_hex_mesh *m, orc_ctx **out);
 void orc_destroy(orc_ctx *c);
 int orc_sizes(const orc_ctx *c, int *NE, int *K, int *nv);
 int orc_coords(const orc_ctx *c, double *X /* [dim][NE][K] */);

@@ -84,6 +84,9 @@ def lib():
         L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(ctypes.c_void_p)]
         L.orc_create_quad.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcQuadMesh),
                                       ctypes.POINTER(ctypes.c_void_p)]
+        # orc_hex_mesh has orc_quad_mesh's layout
+        L.orc_create_hex.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcQuadMesh),
+                                     ctypes.POINTER(ctypes.c_void_p)]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_sizes.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 3
         L.orc_axis_recover.argtypes = [ctypes.c_void_p, _P, _P]
@@ -220,16 +223,23 @@ class Oracle:
         self.NE, self.K, self.nv, self.dim = NE.value, K.value, nv.value, dim
 
     @classmethod
+    def hex(cls, mesh, order, **kw):
+        """The oracle on a hexahedral mesh: `mesh` a dict with vert [nv,3],
+        elem [ne,8], optional elem_coords [ne,8,3], bdr [nb,4], bdr_attr [nb],
+        attr_bc [na], attr_state [na,5]; keywords as quad()."""
+        return cls.quad(mesh, order, _dim=3, **kw)
+
+    @classmethod
     def quad(cls, mesh, order, gamma=1.4, blending=False, alpha_max=0.5, alpha_min=0.001,
              quirks=True, threads=1, ec_surface=False, parabolic=False, mu=0.0, Pr=0.72,
-             sutherland=False, R_gas=287.05, axisymmetric=False, axis_rho_floor=1e-12):
+             sutherland=False, R_gas=287.05, axisymmetric=False, axis_rho_floor=1e-12, _dim=2):
         """The oracle on a quadrilateral mesh: `mesh` a dict with vert [nv,2],
         elem [ne,4], optional elem_coords [ne,4,2], bdr [nb,2], bdr_attr [nb],
         attr_bc [na] (names or codes), attr_state [na,4].  axisymmetric: the
         state is r U (r = y)."""
         self = cls.__new__(cls)
         p = OrcParams()
-        p.dim, p.order = 2, order
+        p.dim, p.order = _dim, order
         for r in range(3):
             p.nelem[r], p.lo[r], p.hi[r] = 1, 0.0, 1.0
         p.gamma, p.Pr, p.mu = gamma, Pr, mu
@@ -263,13 +273,14 @@ class Oracle:
         q.attr_bc = arr("attr_bc", np.int32, ctypes.c_int)
         q.attr_state = arr("attr_state", np.float64, ctypes.c_double)
         h = ctypes.c_void_p()
-        rc = lib().orc_create_quad(ctypes.byref(p), ctypes.byref(q), ctypes.byref(h))
+        create = lib().orc_create_hex if _dim == 3 else lib().orc_create_quad
+        rc = create(ctypes.byref(p), ctypes.byref(q), ctypes.byref(h))
         if rc != 0:
-            raise ValueError("orc_create_quad failed: %d" % rc)
+            raise ValueError("orc_create_%s failed: %d" % ("hex" if _dim == 3 else "quad", rc))
         self.params, self.h = p, h
         NE, K, nv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         lib().orc_sizes(h, ctypes.byref(NE), ctypes.byref(K), ctypes.byref(nv))
-        self.NE, self.K, self.nv, self.dim = NE.value, K.value, nv.value, 2
+        self.NE, self.K, self.nv, self.dim = NE.value, K.value, nv.value, _dim
         return self
 
     def __del__(self):
