@@ -148,6 +148,36 @@ typedef struct prandtl_quad_mesh {
 int prandtl_create_quad(const prandtl_quad_mesh *mesh, int order, const prandtl_params *params,
                         prandtl_ctx **out);
 
+/* ---- Hexahedral meshes ----
+ * The 3-D counterpart (MFEM and Gmsh hexahedral meshes): trilinear
+ * elements with vertices in MFEM's order (the zeta = 0 face counterclockwise
+ * seen from +zeta, then the zeta = 1 face); per-node adjugates; faces paired
+ * through their four vertices in any of the 8 relative orientations; the
+ * normal at each face point is CalcOrtho of the face Jacobian (the first
+ * element's adjugate row, outward).  Element e's nodes are lexicographic in
+ * its own frame. */
+typedef struct prandtl_hex_mesh {
+    int nvert;
+    const double *vert;         /* [nvert][3] */
+    int nelem;
+    const int *elem;            /* [nelem][8] vertex indices */
+    const double *elem_coords;  /* NULL, or [nelem][8][3] (periodic meshes) */
+    int nbdr;
+    const int *bdr;             /* [nbdr][4] boundary quadrilaterals (vertex indices) */
+    const int *bdr_attr;        /* [nbdr] boundary attribute, 1..nattr */
+    int nattr;
+    const int *attr_bc;         /* [nattr] PRANDTL_BC_* of each attribute */
+    const double *attr_state;   /* [nattr][5] outer states (inflow / specified state:
+                                   conservative; no-slip walls: wall velocity in
+                                   [0..2], heat flux in [3]) */
+} prandtl_hex_mesh;
+
+/* DGSEMOperator constructor on a hexahedral mesh (orders 2..4; Euler or
+ * Navier-Stokes, with or without blending; the faithful Navier-Stokes 3-D
+ * form is not built here).  Uses the current device. */
+int prandtl_create_hex(const prandtl_hex_mesh *mesh, int order, const prandtl_params *params,
+                       prandtl_ctx **out);
+
 /* Mesh files: mfem::Mesh's readers for "MFEM mesh v1.0" (vertices, or L2_T1
  * first-order nodes for periodic meshes) and Gmsh 2.2 ASCII ($PhysicalNames,
  * $Nodes, $Elements with 4-node quadrilaterals and 2-node boundary lines;

@@ -32,6 +32,7 @@ EXPORTED = (
     "prandtl_checkpoint_write", "prandtl_checkpoint_read", "prandtl_checkpoint_save", "prandtl_checkpoint_load",
     "prandtl_create_quad", "prandtl_hmin", "prandtl_mesh_read", "prandtl_mesh_refine", "prandtl_mesh_view",
     "prandtl_mesh_name", "prandtl_mesh_free", "prandtl_mesh_last_error", "prandtl_axis_recover",
+    "prandtl_create_hex",
 )
 
 
@@ -49,6 +50,11 @@ class QuadMesh(ctypes.Structure):
                 ("bdr_attr", ctypes.POINTER(ctypes.c_int)),
                 ("nattr", ctypes.c_int), ("attr_bc", ctypes.POINTER(ctypes.c_int)),
                 ("attr_state", ctypes.POINTER(ctypes.c_double))]
+
+
+class HexMesh(QuadMesh):
+    """prandtl_hex_mesh: the same fields (vert [nv][3], elem [ne][8], elem_coords
+    [ne][8][3], bdr [nb][4], attr_state [nattr][5])."""
 
 
 # boundary types (PRANDTL_BC_*) by the reference's configuration names
@@ -124,6 +130,8 @@ def lib():
     L.prandtl_ensemble_last_error.restype = ctypes.c_char_p
     L.prandtl_create_quad.argtypes = [ctypes.POINTER(QuadMesh), ctypes.c_int, ctypes.POINTER(Params),
                                       ctypes.POINTER(vp)]
+    L.prandtl_create_hex.argtypes = [ctypes.POINTER(HexMesh), ctypes.c_int, ctypes.POINTER(Params),
+                                     ctypes.POINTER(vp)]
     L.prandtl_hmin.argtypes = [vp, dp]
     L.prandtl_mesh_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
     L.prandtl_mesh_refine.argtypes = [vp]

@@ -326,7 +326,7 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false, bool
     double G[GE][C::NV][GK];
     double Al[C::EPB];
     double Red[(C::BLOCK + 63) / 64];
-    double M[GEN ? C::EPB : 1][4][GEN ? C::K : 1];   // per-node adjugate rows (general geometry)
+    double M[GEN ? C::EPB : 1][GEN ? DIM * DIM : 1][GEN ? C::K : 1];   // per-node adjugate rows (general geometry)
     double R[AXI ? C::EPB : 1][AXI ? C::K : 1];      // per-node r (axisymmetric)
 };
 
@@ -390,6 +390,14 @@ __device__ __forceinline__ void axis_node(const double *rU, const Geo &g, int K,
     }
 }
 
+// Face point l (a + NP b, a and b the other two coordinates in increasing
+// order) of a 3-D element's node on a face of direction r.
+template <int NP> __device__ __forceinline__ int face_point_3(int node, int r)
+{
+    const int i = node % NP, j = (node / NP) % NP, k = node / (NP * NP);
+    return r == 0 ? j + NP * k : (r == 1 ? i + NP * k : i + NP * j);
+}
+
 // One tile of k_rhs: elements [tile*EPB, tile*EPB + EPB) of the case A
 // describes; slot indexes the tile's running wave-speed maximum.
 // BLEND: modal indicator + subcell FV blending compiled in.
@@ -411,8 +419,8 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, NPAIR = C::NPAIR, NFV = C::NFV;
     constexpr int EPB = C::EPB, LC = C::LC, BLOCK = C::BLOCK, FXN = C::FXN, JF = C::JF;
     constexpr int NQ = RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN, AXI>::NQ;
-    static_assert(!GEN || DIM == 2, "general geometry is 2-D");
-    static_assert(!AXI || GEN, "axisymmetric meshes are quadrilateral meshes");
+    static_assert(!AXI || (GEN && DIM == 2), "axisymmetric meshes are quadrilateral meshes");
+    constexpr int MC = DIM * DIM;   // adjugate entries per node (general geometry)
     auto &sM = S.M;
     (void)sM;
     auto &sR = S.R;
@@ -460,9 +468,11 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
 #pragma unroll
         for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)e * K + n];
     // general geometry: this node's adjugate rows and Jacobian (and r)
-    double jn = 1.0, mown[4] = {0.0, 0.0, 0.0, 0.0}, rown = 0.0;
+    double jn = 1.0, mown[GEN ? MC : 1], rown = 0.0;
     int axown = -1;
     double yw[AXI ? NV : 1];   // axisymmetric: the weighted state of the update
+#pragma unroll
+    for (int c = 0; c < (GEN ? MC : 1); c++) mown[c] = 0.0;
     if (AXI && act) {
         rown = A.geo.rnode[(size_t)e * K + n];
         axown = A.geo.axm[(size_t)e * K + n];
@@ -470,10 +480,10 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         for (int v = 0; v < NV; v++) yw[v] = ys[v];
         axis_node(A.y, A.geo, K, NK, (size_t)e * K + n, ys);
     }
-    if (GEN && act) {
+    if constexpr (GEN) if (act) {
         jn = A.geo.jac[(size_t)e * K + n];
 #pragma unroll
-        for (int c = 0; c < 4; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
+        for (int c = 0; c < MC; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
     }
     // Blended kernels carry the subcell-flux work and its accumulator; they
     // do not hold loads in flight across phases (the surface data is loaded
@@ -509,7 +519,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         for (int v = 0; v < NV; v++) sU[el][v][n] = ys[v];
         if constexpr (GEN)
 #pragma unroll
-            for (int c = 0; c < 4; c++) sM[el][c][n] = mown[c];
+            for (int c = 0; c < MC; c++) sM[el][c][n] = mown[c];
         if constexpr (AXI) sR[el][n] = rown;
         const NodeQ<DIM> q = node_quants<DIM>(ys, P);
 #pragma unroll
@@ -642,17 +652,19 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             // general geometry: the face's normal and first element from the
             // face tables (owner computes, as on the box), boundary points by
             // attribute
-            const size_t efo = (size_t)eo * 4 + f;
+            const size_t efo = (size_t)eo * (2 * DIM) + f;
             const int code = A.geo.fface[efo];
             const int bt = (code >> 1) & 7;
-            const double nvec[2] = {A.geo.fnor[(efo * NFP + l) * 2], A.geo.fnor[(efo * NFP + l) * 2 + 1]};
+            double nvec[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) nvec[d] = A.geo.fnor[(efo * NFP + l) * DIM + d];
             // viscous traces Fv.n (the face's normal) of this side, and of the
             // neighbour's matching point
             const VT *fvfp = static_cast<const VT *>(A.fvf);
             double F[NV];
             if (bt != BC_PERIODIC) {
                 const double *bs = A.geo.bcs + (code >> 4) * 5;
-                const double lb = boundary_flux_v2(bt, nvec, so, bs, P, F);
+                const double lb = boundary_flux_vn<DIM>(bt, nvec, so, bs, P, quirk3d, F);
                 lam = (lam < lb) ? lb : lam;
                 if constexpr (AXI) {
                     // dU_face1 *= r (BdrFaceIntegrator.cpp:109-111)
@@ -667,8 +679,15 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     double fn[NV];
 #pragma unroll
                     for (int v = 1; v < NV; v++) fn[v] = (double)fvfp[(efo * (NV - 1) + (v - 1)) * NFP + l];
-                    if (bt == BC_NOSLIP)
-                        fn[NV - 1] = (bs[0] * fn[1] + bs[1] * fn[2]) + bs[3] * sqrt(nvec[0] * nvec[0] + nvec[1] * nvec[1]);
+                    if (bt == BC_NOSLIP) {
+                        double en = bs[0] * fn[1], nn2 = nvec[0] * nvec[0];
+#pragma unroll
+                        for (int d = 1; d < DIM; d++) {
+                            en = en + bs[d] * fn[1 + d];
+                            nn2 = nn2 + nvec[d] * nvec[d];
+                        }
+                        fn[NV - 1] = en + bs[3] * sqrt(nn2);
+                    }
 #pragma unroll
                     for (int v = 1; v < NV; v++) d[v] += fn[v];
                 }
@@ -691,7 +710,9 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             if (A.parabolic) {
                 const int nf = A.geo.fnf[efo];
                 const size_t fid = (size_t)(nf >> 1);
-                const int l2 = (nf & 1) ? NP - 1 - l : l;
+                int l2;
+                if constexpr (DIM == 2) l2 = (nf & 1) ? NP - 1 - l : l;
+                else l2 = face_point_3<NP>(gn - (int)(fid / 6) * K, (int)(fid % 6) >> 1);
 #pragma unroll
                 for (int v = 1; v < NV; v++)
                     fvn[v] = ((double)fvfp[(efo * (NV - 1) + (v - 1)) * NFP + l] +
@@ -706,7 +727,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             }
             const NodeQ<DIM> q1 = node_quants<DIM>(g1, P);
             const NodeQ<DIM> q2 = node_quants<DIM>(g2, P);
-            two_point_flux_v<DIM, true>(nvec, norml2_2(nvec), q1, q2, g1, g2, P, false, F);
+            two_point_flux_v<DIM, true>(nvec, norml2_v<DIM>(nvec), q1, q2, g1, g2, P, quirk3d, F);
             if constexpr (AXI) {
                 // flux_num *= r (DGSEMIntegrator.cpp:190-205), the r of the
                 // first element's node on both sides
@@ -844,8 +865,9 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     double F[NV];
                     if constexpr (GEN) {
                         // ComputeVolumeFlux with the mean of the two adjugate rows
-                        const double mm[2] = {(sM[ej][2 * r][na] + sM[ej][2 * r][nb]) * 0.5,
-                                              (sM[ej][2 * r + 1][na] + sM[ej][2 * r + 1][nb]) * 0.5};
+                        double mm[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) mm[d] = (sM[ej][DIM * r + d][na] + sM[ej][DIM * r + d][nb]) * 0.5;
                         two_point_flux_v<DIM, false>(mm, 0.0, qa, qb, nullptr, nullptr, P, false, F);
                         if constexpr (AXI) {
                             // f *= r_hat, r at the pair's midpoint (DGSEMIntegrator.cpp:272-292)
@@ -888,9 +910,11 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     qa.lam = sQ[ej][NQ - 1][na]; qb.lam = sQ[ej][NQ - 1][nb];
                     double F[NV];
                     if constexpr (GEN) {
-                        const double *sn = A.geo.subn + (((((size_t)(ebase + ej) * 2 + r) * NFP + l0 + lc) * (NP - 1) + i) * 2);
-                        const double nv2[2] = {sn[0], sn[1]};
-                        two_point_flux_v<DIM, true>(nv2, norml2_2(nv2), qa, qb, s1, s2, P, quirk3d, F);
+                        const double *sn = A.geo.subn + (((((size_t)(ebase + ej) * DIM + r) * NFP + l0 + lc) * (NP - 1) + i) * DIM);
+                        double nv2[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) nv2[d] = sn[d];
+                        two_point_flux_v<DIM, true>(nv2, norml2_v<DIM>(nv2), qa, qb, s1, s2, P, quirk3d, F);
                         if constexpr (AXI) {
                             // flux_num *= mid_r (DGSEMIntegrator.cpp:1030-1032, 1110-1112)
                             const double rh = (sR[ej][na] + sR[ej][nb]) * 0.5;
@@ -1070,8 +1094,8 @@ template <int DIM, int NP, bool MIXED, int FB, bool OPEN = false, bool GEN = fal
 __global__ void __launch_bounds__((VCfg<DIM, NP, FB>::BLOCK))
 __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArgs A)
 {
-    static_assert(!GEN || DIM == 2, "general geometry is 2-D");
     using C = VCfg<DIM, NP, FB>;
+    constexpr int MC = DIM * DIM;   // adjugate entries per node (general geometry)
     using R = typename std::conditional<MIXED, float, double>::type;
     if (A.skip && *A.skip) return;   // uniform: before any barrier
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, EPB = C::EPB, BLOCK = C::BLOCK;
@@ -1118,11 +1142,13 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
     if (act)
 #pragma unroll
         for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)e * K + n];
-    double jn = 1.0, mown[4] = {0.0, 0.0, 0.0, 0.0};
-    if (GEN && act) {
+    double jn = 1.0, mown[GEN ? MC : 1];
+#pragma unroll
+    for (int c = 0; c < (GEN ? MC : 1); c++) mown[c] = 0.0;
+    if constexpr (GEN) if (act) {
         jn = A.geo.jac[(size_t)e * K + n];
 #pragma unroll
-        for (int c = 0; c < 4; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
+        for (int c = 0; c < MC; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
     }
 #pragma unroll
     for (int k = 0; k < JT; k++) {
@@ -1141,7 +1167,7 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
                 nn = line_node<DIM, NP>(r, q, side ? NP - 1 : 0);
             }
             if constexpr (GEN) {
-                const size_t efo = (size_t)eo * 4 + f;
+                const size_t efo = (size_t)eo * (2 * DIM) + f;
                 if ((A.geo.fface[efo] >> 1) & 7) {
                     en = eo;
                     nn = line_node<DIM, NP>(r, q, side ? NP - 1 : 0);
@@ -1179,7 +1205,7 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
             const int f = rem / NFP, q = rem - f * NFP;
             double w[NV];
             cons2entropy<DIM>(tv[k], P, w);
-            const int code = GEN ? A.geo.fface[(size_t)eo * 4 + f] : 0;
+            const int code = GEN ? A.geo.fface[(size_t)eo * (2 * DIM) + f] : 0;
             const int bt = GEN ? ((code >> 1) & 7) : OPEN ? boundary_of(A.geo, eo, f >> 1, f & 1) : BC_PERIODIC;
             if ((OPEN || GEN) && bt != BC_PERIODIC) {
                 // the lifting flux itself (not a trace) for the surface part
@@ -1222,16 +1248,23 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
         for (int cc = 0; cc < DIM; cc++)
 #pragma unroll
             for (int v = 0; v < NV; v++)
-                gp[cc][v] = (sd[0][v] * (R)mown[cc] + sd[1][v] * (R)mown[2 + cc]) * invJ;
+            {
+                R a = sd[0][v] * (R)mown[cc];
+#pragma unroll
+                for (int c = 1; c < DIM; c++) a = a + sd[c][v] * (R)mown[DIM * c + cc];
+                gp[cc][v] = a * invJ;
+            }
 #pragma unroll
         for (int c = 0; c < DIM; c++) {
             const int a = ii[c];
             if (a == 0 || a == NP - 1) {
                 const int side = (a == NP - 1), f = 2 * c + side;
                 const int l = line_of<DIM, NP>(c, ii[0], ii[1], ii[2]);
-                const size_t ef = (size_t)e * 4 + f;
+                const size_t ef = (size_t)e * (2 * DIM) + f;
                 const int code = A.geo.fface[ef];
-                const R nv2[2] = {(R)A.geo.fnor[(ef * NFP + l) * 2], (R)A.geo.fnor[(ef * NFP + l) * 2 + 1]};
+                R nv2[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) nv2[d] = (R)A.geo.fnor[(ef * NFP + l) * DIM + d];
                 const bool bnd = ((code >> 1) & 7) != BC_PERIODIC;
                 const bool own = code & 1;
 #pragma unroll
@@ -1315,9 +1348,9 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
             for (int v = 1; v < NV; v++) {
                 if constexpr (GEN) {
                     // contravariant: adj row r times the physical flux
-                    R a = (R)mown[2 * r] * fl[v - 1][0];
+                    R a = (R)mown[DIM * r] * fl[v - 1][0];
 #pragma unroll
-                    for (int c = 1; c < DIM; c++) a += (R)mown[2 * r + c] * fl[v - 1][c];
+                    for (int c = 1; c < DIM; c++) a += (R)mown[DIM * r + c] * fl[v - 1][c];
                     sFV[el][r][v - 1][n] = a;
                 } else {
                     sFV[el][r][v - 1][n] = (R)A.geo.nrm[r] * fl[v - 1][r];
@@ -1336,10 +1369,17 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
                     R *dst = fvf + (((size_t)e * 2 * DIM + 2 * c + side) * (NV - 1)) * NFP;
                     if constexpr (GEN) {
                         // Fv.n with the face's normal (outward of its first element)
-                        const size_t ef = (size_t)e * 4 + 2 * c + side;
-                        const R n0 = (R)A.geo.fnor[(ef * NFP + l) * 2], n1 = (R)A.geo.fnor[(ef * NFP + l) * 2 + 1];
+                        const size_t ef = (size_t)e * (2 * DIM) + 2 * c + side;
+                        R nf[DIM];
 #pragma unroll
-                        for (int v = 1; v < NV; v++) dst[(v - 1) * NFP + l] = fl[v - 1][0] * n0 + fl[v - 1][1] * n1;
+                        for (int d = 0; d < DIM; d++) nf[d] = (R)A.geo.fnor[(ef * NFP + l) * DIM + d];
+#pragma unroll
+                        for (int v = 1; v < NV; v++) {
+                            R a = fl[v - 1][0] * nf[0];
+#pragma unroll
+                            for (int d = 1; d < DIM; d++) a = a + fl[v - 1][d] * nf[d];
+                            dst[(v - 1) * NFP + l] = a;
+                        }
                     } else {
 #pragma unroll
                         for (int v = 1; v < NV; v++) dst[(v - 1) * NFP + l] = fl[v - 1][c];
@@ -1711,6 +1751,9 @@ template <int DIM, int NP, int FB> void launch_visc_fb(prandtl_ctx *c, const VAr
     else hipLaunchKernelGGL((k_visc<DIM, NP, false, FB>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
 }
 
+// Orders the general-geometry kernels are built for: 2-D N = 1..5, 3-D N = 2..4.
+constexpr bool gen_built(int DIM, int NP) { return DIM == 2 ? NP <= 6 : NP <= 5; }
+
 template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
 {
     VArgs a;
@@ -1719,9 +1762,9 @@ template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, doub
     a.y = y; a.visc = c->d_visc; a.fvf = c->d_fvf; a.grad = grad;
     a.skip = c->skip;
     ProfScope ps(c, s, 1);
-    if constexpr (DIM == 2 && NP <= 6) {
+    if constexpr (gen_built(DIM, NP)) {
         if (c->gen) {
-            // quadrilateral mesh, natural shape
+            // quadrilateral or hexahedral mesh, natural shape
             using Cf = VCfg<DIM, NP, 0>;
             const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
             if (c->prm.precision == PRANDTL_MIXED)
@@ -1771,16 +1814,18 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
 {
     ProfScope ps(c, s, 0);
     const bool qns3 = DIM == 3 && c->prm.parabolic && c->prm.quirks;
-    if constexpr (DIM == 2 && NP <= 6) {
+    if constexpr (gen_built(DIM, NP)) {
         if (c->gen) {
-            // quadrilateral mesh, natural shape
+            // quadrilateral or hexahedral mesh, natural shape
             using Cf = RCfg<DIM, NP, BLEND>;
             const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
-            if (c->axi) {
-                // the axisymmetric RHS (Euler), recovering U from r U as it loads
-                hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, false, true, true>), dim3(nb),
-                                   dim3(Cf::BLOCK), 0, s, a);
-                return hip_check(c, hipGetLastError(), "k_rhs launch");
+            if constexpr (DIM == 2) {
+                if (c->axi) {
+                    // the axisymmetric RHS (Euler), recovering U from r U as it loads
+                    hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, false, true, true>), dim3(nb),
+                                       dim3(Cf::BLOCK), 0, s, a);
+                    return hip_check(c, hipGetLastError(), "k_rhs launch");
+                }
             }
             if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
                 hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, float, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
@@ -1926,10 +1971,11 @@ int prandtl_lgl(int order, double *x, double *w)
 
 namespace {
 
-// prandtl_create and prandtl_create_quad: qm null for a Cartesian box (mesh),
-// else a quadrilateral mesh (mesh then only carries dim = 2 and the order).
+// prandtl_create, prandtl_create_quad and prandtl_create_hex: qm and hm
+// null for a Cartesian box (mesh), else a quadrilateral or hexahedral mesh
+// (mesh then only carries dim and the order).
 int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, const prandtl_params *prm,
-                prandtl_ctx **out)
+                prandtl_ctx **out, const prandtl_hex_mesh *hm = nullptr)
 {
     if (!mesh || !prm || !out) return fail(nullptr, PRANDTL_ERR_ARG, "null argument");
     *out = nullptr;
@@ -1950,14 +1996,23 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         if (qm->nelem < 1) return fail(nullptr, PRANDTL_ERR_ARG, "empty mesh");
         NE = qm->nelem;
     }
-    for (int r = 0; r < dim && !qm; r++) {
+    if (hm) {
+        if (hm->nelem < 1) return fail(nullptr, PRANDTL_ERR_ARG, "empty mesh");
+        if (N > 4) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "hexahedral meshes are built for orders 2..4");
+        if (prm->parabolic && prm->quirks)
+            return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
+                        "hexahedral meshes are not built with the faithful Navier-Stokes 3-D form");
+        NE = hm->nelem;
+    }
+    const bool umesh = qm || hm;   // an unstructured mesh, not a box
+    for (int r = 0; r < dim && !umesh; r++) {
         if (mesh->nelem[r] < 2) return fail(nullptr, PRANDTL_ERR_ARG, "need >= 2 elements per direction");
         if (!(mesh->hi[r] > mesh->lo[r])) return fail(nullptr, PRANDTL_ERR_ARG, "empty box");
         NE *= mesh->nelem[r];
     }
     if (NE > (1LL << 30)) return fail(nullptr, PRANDTL_ERR_ARG, "mesh too large");
     bool open = false;
-    for (int r = 0; r < dim && !qm; r++) {
+    for (int r = 0; r < dim && !umesh; r++) {
         for (int side = 0; side < 2; side++) {
             const int t = mesh->bc[r][side];
             if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_NO_SLIP_ADIABATIC)
@@ -1990,12 +2045,16 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (prm->axisymmetric && prm->parabolic)
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
                     "axisymmetric mode is built for Euler (the reference weights only the inviscid terms by r)");
-    std::vector<double> qstates;   // quadrilateral meshes: outer states per attribute
-    if (qm) {
-        if (qm->nattr < 0 || (qm->nattr > 0 && !qm->attr_bc)) return fail(nullptr, PRANDTL_ERR_ARG, "bad attribute table");
-        qstates.assign(5 * (size_t)std::max(qm->nattr, 1), 0.0);
-        for (int a = 0; a < qm->nattr; a++) {
-            const int t = qm->attr_bc[a];
+    std::vector<double> qstates;   // unstructured meshes: outer states per attribute
+    if (umesh) {
+        const int nattr = qm ? qm->nattr : hm->nattr;
+        const int *attr_bc = qm ? qm->attr_bc : hm->attr_bc;
+        const double *attr_state = qm ? qm->attr_state : hm->attr_state;
+        const int NS = qm ? 4 : 5;   // values per attribute state
+        if (nattr < 0 || (nattr > 0 && !attr_bc)) return fail(nullptr, PRANDTL_ERR_ARG, "bad attribute table");
+        qstates.assign(5 * (size_t)std::max(nattr, 1), 0.0);
+        for (int a = 0; a < nattr; a++) {
+            const int t = attr_bc[a];
             if (t < PRANDTL_BC_PERIODIC || t > PRANDTL_BC_AXIS)
                 return fail(nullptr, PRANDTL_ERR_ARG, "unknown boundary type");
             if (prm->parabolic && (t == PRANDTL_BC_SLIP || t == PRANDTL_BC_SYMMETRY))
@@ -2003,21 +2062,25 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
                             "slip and symmetry sides have no viscous flux (Navier-Stokes: no-slip walls)");
             if (!prm->parabolic && t == PRANDTL_BC_NO_SLIP_ADIABATIC)
                 return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "no-slip walls need parabolic = 1");
+            if (hm && t == PRANDTL_BC_AXIS)
+                return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "axis boundaries are for 2-D axisymmetric meshes");
             if (t == PRANDTL_BC_NO_SLIP_ADIABATIC) {
-                if (!qm->attr_state) return fail(nullptr, PRANDTL_ERR_ARG, "wall state missing");
-                const double *g = qm->attr_state + 4 * (size_t)a;
+                if (!attr_state) return fail(nullptr, PRANDTL_ERR_ARG, "wall state missing");
+                const double *g = attr_state + NS * (size_t)a;
                 if (prm->quirks && g[3] != 0.0)
                     return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
                                 "faithful mode reproduces no-slip walls with zero heat flux only");
-                for (int v = 0; v < 4; v++) qstates[5 * (size_t)a + v] = g[v];
+                for (int v = 0; v < NS; v++) qstates[5 * (size_t)a + v] = g[v];
             }
             if (t == PRANDTL_BC_SUPERSONIC_INFLOW || t == PRANDTL_BC_SPECIFIED_STATE) {
-                if (!qm->attr_state) return fail(nullptr, PRANDTL_ERR_ARG, "boundary state missing");
-                const double *g = qm->attr_state + 4 * (size_t)a;
-                const double pg = (prm->gamma - 1.0) * (g[3] - 0.5 * (g[1] * g[1] + g[2] * g[2]) / g[0]);
+                if (!attr_state) return fail(nullptr, PRANDTL_ERR_ARG, "boundary state missing");
+                const double *g = attr_state + NS * (size_t)a;
+                double ke = 0.0;
+                for (int d = 0; d < dim; d++) ke += g[1 + d] * g[1 + d];
+                const double pg = (prm->gamma - 1.0) * (g[dim + 1] - 0.5 * ke / g[0]);
                 if (!(g[0] > 0.0) || !(pg > 0.0) || !std::isfinite(pg))
                     return fail(nullptr, PRANDTL_ERR_ARG, "boundary state needs positive density and pressure");
-                for (int v = 0; v < 4; v++) qstates[5 * (size_t)a + v] = g[v];
+                for (int v = 0; v < NS; v++) qstates[5 * (size_t)a + v] = g[v];
             }
         }
     }
@@ -2088,13 +2151,22 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         c->geo.axis_floor = prm->axis_rho_floor > 0.0 ? prm->axis_rho_floor : 1e-12;
         c->hmin = qg.hmin;
     }
+    if (hm) {
+        std::string err;
+        if (!quad::build_hex(hm, N, x, w, D, qg, err)) {
+            delete c;
+            return fail(nullptr, PRANDTL_ERR_ARG, err);
+        }
+        c->gen = true;
+        c->hmin = qg.hmin;
+    }
     // Cartesian geometry: J = det diag(h), adj = diag of cofactors
     double h[3] = {1, 1, 1};
     for (int r = 0; r < dim; r++) {
-        h[r] = qm ? 1.0 : (mesh->hi[r] - mesh->lo[r]) / mesh->nelem[r];
-        c->geo.n[r] = qm ? (r == 0 ? c->NE : 1) : mesh->nelem[r];
+        h[r] = umesh ? 1.0 : (mesh->hi[r] - mesh->lo[r]) / mesh->nelem[r];
+        c->geo.n[r] = umesh ? (r == 0 ? c->NE : 1) : mesh->nelem[r];
     }
-    if (!qm) {
+    if (!umesh) {
         c->hmin = h[0];
         for (int r = 1; r < dim; r++) c->hmin = h[r] < c->hmin ? h[r] : c->hmin;
     }
@@ -2112,7 +2184,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     c->geo.aface = 1.0 / (w[0] * J);
     c->geo.bcmask = 0;
     c->geo.bcs = nullptr;
-    for (int r = 0; r < dim && !qm; r++)
+    for (int r = 0; r < dim && !umesh; r++)
         for (int side = 0; side < 2; side++) c->geo.bcmask |= mesh->bc[r][side] << (3 * (2 * r + side));
     for (int i = 0; i < Np; i++) c->tab.fvs[i] = 1.0 / (J * w[i]);
     // subcell normals n_{i+1/2} = Ja + sum_{l<=i} w_l sum_q D_lq Ja (lines 1212-1233)
@@ -2162,7 +2234,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         if (e == hipSuccess) e = hipMemcpy(c->d_bcs, hb, sizeof(hb), hipMemcpyHostToDevice);
         c->geo.bcs = c->d_bcs;
     }
-    if (e == hipSuccess && qm) {
+    if (e == hipSuccess && umesh) {
         // general geometry tables, one allocation
         auto bytes = [](const auto &v) { return v.size() * sizeof(v[0]); };
         constexpr int NPART = 12;
@@ -2226,6 +2298,17 @@ int prandtl_create_quad(const prandtl_quad_mesh *mesh, int order, const prandtl_
     d.order = order;
     d.nelem[0] = mesh->nelem; d.nelem[1] = 1; d.nelem[2] = 1;
     return create_impl(&d, mesh, params, out);
+}
+
+int prandtl_create_hex(const prandtl_hex_mesh *mesh, int order, const prandtl_params *params, prandtl_ctx **out)
+{
+    if (!mesh) return fail(nullptr, PRANDTL_ERR_ARG, "null argument");
+    prandtl_mesh_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.dim = 3;
+    d.order = order;
+    d.nelem[0] = mesh->nelem; d.nelem[1] = 1; d.nelem[2] = 1;
+    return create_impl(&d, nullptr, params, out, mesh);
 }
 
 int prandtl_hmin(const prandtl_ctx *c, double *hmin)

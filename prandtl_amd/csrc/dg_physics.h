@@ -176,6 +176,33 @@ __device__ __forceinline__ double norml2_2(const double *n)
     return scale * sqrt(sum);
 }
 
+// MFEM Vector::Norml2 of a 3-vector (the same scaled sum).
+__device__ __forceinline__ double norml2_3(const double *n)
+{
+    double scale = 0.0, sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (n[i] != 0.0) {
+            const double a = fabs(n[i]);
+            if (scale <= a) {
+                const double q = scale / a;
+                sum = 1.0 + sum * (q * q);
+                scale = a;
+            } else {
+                const double q = a / scale;
+                sum += q * q;
+            }
+        }
+    }
+    return scale * sqrt(sum);
+}
+
+template <int DIM> __device__ __forceinline__ double norml2_v(const double *n)
+{
+    if constexpr (DIM == 2) return norml2_2(n);
+    else return norml2_3(n);
+}
+
 // two_point_flux for a general metric / normal vector nv (volume pairs:
 // the mean of the two nodes' adjugate rows; faces: the face normal), the
 // dissipation scaled by nd = |nv|.
@@ -339,6 +366,70 @@ __device__ __forceinline__ double boundary_flux_v2(int type, const double *n, co
     const NodeQ<2> q2 = node_quants<2>(g, P);
     two_point_flux_v<2, true>(n, nd, q1, q2, s, g, P, false, F);
     return (q1.lam < q2.lam) ? q2.lam : q1.lam;
+}
+
+// slip_wall_flux for a general 3-D outward normal n.  RotateState with the
+// unit normal u and its two tangents (BasicOperations.cpp:211-259) gives the
+// normal momentum m.u; the tangential ones enter only |v|, which the
+// rotation keeps, so |m| is used directly.
+__device__ __forceinline__ double slip_wall_flux_v3(const double *n, const double *s, const Phys &P, double *F)
+{
+    const double inv = 1.0 / norml2_3(n);
+    const double u0 = n[0] * inv, u1 = n[1] * inv, u2 = n[2] * inv;
+    const double mn = s[1] * u0 + s[2] * u1 + s[3] * u2;
+    const double rho = s[0];
+    const double ri = 1.0 / rho;
+    const double mm = s[1] * s[1] + s[2] * s[2] + s[3] * s[3];
+    const double vv = mm / (rho * rho);
+    const double VV = mm * (ri * ri);
+    const double p = P.gm1 * (s[4] - 0.5 * rho * VV);
+    const double an = sqrt(P.gamma * p / rho);
+    const double un = mn / rho;
+    const double gp1 = 1.0 + P.gamma, gp1i = 1.0 / gp1;
+    double p_star;
+    if (un > 0.0)
+        p_star = p + 0.25 * un * gp1 * rho * (un + sqrt(un * un + 8.0 * gp1i / rho * p * (P.gm1 * gp1i + 1.0)));
+    else {
+        double base = 1.0 + 0.5 * P.gm1 * un / an;
+        base = (base < 0.0001) ? 0.0001 : base;
+        p_star = p * pow(base, 2.0 * P.gamma * P.gm1i);
+    }
+    F[0] = 0.0;
+    F[1] = p_star * n[0];
+    F[2] = p_star * n[1];
+    F[3] = p_star * n[2];
+    F[4] = 0.0;
+    return sqrt(vv) + an;
+}
+
+// boundary_flux for a general 3-D outward normal n (hexahedral meshes).
+__device__ __forceinline__ double boundary_flux_v3(int type, const double *n, const double *s, const double *outer,
+                                                   const Phys &P, bool quirk3d, double *F)
+{
+    if (type == BC_SLIP || type == BC_NOSLIP) return slip_wall_flux_v3(n, s, P, F);
+    const double nd = norml2_3(n);
+    double g[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) g[v] = (type == BC_INFLOW || type == BC_SPECIFIED) ? outer[v] : s[v];
+    if (type == BC_SYMMETRY) {
+        const double inv = 1.0 / nd;
+        const double u[3] = {n[0] * inv, n[1] * inv, n[2] * inv};
+        const double mn = s[1] * u[0] + s[2] * u[1] + s[3] * u[2];
+#pragma unroll
+        for (int d = 0; d < 3; d++) g[1 + d] = s[1 + d] + -2.0 * mn * u[d];
+    }
+    const NodeQ<3> q1 = node_quants<3>(s, P);
+    const NodeQ<3> q2 = node_quants<3>(g, P);
+    two_point_flux_v<3, true>(n, nd, q1, q2, s, g, P, quirk3d, F);
+    return (q1.lam < q2.lam) ? q2.lam : q1.lam;
+}
+
+template <int DIM>
+__device__ __forceinline__ double boundary_flux_vn(int type, const double *n, const double *s, const double *outer,
+                                                   const Phys &P, bool quirk3d, double *F)
+{
+    if constexpr (DIM == 2) return boundary_flux_v2(type, n, s, outer, P, F);
+    else return boundary_flux_v3(type, n, s, outer, P, quirk3d, F);
 }
 
 // ComputeBdrFaceInviscidFlux of a boundary point (BdrFaceIntegrator.cpp:

@@ -651,3 +651,277 @@ void prandtl_mesh_free(prandtl_mesh_file *m) { delete m; }
 const char *prandtl_mesh_last_error(void) { return g_mesh_err.c_str(); }
 
 } // extern "C"
+
+namespace prandtl {
+namespace quad {
+
+namespace {
+
+// Eigenvalues of a symmetric 3x3 matrix by cyclic Jacobi rotations (the
+// off-diagonal sum falls quadratically); returns the smallest.
+double smallest_eig_sym3(double A[3][3])
+{
+    for (int sweep = 0; sweep < 50; sweep++) {
+        double off = 0.0, dia = 0.0;
+        for (int i = 0; i < 3; i++) {
+            dia += A[i][i] * A[i][i];
+            for (int j = i + 1; j < 3; j++) off += A[i][j] * A[i][j];
+        }
+        if (!(off > 1e-32 * dia)) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                if (A[p][q] == 0.0) continue;
+                const double tau = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < 3; k++) {
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; k++) {
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+            }
+    }
+    return std::min(A[0][0], std::min(A[1][1], A[2][2]));
+}
+
+// reference-cube corner k of MFEM's hexahedron vertex order
+const int HEX_C[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+// corners of local face f = 2r + side at face coordinates (a, b) = (0,0),
+// (1,0), (0,1), (1,1), a and b the other two directions in increasing order
+const int HEX_FACE[6][4] = {{0, 3, 4, 7}, {1, 2, 5, 6}, {0, 1, 4, 5}, {3, 2, 7, 6}, {0, 1, 3, 2}, {4, 5, 7, 6}};
+
+inline int line_node3(int NP, int r, int l, int m)
+{
+    const int a = l % NP, b = l / NP;
+    if (r == 0) return (b * NP + a) * NP + m;
+    if (r == 1) return (b * NP + m) * NP + a;
+    return (m * NP + b) * NP + a;
+}
+
+// Jacobian of the trilinear map at xi (TriLinear3DFiniteElement derivatives,
+// summed over the corners in order)
+void jacobian3(const double c[8][3], const double xi[3], double J[3][3])
+{
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) J[i][j] = 0.0;
+    for (int k = 0; k < 8; k++) {
+        double f[3], df[3];
+        for (int d = 0; d < 3; d++) {
+            f[d] = HEX_C[k][d] ? xi[d] : 1.0 - xi[d];
+            df[d] = HEX_C[k][d] ? 1.0 : -1.0;
+        }
+        const double dN[3] = {df[0] * f[1] * f[2], f[0] * df[1] * f[2], f[0] * f[1] * df[2]};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) J[i][j] += dN[j] * c[k][i];
+    }
+}
+
+struct Key4 {
+    int v[4];
+    bool operator<(const Key4 &o) const { return std::lexicographical_compare(v, v + 4, o.v, o.v + 4); }
+};
+
+Key4 key4(int a, int b, int c, int d)
+{
+    Key4 k{{a, b, c, d}};
+    std::sort(k.v, k.v + 4);
+    return k;
+}
+
+} // namespace
+
+bool build_hex(const prandtl_hex_mesh *m, int order, const std::vector<double> &x,
+               const std::vector<double> &w, const std::vector<double> &D, Geometry &g, std::string &err)
+{
+    const int NE = m->nelem, NP = order + 1, K = NP * NP * NP, NFP = NP * NP, N = order;
+    if (NE < 1 || !m->elem || (!m->vert && !m->elem_coords)) { err = "empty mesh"; return false; }
+    g.NE = NE; g.NP = NP; g.K = K; g.NFP = NFP;
+    for (size_t i = 0; i < (size_t)NE * 8; i++)
+        if (m->elem[i] < 0 || (!m->elem_coords && m->elem[i] >= m->nvert)) {
+            err = "element vertex out of range";
+            return false;
+        }
+    auto corners = [&](int e, double c[8][3]) {
+        for (int k = 0; k < 8; k++)
+            for (int i = 0; i < 3; i++)
+                c[k][i] = m->elem_coords ? m->elem_coords[(size_t)e * 24 + 3 * k + i]
+                                         : m->vert[3 * (size_t)m->elem[(size_t)e * 8 + k] + i];
+    };
+    const size_t NK = (size_t)NE * K;
+    g.met.assign(9 * NK, 0.0);
+    g.jac.assign(NK, 0.0);
+    g.hmin = INFINITY;
+    for (int e = 0; e < NE; e++) {
+        double c[8][3];
+        corners(e, c);
+        for (int k = 0; k < NP; k++)
+            for (int j = 0; j < NP; j++)
+                for (int i = 0; i < NP; i++) {
+                    const double xi[3] = {x[i], x[j], x[k]};
+                    double J[3][3];
+                    jacobian3(c, xi, J);
+                    // adj(J) (CalcAdjugate): row r is the contravariant metric Ja^r
+                    double a[3][3];
+                    a[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+                    a[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+                    a[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+                    a[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+                    a[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+                    a[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+                    a[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+                    a[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+                    a[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+                    const double det = J[0][0] * a[0][0] + J[0][1] * a[1][0] + J[0][2] * a[2][0];
+                    if (!(det > 0.0)) {
+                        err = "element " + std::to_string(e) + " has a non-positive Jacobian";
+                        return false;
+                    }
+                    const size_t q = (size_t)e * K + (k * NP + j) * NP + i;
+                    for (int r = 0; r < 3; r++)
+                        for (int d = 0; d < 3; d++) g.met[(3 * r + d) * NK + q] = a[r][d];
+                    g.jac[q] = det;
+                }
+        // GetElementSize(e, 1): the smallest singular value of J at the centre
+        const double xc[3] = {0.5, 0.5, 0.5};
+        double J[3][3], A[3][3];
+        jacobian3(c, xc, J);
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                A[a][b] = 0.0;
+                for (int k = 0; k < 3; k++) A[a][b] += J[k][a] * J[k][b];
+            }
+        g.hmin = std::min(g.hmin, std::sqrt(std::max(smallest_eig_sym3(A), 0.0)));
+    }
+    // faces: element faces keyed by their four vertices
+    std::map<Key4, std::pair<int, int>> first;
+    std::vector<int> nbr_e((size_t)NE * 6, -1), nbr_f((size_t)NE * 6, -1);
+    auto vtx = [&](int e, int k) { return m->elem[(size_t)e * 8 + k]; };
+    for (int e = 0; e < NE; e++)
+        for (int f = 0; f < 6; f++) {
+            const int *fc = HEX_FACE[f];
+            const Key4 kk = key4(vtx(e, fc[0]), vtx(e, fc[1]), vtx(e, fc[2]), vtx(e, fc[3]));
+            if (kk.v[0] == kk.v[1] || kk.v[1] == kk.v[2] || kk.v[2] == kk.v[3]) {
+                err = "degenerate element face";
+                return false;
+            }
+            auto it = first.find(kk);
+            if (it == first.end()) {
+                first.emplace(kk, std::make_pair(e, f));
+            } else {
+                const int e2 = it->second.first, f2 = it->second.second;
+                if (nbr_e[(size_t)e2 * 6 + f2] >= 0) { err = "a face is shared by more than two elements"; return false; }
+                nbr_e[(size_t)e2 * 6 + f2] = e; nbr_f[(size_t)e2 * 6 + f2] = f;
+                nbr_e[(size_t)e * 6 + f] = e2; nbr_f[(size_t)e * 6 + f] = f2;
+            }
+        }
+    std::map<Key4, int> battr;
+    for (int i = 0; i < m->nbdr; i++) {
+        const int *b = &m->bdr[4 * (size_t)i];
+        battr[key4(b[0], b[1], b[2], b[3])] = m->bdr_attr[i];
+    }
+    g.fface.assign((size_t)NE * 6, 0);
+    g.fnb.assign((size_t)NE * 6 * NFP, -1);
+    g.fnf.assign((size_t)NE * 6, -1);
+    g.fnor.assign((size_t)NE * 6 * NFP * 3, 0.0);
+    for (int e = 0; e < NE; e++)
+        for (int f = 0; f < 6; f++) {
+            const size_t ef = (size_t)e * 6 + f;
+            const int e2 = nbr_e[ef];
+            int e1 = e, f1 = f, code = 0;
+            if (e2 >= 0) {
+                const int f2 = nbr_f[ef];
+                if (e2 < e) { e1 = e2; f1 = f2; }
+                else code = FACE_OWN;
+                // where this element's face corners (0,0), (1,0), (0,1) sit
+                // on the neighbour's face
+                int ca[3], cb[3];
+                for (int t = 0; t < 3; t++) {
+                    const int gv = vtx(e, HEX_FACE[f][t]);
+                    int hit = -1;
+                    for (int u = 0; u < 4; u++)
+                        if (vtx(e2, HEX_FACE[f2][u]) == gv) hit = u;
+                    if (hit < 0) { err = "face vertices do not match"; return false; }
+                    ca[t] = hit & 1;
+                    cb[t] = hit >> 1;
+                }
+                g.fnf[ef] = 2 * (e2 * 6 + f2);
+                const int r2 = f2 >> 1, s2 = f2 & 1;
+                for (int l = 0; l < NFP; l++) {
+                    const int a = l % NP, b = l / NP;
+                    const int a2 = N * ca[0] + (ca[1] - ca[0]) * a + (ca[2] - ca[0]) * b;
+                    const int b2 = N * cb[0] + (cb[1] - cb[0]) * a + (cb[2] - cb[0]) * b;
+                    g.fnb[ef * NFP + l] = e2 * K + line_node3(NP, r2, a2 + NP * b2, s2 ? N : 0);
+                }
+            } else {
+                const int *fc = HEX_FACE[f];
+                auto it = battr.find(key4(vtx(e, fc[0]), vtx(e, fc[1]), vtx(e, fc[2]), vtx(e, fc[3])));
+                if (it == battr.end()) {
+                    err = "face of element " + std::to_string(e) + " has no neighbour and no boundary quadrilateral";
+                    return false;
+                }
+                const int attr = it->second;
+                if (attr < 1 || attr > m->nattr || !m->attr_bc) {
+                    err = "boundary attribute " + std::to_string(attr) + " has no boundary condition";
+                    return false;
+                }
+                const int t = m->attr_bc[attr - 1];
+                if (t < PRANDTL_BC_SLIP || t > PRANDTL_BC_NO_SLIP_ADIABATIC) {
+                    err = "boundary attribute " + std::to_string(attr) + " needs a non-periodic boundary type";
+                    return false;
+                }
+                code = FACE_OWN | (t << 1) | ((attr - 1) << 4);
+            }
+            g.fface[ef] = code;
+        }
+    // normals: CalcOrtho of the face Jacobian, the first element's adjugate
+    // row at the face point with the sign of its side; the second element's
+    // points take the first element's value at the matching point
+    for (int e = 0; e < NE; e++)
+        for (int f = 0; f < 6; f++) {
+            const size_t ef = (size_t)e * 6 + f;
+            const bool own = g.fface[ef] & FACE_OWN;
+            for (int l = 0; l < NFP; l++) {
+                int e1 = e, f1 = f, n1 = line_node3(NP, f >> 1, l, (f & 1) ? N : 0);
+                if (!own) {
+                    const int gn = g.fnb[ef * NFP + l];
+                    e1 = gn / K;
+                    n1 = gn - e1 * K;
+                    f1 = (g.fnf[ef] >> 1) % 6;
+                }
+                const double sg = (f1 & 1) ? 1.0 : -1.0;
+                const int r1 = f1 >> 1;
+                for (int d = 0; d < 3; d++)
+                    g.fnor[(ef * NFP + l) * 3 + d] = sg * g.met[(3 * r1 + d) * NK + (size_t)e1 * K + n1];
+            }
+        }
+    // subcell interface normals n_{i+1/2} = Ja^r(line start) + sum_{l<=i} w_l sum_q D_lq Ja^r_q
+    g.subn.assign((size_t)NE * 3 * NFP * N * 3, 0.0);
+    for (int e = 0; e < NE; e++)
+        for (int r = 0; r < 3; r++)
+            for (int l = 0; l < NFP; l++) {
+                const int n0 = line_node3(NP, r, l, 0);
+                for (int i = 1; i < NP; i++) {
+                    double nor[3];
+                    for (int d = 0; d < 3; d++) nor[d] = g.met[(3 * r + d) * NK + (size_t)e * K + n0];
+                    for (int ll = 0; ll < i; ll++) {
+                        double tmp[3] = {0.0, 0.0, 0.0};
+                        for (int q = 0; q < NP; q++) {
+                            const int nq = line_node3(NP, r, l, q);
+                            for (int d = 0; d < 3; d++) tmp[d] += g.met[(3 * r + d) * NK + (size_t)e * K + nq] * D[ll * NP + q];
+                        }
+                        for (int d = 0; d < 3; d++) { tmp[d] *= w[ll]; nor[d] += tmp[d]; }
+                    }
+                    for (int d = 0; d < 3; d++)
+                        g.subn[((((size_t)e * 3 + r) * NFP + l) * N + (i - 1)) * 3 + d] = nor[d];
+                }
+            }
+    return true;
+}
+
+} // namespace quad
+} // namespace prandtl

@@ -19,7 +19,9 @@ inline int face_type(int code) { return (code >> 1) & 7; }
 inline int face_attr(int code) { return code >> 4; }
 
 // Local faces f = 2 r + side: r = 0 the xi = 0 / xi = 1 faces, r = 1 the
-// eta = 0 / eta = 1 faces; face point l runs along the other coordinate.
+// eta = 0 / eta = 1 faces (and r = 2 the zeta faces in 3-D); face point l
+// runs along the other coordinate (3-D: l = a + NP b over the other two in
+// increasing order).
 struct Geometry {
     int NE = 0, NP = 0, K = 0, NFP = 0;
     std::vector<double> met;    // [4][NE*K]: adjugate row r, component d at 2r+d
@@ -47,6 +49,12 @@ struct Geometry {
 bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
            const std::vector<double> &w, const std::vector<double> &D, bool axisymmetric, Geometry &g,
            std::string &err);
+
+// The same tables for a hexahedral mesh (3-D layouts: met [9][NE*K], six
+// faces per element, normals with 3 components).  fnf holds 2 (e2*6 + f2);
+// the neighbour's face point follows from fnb's node.
+bool build_hex(const prandtl_hex_mesh *m, int order, const std::vector<double> &x,
+               const std::vector<double> &w, const std::vector<double> &D, Geometry &g, std::string &err);
 
 } // namespace quad
 } // namespace prandtl

@@ -181,6 +181,54 @@ class QuadMesh:
         return nattr, attr_bc, attr_state
 
 
+class HexMesh:
+    """A 3-D hexahedral mesh: vert [nv,3], elem [ne,8] (MFEM's vertex order),
+    optional elem_coords [ne,8,3] (periodic meshes), bdr [nb,4] boundary
+    quadrilaterals with bdr_attr [nb], names {name: attribute}."""
+
+    def __init__(self, vert, elem, bdr=None, bdr_attr=None, elem_coords=None, names=None):
+        self.vert = np.ascontiguousarray(vert, dtype=np.float64).reshape(-1, 3)
+        self.elem = np.ascontiguousarray(elem, dtype=np.int32).reshape(-1, 8)
+        self.elem_coords = None if elem_coords is None else \
+            np.ascontiguousarray(elem_coords, dtype=np.float64).reshape(-1, 8, 3)
+        self.bdr = np.zeros((0, 4), np.int32) if bdr is None else np.ascontiguousarray(bdr, dtype=np.int32).reshape(-1, 4)
+        self.bdr_attr = np.zeros(0, np.int32) if bdr_attr is None else np.ascontiguousarray(bdr_attr, dtype=np.int32)
+        self.names = dict(names or {})
+
+    @property
+    def nelem(self):
+        return len(self.elem)
+
+    def corners(self):
+        """[ne,8,3] corner coordinates."""
+        return self.elem_coords if self.elem_coords is not None else self.vert[self.elem]
+
+    def node_coords(self, order):
+        """Node coordinates X[3][NE][K] of the trilinear element maps."""
+        x, _ = lgl_nodes(order)
+        Np = order + 1
+        i = np.arange(Np ** 3)
+        xi, eta, zeta = x[i % Np], x[(i // Np) % Np], x[i // (Np * Np)]
+        cv = ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1))
+        N = np.stack([(xi if a else 1 - xi) * (eta if b else 1 - eta) * (zeta if c else 1 - zeta) for a, b, c in cv])
+        return np.einsum("ekd,kn->den", self.corners(), N)
+
+    def attribute_tables(self, bc):
+        """bc: {name or attribute: type or (type, state)} -> (nattr, attr_bc, attr_state [nattr][5])."""
+        nattr = int(self.bdr_attr.max()) if len(self.bdr_attr) else 0
+        for key in bc or {}:
+            nattr = max(nattr, self.names[key] if isinstance(key, str) else int(key))
+        attr_bc = np.zeros(max(nattr, 1), np.int32)
+        attr_state = np.zeros((max(nattr, 1), 5))
+        for key, val in (bc or {}).items():
+            a = self.names[key] if isinstance(key, str) else int(key)
+            kind, st = (val, None) if isinstance(val, (str, int)) else (val[0], val[1])
+            attr_bc[a - 1] = _bc_code(kind)
+            if st is not None:
+                attr_state[a - 1] = np.asarray(st, float)[:5]
+        return nattr, attr_bc, attr_state
+
+
 class DGSEMOperator:
     """One GPU's DGSEM semi-discrete operator on a Cartesian box, periodic or
     with boundary conditions on its sides (bc, bc_state: see _descs), or on a
@@ -232,6 +280,38 @@ class DGSEMOperator:
         op._h = h
         op._owner = True
         op._describe(2, order, (len(mesh.elem), 1), (0.0, 0.0), (1.0, 1.0))
+        op.mesh = mesh
+        return op
+
+    @classmethod
+    def hex(cls, mesh, order, bc=None, gamma=1.4, blending=False, alpha_max=0.5, alpha_min=0.001,
+            quirks=True, precision="fp64", parabolic=False, mu=0.0, Pr=0.72, sutherland=False,
+            R_gas=287.05):
+        """The operator on HexMesh `mesh` (prandtl_create_hex); bc as quad()
+        with 5-value states (no-slip walls: (V0, V1, V2, qn, 0))."""
+        _, p = _descs(3, order, (1, 1, 1), (0.0,) * 3, (1.0,) * 3, gamma=gamma, blending=blending,
+                      alpha_max=alpha_max, alpha_min=alpha_min, quirks=quirks, precision=precision,
+                      parabolic=parabolic, mu=mu, Pr=Pr, sutherland=sutherland, R_gas=R_gas)
+        nattr, attr_bc, attr_state = mesh.attribute_tables(bc)
+        dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        q = _lib.HexMesh()
+        q.nvert = len(mesh.vert)
+        q.vert = mesh.vert.ctypes.data_as(dp)
+        q.nelem = len(mesh.elem)
+        q.elem = mesh.elem.ctypes.data_as(ip)
+        q.elem_coords = mesh.elem_coords.ctypes.data_as(dp) if mesh.elem_coords is not None else dp()
+        q.nbdr = len(mesh.bdr_attr)
+        q.bdr = mesh.bdr.ctypes.data_as(ip)
+        q.bdr_attr = mesh.bdr_attr.ctypes.data_as(ip)
+        q.nattr = nattr
+        q.attr_bc = attr_bc.ctypes.data_as(ip)
+        q.attr_state = np.ascontiguousarray(attr_state).ctypes.data_as(dp)
+        h = ctypes.c_void_p()
+        check(lib().prandtl_create_hex(ctypes.byref(q), int(order), ctypes.byref(p), ctypes.byref(h)), None)
+        op = cls.__new__(cls)
+        op._h = h
+        op._owner = True
+        op._describe(3, order, (len(mesh.elem), 1, 1), (0.0,) * 3, (1.0,) * 3)
         op.mesh = mesh
         return op
 
