@@ -180,8 +180,9 @@ int prandtl_create_hex(const prandtl_hex_mesh *mesh, int order, const prandtl_pa
 
 /* Mesh files: mfem::Mesh's readers for "MFEM mesh v1.0" (vertices, or L2_T1
  * first-order nodes for periodic meshes) and Gmsh 2.2 ASCII ($PhysicalNames,
- * $Nodes, $Elements with 4-node quadrilaterals and 2-node boundary lines;
- * attributes are the physical tags).  Elements keep file order.  Elements
+ * $Nodes, $Elements with 4-node quadrilaterals and 2-node boundary lines, or
+ * 8-node hexahedra and 4-node boundary quadrilaterals; attributes are the
+ * physical tags).  Elements keep file order.  Elements
  * with clockwise vertices are reoriented.  _refine splits every element into
  * four (children of element i at 4i..4i+3, corners in the parent's frame
  * (0,0), (1,0), (1,1), (0,1)), as ser_ref_levels does with
@@ -193,6 +194,12 @@ typedef struct prandtl_mesh_file prandtl_mesh_file;
 int prandtl_mesh_read(const char *path, prandtl_mesh_file **out);
 int prandtl_mesh_refine(prandtl_mesh_file *m);
 int prandtl_mesh_view(const prandtl_mesh_file *m, prandtl_quad_mesh *view);
+/* 2 for a quadrilateral mesh, 3 for a hexahedral one (Gmsh 8-node
+ * hexahedra with 4-node boundary quadrilaterals and 2-D physical names, or
+ * MFEM geometry 5 with boundary geometry 3); _view_hex fills a
+ * prandtl_hex_mesh (no refinement and no periodic node sets in 3-D) */
+int prandtl_mesh_dim(const prandtl_mesh_file *m);
+int prandtl_mesh_view_hex(const prandtl_mesh_file *m, prandtl_hex_mesh *view);
 const char *prandtl_mesh_name(const prandtl_mesh_file *m, int i, int *attr);
 void prandtl_mesh_free(prandtl_mesh_file *m);
 const char *prandtl_mesh_last_error(void);

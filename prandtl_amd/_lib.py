@@ -32,7 +32,7 @@ EXPORTED = (
     "prandtl_checkpoint_write", "prandtl_checkpoint_read", "prandtl_checkpoint_save", "prandtl_checkpoint_load",
     "prandtl_create_quad", "prandtl_hmin", "prandtl_mesh_read", "prandtl_mesh_refine", "prandtl_mesh_view",
     "prandtl_mesh_name", "prandtl_mesh_free", "prandtl_mesh_last_error", "prandtl_axis_recover",
-    "prandtl_create_hex",
+    "prandtl_create_hex", "prandtl_mesh_dim", "prandtl_mesh_view_hex",
 )
 
 
@@ -133,6 +133,9 @@ def lib():
     L.prandtl_create_hex.argtypes = [ctypes.POINTER(HexMesh), ctypes.c_int, ctypes.POINTER(Params),
                                      ctypes.POINTER(vp)]
     L.prandtl_hmin.argtypes = [vp, dp]
+    L.prandtl_mesh_dim.argtypes = [vp]
+    L.prandtl_mesh_dim.restype = ctypes.c_int
+    L.prandtl_mesh_view_hex.argtypes = [vp, ctypes.POINTER(HexMesh)]
     L.prandtl_mesh_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
     L.prandtl_mesh_refine.argtypes = [vp]
     L.prandtl_mesh_view.argtypes = [vp, ctypes.POINTER(QuadMesh)]

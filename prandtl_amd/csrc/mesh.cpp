@@ -15,10 +15,11 @@
 #include <utility>
 
 struct prandtl_mesh_file {
-    std::vector<double> vert;      // [nv][2]
-    std::vector<int> elem;         // [ne][4]
-    std::vector<double> ecoords;   // empty, or [ne][4][2]
-    std::vector<int> bdr;          // [nb][2]
+    int dim = 2;                   // 2: quadrilaterals, 3: hexahedra
+    std::vector<double> vert;      // [nv][dim]
+    std::vector<int> elem;         // [ne][4] (2-D) or [ne][8] (3-D)
+    std::vector<double> ecoords;   // empty, or [ne][4][2] (2-D only)
+    std::vector<int> bdr;          // [nb][2] segments (2-D) or [nb][4] quadrilaterals (3-D)
     std::vector<int> battr;        // [nb]
     std::vector<std::pair<std::string, int>> names;   // boundary physical groups
 };
@@ -99,6 +100,9 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
     }
     std::unordered_map<long long, int> node_of;
     bool have_nodes = false, have_elems = false;
+    std::vector<double> v3;                        // node coordinates, 3 per node
+    std::vector<int> quads, qattr, lines, lattr, hexes;
+    std::vector<std::pair<std::string, int>> names[4];   // physical groups by dimension
     while (in) {
         line = next_line(in);
         if (line.empty()) break;
@@ -112,20 +116,21 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
                 std::getline(ls, name);
                 const size_t a = name.find('"'), b = name.rfind('"');
                 if (a != std::string::npos && b > a) name = name.substr(a + 1, b - a - 1);
-                if (dim == 1) m.names.emplace_back(name, tag);
+                if (dim >= 0 && dim <= 3) names[dim].emplace_back(name, tag);
             }
         } else if (line == "$Nodes") {
             const int n = std::atoi(next_line(in).c_str());
             if (n <= 0) return mesh_fail("empty $Nodes");
-            m.vert.resize(2 * (size_t)n);
+            v3.resize(3 * (size_t)n);
             for (int i = 0; i < n; i++) {
                 std::istringstream ls(next_line(in));
                 long long id = 0;
                 double x = 0, y = 0, z = 0;
                 if (!(ls >> id >> x >> y >> z)) return mesh_fail("bad $Nodes record");
                 node_of[id] = i;
-                m.vert[2 * (size_t)i] = x;
-                m.vert[2 * (size_t)i + 1] = y;
+                v3[3 * (size_t)i] = x;
+                v3[3 * (size_t)i + 1] = y;
+                v3[3 * (size_t)i + 2] = z;
             }
             have_nodes = true;
         } else if (line == "$Elements") {
@@ -138,9 +143,11 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
                 if (!(ls >> id >> type >> ntags)) return mesh_fail("bad $Elements record");
                 std::vector<long long> tags(ntags);
                 for (int t = 0; t < ntags; t++) ls >> tags[t];
-                const int nn = type == 1 ? 2 : type == 3 ? 4 : type == 15 ? 1 : -1;
-                if (nn < 0) return mesh_fail("element type " + std::to_string(type) + ": only quadrilaterals and boundary lines are read");
-                int v[4];
+                const int nn = type == 1 ? 2 : type == 3 ? 4 : type == 5 ? 8 : type == 15 ? 1 : -1;
+                if (nn < 0)
+                    return mesh_fail("element type " + std::to_string(type) +
+                                     ": only quadrilaterals, hexahedra and their boundary lines / quadrilaterals are read");
+                int v[8];
                 for (int k = 0; k < nn; k++) {
                     long long nid = 0;
                     if (!(ls >> nid)) return mesh_fail("short $Elements record");
@@ -149,12 +156,15 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
                     v[k] = it->second;
                 }
                 const int attr = ntags > 0 ? (int)tags[0] : 1;
-                if (type == 3)
-                    for (int k = 0; k < 4; k++) m.elem.push_back(v[k]);
-                else if (type == 1) {
-                    m.bdr.push_back(v[0]);
-                    m.bdr.push_back(v[1]);
-                    m.battr.push_back(attr);
+                if (type == 5)   // Gmsh's 8-node hexahedron has MFEM's vertex order
+                    for (int k = 0; k < 8; k++) hexes.push_back(v[k]);
+                else if (type == 3) {
+                    for (int k = 0; k < 4; k++) quads.push_back(v[k]);
+                    qattr.push_back(attr);
+                } else if (type == 1) {
+                    lines.push_back(v[0]);
+                    lines.push_back(v[1]);
+                    lattr.push_back(attr);
                 }
             }
             have_elems = true;
@@ -164,7 +174,29 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
             while (in && next_line(in) != end) {}
         }
     }
-    if (!have_elems || m.elem.empty()) return mesh_fail("no quadrilateral elements");
+    if (!have_elems) return mesh_fail("no $Elements section");
+    if (!hexes.empty()) {
+        // 3-D: hexahedra, boundary quadrilaterals, 2-D physical names
+        m.dim = 3;
+        m.vert.swap(v3);
+        m.elem.swap(hexes);
+        m.bdr.swap(quads);
+        m.battr.swap(qattr);
+        m.names = names[2];
+        return PRANDTL_OK;
+    }
+    if (quads.empty()) return mesh_fail("no quadrilateral or hexahedral elements");
+    m.dim = 2;
+    const size_t nv = v3.size() / 3;
+    m.vert.resize(2 * nv);
+    for (size_t i = 0; i < nv; i++) {
+        m.vert[2 * i] = v3[3 * i];
+        m.vert[2 * i + 1] = v3[3 * i + 1];
+    }
+    m.elem.swap(quads);
+    m.bdr.swap(lines);
+    m.battr.swap(lattr);
+    m.names = names[1];
     return PRANDTL_OK;
 }
 
@@ -177,32 +209,40 @@ int read_mfem(std::istream &in, prandtl_mesh_file &m)
         if (line.empty()) break;
         if (line == "dimension") {
             dim = std::atoi(next_line(in).c_str());
-            if (dim != 2) return mesh_fail("only 2-D MFEM meshes are read");
+            if (dim != 2 && dim != 3) return mesh_fail("only 2-D and 3-D MFEM meshes are read");
+            m.dim = dim;
         } else if (line == "elements") {
+            // geometry 3: quadrilateral (2-D), 5: hexahedron (3-D, MFEM's vertex order)
             const int n = std::atoi(next_line(in).c_str());
+            const int nvx = dim == 3 ? 8 : 4, want = dim == 3 ? 5 : 3;
             for (int i = 0; i < n; i++) {
                 std::istringstream ls(next_line(in));
-                int attr = 0, geom = 0, v[4];
+                int attr = 0, geom = 0, v[8];
                 ls >> attr >> geom;
-                if (geom != 3) return mesh_fail("only quadrilateral elements (geometry 3) are read");
-                for (int k = 0; k < 4; k++)
+                if (geom != want)
+                    return mesh_fail(dim == 3 ? "only hexahedral elements (geometry 5) are read in 3-D"
+                                              : "only quadrilateral elements (geometry 3) are read");
+                for (int k = 0; k < nvx; k++)
                     if (!(ls >> v[k])) return mesh_fail("short element record");
-                for (int k = 0; k < 4; k++) m.elem.push_back(v[k]);
+                for (int k = 0; k < nvx; k++) m.elem.push_back(v[k]);
             }
         } else if (line == "boundary") {
             const int n = std::atoi(next_line(in).c_str());
+            const int nvx = dim == 3 ? 4 : 2, want = dim == 3 ? 3 : 1;
             for (int i = 0; i < n; i++) {
                 std::istringstream ls(next_line(in));
-                int attr = 0, geom = 0, a = 0, b = 0;
-                if (!(ls >> attr >> geom >> a >> b) || geom != 1) return mesh_fail("bad boundary record");
-                m.bdr.push_back(a);
-                m.bdr.push_back(b);
+                int attr = 0, geom = 0, v[4];
+                if (!(ls >> attr >> geom) || geom != want) return mesh_fail("bad boundary record");
+                for (int k = 0; k < nvx; k++)
+                    if (!(ls >> v[k])) return mesh_fail("bad boundary record");
+                for (int k = 0; k < nvx; k++) m.bdr.push_back(v[k]);
                 m.battr.push_back(attr);
             }
         } else if (line == "vertices") {
             nv = std::atoi(next_line(in).c_str());
             const std::string nxt = next_line(in);
             if (nxt == "nodes") {
+                if (dim == 3) return mesh_fail("3-D mesh nodes (periodic hexahedral meshes) are not read");
                 // first-order L2 nodes with Gauss-Lobatto points: per element
                 // the corners in lexicographic order (0,0), (1,0), (0,1), (1,1)
                 std::string fes = next_line(in), fec = next_line(in), vd = next_line(in), ord = next_line(in);
@@ -235,11 +275,12 @@ int read_mfem(std::istream &in, prandtl_mesh_file &m)
                     }
             } else {
                 const int vdim = std::atoi(nxt.c_str());
-                if (vdim != 2) return mesh_fail("vertex dimension must be 2");
-                m.vert.resize(2 * (size_t)nv);
+                if (vdim != dim) return mesh_fail("vertex dimension must equal the mesh dimension");
+                m.vert.resize((size_t)vdim * nv);
                 for (int i = 0; i < nv; i++) {
                     std::istringstream ls(next_line(in));
-                    if (!(ls >> m.vert[2 * (size_t)i] >> m.vert[2 * (size_t)i + 1])) return mesh_fail("bad vertex record");
+                    for (int d = 0; d < vdim; d++)
+                        if (!(ls >> m.vert[(size_t)vdim * i + d])) return mesh_fail("bad vertex record");
                 }
             }
         } else if (line == "mfem_mesh_end") {
@@ -550,14 +591,38 @@ int prandtl_mesh_read(const char *path, prandtl_mesh_file **out)
         delete m;
         return rc;
     }
-    orient(*m);
+    if (m->dim == 2) orient(*m);
     *out = m;
+    return PRANDTL_OK;
+}
+
+int prandtl_mesh_dim(const prandtl_mesh_file *m) { return m ? m->dim : 0; }
+
+int prandtl_mesh_view_hex(const prandtl_mesh_file *m, prandtl_hex_mesh *v)
+{
+    if (!m || !v) return PRANDTL_ERR_ARG;
+    if (m->dim != 3) {
+        g_mesh_err = "a quadrilateral mesh: use prandtl_mesh_view";
+        return PRANDTL_ERR_ARG;
+    }
+    v->nvert = (int)(m->vert.size() / 3);
+    v->vert = m->vert.data();
+    v->nelem = (int)(m->elem.size() / 8);
+    v->elem = m->elem.data();
+    v->elem_coords = nullptr;
+    v->nbdr = (int)m->battr.size();
+    v->bdr = m->bdr.data();
+    v->bdr_attr = m->battr.data();
     return PRANDTL_OK;
 }
 
 int prandtl_mesh_refine(prandtl_mesh_file *m)
 {
     if (!m) return PRANDTL_ERR_ARG;
+    if (m->dim != 2) {
+        g_mesh_err = "uniform refinement is built for quadrilateral meshes";
+        return PRANDTL_ERR_UNSUPPORTED;
+    }
     const int ne = (int)(m->elem.size() / 4);
     int nv = (int)(m->vert.size() / 2);
     std::unordered_map<long long, int> mid;
@@ -628,6 +693,10 @@ int prandtl_mesh_refine(prandtl_mesh_file *m)
 int prandtl_mesh_view(const prandtl_mesh_file *m, prandtl_quad_mesh *v)
 {
     if (!m || !v) return PRANDTL_ERR_ARG;
+    if (m->dim != 2) {
+        g_mesh_err = "a hexahedral mesh: use prandtl_mesh_view_hex";
+        return PRANDTL_ERR_ARG;
+    }
     v->nvert = (int)(m->vert.size() / 2);
     v->vert = m->vert.data();
     v->nelem = (int)(m->elem.size() / 4);

@@ -129,6 +129,8 @@ class QuadMesh:
         if rc != _lib.PRANDTL_OK:
             raise _lib.PrandtlError(rc, lib().prandtl_mesh_last_error().decode())
         try:
+            if lib().prandtl_mesh_dim(h) != 2:
+                raise ValueError("%s is not a quadrilateral mesh (use HexMesh.read)" % path)
             for _ in range(refine):
                 check(lib().prandtl_mesh_refine(h))
             v = _lib.QuadMesh()
@@ -198,6 +200,34 @@ class HexMesh:
     @property
     def nelem(self):
         return len(self.elem)
+
+    @classmethod
+    def read(cls, path):
+        """Read a Gmsh 2.2 or MFEM v1.0 hexahedral mesh (prandtl_mesh_read)."""
+        h = ctypes.c_void_p()
+        rc = lib().prandtl_mesh_read(str(path).encode(), ctypes.byref(h))
+        if rc != _lib.PRANDTL_OK:
+            raise _lib.PrandtlError(rc, lib().prandtl_mesh_last_error().decode())
+        try:
+            if lib().prandtl_mesh_dim(h) != 3:
+                raise ValueError("%s is not a hexahedral mesh (use QuadMesh.read)" % path)
+            v = _lib.HexMesh()
+            check(lib().prandtl_mesh_view_hex(h, ctypes.byref(v)))
+            vert = np.ctypeslib.as_array(v.vert, shape=(v.nvert * 3,)).copy()
+            elem = np.ctypeslib.as_array(v.elem, shape=(v.nelem * 8,)).copy()
+            bdr = np.ctypeslib.as_array(v.bdr, shape=(v.nbdr * 4,)).copy() if v.nbdr else None
+            battr = np.ctypeslib.as_array(v.bdr_attr, shape=(v.nbdr,)).copy() if v.nbdr else None
+            names, i = {}, 0
+            while True:
+                a = ctypes.c_int()
+                nm = lib().prandtl_mesh_name(h, i, ctypes.byref(a))
+                if nm is None:
+                    break
+                names[nm.decode()] = a.value
+                i += 1
+        finally:
+            lib().prandtl_mesh_free(h)
+        return cls(vert, elem, bdr, battr, None, names)
 
     def corners(self):
         """[ne,8,3] corner coordinates."""

@@ -80,7 +80,7 @@ def test_mfem_vertex_mesh_and_orientation(tmp_path):
 @pytest.mark.parametrize("text,needle", [
     ("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n3\n1 0 0 0\n2 1 0 0\n3 0 1 0\n$EndNodes\n"
      "$Elements\n1\n1 2 2 1 1 1 2 3\n$EndElements\n", "element type 2"),
-    ("MFEM mesh v1.0\ndimension\n3\n", "2-D"),
+    ("MFEM mesh v1.0\ndimension\n1\n", "2-D and 3-D"),
     ("something else\n", "unknown mesh format"),
 ])
 def test_reader_errors(tmp_path, text, needle):
@@ -123,3 +123,57 @@ def test_create_quad_rejects_open_meshes():
     h = ctypes.c_void_p()
     rc = lib().prandtl_create_quad(ctypes.byref(q), 3, ctypes.byref(prm), ctypes.byref(h))
     assert rc == _lib.PRANDTL_ERR_ARG and not h.value
+
+
+def _hex_files(tmp_path):
+    """A distorted 3 x 2 x 2 hexahedral box written as Gmsh 2.2 and MFEM v1.0
+    files (attributes 1..6 per side, named in the Gmsh file)."""
+    import quad_meshes as Q
+    m = Q.hex_grid(3, 2, 2, (0.0, 0.0, 0.0), (1.5, 1.0, 0.8), periodic=(False, False, False), perturb=0.2, seed=3)
+    names = ["xlo", "xhi", "ylo", "yhi", "zlo", "zhi"]
+    g = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", "7"]
+    g += ['2 %d "%s"' % (i + 1, n) for i, n in enumerate(names)] + ['3 7 "fluid"', "$EndPhysicalNames"]
+    g += ["$Nodes", str(len(m["vert"]))] + ["%d %.17g %.17g %.17g" % (i + 1, *v) for i, v in enumerate(m["vert"])]
+    g += ["$EndNodes", "$Elements", str(len(m["bdr"]) + len(m["elem"]))]
+    k = 1
+    for q, a in zip(m["bdr"], m["bdr_attr"]):
+        g.append("%d 3 2 %d %d %s" % (k, a, a, " ".join(str(v + 1) for v in q)))
+        k += 1
+    for e in m["elem"]:
+        g.append("%d 5 2 7 7 %s" % (k, " ".join(str(v + 1) for v in e)))
+        k += 1
+    g += ["$EndElements"]
+    (tmp_path / "box.msh").write_text("\n".join(g) + "\n")
+    f = ["MFEM mesh v1.0", "", "dimension", "3", "", "elements", str(len(m["elem"]))]
+    f += ["1 5 " + " ".join(str(v) for v in e) for e in m["elem"]]
+    f += ["", "boundary", str(len(m["bdr"]))]
+    f += ["%d 3 %s" % (a, " ".join(str(v) for v in q)) for q, a in zip(m["bdr"], m["bdr_attr"])]
+    f += ["", "vertices", str(len(m["vert"])), "3"] + ["%.17g %.17g %.17g" % tuple(v) for v in m["vert"]]
+    (tmp_path / "box.mesh").write_text("\n".join(f) + "\n")
+    return m, names
+
+
+@pytest.mark.parametrize("fname", ["box.msh", "box.mesh"])
+def test_hexahedral_mesh_files(tmp_path, fname):
+    """Gmsh 8-node hexahedra with 4-node boundary quadrilaterals and MFEM
+    geometry-5 meshes read back to the same topology and coordinates; the
+    oracle builds the same operator from either."""
+    import oracle_lib as O
+    from prandtl_amd import cases
+    from prandtl_amd.operator import HexMesh
+    m, names = _hex_files(tmp_path)
+    h = HexMesh.read(tmp_path / fname)
+    assert np.array_equal(h.elem, m["elem"]) and np.array_equal(h.bdr, m["bdr"])
+    assert np.array_equal(h.bdr_attr, m["bdr_attr"]) and np.abs(h.vert - m["vert"]).max() < 1e-15
+    if fname.endswith(".msh"):
+        assert h.names == {n: i + 1 for i, n in enumerate(names)}
+    kinds = ["supersonic-inflow", "supersonic-outflow", "slip", "slip", "symmetry", "slip"]
+    st = [np.array([1.2, 2.0, 0.1, 0.0, 4.5])] * 6
+    d = dict(vert=h.vert, elem=h.elem, bdr=h.bdr, bdr_attr=h.bdr_attr, attr_bc=kinds, attr_state=st)
+    a = O.Oracle.hex(d, 3, quirks=False)
+    b = O.Oracle.hex(dict(m, attr_bc=kinds, attr_state=st), 3, quirks=False)
+    assert np.abs(h.node_coords(3) - a.coords()).max() < 1e-14
+    u = cases.random_smooth_state(a.coords(), 3, 1, L=[1.5, 1.0, 0.8])
+    assert np.array_equal(a.Mult(u), b.Mult(u))
+    with pytest.raises(ValueError):
+        QuadMesh.read(tmp_path / fname)
