@@ -45,6 +45,10 @@ WORKLOADS = {
                      bc={"left": ("supersonic-inflow", (1.4, 4.2, 0.0, 1.0 / 0.4 + 0.5 * 1.4 * 9.0)),
                          "right": "supersonic-outflow", "top": "slip", "bottom": "slip"}),
 }
+# the Taylor-Green workload on a hexahedral mesh of the same box whose
+# vertices are moved by a smooth periodic warp (10% of h): the general
+# (curvilinear) geometry kernels at the headline size
+WORKLOADS["tgv3d_n3_hex"] = dict(WORKLOADS["tgv3d_n3"], hex_warp=0.1)
 # the reference's axisymmetric cases on their own meshes (order 1, blending,
 # fixed dt; the flow is developed by spin-up steps of that dt)
 for _name, _case in (("ramp2d_n1", "RAMP"), ("nagashima2d_n1", "NAGASHIMA")):
@@ -76,8 +80,20 @@ def axi_kwargs(cfg):
     return dict(axisymmetric=True, axis_rho_floor=cfg["axis_rho_floor"]) if cfg.get("axisymmetric") else {}
 
 
+def hex_mesh(cfg):
+    from prandtl_amd import cases
+    return cases.hex_box_mesh(cfg["nelem"], cfg["lo"], cfg["hi"], warp=cfg["hex_warp"])
+
+
+def box_kwargs(cfg):
+    return {k: v for k, v in cfg.items() if k not in ("hex_warp", "dim", "nelem", "lo", "hi")}
+
+
 def make_operator(cfg, quirks, precision):
     from prandtl_amd import DGSEMOperator
+    if "hex_warp" in cfg:
+        return DGSEMOperator.hex(hex_mesh(cfg), cfg["order"], quirks=quirks, precision=precision,
+                                 **{k: v for k, v in box_kwargs(cfg).items() if k != "order"})
     if "mesh" in cfg:
         return DGSEMOperator.quad(quad_mesh(cfg), cfg["order"], bc=cfg["bc"], gamma=cfg["gamma"],
                                   blending=cfg.get("blending", False), alpha_max=cfg.get("alpha_max", 0.5),
@@ -89,6 +105,11 @@ def make_oracle(cfg, quirks, threads):
     """The oracle on the same workload (test infrastructure, CPU baseline and op counts)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    if "hex_warp" in cfg:
+        m = hex_mesh(cfg)
+        d = dict(vert=m.vert, elem=m.elem, elem_coords=m.elem_coords)
+        return O.Oracle.hex(d, cfg["order"], quirks=quirks, threads=threads,
+                            **{k: v for k, v in box_kwargs(cfg).items() if k != "order"})
     if "mesh" in cfg:
         m = quad_mesh(cfg)
         nattr = int(m.bdr_attr.max())
@@ -118,7 +139,10 @@ def initial_state(name, cfg):
         if cfg.get("axisymmetric"):
             return cases.axisymmetric_state(U, m.node_coords(cfg["order"])).reshape(4, -1)
         return np.repeat(U[:, None], m.nelem * K, axis=1)
-    X = cases.node_coords(cfg["dim"], cfg["order"], cfg["nelem"], cfg["lo"], cfg["hi"])
+    if "hex_warp" in cfg:
+        X = hex_mesh(cfg).node_coords(cfg["order"])
+    else:
+        X = cases.node_coords(cfg["dim"], cfg["order"], cfg["nelem"], cfg["lo"], cfg["hi"])
     if name.startswith("tgv"):
         return cases.taylor_green(X)
     if name.startswith("kh"):
@@ -463,6 +487,13 @@ def main():
         by_rhs += 8 * 5 + (4 * Np * 20 + 4 * 4) / K
         if cfg.get("blending"):
             by_rhs += 2 * Np * (Np - 1) * 2 * 8 / K
+    if "hex_warp" in cfg:
+        # 3-D general geometry: 9 adjugate entries and J per node, per face
+        # point the neighbour node (4 B) and normal (24 B), a code and a
+        # neighbour face per face
+        Np = cfg["order"] + 1
+        K = Np ** 3
+        by_rhs += 8 * 10 + (6 * Np * Np * 28 + 6 * 8) / K
         if cfg.get("axisymmetric"):
             # r and the axis code per node (U is recovered from r U in registers)
             by_rhs += 12
@@ -492,7 +523,8 @@ def main():
                    "kernel_ms": {"k_rhs": ms_rhs / max(n_rhs, 1), "k_visc": ms_visc / max(n_visc, 1)}},
         "roofline": roof,
     }
-    if args.precision == "fp64" and cfg.get("parabolic") and not quirks and not args.no_mixed:
+    if args.precision == "fp64" and cfg.get("parabolic") and not quirks and not args.no_mixed \
+            and "hex_warp" not in cfg:
         m = mixed_comparison(name, cfg, u_host, dt, args.steps, args.warmup, not args.no_autotune, dev)
         m["speedup_vs_fp64"] = m["value"] / value
         line["config"]["mixed"] = m

@@ -233,6 +233,37 @@ def axisymmetric_state(U, X):
     return r[None] * U
 
 
+def hex_box_mesh(n, lo, hi, warp=0.0):
+    """A periodic n[0] x n[1] x n[2] hexahedral mesh of the box [lo, hi]
+    (MFEM's vertex order, elements x fastest) whose vertices are moved by
+    the smooth periodic warp x_d += warp h_d sin(2 pi x_{d+1} / L_{d+1}), the
+    shared vertices of periodic faces identified; returns a HexMesh with
+    per-element corners."""
+    from .operator import HexMesh
+    n = tuple(int(v) for v in n)
+    L = [hi[d] - lo[d] for d in range(3)]
+    h = [L[d] / n[d] for d in range(3)]
+    corners = ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1))
+
+    def pos(i, j, k):
+        p = np.array([lo[0] + i * h[0], lo[1] + j * h[1], lo[2] + k * h[2]])
+        q = p.copy()
+        for d in range(3):
+            e = (d + 1) % 3
+            q[d] += warp * h[d] * np.sin(2 * np.pi * (p[e] - lo[e]) / L[e])
+        return q
+
+    elem, ec = [], []
+    for k in range(n[2]):
+        for j in range(n[1]):
+            for i in range(n[0]):
+                cs = [(i + a, j + b, k + c) for a, b, c in corners]
+                elem.append([((c2 % n[2]) * n[1] + (b2 % n[1])) * n[0] + (a2 % n[0]) for a2, b2, c2 in cs])
+                ec.append([pos(*c) for c in cs])
+    vert = np.zeros((n[0] * n[1] * n[2], 3))
+    return HexMesh(vert, np.array(elem), None, None, np.array(ec))
+
+
 def random_smooth_state(X, dim, seed, amp=0.2, gamma=1.4, rho0=1.0, p0=1.0, L=None):
     """Smooth positive periodic state: a few random Fourier modes on each primitive."""
     rng = np.random.default_rng(seed)
