@@ -182,7 +182,8 @@ int prandtl_create_hex(const prandtl_hex_mesh *mesh, int order, const prandtl_pa
  * first-order nodes for periodic meshes) and Gmsh 2.2 ASCII ($PhysicalNames,
  * $Nodes, $Elements with 4-node quadrilaterals and 2-node boundary lines, or
  * 8-node hexahedra and 4-node boundary quadrilaterals; attributes are the
- * physical tags).  Elements keep file order.  Elements
+ * physical tags; a $Periodic section identifies slave nodes with their
+ * masters and keeps per-element corner coordinates, as mfem::Mesh does).  Elements keep file order.  Elements
  * with clockwise vertices are reoriented.  _refine splits every element into
  * four (children of element i at 4i..4i+3, corners in the parent's frame
  * (0,0), (1,0), (1,1), (0,1)), as ser_ref_levels does with
@@ -197,7 +198,8 @@ int prandtl_mesh_view(const prandtl_mesh_file *m, prandtl_quad_mesh *view);
 /* 2 for a quadrilateral mesh, 3 for a hexahedral one (Gmsh 8-node
  * hexahedra with 4-node boundary quadrilaterals and 2-D physical names, or
  * MFEM geometry 5 with boundary geometry 3); _view_hex fills a
- * prandtl_hex_mesh (no refinement and no periodic node sets in 3-D) */
+ * prandtl_hex_mesh (hexahedral meshes are not refined; periodic ones are
+ * read from Gmsh's $Periodic section, not from MFEM node sets) */
 int prandtl_mesh_dim(const prandtl_mesh_file *m);
 int prandtl_mesh_view_hex(const prandtl_mesh_file *m, prandtl_hex_mesh *view);
 const char *prandtl_mesh_name(const prandtl_mesh_file *m, int i, int *attr);

@@ -18,7 +18,7 @@ struct prandtl_mesh_file {
     int dim = 2;                   // 2: quadrilaterals, 3: hexahedra
     std::vector<double> vert;      // [nv][dim]
     std::vector<int> elem;         // [ne][4] (2-D) or [ne][8] (3-D)
-    std::vector<double> ecoords;   // empty, or [ne][4][2] (2-D only)
+    std::vector<double> ecoords;   // empty, or [ne][4][2] (2-D) / [ne][8][3] (3-D): periodic meshes
     std::vector<int> bdr;          // [nb][2] segments (2-D) or [nb][4] quadrilaterals (3-D)
     std::vector<int> battr;        // [nb]
     std::vector<std::pair<std::string, int>> names;   // boundary physical groups
@@ -103,6 +103,7 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
     std::vector<double> v3;                        // node coordinates, 3 per node
     std::vector<int> quads, qattr, lines, lattr, hexes;
     std::vector<std::pair<std::string, int>> names[4];   // physical groups by dimension
+    std::vector<std::pair<long long, long long>> periodic;   // (slave, master) node ids
     while (in) {
         line = next_line(in);
         if (line.empty()) break;
@@ -168,6 +169,22 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
                 }
             }
             have_elems = true;
+        } else if (line == "$Periodic") {
+            // entity pairs, each with its node correspondences (an "Affine"
+            // line may precede the node count)
+            const int ne = std::atoi(next_line(in).c_str());
+            for (int i = 0; i < ne; i++) {
+                next_line(in);   // dimension slave-entity master-entity
+                std::string cnt = next_line(in);
+                if (cnt.compare(0, 6, "Affine") == 0) cnt = next_line(in);
+                const int nn = std::atoi(cnt.c_str());
+                for (int k = 0; k < nn; k++) {
+                    std::istringstream ls(next_line(in));
+                    long long a = 0, b = 0;
+                    if (!(ls >> a >> b)) return mesh_fail("bad $Periodic record");
+                    periodic.emplace_back(a, b);
+                }
+            }
         } else if (line[0] == '$' && line.compare(0, 4, "$End") != 0) {
             // skip an unknown section
             const std::string end = "$End" + line.substr(1);
@@ -175,6 +192,44 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
         }
     }
     if (!have_elems) return mesh_fail("no $Elements section");
+    // periodic meshes: each element keeps its corner coordinates, and slave
+    // nodes are identified with their masters (chains followed to the end),
+    // as mfem::Mesh does for Gmsh's $Periodic section
+    std::vector<double> ecoords;
+    if (!periodic.empty()) {
+        const int nn = (int)(v3.size() / 3);
+        std::vector<int> rep(nn);
+        for (int i = 0; i < nn; i++) rep[i] = i;
+        for (auto &pr : periodic) {
+            auto a = node_of.find(pr.first), b = node_of.find(pr.second);
+            if (a == node_of.end() || b == node_of.end()) return mesh_fail("$Periodic references an unknown node");
+            if (a->second != b->second) rep[a->second] = b->second;
+        }
+        auto root = [&](int i) {
+            int steps = 0;
+            while (rep[i] != i) {
+                i = rep[i];
+                if (++steps > nn) return -1;   // a cycle
+            }
+            return i;
+        };
+        std::vector<int> &el = hexes.empty() ? quads : hexes;
+        const int nvx = hexes.empty() ? 4 : 8, dim = hexes.empty() ? 2 : 3;
+        ecoords.resize(el.size() * dim);
+        for (size_t i = 0; i < el.size(); i++) {
+            for (int d = 0; d < dim; d++) ecoords[i * dim + d] = v3[3 * (size_t)el[i] + d];
+            const int r = root(el[i]);
+            if (r < 0) return mesh_fail("$Periodic correspondences form a cycle");
+            el[i] = r;
+        }
+        std::vector<int> &bd = hexes.empty() ? lines : quads;
+        for (auto &v : bd) {
+            const int r = root(v);
+            if (r < 0) return mesh_fail("$Periodic correspondences form a cycle");
+            v = r;
+        }
+        (void)nvx;
+    }
     if (!hexes.empty()) {
         // 3-D: hexahedra, boundary quadrilaterals, 2-D physical names
         m.dim = 3;
@@ -182,6 +237,7 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
         m.elem.swap(hexes);
         m.bdr.swap(quads);
         m.battr.swap(qattr);
+        m.ecoords.swap(ecoords);
         m.names = names[2];
         return PRANDTL_OK;
     }
@@ -196,6 +252,7 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
     m.elem.swap(quads);
     m.bdr.swap(lines);
     m.battr.swap(lattr);
+    m.ecoords.swap(ecoords);
     m.names = names[1];
     return PRANDTL_OK;
 }
@@ -609,7 +666,7 @@ int prandtl_mesh_view_hex(const prandtl_mesh_file *m, prandtl_hex_mesh *v)
     v->vert = m->vert.data();
     v->nelem = (int)(m->elem.size() / 8);
     v->elem = m->elem.data();
-    v->elem_coords = nullptr;
+    v->elem_coords = m->ecoords.empty() ? nullptr : m->ecoords.data();
     v->nbdr = (int)m->battr.size();
     v->bdr = m->bdr.data();
     v->bdr_attr = m->battr.data();

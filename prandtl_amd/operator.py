@@ -215,6 +215,7 @@ class HexMesh:
             check(lib().prandtl_mesh_view_hex(h, ctypes.byref(v)))
             vert = np.ctypeslib.as_array(v.vert, shape=(v.nvert * 3,)).copy()
             elem = np.ctypeslib.as_array(v.elem, shape=(v.nelem * 8,)).copy()
+            ec = np.ctypeslib.as_array(v.elem_coords, shape=(v.nelem * 24,)).copy() if v.elem_coords else None
             bdr = np.ctypeslib.as_array(v.bdr, shape=(v.nbdr * 4,)).copy() if v.nbdr else None
             battr = np.ctypeslib.as_array(v.bdr_attr, shape=(v.nbdr,)).copy() if v.nbdr else None
             names, i = {}, 0
@@ -227,7 +228,7 @@ class HexMesh:
                 i += 1
         finally:
             lib().prandtl_mesh_free(h)
-        return cls(vert, elem, bdr, battr, None, names)
+        return cls(vert, elem, bdr, battr, ec, names)
 
     def corners(self):
         """[ne,8,3] corner coordinates."""

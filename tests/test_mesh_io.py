@@ -177,3 +177,53 @@ def test_hexahedral_mesh_files(tmp_path, fname):
     assert np.array_equal(a.Mult(u), b.Mult(u))
     with pytest.raises(ValueError):
         QuadMesh.read(tmp_path / fname)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gmsh_periodic_section(tmp_path, dim):
+    """A Gmsh 2.2 box whose high faces are slaves of the low ones ($Periodic,
+    with an Affine line and a chained corner): the mesh identifies the
+    slave nodes with their masters, keeps each element's own corner
+    coordinates, and the oracle on it equals the periodic box operator."""
+    import oracle_lib as O
+    from prandtl_amd import cases
+    from prandtl_amd.operator import HexMesh
+    n, L = (3, 3, 3)[:dim], (1.0, 0.8, 0.6)[:dim]
+    nv = [k + 1 for k in n]
+    ids = {}
+    nodes = []
+    for idx in np.ndindex(*reversed(nv)):
+        ijk = tuple(reversed(idx))
+        ids[ijk] = len(nodes) + 1
+        nodes.append([ijk[d] * L[d] / n[d] for d in range(dim)] + [0.0] * (3 - dim))
+    g = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(len(nodes))]
+    g += ["%d %.17g %.17g %.17g" % (i + 1, *p) for i, p in enumerate(nodes)] + ["$EndNodes"]
+    corners = [(0, 0), (1, 0), (1, 1), (0, 1)] if dim == 2 else \
+        [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    els = []
+    for idx in np.ndindex(*reversed(n)):
+        base = tuple(reversed(idx))
+        els.append([ids[tuple(base[d] + c[d] for d in range(dim))] for c in corners])
+    g += ["$Elements", str(len(els))]
+    g += ["%d %d 2 1 1 %s" % (k + 1, 3 if dim == 2 else 5, " ".join(map(str, e))) for k, e in enumerate(els)]
+    g += ["$EndElements", "$Periodic", str(dim)]
+    for d in range(dim):
+        pairs = [(ids[ijk], ids[tuple(0 if q == d else ijk[q] for q in range(dim))]) for ijk in ids if ijk[d] == n[d]]
+        g += ["%d %d %d" % (dim - 1, 2 * d + 2, 2 * d + 1), "Affine 1 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1", str(len(pairs))]
+        g += ["%d %d" % p for p in pairs]
+    g += ["$EndPeriodic"]
+    (tmp_path / "per.msh").write_text("\n".join(g) + "\n")
+    if dim == 3:
+        h = HexMesh.read(tmp_path / "per.msh")
+        d = dict(vert=h.vert, elem=h.elem, elem_coords=h.elem_coords)
+        orc = O.Oracle.hex(d, 3, quirks=False, blending=True)
+    else:
+        h = QuadMesh.read(tmp_path / "per.msh")
+        d = dict(vert=h.vert, elem=h.elem, elem_coords=h.elem_coords)
+        orc = O.Oracle.quad(d, 3, quirks=False, blending=True)
+    assert len(np.unique(h.elem)) == int(np.prod(n))      # one vertex per periodic class
+    box = O.Oracle(dim, 3, n, (0.0,) * dim, L, quirks=False, blending=True)
+    X = box.coords()
+    assert np.abs(orc.coords() - X).max() < 1e-15
+    u = cases.random_smooth_state(X, dim, 2, L=list(L))
+    assert np.abs(orc.Mult(u) - box.Mult(u)).max() < 1e-13 * np.abs(box.Mult(u)).max()
