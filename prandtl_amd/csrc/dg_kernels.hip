@@ -74,8 +74,10 @@ struct Geo {
     double aface;    // 1/(w_0 J)
     int bcmask;      // boundary type (BC_*) of side (r, s) in bits 3(2r+s)..+2; 0: periodic box
     const double *bcs;    // device outer states of inflow / specified-state boundaries, 5 per
-                          // box side [3][2] (Cartesian) or per attribute (quadrilateral meshes)
-    // general 2-D quadrilateral geometry (null on Cartesian boxes), local faces f = 2r + side
+                          // box side [3][2] (Cartesian) or per attribute (unstructured meshes)
+    // general geometry of quadrilateral and hexahedral meshes (null on
+    // Cartesian boxes), local faces f = 2r + side; per-node tables hold DIM*DIM
+    // adjugate entries, per-point normals DIM components
     const double *met;    // [4][NE*K] adjugate row r, component d at 2r+d
     const double *jac;    // [NE*K] Jacobian determinant
     const int *fface;     // [NE*4] bit 0: first element of the face; bits 1-3 boundary type; 4+: attribute-1
@@ -404,8 +406,9 @@ template <int NP> __device__ __forceinline__ int face_point_3(int node, int r)
 // QNS3: the reference's NS-3D zeta-partner defect (faithful NS in 3-D).
 // VT: element type of the viscous inputs from k_visc (float in the mixed form).
 // FB: forced workgroup size (0: the natural one of RCfg).
-// OPEN: boundary sides on a Cartesian box.  GEN: general 2-D quadrilateral
-// geometry (per-node metrics, face tables; boundaries by attribute).  AXI
+// OPEN: boundary sides on a Cartesian box.  GEN: general geometry of a
+// quadrilateral or hexahedral mesh (per-node metrics, face tables; boundaries
+// by attribute).  AXI
 // (with GEN, Euler): axisymmetric, A.y the weighted state r U, recovered
 // node by node as it is loaded (this element's nodes and the neighbours'
 // face points); two-point, subcell and face fluxes weighted by r, the radial
@@ -1008,7 +1011,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
 }
 
 // OPEN: the box has non-periodic sides (Euler, natural shape); the periodic
-// kernels carry no boundary code.  GEN: a 2-D quadrilateral mesh.
+// kernels carry no boundary code.  GEN: a quadrilateral or hexahedral mesh.
 template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false, bool GEN = false,
           bool AXI = false>
 __global__ void __launch_bounds__((RCfg<DIM, NP, BLEND, FB>::BLOCK))
@@ -1088,7 +1091,7 @@ __global__ void k_set_dts(double *dst, int n, const DtArgs a)
 // outputs visc and fvf are FP32 in that form.
 // OPEN: a box with boundary sides; their face points take the boundary
 // type's lifting flux (LiftingBR1.cpp:311-407) instead of the neighbour jump.
-// GEN: a 2-D quadrilateral mesh (per-node metrics, face tables; traces are
+// GEN: a quadrilateral or hexahedral mesh (per-node metrics, face tables; traces are
 // Fv.n with the face's normal).
 template <int DIM, int NP, bool MIXED, int FB, bool OPEN = false, bool GEN = false>
 __global__ void __launch_bounds__((VCfg<DIM, NP, FB>::BLOCK))
@@ -1682,7 +1685,7 @@ struct prandtl_ctx {
     double *d_dt = nullptr;
     double *d_bcs = nullptr;       // [3][2][5] outer boundary states (open boxes only)
     void *d_geo = nullptr;         // general quadrilateral geometry tables
-    bool gen = false;              // quadrilateral mesh (k_rhs general-geometry variant)
+    bool gen = false;              // quadrilateral or hexahedral mesh (general-geometry kernels)
     bool axi = false;              // axisymmetric (state r U)
     double *d_axU = nullptr;       // [nv][NE][K] the state recovered from r U (checks, output)
     double hmin = 0.0;             // smallest element size (the dt policy's h_min)

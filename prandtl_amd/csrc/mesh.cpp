@@ -1,6 +1,7 @@
-// Quadrilateral mesh files (MFEM mesh v1.0, Gmsh 2.2 ASCII), uniform
-// refinement, and the per-node / per-face geometry the general-geometry
-// k_rhs reads.  Host code only.
+// Quadrilateral and hexahedral mesh files (MFEM mesh v1.0, Gmsh 2.2 ASCII),
+// uniform refinement of quadrilateral meshes, and the per-node / per-face
+// geometry the general-geometry kernels read (with the axis tables of
+// axisymmetric meshes).  Host code only.
 #include "mesh.h"
 
 #include <algorithm>

@@ -1,5 +1,5 @@
-// Host-side geometry of unstructured 2-D quadrilateral meshes for k_rhs's
-// general-geometry variant.
+// Host-side geometry of unstructured quadrilateral and hexahedral meshes for
+// the general-geometry kernels.
 #ifndef PRANDTL_MESH_H
 #define PRANDTL_MESH_H
 
