@@ -215,7 +215,7 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
             return i;
         };
         std::vector<int> &el = hexes.empty() ? quads : hexes;
-        const int nvx = hexes.empty() ? 4 : 8, dim = hexes.empty() ? 2 : 3;
+        const int dim = hexes.empty() ? 2 : 3;
         ecoords.resize(el.size() * dim);
         for (size_t i = 0; i < el.size(); i++) {
             for (int d = 0; d < dim; d++) ecoords[i * dim + d] = v3[3 * (size_t)el[i] + d];
@@ -229,7 +229,6 @@ int read_gmsh(std::istream &in, prandtl_mesh_file &m)
             if (r < 0) return mesh_fail("$Periodic correspondences form a cycle");
             v = r;
         }
-        (void)nvx;
     }
     if (!hexes.empty()) {
         // 3-D: hexahedra, boundary quadrilaterals, 2-D physical names
