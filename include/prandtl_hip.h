@@ -198,8 +198,8 @@ int prandtl_mesh_view(const prandtl_mesh_file *m, prandtl_quad_mesh *view);
 /* 2 for a quadrilateral mesh, 3 for a hexahedral one (Gmsh 8-node
  * hexahedra with 4-node boundary quadrilaterals and 2-D physical names, or
  * MFEM geometry 5 with boundary geometry 3); _view_hex fills a
- * prandtl_hex_mesh (hexahedral meshes are not refined; periodic ones are
- * read from Gmsh's $Periodic section, not from MFEM node sets) */
+ * prandtl_hex_mesh (hexahedral meshes are not refined; periodic ones come
+ * from Gmsh's $Periodic section or MFEM's L2_T1_3D_P1 node sets) */
 int prandtl_mesh_dim(const prandtl_mesh_file *m);
 int prandtl_mesh_view_hex(const prandtl_mesh_file *m, prandtl_hex_mesh *view);
 const char *prandtl_mesh_name(const prandtl_mesh_file *m, int i, int *attr);

@@ -39,13 +39,22 @@ class DGSEMOperator {
         : stream_(stream)
     {
         check(prandtl_create(&mesh, &params, &ctx_), nullptr);
-        check(prandtl_sizes(ctx_, &NE_, &K_, &nv_), ctx_);
-        order_ = mesh.order;
-        hmin_ = (mesh.hi[0] - mesh.lo[0]) / mesh.nelem[0];
-        for (int r = 1; r < mesh.dim; r++) {
-            const double h = (mesh.hi[r] - mesh.lo[r]) / mesh.nelem[r];
-            hmin_ = h < hmin_ ? h : hmin_;
-        }
+        init(mesh.order);
+    }
+    // on a quadrilateral mesh (Simulation.cpp:271-346's mesh set-up; axisymmetric
+    // with params.axisymmetric)
+    DGSEMOperator(const prandtl_quad_mesh &mesh, int order, const prandtl_params &params, void *stream = nullptr)
+        : stream_(stream)
+    {
+        check(prandtl_create_quad(&mesh, order, &params, &ctx_), nullptr);
+        init(order);
+    }
+    // on a hexahedral mesh
+    DGSEMOperator(const prandtl_hex_mesh &mesh, int order, const prandtl_params &params, void *stream = nullptr)
+        : stream_(stream)
+    {
+        check(prandtl_create_hex(&mesh, order, &params, &ctx_), nullptr);
+        init(order);
     }
     ~DGSEMOperator() { prandtl_destroy(ctx_); }
     DGSEMOperator(const DGSEMOperator &) = delete;
@@ -104,12 +113,24 @@ class DGSEMOperator {
     }
     void SetShape(int kernel, int shape) { check(prandtl_set_shape(ctx_, kernel, shape), ctx_); }
 
+    // RecoverStateFromWeighted (axisymmetric operators): U of the weighted state rU
+    void RecoverState(const double *rU, double *U) const
+    {
+        check(prandtl_axis_recover(ctx_, rU, U, stream_), ctx_);
+    }
+
     prandtl_ctx *Handle() const { return ctx_; }
 
   private:
     static void check(int rc, const prandtl_ctx *c)
     {
         if (rc != PRANDTL_OK) throw Error(rc, prandtl_last_error(c));
+    }
+    void init(int order)
+    {
+        check(prandtl_sizes(ctx_, &NE_, &K_, &nv_), ctx_);
+        check(prandtl_hmin(ctx_, &hmin_), ctx_);
+        order_ = order;
     }
     prandtl_ctx *ctx_ = nullptr;
     void *stream_ = nullptr;

@@ -299,36 +299,38 @@ int read_mfem(std::istream &in, prandtl_mesh_file &m)
             nv = std::atoi(next_line(in).c_str());
             const std::string nxt = next_line(in);
             if (nxt == "nodes") {
-                if (dim == 3) return mesh_fail("3-D mesh nodes (periodic hexahedral meshes) are not read");
                 // first-order L2 nodes with Gauss-Lobatto points: per element
-                // the corners in lexicographic order (0,0), (1,0), (0,1), (1,1)
+                // the corners in lexicographic order ((0,0), (1,0), (0,1),
+                // (1,1), then the same at z = 1 in 3-D)
                 std::string fes = next_line(in), fec = next_line(in), vd = next_line(in), ord = next_line(in);
-                if (fec.find("L2_T1_2D_P1") == std::string::npos)
-                    return mesh_fail("mesh nodes: only the L2_T1_2D_P1 collection is read");
+                const char *want = dim == 3 ? "L2_T1_3D_P1" : "L2_T1_2D_P1";
+                if (fec.find(want) == std::string::npos)
+                    return mesh_fail(std::string("mesh nodes: only the ") + want + " collection is read");
                 const int ordering = std::atoi(ord.substr(ord.find(':') + 1).c_str());
-                const int ne = (int)(m.elem.size() / 4);
+                const int nc = dim == 3 ? 8 : 4;   // corners per element
+                const int ne = (int)(m.elem.size() / nc);
+                const size_t nval = (size_t)ne * nc * dim;
                 std::vector<double> vals;
-                vals.reserve((size_t)ne * 8);
+                vals.reserve(nval);
                 double val;
-                while ((int)vals.size() < ne * 8 && in >> val) vals.push_back(val);
-                if ((int)vals.size() < ne * 8) return mesh_fail("short nodes section");
-                m.ecoords.assign((size_t)ne * 8, 0.0);
-                static const int lex_to_vertex[4] = {0, 1, 3, 2};
+                while (vals.size() < nval && in >> val) vals.push_back(val);
+                if (vals.size() < nval) return mesh_fail("short nodes section");
+                m.ecoords.assign(nval, 0.0);
+                static const int lex_to_vertex[8] = {0, 1, 3, 2, 4, 5, 7, 6};
                 for (int e = 0; e < ne; e++)
-                    for (int q = 0; q < 4; q++)
-                        for (int c = 0; c < 2; c++) {
-                            const double x = ordering == 1 ? vals[((size_t)e * 4 + q) * 2 + c]
-                                                           : vals[(size_t)c * ne * 4 + (size_t)e * 4 + q];
-                            m.ecoords[(size_t)e * 8 + 2 * lex_to_vertex[q] + c] = x;
+                    for (int q = 0; q < nc; q++)
+                        for (int c = 0; c < dim; c++) {
+                            const double x = ordering == 1 ? vals[((size_t)e * nc + q) * dim + c]
+                                                           : vals[(size_t)c * ne * nc + (size_t)e * nc + q];
+                            m.ecoords[((size_t)e * nc + lex_to_vertex[q]) * dim + c] = x;
                         }
-                m.vert.assign(2 * (size_t)std::max(nv, 0), 0.0);
+                m.vert.assign((size_t)dim * std::max(nv, 0), 0.0);
                 for (int e = 0; e < ne; e++)
-                    for (int k = 0; k < 4; k++) {
-                        const int v = m.elem[(size_t)e * 4 + k];
-                        if (v >= 0 && v < nv) {
-                            m.vert[2 * (size_t)v] = m.ecoords[(size_t)e * 8 + 2 * k];
-                            m.vert[2 * (size_t)v + 1] = m.ecoords[(size_t)e * 8 + 2 * k + 1];
-                        }
+                    for (int k = 0; k < nc; k++) {
+                        const int v = m.elem[(size_t)e * nc + k];
+                        if (v >= 0 && v < nv)
+                            for (int c = 0; c < dim; c++)
+                                m.vert[(size_t)dim * v + c] = m.ecoords[((size_t)e * nc + k) * dim + c];
                     }
             } else {
                 const int vdim = std::atoi(nxt.c_str());

@@ -227,3 +227,36 @@ def test_gmsh_periodic_section(tmp_path, dim):
     assert np.abs(orc.coords() - X).max() < 1e-15
     u = cases.random_smooth_state(X, dim, 2, L=list(L))
     assert np.abs(orc.Mult(u) - box.Mult(u)).max() < 1e-13 * np.abs(box.Mult(u)).max()
+
+
+def test_mfem_periodic_hex_nodes(tmp_path):
+    """An MFEM v1.0 periodic hexahedral mesh (L2_T1_3D_P1 nodes, one corner
+    set per element in lexicographic order, byVDIM): read with per-element
+    corners, the oracle on it equals the periodic box operator."""
+    import oracle_lib as O
+    from prandtl_amd import cases
+    from prandtl_amd.operator import HexMesh
+    n, L = (3, 3, 3), (1.0, 0.8, 0.6)
+    h = [L[d] / n[d] for d in range(3)]
+    order = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    lex = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1)]
+    els, nodes = [], []
+    for k in range(n[2]):
+        for j in range(n[1]):
+            for i in range(n[0]):
+                els.append([(((k + c) % n[2]) * n[1] + (j + b) % n[1]) * n[0] + (i + a) % n[0] for a, b, c in order])
+                nodes += [((i + a) * h[0], (j + b) * h[1], (k + c) * h[2]) for a, b, c in lex]
+    f = ["MFEM mesh v1.0", "", "dimension", "3", "", "elements", str(len(els))]
+    f += ["1 5 " + " ".join(map(str, e)) for e in els]
+    f += ["", "boundary", "0", "", "vertices", str(int(np.prod(n))), "", "nodes", "FiniteElementSpace",
+          "FiniteElementCollection: L2_T1_3D_P1", "VDim: 3", "Ordering: 1", ""]
+    f += ["%.17g %.17g %.17g" % p for p in nodes]
+    (tmp_path / "per.mesh").write_text("\n".join(f) + "\n")
+    hm = HexMesh.read(tmp_path / "per.mesh")
+    assert hm.elem_coords is not None
+    orc = O.Oracle.hex(dict(vert=hm.vert, elem=hm.elem, elem_coords=hm.elem_coords), 3, quirks=False)
+    box = O.Oracle(3, 3, n, (0.0,) * 3, L, quirks=False)
+    X = box.coords()
+    assert np.abs(orc.coords() - X).max() < 1e-15
+    u = cases.random_smooth_state(X, 3, 4, L=list(L))
+    assert np.abs(orc.Mult(u) - box.Mult(u)).max() < 1e-13 * np.abs(box.Mult(u)).max()
