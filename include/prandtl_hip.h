@@ -186,7 +186,8 @@ int prandtl_create_hex(const prandtl_hex_mesh *mesh, int order, const prandtl_pa
  * masters and keeps per-element corner coordinates, as mfem::Mesh does).  Elements keep file order.  Elements
  * with clockwise vertices are reoriented.  _refine splits every element into
  * four (children of element i at 4i..4i+3, corners in the parent's frame
- * (0,0), (1,0), (1,1), (0,1)), as ser_ref_levels does with
+ * (0,0), (1,0), (1,1), (0,1)), a hexahedron into eight (children at
+ * 8i..8i+7, child (a,b,c) at 8i + a + 2b + 4c), as ser_ref_levels does with
  * Mesh::UniformRefinement (Simulation.cpp:293-300).  _view fills the
  * geometry fields of a prandtl_quad_mesh (pointers into the file object;
  * the caller sets nattr / attr_bc / attr_state).  _name returns the name of
@@ -198,8 +199,8 @@ int prandtl_mesh_view(const prandtl_mesh_file *m, prandtl_quad_mesh *view);
 /* 2 for a quadrilateral mesh, 3 for a hexahedral one (Gmsh 8-node
  * hexahedra with 4-node boundary quadrilaterals and 2-D physical names, or
  * MFEM geometry 5 with boundary geometry 3); _view_hex fills a
- * prandtl_hex_mesh (hexahedral meshes are not refined; periodic ones come
- * from Gmsh's $Periodic section or MFEM's L2_T1_3D_P1 node sets) */
+ * prandtl_hex_mesh (periodic ones come from Gmsh's $Periodic section or
+ * MFEM's L2_T1_3D_P1 node sets) */
 int prandtl_mesh_dim(const prandtl_mesh_file *m);
 int prandtl_mesh_view_hex(const prandtl_mesh_file *m, prandtl_hex_mesh *view);
 const char *prandtl_mesh_name(const prandtl_mesh_file *m, int i, int *attr);

@@ -675,13 +675,119 @@ int prandtl_mesh_view_hex(const prandtl_mesh_file *m, prandtl_hex_mesh *v)
     return PRANDTL_OK;
 }
 
+// One uniform refinement of a hexahedral mesh: every element splits into
+// eight (children of element i at 8i..8i+7, child (a, b, c) at 8i + a + 2b +
+// 4c occupying [a/2, (a+1)/2] x [b/2, (b+1)/2] x [c/2, (c+1)/2] of the
+// parent's reference cube, corners in MFEM's vertex order), with new
+// vertices at edge midpoints, face centres and the element centre (the
+// trilinear map there: the mean of the edge's, face's or element's
+// corners); boundary quadrilaterals split into four.
+static int refine_hex(prandtl_mesh_file *m)
+{
+    const int ne = (int)(m->elem.size() / 8);
+    int nv = (int)(m->vert.size() / 3);
+    std::vector<double> vert = m->vert;
+    std::map<std::vector<int>, int> made;   // sorted parent vertex set -> new vertex
+    auto vertex_of = [&](std::vector<int> key, const double *x) {
+        std::sort(key.begin(), key.end());
+        auto it = made.find(key);
+        if (it != made.end()) return it->second;
+        for (int d = 0; d < 3; d++) vert.push_back(x[d]);
+        made.emplace(key, nv);
+        return nv++;
+    };
+    static const int C[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    auto trilinear = [&](const double c[8][3], const double xi[3], double *x) {
+        for (int d = 0; d < 3; d++) x[d] = 0.0;
+        for (int k = 0; k < 8; k++) {
+            const double w = (C[k][0] ? xi[0] : 1 - xi[0]) * (C[k][1] ? xi[1] : 1 - xi[1]) * (C[k][2] ? xi[2] : 1 - xi[2]);
+            for (int d = 0; d < 3; d++) x[d] += w * c[k][d];
+        }
+    };
+    const bool per = !m->ecoords.empty();
+    std::vector<int> elem((size_t)ne * 64);
+    std::vector<double> ecoords(per ? (size_t)ne * 192 : 0);
+    for (int e = 0; e < ne; e++) {
+        const int *v = &m->elem[(size_t)e * 8];
+        double c[8][3];
+        for (int k = 0; k < 8; k++)
+            for (int d = 0; d < 3; d++)
+                c[k][d] = per ? m->ecoords[((size_t)e * 8 + k) * 3 + d] : m->vert[3 * (size_t)v[k] + d];
+        // the vertex at reference point h/2 (h in {0,1,2}^3): the parent corners
+        // whose coordinate matches h wherever h is 0 or 2
+        int centre = -1;
+        auto at = [&](int hx, int hy, int hz) {
+            const int h[3] = {hx, hy, hz};
+            std::vector<int> key;
+            for (int k = 0; k < 8; k++) {
+                bool in = true;
+                for (int d = 0; d < 3; d++)
+                    if (h[d] != 1 && C[k][d] * 2 != h[d]) in = false;
+                if (in) key.push_back(v[k]);
+            }
+            if (key.size() == 1) return key[0];
+            const double xi[3] = {0.5 * hx, 0.5 * hy, 0.5 * hz};
+            double x[3];
+            trilinear(c, xi, x);
+            if (key.size() == 8) {   // the element centre is this element's own
+                if (centre < 0) {
+                    for (int d = 0; d < 3; d++) vert.push_back(x[d]);
+                    centre = nv++;
+                }
+                return centre;
+            }
+            return vertex_of(key, x);
+        };
+        for (int cz = 0; cz < 2; cz++)
+            for (int cy = 0; cy < 2; cy++)
+                for (int cx = 0; cx < 2; cx++) {
+                    const size_t ch = (size_t)e * 8 + cx + 2 * cy + 4 * cz;
+                    for (int k = 0; k < 8; k++) {
+                        const int hx = cx + C[k][0], hy = cy + C[k][1], hz = cz + C[k][2];
+                        elem[ch * 8 + k] = at(hx, hy, hz);
+                        if (per) {
+                            const double xi[3] = {0.5 * hx, 0.5 * hy, 0.5 * hz};
+                            trilinear(c, xi, &ecoords[(ch * 8 + k) * 3]);
+                        }
+                    }
+                }
+    }
+    // boundary quadrilaterals (a, b, c, d in cyclic order): four children
+    std::vector<int> bdr, battr;
+    const int nb = (int)m->battr.size();
+    for (int i = 0; i < nb; i++) {
+        const int *q = &m->bdr[(size_t)i * 4];
+        auto find = [&](std::vector<int> key) {
+            std::sort(key.begin(), key.end());
+            auto it = made.find(key);
+            return it == made.end() ? -1 : it->second;
+        };
+        const int e01 = find({q[0], q[1]}), e12 = find({q[1], q[2]}), e23 = find({q[2], q[3]});
+        const int e30 = find({q[3], q[0]}), fc = find({q[0], q[1], q[2], q[3]});
+        if (e01 < 0 || e12 < 0 || e23 < 0 || e30 < 0 || fc < 0) {
+            // a quadrilateral that bounds no element stays whole
+            bdr.insert(bdr.end(), q, q + 4);
+            battr.push_back(m->battr[i]);
+            continue;
+        }
+        const int kids[4][4] = {{q[0], e01, fc, e30}, {e01, q[1], e12, fc}, {fc, e12, q[2], e23}, {e30, fc, e23, q[3]}};
+        for (auto &k : kids) {
+            bdr.insert(bdr.end(), k, k + 4);
+            battr.push_back(m->battr[i]);
+        }
+    }
+    m->vert.swap(vert);
+    m->elem.swap(elem);
+    m->ecoords.swap(ecoords);
+    m->bdr.swap(bdr);
+    m->battr.swap(battr);
+    return PRANDTL_OK;
+}
+
 int prandtl_mesh_refine(prandtl_mesh_file *m)
 {
     if (!m) return PRANDTL_ERR_ARG;
-    if (m->dim != 2) {
-        g_mesh_err = "uniform refinement is built for quadrilateral meshes";
-        return PRANDTL_ERR_UNSUPPORTED;
-    }
+    if (m->dim == 3) return refine_hex(m);
     const int ne = (int)(m->elem.size() / 4);
     int nv = (int)(m->vert.size() / 2);
     std::unordered_map<long long, int> mid;

@@ -202,8 +202,9 @@ class HexMesh:
         return len(self.elem)
 
     @classmethod
-    def read(cls, path):
-        """Read a Gmsh 2.2 or MFEM v1.0 hexahedral mesh (prandtl_mesh_read)."""
+    def read(cls, path, refine=0):
+        """Read a Gmsh 2.2 or MFEM v1.0 hexahedral mesh (prandtl_mesh_read) and
+        refine it `refine` times."""
         h = ctypes.c_void_p()
         rc = lib().prandtl_mesh_read(str(path).encode(), ctypes.byref(h))
         if rc != _lib.PRANDTL_OK:
@@ -211,6 +212,8 @@ class HexMesh:
         try:
             if lib().prandtl_mesh_dim(h) != 3:
                 raise ValueError("%s is not a hexahedral mesh (use QuadMesh.read)" % path)
+            for _ in range(refine):
+                check(lib().prandtl_mesh_refine(h))
             v = _lib.HexMesh()
             check(lib().prandtl_mesh_view_hex(h, ctypes.byref(v)))
             vert = np.ctypeslib.as_array(v.vert, shape=(v.nvert * 3,)).copy()

@@ -260,3 +260,63 @@ def test_mfem_periodic_hex_nodes(tmp_path):
     assert np.abs(orc.coords() - X).max() < 1e-15
     u = cases.random_smooth_state(X, 3, 4, L=list(L))
     assert np.abs(orc.Mult(u) - box.Mult(u)).max() < 1e-13 * np.abs(box.Mult(u)).max()
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_hexahedral_refinement(tmp_path, periodic):
+    """One uniform refinement of a hexahedral box equals the box of twice the
+    elements (elements matched by centre); boundary quadrilaterals split into
+    four with their attributes; refining a distorted mesh keeps its volume."""
+    import oracle_lib as O
+    from prandtl_amd import cases
+    from prandtl_amd.operator import HexMesh
+    n, L = (3, 3, 3), (1.5, 1.0, 0.8)
+    if periodic:
+        f = tmp_path / "per.mesh"
+        order = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+        lex = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1)]
+        h = [L[d] / n[d] for d in range(3)]
+        els, nodes = [], []
+        for k in range(n[2]):
+            for j in range(n[1]):
+                for i in range(n[0]):
+                    els.append([(((k + c) % n[2]) * n[1] + (j + b) % n[1]) * n[0] + (i + a) % n[0]
+                                for a, b, c in order])
+                    nodes += [((i + a) * h[0], (j + b) * h[1], (k + c) * h[2]) for a, b, c in lex]
+        txt = ["MFEM mesh v1.0", "dimension", "3", "elements", str(len(els))]
+        txt += ["1 5 " + " ".join(map(str, e)) for e in els]
+        txt += ["boundary", "0", "vertices", str(int(np.prod(n))), "nodes", "FiniteElementSpace",
+                "FiniteElementCollection: L2_T1_3D_P1", "VDim: 3", "Ordering: 1", ""]
+        txt += ["%.17g %.17g %.17g" % p for p in nodes]
+        f.write_text("\n".join(txt) + "\n")
+    else:
+        _hex_files(tmp_path)
+        f = tmp_path / "box.mesh"
+    base = HexMesh.read(f)
+    ref = HexMesh.read(f, refine=1)
+    assert ref.nelem == 8 * base.nelem and len(ref.bdr_attr) == 4 * len(base.bdr_attr)
+    assert len(np.unique(ref.elem)) == (int(np.prod([2 * v for v in n])) if periodic else
+                                        int(np.prod([2 * v + 1 for v in (3, 2, 2)])))
+    if not periodic:
+        assert sorted(ref.bdr_attr.tolist()) == sorted(np.repeat(base.bdr_attr, 4).tolist())
+        # the distorted mesh keeps its volume under refinement
+        va = O.Oracle.hex(dict(vert=base.vert, elem=base.elem, bdr=base.bdr, bdr_attr=base.bdr_attr,
+                               attr_bc=["slip"] * 6, attr_state=[np.array([1, 0, 0, 0, 2.5])] * 6), 2).weights().sum()
+        vb = O.Oracle.hex(dict(vert=ref.vert, elem=ref.elem, bdr=ref.bdr, bdr_attr=ref.bdr_attr,
+                               attr_bc=["slip"] * 6, attr_state=[np.array([1, 0, 0, 0, 2.5])] * 6), 2).weights().sum()
+        assert abs(va - vb) < 1e-13 * va
+        return
+    d = dict(vert=ref.vert, elem=ref.elem, elem_coords=ref.elem_coords)
+    orc = O.Oracle.hex(d, 2, quirks=False, blending=True)
+    n2 = tuple(2 * v for v in n)
+    box = O.Oracle(3, 2, n2, (0.0,) * 3, L, quirks=False, blending=True)
+    Xr, Xb = orc.coords(), box.coords()
+    cr, cb = Xr.mean(axis=2).T, Xb.mean(axis=2).T
+    perm = [int(np.argmin(np.linalg.norm(cb - c, axis=1))) for c in cr]
+    assert len(set(perm)) == len(perm)
+    assert np.abs(Xr - Xb[:, perm, :]).max() < 1e-14
+    u = cases.random_smooth_state(Xb, 3, 5, L=list(L))
+    ur = u[:, perm, :]
+    a = orc.Mult(ur).reshape(5, -1, orc.K)
+    b = box.Mult(u).reshape(5, -1, box.K)[:, perm, :]
+    assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
