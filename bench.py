@@ -488,12 +488,12 @@ def main():
         if cfg.get("blending"):
             by_rhs += 2 * Np * (Np - 1) * 2 * 8 / K
     if "hex_warp" in cfg:
-        # 3-D general geometry: 9 adjugate entries and J per node, per face
-        # point the neighbour node (4 B) and normal (24 B), a code and a
-        # neighbour face per face
+        # 3-D general geometry: the element's 8 corners (the kernels recompute
+        # the per-node metrics from them), per face point the neighbour node
+        # (4 B) and normal (24 B), a code and a neighbour face per face
         Np = cfg["order"] + 1
         K = Np ** 3
-        by_rhs += 8 * 10 + (6 * Np * Np * 28 + 6 * 8) / K
+        by_rhs += (24 * 8 + 6 * Np * Np * 28 + 6 * 8) / K
         if cfg.get("axisymmetric"):
             # r and the axis code per node (U is recovered from r U in registers)
             by_rhs += 12

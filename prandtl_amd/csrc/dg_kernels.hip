@@ -63,6 +63,7 @@ struct Tables {
     double V1i[MAXNP * MAXNP];  // inverse 1-D shifted-Legendre Vandermonde
     double fvs[MAXNP];          // 1/(J w_i)            (line 1046)
     double w[MAXNP];            // LGL weights on [0,1]
+    double x[MAXNP];            // LGL nodes on [0,1]
     double subn[3][MAXNP];      // subcell normal magnitude per interface (1199-1304)
 };
 
@@ -85,6 +86,8 @@ struct Geo {
     const int *fnf;       // [NE*4] 2 (e2*4 + f2) + reversed: the neighbour's face of each interior face
     const double *fnor;   // [NE*4*NFP*2] normal outward of the face's first element
     const double *subn;   // [NE][2][NFP][NP-1][2] subcell interface normals
+    const double *corner; // hexahedral meshes: [NE][8][3] element corners; the per-node metrics
+                          // are recomputed from them (met and jac are not stored)
     // axisymmetric quadrilateral meshes (null otherwise)
     const double *rnode;  // [NE*K] r = y of each node
     const int *axm;       // [NE*K] -1 off the axis, else 2 d + (1: radial RHS zeroed), d the descriptor
@@ -330,6 +333,7 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false, bool
     double Red[(C::BLOCK + 63) / 64];
     double M[GEN ? C::EPB : 1][GEN ? DIM * DIM : 1][GEN ? C::K : 1];   // per-node adjugate rows (general geometry)
     double R[AXI ? C::EPB : 1][AXI ? C::K : 1];      // per-node r (axisymmetric)
+    double Cn[GEN && DIM == 3 ? C::EPB : 1][GEN && DIM == 3 ? 24 : 1];   // hexahedron corners
 };
 
 // DGSEMOperator::RecoverStateFromWeighted (DGSEMOperator.cpp:161-424) for one
@@ -390,6 +394,48 @@ __device__ __forceinline__ void axis_node(const double *rU, const Geo &g, int K,
         U[2] = 0.0;
         U[3] = E;
     }
+}
+
+// Adjugate rows (row r = Ja^r, the contravariant metric) and determinant of
+// a hexahedron's trilinear map at reference point xi, from its corners c
+// ([8][3], MFEM's vertex order): the metrics the host builds per node, here
+// recomputed where they are used, which saves the 80 bytes per node of a
+// stored table.
+__device__ __forceinline__ void hex_metrics(const double *c, double x0, double x1, double x2, double *adj,
+                                            double &det)
+{
+    constexpr int CV[8] = {0, 1, 3, 2, 4, 5, 7, 6};   // vertex k -> bits (x + 2y + 4z) of its corner
+    double J[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) J[a][b] = 0.0;
+    const double xi[3] = {x0, x1, x2};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        double f[3], df[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const bool one = (CV[k] >> d) & 1;
+            f[d] = one ? xi[d] : 1.0 - xi[d];
+            df[d] = one ? 1.0 : -1.0;
+        }
+        const double dN[3] = {df[0] * f[1] * f[2], f[0] * df[1] * f[2], f[0] * f[1] * df[2]};
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) J[a][b] += dN[b] * c[3 * k + a];
+    }
+    adj[0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    adj[1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    adj[2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    adj[3] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    adj[4] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    adj[5] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    adj[6] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    adj[7] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    adj[8] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    det = J[0][0] * adj[0] + J[0][1] * adj[3] + J[0][2] * adj[6];
 }
 
 // Face point l (a + NP b, a and b the other two coordinates in increasing
@@ -483,10 +529,18 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         for (int v = 0; v < NV; v++) yw[v] = ys[v];
         axis_node(A.y, A.geo, K, NK, (size_t)e * K + n, ys);
     }
-    if constexpr (GEN) if (act) {
-        jn = A.geo.jac[(size_t)e * K + n];
+    if constexpr (GEN && DIM == 3) {
+        // the tile's corners into LDS, then each node's metrics from them
+        for (int i = t; i < EPB * 24; i += BLOCK)
+            S.Cn[i / 24][i % 24] = (ebase + i / 24 < NE) ? A.geo.corner[(size_t)(ebase + i / 24) * 24 + i % 24] : 0.0;
+        __syncthreads();
+        if (act) hex_metrics(S.Cn[el], A.tab.x[ii[0]], A.tab.x[ii[1]], A.tab.x[ii[2]], mown, jn);
+    } else if constexpr (GEN) {
+        if (act) {
+            jn = A.geo.jac[(size_t)e * K + n];
 #pragma unroll
-        for (int c = 0; c < MC; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
+            for (int c = 0; c < MC; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
+        }
     }
     // Blended kernels carry the subcell-flux work and its accumulator; they
     // do not hold loads in flight across phases (the surface data is loaded
@@ -1111,6 +1165,7 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
     __shared__ R sD[NP * NP];
     __shared__ R sDh[NP * NP];
     __shared__ double sUl[EPB][NV];   // state of node K-1 (the faithful stale state)
+    __shared__ double sCn[GEN && DIM == 3 ? EPB : 1][GEN && DIM == 3 ? 24 : 1];   // hexahedron corners
     __shared__ double sWref[EPB][NV];
     __shared__ R sScr[SCR];
     R (*sW)[NV][K] = reinterpret_cast<R (*)[NV][K]>(sScr);
@@ -1148,10 +1203,17 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
     double jn = 1.0, mown[GEN ? MC : 1];
 #pragma unroll
     for (int c = 0; c < (GEN ? MC : 1); c++) mown[c] = 0.0;
-    if constexpr (GEN) if (act) {
-        jn = A.geo.jac[(size_t)e * K + n];
+    if constexpr (GEN && DIM == 3) {
+        for (int i = t; i < EPB * 24; i += BLOCK)
+            sCn[i / 24][i % 24] = (ebase + i / 24 < NE) ? A.geo.corner[(size_t)(ebase + i / 24) * 24 + i % 24] : 0.0;
+        __syncthreads();
+        if (act) hex_metrics(sCn[el], A.tab.x[ii[0]], A.tab.x[ii[1]], A.tab.x[ii[2]], mown, jn);
+    } else if constexpr (GEN) {
+        if (act) {
+            jn = A.geo.jac[(size_t)e * K + n];
 #pragma unroll
-        for (int c = 0; c < MC; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
+            for (int c = 0; c < MC; c++) mown[c] = A.geo.met[c * NK + (size_t)e * K + n];
+        }
     }
 #pragma unroll
     for (int k = 0; k < JT; k++) {
@@ -2141,7 +2203,10 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     for (int a = 0; a < Np; a++)
         for (int i = 0; i < Np; i++) c->tab.V1i[a * MAXNP + i] = Vi[a * Np + i];
 
-    for (int i = 0; i < Np; i++) c->tab.w[i] = w[i];
+    for (int i = 0; i < Np; i++) {
+        c->tab.w[i] = w[i];
+        c->tab.x[i] = x[i];
+    }
     quad::Geometry qg;
     if (qm) {
         std::string err;
@@ -2160,6 +2225,9 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
             delete c;
             return fail(nullptr, PRANDTL_ERR_ARG, err);
         }
+        // the kernels recompute the per-node metrics from the corners
+        qg.met.clear();
+        qg.jac.clear();
         c->gen = true;
         c->hmin = qg.hmin;
     }
@@ -2240,16 +2308,16 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (e == hipSuccess && umesh) {
         // general geometry tables, one allocation
         auto bytes = [](const auto &v) { return v.size() * sizeof(v[0]); };
-        constexpr int NPART = 12;
+        constexpr int NPART = 13;
         const size_t parts[NPART] = {bytes(qg.met), bytes(qg.jac), bytes(qg.fnor), bytes(qg.subn), bytes(qstates),
                                      bytes(qg.fface), bytes(qg.fnb), bytes(qg.fnf), bytes(qg.rnode),
-                                     bytes(qg.axm), bytes(qg.axj), bytes(qg.axr)};
+                                     bytes(qg.axm), bytes(qg.axj), bytes(qg.axr), bytes(qg.corner)};
         size_t off[NPART + 1] = {0};
         for (int i = 0; i < NPART; i++) off[i + 1] = off[i] + ((parts[i] + 255) / 256) * 256;
         e = hipMalloc(&c->d_geo, off[NPART]);
         const void *src[NPART] = {qg.met.data(), qg.jac.data(), qg.fnor.data(), qg.subn.data(), qstates.data(),
                                   qg.fface.data(), qg.fnb.data(), qg.fnf.data(), qg.rnode.data(),
-                                  qg.axm.data(), qg.axj.data(), qg.axr.data()};
+                                  qg.axm.data(), qg.axj.data(), qg.axr.data(), qg.corner.data()};
         for (int i = 0; i < NPART && e == hipSuccess; i++)
             if (parts[i])
                 e = hipMemcpy(static_cast<char *>(c->d_geo) + off[i], src[i], parts[i], hipMemcpyHostToDevice);
@@ -2262,6 +2330,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         c->geo.fface = reinterpret_cast<const int *>(b + off[5]);
         c->geo.fnb = reinterpret_cast<const int *>(b + off[6]);
         c->geo.fnf = reinterpret_cast<const int *>(b + off[7]);
+        if (hm) c->geo.corner = reinterpret_cast<const double *>(b + off[12]);
         if (c->axi) {
             c->geo.rnode = reinterpret_cast<const double *>(b + off[8]);
             c->geo.axm = reinterpret_cast<const int *>(b + off[9]);

@@ -989,10 +989,13 @@ bool build_hex(const prandtl_hex_mesh *m, int order, const std::vector<double> &
     const size_t NK = (size_t)NE * K;
     g.met.assign(9 * NK, 0.0);
     g.jac.assign(NK, 0.0);
+    g.corner.assign((size_t)NE * 24, 0.0);
     g.hmin = INFINITY;
     for (int e = 0; e < NE; e++) {
         double c[8][3];
         corners(e, c);
+        for (int k = 0; k < 8; k++)
+            for (int d = 0; d < 3; d++) g.corner[(size_t)e * 24 + 3 * k + d] = c[k][d];
         for (int k = 0; k < NP; k++)
             for (int j = 0; j < NP; j++)
                 for (int i = 0; i < NP; i++) {

@@ -32,6 +32,7 @@ struct Geometry {
     std::vector<double> fnor;   // [NE*4*NFP*2]: normal outward of the face's first element
     std::vector<double> subn;   // [NE][2][NFP][NP-1][2]: subcell interface normals
     double hmin = 0.0;
+    std::vector<double> corner; // hexahedral meshes: [NE][8][3] element corners
     // axisymmetric meshes (r = y): r per node; per node -1 off the axis, else
     // 2 d + z with d the node's axis descriptor and z = 1 when the node is in
     // the axis index (its radial-momentum RHS is zeroed); per descriptor the
