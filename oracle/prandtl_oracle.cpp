@@ -18,8 +18,20 @@
  * log, exp, pow counts one).
  *
  * Geometry is general per-node (J, adjugate rows, face normals, subcell
- * normals) so the restatement follows the reference's metric use; the mesh
- * generator provided here builds periodic Cartesian boxes.
+ * normals) so the restatement follows the reference's metric use; it is
+ * built for Cartesian boxes (periodic or with boundary sides), quadrilateral
+ * meshes (bilinear) and hexahedral meshes (trilinear), and the axisymmetric
+ * mode recovers U from r U (RecoverStateFromWeighted).
+ *
+ * Pins (tests/): each function is checked against closed forms, invariants
+ * or equivalences that a plausible slip breaks — test_oracle.py (fluxes,
+ * operators, RK3), test_oracle_bc.py and test_oracle_ns_walls.py (boundary
+ * fluxes and lifting), test_oracle_quad.py and test_oracle_hex.py (mesh
+ * geometry: equality with the box operator, free stream, conservation,
+ * rotation and relabelling), test_oracle_axisym.py (recovery closed forms,
+ * exact steady states, conservation, convergence).  Parity unpinned: the
+ * last bits that MFEM's own LGL rounding, dense inverse and RK3 axpy order
+ * would fix (MFEM is not in the reference tree).
  */
 #include "prandtl_oracle.h"
 
