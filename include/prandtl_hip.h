@@ -173,8 +173,8 @@ typedef struct prandtl_hex_mesh {
 } prandtl_hex_mesh;
 
 /* DGSEMOperator constructor on a hexahedral mesh (orders 2..4; Euler or
- * Navier-Stokes, with or without blending; the faithful Navier-Stokes 3-D
- * form is not built here).  Uses the current device. */
+ * Navier-Stokes, corrected or faithful, with or without blending).  Uses the
+ * current device. */
 int prandtl_create_hex(const prandtl_hex_mesh *mesh, int order, const prandtl_params *params,
                        prandtl_ctx **out);
 

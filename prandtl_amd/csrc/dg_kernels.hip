@@ -1891,6 +1891,13 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
                                        dim3(Cf::BLOCK), 0, s, a);
                     return hip_check(c, hipGetLastError(), "k_rhs launch");
                 }
+            } else {
+                if (qns3) {
+                    // the faithful NS-3D form on a hexahedral mesh
+                    hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, true, double, 0, false, true>), dim3(nb),
+                                       dim3(Cf::BLOCK), 0, s, a);
+                    return hip_check(c, hipGetLastError(), "k_rhs launch");
+                }
             }
             if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
                 hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, float, 0, false, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
@@ -2064,9 +2071,6 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (hm) {
         if (hm->nelem < 1) return fail(nullptr, PRANDTL_ERR_ARG, "empty mesh");
         if (N > 4) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "hexahedral meshes are built for orders 2..4");
-        if (prm->parabolic && prm->quirks)
-            return fail(nullptr, PRANDTL_ERR_UNSUPPORTED,
-                        "hexahedral meshes are not built with the faithful Navier-Stokes 3-D form");
         NE = hm->nelem;
     }
     const bool umesh = qm || hm;   // an unstructured mesh, not a box

@@ -3,7 +3,7 @@
 
 Bar: the plain FP64 RHS (and BR1 gradients) within 1e-12 relative max-norm
 of the oracle's on distorted periodic and bounded meshes, Euler and
-Navier-Stokes, with and without blending; the wave speed within 1e-13; one
+Navier-Stokes (corrected and faithful), with and without blending; the wave speed within 1e-13; one
 RK3 step within 1e-12; the mixed form within 1e-6 (relative L2) of FP64; a
 uniform grid given as a hexahedral mesh equals the Cartesian kernels to
 1e-13; renumbered elements with turned frames give the same RHS;
@@ -60,6 +60,11 @@ CASES = {
                           (["supersonic-inflow", "supersonic-outflow", "slip", "slip", "symmetry", "slip"],
                            [IN] * 6)),
     "ns_periodic_n3": (dict(periodic=(True,) * 3, perturb=0.2, seed=6), 3, dict(NS), None),
+    "ns_periodic_n3_faithful": (dict(periodic=(True,) * 3, perturb=0.2, seed=8), 3, dict(NS, quirks=True), None),
+    "ns_walls_n2_faithful": (dict(periodic=(False, True, False), perturb=0.2, seed=9), 2,
+                             dict(NS, quirks=True, blending=True),
+                             (["supersonic-inflow", "supersonic-outflow", "no-slip-adiabatic", "no-slip-adiabatic",
+                               "no-slip-adiabatic", "no-slip-adiabatic"], [IN, IN, WALL, WALL, WALL, LID])),
     "ns_walls_n3_blend": (dict(periodic=(False, True, False), perturb=0.2, seed=7), 3, dict(NS, blending=True),
                           (["supersonic-inflow", "supersonic-outflow", "no-slip-adiabatic", "no-slip-adiabatic",
                             "no-slip-adiabatic", "no-slip-adiabatic"], [IN, IN, WALL, WALL, WALL, LID])),
