@@ -1906,9 +1906,16 @@ int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
         }
     }
     if (c->geo.bcmask) {
-        // boundary sides, natural shape (not with the faithful NS-3D form)
+        // boundary sides, natural shape
         using Cf = RCfg<DIM, NP, BLEND>;
         const int nb = (c->NE + Cf::EPB - 1) / Cf::EPB;
+        if constexpr (DIM == 3) {
+            if (qns3) {
+                // the faithful NS-3D form with boundary sides
+                hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, true, double, 0, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
+                return hip_check(c, hipGetLastError(), "k_rhs launch");
+            }
+        }
         if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
             hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, float, 0, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, 0, true>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
@@ -2080,7 +2087,6 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         NE *= mesh->nelem[r];
     }
     if (NE > (1LL << 30)) return fail(nullptr, PRANDTL_ERR_ARG, "mesh too large");
-    bool open = false;
     for (int r = 0; r < dim && !umesh; r++) {
         for (int side = 0; side < 2; side++) {
             const int t = mesh->bc[r][side];
@@ -2105,10 +2111,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         }
         if ((mesh->bc[r][0] == PRANDTL_BC_PERIODIC) != (mesh->bc[r][1] == PRANDTL_BC_PERIODIC))
             return fail(nullptr, PRANDTL_ERR_ARG, "a direction is periodic on both sides or on neither");
-        open = open || mesh->bc[r][0] != PRANDTL_BC_PERIODIC;
     }
-    if (open && prm->parabolic && dim == 3 && prm->quirks)
-        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "boundary sides are not built with the faithful NS-3D form");
     if (prm->axisymmetric && !qm)
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "axisymmetric mode is built for quadrilateral meshes");
     if (prm->axisymmetric && prm->parabolic)

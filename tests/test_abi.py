@@ -78,12 +78,10 @@ def _create(mesh_kw, prm_kw):
     (dict(bc=((1, 1), (5, 5), (0, 0)), bc_state=(-1.0, 0.0, 0.0, 2.5, 0.0)), {}, _lib.PRANDTL_ERR_ARG),
     # Navier-Stokes sides: no slip / symmetry (no viscous flux in the
     # reference), no-slip walls without viscosity, heat flux in faithful
-    # mode, walls with the faithful NS-3D form
+    # mode
     (dict(bc=((4, 4), (0, 0), (0, 0))), dict(parabolic=1), _lib.PRANDTL_ERR_UNSUPPORTED),
     (dict(bc=((6, 6), (0, 0), (0, 0)), bc_state=(0.0,) * 5), {}, _lib.PRANDTL_ERR_UNSUPPORTED),
     (dict(bc=((6, 6), (0, 0), (0, 0)), bc_state=(0.0, 0.0, 0.0, 0.5, 0.0)), dict(parabolic=1),
-     _lib.PRANDTL_ERR_UNSUPPORTED),
-    (dict(dim=3, bc=((6, 6), (0, 0), (0, 0)), bc_state=(0.0,) * 5), dict(parabolic=1),
      _lib.PRANDTL_ERR_UNSUPPORTED),
 ])
 def test_create_rejects(mesh_kw, prm_kw, code):

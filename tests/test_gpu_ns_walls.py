@@ -52,6 +52,10 @@ CASES = {
                                   bc=[("supersonic-inflow", "supersonic-outflow"),
                                       ("no-slip-adiabatic", "no-slip-adiabatic")],
                                   bc_state=[(IN2, None), (WALL_HEAT, WALL_MOVING)], **NS),
+    "duct3d_n3_faithful": dict(dim=3, order=3, nelem=(3, 3, 2), lo=(0.0,) * 3, hi=(1.0, 1.0, 0.8), quirks=True,
+                               bc=[("supersonic-inflow", "supersonic-outflow"),
+                                   ("no-slip-adiabatic", "no-slip-adiabatic"), ("periodic", "periodic")],
+                               bc_state=[(IN3, None), (REST, WALL_MOVING), (None, None)], **NS),
     "duct3d_n3": dict(dim=3, order=3, nelem=(3, 3, 2), lo=(0.0,) * 3, hi=(1.0, 1.0, 0.8), quirks=False,
                       bc=[("supersonic-inflow", "supersonic-outflow"), ("no-slip-adiabatic", "no-slip-adiabatic"),
                           ("periodic", "periodic")],
@@ -95,7 +99,7 @@ def test_ns_boundary_rhs_and_gradient_parity(cuda, name):
     assert abs(op.GetMaxCharSpeed() - orc.GetMaxCharSpeed()) <= 1e-13 * orc.GetMaxCharSpeed()
 
 
-@pytest.mark.parametrize("name", ["channel2d_n3", "duct3d_n3"])
+@pytest.mark.parametrize("name", ["channel2d_n3", "duct3d_n3", "duct3d_n3_faithful"])
 def test_ns_boundary_rk3_step_and_graph(cuda, name):
     import torch
     cfg = CASES[name]
