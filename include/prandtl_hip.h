@@ -32,13 +32,15 @@ enum { PRANDTL_FP64 = 0, PRANDTL_MIXED = 1 };
 
 typedef struct prandtl_ctx prandtl_ctx;
 
-/* Periodic Cartesian box of hexahedra/quadrilaterals with LGL nodes.
- * Replaces the mesh/space set-up the operator receives in
+/* Cartesian box of hexahedra/quadrilaterals (or a segment of intervals)
+ * with LGL nodes.  Replaces the mesh/space set-up the operator receives in
  * DGSEMOperator::DGSEMOperator (DGSEMOperator.hpp:87-99; spaces built at
- * Simulation.cpp:338-346).  Element index = (ez*ny + ey)*nx + ex. */
+ * Simulation.cpp:338-346).  Element index = (ez*ny + ey)*nx + ex.  1-D
+ * (the reference's TestCases/Euler/1D, MFEM inline segment meshes): Euler
+ * only; face normals are +-1 and J = h (DGSEMIntegrator.cpp:180-183). */
 typedef struct prandtl_mesh_desc {
-    int dim;            /* 2 or 3 */
-    int order;          /* polynomial degree N (Np = N+1): 1..5 in 2-D, 2..7 in 3-D */
+    int dim;            /* 1, 2 or 3 */
+    int order;          /* polynomial degree N (Np = N+1): 1..7 in 1-D, 1..5 in 2-D, 2..7 in 3-D */
     int nelem[3];       /* elements per direction */
     double lo[3], hi[3];
     /* Boundary condition of each box side: bc[r][0] at lo[r], bc[r][1] at

@@ -410,7 +410,8 @@ template <class T> T slip_wall_flux(int dim, double gamma, const T *s1, const T 
     T unit[3], st[5], prim[5];
     normalize(dim, nor, unit);
     for (int v = 0; v < nv; v++) st[v] = s1[v];
-    rotate_state(dim, st, unit);
+    if (dim == 1) st[1] = st[1] * nor[0];   /* SlipWallBdrFaceIntegrator.cpp:28-30 */
+    else rotate_state(dim, st, unit);
     /* Conserv2Prim (BasicOperations.cpp:172-186): velocities by division,
      * pressure by ComputePressure */
     prim[0] = st[0];
@@ -661,6 +662,7 @@ struct Mesh {
 
 inline int line_node(int dim, int Np, int r, int l, int m)
 {
+    if (dim == 1) return m;
     if (dim == 2) return r == 0 ? l * Np + m : m * Np + l;
     int a = l % Np, b = l / Np;
     if (r == 0) return (b * Np + a) * Np + m;
@@ -693,8 +695,8 @@ void build_mesh(orc_ctx *c)
     const orc_params &p = c->p;
     m.dim = p.dim; m.N = p.order; m.Np = p.order + 1;
     m.nv = p.dim + 2;
-    m.K = m.Np * m.Np * (p.dim == 3 ? m.Np : 1);
-    m.Nfp = (p.dim == 3) ? m.Np * m.Np : m.Np;
+    m.K = m.Np * (p.dim >= 2 ? m.Np : 1) * (p.dim == 3 ? m.Np : 1);
+    m.Nfp = (p.dim == 3) ? m.Np * m.Np : p.dim == 2 ? m.Np : 1;
     m.NE = 1;
     for (int r = 0; r < 3; r++) m.n[r] = (r < p.dim) ? p.nelem[r] : 1;
     for (int r = 0; r < p.dim; r++) {
@@ -705,7 +707,10 @@ void build_mesh(orc_ctx *c)
     const int dim = m.dim, K = m.K, NE = m.NE, Np = m.Np, N = m.N, Nfp = m.Nfp;
     /* Cartesian Jacobian diag(h); det and adjugate with MFEM's association */
     double Jd, ad[3] = {0, 0, 0};
-    if (dim == 2) { Jd = m.h[0] * m.h[1]; ad[0] = m.h[1]; ad[1] = m.h[0]; }
+    /* 1-D: MFEM's adjugate of a 1 x 1 matrix is 1, so the face normal is
+     * +-1 (DGSEMIntegrator.cpp:180-183 set it from the face's side) */
+    if (dim == 1) { Jd = m.h[0]; ad[0] = 1.0; }
+    else if (dim == 2) { Jd = m.h[0] * m.h[1]; ad[0] = m.h[1]; ad[1] = m.h[0]; }
     else {
         Jd = m.h[0] * (m.h[1] * m.h[2]);
         ad[0] = m.h[1] * m.h[2]; ad[1] = m.h[0] * m.h[2]; ad[2] = m.h[0] * m.h[1];
@@ -1499,12 +1504,12 @@ template <class T> struct Engine {
             }
         }
         /* inviscid pairs, node loop (k, j, i) as the reference */
-        const int Nz = dim == 3 ? Np : 1;
+        const int Nz = dim == 3 ? Np : 1, Ny = dim >= 2 ? Np : 1;
         auto FI = [&](int r, int node, int part) -> T * {
             return &Fi[(((size_t)r * K + node) * Np + part) * nv];
         };
         for (int k = 0; k < Nz; k++)
-            for (int j = 0; j < Np; j++)
+            for (int j = 0; j < Ny; j++)
                 for (int i = 0; i < Np; i++) {
                     g_cat = CAT_VOL;
                     int id1 = (k * Np + j) * Np + i;
@@ -1977,7 +1982,10 @@ int orc_create_quad(const orc_params *p, const orc_quad_mesh *m, orc_ctx **out)
 static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **out, const orc_hex_mesh *hm)
 {
     if (!p || !out) return 1;
-    if (p->dim < 2 || p->dim > 3 || p->order < 1 || p->order > 12) return 2;
+    if (p->dim < 1 || p->dim > 3 || p->order < 1 || p->order > 12) return 2;
+    /* 1-D: the Euler equations on a segment (every 1-D case of the
+     * reference, TestCases/Euler/1D, is an Euler case) */
+    if (p->dim == 1 && (p->parabolic || qm || hm)) return 2;
     const bool mesh = qm || hm;
     for (int r = 0; r < p->dim && !mesh; r++)
         if (p->nelem[r] < 1 || !(p->hi[r] > p->lo[r])) return 3;
@@ -2045,7 +2053,8 @@ static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **o
     const int Np = m.Np, K = m.K, dim = m.dim;
     c->degs.assign(K * dim, 0);
     for (int n = 0; n < K; n++) {
-        if (dim == 2) { c->degs[n * 2 + 0] = n / Np; c->degs[n * 2 + 1] = n % Np; }
+        if (dim == 1) c->degs[n] = n;
+        else if (dim == 2) { c->degs[n * 2 + 0] = n / Np; c->degs[n * 2 + 1] = n % Np; }
         else {
             c->degs[n * 3 + 0] = n / (Np * Np);
             c->degs[n * 3 + 1] = (n / Np) % Np;
@@ -2065,7 +2074,8 @@ static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **o
     c->wJ.assign((size_t)m.NE * K, 0.0);
     for (int e = 0; e < m.NE; e++)
         for (int i = 0; i < K; i++) {
-            double wt = c->w[i % Np] * c->w[(i / Np) % Np];
+            double wt = c->w[i % Np];
+            if (dim >= 2) wt *= c->w[(i / Np) % Np];
             if (dim == 3) wt *= c->w[i / (Np * Np)];
             c->wJ[(size_t)e * K + i] = wt * m.J[(size_t)e * K + i];
         }

@@ -193,6 +193,55 @@ TRIPLE_POINT = dict(dim=2, order=3, nelem=(112, 48), lo=(0.0, 0.0), hi=(7.0, 3.0
                     blending=True, alpha_max=0.5, bc=[("slip", "slip"), ("slip", "slip")])
 
 
+# The reference's 1-D Euler cases (TestCases/Euler/1D: order 3,
+# SUBCELL_FV_BLENDING, cfl 1.0, an MFEM inline segment mesh of nx elements on
+# [0, sx] refined ser_ref_levels times).  Each is a piecewise state: `states`
+# holds (x_end, rho, u, p) pieces in order, x <= x_end for "le" cases, x <
+# x_end otherwise; specified-state and inflow sides take the state the
+# initial condition gives at that end.
+SHOCK_TUBES = {
+    # SodShockTube.hpp:9-31: nx 4, sx 1, 5 levels, t 0.25
+    "sod": dict(nelem=128, hi=1.0, t_final=0.25, bc=("specified-state", "specified-state"),
+                states=((0.5, 1.0, 0.0, 1.0), (None, 0.125, 0.0, 0.1))),
+    # LaxShockTube.hpp: t 0.14
+    "lax": dict(nelem=128, hi=1.0, t_final=0.14, bc=("specified-state", "specified-state"),
+                states=((0.5, 0.445, 0.698, 3.528), (None, 0.5, 0.0, 0.571))),
+    # WoodwardColellaBlastWave.hpp: nx 4, 8 levels, slip walls, t 0.038
+    "blast": dict(nelem=1024, hi=1.0, t_final=0.038, bc=("slip", "slip"), le=True,
+                  states=((0.1, 1.0, 0.0, 1000.0), (0.9, 1.0, 0.0, 0.01), (None, 1.0, 0.0, 100.0))),
+}
+
+
+def shock_tube(name, order=3):
+    """DGSEMOperator keywords of a 1-D case of SHOCK_TUBES, and a function of
+    the node coordinates X [1][NE][K] giving its initial state."""
+    c = SHOCK_TUBES[name]
+    gamma = 1.4
+
+    def prim_at(x):
+        x = np.asarray(x, float)
+        rho, u, p = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+        done = np.zeros(x.shape, bool)
+        for x_end, r, v, q in c["states"]:
+            sel = ~done if x_end is None else ~done & ((x <= x_end) if c.get("le") else (x < x_end))
+            rho[sel], u[sel], p[sel] = r, v, q
+            done |= sel
+        return rho, u, p
+
+    def state(X):
+        rho, u, p = prim_at(X[0])
+        return prim_to_cons(rho, [u], p, gamma)
+
+    ends = []
+    for x in (0.0, c["hi"]):
+        r, u, p = (float(a[0]) for a in prim_at(np.array([x])))
+        ends.append(tuple(prim_to_cons(np.array(r), [np.array(u)], np.array(p), gamma).tolist())
+                    if c["bc"][len(ends)] in ("specified-state", "supersonic-inflow") else None)
+    kw = dict(dim=1, order=order, nelem=(c["nelem"],), lo=(0.0,), hi=(c["hi"],), gamma=gamma,
+              blending=True, alpha_max=0.5, bc=[c["bc"]], bc_state=[tuple(ends)])
+    return kw, state
+
+
 # The reference's axisymmetric Euler cases (AXISYMMETRIC build, order 1,
 # SUBCELL_FV_BLENDING with alpha_max 0.5, fixed dt), on their Gmsh meshes:
 # the 15-degree ramp (TestCases/Euler/2D/Ramp: RampIC and RampInletBCVector
@@ -306,9 +355,11 @@ def quadrature_weights(dim, order, nelem, lo, hi):
     _, w = lgl_nodes(order)
     Np = order + 1
     h = [(hi[r] - lo[r]) / nelem[r] for r in range(dim)]
-    J = h[0] * h[1] if dim == 2 else h[0] * (h[1] * h[2])
+    J = h[0] if dim == 1 else h[0] * h[1] if dim == 2 else h[0] * (h[1] * h[2])
     i = np.arange(Np ** dim)
-    wt = w[i % Np] * w[(i // Np) % Np]
+    wt = w[i % Np]
+    if dim >= 2:
+        wt = wt * w[(i // Np) % Np]
     if dim == 3:
         wt = wt * w[i // (Np * Np)]
     NE = int(np.prod([nelem[r] for r in range(dim)]))

@@ -130,6 +130,8 @@ struct VArgs {
 constexpr int cmx(int a, int b) { return a > b ? a : b; }
 constexpr int pow2_at_least(int x) { return x <= 1 ? 1 : 2 * pow2_at_least((x + 1) / 2); }
 constexpr int r64(int x) { return ((x + 63) / 64) * 64; }
+// NP^d: nodes per element (d = DIM) and per face (d = DIM - 1)
+constexpr int npow(int NP, int d) { return d <= 0 ? 1 : NP * npow(NP, d - 1); }
 
 // Launch shapes of both kernels: 0 natural, 1..3 forced blocks of 64, 128
 // and 256 threads (autotuned per case).
@@ -140,21 +142,21 @@ constexpr int shape_fb(int shape) { return shape == 1 ? 64 : shape == 2 ? 128 : 
 // workgroup.
 constexpr int visc_block(int DIM, int NP, int FB)
 {
-    const int K = DIM == 2 ? NP * NP : NP * NP * NP;
+    const int K = npow(NP, DIM);
     return FB ? FB : K > 256 ? r64(K) : 256;
 }
 constexpr bool visc_shape_ok(int DIM, int NP, int shape)
 {
     if (shape == 0) return true;
     if (shape < 0 || shape >= NSHAPES) return false;
-    const int K = DIM == 2 ? NP * NP : NP * NP * NP;
+    const int K = npow(NP, DIM);
     const int fb = shape_fb(shape);
     return K <= fb && fb != visc_block(DIM, NP, 0);
 }
 template <int DIM, int NP, int FB = 0> struct VCfg {
     static constexpr int NV = DIM + 2;
-    static constexpr int K = DIM == 2 ? NP * NP : NP * NP * NP;
-    static constexpr int NFP = DIM == 2 ? NP : NP * NP;
+    static constexpr int K = npow(NP, DIM);
+    static constexpr int NFP = npow(NP, DIM - 1);
     static constexpr int BLOCK = visc_block(DIM, NP, FB);
     static constexpr int EPB = BLOCK / K;
     // waves per SIMD to ask the register allocator for
@@ -165,6 +167,7 @@ template <int DIM, int NP, int FB = 0> struct VCfg {
 template <int DIM, int NP>
 __device__ __forceinline__ int line_node(int r, int l, int m)
 {
+    if (DIM == 1) return m;
     if (DIM == 2) return r == 0 ? l * NP + m : m * NP + l;
     const int a = l % NP, b = l / NP;
     if (r == 0) return (b * NP + a) * NP + m;
@@ -175,6 +178,7 @@ __device__ __forceinline__ int line_node(int r, int l, int m)
 template <int DIM, int NP>
 __device__ __forceinline__ int line_of(int r, int i, int j, int k)
 {
+    if (DIM == 1) return 0;
     if (DIM == 2) return r == 0 ? j : i;
     if (r == 0) return k * NP + j;
     if (r == 1) return k * NP + i;
@@ -266,8 +270,8 @@ struct RShape {
 };
 constexpr RShape rshape(int DIM, int NP, bool BLEND, int FB)
 {
-    const int K = DIM == 2 ? NP * NP : NP * NP * NP;
-    const int NFP = DIM == 2 ? NP : NP * NP;
+    const int K = npow(NP, DIM);
+    const int NFP = npow(NP, DIM - 1);
     const int JP = NFP * (NP * (NP - 1) / 2 + (BLEND ? NP - 1 : 0));
     const int WORK = cmx(K, cmx(JP, 2 * DIM * NFP));
     const int EPB = K > 128 ? 1 : cmx(1, (FB ? FB : 256) / WORK);
@@ -279,7 +283,7 @@ constexpr RShape rshape(int DIM, int NP, bool BLEND, int FB)
 constexpr bool rhs_shape_ok(int DIM, int NP, bool BLEND, int shape)
 {
     if (shape == 0) return true;
-    if (shape < 0 || shape >= NSHAPES) return false;
+    if (shape < 0 || shape >= NSHAPES || DIM == 1) return false;
     const RShape n = rshape(DIM, NP, BLEND, 0);
     const RShape f = rshape(DIM, NP, BLEND, shape_fb(shape));
     return f.EPB * f.K <= f.BLOCK && !(f.EPB == n.EPB && f.BLOCK == n.BLOCK);
@@ -287,8 +291,8 @@ constexpr bool rhs_shape_ok(int DIM, int NP, bool BLEND, int shape)
 
 template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
     static constexpr int NV = DIM + 2;
-    static constexpr int K = DIM == 2 ? NP * NP : NP * NP * NP;
-    static constexpr int NFP = DIM == 2 ? NP : NP * NP;
+    static constexpr int K = npow(NP, DIM);
+    static constexpr int NFP = npow(NP, DIM - 1);
     static constexpr int NPAIR = NP * (NP - 1) / 2;
     static constexpr int NFV = BLEND ? NP - 1 : 0;
     static constexpr int JP = NFP * (NPAIR + NFV);
@@ -310,7 +314,6 @@ template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
     // 128 registers against 4.3 ms at 180).  Otherwise PRANDTL_WAVES.
     static constexpr int WAVES = (K > 256 || BLOCK <= 128) ? 4 : PRANDTL_WAVES;
     static_assert(EPB * K <= BLOCK, "one node per thread");
-    static_assert(!BLEND || 4 * K <= NV * (JF > FXN ? JF : FXN), "indicator scratch fits the flux buffer");
 };
 
 // LDS of one k_rhs tile (EPB elements).
@@ -326,7 +329,9 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false, bool
     double Q[C::EPB][NQ][C::K];
     // signed surface contributions (surface phase), then the pair and
     // subcell fluxes of one line chunk (direction phases): one buffer
-    static constexpr int FF = C::JF > C::FXN ? C::JF : C::FXN;
+    // (also the indicator's scratch: 4 K values per element)
+    static constexpr int FF = cmx(cmx(C::JF, C::FXN), BLEND ? (4 * C::K + C::NV - 1) / C::NV : 1);
+    static_assert(!BLEND || 4 * C::K <= C::NV * FF, "indicator scratch fits the flux buffer");
     double F[C::EPB][C::NV][FF];
     double G[GE][C::NV][GK];
     double Al[C::EPB];
@@ -1817,9 +1822,9 @@ template <int DIM, int NP, int FB> void launch_visc_fb(prandtl_ctx *c, const VAr
 }
 
 // Orders the general-geometry kernels are built for: 2-D N = 1..5, 3-D N = 2..4.
-constexpr bool gen_built(int DIM, int NP) { return DIM == 2 ? NP <= 6 : NP <= 5; }
+constexpr bool gen_built(int DIM, int NP) { return DIM == 2 ? NP <= 6 : DIM == 3 && NP <= 5; }
 
-template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
+template <int DIM, int NP> int launch_visc_nd(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
 {
     VArgs a;
     a.tab = c->tab; a.geo = c->geo; a.phys = c->phys;
@@ -1862,6 +1867,14 @@ template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, doub
         break;
     }
     return hip_check(c, hipGetLastError(), "k_visc launch");
+}
+
+// 1-D operators are Euler operators (prandtl_create refuses parabolic ones):
+// no viscous kernel is built for them.
+template <int DIM, int NP> int launch_visc(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
+{
+    if constexpr (DIM == 1) return fail(c, PRANDTL_ERR_UNSUPPORTED, "no viscous kernel in 1-D");
+    else return launch_visc_nd<DIM, NP>(c, y, grad, s);
 }
 
 template <int DIM, int NP, bool BLEND, int FB>
@@ -1983,6 +1996,13 @@ int stage_t(prandtl_ctx *c, const double *y, const double *u0, double *out, int 
 
 #define PRANDTL_DISPATCH(FN, ...)                                                   \
     switch (c->dim * 16 + c->Np) {                                                  \
+    case 1 * 16 + 2: return FN<1, 2>(__VA_ARGS__);                                  \
+    case 1 * 16 + 3: return FN<1, 3>(__VA_ARGS__);                                  \
+    case 1 * 16 + 4: return FN<1, 4>(__VA_ARGS__);                                  \
+    case 1 * 16 + 5: return FN<1, 5>(__VA_ARGS__);                                  \
+    case 1 * 16 + 6: return FN<1, 6>(__VA_ARGS__);                                  \
+    case 1 * 16 + 7: return FN<1, 7>(__VA_ARGS__);                                  \
+    case 1 * 16 + 8: return FN<1, 8>(__VA_ARGS__);                                  \
     case 2 * 16 + 2: return FN<2, 2>(__VA_ARGS__);                                  \
     case 2 * 16 + 3: return FN<2, 3>(__VA_ARGS__);                                  \
     case 2 * 16 + 4: return FN<2, 4>(__VA_ARGS__);                                  \
@@ -2059,9 +2079,11 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (!mesh || !prm || !out) return fail(nullptr, PRANDTL_ERR_ARG, "null argument");
     *out = nullptr;
     const int dim = mesh->dim, N = mesh->order;
-    if (dim < 2 || dim > 3) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "dim must be 2 or 3");
-    if (N < (dim == 2 ? 1 : 2) || N > (dim == 2 ? 5 : 7))
+    if (dim < 1 || dim > 3) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "dim must be 1, 2 or 3");
+    if (N < (dim == 3 ? 2 : 1) || N > (dim == 2 ? 5 : 7))
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "order not built for this dim");
+    if (dim == 1 && prm->parabolic)
+        return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "1-D is built for the Euler equations");
     if (prm->precision != PRANDTL_FP64 && prm->precision != PRANDTL_MIXED)
         return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "precision must be PRANDTL_FP64 or PRANDTL_MIXED");
     if (prm->precision == PRANDTL_MIXED && prm->parabolic && prm->quirks)
@@ -2160,8 +2182,8 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (!c) return fail(nullptr, PRANDTL_ERR_NOMEM, "host allocation");
     c->dim = dim; c->order = N; c->Np = N + 1; c->nv = dim + 2;
     c->mesh = *mesh;
-    c->K = dim == 2 ? c->Np * c->Np : c->Np * c->Np * c->Np;
-    c->Nfp = dim == 2 ? c->Np : c->Np * c->Np;
+    c->K = dim == 1 ? c->Np : dim == 2 ? c->Np * c->Np : c->Np * c->Np * c->Np;
+    c->Nfp = dim == 1 ? 1 : dim == 2 ? c->Np : c->Np * c->Np;
     c->NE = (int)NE;
     c->prm = *prm;
     const int Np = c->Np;
@@ -2251,7 +2273,11 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     for (int r = dim; r < 3; r++) c->geo.n[r] = 1;
     c->geo.NE = c->NE;
     double J;
-    if (dim == 2) {
+    if (dim == 1) {
+        // the adjugate of a 1 x 1 Jacobian is 1: unit face normals
+        J = h[0];
+        c->geo.nrm[0] = 1.0; c->geo.nrm[1] = 0.0; c->geo.nrm[2] = 0.0;
+    } else if (dim == 2) {
         J = h[0] * h[1];
         c->geo.nrm[0] = h[1]; c->geo.nrm[1] = h[0]; c->geo.nrm[2] = 0.0;
     } else {
@@ -2583,7 +2609,9 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
                     hipLaunchKernelGGL(k_axis, dim3(nb), dim3(256), 0, s, u, c->d_axU, c->geo, c->K, c->skip);
                     uc = c->d_axU;
                 }
-                if (c->dim == 2)
+                if (c->dim == 1)
+                    hipLaunchKernelGGL(k_loop_check<1>, dim3(nb), dim3(256), 0, s, c->d_loop, uc, NK, c->phys.gm1, check_every);
+                else if (c->dim == 2)
                     hipLaunchKernelGGL(k_loop_check<2>, dim3(nb), dim3(256), 0, s, c->d_loop, uc, NK, c->phys.gm1, check_every);
                 else
                     hipLaunchKernelGGL(k_loop_check<3>, dim3(nb), dim3(256), 0, s, c->d_loop, uc, NK, c->phys.gm1, check_every);
@@ -2671,7 +2699,8 @@ int prandtl_check_state(prandtl_ctx *c, const double *u, void *stream)
         hipLaunchKernelGGL(k_axis, dim3(nb), dim3(256), 0, s, u, c->d_axU, c->geo, c->K, nullptr);
         u = c->d_axU;
     }
-    if (c->dim == 2) hipLaunchKernelGGL(k_check<2>, dim3(nb), dim3(256), 0, s, u, NK, c->phys.gm1, c->d_flag);
+    if (c->dim == 1) hipLaunchKernelGGL(k_check<1>, dim3(nb), dim3(256), 0, s, u, NK, c->phys.gm1, c->d_flag);
+    else if (c->dim == 2) hipLaunchKernelGGL(k_check<2>, dim3(nb), dim3(256), 0, s, u, NK, c->phys.gm1, c->d_flag);
     else hipLaunchKernelGGL(k_check<3>, dim3(nb), dim3(256), 0, s, u, NK, c->phys.gm1, c->d_flag);
     int flag = 0;
     int rc = hip_check(c, hipGetLastError(), "k_check launch");
@@ -2703,7 +2732,8 @@ int prandtl_primitives(prandtl_ctx *c, const double *u, double *prim, void *stre
         if (rc) return rc;
         u = c->d_axU;
     }
-    if (c->dim == 2) hipLaunchKernelGGL(k_primitives<2>, dim3(nb), dim3(256), 0, s, u, prim, NK, c->phys.gm1);
+    if (c->dim == 1) hipLaunchKernelGGL(k_primitives<1>, dim3(nb), dim3(256), 0, s, u, prim, NK, c->phys.gm1);
+    else if (c->dim == 2) hipLaunchKernelGGL(k_primitives<2>, dim3(nb), dim3(256), 0, s, u, prim, NK, c->phys.gm1);
     else hipLaunchKernelGGL(k_primitives<3>, dim3(nb), dim3(256), 0, s, u, prim, NK, c->phys.gm1);
     return hip_check(c, hipGetLastError(), "k_primitives launch");
 }

@@ -269,7 +269,10 @@ __device__ __forceinline__ double slip_wall_flux(int r, double ns, const double 
 {
     const double u = ns * (1.0 / fabs(ns));
     double m[DIM];
-    if (DIM == 2) {
+    if (DIM == 1) {
+        // 1-D: state2(1) * nor(0) (SlipWallBdrFaceIntegrator.cpp:22-30)
+        m[0] = s[1] * ns;
+    } else if (DIM == 2) {
         if (r == 0) { m[0] = s[1] * u; m[1] = s[2] * u; }
         else { m[0] = s[2] * u; m[1] = -(s[1] * u); }
     } else {
