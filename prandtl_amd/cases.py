@@ -212,6 +212,23 @@ SHOCK_TUBES = {
 }
 
 
+# AcousticWave (TestCases/Euler/1D/AcousticWave: AcousticWave.hpp:9-33,
+# order 3, blending, the periodic 4-element segment refined 6 times, t 1.0):
+# rho = 1.225 + 0.2 sin(2 pi x) carried at u = M a (M = 0.5, a from p = 1,
+# rho = 1.225) through constant pressure; the exact solution is the same
+# profile shifted by u t (AcousticWaveExactSolution, lines 35-71).
+ACOUSTIC_WAVE = dict(dim=1, order=3, nelem=(256,), lo=(0.0,), hi=(1.0,), gamma=1.4, blending=True,
+                     alpha_max=0.5, t_final=1.0)
+
+
+def acoustic_wave(X, t=0.0, gamma=1.4):
+    """AcousticWaveIC at t = 0, its exact solution at t (periodic on [0, 1])."""
+    u = 0.5 * np.sqrt(gamma * 1.0 / 1.225)
+    x = np.asarray(X[0], float)
+    rho = 1.225 + 0.2 * np.sin(2.0 * np.pi * (x - u * t))
+    return prim_to_cons(rho, [np.full_like(x, u)], np.ones_like(x), gamma)
+
+
 def shock_tube(name, order=3):
     """DGSEMOperator keywords of a 1-D case of SHOCK_TUBES, and a function of
     the node coordinates X [1][NE][K] giving its initial state."""

@@ -102,6 +102,66 @@ def _stream_ptr():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def read_segment_mesh(path, refine=0):
+    """The box of a 1-D MFEM mesh file (the reference's TestCases/Euler/1D
+    meshes) after `refine` uniform refinements: dict(dim=1, nelem=(n,),
+    lo=(a,), hi=(b,), periodic=bool).  Reads "MFEM INLINE mesh v1.0" with
+    type = segment, nx, sx (the segment [0, sx]) and "MFEM mesh v1.0" of
+    dimension 1 with uniform elements, periodic when it has no boundary
+    points (its elements then close a cycle and L2_T1_1D_P1 nodes give their
+    coordinates).  Raises ValueError for anything else."""
+    with open(path) as f:
+        lines = [ln.split("#")[0].strip() for ln in f]
+    lines = [ln for ln in lines if ln]
+    if not lines:
+        raise ValueError("%s: empty mesh file" % path)
+    if lines[0].startswith("MFEM INLINE mesh"):
+        kv = {}
+        for ln in lines[1:]:
+            if "=" in ln:
+                k, v = (s.strip() for s in ln.split("=", 1))
+                kv[k] = v
+        if kv.get("type") != "segment":
+            raise ValueError("%s: inline mesh type %r is not a segment" % (path, kv.get("type")))
+        n, sx = int(kv["nx"]), float(kv.get("sx", 1.0))
+        return dict(dim=1, nelem=(n << refine,), lo=(0.0,), hi=(sx,), periodic=False)
+    if not lines[0].startswith("MFEM mesh v1.0"):
+        raise ValueError("%s: not an MFEM v1.0 mesh" % path)
+
+    def section(name):
+        if name not in lines:
+            raise ValueError("%s: no %s section" % (path, name))
+        return lines.index(name) + 1
+
+    i = section("dimension")
+    if int(lines[i]) != 1:
+        raise ValueError("%s: not a 1-D mesh" % path)
+    i = section("elements")
+    ne = int(lines[i])
+    elems = [tuple(int(t) for t in lines[i + 1 + k].split()[2:4]) for k in range(ne)]
+    nb = int(lines[section("boundary")])
+    if "nodes" in lines:
+        j = lines.index("nodes") + 1
+        while not lines[j][0].isdigit() and lines[j][0] not in "-.":
+            if "L2_T1_1D_P1" not in lines[j] and ":" in lines[j] and lines[j].startswith("FiniteElementCollection"):
+                raise ValueError("%s: nodes of %s are not read" % (path, lines[j]))
+            j += 1
+        x = np.array([[float(t) for t in lines[j + k].split()] for k in range(ne)])
+    else:
+        j = section("vertices")
+        nv, sd = int(lines[j]), int(lines[j + 1])
+        if sd != 1:
+            raise ValueError("%s: vertices are not 1-D" % path)
+        v = np.array([float(lines[j + 2 + k]) for k in range(nv)])
+        x = np.array([[v[a], v[b]] for a, b in elems])
+    lo, hi = float(x.min()), float(x.max())
+    h = (hi - lo) / ne
+    if not np.allclose(np.sort(x[:, 0]), lo + h * np.arange(ne), rtol=0, atol=1e-12 * (hi - lo)) or \
+            not np.allclose(x[:, 1] - x[:, 0], h, rtol=1e-12, atol=0):
+        raise ValueError("%s: elements are not a uniform left-to-right segment" % path)
+    return dict(dim=1, nelem=(ne << refine,), lo=(lo,), hi=(hi,), periodic=nb == 0)
+
+
 class QuadMesh:
     """A 2-D quadrilateral mesh: vert [nv,2], elem [ne,4] (counterclockwise),
     optional elem_coords [ne,4,2] (per-element corners of periodic meshes),

@@ -7,8 +7,9 @@ and with specified-state, inflow, outflow and slip ends; the wave speed
 within 1e-13; RK3 steps within 1e-12.  At the reference cases' full sizes
 the checks are size-independent: Sod's tube run to t = 0.25 on the device
 loop stays within the exact Riemann solution's L1 bound the oracle meets,
-and the Woodward-Colella blast wave (1024 elements, slip walls, t = 0.038)
-keeps its total mass and energy to rounding.
+the Woodward-Colella blast wave (1024 elements, slip walls, t = 0.038)
+keeps its total mass and energy to rounding, and the AcousticWave case
+(256 elements, t = 1) stays on its exact solution.
 """
 import numpy as np
 import pytest
@@ -147,3 +148,26 @@ def test_blast_wave_full_run_conserves_mass_and_energy(cuda):
         assert abs(m1 - m0) <= 1e-11 * abs(m0)
     # the two blast waves have collided: the density peak exceeds 5
     assert U[0].max() > 5.0
+
+
+def test_acoustic_wave_full_run_matches_exact_solution(cuda):
+    """AcousticWave at the reference's size (order 3, blending, 256
+    elements, cfl 1, t = 1.0) on the device loop: the wave is smooth, so no
+    element blends, and the density stays within 1e-9 (L2) of the exact
+    advected profile (the oracle's run: 6977 steps, 3.5e-11)."""
+    import torch
+    c = dict(cases.ACOUSTIC_WAVE)
+    T = c.pop("t_final")
+    op = DGSEMOperator(**c)
+    X = cases.node_coords(1, c["order"], c["nelem"], c["lo"], c["hi"])
+    u = to_dev(cases.acoustic_wave(X), cuda)
+    t, n = op.Run(u, 0.0, T, cfl=1.0)
+    torch.cuda.synchronize()
+    U = u.cpu().numpy().reshape(3, -1)
+    ex = cases.acoustic_wave(X, t).reshape(3, -1)
+    W = cases.quadrature_weights(1, c["order"], c["nelem"], c["lo"], c["hi"]).reshape(-1)
+    l2 = float(np.sqrt(np.sum(W * (U[0] - ex[0]) ** 2)))
+    print("acoustic wave: %d steps to t=%.17g, L2(rho) = %g" % (n, t, l2), flush=True)
+    assert abs(t - T) <= 1e-14
+    assert (op.alpha()[0] == 0).all()
+    assert l2 <= 1e-9

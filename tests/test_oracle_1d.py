@@ -16,6 +16,10 @@ operator.
 - Sod's shock tube run to t = 0.25 (the reference case's settings: order 3,
   blending, 128 elements, cfl 1) stays close to the exact Riemann solution
   (Toro's solver), and Lax's to t = 0.14.
+- The AcousticWave case's density wave (constant u and p) converges to its
+  exact solution rho(x - u t) at order N + 1.
+- The reference's 1-D mesh files (inline segment, periodic L2 nodes) read
+  as the boxes of their cases.
 """
 import numpy as np
 import pytest
@@ -135,3 +139,40 @@ def test_shock_tube_close_to_exact_riemann(name, l1max):
     print(name, "L1(rho) =", l1, flush=True)
     assert np.isfinite(U).all() and (U[0] > 0).all()
     assert l1 <= l1max
+
+
+def acoustic_error(nelem, order, T):
+    o = O.Oracle(1, order, [nelem], [0.0], [1.0], blending=False)
+    X = o.coords()
+    u = cases.acoustic_wave(X).reshape(-1)
+    n = int(np.ceil(T / (0.2 / nelem / (order + 1) ** 2 / 2.0)))
+    dt = T / n
+    for _ in range(n):
+        u = o.Step(u, dt)
+    ex = cases.acoustic_wave(X, T)
+    W = o.weights()
+    return float(np.sqrt(np.sum(W * (u.reshape(3, nelem, -1)[0] - ex[0]) ** 2)))
+
+
+def test_acoustic_wave_converges_at_order_n_plus_1():
+    """The density wave of AcousticWave.hpp carried at constant u and p has
+    the exact solution rho(x - u t): the L2 error falls as h^(N+1)."""
+    errs = [acoustic_error(n, 3, 0.2) for n in (4, 8, 16)]
+    rates = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    print("acoustic wave L2 errors", errs, "rates", rates, flush=True)
+    assert rates[-1] > 3.6 and errs[-1] < 1e-5
+
+
+def test_reads_reference_segment_meshes():
+    """The AcousticWave mesh (periodic, L2 nodes) and Sod's inline mesh give
+    the boxes of their cases after the configs' ser_ref_levels."""
+    import os
+    from prandtl_amd.operator import read_segment_mesh
+    d = os.path.join(os.path.dirname(__file__), "golden", "meshes")
+    a = read_segment_mesh(os.path.join(d, "AcousticWave.mesh"), refine=6)
+    assert a == dict(dim=1, nelem=(256,), lo=(0.0,), hi=(1.0,), periodic=True)
+    s = read_segment_mesh(os.path.join(d, "SodShockTube.mesh"), refine=5)
+    assert s == dict(dim=1, nelem=(128,), lo=(0.0,), hi=(1.0,), periodic=False)
+    assert s["nelem"] == cases.shock_tube("sod")[0]["nelem"]
+    with pytest.raises(ValueError):
+        read_segment_mesh(os.path.join(d, "IsentropicVortex.mesh"))
