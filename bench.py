@@ -186,7 +186,8 @@ def cpu_baseline(name, cfg, quirks, u, budget_s):
         orc.Mult(u)
         n += 1
         el = time.perf_counter() - t0
-        if el > budget_s or n >= 50:
+        # at least 50 evaluations and a second of work, at most the budget
+        if el > budget_s or (n >= 50 and el >= 1.0):
             break
     return {"value": nodes * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
             "sample": "%d RHS evaluations of %s (%d nodes) in %.1f s, OpenMP over elements/faces"
