@@ -19,9 +19,10 @@
  *
  * Geometry is general per-node (J, adjugate rows, face normals, subcell
  * normals) so the restatement follows the reference's metric use; it is
- * built for Cartesian boxes (periodic or with boundary sides), quadrilateral
- * meshes (bilinear) and hexahedral meshes (trilinear), and the axisymmetric
- * mode recovers U from r U (RecoverStateFromWeighted).
+ * built for Cartesian boxes (periodic or with boundary sides; 1-D segments
+ * with the Euler equations included), quadrilateral meshes (bilinear) and
+ * hexahedral meshes (trilinear), and the axisymmetric mode recovers U from
+ * r U (RecoverStateFromWeighted).
  *
  * Pins (tests/): each function is checked against closed forms, invariants
  * or equivalences that a plausible slip breaks — test_oracle.py (fluxes,
@@ -29,7 +30,9 @@
  * fluxes and lifting), test_oracle_quad.py and test_oracle_hex.py (mesh
  * geometry: equality with the box operator, free stream, conservation,
  * rotation and relabelling), test_oracle_axisym.py (recovery closed forms,
- * exact steady states, conservation, convergence).  Parity unpinned: the
+ * exact steady states, conservation, convergence), test_oracle_1d.py (1-D:
+ * equality with the y-uniform 2-D box, free stream, conservation, the 1-D
+ * slip wall, exact Riemann solutions, convergence).  Parity unpinned: the
  * last bits that MFEM's own LGL rounding, dense inverse and RK3 axpy order
  * would fix (MFEM is not in the reference tree).
  */
