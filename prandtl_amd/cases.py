@@ -197,8 +197,8 @@ TRIPLE_POINT = dict(dim=2, order=3, nelem=(112, 48), lo=(0.0, 0.0), hi=(7.0, 3.0
 # SUBCELL_FV_BLENDING, cfl 1.0, an MFEM inline segment mesh of nx elements on
 # [0, sx] refined ser_ref_levels times).  Each is a piecewise state: `states`
 # holds (x_end, rho, u, p) pieces in order, x <= x_end for "le" cases, x <
-# x_end otherwise; specified-state and inflow sides take the state the
-# initial condition gives at that end.
+# x_end otherwise (rho may be a function of x); specified-state and inflow
+# sides take the state the initial condition gives at that end.
 SHOCK_TUBES = {
     # SodShockTube.hpp:9-31: nx 4, sx 1, 5 levels, t 0.25
     "sod": dict(nelem=128, hi=1.0, t_final=0.25, bc=("specified-state", "specified-state"),
@@ -206,6 +206,12 @@ SHOCK_TUBES = {
     # LaxShockTube.hpp: t 0.14
     "lax": dict(nelem=128, hi=1.0, t_final=0.14, bc=("specified-state", "specified-state"),
                 states=((0.5, 0.445, 0.698, 3.528), (None, 0.5, 0.0, 0.571))),
+    # ShuOsherShock.hpp: nx 16, sx 10, 4 levels, supersonic inflow on the
+    # left, outflow on the right, t 1.8; the density ahead of the shock is
+    # 1 + 0.2 sin(5 (x - 5))
+    "shu_osher": dict(nelem=256, hi=10.0, t_final=1.8, bc=("supersonic-inflow", "supersonic-outflow"),
+                      states=((1.0, 3.857, 2.629, 10.333),
+                              (None, lambda x: 1.0 + 0.2 * np.sin(5.0 * (x - 5.0)), 0.0, 1.0))),
     # WoodwardColellaBlastWave.hpp: nx 4, 8 levels, slip walls, t 0.038
     "blast": dict(nelem=1024, hi=1.0, t_final=0.038, bc=("slip", "slip"), le=True,
                   states=((0.1, 1.0, 0.0, 1000.0), (0.9, 1.0, 0.0, 0.01), (None, 1.0, 0.0, 100.0))),
@@ -241,7 +247,7 @@ def shock_tube(name, order=3):
         done = np.zeros(x.shape, bool)
         for x_end, r, v, q in c["states"]:
             sel = ~done if x_end is None else ~done & ((x <= x_end) if c.get("le") else (x < x_end))
-            rho[sel], u[sel], p[sel] = r, v, q
+            rho[sel], u[sel], p[sel] = r(x[sel]) if callable(r) else r, v, q
             done |= sel
         return rho, u, p
 
