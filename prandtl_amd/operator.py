@@ -141,9 +141,11 @@ def read_segment_mesh(path, refine=0):
     elems = [tuple(int(t) for t in lines[i + 1 + k].split()[2:4]) for k in range(ne)]
     nb = int(lines[section("boundary")])
     if "nodes" in lines:
+        # header lines (FiniteElementSpace, FiniteElementCollection, VDim,
+        # Ordering) up to the first coordinate line
         j = lines.index("nodes") + 1
-        while not lines[j][0].isdigit() and lines[j][0] not in "-.":
-            if "L2_T1_1D_P1" not in lines[j] and ":" in lines[j] and lines[j].startswith("FiniteElementCollection"):
+        while not (lines[j][0].isdigit() or lines[j][0] in "-."):
+            if lines[j].startswith("FiniteElementCollection") and "L2_T1_1D_P1" not in lines[j]:
                 raise ValueError("%s: nodes of %s are not read" % (path, lines[j]))
             j += 1
         x = np.array([[float(t) for t in lines[j + k].split()] for k in range(ne)])
