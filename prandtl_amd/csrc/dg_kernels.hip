@@ -25,6 +25,8 @@
 // order.  Face jobs are computed by both neighbouring elements ("owner
 // computes") with the face's canonical orientation, so each face point's flux
 // is bitwise the same on both sides and no atomics or scatter are needed.
+// k_rhs is instantiated for DIM = 1, 2 and 3 (1-D: one line per element, a
+// single point per face, unit face normals; Euler only, so no k_visc).
 //
 // Both kernels are also built for forced workgroup sizes (64, 128 or 256
 // threads, "launch shapes"); prandtl_autotune picks one per context by
