@@ -62,6 +62,12 @@ def _create(mesh_kw, prm_kw):
 
 @pytest.mark.parametrize("mesh_kw,prm_kw,code", [
     (dict(dim=4), {}, _lib.PRANDTL_ERR_UNSUPPORTED),
+    (dict(dim=0), {}, _lib.PRANDTL_ERR_UNSUPPORTED),
+    # 1-D: Euler only, orders 1..7; a 1-D side needs an outer state with
+    # positive density and pressure like any other
+    (dict(dim=1), dict(parabolic=1), _lib.PRANDTL_ERR_UNSUPPORTED),
+    (dict(dim=1, order=8), {}, _lib.PRANDTL_ERR_UNSUPPORTED),
+    (dict(dim=1, bc=((5, 5), (0, 0), (0, 0)), bc_state=(1.0, 0.0, -2.5, 0.0, 0.0)), {}, _lib.PRANDTL_ERR_ARG),
     (dict(dim=2, order=6), {}, _lib.PRANDTL_ERR_UNSUPPORTED),
     ({}, dict(parabolic=1, precision=1), _lib.PRANDTL_ERR_UNSUPPORTED),
     (dict(dim=3, order=1), {}, _lib.PRANDTL_ERR_UNSUPPORTED),
