@@ -202,9 +202,7 @@ def test_entropy_gradient_to_primitive_gradient(dim):
     for _ in range(50):
         u = cons(rng.uniform(0.3, 3), rng.normal(size=dim), rng.uniform(0.3, 3))
         du = rng.normal(size=dim + 2)
-        h = 1e-30
-        dw = np.array([O.cons2entropy(dim, G, u + 1j * 0)[k] for k in range(dim + 2)])  # noqa: F841
-        # complex step of w(u) along du
+        # central differences of w(u) and of the primitives along du
         wc = lambda uu: O.cons2entropy(dim, G, uu)
         eps = 1e-6
         dwdx = (wc(u + eps * du) - wc(u - eps * du)) / (2 * eps)
