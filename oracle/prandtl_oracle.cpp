@@ -1421,7 +1421,7 @@ template <class T> struct Engine {
                 for (int cc = 0; cc < dim; cc++) {
                     T g[5];
                     for (int v = 0; v < nv; v++) g[v] = G[(cc * nv + v) * NK + (size_t)e * K + i];
-                    entgrad2primgrad(dim, gamma, c->p.quirks, st, g);
+                    entgrad2primgrad(dim, gamma, c->p.quirks && !(c->p.plant & ORC_PLANT_GRAD_NO_ALIAS), st, g);
                     for (int v = 0; v < nv; v++) G[(cc * nv + v) * NK + (size_t)e * K + i] = g[v];
                 }
             }
@@ -1488,7 +1488,11 @@ template <class T> struct Engine {
         if (par) {
             g_cat = CAT_VISC;
             for (int i = 0; i < K; i++) {
-                int src = quirks ? K - 1 : i; /* stale member id1 (line 620) */
+                /* stale member id1 (line 620): the last node of the previous
+                 * element loop, or of ComputeFVFluxes' zeta sweep (lines
+                 * 1165-1190), both K-1 */
+                const int stale = (c->p.plant & ORC_PLANT_STALE_NODE0) ? 0 : K - 1;
+                int src = quirks ? stale : i;
                 T st[5], dq[3][5], fl[5 * 3];
                 state(u, e, src, st);
                 const T *dqp[3];
@@ -1544,10 +1548,11 @@ template <class T> struct Engine {
                             for (int v = 0; v < nv; v++) {
                                 a[v] = f[v];
                                 /* NS-3D: H_inviscid(id2).SetCol(k, g) (line 702) */
-                                b[v] = (par && quirks && dim == 3 && r == 2) ? glast[v] : f[v];
+                                b[v] = (par && quirks && dim == 3 && r == 2 && !(c->p.plant & ORC_PLANT_ZETA_F))
+                                           ? glast[v] : f[v];
                             }
                         }
-                        if (r == 1)
+                        if (r == ((c->p.plant & ORC_PLANT_ZETA_XI) ? 0 : 1))
                             for (int v = 0; v < nv; v++) glast[v] = f[v];
                     }
                     g_cat = CAT_ASM;
@@ -1632,7 +1637,7 @@ template <class T> struct Engine {
                             fn[v] = (cc == 0) ? avg * nor[0] : fn[v] + avg * nor[cc];
                         }
                     }
-                    if (quirks) {
+                    if (quirks && !(c->p.plant & ORC_PLANT_FACE_NO_ALIAS)) {
                         /* ComputeViscousFlux -> Conserv2Prim -> ComputePressure
                          * (NavierStokesFlux.cpp:14, BasicOperations.cpp:33-34,185)
                          * divides the momentum slots of state1/state2 in place,
@@ -2050,7 +2055,8 @@ static int create_impl(const orc_params *p, const orc_quad_mesh *qm, orc_ctx **o
     c->alpha.assign(m.NE, 0.0);
     c->eta.assign(m.NE, 0.0);
     c->lam = -1.0; /* DGSEMIntegrator.cpp:39 */
-    c->thresh = 0.5 * std::pow(10.0, -1.8 * std::pow((double)p->order, 0.25)); /* DGSEMOperator.cpp:25 */
+    const double tdeg = p->order + ((p->plant & ORC_PLANT_THRESH_NP1) ? 1 : 0);
+    c->thresh = 0.5 * std::pow(10.0, -1.8 * std::pow(tdeg, 0.25)); /* DGSEMOperator.cpp:25 */
     /* ModalBasis (ModalBasis.cpp:42-135): mode n <-> per-dim degrees,
      * V(node, n) = prod_d P_deg(x_d); V^-1 = kron of the 1-D inverses */
     const int Np = m.Np, K = m.K, dim = m.dim;

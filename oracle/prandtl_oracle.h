@@ -46,7 +46,22 @@ typedef struct orc_params {
      * reconstruction (rho_floor_abs, DGSEMOperator.hpp:44; <= 0: 1e-12) */
     int axisymmetric;
     double axis_rho_floor;
+    /* TEST-ONLY: a bitmask of ORC_PLANT_* misreadings of the faithful-mode
+     * branches and of the indicator.  The pins of tests/test_oracle_faithful.py
+     * plant each one and show that they fail on it; every other use leaves
+     * it 0. */
+    int plant;
 } orc_params;
+
+/* planted misreadings (orc_params.plant) */
+enum {
+    ORC_PLANT_STALE_NODE0 = 1,     /* A-3: the viscous volume flux takes node 0's state, not node K-1's */
+    ORC_PLANT_ZETA_F = 2,          /* A-2: the zeta partner's column takes its own flux h, not g */
+    ORC_PLANT_ZETA_XI = 4,         /* A-2: g is the last xi-direction flux, not the last eta one */
+    ORC_PLANT_FACE_NO_ALIAS = 8,   /* A-8: the Navier-Stokes face flux reads momenta, not velocities */
+    ORC_PLANT_GRAD_NO_ALIAS = 16,  /* A-8: EntropyGrad2PrimGrad's density row multiplies momenta */
+    ORC_PLANT_THRESH_NP1 = 32      /* indicator threshold 0.5 10^(-1.8 (N+1)^0.25) instead of N */
+};
 
 enum {
     ORC_BC_PERIODIC = 0,

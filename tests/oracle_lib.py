@@ -27,7 +27,13 @@ class OrcParams(ctypes.Structure):
         ("sutherland", ctypes.c_int), ("R_gas", ctypes.c_double),
         ("bc", (ctypes.c_int * 2) * 3), ("bc_state", ((ctypes.c_double * 5) * 2) * 3),
         ("axisymmetric", ctypes.c_int), ("axis_rho_floor", ctypes.c_double),
+        ("plant", ctypes.c_int),
     ]
+
+
+# planted misreadings (orc_params.plant; tests/test_oracle_faithful.py only)
+PLANT_STALE_NODE0, PLANT_ZETA_F, PLANT_ZETA_XI = 1, 2, 4
+PLANT_FACE_NO_ALIAS, PLANT_GRAD_NO_ALIAS, PLANT_THRESH_NP1 = 8, 16, 32
 
 
 # boundary types by the names of the reference's configuration files
@@ -199,7 +205,7 @@ class Oracle:
     def __init__(self, dim, order, nelem, lo, hi, gamma=1.4, Pr=0.72, mu=0.0,
                  parabolic=False, blending=False, alpha_max=0.5, alpha_min=0.001,
                  ec_surface=False, quirks=True, threads=1, sutherland=False, R_gas=287.05,
-                 bc=None, bc_state=None):
+                 bc=None, bc_state=None, plant=0):
         p = OrcParams()
         p.dim, p.order = dim, order
         for r in range(3):
@@ -212,6 +218,7 @@ class Oracle:
         p.ec_surface, p.quirks, p.threads = int(ec_surface), int(quirks), int(threads)
         p.sutherland, p.R_gas = int(sutherland), float(R_gas)
         set_bcs(p, dim, bc, bc_state)
+        p.plant = int(plant)
         self.params = p
         h = ctypes.c_void_p()
         rc = lib().orc_create(ctypes.byref(p), ctypes.byref(h))

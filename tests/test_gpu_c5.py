@@ -1,0 +1,138 @@
+"""The large Taylor-Green configuration: 3-D Navier-Stokes with BR1 lifting,
+the Chandrashekar volume flux and subcell-FV blending at alpha_max = 0.1, at
+polynomial degree N = 7 (DGSEMIntegrator.cpp:601-750 with :972-1196,
+DGSEMOperator.cpp:60-80).
+
+ - Parity: the N = 7 kernels with Navier-Stokes AND blending on a 2^3 mesh,
+   faithful and corrected, against the FP64 oracle at 1e-12 relative
+   max-norm, on a Taylor-Green state with a density/pressure jump inside the
+   x-low elements, so that some elements blend (0 < alpha <= 0.1) and
+   others do not; alpha and eta against the oracle's.
+ - The full size, 32^3 hexahedra at N = 7 (16.8 M nodes): five RK3 steps of
+   the FP64 and the mixed form stay physical, conserve mass, momentum and
+   energy to round-off (periodic box), and the mixed form's kinetic-energy
+   dissipation rate -dE_k/dt stays within 1e-4 relative of the FP64 form's.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from prandtl_amd import DGSEMOperator, cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+L = 2.0 * math.pi
+C5 = dict(dim=3, order=7, lo=(0.0,) * 3, hi=(L,) * 3, parabolic=True, mu=1.0 / 1600.0, Pr=0.71,
+          blending=True, alpha_max=0.1)
+
+
+def rel_max(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def to_dev(u, dev):
+    import torch
+    return torch.tensor(np.ascontiguousarray(u).reshape(-1), dtype=torch.float64, device=dev)
+
+
+def tgv_with_jump(X, gamma=1.4):
+    """Taylor-Green state (Ma 0.1) whose density and pressure jump by 20% at
+    x = 0.31 L, inside the elements of the low x half."""
+    u = cases.taylor_green(X, gamma)
+    jump = 1.0 + 0.2 * (X[0] > 0.31 * L)
+    rho = u[0] * jump
+    vel = [u[1 + d] / u[0] for d in range(3)]
+    p = (gamma - 1.0) * (u[4] - 0.5 * u[0] * sum(v * v for v in vel)) * jump
+    return cases.prim_to_cons(rho, vel, p, gamma)
+
+
+@pytest.mark.parametrize("quirks", [True, False])
+def test_c5_ns_blend_n7_parity(cuda, quirks):
+    import torch
+    cfg = dict(C5, nelem=(2, 2, 2), quirks=quirks)
+    op = DGSEMOperator(**cfg)
+    orc = O.Oracle(threads=8, **cfg)
+    X = cases.node_coords(3, 7, cfg["nelem"], cfg["lo"], cfg["hi"])
+    for name, u in (("jump", tgv_with_jump(X)),
+                    ("random", cases.random_smooth_state(X, 3, 29, L=[L] * 3))):
+        du = to_dev(u, cuda)
+        out = torch.empty_like(du)
+        op.Mult(du, out)
+        torch.cuda.synchronize()
+        ref = orc.Mult(u)
+        r = rel_max(out.cpu().numpy(), ref)
+        a_gpu, e_gpu = op.alpha()
+        a_ref, e_ref = orc.alpha(), orc.eta()
+        print("C5 N=7 NS+blend quirks=%d %s: rhs %.3e alpha %s eta max|d| %.2e" % (
+            quirks, name, r, np.array2string(a_ref, precision=4), np.abs(e_gpu - e_ref).max()), flush=True)
+        assert r <= TOL, (name, r)
+        assert np.abs(e_gpu - e_ref).max() <= 1e-12
+        assert np.abs(a_gpu - a_ref).max() <= 1e-10
+        assert a_ref.max() <= 0.1
+        if name == "jump":
+            # the jump blends the x-low elements; the smooth ones stay unblended
+            assert (a_ref > 0).sum() > 0 and (a_ref == 0).sum() > 0
+    # one RK3 step
+    u = tgv_with_jump(X)
+    du = to_dev(u, cuda)
+    op.Step(du, 1e-3)
+    torch.cuda.synchronize()
+    r = rel_max(du.cpu().numpy(), orc.Step(u.reshape(-1).copy(), 1e-3))
+    print("C5 N=7 quirks=%d RK3 step %.3e" % (quirks, r), flush=True)
+    assert r <= TOL
+
+
+def test_c5_large_tgv_steps(cuda):
+    """32^3 N=7, both forms: physical, conservative, mixed -dE_k/dt ~ FP64."""
+    import torch
+    cfg = dict(C5, nelem=(32, 32, 32), quirks=False)
+    X = cases.node_coords(3, 7, cfg["nelem"], cfg["lo"], cfg["hi"])
+    u0 = cases.taylor_green(X)
+    del X
+    WJ = cases.quadrature_weights(3, 7, cfg["nelem"], cfg["lo"], cfg["hi"])
+    wj = torch.tensor(WJ.reshape(-1), dtype=torch.float64, device=cuda)
+    vol = L ** 3
+
+    def totals(du):
+        U = du.view(5, -1)
+        return (U * wj).sum(dim=1).cpu().numpy(), (U.abs() * wj).sum(dim=1).cpu().numpy()
+
+    def ek_rate(du, op):
+        out = torch.empty_like(du)
+        op.Mult(du, out)
+        U, dU = du.view(5, -1), out.view(5, -1)
+        vel = U[1:4] / U[0]
+        integrand = (vel * dU[1:4]).sum(dim=0) - 0.5 * (vel * vel).sum(dim=0) * dU[0]
+        return float((wj * integrand).sum()) / vol
+
+    ops = {p: DGSEMOperator(precision=p, **cfg) for p in ("fp64", "mixed")}
+    states = {p: to_dev(u0, cuda) for p in ops}
+    del u0
+    t0, s0 = totals(states["fp64"])
+    r0 = {p: ek_rate(states[p], ops[p]) for p in ops}
+    dt = 1e-4
+    for p, op in ops.items():
+        for _ in range(5):
+            op.Step(states[p], dt)
+    torch.cuda.synchronize()
+    r5 = {p: ek_rate(states[p], ops[p]) for p in ops}
+    for p, op in ops.items():
+        t1, _ = totals(states[p])
+        dcons = np.abs(t1 - t0) / s0
+        a, _ = op.alpha()
+        print("C5 32^3 N=7 %s: conservation |d total|/sum|.| %s, alpha max %.3g, state ok %s" % (
+            p, np.array2string(dcons, precision=2), a.max(), op.check_state(states[p])), flush=True)
+        assert op.check_state(states[p])
+        assert dcons.max() <= 1e-12
+        assert a.max() <= 0.1
+    for tag, r in (("t=0", r0), ("t=5dt", r5)):
+        rel = abs(r["mixed"] - r["fp64"]) / abs(r["fp64"])
+        print("C5 32^3 N=7 %s -dEk/dt fp64 %.10e mixed %.10e rel %.2e" % (tag, -r["fp64"], -r["mixed"], rel),
+              flush=True)
+        assert r["fp64"] < 0
+        assert rel <= 1e-4
+    for op in ops.values():
+        op.close()
