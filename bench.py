@@ -175,23 +175,72 @@ def algorithmic_counts(name, quirks):
     return table["%s_%s" % (name, "faithful" if quirks else "corrected")]
 
 
-def cpu_baseline(name, cfg, quirks, u, budget_s):
-    """Oracle RHS on this host's cores, bounded sample of the same workload."""
-    threads = min(os.cpu_count() or 1, 16)
-    orc = make_oracle(cfg, quirks, threads)
-    nodes = orc.NE * orc.K
-    orc.Mult(u)                       # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        orc.Mult(u)
-        n += 1
+def host_info():
+    """CPU model and core counts of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def host_threads():
+    """Threads for the all-cores baseline: the job's CPU share where the
+    environment states it (OMP_NUM_THREADS), else every core of the host."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return os.cpu_count() or 1
+
+
+def median_rate(orc, u, nodes, reps, per_rep):
+    """Median over reps repetitions of per_rep RHS evaluations (after a warm-up)."""
+    orc.Mult(u)
+    rates, spent = [], 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for _ in range(per_rep):
+            orc.Mult(u)
         el = time.perf_counter() - t0
-        # at least 50 evaluations and a second of work, at most the budget
-        if el > budget_s or (n >= 50 and el >= 1.0):
-            break
-    return {"value": nodes * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
-            "sample": "%d RHS evaluations of %s (%d nodes) in %.1f s, OpenMP over elements/faces"
-                      % (n, name, nodes, el)}
+        spent += el
+        rates.append(nodes * per_rep / el)
+    rates.sort()
+    return rates[len(rates) // 2], spent
+
+
+def cpu_baseline(name, cfg, quirks, u, budget_s):
+    """Oracle RHS on this host's cores, bounded sample of the same workload:
+    the median of 5 repetitions on all the job's cores and on one core.  The
+    repetition length is sized from one timed evaluation so that both legs
+    together take about budget_s."""
+    info = host_info()
+    nthr = host_threads()
+    out = {}
+    for label, threads in (("all", nthr), ("one", 1)):
+        orc = make_oracle(cfg, quirks, threads)
+        nodes = orc.NE * orc.K
+        t0 = time.perf_counter()
+        orc.Mult(u)
+        one = max(time.perf_counter() - t0, 1e-6)
+        share = budget_s * (0.4 if label == "all" else 0.6)
+        per_rep = max(1, min(20, int(share / 5 / one)))
+        rate, spent = median_rate(orc, u, nodes, 5, per_rep)
+        out[label] = (rate, per_rep, spent, threads)
+    rate, per_rep, spent, threads = out["all"]
+    r1, p1, s1, _ = out["one"]
+    return {"value": rate, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": "median of 5 repetitions of %d RHS evaluations of %s (%d nodes) on %d threads "
+                      "(OpenMP over elements/faces; the job's CPU share, OMP_NUM_THREADS) in %.1f s; "
+                      "CPU restatement of the reference RHS (not MFEM)" % (per_rep, name, nodes, threads, spent),
+            "one_thread": {"value": r1, "sample": "median of 5 repetitions of %d RHS evaluations in %.1f s"
+                                                  % (p1, s1)},
+            "host": info}
 
 
 def cpu_baseline_ensemble(case_list, budget_s):
@@ -199,7 +248,7 @@ def cpu_baseline_ensemble(case_list, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from prandtl_amd import cases
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     nodes, n, el = 0, 0, 0.0
     for c in case_list:
         kw = cases.operator_kwargs(c)
@@ -215,7 +264,7 @@ def cpu_baseline_ensemble(case_list, budget_s):
             break
     return {"value": nodes / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
             "sample": "one RHS evaluation of each of the first %d ensemble cases (%d nodes) in %.1f s, "
-                      "OpenMP over elements/faces" % (n, nodes, el)}
+                      "OpenMP over elements/faces" % (n, nodes, el), "host": host_info()}
 
 
 def run_ensemble(args):
@@ -482,36 +531,41 @@ def main():
 
     counts = algorithmic_counts(name, quirks)
     nv = op.nv
-    # k_rhs: inviscid volume pairs, surface fluxes, FV, indicator, assembly, RK stage
-    fl_rhs = counts["volume"] + counts["surface"] + counts["fv"] + counts["indicator"] + counts["assembly"] + 4 * nv
-    # compulsory bytes of k_rhs per node: read y, write out, read u0 on 2 of 3
-    # stages; viscous inputs (nv-1 volume + face traces) when parabolic
-    by_rhs = 8 * nv * (1 + 1 + 2.0 / 3.0)
-    if cfg.get("parabolic"):
-        Np = cfg["order"] + 1
-        by_rhs += 8 * (nv - 1) * (1 + 2 * cfg["dim"] * Np ** (cfg["dim"] - 1) / Np ** cfg["dim"])
-    if "mesh" in cfg:
-        # general geometry: adjugate rows and J per node, per face point the
-        # neighbour node (4 B) and normal (16 B), a code per face, and the
-        # subcell normals of blended elements
-        Np = cfg["order"] + 1
-        K = Np * Np
-        by_rhs += 8 * 5 + (4 * Np * 20 + 4 * 4) / K
-        if cfg.get("blending"):
-            by_rhs += 2 * Np * (Np - 1) * 2 * 8 / K
-        if cfg.get("axisymmetric"):
-            # r and the axis code per node (U is recovered from r U in registers)
-            by_rhs += 12
-    if "hex_warp" in cfg:
-        # 3-D general geometry: the element's 8 corners (the kernels recompute
-        # the per-node metrics from them), per face point the neighbour node
-        # (4 B) and normal (24 B), a code and a neighbour face per face
-        Np = cfg["order"] + 1
-        K = Np ** 3
-        by_rhs += (24 * 8 + 6 * Np * Np * 28 + 6 * 8) / K
+    # Algorithmic FP64 flops per node (oracle op counts of the reference
+    # formulas).  The reference computes subcell FV fluxes for every element
+    # and multiplies them by alpha = 0; those count only for the elements that
+    # blend (alpha > 0) at the timed state.
+    alpha, _ = op.alpha()
+    blend_frac = float((alpha > 0).mean()) if cfg.get("blending") else 0.0
+    fl_rhs = (counts["volume"] + counts["surface"] + counts["fv"] * blend_frac + counts["indicator"]
+              + counts["assembly"] + 4 * nv)          # k_rhs, with the RK stage combination
+    fl_visc = counts["lifting"] + counts["viscous"]    # k_visc
+    # Compulsory HBM bytes per node of one RHS + RK stage: read the stage
+    # state, read u^n on two stages of three, write the stage output.  The
+    # viscous intermediates and geometry tables are implementation bytes
+    # (measured, in "traffic").
+    by_node = 8 * nv * (1 + 1 + 2.0 / 3.0)
     kname = "k_rhs<%d,%d>" % (cfg["dim"], cfg["order"] + 1)
     traffic = measured_traffic(name, kname) if (args.precision == "fp64" and not quirks) else None
-    roof = roofline(fl_rhs * nodes, by_rhs * nodes, t_rhs, kname, traffic)
+    roof = roofline(fl_rhs * nodes, by_node * nodes, t_rhs, kname, traffic)
+    t_visc = ms_visc / max(n_visc, 1) * 1e-3
+    kernels = {kname: {"avg_launch_us": t_rhs * 1e6, "flops_per_node": fl_rhs,
+                       "tflops": fl_rhs * nodes / t_rhs / 1e12,
+                       "frac_fp64": fl_rhs * nodes / t_rhs / PEAK_FP64}}
+    if n_visc:
+        vname = "k_visc<%d,%d>" % (cfg["dim"], cfg["order"] + 1)
+        kernels[vname] = {"avg_launch_us": t_visc * 1e6, "flops_per_node": fl_visc,
+                          "tflops": fl_visc * nodes / t_visc / 1e12,
+                          "frac_fp64": fl_visc * nodes / t_visc / PEAK_FP64,
+                          "traffic": measured_traffic(name, vname) if (args.precision == "fp64" and not quirks)
+                          else None}
+    # the whole RHS at the measured DOF rate
+    fl_node = fl_rhs + (fl_visc if n_visc else 0.0)
+    whole = {"flops_per_dof": fl_node, "bytes_per_dof": by_node,
+             "tflops": value * fl_node / 1e12, "frac_fp64": value * fl_node / PEAK_FP64,
+             "hbm_GBps": value * by_node / 1e9, "frac_hbm": value * by_node / PEAK_HBM,
+             "bound": "fp64" if fl_node / by_node >= PEAK_FP64 / PEAK_HBM else "hbm",
+             "blend_frac": blend_frac}
 
     line = {
         "metric": METRIC,
@@ -532,7 +586,8 @@ def main():
                    "rhs_per_step": 3, "dt": dt, "state_ok": bool(ok), "hip_graph": True,
                    "launch_shapes": shape, "launch_shape_us": shape_us,
                    "ms_per_step_event_profiled": elapsed_prof / psteps * 1e3,
-                   "kernel_ms": {"k_rhs": ms_rhs / max(n_rhs, 1), "k_visc": ms_visc / max(n_visc, 1)}},
+                   "kernel_ms": {"k_rhs": ms_rhs / max(n_rhs, 1), "k_visc": ms_visc / max(n_visc, 1)},
+                   "kernels": kernels, "rhs_roofline": whole},
         "roofline": roof,
     }
     if args.precision == "fp64" and cfg.get("parabolic") and not quirks and not args.no_mixed \
