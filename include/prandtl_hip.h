@@ -263,6 +263,20 @@ int prandtl_set_graph(prandtl_ctx *ctx, int enable);
 #define PRANDTL_KERNEL_VISC 1
 int prandtl_set_shape(prandtl_ctx *ctx, int kernel, int shape);
 
+/* Kernel algorithm of the flux-differencing RHS (no counterpart in the
+ * reference; the operator is the same, DGSEMIntegrator.cpp:151-1196).
+ * PRANDTL_ALGO_TILE: k_rhs, a workgroup's threads take the element's
+ * two-point fluxes as jobs through LDS (every mesh kind).  PRANDTL_ALGO_LINE:
+ * k_rhs_ln, one thread per tensor line (periodic Cartesian boxes in 3-D,
+ * N = 2..4), the default where it exists.  The two differ by rounding only;
+ * launch shapes (prandtl_set_shape, prandtl_autotune) belong to the current
+ * algorithm (the line kernel has shape 0 only).  PRANDTL_ERR_UNSUPPORTED for
+ * an algorithm this case does not have. */
+#define PRANDTL_ALGO_TILE 0
+#define PRANDTL_ALGO_LINE 1
+int prandtl_set_algorithm(prandtl_ctx *ctx, int kernel, int algo);
+int prandtl_get_algorithm(const prandtl_ctx *ctx, int kernel, int *algo);
+
 /* Autotuner: times every shape this case has, of each kernel, on state
  * u_dev (2 + 10 launches each, into context scratch; the running wave speed
  * sees u_dev) and keeps the fastest of each.  shapes_out (may be NULL)

@@ -21,6 +21,8 @@ PRANDTL_MIXED = 1
 PRANDTL_SHAPES = 4
 PRANDTL_KERNEL_RHS = 0
 PRANDTL_KERNEL_VISC = 1
+PRANDTL_ALGO_TILE = 0
+PRANDTL_ALGO_LINE = 1
 
 EXPORTED = (
     "prandtl_create", "prandtl_destroy", "prandtl_sizes", "prandtl_rhs", "prandtl_rk3_step",
@@ -33,6 +35,7 @@ EXPORTED = (
     "prandtl_create_quad", "prandtl_hmin", "prandtl_mesh_read", "prandtl_mesh_refine", "prandtl_mesh_view",
     "prandtl_mesh_name", "prandtl_mesh_free", "prandtl_mesh_last_error", "prandtl_axis_recover",
     "prandtl_create_hex", "prandtl_mesh_dim", "prandtl_mesh_view_hex",
+    "prandtl_set_algorithm", "prandtl_get_algorithm",
 )
 
 
@@ -101,6 +104,8 @@ def lib():
     L.prandtl_rk3_step.argtypes = [vp, vp, ctypes.c_double, vp]
     L.prandtl_set_graph.argtypes = [vp, ctypes.c_int]
     L.prandtl_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.prandtl_set_algorithm.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.prandtl_get_algorithm.argtypes = [vp, ctypes.c_int, ip]
     L.prandtl_autotune.argtypes = [vp, vp, ip, dp, vp]
     L.prandtl_run.argtypes = [vp, vp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                               ctypes.c_int, ip, vp]

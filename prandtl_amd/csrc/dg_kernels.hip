@@ -384,6 +384,7 @@ bool shape_ok(const prandtl_ctx *c, int kernel, int shape)
         return c->prm.parabolic && (shape == 0 || (!c->geo.bcmask && !c->gen && visc_shape_ok(c->dim, c->Np, shape)));
     if (kernel != PRANDTL_KERNEL_RHS) return false;
     if (shape == 0) return true;
+    if (c->rhs_algo == PRANDTL_ALGO_LINE) return false;
     if (c->dim == 3 && c->prm.parabolic && c->prm.quirks) return false;
     if (c->geo.bcmask || c->gen) return false;
     return rhs_shape_ok(c->dim, c->Np, c->prm.blending != 0, shape);
@@ -725,6 +726,8 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         prandtl_destroy(c);
         return PRANDTL_ERR_NOMEM;
     }
+    // the line kernel where it serves this case
+    c->rhs_algo = line_ok(c) ? PRANDTL_ALGO_LINE : PRANDTL_ALGO_TILE;
     *out = c;
     return PRANDTL_OK;
 }
@@ -852,6 +855,28 @@ int prandtl_set_shape(prandtl_ctx *c, int kernel, int shape)
     int &cur = kernel == PRANDTL_KERNEL_RHS ? c->rhs_shape : c->visc_shape;
     if (shape != cur) drop_graph(c);
     cur = shape;
+    return PRANDTL_OK;
+}
+
+int prandtl_set_algorithm(prandtl_ctx *c, int kernel, int algo)
+{
+    if (!c) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    if (kernel != PRANDTL_KERNEL_RHS || (algo != PRANDTL_ALGO_TILE && algo != PRANDTL_ALGO_LINE))
+        return fail(c, PRANDTL_ERR_ARG, "unknown kernel or algorithm");
+    if (algo == PRANDTL_ALGO_LINE && !line_ok(c))
+        return fail(c, PRANDTL_ERR_UNSUPPORTED, "the line kernel serves periodic 3-D boxes of order 2..4");
+    if (algo != c->rhs_algo) {
+        drop_graph(c);
+        c->rhs_algo = algo;
+        c->rhs_shape = 0;
+    }
+    return PRANDTL_OK;
+}
+
+int prandtl_get_algorithm(const prandtl_ctx *c, int kernel, int *algo)
+{
+    if (!c || !algo || kernel != PRANDTL_KERNEL_RHS) return PRANDTL_ERR_ARG;
+    *algo = c->rhs_algo;
     return PRANDTL_OK;
 }
 

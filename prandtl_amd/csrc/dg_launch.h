@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include "dg_device.h"
+#include "dg_line.h"
 
 using namespace prandtl;
 
@@ -42,6 +43,7 @@ struct prandtl_ctx {
     const double *dtp = nullptr;  // set while launching stages for a graph
     double graph_dt = 0.0;        // value last copied to d_dt
     bool graph_dt_set = false;
+    int rhs_algo = 0;             // flux-differencing kernel: PRANDTL_ALGO_TILE (k_rhs) or _LINE (k_rhs_ln)
     int rhs_shape = 0;            // k_rhs launch shape (only shapes rhs_shape_ok accepts are set)
     int visc_shape = 0;           // k_visc launch shape (visc_shape_ok)
     // device time loop (prandtl_run)
@@ -167,11 +169,30 @@ void launch_rhs_fb(prandtl_ctx *c, const KArgs &a, hipStream_t s)
     else hipLaunchKernelGGL((k_rhs<DIM, NP, BLEND, false, double, FB>), dim3(nb), dim3(Cf::BLOCK), 0, s, a);
 }
 
+// The line kernel serves periodic Cartesian boxes of the orders line_built
+// lists.
+inline bool line_ok(const prandtl_ctx *c)
+{
+    return line_built(c->dim, c->Np) && !c->gen && !c->axi && c->geo.bcmask == 0;
+}
+
 template <int DIM, int NP, bool BLEND>
 int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
 {
     ProfScope ps(c, s, 0);
     const bool qns3 = DIM == 3 && c->prm.parabolic && c->prm.quirks;
+    if constexpr (line_built(DIM, NP)) {
+        if (c->rhs_algo == PRANDTL_ALGO_LINE && line_ok(c)) {
+            using Lc = LCfg<DIM, NP>;
+            const int nb = (c->NE + Lc::EPB - 1) / Lc::EPB;
+            if (qns3)
+                hipLaunchKernelGGL((k_rhs_ln<DIM, NP, BLEND, true, double>), dim3(nb), dim3(Lc::BLOCK), 0, s, a);
+            else if (c->prm.parabolic && c->prm.precision == PRANDTL_MIXED)
+                hipLaunchKernelGGL((k_rhs_ln<DIM, NP, BLEND, false, float>), dim3(nb), dim3(Lc::BLOCK), 0, s, a);
+            else hipLaunchKernelGGL((k_rhs_ln<DIM, NP, BLEND, false, double>), dim3(nb), dim3(Lc::BLOCK), 0, s, a);
+            return hip_check(c, hipGetLastError(), "k_rhs_ln launch");
+        }
+    }
     if constexpr (gen_built(DIM, NP)) {
         if (c->gen) {
             // quadrilateral or hexahedral mesh, natural shape

@@ -504,6 +504,19 @@ class DGSEMOperator:
         k = {"rhs": _lib.PRANDTL_KERNEL_RHS, "visc": _lib.PRANDTL_KERNEL_VISC}[kernel]
         check(lib().prandtl_set_shape(self._h, k, int(shape)), self._h)
 
+    def set_algorithm(self, algo):
+        """Kernel algorithm of the flux-differencing RHS: "tile" (k_rhs, every
+        mesh kind) or "line" (k_rhs_ln, one thread per tensor line: periodic
+        3-D boxes of order 2..4, the default there)."""
+        a = {"tile": _lib.PRANDTL_ALGO_TILE, "line": _lib.PRANDTL_ALGO_LINE}[algo]
+        check(lib().prandtl_set_algorithm(self._h, _lib.PRANDTL_KERNEL_RHS, a), self._h)
+
+    @property
+    def algorithm(self):
+        a = ctypes.c_int()
+        check(lib().prandtl_get_algorithm(self._h, _lib.PRANDTL_KERNEL_RHS, ctypes.byref(a)), self._h)
+        return "line" if a.value == _lib.PRANDTL_ALGO_LINE else "tile"
+
     def autotune(self, u):
         """Time every launch shape of both kernels on state u; keep the fastest.
 
