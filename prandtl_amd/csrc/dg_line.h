@@ -43,16 +43,6 @@
 
 namespace prandtl {
 
-// 1/x: hardware estimate plus two Newton-Raphson refinements.
-__device__ __forceinline__ double rcp_nr(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
-
 // Log mean LM(x, y) = num / den, the branches of ComputeLogMean
 // (BasicOperations.cpp:6-12, eps 1e-4) as in logmean(): u = d^2/s^2.
 __device__ __forceinline__ void lm_parts(double x, double y, double &num, double &den)

@@ -38,20 +38,39 @@ __device__ __forceinline__ R sutherland_mu(R rho, R p, R R_gas)
     return mu0 * T0pTs / (T + Ts) * (T / T0) * sqrt(T / T0);
 }
 
+// Reciprocals: the hardware estimate (v_rcp_f64 / v_rcp_f32) refined by
+// Newton-Raphson steps r <- r + r (1 - x r) (two for FP64, one for FP32),
+// within an ulp or two of 1/x; the pointwise transforms multiply by them
+// where the reference divides.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ float rcp_nr(float x)
+{
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+
 // Per-node quantities shared by every two-point flux touching the node.
 template <int DIM> struct NodeQ {
     double rho, v[DIM], beta, ibeta, lam;
 };
 
 // rho, v = m/rho, p = (gamma-1)(E - rho|v|^2/2), beta = rho/(2p), 1/beta,
-// lam = |v| + sqrt(gamma p/rho)  (ChandrashekarFlux.cpp:17-68).  One
-// reciprocal of rho replaces the reference's per-component divisions.
+// lam = |v| + sqrt(gamma p/rho)  (ChandrashekarFlux.cpp:17-68).  Reciprocals
+// of rho and p replace the reference's divisions.
 template <int DIM>
 __device__ __forceinline__ NodeQ<DIM> node_quants(const double *s, const Phys &P)
 {
     NodeQ<DIM> q;
     q.rho = s[0];
-    const double ri = 1.0 / q.rho;
+    const double ri = rcp_nr(q.rho);
     double vsq = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -59,7 +78,7 @@ __device__ __forceinline__ NodeQ<DIM> node_quants(const double *s, const Phys &P
         vsq = (d == 0) ? q.v[d] * q.v[d] : vsq + q.v[d] * q.v[d];
     }
     const double p = P.gm1 * (s[DIM + 1] - 0.5 * q.rho * vsq);
-    q.beta = 0.5 * q.rho / p;
+    q.beta = 0.5 * q.rho * rcp_nr(p);
     q.ibeta = 2.0 * p * ri;
     q.lam = sqrt(vsq) + sqrt(P.gamma * p * ri);
     return q;
@@ -470,7 +489,7 @@ __device__ __forceinline__ double boundary_flux(int type, int r, double ns, doub
 template <int DIM>
 __device__ __forceinline__ void cons2entropy(const double *s, const Phys &P, double *w)
 {
-    const double r = 1.0 / s[0];
+    const double r = rcp_nr(s[0]);
     double vel[DIM], vv = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -479,7 +498,7 @@ __device__ __forceinline__ void cons2entropy(const double *s, const Phys &P, dou
     }
     const double v_sq = 0.5 * vv;
     const double p = P.gm1 * (s[DIM + 1] - s[0] * v_sq);
-    const double beta = s[0] / p;
+    const double beta = s[0] * rcp_nr(p);
     // log(p rho^-gamma) as log p - gamma log rho: two logs instead of a log
     // and a pow (rounding-level difference)
     const double sent = log(p) - P.gamma * log(s[0]);
@@ -541,8 +560,8 @@ __device__ __forceinline__ void lift_flux(int type, const double *w, const doubl
 template <int DIM, typename R>
 __device__ __forceinline__ void entgrad2primgrad(const R *s, R gm1, R gm1i, bool quirks, R *g)
 {
-    const R one = 1, half = 0.5;
-    const R r = one / s[0];
+    const R half = 0.5;
+    const R r = rcp_nr(s[0]);
     R vel[DIM], vv = 0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -552,16 +571,17 @@ __device__ __forceinline__ void entgrad2primgrad(const R *s, R gm1, R gm1i, bool
     const R v_sq = half * vv;
     const R p = gm1 * (s[DIM + 1] - s[0] * v_sq);
     const R KE = v_sq * s[0];
+    const R pr = p * r, rp = s[0] * rcp_nr(p);   // p/rho, rho/p
     const R g0 = g[0], gL = g[DIM + 1];
     R mdot = 0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
-        g[1 + d] = p / s[0] * (g[1 + d] + vel[d] * gL);
+        g[1 + d] = pr * (g[1 + d] + vel[d] * gL);
         const R md = quirks ? vel[d] : s[1 + d];
         mdot = (d == 0) ? md * g[1 + d] : mdot + md * g[1 + d];
     }
-    g[0] = s[0] * g0 - gL * (KE - p * gm1i) + s[0] / p * mdot;
-    g[DIM + 1] = p / s[0] * (g[0] + p * gL);
+    g[0] = s[0] * g0 - gL * (KE - p * gm1i) + rp * mdot;
+    g[DIM + 1] = pr * (g[0] + p * gL);
 }
 
 // NavierStokesFlux::ComputeViscousFlux (NavierStokesFlux.cpp:12-90), mu_bulk =
@@ -572,19 +592,15 @@ __device__ __forceinline__ void viscous_flux(const R *s, const R (*dq)[DIM + 2],
                                              const ViscCoef<R> &vc, R (*fl)[DIM])
 {
     constexpr int NV = DIM + 2;
-    const R mu_bulk = R(2) / R(3), one = 1, two = 2, half = 0.5;
+    const R mu_bulk = R(2) / R(3), two = 2, half = 0.5;
     const R rho = s[0];
+    const R rr = rcp_nr(rho);
     R vel[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; d++) vel[d] = s[1 + d] / rho;
-    // ComputePressure: V /= rho multiplies by 1/rho (BasicOperations.cpp:33-36)
-    const R rr = one / rho;
+    for (int d = 0; d < DIM; d++) vel[d] = s[1 + d] * rr;
     R vv = 0;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        const R v = s[1 + d] * rr;
-        vv = (d == 0) ? v * v : vv + v * v;
-    }
+    for (int d = 0; d < DIM; d++) vv = (d == 0) ? vel[d] * vel[d] : vv + vel[d] * vel[d];
     const R p = gm1 * (s[NV - 1] - half * rho * vv);
     R mu = vc.mu, lam_h = vc.lam_h;
     if (vc.suth) {
@@ -595,9 +611,10 @@ __device__ __forceinline__ void viscous_flux(const R *s, const R (*dq)[DIM + 2],
     R div = dq[0][1];
 #pragma unroll
     for (int d = 1; d < DIM; d++) div += dq[d][1 + d];
+    const R gr = gm1i * rr, pr = p * rr;
 #pragma unroll
     for (int c = 0; c < DIM; c++) {
-        const R cvdT = gm1i / rho * (dq[c][NV - 1] - p / rho * dq[c][0]);
+        const R cvdT = gr * (dq[c][NV - 1] - pr * dq[c][0]);
 #pragma unroll
         for (int a = 0; a < DIM; a++) {
             R t;
