@@ -76,6 +76,7 @@ struct Geo {
     int n[3];
     int NE;
     double nrm[3];   // adj(r,r) = face measure of an r-face
+    double inv_n0, inv_n01;   // 1 / n[0], 1 / (n[0] n[1]): box element index arithmetic
     double invJ;     // 1/J
     double aface;    // 1/(w_0 J)
     int bcmask;      // boundary type (BC_*) of side (r, s) in bits 3(2r+s)..+2; 0: periodic box
@@ -213,9 +214,34 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
     return (x < rr) ? x * (q + 1) + k : rr * (q + 1) + (x - rr) * q + k;
 }
 
+// q = e / d for 0 <= e < 2^31 and d >= 1 given dinv = 1/d: the truncated
+// double-precision product is off by at most one, and one remainder test
+// corrects it (cheaper than an integer division by a run-time divisor).
+__device__ __forceinline__ int quot(int e, int d, double dinv)
+{
+    int q = (int)((double)e * dinv);
+    const int rr = e - q * d;
+    q += (rr >= d) ? 1 : 0;
+    q -= (rr < 0) ? 1 : 0;
+    return q;
+}
+
+// (i, j, k) of box element e, elements numbered x fastest
+__device__ __forceinline__ void box_index(const Geo &g, int e, int *idx)
+{
+    const int n01 = g.n[0] * g.n[1];
+    const int k = quot(e, n01, g.inv_n01);
+    const int rem = e - k * n01;
+    const int j = quot(rem, g.n[0], g.inv_n0);
+    idx[0] = rem - j * g.n[0];
+    idx[1] = j;
+    idx[2] = k;
+}
+
 __device__ __forceinline__ int neighbor(const Geo &g, int e, int r, int side)
 {
-    int idx[3] = {e % g.n[0], (e / g.n[0]) % g.n[1], e / (g.n[0] * g.n[1])};
+    int idx[3];
+    box_index(g, e, idx);
     idx[r] = side ? (idx[r] + 1 == g.n[r] ? 0 : idx[r] + 1) : (idx[r] == 0 ? g.n[r] - 1 : idx[r] - 1);
     return (idx[2] * g.n[1] + idx[1]) * g.n[0] + idx[0];
 }
@@ -224,7 +250,9 @@ __device__ __forceinline__ int neighbor(const Geo &g, int e, int r, int side)
 // a neighbour.
 __device__ __forceinline__ int boundary_of(const Geo &g, int e, int r, int side)
 {
-    const int idx = r == 0 ? e % g.n[0] : r == 1 ? (e / g.n[0]) % g.n[1] : e / (g.n[0] * g.n[1]);
+    int ix[3];
+    box_index(g, e, ix);
+    const int idx = ix[r];
     const int nr = r == 0 ? g.n[0] : r == 1 ? g.n[1] : g.n[2];
     const bool at = side ? idx == nr - 1 : idx == 0;
     return at ? (g.bcmask >> (3 * (2 * r + side))) & 7 : BC_PERIODIC;

@@ -616,6 +616,8 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         for (int r = 1; r < dim; r++) c->hmin = h[r] < c->hmin ? h[r] : c->hmin;
     }
     for (int r = dim; r < 3; r++) c->geo.n[r] = 1;
+    c->geo.inv_n0 = 1.0 / c->geo.n[0];
+    c->geo.inv_n01 = 1.0 / ((double)c->geo.n[0] * c->geo.n[1]);
     c->geo.NE = c->NE;
     double J;
     if (dim == 1) {

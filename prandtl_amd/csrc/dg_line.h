@@ -164,11 +164,16 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
     // sQ (node quantities, read in the line phase) and sP (partial sums,
     // written after it) share storage unless blending, whose indicator uses
     // sP as scratch while sQ is live
+    //
+    // Node n sits at n + n / NP in each quantity's row (one pad per x-row),
+    // so the lines of all three directions read distinct LDS banks.
+    constexpr int KP = K + K / NP;
     constexpr int NQP = BLEND ? NQ + NV : (NQ > NV ? NQ : NV);
-    __shared__ double sU[EPB][NV][K];
-    __shared__ double sQP[EPB * NQP * K];
-    auto sQ = reinterpret_cast<double (*)[NQ][K]>(sQP);
-    auto sP = reinterpret_cast<double (*)[NV][K]>(sQP + (BLEND ? EPB * NQ * K : 0));
+    __shared__ double sU[EPB][NV][KP];
+    __shared__ double sQP[EPB * NQP * KP];
+    auto sQ = reinterpret_cast<double (*)[NQ][KP]>(sQP);
+    auto sP = reinterpret_cast<double (*)[NV][KP]>(sQP + (BLEND ? EPB * NQ * KP : 0));
+    auto pad = [](int n) { return n + n / NP; };
     __shared__ double sV1i[BLEND ? NP * NP : 1];
     __shared__ double sAl[EPB];
     __shared__ double sRed[(BLOCK + 63) / 64];
@@ -195,33 +200,21 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
     // node of position m on the line: base + m * stride (x fastest)
     const int base = r == 0 ? l * NP : (r == 1 ? (l / NP) * NP * NP + l % NP : l);
     const int stride = r == 0 ? 1 : (r == 1 ? NP : NP * NP);
+    // the same line in the padded LDS rows: pad(base + m stride) = pb + m ps
+    const int pb = r == 0 ? l * (NP + 1) : (r == 1 ? (l / NP) * NP * (NP + 1) + l % NP : l + l / NP);
+    const int ps = r == 0 ? 1 : (r == 1 ? NP + 1 : NP * (NP + 1));
     const double nr = r == 0 ? A.geo.nrm[0] : (r == 1 ? A.geo.nrm[1] : A.geo.nrm[2]);
 
-    // neighbour traces at both ends of the line (and the viscous face traces
-    // Fv.n of both sides), in flight during the node phase
+    // neighbour traces at both ends of the line, in flight during the node
+    // phase
     double pl[NV], ph[NV];
-    double fol[NV], foh[NV], fnl[NV], fnh[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) { fol[v] = foh[v] = fnl[v] = fnh[v] = 0.0; pl[v] = ph[v] = 1.0; }
+    for (int v = 0; v < NV; v++) pl[v] = ph[v] = 1.0;
+    const int enl = lact ? neighbor(A.geo, e, r, 0) : 0, enh = lact ? neighbor(A.geo, e, r, 1) : 0;
     if (lact) {
-        const int enl = neighbor(A.geo, e, r, 0), enh = neighbor(A.geo, e, r, 1);
         const size_t gl = (size_t)enl * K + base + (NP - 1) * stride, gh = (size_t)enh * K + base;
 #pragma unroll
         for (int v = 0; v < NV; v++) { pl[v] = A.y[v * NK + gl]; ph[v] = A.y[v * NK + gh]; }
-        if (A.parabolic) {
-            const VT *fv = static_cast<const VT *>(A.fvf);
-            const VT *ol = fv + (((size_t)e * 2 * DIM + 2 * r) * (NV - 1)) * NFP;
-            const VT *oh = fv + (((size_t)e * 2 * DIM + 2 * r + 1) * (NV - 1)) * NFP;
-            const VT *nl = fv + (((size_t)enl * 2 * DIM + 2 * r + 1) * (NV - 1)) * NFP;
-            const VT *nh = fv + (((size_t)enh * 2 * DIM + 2 * r) * (NV - 1)) * NFP;
-#pragma unroll
-            for (int v = 1; v < NV; v++) {
-                fol[v] = ol[(v - 1) * NFP + l];
-                foh[v] = oh[(v - 1) * NFP + l];
-                fnl[v] = nl[(v - 1) * NFP + l];
-                fnh[v] = nh[(v - 1) * NFP + l];
-            }
-        }
     }
 
     // ---- node phase: state and per-node flux quantities into LDS ----
@@ -232,14 +225,15 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         double ys[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)ebase * K + i];
+        const int np = pad(n);
 #pragma unroll
-        for (int v = 0; v < NV; v++) sU[e2][v][n] = ys[v];
+        for (int v = 0; v < NV; v++) sU[e2][v][np] = ys[v];
         const NodeQ<DIM> q = node_quants<DIM>(ys, P);
 #pragma unroll
-        for (int d = 0; d < DIM; d++) sQ[e2][d][n] = q.v[d];
-        sQ[e2][DIM][n] = q.beta;
-        sQ[e2][DIM + 1][n] = q.ibeta;
-        sQ[e2][DIM + 2][n] = q.lam;
+        for (int d = 0; d < DIM; d++) sQ[e2][d][np] = q.v[d];
+        sQ[e2][DIM][np] = q.beta;
+        sQ[e2][DIM + 1][np] = q.ibeta;
+        sQ[e2][DIM + 2][np] = q.lam;
         const double lq = (q.lam == q.lam) ? q.lam : 0.0;
         lam = (lam < lq) ? lq : lam;
     }
@@ -253,7 +247,7 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
             const int e2 = i / K, n = i - e2 * K;
             double ys[NV];
 #pragma unroll
-            for (int v = 0; v < NV; v++) ys[v] = sU[e2][v][n];
+            for (int v = 0; v < NV; v++) ys[v] = sU[e2][v][pad(n)];
             // rho * ComputePressure(state) with V /= rho (BasicOperations.cpp:31-37)
             const double rr = 1.0 / ys[0];
             double vv = 0.0;
@@ -351,7 +345,7 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         // face (this element = elem1)
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            const int no = base + (side ? (NP - 1) * stride : 0);
+            const int no = pb + (side ? (NP - 1) * ps : 0);
             double so[NV], pn[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) { so[v] = sU[el][v][no]; pn[v] = side ? ph[v] : pl[v]; }
@@ -361,9 +355,13 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
             fvn[0] = 0.0;
             bool own_q = true;
             if (A.parabolic) {
+                // viscous face traces Fv.n of both sides of this face point
+                const VT *fv = static_cast<const VT *>(A.fvf);
+                const VT *fo = fv + (((size_t)e * 2 * DIM + 2 * r + side) * (NV - 1)) * NFP;
+                const VT *fn = fv + (((size_t)(side ? enh : enl) * 2 * DIM + 2 * r + 1 - side) * (NV - 1)) * NFP;
 #pragma unroll
                 for (int v = 1; v < NV; v++)
-                    fvn[v] = side ? ((foh[v] + fnh[v]) * 0.5) * nr : ((fol[v] + fnl[v]) * 0.5) * nr;
+                    fvn[v] = (((double)fo[(v - 1) * NFP + l] + (double)fn[(v - 1) * NFP + l]) * 0.5) * nr;
                 if (A.quirks) {
                     // ComputeViscousFlux left velocities in the momentum slots
                     // of state1/state2 (DGSEMIntegrator.cpp:471-481)
@@ -406,7 +404,7 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
 #pragma unroll
         for (int a = 0; a < NP; a++) {
             NodeQ<DIM> qa;
-            const int na = base + a * stride;
+            const int na = pb + a * ps;
             qa.rho = sU[el][0][na];
 #pragma unroll
             for (int d = 0; d < DIM; d++) qa.v[d] = sQ[el][d][na];
@@ -414,7 +412,7 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
 #pragma unroll
             for (int b = a + 1; b < NP; b++) {
                 NodeQ<DIM> qb;
-                const int nb = base + b * stride;
+                const int nb = pb + b * ps;
                 qb.rho = sU[el][0][nb];
 #pragma unroll
                 for (int d = 0; d < DIM; d++) qb.v[d] = sQ[el][d][nb];
@@ -433,7 +431,7 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
                 // node (i, NP-1, a) of its eta line; 0 on the eta line's last node
                 const int i0 = l % NP, j0 = l / NP;
                 if (j0 < NP - 1) {
-                    const int ne = (a * NP + NP - 1) * NP + i0;
+                    const int ne = pad((a * NP + NP - 1) * NP + i0);
                     NodeQ<DIM> qe;
                     qe.rho = sU[el][0][ne];
 #pragma unroll
@@ -460,7 +458,7 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
                 for (int v = 0; v < NV; v++) gprev[v] = 0.0;
 #pragma unroll
                 for (int i = 0; i < NP - 1; i++) {
-                    const int na = base + i * stride, nb = na + stride;
+                    const int na = pb + i * ps, nb = na + ps;
                     double s1[NV], s2[NV];
 #pragma unroll
                     for (int v = 0; v < NV; v++) { s1[v] = sU[el][v][na]; s2[v] = sU[el][v][nb]; }
@@ -495,7 +493,7 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         if (lact && r == rr) {
 #pragma unroll
             for (int a = 0; a < NP; a++) {
-                const int nd = base + a * stride;
+                const int nd = pb + a * ps;
 #pragma unroll
                 for (int v = 0; v < NV; v++) sP[el][v][nd] = (rr == 0) ? acc[a][v] : sP[el][v][nd] + acc[a][v];
             }
@@ -509,10 +507,10 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         const size_t gi = (size_t)ebase * K + i;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            double tot = sP[e2][v][n];
+            double tot = sP[e2][v][pad(n)];
             if (A.parabolic && v > 0)
                 tot += (double)static_cast<const VT *>(A.visc)[(v - 1) * NK + gi] * A.geo.invJ;
-            const double yv = sU[e2][v][n];
+            const double yv = sU[e2][v][pad(n)];
             double o;
             if (A.mode == 0) o = tot;
             else if (A.mode == 1) o = yv + dt * tot;
