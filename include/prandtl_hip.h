@@ -222,6 +222,25 @@ int prandtl_hmin(const prandtl_ctx *ctx, double *hmin);
  * with the semi-discrete RHS of u.  u and dudt must not alias. */
 int prandtl_rhs(prandtl_ctx *ctx, const double *u_dev, double *dudt_dev, void *stream);
 
+/* No counterpart in the reference: the RHS of prandtl_rhs on a Cartesian box
+ * (periodic or with boundary sides) for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It takes no context: the operator's tables are built on
+ * the host at each call and passed to the kernels by value, and the kernels'
+ * scratch (k_visc's viscous sums and face traces, the wave-speed slots,
+ * alpha, eta, the box's boundary states) lives in a fixed 32 MiB device array
+ * of the library, one per GPU, held until the process ends.  The call makes
+ * no hipMalloc, hipFree or other allocating call on the device; it returns
+ * PRANDTL_ERR_ARG for a case whose scratch exceeds 32 MiB (3-D
+ * Navier-Stokes: about 400k nodes; the message gives the bytes needed) and
+ * the create-time errors of prandtl_create.  The result equals prandtl_rhs
+ * of a context created from the same mesh and params, bitwise.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other host threads) race and give wrong results.  The running
+ * wave speed is not kept.  Errors: prandtl_last_error(NULL). */
+int prandtl_rhs_noalloc(const prandtl_mesh_desc *mesh, const prandtl_params *params, const double *u_dev,
+                        double *dudt_dev, void *stream);
+
 /* RK3SSPSolver::Step(x, t, dt) as driven at Simulation.cpp:1010: three RHS
  * evaluations with the Shu-Osher stage combinations fused into the RHS
  * kernel; u is advanced in place. */

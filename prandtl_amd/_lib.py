@@ -35,7 +35,7 @@ EXPORTED = (
     "prandtl_create_quad", "prandtl_hmin", "prandtl_mesh_read", "prandtl_mesh_refine", "prandtl_mesh_view",
     "prandtl_mesh_name", "prandtl_mesh_free", "prandtl_mesh_last_error", "prandtl_axis_recover",
     "prandtl_create_hex", "prandtl_mesh_dim", "prandtl_mesh_view_hex",
-    "prandtl_set_algorithm", "prandtl_get_algorithm",
+    "prandtl_set_algorithm", "prandtl_get_algorithm", "prandtl_rhs_noalloc",
 )
 
 
@@ -101,6 +101,7 @@ def lib():
     L.prandtl_destroy.restype = None
     L.prandtl_sizes.argtypes = [vp, ip, ip, ip]
     L.prandtl_rhs.argtypes = [vp, vp, vp, vp]
+    L.prandtl_rhs_noalloc.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params), vp, vp, vp]
     L.prandtl_rk3_step.argtypes = [vp, vp, ctypes.c_double, vp]
     L.prandtl_set_graph.argtypes = [vp, ctypes.c_int]
     L.prandtl_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -418,10 +419,14 @@ namespace {
 // prandtl_create, prandtl_create_quad and prandtl_create_hex: qm and hm
 // null for a Cartesian box (mesh), else a quadrilateral or hexahedral mesh
 // (mesh then only carries dim and the order).
+//
+// host_ctx (boxes only): fill this caller-owned context's tables and stop
+// before any device memory is allocated (prandtl_rhs_noalloc).
 int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, const prandtl_params *prm,
-                prandtl_ctx **out, const prandtl_hex_mesh *hm = nullptr)
+                prandtl_ctx **out, const prandtl_hex_mesh *hm = nullptr, prandtl_ctx *host_ctx = nullptr)
 {
     if (!mesh || !prm || !out) return fail(nullptr, PRANDTL_ERR_ARG, "null argument");
+    if (host_ctx && (qm || hm)) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "host-only set-up is for boxes");
     *out = nullptr;
     const int dim = mesh->dim, N = mesh->order;
     if (dim < 1 || dim > 3) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "dim must be 1, 2 or 3");
@@ -523,7 +528,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
             }
         }
     }
-    prandtl_ctx *c = new (std::nothrow) prandtl_ctx();
+    prandtl_ctx *c = host_ctx ? host_ctx : new (std::nothrow) prandtl_ctx();
     if (!c) return fail(nullptr, PRANDTL_ERR_NOMEM, "host allocation");
     c->dim = dim; c->order = N; c->Np = N + 1; c->nv = dim + 2;
     c->mesh = *mesh;
@@ -571,7 +576,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         for (int a = 0; a < Np; a++) V[i * Np + a] = L[a];
     }
     if (!host::invert(Np, V, Vi)) {
-        delete c;
+        if (!host_ctx) delete c;
         return fail(nullptr, PRANDTL_ERR_ARG, "singular Vandermonde");
     }
     for (int a = 0; a < Np; a++)
@@ -585,7 +590,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (qm) {
         std::string err;
         if (!quad::build(qm, N, x, w, D, prm->axisymmetric != 0, qg, err)) {
-            delete c;
+            if (!host_ctx) delete c;
             return fail(nullptr, PRANDTL_ERR_ARG, err);
         }
         c->gen = true;
@@ -596,7 +601,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (hm) {
         std::string err;
         if (!quad::build_hex(hm, N, x, w, D, qg, err)) {
-            delete c;
+            if (!host_ctx) delete c;
             return fail(nullptr, PRANDTL_ERR_ARG, err);
         }
         // the kernels recompute the per-node metrics from the corners
@@ -663,6 +668,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     c->phys.R_gas = prm->R_gas;
     c->phys.PrInv = 1.0 / prm->Pr;
 
+    if (host_ctx) return PRANDTL_OK;   // tables only: the caller brings device memory
     const size_t NK = (size_t)c->NE * c->K;
     hipError_t e = hipSuccess;
     if (prm->parabolic) {
@@ -805,6 +811,111 @@ int prandtl_rhs(prandtl_ctx *c, const double *u, double *dudt, void *stream)
     return stage(c, u, nullptr, dudt, 0, 0.0, 0.0, 0.0, (hipStream_t)stream);
 }
 
+} // extern "C"
+
+namespace {
+
+// Device scratch of prandtl_rhs_noalloc: one copy of this array per GPU the
+// library runs on, held until the process ends.  Host code never uses the
+// name itself as an address (it is a host-side stand-in there): each
+// device's address comes from hipGetSymbolAddress, looked up once per device
+// with that device current, and only that address is handed to kernels.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int scratch_address(void **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, PRANDTL_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(nullptr, PRANDTL_ERR_UNSUPPORTED, "device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(nullptr, PRANDTL_ERR_HIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PRANDTL_OK;
+}
+
+struct BoxStates {
+    double v[30];
+};
+
+// Clears the running wave-speed slots and writes the boundary states (a
+// kernel, not a memset, so the call may sit in a captured graph).
+__global__ void k_noalloc_init(double *lam, int nlam, double *bcs, BoxStates b, int nbcs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nlam) lam[i] = 0.0;
+    if (i < nbcs) bcs[i] = b.v[i];
+}
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+} // namespace
+
+extern "C" {
+
+int prandtl_rhs_noalloc(const prandtl_mesh_desc *mesh, const prandtl_params *params, const double *u,
+                        double *dudt, void *stream)
+{
+    if (!mesh || !params || !u || !dudt) return fail(nullptr, PRANDTL_ERR_ARG, "null argument");
+    if (u == dudt) return fail(nullptr, PRANDTL_ERR_ARG, "u and dudt alias");
+    prandtl_ctx c;
+    prandtl_ctx *unused = nullptr;
+    int rc = create_impl(mesh, nullptr, params, &unused, nullptr, &c);
+    if (rc) return rc;
+    // scratch layout: wave-speed slots, alpha, eta, box states, viscous volume
+    // sums and face traces, each 256-byte aligned
+    const size_t NK = (size_t)c.NE * c.K;
+    const size_t sizes[6] = {sizeof(double) * c.NE, sizeof(double) * c.NE, sizeof(double) * c.NE,
+                             sizeof(double) * 30,
+                             c.prm.parabolic ? sizeof(double) * (c.nv - 1) * NK : 0,
+                             c.prm.parabolic ? sizeof(double) * (size_t)c.NE * 2 * c.dim * (c.nv - 1) * c.Nfp : 0};
+    size_t off[7] = {0};
+    for (int i = 0; i < 6; i++) off[i + 1] = off[i] + align256(sizes[i]);
+    if (off[6] > NOALLOC_SCRATCH_BYTES) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "the case needs %zu bytes of scratch; prandtl_rhs_noalloc holds %zu",
+                      off[6], NOALLOC_SCRATCH_BYTES);
+        return fail(nullptr, PRANDTL_ERR_ARG, msg);
+    }
+    void *base = nullptr;
+    rc = scratch_address(&base);
+    if (rc) return rc;
+    char *b = static_cast<char *>(base);
+    c.d_lam = reinterpret_cast<double *>(b + off[0]);
+    c.d_alpha = reinterpret_cast<double *>(b + off[1]);
+    c.d_eta = reinterpret_cast<double *>(b + off[2]);
+    c.d_bcs = reinterpret_cast<double *>(b + off[3]);
+    c.d_visc = c.prm.parabolic ? reinterpret_cast<double *>(b + off[4]) : nullptr;
+    c.d_fvf = c.prm.parabolic ? reinterpret_cast<double *>(b + off[5]) : nullptr;
+    c.geo.bcs = c.geo.bcmask ? c.d_bcs : nullptr;
+    c.rhs_algo = line_ok(&c) ? PRANDTL_ALGO_LINE : PRANDTL_ALGO_TILE;
+    BoxStates bs;
+    std::memcpy(bs.v, mesh->bc_state, sizeof(bs.v));
+    hipStream_t s = (hipStream_t)stream;
+    const int n = c.NE > 30 ? c.NE : 30;
+    hipLaunchKernelGGL(k_noalloc_init, dim3((n + 255) / 256), dim3(256), 0, s, c.d_lam, c.NE, c.d_bcs, bs,
+                       c.geo.bcmask ? 30 : 0);
+    rc = hip_check(&c, hipGetLastError(), "k_noalloc_init launch");
+    if (!rc) rc = stage(&c, u, nullptr, dudt, 0, 0.0, 0.0, 0.0, s);
+    if (rc) g_create_err = c.err;
+    return rc;
+}
+
+} // extern "C"
+
+namespace {
+
 static int rk3_stages(prandtl_ctx *c, double *u, double dt, hipStream_t s)
 {
     // y1 = u + dt L(u); y2 = 3/4 u + 1/4 (y1 + dt L(y1)); u = 1/3 u + 2/3 (y2 + dt L(y2))
@@ -813,6 +924,10 @@ static int rk3_stages(prandtl_ctx *c, double *u, double dt, hipStream_t s)
     if (!rc) rc = stage(c, c->d_yb, u, u, 2, dt, 1.0 / 3, 2.0 / 3, s);
     return rc;
 }
+
+} // namespace
+
+extern "C" {
 
 int prandtl_rk3_step(prandtl_ctx *c, double *u, double dt, void *stream)
 {
