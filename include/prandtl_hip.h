@@ -287,7 +287,9 @@ int prandtl_set_shape(prandtl_ctx *ctx, int kernel, int shape);
  * PRANDTL_ALGO_TILE: k_rhs, a workgroup's threads take the element's
  * two-point fluxes as jobs through LDS (every mesh kind).  PRANDTL_ALGO_LINE:
  * k_rhs_ln, one thread per tensor line (periodic Cartesian boxes in 3-D,
- * N = 2..4), the default where it exists.  The two differ by rounding only;
+ * N = 2..4), the default where it exists; with it, the mixed form at N = 3
+ * runs its viscous kernel's derivative contractions on the FP32 matrix cores
+ * (k_visc_mx).  The two differ by rounding only;
  * launch shapes (prandtl_set_shape, prandtl_autotune) belong to the current
  * algorithm (the line kernel has shape 0 only).  PRANDTL_ERR_UNSUPPORTED for
  * an algorithm this case does not have. */

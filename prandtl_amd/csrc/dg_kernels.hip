@@ -382,7 +382,8 @@ int rhs_only(prandtl_ctx *c, const double *y, double *out, hipStream_t s)
 bool shape_ok(const prandtl_ctx *c, int kernel, int shape)
 {
     if (kernel == PRANDTL_KERNEL_VISC)
-        return c->prm.parabolic && (shape == 0 || (!c->geo.bcmask && !c->gen && visc_shape_ok(c->dim, c->Np, shape)));
+        return c->prm.parabolic &&
+               (shape == 0 || (!c->geo.bcmask && !c->gen && !visc_mx_ok(c) && visc_shape_ok(c->dim, c->Np, shape)));
     if (kernel != PRANDTL_KERNEL_RHS) return false;
     if (shape == 0) return true;
     if (c->rhs_algo == PRANDTL_ALGO_LINE) return false;
@@ -986,6 +987,7 @@ int prandtl_set_algorithm(prandtl_ctx *c, int kernel, int algo)
         drop_graph(c);
         c->rhs_algo = algo;
         c->rhs_shape = 0;
+        c->visc_shape = 0;
     }
     return PRANDTL_OK;
 }

@@ -6,6 +6,7 @@
 #include <vector>
 #include "dg_device.h"
 #include "dg_line.h"
+#include "dg_visc_mx.h"
 
 using namespace prandtl;
 
@@ -106,6 +107,14 @@ template <int DIM, int NP, int FB> void launch_visc_fb(prandtl_ctx *c, const VAr
 // Orders the general-geometry kernels are built for: 2-D N = 1..5, 3-D N = 2..4.
 constexpr bool gen_built(int DIM, int NP) { return DIM == 2 ? NP <= 6 : DIM == 3 && NP <= 5; }
 
+// The MFMA viscous kernel serves the mixed form of the line algorithm on
+// periodic 3-D boxes at N = 3.
+inline bool visc_mx_ok(const prandtl_ctx *c)
+{
+    return c->rhs_algo == PRANDTL_ALGO_LINE && c->prm.precision == PRANDTL_MIXED && c->dim == 3 && c->Np == 4 &&
+           !c->gen && !c->axi && c->geo.bcmask == 0;
+}
+
 template <int DIM, int NP> int launch_visc_nd(prandtl_ctx *c, const double *y, double *grad, hipStream_t s)
 {
     VArgs a;
@@ -114,6 +123,14 @@ template <int DIM, int NP> int launch_visc_nd(prandtl_ctx *c, const double *y, d
     a.y = y; a.visc = c->d_visc; a.fvf = c->d_fvf; a.grad = grad;
     a.skip = c->skip;
     ProfScope ps(c, s, 1);
+    if constexpr (DIM == 3 && NP == 4) {
+        if (visc_mx_ok(c)) {
+            using Mc = MxCfg<NP>;
+            const int nb = (c->NE + Mc::EPB - 1) / Mc::EPB;
+            hipLaunchKernelGGL((k_visc_mx<NP>), dim3(nb), dim3(Mc::BLOCK), 0, s, a);
+            return hip_check(c, hipGetLastError(), "k_visc_mx launch");
+        }
+    }
     if constexpr (gen_built(DIM, NP)) {
         if (c->gen) {
             // quadrilateral or hexahedral mesh, natural shape

@@ -136,3 +136,44 @@ def test_line_kernel_graph_and_loop(cuda):
     torch.cuda.synchronize()
     assert torch.equal(ua, ub)
     assert n == 4 and torch.equal(uc, ua)
+
+
+def test_mixed_mfma_viscous_kernel(cuda):
+    """The mixed form's viscous kernel on the FP32 matrix cores (k_visc_mx, line
+    algorithm, 3-D N = 3) against the FP32 VALU kernel of the tile algorithm
+    and against the FP64 oracle: gradients within FP32 rounding of the
+    oracle's, RHS within 1e-6 relative L2 of the FP64 GPU RHS."""
+    import torch
+    cfg = dict(dim=3, order=3, nelem=(4, 3, 5), lo=(0.0,) * 3, hi=(TWO_PI, 2.0, 3.0), quirks=False, **NS)
+    X = cases.node_coords(3, 3, cfg["nelem"], cfg["lo"], cfg["hi"])
+    u = cases.random_smooth_state(X, 3, 41, L=[TWO_PI, 2.0, 3.0])
+    line = DGSEMOperator(precision="mixed", **cfg)
+    tile = DGSEMOperator(precision="mixed", **cfg)
+    tile.set_algorithm("tile")
+    fp64 = DGSEMOperator(precision="fp64", **cfg)
+    assert line.algorithm == "line"
+    a, b, ref = rhs(line, u, cuda), rhs(tile, u, cuda), rhs(fp64, u, cuda)
+    l2 = lambda x, y: float(np.linalg.norm(x - y) / np.linalg.norm(y))
+    print("mixed line vs fp64 %.3e, mixed tile vs fp64 %.3e, line vs tile %.3e" % (
+        l2(a, ref), l2(b, ref), l2(a, b)), flush=True)
+    assert l2(a, ref) <= 1e-6 and l2(b, ref) <= 1e-6
+    orc = O.Oracle(threads=8, **cfg)
+    orc.Mult(u)
+    g_ref = orc.gradients()
+    g = line.gradients(to_dev(u, cuda))
+    torch.cuda.synchronize()
+    rg = rel_max(g.cpu().numpy(), g_ref)
+    print("mixed line gradients vs oracle %.3e" % rg, flush=True)
+    assert rg <= 1e-5
+    # graph-captured steps of the mixed line form equal eager ones bitwise
+    s = torch.cuda.Stream(device=cuda)
+    with torch.cuda.stream(s):
+        p = DGSEMOperator(precision="mixed", **cfg)
+        q = DGSEMOperator(precision="mixed", **cfg)
+        q.set_graph(False)
+        up, uq = to_dev(u, cuda), to_dev(u, cuda)
+        for _ in range(3):
+            p.Step(up, 1e-3)
+            q.Step(uq, 1e-3)
+    torch.cuda.synchronize()
+    assert torch.equal(up, uq)
