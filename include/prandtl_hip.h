@@ -307,6 +307,32 @@ int prandtl_get_algorithm(const prandtl_ctx *ctx, int kernel, int *algo);
 int prandtl_autotune(prandtl_ctx *ctx, const double *u_dev, int *shapes_out, double *us_out,
                      void *stream);
 
+/* TimeDependentOperator::SetTime / GetTime (DGSEMOperator::GetTimeRef,
+ * DGSEMOperator.hpp:111-114): the time of the next RHS.  prandtl_rhs
+ * evaluates at it; prandtl_rk3_step evaluates its stages at t, t + dt and
+ * t + dt/2 (RK3SSPSolver's SetTime calls, Simulation.cpp:1010) and advances
+ * it by dt; prandtl_run runs from and leaves it at the loop's time.  Only
+ * time-dependent boundary states read it.  set_time orders the update on
+ * `stream` (a hipStream_t). */
+int prandtl_set_time(prandtl_ctx *ctx, double t, void *stream);
+int prandtl_get_time(const prandtl_ctx *ctx, double *t);
+
+/* A time-dependent outer state of boundary attribute attr (1-based) of a 2-D
+ * quadrilateral mesh, for its specified-state or supersonic-inflow type: the
+ * reference's vector TD-function boundary conditions (Simulation.cpp:
+ * 541-563, "time_dependent": true; SpecifiedStateBdrFaceIntegrator.cpp:
+ * 24-34 evaluates the function at each boundary point and the operator's
+ * time), in the form they take in the reference's cases: a straight
+ * discontinuity moving at constant speed.  At a boundary point x and stage
+ * time t the outer conservative state is below[] where
+ * normal[0] x0 + normal[1] x1 - offset - speed t < 0 and above[] elsewhere
+ * (DoubleMachReflectionTopBCFunction, TestCases/Euler/2D/
+ * DoubleMachReflection/DoubleMachReflection.hpp:33-57: normal = (1,
+ * -1/sqrt(3)), offset 1/6, speed 20/sqrt(3)).  Euler, non-axisymmetric
+ * contexts; PRANDTL_ERR_UNSUPPORTED otherwise. */
+int prandtl_set_boundary_split(prandtl_ctx *ctx, int attr, const double *normal, double offset, double speed,
+                               const double *below, const double *above);
+
 /* DGSEMOperator::GetMaxCharSpeed (DGSEMOperator.hpp:106-109): running
  * maximum of |v|+c over every state the operator has evaluated since
  * creation or the last reset (DGSEMIntegrator.cpp:39,188,270).  Synchronises

@@ -686,6 +686,11 @@ struct orc_ctx {
     std::vector<double> stage_y, stage_k;
     double lam;
     double thresh;
+    /* time of the next RHS (TimeDependentOperator::t) and of the RHS being
+     * evaluated; per boundary attribute of a quadrilateral mesh an optional
+     * split outer state [kind, n0, n1, n2, offset, speed, below[5], above[5]] */
+    double time = 0.0, t_stage = 0.0;
+    std::vector<double> attr_fn;
 };
 
 namespace {
@@ -1232,6 +1237,18 @@ void axis_recover(const orc_ctx *c, const double *rU, double *U)
 /* ------------------------------------------------------------------ */
 /* the RHS engine                                                       */
 /* ------------------------------------------------------------------ */
+/* n . x - offset - speed t for a split outer state fn = [kind, n0, n1, n2,
+ * offset, speed, ...], summed left to right without contraction */
+inline double split_side(const double *fn, const double *x, double t)
+{
+    double s = fn[1] * x[0];
+    s = s + fn[2] * x[1];
+    s = s + fn[3] * x[2];
+    s = s - fn[4];
+    s = s - fn[5] * t;
+    return s;
+}
+
 template <class T> struct Engine {
     orc_ctx *c;
     const Mesh &m;
@@ -1740,8 +1757,21 @@ template <class T> struct Engine {
             const int e = m.bq_e[b];
             T outer[5];
             for (int v = 0; v < nv; v++) outer[v] = m.attr_state[5 * (size_t)m.bq_attr[b] + v];
+            const double *fn = (c->attr_fn.size() > 16 * (size_t)m.bq_attr[b] &&
+                                c->attr_fn[16 * (size_t)m.bq_attr[b]] != 0.0)
+                                   ? &c->attr_fn[16 * (size_t)m.bq_attr[b]] : nullptr;
             for (int q = 0; q < Nfp; q++) {
                 const int n1 = m.bq_n[(size_t)b * Nfp + q];
+                if (fn) {
+                    /* the time-dependent outer state at this boundary point
+                     * (conserv_state_fun.SetTime(time); Eval at the point,
+                     * SpecifiedStateBdrFaceIntegrator.cpp:24-34): fn's "below"
+                     * state where n . x - offset - speed t < 0 */
+                    double x[3] = {0.0, 0.0, 0.0};
+                    for (int d = 0; d < dim; d++) x[d] = m.X[d * NK + (size_t)e * K + n1];
+                    const double sgn = split_side(fn, x, c->t_stage);
+                    for (int v = 0; v < nv; v++) outer[v] = sgn < 0.0 ? fn[6 + v] : fn[11 + v];
+                }
                 T s1[5], F[5], nor[3];
                 state(u, e, n1, s1);
                 for (int d = 0; d < dim; d++) nor[d] = m.bq_nor[((size_t)b * Nfp + q) * dim + d];
@@ -1863,14 +1893,20 @@ void rk3_step(orc_ctx *c, double *x, double dt)
     const size_t n = (size_t)c->m.nv * c->m.NE * c->m.K;
     std::vector<double> &y = c->stage_y, &k = c->stage_k;
     y.resize(n); k.resize(n);
+    /* stage times t, t + dt, t + dt/2 (RK3SSPSolver's SetTime calls) */
+    const double t = c->time;
+    c->t_stage = t;
     E.rhs(x, k.data(), c->p.threads);
     for (size_t i = 0; i < n; i++) y[i] = x[i] + dt * k[i];
+    c->t_stage = t + dt;
     E.rhs(y.data(), k.data(), c->p.threads);
     for (size_t i = 0; i < n; i++) y[i] += dt * k[i];
     for (size_t i = 0; i < n; i++) y[i] = 3. / 4 * x[i] + 1. / 4 * y[i];
+    c->t_stage = t + dt / 2;
     E.rhs(y.data(), k.data(), c->p.threads);
     for (size_t i = 0; i < n; i++) y[i] += dt * k[i];
     for (size_t i = 0; i < n; i++) x[i] = 1. / 3 * x[i] + 2. / 3 * y[i];
+    c->time = t + dt;
 }
 
 } /* namespace */
@@ -2125,8 +2161,33 @@ int orc_weights(const orc_ctx *c, double *WJ)
     return 0;
 }
 
+int orc_set_time(orc_ctx *c, double t)
+{
+    c->time = t;
+    return 0;
+}
+
+double orc_get_time(const orc_ctx *c) { return c->time; }
+
+int orc_set_boundary_split(orc_ctx *c, int attr, const double *normal, double offset, double speed,
+                           const double *below, const double *above)
+{
+    if (!c->m.quad || attr < 1) return 1;
+    const size_t na = c->m.attr_state.size() / 5;
+    if ((size_t)attr > na) return 1;
+    if (c->attr_fn.size() < 16 * na) c->attr_fn.assign(16 * na, 0.0);
+    double *f = &c->attr_fn[16 * (size_t)(attr - 1)];
+    f[0] = 1.0;
+    for (int d = 0; d < 3; d++) f[1 + d] = normal[d];
+    f[4] = offset;
+    f[5] = speed;
+    for (int v = 0; v < 5; v++) { f[6 + v] = below[v]; f[11 + v] = above[v]; }
+    return 0;
+}
+
 int orc_rhs(orc_ctx *c, const double *u, double *dudt)
 {
+    c->t_stage = c->time;
     Engine<double> E(c);
     E.rhs(u, dudt, c->p.threads);
     if (c->p.parabolic) c->grad.assign(E.G.begin(), E.G.end());

@@ -158,6 +158,18 @@ int orc_rhs(orc_ctx *c, const double *u, double *dudt);
  * higher-precision reference for rounding-error studies */
 int orc_rhs_ld(orc_ctx *c, const double *u, double *dudt);
 int orc_rk3_step(orc_ctx *c, double *u, double dt);
+/* TimeDependentOperator::SetTime/GetTime: the time of the next RHS; an RK3
+ * step evaluates its stages at t, t + dt, t + dt/2 (MFEM RK3SSPSolver) and
+ * advances the time by dt */
+int orc_set_time(orc_ctx *c, double t);
+double orc_get_time(const orc_ctx *c);
+/* A time-dependent outer state of boundary attribute attr (1-based) of a
+ * quadrilateral mesh (specified-state or supersonic-inflow boundaries, the
+ * reference's vector TD-function conditions, Simulation.cpp:541-563,
+ * SpecifiedStateBdrFaceIntegrator.cpp:24-34): below[] at the boundary points
+ * x with n . x - offset - speed t < 0, above[] elsewhere */
+int orc_set_boundary_split(orc_ctx *c, int attr, const double *normal, double offset, double speed,
+                           const double *below, const double *above);
 double orc_max_wave_speed(const orc_ctx *c);
 void orc_reset_wave_speed(orc_ctx *c);
 int orc_alpha(const orc_ctx *c, double *alpha /* NE */);

@@ -36,6 +36,7 @@ EXPORTED = (
     "prandtl_mesh_name", "prandtl_mesh_free", "prandtl_mesh_last_error", "prandtl_axis_recover",
     "prandtl_create_hex", "prandtl_mesh_dim", "prandtl_mesh_view_hex",
     "prandtl_set_algorithm", "prandtl_get_algorithm", "prandtl_rhs_noalloc",
+    "prandtl_set_time", "prandtl_get_time", "prandtl_set_boundary_split",
 )
 
 
@@ -106,6 +107,9 @@ def lib():
     L.prandtl_set_graph.argtypes = [vp, ctypes.c_int]
     L.prandtl_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.prandtl_set_algorithm.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.prandtl_set_time.argtypes = [vp, ctypes.c_double, vp]
+    L.prandtl_get_time.argtypes = [vp, dp]
+    L.prandtl_set_boundary_split.argtypes = [vp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, dp, dp]
     L.prandtl_get_algorithm.argtypes = [vp, ctypes.c_int, ip]
     L.prandtl_autotune.argtypes = [vp, vp, ip, dp, vp]
     L.prandtl_run.argtypes = [vp, vp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,

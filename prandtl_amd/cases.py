@@ -283,6 +283,39 @@ def _nagashima_free_stream(gamma=1.4, R_gas=287.05, Ma=2.0, p0=303975.0, a0=360.
     return np.array([rhoa, rhoa * ua, 0.0, pa / (gamma - 1.0) + 0.5 * rhoa * ua * ua])
 
 
+# The reference's double Mach reflection (TestCases/Euler/2D/
+# DoubleMachReflection: DoubleMachReflection.hpp, config.json): a Mach-10
+# shock at 60 degrees to the wall, post-shock state (8, 8 * 7.1447..,
+# -8 * 4.125, p = 116.5) ahead of x < 1/6 + y/sqrt(3), quiescent (1.4, 0, 0,
+# p = 1) behind; left and bottom1 hold the post-shock state, bottom2 is a
+# slip wall, right a supersonic outflow; the top boundary follows the shock:
+# the post-shock state where x < 1/6 + (y + 20 t)/sqrt(3).  Order 3,
+# blending (alpha_max 0.5, the reference's default), fixed dt 1e-4, one
+# refinement of its mesh (par_ref_levels 1), t_final 0.04.
+_DMR_U, _DMR_V = 7.144709581221619, 4.125
+DMR_POST = np.array([8.0, 8.0 * _DMR_U, -8.0 * _DMR_V,
+                     116.5 * 1.0 / (1.4 - 1.0) + 0.5 * 8.0 * (_DMR_U * _DMR_U + _DMR_V * _DMR_V)])
+DMR_PRE = np.array([1.4, 0.0, 0.0, 1.0 / (1.4 - 1.0)])
+DMR_TOP_SPLIT = dict(normal=(1.0, -1.0 / np.sqrt(3.0)), offset=1.0 / 6.0, speed=20.0 / np.sqrt(3.0),
+                     below=DMR_POST, above=DMR_PRE)
+# dt 1e-4 is the configured step; from the discontinuous initial state it
+# drives the pressure negative at 155 nodes next to the shock foot (1/6, 0) in
+# the first stage (alpha_max 0.5 caps the blending), while 5e-5 and 2e-5 keep
+# every node positive: dt_start is the step used for starting from the
+# initial condition.
+DMR = dict(mesh="DoubleMachReflection.msh", order=3, blending=True, alpha_max=0.5, dt=1e-4, dt_start=2e-5,
+           t_final=0.04,
+           refine=1, bc={"left": ("specified-state", DMR_POST), "bottom1": ("specified-state", DMR_POST),
+                         "bottom2": "slip", "right": "supersonic-outflow", "top": ("specified-state", DMR_PRE)},
+           split={"top": DMR_TOP_SPLIT})
+
+
+def dmr_state(X, gamma=1.4):
+    """DoubleMachReflectionIC: the post-shock state where x < 1/6 + y/sqrt(3)."""
+    post = X[0] < 1.0 / 6.0 + X[1] / np.sqrt(3.0)
+    return np.where(post[None], DMR_POST[:, None, None], DMR_PRE[:, None, None])
+
+
 RAMP_STATE = np.array([1.0, 2.0, 0.0, (1.0 / 1.4) / (1.4 - 1.0) + 0.5 * 1.0 * 2.0 * 2.0])
 RAMP = dict(mesh="Ramp15deg_v2.msh", order=1, blending=True, alpha_max=0.5, dt=1e-4, t_final=3.0,
             state=RAMP_STATE, axis_rho_floor=1e-8 * RAMP_STATE[0],

@@ -91,6 +91,9 @@ struct Geo {
     const int *fnb;       // [NE*4*NFP] neighbour node e2*K + n2 of each interior face point
     const int *fnf;       // [NE*4] 2 (e2*4 + f2) + reversed: the neighbour's face of each interior face
     const double *fnor;   // [NE*4*NFP*2] normal outward of the face's first element
+    const double *fxy;    // [NE*4*NFP*2] boundary face points' coordinates (quadrilateral meshes)
+    const double *bfn;    // null, or per attribute [16]: a split outer state [kind, n0, n1, n2,
+                          // offset, speed, below[5], above[5]] (0 kind: the constant state)
     const double *subn;   // [NE][2][NFP][NP-1][2] subcell interface normals
     const double *corner; // hexahedral meshes: [NE][8][3] element corners; the per-node metrics
                           // are recomputed from them (met and jac are not stored)
@@ -110,6 +113,8 @@ struct KArgs {
     int mode;             // 0: out = rhs; 1: out = y + dt rhs; 2: out = c0 u0 + c1 (y + dt rhs)
     double dt, c0, c1;
     const double *dtp;    // if set, dt is read from device memory (graph replays)
+    const double *tp;     // if set, the time of the step's first stage (device memory)
+    double tc;            // this stage's time is *tp + tc dt (RK3SSPSolver: 0, 1, 1/2)
     const double *y;      // the state (axisymmetric: the weighted state r U)
     const double *u0;
     double *out;
@@ -759,6 +764,24 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             double F[NV];
             if (bt != BC_PERIODIC) {
                 const double *bs = A.geo.bcs + (code >> 4) * 5;
+                double fs[5];
+                if (A.geo.bfn && A.geo.bfn[(code >> 4) * 16] != 0.0) {
+                    // the attribute's time-dependent outer state at this
+                    // point: "below" where n . x - offset - speed t < 0
+                    // (SpecifiedStateBdrFaceIntegrator.cpp:24-34), evaluated
+                    // left to right without contraction
+                    const double *fn = A.geo.bfn + (code >> 4) * 16;
+                    const double *xp = A.geo.fxy + (efo * NFP + l) * 2;
+                    const double tt = (A.tp ? *A.tp : 0.0) + A.tc * dt;
+                    double sd = __dmul_rn(fn[1], xp[0]);
+                    sd = __dadd_rn(sd, __dmul_rn(fn[2], xp[1]));
+                    sd = __dadd_rn(sd, __dmul_rn(fn[3], 0.0));
+                    sd = __dsub_rn(sd, fn[4]);
+                    sd = __dsub_rn(sd, __dmul_rn(fn[5], tt));
+#pragma unroll
+                    for (int v = 0; v < 5; v++) fs[v] = (sd < 0.0) ? fn[6 + v] : fn[11 + v];
+                    bs = fs;
+                }
                 const double lb = boundary_flux_vn<DIM>(bt, nvec, so, bs, P, quirk3d, F);
                 lam = (lam < lb) ? lb : lam;
                 if constexpr (AXI) {

@@ -113,6 +113,12 @@ __global__ void k_max_reduce(const double *slots, int n, double *out)
     }
 }
 __global__ void k_set_double(double *p, double v) { *p = v; }
+// t += dt after an RK3 step (dt from device memory in graph replays)
+__global__ void k_time_advance(double *t, const double *dtp, double dt, const int *skip)
+{
+    if (skip && *skip) return;
+    *t += dtp ? *dtp : dt;
+}
 __global__ void k_reset_int(int *p) { *p = 0; }
 
 
@@ -685,6 +691,8 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (e == hipSuccess) e = hipMalloc(&c->d_lam_out, sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&c->d_flag, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&c->d_dt, sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->d_time, sizeof(double));
+    if (e == hipSuccess) e = hipMemset(c->d_time, 0, sizeof(double));
     if (e == hipSuccess && c->geo.bcmask) {
         double hb[3][2][5];
         std::memcpy(hb, mesh->bc_state, sizeof(hb));
@@ -695,16 +703,16 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (e == hipSuccess && umesh) {
         // general geometry tables, one allocation
         auto bytes = [](const auto &v) { return v.size() * sizeof(v[0]); };
-        constexpr int NPART = 13;
+        constexpr int NPART = 14;
         const size_t parts[NPART] = {bytes(qg.met), bytes(qg.jac), bytes(qg.fnor), bytes(qg.subn), bytes(qstates),
                                      bytes(qg.fface), bytes(qg.fnb), bytes(qg.fnf), bytes(qg.rnode),
-                                     bytes(qg.axm), bytes(qg.axj), bytes(qg.axr), bytes(qg.corner)};
+                                     bytes(qg.axm), bytes(qg.axj), bytes(qg.axr), bytes(qg.corner), bytes(qg.fxy)};
         size_t off[NPART + 1] = {0};
         for (int i = 0; i < NPART; i++) off[i + 1] = off[i] + ((parts[i] + 255) / 256) * 256;
         e = hipMalloc(&c->d_geo, off[NPART]);
         const void *src[NPART] = {qg.met.data(), qg.jac.data(), qg.fnor.data(), qg.subn.data(), qstates.data(),
                                   qg.fface.data(), qg.fnb.data(), qg.fnf.data(), qg.rnode.data(),
-                                  qg.axm.data(), qg.axj.data(), qg.axr.data(), qg.corner.data()};
+                                  qg.axm.data(), qg.axj.data(), qg.axr.data(), qg.corner.data(), qg.fxy.data()};
         for (int i = 0; i < NPART && e == hipSuccess; i++)
             if (parts[i])
                 e = hipMemcpy(static_cast<char *>(c->d_geo) + off[i], src[i], parts[i], hipMemcpyHostToDevice);
@@ -718,6 +726,8 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         c->geo.fnb = reinterpret_cast<const int *>(b + off[6]);
         c->geo.fnf = reinterpret_cast<const int *>(b + off[7]);
         if (hm) c->geo.corner = reinterpret_cast<const double *>(b + off[12]);
+        if (qm) c->geo.fxy = reinterpret_cast<const double *>(b + off[13]);
+        c->nattr = qm ? qm->nattr : hm->nattr;
         if (c->axi) {
             c->geo.rnode = reinterpret_cast<const double *>(b + off[8]);
             c->geo.axm = reinterpret_cast<const int *>(b + off[9]);
@@ -790,6 +800,8 @@ void prandtl_destroy(prandtl_ctx *c)
     if (c->d_lam_out) (void)hipFree(c->d_lam_out);
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_dt) (void)hipFree(c->d_dt);
+    if (c->d_time) (void)hipFree(c->d_time);
+    if (c->d_bfn) (void)hipFree(c->d_bfn);
     if (c->d_bcs) (void)hipFree(c->d_bcs);
     if (c->d_geo) (void)hipFree(c->d_geo);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -919,10 +931,20 @@ namespace {
 
 static int rk3_stages(prandtl_ctx *c, double *u, double dt, hipStream_t s)
 {
-    // y1 = u + dt L(u); y2 = 3/4 u + 1/4 (y1 + dt L(y1)); u = 1/3 u + 2/3 (y2 + dt L(y2))
+    // y1 = u + dt L(u); y2 = 3/4 u + 1/4 (y1 + dt L(y1)); u = 1/3 u + 2/3 (y2 + dt L(y2)),
+    // evaluated at t, t + dt and t + dt/2 (RK3SSPSolver's SetTime calls)
+    c->stage_tc = 0.0;
     int rc = stage(c, u, nullptr, c->d_ya, 1, dt, 0.0, 0.0, s);
+    c->stage_tc = 1.0;
     if (!rc) rc = stage(c, c->d_ya, u, c->d_yb, 2, dt, 3.0 / 4, 1.0 / 4, s);
+    c->stage_tc = 0.5;
     if (!rc) rc = stage(c, c->d_yb, u, u, 2, dt, 1.0 / 3, 2.0 / 3, s);
+    c->stage_tc = 0.0;
+    if (!rc && !c->tp) {
+        // t += dt on the device (the device loop advances its own time)
+        hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(1), 0, s, c->d_time, c->dtp, dt, c->skip);
+        rc = hip_check(c, hipGetLastError(), "time advance");
+    }
     return rc;
 }
 
@@ -934,6 +956,7 @@ int prandtl_rk3_step(prandtl_ctx *c, double *u, double dt, void *stream)
 {
     if (!c || !u) return fail(c, PRANDTL_ERR_ARG, "null argument");
     hipStream_t s = (hipStream_t)stream;
+    c->t += dt;   // the host mirror of the device time the step advances
     // The null stream cannot be captured; profiling brackets single launches.
     if (!c->use_graph || c->prof || s == nullptr) return rk3_stages(c, u, dt, s);
     int rc = PRANDTL_OK;
@@ -973,6 +996,51 @@ int prandtl_set_shape(prandtl_ctx *c, int kernel, int shape)
     int &cur = kernel == PRANDTL_KERNEL_RHS ? c->rhs_shape : c->visc_shape;
     if (shape != cur) drop_graph(c);
     cur = shape;
+    return PRANDTL_OK;
+}
+
+int prandtl_set_time(prandtl_ctx *c, double t, void *stream)
+{
+    if (!c) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    if (!std::isfinite(t)) return fail(c, PRANDTL_ERR_ARG, "time must be finite");
+    c->t = t;
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(1), 0, (hipStream_t)stream, c->d_time, t);
+    return hip_check(c, hipGetLastError(), "set time");
+}
+
+int prandtl_get_time(const prandtl_ctx *c, double *t)
+{
+    if (!c || !t) return PRANDTL_ERR_ARG;
+    *t = c->t;
+    return PRANDTL_OK;
+}
+
+int prandtl_set_boundary_split(prandtl_ctx *c, int attr, const double *normal, double offset, double speed,
+                               const double *below, const double *above)
+{
+    if (!c || !normal || !below || !above) return fail(c, PRANDTL_ERR_ARG, "null argument");
+    if (!c->gen || c->dim != 2 || c->axi || c->prm.parabolic)
+        return fail(c, PRANDTL_ERR_UNSUPPORTED,
+                    "time-dependent outer states are built for 2-D quadrilateral meshes, Euler, non-axisymmetric");
+    if (attr < 1 || attr > c->nattr) return fail(c, PRANDTL_ERR_ARG, "boundary attribute out of range");
+    for (const double *g : {below, above}) {
+        const double ke = 0.5 * (g[1] * g[1] + g[2] * g[2]) / g[0];
+        const double pg = (c->prm.gamma - 1.0) * (g[3] - ke);
+        if (!(g[0] > 0.0) || !(pg > 0.0) || !std::isfinite(pg))
+            return fail(c, PRANDTL_ERR_ARG, "boundary state needs positive density and pressure");
+    }
+    hipError_t e = hipSuccess;
+    if (!c->d_bfn) {
+        e = hipMalloc(&c->d_bfn, sizeof(double) * 16 * c->nattr);
+        if (e == hipSuccess) e = hipMemset(c->d_bfn, 0, sizeof(double) * 16 * c->nattr);
+        if (e != hipSuccess) return hip_check(c, e, "boundary state table");
+    }
+    double f[16] = {1.0, normal[0], normal[1], 0.0, offset, speed};
+    for (int v = 0; v < 4; v++) { f[6 + v] = below[v]; f[11 + v] = above[v]; }
+    e = hipMemcpy(c->d_bfn + 16 * (size_t)(attr - 1), f, sizeof(f), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_check(c, e, "boundary state table");
+    c->geo.bfn = c->d_bfn;
+    drop_graph(c);
     return PRANDTL_OK;
 }
 
@@ -1067,7 +1135,9 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
     rc = hip_check(c, hipMemcpyAsync(c->d_loop, &L0, sizeof(L0), hipMemcpyHostToDevice, s), "loop init");
     if (!rc && variable) {
         // the initial RHS evaluation that seeds the wave speed (Simulation.cpp:903)
+        c->tp = &c->d_loop->t;
         rc = stage(c, u, nullptr, c->d_yb, 0, 0.0, 0.0, 0.0, s);
+        c->tp = nullptr;
         if (!rc) {
             hipLaunchKernelGGL(k_loop_dt, dim3(1), dim3(LOOP_BLOCK), 0, s, c->d_loop, c->d_lam, c->NE, cfl_hmin, np1sq,
                                t_final);
@@ -1080,6 +1150,7 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
     auto batch = [&]() -> int {
         c->skip = &c->d_loop->done;
         c->dtp = &c->d_loop->dt_real;
+        c->tp = &c->d_loop->t;
         int r2 = PRANDTL_OK;
         for (int k = 0; k < BATCH && !r2; k++) {
             r2 = rk3_stages(c, u, 0.0, s);
@@ -1108,6 +1179,7 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
         }
         c->skip = nullptr;
         c->dtp = nullptr;
+        c->tp = nullptr;
         return r2;
     };
     const bool use_graph = c->use_graph && !c->prof && s != nullptr;
@@ -1132,6 +1204,11 @@ int prandtl_run(prandtl_ctx *c, double *u, double *t_io, double t_final, double 
     if (g) (void)hipGraphDestroy(g);
     if (rc) return rc;
     *t_io = L.t;
+    // the operator's time follows the loop's
+    c->t = L.t;
+    hipLaunchKernelGGL(k_set_double, dim3(1), dim3(1), 0, s, c->d_time, L.t);
+    rc = hip_check(c, hipGetLastError(), "set time");
+    if (rc) return rc;
     if (steps_out) *steps_out = L.steps;
     if (L.bad)
         return fail(c, PRANDTL_ERR_STATE,

@@ -42,6 +42,15 @@ struct prandtl_ctx {
     const double *graph_u = nullptr;
     hipStream_t graph_stream = nullptr;
     const double *dtp = nullptr;  // set while launching stages for a graph
+    // time (TimeDependentOperator::t): host mirror, device copy, the device
+    // time a stage reads (the loop's while prandtl_run launches) and the
+    // stage's offset factor (0, 1, 1/2 of dt in an RK3 step)
+    double t = 0.0;
+    double *d_time = nullptr;
+    const double *tp = nullptr;
+    double stage_tc = 0.0;
+    double *d_bfn = nullptr;       // split outer states per attribute (quadrilateral meshes)
+    int nattr = 0;
     double graph_dt = 0.0;        // value last copied to d_dt
     bool graph_dt_set = false;
     int rhs_algo = 0;             // flux-differencing kernel: PRANDTL_ALGO_TILE (k_rhs) or _LINE (k_rhs_ln)
@@ -285,6 +294,8 @@ inline KArgs make_kargs(const prandtl_ctx *c, const double *y, const double *u0,
     a.ec = c->prm.ec_surface; a.quirks = c->prm.quirks;
     a.mode = mode; a.dt = dt; a.c0 = c0; a.c1 = c1;
     a.dtp = dtp;
+    a.tp = c->tp ? c->tp : c->d_time;
+    a.tc = c->stage_tc;
     a.y = y; a.u0 = u0; a.out = out;
     a.visc = c->d_visc; a.fvf = c->d_fvf; a.alpha = c->d_alpha; a.eta = c->d_eta;
     a.lam = c->d_lam;

@@ -35,6 +35,19 @@ int mesh_fail(const std::string &msg)
     return PRANDTL_ERR_IO;
 }
 
+// a + b and a * b rounded as written (no fused multiply-add), so that
+// coordinates match a restatement compiled without contraction
+double add_nc(double a, double b)
+{
+    volatile double r = a + b;
+    return r;
+}
+double mul_nc(double a, double b)
+{
+    volatile double r = a * b;
+    return r;
+}
+
 long long edge_key(int a, int b)
 {
     const int lo = a < b ? a : b, hi = a < b ? b : a;
@@ -540,6 +553,7 @@ bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
     g.fnb.assign((size_t)NE * 4 * NFP, -1);
     g.fnf.assign((size_t)NE * 4, -1);
     g.fnor.assign((size_t)NE * 4 * NFP * 2, 0.0);
+    g.fxy.assign((size_t)NE * 4 * NFP * 2, 0.0);
     std::vector<char> axis_el(NE, 0);   // elements with an axis boundary face
     for (int e = 0; e < NE; e++)
         for (int f = 0; f < 4; f++) {
@@ -584,6 +598,22 @@ bool build(const prandtl_quad_mesh *m, int order, const std::vector<double> &x,
                     t = PRANDTL_BC_SYMMETRY;
                 }
                 code = FACE_OWN | (t << 1) | ((attr - 1) << 4);
+                // the boundary points' coordinates, for outer states that
+                // depend on position (the bilinear map of the corners, summed
+                // in corner order without contraction)
+                double c[4][2];
+                corners(e, c);
+                const int r = f >> 1, side = f & 1;
+                for (int l = 0; l < NFP; l++) {
+                    const int nd = line_node2(NP, r, l, side ? N : 0);
+                    const double xi = x[nd % NP], eta = x[nd / NP];
+                    const double Nk[4] = {(1 - xi) * (1 - eta), xi * (1 - eta), xi * eta, (1 - xi) * eta};
+                    for (int a = 0; a < 2; a++) {
+                        double xa = 0.0;
+                        for (int k = 0; k < 4; k++) xa = add_nc(xa, mul_nc(Nk[k], c[k][a]));
+                        g.fxy[(ef * NFP + l) * 2 + a] = xa;
+                    }
+                }
             }
             g.fface[ef] = code;
             // CalcOrtho of the first element's edge, counterclockwise: (dy, -dx)

@@ -30,6 +30,7 @@ struct Geometry {
     std::vector<int> fnb;       // [NE*4*NFP]: neighbour node e2*K + n2 (interior faces)
     std::vector<int> fnf;       // [NE*4]: 2 (e2*4 + f2) + reversed, the neighbour's face (interior faces)
     std::vector<double> fnor;   // [NE*4*NFP*2]: normal outward of the face's first element
+    std::vector<double> fxy;    // [NE*4*NFP*2]: coordinates of boundary face points (0 on interior faces)
     std::vector<double> subn;   // [NE][2][NFP][NP-1][2]: subcell interface normals
     double hmin = 0.0;
     std::vector<double> corner; // hexahedral meshes: [NE][8][3] element corners

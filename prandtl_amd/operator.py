@@ -504,6 +504,31 @@ class DGSEMOperator:
         k = {"rhs": _lib.PRANDTL_KERNEL_RHS, "visc": _lib.PRANDTL_KERNEL_VISC}[kernel]
         check(lib().prandtl_set_shape(self._h, k, int(shape)), self._h)
 
+    def set_time(self, t):
+        """TimeDependentOperator::SetTime: the time of the next RHS (RK3 steps
+        evaluate their stages at t, t + dt, t + dt/2 and advance it by dt)."""
+        check(lib().prandtl_set_time(self._h, float(t), _stream_ptr()), self._h)
+
+    @property
+    def time(self):
+        t = ctypes.c_double()
+        check(lib().prandtl_get_time(self._h, ctypes.byref(t)), self._h)
+        return t.value
+
+    def set_boundary_split(self, boundary, normal, offset, speed, below, above):
+        """A time-dependent outer state of a specified-state or supersonic-inflow
+        boundary (name or attribute) of a quadrilateral mesh: `below` where
+        normal . x - offset - speed t < 0 at the boundary point, `above`
+        elsewhere (conservative states)."""
+        a = self.mesh.names[boundary] if isinstance(boundary, str) else int(boundary)
+        dp = ctypes.POINTER(ctypes.c_double)
+        nv = np.zeros(3)
+        nv[:len(normal)] = normal
+        b = np.ascontiguousarray(below, dtype=np.float64)
+        ab = np.ascontiguousarray(above, dtype=np.float64)
+        check(lib().prandtl_set_boundary_split(self._h, a, nv.ctypes.data_as(dp), float(offset), float(speed),
+                                               b.ctypes.data_as(dp), ab.ctypes.data_as(dp)), self._h)
+
     def set_algorithm(self, algo):
         """Kernel algorithm of the flux-differencing RHS: "tile" (k_rhs, every
         mesh kind) or "line" (k_rhs_ln, one thread per tensor line: periodic

@@ -101,6 +101,10 @@ def lib():
         L.orc_rhs.argtypes = [ctypes.c_void_p, _P, _P]
         L.orc_rhs_ld.argtypes = [ctypes.c_void_p, _P, _P]
         L.orc_rk3_step.argtypes = [ctypes.c_void_p, _P, _D]
+        L.orc_set_time.argtypes = [ctypes.c_void_p, _D]
+        L.orc_get_time.restype = _D
+        L.orc_get_time.argtypes = [ctypes.c_void_p]
+        L.orc_set_boundary_split.argtypes = [ctypes.c_void_p, ctypes.c_int, _P, _D, _D, _P, _P]
         L.orc_max_wave_speed.restype = _D
         L.orc_max_wave_speed.argtypes = [ctypes.c_void_p]
         L.orc_reset_wave_speed.argtypes = [ctypes.c_void_p]
@@ -321,6 +325,25 @@ class Oracle:
         out = np.zeros_like(u)
         assert lib().orc_rhs_ld(self.h, ptr(u), ptr(out)) == 0
         return out
+
+    def set_time(self, t):
+        assert lib().orc_set_time(self.h, float(t)) == 0
+
+    @property
+    def time(self):
+        return lib().orc_get_time(self.h)
+
+    def set_boundary_split(self, attr, normal, offset, speed, below, above):
+        """below where normal . x - offset - speed t < 0 at a boundary point of
+        attribute attr (1-based), above elsewhere."""
+        nv = np.zeros(3)
+        nv[:len(normal)] = normal
+        b = np.zeros(5)
+        b[:len(below)] = below
+        a = np.zeros(5)
+        a[:len(above)] = above
+        assert lib().orc_set_boundary_split(self.h, int(attr), ptr(nv), float(offset), float(speed), ptr(b),
+                                            ptr(a)) == 0
 
     def Step(self, u, dt):
         u = np.array(u, dtype=np.float64).reshape(-1)
