@@ -288,6 +288,9 @@ def run_ensemble(args):
         dts.append(0.5 * op.hmin / (op.GetMaxCharSpeed(reset=True) * (c["order"] + 1) ** 2))
         us.append(u)
     nodes = sum(op.NE * op.K for op in ens.ops)
+    # launch form: one workgroup per tile, or persistent with g workgroups
+    # per CU; the fastest first stage on these states is kept
+    form, form_us = (0, None) if args.no_autotune else ens.autotune(us, dts)
 
     def ens_steps(k):
         for _ in range(k):
@@ -327,8 +330,9 @@ def run_ensemble(args):
     flops_launch = (counts["volume"] + counts["surface"] + counts["fv"] + counts["indicator"]
                     + counts["assembly"] + 4 * nv * nodes)
     bytes_launch = 8 * nv * (1 + 1 + 2.0 / 3.0) * nodes
-    roof = roofline(flops_launch, bytes_launch, t_launch, "k_ens (8 variants N=2..5 x blending)",
-                    measured_traffic(ENSEMBLE, "k_ens"))
+    kname = "k_ens" if form == 0 else "k_ens_q"
+    roof = roofline(flops_launch, bytes_launch, t_launch, kname + " (8 variants N=2..5 x blending)",
+                    measured_traffic(ENSEMBLE, kname))
     value = nodes * 3 * args.steps / elapsed
     line = {
         "metric": METRIC,
@@ -347,6 +351,8 @@ def run_ensemble(args):
                                "Kelvin-Helmholtz, meshes {8..64}^2, N 2..5, seed 20261202",
                    "cases": len(case_list), "nodes": nodes, "vars": nv, "rhs_per_step": 3,
                    "precision": precision, "state_ok": bool(ok), "hip_graph": True,
+                   "launch_form": {"workgroups_per_cu": form or None,
+                                   "stage_us_by_form": [round(t, 2) for t in form_us] if form_us else None},
                    "sequential_per_case": {"value": nodes * 3 * args.steps / elapsed_seq,
                                            "ms_per_step": elapsed_seq / args.steps * 1e3,
                                            "launches_per_step": 3 * len(case_list)},

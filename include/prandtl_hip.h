@@ -420,6 +420,21 @@ int prandtl_ensemble_rk3_step(prandtl_ensemble *ens, double *const *u_dev, const
                               void *stream);
 const char *prandtl_ensemble_last_error(const prandtl_ensemble *ens);
 
+/* Launch form of the ensemble's stages (no counterpart in the reference):
+ * mode 0 launches one workgroup per element tile (the cost-sorted tile list
+ * is the dispatch order); mode g = 1..PRANDTL_ENS_MAX_PER_CU launches g
+ * persistent workgroups per CU that take tiles from a device counter until
+ * the list is exhausted.  Every form gives bitwise the same results. */
+#define PRANDTL_ENS_MAX_PER_CU 4
+int prandtl_ensemble_set_mode(prandtl_ensemble *ens, int mode);
+/* Autotuner of the launch form: binds u_dev / dt as prandtl_ensemble_rk3_step
+ * would and times the first RK stage (which writes only the cases' stage
+ * buffers, not u_dev) in every form, 2 + 10 launches each; keeps the fastest.
+ * mode_out (may be NULL) receives it, us_out (may be NULL) the
+ * PRANDTL_ENS_MAX_PER_CU + 1 mean microseconds per stage, form 0 first. */
+int prandtl_ensemble_autotune(prandtl_ensemble *ens, double *const *u_dev, const double *dt, int *mode_out,
+                              double *us_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,7 @@ PRANDTL_KERNEL_RHS = 0
 PRANDTL_KERNEL_VISC = 1
 PRANDTL_ALGO_TILE = 0
 PRANDTL_ALGO_LINE = 1
+PRANDTL_ENS_MAX_PER_CU = 4
 
 EXPORTED = (
     "prandtl_create", "prandtl_destroy", "prandtl_sizes", "prandtl_rhs", "prandtl_rk3_step",
@@ -37,6 +38,7 @@ EXPORTED = (
     "prandtl_create_hex", "prandtl_mesh_dim", "prandtl_mesh_view_hex",
     "prandtl_set_algorithm", "prandtl_get_algorithm", "prandtl_rhs_noalloc",
     "prandtl_set_time", "prandtl_get_time", "prandtl_set_boundary_split",
+    "prandtl_ensemble_set_mode", "prandtl_ensemble_autotune",
 )
 
 
@@ -108,6 +110,8 @@ def lib():
     L.prandtl_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.prandtl_set_algorithm.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.prandtl_set_time.argtypes = [vp, ctypes.c_double, vp]
+    L.prandtl_ensemble_set_mode.argtypes = [vp, ctypes.c_int]
+    L.prandtl_ensemble_autotune.argtypes = [vp, vp, dp, ip, dp, vp]
     L.prandtl_get_time.argtypes = [vp, dp]
     L.prandtl_set_boundary_split.argtypes = [vp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, dp, dp]
     L.prandtl_get_algorithm.argtypes = [vp, ctypes.c_int, ip]

@@ -505,7 +505,7 @@ template <int NP> __device__ __forceinline__ int face_point_3(int node, int r)
 template <int DIM, int NP, bool BLEND, bool QNS3, typename VT, int FB, bool OPEN = false, bool GEN = false,
           bool AXI = false>
 __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const int slot,
-                                         RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN, AXI> &S)
+                                         RhsSmem<DIM, NP, BLEND, QNS3, FB, GEN, AXI> &S, const int tid = -1)
 {
     using C = RCfg<DIM, NP, BLEND, FB>;
     constexpr int NV = C::NV, K = C::K, NFP = C::NFP, NPAIR = C::NPAIR, NFV = C::NFV;
@@ -531,7 +531,9 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     auto &sRed = S.Red;
 
     if (A.skip && *A.skip) return;   // uniform: before any barrier
-    const int t = threadIdx.x;
+    // tid: the thread's index as the caller sees it (a loop over tiles passes
+    // one the compiler cannot hoist out of the loop)
+    const int t = tid < 0 ? (int)threadIdx.x : tid;
     for (int i = t; i < NP * NP; i += BLOCK) {
         sDh2[i] = A.tab.Dh2[(i / NP) * MAXNP + i % NP];
         if (BLEND) sV1i[i] = A.tab.V1i[(i / NP) * MAXNP + i % NP];

@@ -54,9 +54,10 @@ constexpr size_t ENS_SMEM = cmax(cmax(cmax(ens_smem<3, false>(), ens_smem<3, tru
                                  cmax(cmax(ens_smem<5, false>(), ens_smem<5, true>()), cmax(ens_smem<6, false>(), ens_smem<6, true>())));
 
 template <int NP, bool BLEND>
-__device__ __forceinline__ void ens_tile(const KArgs &A, int g, double *smem)
+__device__ __forceinline__ void ens_tile(const KArgs &A, int g, double *smem, int tid = -1)
 {
-    rhs_body<2, NP, BLEND, false, double, ENS_BLOCK>(A, g, g, *reinterpret_cast<RhsSmem<2, NP, BLEND, false, ENS_BLOCK> *>(smem));
+    rhs_body<2, NP, BLEND, false, double, ENS_BLOCK>(A, g, g, *reinterpret_cast<RhsSmem<2, NP, BLEND, false, ENS_BLOCK> *>(smem),
+                                                     tid);
 }
 
 // grid = number of tiles; every field of a tile is uniform over the workgroup
@@ -77,6 +78,60 @@ k_ens(const KArgs *__restrict__ cases, const EnsTile *__restrict__ tiles)
     case 7: ens_tile<6, true>(A, T.g, smem); break;
     default: break;
     }
+}
+
+// Persistent form: a fixed grid (a chosen number of workgroups per CU) loops
+// over the same cost-sorted tile list, each workgroup taking the next tile
+// from a device counter until the list is exhausted (every workgroup leaves
+// the loop when the counter passes the list's end; no workgroup waits on
+// another).  Arithmetic is that of k_ens, so results are bitwise the same.
+//
+// Two details keep the loop a plain uniform loop in the generated code:
+//  - the thread index is passed through an empty asm once per iteration, and
+//    both the fetch (tid == 0) and the tile body use that value.  With
+//    threadIdx.x itself the compiler threads the body's final thread-0 branch
+//    (the wave-speed slot update) straight into the next iteration's fetch;
+//    the back edge then splits wave 0 (lane 0 to the atomic, lanes 1..63 to
+//    the barrier), and the lanes that skip the fetch re-read a stale tile
+//    index forever.  The opaque value also keeps each variant's index
+//    arithmetic inside the loop instead of hoisted and live across it;
+//  - the tile index read back from LDS goes through readfirstlane, so the
+//    tile's fields, the variant branch and the case's arguments are scalar
+//    as in k_ens.
+__global__ void __launch_bounds__(ENS_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_ens_q(const KArgs *__restrict__ cases, const EnsTile *__restrict__ tiles, int ntiles, int *counter)
+{
+    __shared__ __align__(16) double smem[(ENS_SMEM + 7) / 8];
+    __shared__ int s_tile;
+    for (;;) {
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        if (tid == 0) s_tile = atomicAdd(counter, 1);
+        __syncthreads();
+        const int i = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();   // every thread has read s_tile before the next fetch
+        if (i >= ntiles) break;
+        const EnsTile T = tiles[i];
+        const KArgs &A = cases[T.c];
+        switch (T.v) {
+        case 0: ens_tile<3, false>(A, T.g, smem, tid); break;
+        case 1: ens_tile<3, true>(A, T.g, smem, tid); break;
+        case 2: ens_tile<4, false>(A, T.g, smem, tid); break;
+        case 3: ens_tile<4, true>(A, T.g, smem, tid); break;
+        case 4: ens_tile<5, false>(A, T.g, smem, tid); break;
+        case 5: ens_tile<5, true>(A, T.g, smem, tid); break;
+        case 6: ens_tile<6, false>(A, T.g, smem, tid); break;
+        case 7: ens_tile<6, true>(A, T.g, smem, tid); break;
+        default: break;
+        }
+    }
+}
+
+// the three stage counters to zero (a kernel, not a memset: it sits in the
+// step's graph)
+__global__ void k_zero_counters(int *c)
+{
+    if (threadIdx.x < 3) c[threadIdx.x] = 0;
 }
 
 constexpr int ENS_MAX_CASES = 256;
@@ -1531,6 +1586,10 @@ struct prandtl_ensemble {
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
     std::string err;
+    // launch form: 0 one workgroup per tile; g > 0 persistent, g workgroups per CU
+    int mode = 0;
+    int ncu = 0;
+    int *d_counters = nullptr;   // [3] tile counters of the persistent form, one per stage
 };
 
 namespace {
@@ -1594,10 +1653,20 @@ int ens_bind(prandtl_ensemble *E, double *const *u, hipStream_t s)
     return PRANDTL_OK;
 }
 
+// one stage k of every case in the current launch form
+void ens_stage(prandtl_ensemble *E, int k, int mode, hipStream_t s)
+{
+    if (mode == 0)
+        hipLaunchKernelGGL(k_ens, dim3(E->ntiles), dim3(ENS_BLOCK), 0, s, E->d_args[k], E->d_tiles);
+    else
+        hipLaunchKernelGGL(k_ens_q, dim3(mode * E->ncu), dim3(ENS_BLOCK), 0, s, E->d_args[k], E->d_tiles, E->ntiles,
+                           E->d_counters + k);
+}
+
 int ens_stages(prandtl_ensemble *E, hipStream_t s)
 {
-    for (int k = 0; k < 3; k++)
-        hipLaunchKernelGGL(k_ens, dim3(E->ntiles), dim3(ENS_BLOCK), 0, s, E->d_args[k], E->d_tiles);
+    if (E->mode > 0) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, E->d_counters);
+    for (int k = 0; k < 3; k++) ens_stage(E, k, E->mode, s);
     return ens_hip(E, hipGetLastError(), "k_ens launch");
 }
 
@@ -1646,6 +1715,12 @@ int prandtl_ensemble_create(int ncases, const prandtl_mesh_desc *meshes, const p
     for (int k = 0; k < 3 && e == hipSuccess; k++) e = hipMalloc(&E->d_args[k], sizeof(KArgs) * ncases);
     if (e == hipSuccess) e = hipMalloc(&E->d_tiles, sizeof(EnsTile) * E->ntiles);
     if (e == hipSuccess) e = hipMalloc(&E->d_dts, sizeof(double) * ncases);
+    if (e == hipSuccess) e = hipMalloc(&E->d_counters, sizeof(int) * 3);
+    if (e == hipSuccess) {
+        int dev = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&E->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    }
     if (e == hipSuccess) e = hipMemcpy(E->d_tiles, tiles.data(), sizeof(EnsTile) * E->ntiles, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         g_create_err = std::string("ensemble device allocation: ") + hipGetErrorString(e);
@@ -1665,6 +1740,7 @@ void prandtl_ensemble_destroy(prandtl_ensemble *E)
         if (E->d_args[k]) (void)hipFree(E->d_args[k]);
     if (E->d_tiles) (void)hipFree(E->d_tiles);
     if (E->d_dts) (void)hipFree(E->d_dts);
+    if (E->d_counters) (void)hipFree(E->d_counters);
     for (prandtl_ctx *c : E->ctx) prandtl_destroy(c);
     delete E;
 }
@@ -1716,6 +1792,64 @@ int prandtl_ensemble_rk3_step(prandtl_ensemble *E, double *const *u, const doubl
         E->graph_stream = s;
     }
     return ens_hip(E, hipGraphLaunch(E->exec, s), "graph launch");
+}
+
+int prandtl_ensemble_set_mode(prandtl_ensemble *E, int mode)
+{
+    if (!E || mode < 0 || mode > PRANDTL_ENS_MAX_PER_CU) return ens_fail(E, PRANDTL_ERR_ARG, "bad launch form");
+    if (mode != E->mode) {
+        if (E->exec) { (void)hipGraphExecDestroy(E->exec); E->exec = nullptr; }
+        if (E->graph) { (void)hipGraphDestroy(E->graph); E->graph = nullptr; }
+        E->mode = mode;
+    }
+    return PRANDTL_OK;
+}
+
+int prandtl_ensemble_autotune(prandtl_ensemble *E, double *const *u, const double *dt, int *mode_out, double *us_out,
+                              void *stream)
+{
+    if (!E || !u || !dt) return ens_fail(E, PRANDTL_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    // bind the states and step sizes as a step would, then time the first
+    // stage (it writes only the cases' stage buffers) in every launch form
+    int rc = ens_bind(E, u, s);
+    if (rc) return rc;
+    if (E->exec) { (void)hipGraphExecDestroy(E->exec); E->exec = nullptr; }
+    if (E->graph) { (void)hipGraphDestroy(E->graph); E->graph = nullptr; }
+    DtArgs da;
+    for (int i = 0; i < E->n; i++) da.dt[i] = dt[i];
+    hipLaunchKernelGGL(k_set_dts, dim3(1), dim3(ENS_MAX_CASES), 0, s, E->d_dts, E->n, da);
+    E->dt.assign(dt, dt + E->n);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = ens_hip(E, hipEventCreate(&e0), "event");
+    if (!rc) rc = ens_hip(E, hipEventCreate(&e1), "event");
+    int best = 0;
+    double best_us = 0.0;
+    for (int mode = 0; mode <= PRANDTL_ENS_MAX_PER_CU && !rc; mode++) {
+        constexpr int warm = 2, reps = 10;
+        auto once = [&]() {
+            if (mode > 0) hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, E->d_counters);
+            ens_stage(E, 0, mode, s);
+        };
+        for (int i = 0; i < warm; i++) once();
+        rc = ens_hip(E, hipEventRecord(e0, s), "event record");
+        for (int i = 0; i < reps && !rc; i++) once();
+        if (!rc) rc = ens_hip(E, hipGetLastError(), "k_ens launch");
+        if (!rc) rc = ens_hip(E, hipEventRecord(e1, s), "event record");
+        if (!rc) rc = ens_hip(E, hipEventSynchronize(e1), "event sync");
+        float ms = 0.0f;
+        if (!rc) rc = ens_hip(E, hipEventElapsedTime(&ms, e0, e1), "event time");
+        if (rc) break;
+        const double us = 1e3 * ms / reps;
+        if (us_out) us_out[mode] = us;
+        if (mode == 0 || us < best_us) { best = mode; best_us = us; }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    E->mode = best;
+    if (mode_out) *mode_out = best;
+    return PRANDTL_OK;
 }
 
 const char *prandtl_ensemble_last_error(const prandtl_ensemble *E) { return E ? E->err.c_str() : g_create_err.c_str(); }

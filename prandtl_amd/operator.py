@@ -663,6 +663,22 @@ class Ensemble:
         dt = (ctypes.c_double * self.n)(*[float(v) for v in dts])
         _lib.check_ensemble(lib().prandtl_ensemble_rk3_step(self._h, ptrs, dt, _stream_ptr()), self._h)
 
+    def set_mode(self, mode):
+        """Launch form: 0 one workgroup per tile, g > 0 persistent with g
+        workgroups per CU over a device tile counter (bitwise the same)."""
+        _lib.check_ensemble(lib().prandtl_ensemble_set_mode(self._h, int(mode)), self._h)
+
+    def autotune(self, us, dts):
+        """Time the first RK stage in every launch form on states us (not
+        modified) and keep the fastest; returns (mode, [us per form])."""
+        ptrs = (ctypes.c_void_p * self.n)(*[op._vec(u, "u").value for op, u in zip(self.ops, us)])
+        dt = (ctypes.c_double * self.n)(*[float(v) for v in dts])
+        mode = ctypes.c_int()
+        times = (ctypes.c_double * (_lib.PRANDTL_ENS_MAX_PER_CU + 1))()
+        _lib.check_ensemble(lib().prandtl_ensemble_autotune(self._h, ptrs, dt, ctypes.byref(mode), times,
+                                                            _stream_ptr()), self._h)
+        return mode.value, list(times)
+
 
 class RK3SSPSolver:
     """mfem::RK3SSPSolver surface: Init(op) then Step(x, t, dt) -> (t, dt)."""

@@ -41,11 +41,14 @@ def step_sizes(case_list, scale=1.0):
             / (3.0 * (c["order"] + 1) ** 2) for c in case_list]
 
 
-def run_pair(case_list, dev, dt_sets, graph_stream=True):
-    """Step an ensemble and separate per-case operators through the same
-    sequence of step sizes; return both final state lists."""
+def run_pair(case_list, dev, dt_sets, graph_stream=True, mode=None):
+    """Step an ensemble (launch form `mode`, default: as created) and separate
+    per-case operators through the same sequence of step sizes; return both
+    final state lists."""
     import torch
     ens = Ensemble(case_list)
+    if mode is not None:
+        ens.set_mode(mode)
     ops = [DGSEMOperator(**cases.operator_kwargs(c)) for c in case_list]
     ue = states(case_list, dev)
     us = [u.clone() for u in ue]
@@ -128,3 +131,39 @@ def test_ensemble64_equals_per_case_steps(cuda):
     ue, us, ok = run_pair(cl, cuda, [dts, dts])
     assert all(ok)
     assert_same(ue, us, cl)
+
+
+@pytest.mark.parametrize("mode", [1, 2, 4])
+def test_persistent_ensemble_equals_per_case_steps(cuda, mode):
+    """The persistent launch form (mode workgroups per CU over a device tile
+    counter) gives bitwise the per-case steps."""
+    cl = small_cases()
+    d0 = step_sizes(cl)
+    ue, us, ok = run_pair(cl, cuda, [d0, d0], True, mode)
+    assert all(ok)
+    assert_same(ue, us, cl)
+
+
+def test_ensemble_autotune(cuda):
+    import torch
+    cl = cases.ensemble_cases(64, 20261202)
+    dts = step_sizes(cl)
+    ens = Ensemble(cl)
+    us = states(cl, cuda)
+    before = [u.clone() for u in us]
+    mode, times = ens.autotune(us, dts)
+    torch.cuda.synchronize()
+    print("ensemble launch forms (us per stage):", ["%.1f" % t for t in times], "chosen", mode, flush=True)
+    assert 0 <= mode <= 4 and times[mode] == min(times)
+    assert all(torch.equal(a, b) for a, b in zip(us, before))   # the states are not modified
+    # a step in the chosen form equals per-case steps
+    ops = [DGSEMOperator(**cases.operator_kwargs(c)) for c in cl]
+    ref = [u.clone() for u in us]
+    s = torch.cuda.Stream(device=cuda)
+    with torch.cuda.stream(s):
+        ens.Step(us, dts)
+        for op, u, dt in zip(ops, ref, dts):
+            op.Step(u, dt)
+    torch.cuda.synchronize()
+    assert_same(us, ref, cl)
+    ens.close()
