@@ -5,6 +5,7 @@
 //   Prandtl::DGSEMOperator (DGSEMOperator.hpp:15-141)   -> prandtl_amd::DGSEMOperator
 //     void Mult(const Vector &u, Vector &dudt) const     -> Mult(const double *u, double *dudt) const
 //     real_t GetMaxCharSpeed()                           -> GetMaxCharSpeed()
+//   TimeDependentOperator::SetTime / GetTime           -> SetTime(double) / GetTime()
 //   mfem::RK3SSPSolver::Step(x, t, dt)                   -> RK3SSPSolver::Step(double *x, double &t, double &dt)
 //   Simulation::Run time loop (Simulation.cpp:1007-1022)  -> DGSEMOperator::Run (on the device)
 //   (no counterpart)                                     -> DGSEMOperator::Autotune / SetShape, Ensemble
@@ -72,8 +73,31 @@ class DGSEMOperator {
     // DGSEMOperator::Mult: dudt is overwritten with L(u)
     void Mult(const double *u, double *dudt) const { check(prandtl_rhs(ctx_, u, dudt, stream_), ctx_); }
 
-    // one SSP-RK3 step in place (the three stages fused into the RHS kernels)
+    // one SSP-RK3 step in place (the three stages fused into the RHS kernels),
+    // its stages at t, t + dt and t + dt/2 of the operator's time, which it
+    // then advances by dt
     void Step(double *u, double dt) const { check(prandtl_rk3_step(ctx_, u, dt, stream_), ctx_); }
+
+    // TimeDependentOperator::SetTime / GetTime (DGSEMOperator.hpp:111-114):
+    // the time at which Mult evaluates time-dependent boundary states
+    void SetTime(double t) const { check(prandtl_set_time(ctx_, t, stream_), ctx_); }
+    double GetTime() const
+    {
+        double t = 0.0;
+        check(prandtl_get_time(ctx_, &t), ctx_);
+        return t;
+    }
+
+    // boundary attribute attr's outer state: below where
+    // normal . x - offset - speed t < 0, above elsewhere (2-D Euler quadrilateral meshes)
+    void SetBoundarySplit(int attr, const double normal[2], double offset, double speed, const double *below,
+                          const double *above)
+    {
+        check(prandtl_set_boundary_split(ctx_, attr, normal, offset, speed, below, above), ctx_);
+    }
+
+    // algorithm of the RHS kernel (PRANDTL_ALGO_TILE or PRANDTL_ALGO_LINE)
+    void SetAlgorithm(int algo) { check(prandtl_set_algorithm(ctx_, PRANDTL_KERNEL_RHS, algo), ctx_); }
 
     // Simulation::Run's time loop on the device (prandtl_run): u from t to
     // t_final with variable dt from cfl (> 0) or the fixed dt; returns steps
@@ -160,6 +184,20 @@ class Ensemble {
         const int rc = prandtl_ensemble_rk3_step(ens_, u, dt, stream_);
         if (rc != PRANDTL_OK) throw Error(rc, prandtl_ensemble_last_error(ens_));
     }
+    // launch form of the stages: 0 one workgroup per tile, g > 0 persistent
+    // with g workgroups per CU; Autotune times them on u and keeps the fastest
+    void SetMode(int mode)
+    {
+        const int rc = prandtl_ensemble_set_mode(ens_, mode);
+        if (rc != PRANDTL_OK) throw Error(rc, prandtl_ensemble_last_error(ens_));
+    }
+    int Autotune(double *const *u, const double *dt)
+    {
+        int mode = 0;
+        const int rc = prandtl_ensemble_autotune(ens_, u, dt, &mode, nullptr, stream_);
+        if (rc != PRANDTL_OK) throw Error(rc, prandtl_ensemble_last_error(ens_));
+        return mode;
+    }
     prandtl_ctx *Case(int i)
     {
         prandtl_ctx *c = nullptr;
@@ -179,6 +217,7 @@ class RK3SSPSolver {
     void Init(const DGSEMOperator &f) { f_ = &f; }
     void Step(double *x, double &t, double &dt)
     {
+        f_->SetTime(t);
         f_->Step(x, dt);
         t += dt;
     }

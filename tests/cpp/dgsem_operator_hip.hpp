@@ -16,5 +16,11 @@ public:
     void Mult(const mfem::Vector &u, mfem::Vector &dudt) const override {
         op.Mult(u.Read(true), dudt.Write(true));
     }
+    // RK3SSPSolver sets each stage's time before its Mult
+    void SetTime(const mfem::real_t t_) override {
+        mfem::TimeDependentOperator::SetTime(t_);
+        op.SetTime(t_);
+    }
     double GetMaxCharSpeed() { return op.GetMaxCharSpeed(); }
+    double LibraryTime() const { return op.GetTime(); }
 };

@@ -1,5 +1,6 @@
 """Double Mach reflection on the reference's own mesh (TestCases/Euler/2D/
-DoubleMachReflection: DoubleMachReflection.msh, 36864 quadrilaterals, N = 3,
+DoubleMachReflection: DoubleMachReflection.msh, 36864 quadrilaterals, and
+refined once as config.json's par_ref_levels 1 asks, 147456; N = 3,
 blending), with its time-dependent top boundary (the post-shock state where
 x < 1/6 + (y + 20 t)/sqrt(3), DoubleMachReflection.hpp:33-57, config.json
 "time_dependent": true) evaluated on the device at each RK stage's time
@@ -32,10 +33,10 @@ def to_dev(u, dev):
     return torch.tensor(np.ascontiguousarray(u).reshape(-1), dtype=torch.float64, device=dev)
 
 
-@pytest.fixture(scope="module")
-def dmr():
+@pytest.fixture(scope="module", params=[0, 1], ids=["as_read", "refined_once"])
+def dmr(request):
     c = cases.DMR
-    mesh = QuadMesh.read(MESH)
+    mesh = QuadMesh.read(MESH, refine=request.param)
     op = DGSEMOperator.quad(mesh, c["order"], bc=c["bc"], blending=c["blending"], alpha_max=c["alpha_max"])
     nattr = int(mesh.bdr_attr.max())
     kinds, states = ["slip"] * nattr, [[1.0, 0.0, 0.0, 2.5]] * nattr
