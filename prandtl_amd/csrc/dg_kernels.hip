@@ -34,11 +34,9 @@ namespace prandtl {
 //
 // Workgroup b takes tile b of a host-built list sorted by decreasing cost; the
 // workgroup dispatcher hands workgroups to CUs in index order as slots free,
-// so the list is the work queue and needs no device counter.  A persistent
-// form (a loop over tiles inside the kernel, from an atomic counter or a
-// grid stride) keeps the tile body inside a loop, and register allocation
-// then spills about 1.3 KB per lane at 168 VGPRs; this form stays at the
-// largest variant's own 107 VGPRs with no scratch.
+// so the list is the work queue and needs no device counter.  k_ens_q below
+// is the persistent form over the same list; prandtl_ensemble_autotune
+// times both.
 // ---------------------------------------------------------------------------
 struct EnsTile {
     int c;   // case
