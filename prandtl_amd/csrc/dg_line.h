@@ -216,15 +216,45 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
 #pragma unroll
         for (int v = 0; v < NV; v++) { pl[v] = A.y[v * NK + gl]; ph[v] = A.y[v * NK + gh]; }
     }
+    // viscous face traces Fv.n at the line's two face points, this element's
+    // and the neighbour's, also in flight during the node phase
+    VT fto[2][NV - 1], ftn[2][NV - 1];
+#pragma unroll
+    for (int side = 0; side < 2; side++)
+#pragma unroll
+        for (int v = 0; v < NV - 1; v++) fto[side][v] = ftn[side][v] = VT(0);
+    if (A.parabolic && lact) {
+        const VT *fv = static_cast<const VT *>(A.fvf);
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const VT *fo = fv + (((size_t)e * 2 * DIM + 2 * r + side) * (NV - 1)) * NFP;
+            const VT *fn = fv + (((size_t)(side ? enh : enl) * 2 * DIM + 2 * r + 1 - side) * (NV - 1)) * NFP;
+#pragma unroll
+            for (int v = 0; v < NV - 1; v++) { fto[side][v] = fo[v * NFP + l]; ftn[side][v] = fn[v * NFP + l]; }
+        }
+    }
 
     // ---- node phase: state and per-node flux quantities into LDS ----
     double lam = 0.0;
     const int nact = (NE - ebase) < EPB ? (NE - ebase) * K : EPB * K;
-    for (int i = t; i < nact; i += BLOCK) {
+    // the workgroup's nodes in NPASS passes of BLOCK threads; every pass's
+    // loads are issued before the first is used
+    constexpr int NPASS = (EPB * K + BLOCK - 1) / BLOCK;
+    double ysp[NPASS][NV];
+#pragma unroll
+    for (int p = 0; p < NPASS; p++) {
+        const int i = t + p * BLOCK;
+#pragma unroll
+        for (int v = 0; v < NV; v++) ysp[p][v] = (i < nact) ? A.y[v * NK + (size_t)ebase * K + i] : 1.0;
+    }
+#pragma unroll
+    for (int p = 0; p < NPASS; p++) {
+        const int i = t + p * BLOCK;
+        if (i >= nact) continue;
         const int e2 = i / K, n = i - e2 * K;
         double ys[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) ys[v] = A.y[v * NK + (size_t)ebase * K + i];
+        for (int v = 0; v < NV; v++) ys[v] = ysp[p][v];
         const int np = pad(n);
 #pragma unroll
         for (int v = 0; v < NV; v++) sU[e2][v][np] = ys[v];
@@ -356,12 +386,9 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
             bool own_q = true;
             if (A.parabolic) {
                 // viscous face traces Fv.n of both sides of this face point
-                const VT *fv = static_cast<const VT *>(A.fvf);
-                const VT *fo = fv + (((size_t)e * 2 * DIM + 2 * r + side) * (NV - 1)) * NFP;
-                const VT *fn = fv + (((size_t)(side ? enh : enl) * 2 * DIM + 2 * r + 1 - side) * (NV - 1)) * NFP;
 #pragma unroll
                 for (int v = 1; v < NV; v++)
-                    fvn[v] = (((double)fo[(v - 1) * NFP + l] + (double)fn[(v - 1) * NFP + l]) * 0.5) * nr;
+                    fvn[v] = (((double)fto[side][v - 1] + (double)ftn[side][v - 1]) * 0.5) * nr;
                 if (A.quirks) {
                     // ComputeViscousFlux left velocities in the momentum slots
                     // of state1/state2 (DGSEMIntegrator.cpp:471-481)
@@ -486,6 +513,20 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         }
     }
 
+    // ---- the assembly's inputs (viscous volume term, u0), issued before
+    // the direction sums so their latency overlaps them ----
+    double vsp[NPASS][NV - 1], u0p[NPASS][NV];
+#pragma unroll
+    for (int p = 0; p < NPASS; p++) {
+        const int i = t + p * BLOCK;
+        const size_t gi = (size_t)ebase * K + i;
+#pragma unroll
+        for (int v = 0; v < NV - 1; v++)
+            vsp[p][v] = (A.parabolic && i < nact) ? (double)static_cast<const VT *>(A.visc)[v * NK + gi] : 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; v++) u0p[p][v] = (A.mode == 2 && i < nact) ? A.u0[v * NK + gi] : 0.0;
+    }
+
     // ---- node sums over the directions, in direction order ----
     if (!BLEND) __syncthreads();   // sP overwrites sQ
 #pragma unroll
@@ -502,19 +543,21 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
     }
 
     // ---- viscous volume term, stage update, store ----
-    for (int i = t; i < nact; i += BLOCK) {
+#pragma unroll
+    for (int p = 0; p < NPASS; p++) {
+        const int i = t + p * BLOCK;
+        if (i >= nact) continue;
         const int e2 = i / K, n = i - e2 * K;
         const size_t gi = (size_t)ebase * K + i;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             double tot = sP[e2][v][pad(n)];
-            if (A.parabolic && v > 0)
-                tot += (double)static_cast<const VT *>(A.visc)[(v - 1) * NK + gi] * A.geo.invJ;
+            if (A.parabolic && v > 0) tot += vsp[p][v - 1] * A.geo.invJ;
             const double yv = sU[e2][v][pad(n)];
             double o;
             if (A.mode == 0) o = tot;
             else if (A.mode == 1) o = yv + dt * tot;
-            else o = A.c0 * A.u0[v * NK + gi] + A.c1 * (yv + dt * tot);
+            else o = A.c0 * u0p[p][v] + A.c1 * (yv + dt * tot);
             A.out[v * NK + gi] = o;
         }
     }
