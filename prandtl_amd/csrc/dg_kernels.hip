@@ -993,8 +993,10 @@ static int rk3_stages(prandtl_ctx *c, double *u, double dt, hipStream_t s)
     c->stage_tc = 0.5;
     if (!rc) rc = stage(c, c->d_yb, u, u, 2, dt, 1.0 / 3, 2.0 / 3, s);
     c->stage_tc = 0.0;
-    if (!rc && !c->tp) {
-        // t += dt on the device (the device loop advances its own time)
+    if (!rc && !c->tp && c->d_bfn) {
+        // t += dt on the device, for the time-dependent boundary states that
+        // read it (the device loop advances its own time; without such
+        // states only the host copy is kept, which saves a launch per step)
         hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(1), 0, s, c->d_time, c->dtp, dt, c->skip);
         rc = hip_check(c, hipGetLastError(), "time advance");
     }
@@ -1084,7 +1086,11 @@ int prandtl_set_boundary_split(prandtl_ctx *c, int attr, const double *normal, d
     }
     hipError_t e = hipSuccess;
     if (!c->d_bfn) {
-        e = hipMalloc(&c->d_bfn, sizeof(double) * 16 * c->nattr);
+        // the device time has not been advanced by steps until now: bring it
+        // to the host copy once every stream's work on it is done
+        e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(c->d_time, &c->t, sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&c->d_bfn, sizeof(double) * 16 * c->nattr);
         if (e == hipSuccess) e = hipMemset(c->d_bfn, 0, sizeof(double) * 16 * c->nattr);
         if (e != hipSuccess) return hip_check(c, e, "boundary state table");
     }

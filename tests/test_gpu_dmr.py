@@ -112,3 +112,41 @@ def test_dmr_steps_graph_and_device_loop(cuda, dmr):
     assert n == 4 and r <= TOL
     assert abs(t - orc.time) <= 1e-15 and op.time == t
     assert op.check_state(du)
+
+
+def test_split_set_after_steps_uses_the_advanced_time(cuda, dmr):
+    """Steps taken before any time-dependent state exists advance only the
+    host copy of the time; setting the split state then brings the device
+    time to it, so the next RHS is evaluated at the advanced time.  25 steps
+    move the split line by 5.8e-3 along the top boundary, more than the node
+    spacing there, so the RHS at the old time differs (checked)."""
+    import torch
+    mesh, _, orc, u0 = dmr
+    c = cases.DMR
+    op = DGSEMOperator.quad(mesh, c["order"], bc=c["bc"], blending=c["blending"], alpha_max=c["alpha_max"])
+    dt = c["dt_start"]
+    s = torch.cuda.Stream(device=cuda)
+    with torch.cuda.stream(s):
+        op.set_time(0.003)
+        du = to_dev(u0, cuda)
+        for _ in range(25):
+            op.Step(du, dt)
+    torch.cuda.synchronize()
+    t = op.time
+    assert abs(t - (0.003 + 25 * dt)) <= 1e-15
+    for name, sp in c["split"].items():
+        op.set_boundary_split(name, sp["normal"], sp["offset"], sp["speed"], sp["below"], sp["above"])
+    out = torch.empty_like(du)
+    with torch.cuda.stream(s):
+        op.Mult(du, out)
+    torch.cuda.synchronize()
+    state = du.cpu().numpy()
+    orc.set_time(t)
+    ref = orc.Mult(state)
+    r = rel_max(out.cpu().numpy(), ref)
+    orc.set_time(0.003)
+    moved = rel_max(orc.Mult(state), ref)
+    print("DMR RHS after split set at t=%.17g: %.3e (RHS at the start time differs by %.3e)" % (t, r, moved),
+          flush=True)
+    assert r <= TOL
+    assert moved > 1e-3
