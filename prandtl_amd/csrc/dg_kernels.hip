@@ -482,6 +482,45 @@ namespace {
 //
 // host_ctx (boxes only): fill this caller-owned context's tables and stop
 // before any device memory is allocated (prandtl_rhs_noalloc).
+// FNV-1a, 64 bit, continuing from h (checkpoint checksums and mesh fingerprints)
+uint64_t fnv1a64(const void *data, size_t n, uint64_t h = 1469598103934665603ull)
+{
+    const unsigned char *p = static_cast<const unsigned char *>(data);
+    for (size_t i = 0; i < n; i++) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+// Fingerprint of a case's geometry, kept in checkpoints so that a state is
+// loaded only into the mesh it was written for: a box's element counts and
+// extents, or an unstructured mesh's vertex coordinates, connectivity and
+// per-element corners (order and variable count are in the header itself).
+uint64_t geom_fingerprint(const prandtl_mesh_desc *m, const prandtl_quad_mesh *qm, const prandtl_hex_mesh *hm)
+{
+    if (qm || hm) {
+        const int nvert = qm ? qm->nvert : hm->nvert, ne = qm ? qm->nelem : hm->nelem;
+        const int sd = qm ? 2 : 3, nc = qm ? 4 : 8;
+        const double *vert = qm ? qm->vert : hm->vert, *ec = qm ? qm->elem_coords : hm->elem_coords;
+        const int *elem = qm ? qm->elem : hm->elem;
+        uint64_t h = fnv1a64(&sd, sizeof(sd));
+        h = fnv1a64(&nvert, sizeof(nvert), h);
+        h = fnv1a64(vert, sizeof(double) * sd * (size_t)nvert, h);
+        h = fnv1a64(&ne, sizeof(ne), h);
+        h = fnv1a64(elem, sizeof(int) * nc * (size_t)ne, h);
+        if (ec) h = fnv1a64(ec, sizeof(double) * nc * sd * (size_t)ne, h);
+        return h;
+    }
+    uint64_t h = fnv1a64(&m->dim, sizeof(m->dim));
+    for (int r = 0; r < m->dim && r < 3; r++) {
+        h = fnv1a64(&m->nelem[r], sizeof(int), h);
+        h = fnv1a64(&m->lo[r], sizeof(double), h);
+        h = fnv1a64(&m->hi[r], sizeof(double), h);
+    }
+    return h;
+}
+
 int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, const prandtl_params *prm,
                 prandtl_ctx **out, const prandtl_hex_mesh *hm = nullptr, prandtl_ctx *host_ctx = nullptr)
 {
@@ -592,6 +631,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
     if (!c) return fail(nullptr, PRANDTL_ERR_NOMEM, "host allocation");
     c->dim = dim; c->order = N; c->Np = N + 1; c->nv = dim + 2;
     c->mesh = *mesh;
+    c->geom_fp = geom_fingerprint(mesh, qm, hm);
     c->K = dim == 1 ? c->Np : dim == 2 ? c->Np * c->Np : c->Np * c->Np * c->Np;
     c->Nfp = dim == 1 ? 1 : dim == 2 ? c->Np : c->Np * c->Np;
     c->NE = (int)NE;
@@ -1415,20 +1455,10 @@ struct ChkHeader {
     int32_t version;
     int32_t dim, order, nelem[3], nv;
     int64_t count;
+    uint64_t geom;   // geom_fingerprint of the mesh the state was written on
 };
 constexpr char CHK_MAGIC[8] = {'P', 'R', 'D', 'T', 'L', 'C', 'H', 'K'};
-constexpr int32_t CHK_VERSION = 1;
-
-uint64_t fnv1a64(const void *data, size_t n)
-{
-    const unsigned char *p = static_cast<const unsigned char *>(data);
-    uint64_t h = 1469598103934665603ull;
-    for (size_t i = 0; i < n; i++) {
-        h ^= p[i];
-        h *= 1099511628211ull;
-    }
-    return h;
-}
+constexpr int32_t CHK_VERSION = 2;
 
 std::string chk_dir(const char *dir, int cycle) { return std::string(dir) + "/Cycle" + std::to_string(cycle); }
 std::string chk_state(const char *dir, int cycle)
@@ -1470,15 +1500,9 @@ int io_fail(const std::string &msg)
     return PRANDTL_ERR_IO;
 }
 
-} // namespace
-
-extern "C" {
-
-int prandtl_checkpoint_write(const char *dir, int cycle, double t, const prandtl_mesh_desc *mesh, int nv,
-                             const double *u)
+int chk_write(const char *dir, int cycle, double t, const prandtl_mesh_desc *mesh, int nv, const double *u,
+              uint64_t geom)
 {
-    if (!dir || !mesh || !u || cycle < 0 || nv < 1 || mesh->dim < 1 || mesh->dim > 3)
-        return fail(nullptr, PRANDTL_ERR_ARG, "bad arguments");
     const int64_t count = mesh_count(mesh, nv);
     std::error_code ec;
     std::filesystem::create_directories(chk_dir(dir, cycle), ec);
@@ -1491,6 +1515,7 @@ int prandtl_checkpoint_write(const char *dir, int cycle, double t, const prandtl
     for (int r = 0; r < 3; r++) h.nelem[r] = r < mesh->dim ? mesh->nelem[r] : 1;
     h.nv = nv;
     h.count = count;
+    h.geom = geom;
     const uint64_t sum = fnv1a64(u, sizeof(double) * count);
     const std::string path = chk_state(dir, cycle), tmp = path + ".tmp";
     {
@@ -1512,11 +1537,9 @@ int prandtl_checkpoint_write(const char *dir, int cycle, double t, const prandtl
     return PRANDTL_OK;
 }
 
-int prandtl_checkpoint_read(const char *dir, int cycle, const prandtl_mesh_desc *mesh, int nv, double *u,
-                            double *t_out)
+int chk_read(const char *dir, int cycle, const prandtl_mesh_desc *mesh, int nv, double *u, double *t_out,
+             uint64_t geom)
 {
-    if (!dir || !mesh || !u || !t_out || cycle < 0 || nv < 1 || mesh->dim < 1 || mesh->dim > 3)
-        return fail(nullptr, PRANDTL_ERR_ARG, "bad arguments");
     std::ifstream m(chk_meta(dir, cycle));
     if (!m) return io_fail("cannot open " + chk_meta(dir, cycle));
     const std::string txt((std::istreambuf_iterator<char>(m)), std::istreambuf_iterator<char>());
@@ -1534,6 +1557,7 @@ int prandtl_checkpoint_read(const char *dir, int cycle, const prandtl_mesh_desc 
     bool same = h.dim == mesh->dim && h.order == mesh->order && h.nv == nv && h.count == count;
     for (int r = 0; r < mesh->dim; r++) same = same && h.nelem[r] == mesh->nelem[r];
     if (!same) return io_fail("checkpoint mesh or variables differ from this case: " + path);
+    if (h.geom != geom) return io_fail("checkpoint was written on another mesh (geometry fingerprint differs): " + path);
     std::vector<double> buf((size_t)count);
     if (!f.read(reinterpret_cast<char *>(buf.data()), sizeof(double) * count)) return io_fail("truncated state: " + path);
     uint64_t sum = 0;
@@ -1545,6 +1569,26 @@ int prandtl_checkpoint_read(const char *dir, int cycle, const prandtl_mesh_desc 
     return PRANDTL_OK;
 }
 
+} // namespace
+
+extern "C" {
+
+int prandtl_checkpoint_write(const char *dir, int cycle, double t, const prandtl_mesh_desc *mesh, int nv,
+                             const double *u)
+{
+    if (!dir || !mesh || !u || cycle < 0 || nv < 1 || mesh->dim < 1 || mesh->dim > 3)
+        return fail(nullptr, PRANDTL_ERR_ARG, "bad arguments");
+    return chk_write(dir, cycle, t, mesh, nv, u, geom_fingerprint(mesh, nullptr, nullptr));
+}
+
+int prandtl_checkpoint_read(const char *dir, int cycle, const prandtl_mesh_desc *mesh, int nv, double *u,
+                            double *t_out)
+{
+    if (!dir || !mesh || !u || !t_out || cycle < 0 || nv < 1 || mesh->dim < 1 || mesh->dim > 3)
+        return fail(nullptr, PRANDTL_ERR_ARG, "bad arguments");
+    return chk_read(dir, cycle, mesh, nv, u, t_out, geom_fingerprint(mesh, nullptr, nullptr));
+}
+
 int prandtl_checkpoint_save(prandtl_ctx *c, const double *u, const char *dir, int cycle, double t, void *stream)
 {
     if (!c || !u || !dir) return fail(c, PRANDTL_ERR_ARG, "null argument");
@@ -1553,7 +1597,7 @@ int prandtl_checkpoint_save(prandtl_ctx *c, const double *u, const char *dir, in
     int rc = hip_check(c, hipMemcpyAsync(h.data(), u, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s), "copy state");
     if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
     if (rc) return rc;
-    rc = prandtl_checkpoint_write(dir, cycle, t, &c->mesh, c->nv, h.data());
+    rc = chk_write(dir, cycle, t, &c->mesh, c->nv, h.data(), c->geom_fp);
     if (rc) c->err = g_create_err;
     return rc;
 }
@@ -1563,7 +1607,7 @@ int prandtl_checkpoint_load(prandtl_ctx *c, double *u, const char *dir, int cycl
     if (!c || !u || !dir || !t_out) return fail(c, PRANDTL_ERR_ARG, "null argument");
     std::vector<double> h((size_t)c->nv * c->NE * c->K);
     double t = 0.0;
-    int rc = prandtl_checkpoint_read(dir, cycle, &c->mesh, c->nv, h.data(), &t);
+    int rc = chk_read(dir, cycle, &c->mesh, c->nv, h.data(), &t, c->geom_fp);
     if (rc) {
         c->err = g_create_err;
         return rc;

@@ -47,6 +47,7 @@ struct prandtl_ctx {
     // stage's offset factor (0, 1, 1/2 of dt in an RK3 step)
     double t = 0.0;
     double *d_time = nullptr;
+    uint64_t geom_fp = 0;   // geometry fingerprint kept in checkpoints
     const double *tp = nullptr;
     double stage_tc = 0.0;
     double *d_bfn = nullptr;       // split outer states per attribute (quadrilateral meshes)

@@ -109,7 +109,11 @@ def read_segment_mesh(path, refine=0):
     type = segment, nx, sx (the segment [0, sx]) and "MFEM mesh v1.0" of
     dimension 1 with uniform elements, periodic when it has no boundary
     points (its elements then close a cycle and L2_T1_1D_P1 nodes give their
-    coordinates).  Raises ValueError for anything else."""
+    coordinates).  A bounded file has two boundary points; the reference
+    names attribute 1 "left" and 2 "right" (Simulation.cpp:283-292), and the
+    box's side 0 is lo, so attribute 1 must sit at lo and 2 at hi (an inline
+    segment has them there by construction).  Raises ValueError for anything
+    else."""
     with open(path) as f:
         lines = [ln.split("#")[0].strip() for ln in f]
     lines = [ln for ln in lines if ln]
@@ -139,7 +143,10 @@ def read_segment_mesh(path, refine=0):
     i = section("elements")
     ne = int(lines[i])
     elems = [tuple(int(t) for t in lines[i + 1 + k].split()[2:4]) for k in range(ne)]
-    nb = int(lines[section("boundary")])
+    ib = section("boundary")
+    nb = int(lines[ib])
+    # boundary points: "attribute geometry vertex" (geometry 0, a point)
+    bpts = [tuple(int(t) for t in lines[ib + 1 + k].split()[:3]) for k in range(nb)]
     if "nodes" in lines:
         # header lines (FiniteElementSpace, FiniteElementCollection, VDim,
         # Ordering) up to the first coordinate line
@@ -156,11 +163,23 @@ def read_segment_mesh(path, refine=0):
             raise ValueError("%s: vertices are not 1-D" % path)
         v = np.array([float(lines[j + 2 + k]) for k in range(nv)])
         x = np.array([[v[a], v[b]] for a, b in elems])
+        if nb:
+            xb = {}
+            for attr, geom, vert in bpts:
+                if geom != 0 or not 0 <= vert < nv:
+                    raise ValueError("%s: boundary element %r is not a vertex point" % (path, (attr, geom, vert)))
+                xb[attr] = v[vert]
     lo, hi = float(x.min()), float(x.max())
     h = (hi - lo) / ne
     if not np.allclose(np.sort(x[:, 0]), lo + h * np.arange(ne), rtol=0, atol=1e-12 * (hi - lo)) or \
             not np.allclose(x[:, 1] - x[:, 0], h, rtol=1e-12, atol=0):
         raise ValueError("%s: elements are not a uniform left-to-right segment" % path)
+    if nb:
+        if "vertices" not in lines or nb != 2 or sorted(xb) != [1, 2]:
+            raise ValueError("%s: a bounded segment has boundary attributes 1 and 2, one point each" % path)
+        if not (xb[1] == lo and xb[2] == hi):
+            raise ValueError("%s: boundary attribute 1 ('left') is at x = %g and 2 ('right') at x = %g; the box's "
+                             "side 0 is x = %g" % (path, xb[1], xb[2], lo))
     return dict(dim=1, nelem=(ne << refine,), lo=(lo,), hi=(hi,), periodic=nb == 0)
 
 

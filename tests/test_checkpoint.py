@@ -39,7 +39,7 @@ def test_round_trip_bitwise(tmp_path):
     chk, meta = paths(tmp_path, 7)
     j = json.load(open(meta))
     assert j == {"time": t, "cycle": 7}
-    assert os.path.getsize(chk) == 48 + 8 * u.size + 8
+    assert os.path.getsize(chk) == 56 + 8 * u.size + 8
 
 
 def test_missing_files(tmp_path):
@@ -93,3 +93,14 @@ def test_meta_cycle_and_magic(tmp_path):
     data[0:8] = b"NOTACHK!"
     open(chk, "wb").write(bytes(data))
     assert "not a checkpoint" in expect_io(tmp_path)
+
+
+def test_same_counts_other_geometry_refused(tmp_path):
+    """A checkpoint keeps a fingerprint of the geometry it was written on: a
+    box with the same element counts, order and variables but other extents
+    does not load it."""
+    checkpoint_write(tmp_path, 7, 0.5, state(), **MESH)
+    other = dict(MESH, hi=(1.0, 2.0))
+    assert "another mesh" in expect_io(tmp_path, mesh=other)
+    v, _ = checkpoint_read(tmp_path, 7, **MESH)
+    assert np.array_equal(v, state())

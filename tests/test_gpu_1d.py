@@ -233,13 +233,19 @@ def test_shu_osher_device_loop_matches_oracle_run(cuda):
     u = to_dev(U0, cuda)
     t, n = op.Run(u, 0.0, 1.8, cfl=1.0)
     torch.cuda.synchronize()
-    ref, tt, m = U0.copy(), 0.0, 0
+    # Simulation::Run's loop (Simulation.cpp:893-907, 1007-1024), as the
+    # device loop runs it: dt_real = min(dt, T - t), dt from the running
+    # wave speed after each step, stop when t >= T - 1e-8 dt
+    ref, tt, m, done = U0.copy(), 0.0, 0, False
     orc.Mult(ref)
-    while tt < 1.8 - 1e-14:
-        dt = min(orc.hmin() / (orc.GetMaxCharSpeed() * 16), 1.8 - tt)
-        ref = orc.Step(ref, dt)
-        tt += dt
+    dt = orc.hmin() / (orc.GetMaxCharSpeed() * 16)
+    while not done:
+        dt_real = min(dt, 1.8 - tt)
+        ref = orc.Step(ref, dt_real)
+        tt += dt_real
+        dt = orc.hmin() / (orc.GetMaxCharSpeed() * 16)
         m += 1
+        done = tt >= 1.8 - 1e-8 * dt
     U = u.cpu().numpy()
     r = rel_max(U, ref)
     print("shu-osher: device %d steps, oracle %d steps, rel %g, alpha max %g"

@@ -118,3 +118,26 @@ def test_primitives(cuda, cfg):
     for d in range(dim):
         assert np.array_equal(P[1 + d], vel[d])
     assert np.abs(P[dim + 1] - p).max() <= 1e-15 * np.abs(p).max()
+
+
+def test_checkpoint_refused_on_another_mesh_of_the_same_size(cuda, tmp_path):
+    """Two periodic quadrilateral meshes with the same element count and
+    order (a uniform grid and a perturbed one): a checkpoint written on one
+    does not load into the other (the header's geometry fingerprint), and
+    still loads into an operator on its own mesh."""
+    import torch
+    import quad_meshes as Q
+    from prandtl_amd import PrandtlError, _lib
+    from prandtl_amd.operator import QuadMesh
+    m1 = QuadMesh(**Q.grid(6, 5, lo=(-5.0, -5.0), hi=(5.0, 5.0)))
+    m2 = QuadMesh(**Q.grid(6, 5, lo=(-5.0, -5.0), hi=(5.0, 5.0), perturb=0.2, seed=4))
+    a, b, c = DGSEMOperator.quad(m1, 3), DGSEMOperator.quad(m2, 3), DGSEMOperator.quad(m1, 3)
+    u = torch.tensor(cases.isentropic_vortex(m1.node_coords(3)).reshape(-1), dtype=torch.float64, device=cuda)
+    a.save_checkpoint(u, tmp_path, 4, 0.25)
+    v = torch.empty_like(u)
+    with pytest.raises(PrandtlError) as e:
+        b.load_checkpoint(v, tmp_path, 4)
+    assert e.value.code == _lib.PRANDTL_ERR_IO and "another mesh" in str(e.value)
+    assert c.load_checkpoint(v, tmp_path, 4) == 0.25
+    torch.cuda.synchronize()
+    assert torch.equal(u, v)

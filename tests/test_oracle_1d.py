@@ -176,3 +176,29 @@ def test_reads_reference_segment_meshes():
     assert s["nelem"] == cases.shock_tube("sod")[0]["nelem"]
     with pytest.raises(ValueError):
         read_segment_mesh(os.path.join(d, "IsentropicVortex.mesh"))
+
+
+def _segment_file(tmp_path, bdr):
+    """An MFEM v1.0 segment mesh of 4 elements on [0, 2] with the given
+    boundary lines ("attribute 0 vertex")."""
+    text = ("MFEM mesh v1.0\n\ndimension\n1\n\nelements\n4\n1 1 0 1\n1 1 1 2\n1 1 2 3\n1 1 3 4\n\n"
+            "boundary\n%d\n%s\n\nvertices\n5\n1\n0\n0.5\n1\n1.5\n2\n" % (len(bdr), "\n".join(bdr)))
+    p = tmp_path / "seg.mesh"
+    p.write_text(text)
+    return str(p)
+
+
+def test_segment_mesh_boundary_attributes(tmp_path):
+    """Attribute 1 ('left') must be the lo end and 2 ('right') the hi end
+    (Simulation.cpp:283-292): the box's sides are by position, so a file
+    with them the other way round is refused, not run with its two end
+    conditions swapped."""
+    from prandtl_amd.operator import read_segment_mesh
+    ok = read_segment_mesh(_segment_file(tmp_path, ["1 0 0", "2 0 4"]), refine=1)
+    assert ok == dict(dim=1, nelem=(8,), lo=(0.0,), hi=(2.0,), periodic=False)
+    # the same points listed in the other order are the same mesh
+    assert read_segment_mesh(_segment_file(tmp_path, ["2 0 4", "1 0 0"])) == dict(
+        dim=1, nelem=(4,), lo=(0.0,), hi=(2.0,), periodic=False)
+    for bad in (["1 0 4", "2 0 0"], ["1 0 0", "1 0 4"], ["1 0 0"], ["1 0 0", "2 0 7"], ["1 0 0", "2 0 2"]):
+        with pytest.raises(ValueError):
+            read_segment_mesh(_segment_file(tmp_path, bad))
