@@ -104,6 +104,32 @@ def test_line_kernel_parity(cuda, name):
     assert rel_max(du.cpu().numpy(), ref) <= 1e-12
 
 
+@pytest.mark.parametrize("N,nelem", [(3, (5, 3, 3)), (4, (5, 2, 2)), (2, (5, 2, 2))])
+@pytest.mark.parametrize("ns", [True, False], ids=["ns", "euler"])
+def test_line_kernel_ragged_last_workgroup(cuda, N, nelem, ns):
+    """Element counts the line kernel's elements per workgroup (N = 2, 3, 4:
+    7, 4, 3) do not divide, so the last workgroup is partly empty and its
+    node passes, line ends and assembly loads are guarded: RHS and one RK3
+    stage sequence against the oracle."""
+    import torch
+    lo, hi = (0.0,) * 3, (TWO_PI, 2.0, 3.0)
+    cfg = dict(dim=3, order=N, nelem=nelem, lo=lo, hi=hi, quirks=False, **(NS if ns else {}))
+    op = DGSEMOperator(**cfg)
+    assert op.algorithm == "line"
+    orc = O.Oracle(threads=8, **cfg)
+    X = cases.node_coords(3, N, nelem, lo, hi)
+    u = cases.random_smooth_state(X, 3, 23, L=[hi[r] - lo[r] for r in range(3)])
+    a = rhs(op, u, cuda)
+    ref = orc.Mult(u)
+    r = rel_max(a, ref)
+    du = to_dev(u, cuda)
+    op.Step(du, 1e-3)
+    torch.cuda.synchronize()
+    r_step = rel_max(du.cpu().numpy(), orc.Step(u.reshape(-1).copy(), 1e-3))
+    print("line ragged N=%d %s %s: RHS %.3e, step %.3e" % (N, nelem, "ns" if ns else "euler", r, r_step), flush=True)
+    assert r <= 1e-12 and r_step <= 1e-12
+
+
 def test_line_kernel_not_for_other_meshes(cuda):
     # open boxes and 2-D boxes keep the tile kernel and refuse the line one
     op2 = DGSEMOperator(dim=2, order=3, nelem=(4, 4), lo=(0.0, 0.0), hi=(1.0, 1.0))
