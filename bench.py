@@ -291,6 +291,7 @@ def run_ensemble(args):
     # launch form: one workgroup per tile, or persistent with g workgroups
     # per CU; the fastest first stage on these states is kept
     form, form_us = (0, None) if args.no_autotune else ens.autotune(us, dts)
+    peaks = None if args.no_peaks else measured_peaks(dev)
 
     def ens_steps(k):
         for _ in range(k):
@@ -331,8 +332,8 @@ def run_ensemble(args):
                     + counts["assembly"] + 4 * nv * nodes)
     bytes_launch = 8 * nv * (1 + 1 + 2.0 / 3.0) * nodes
     kname = "k_ens" if form == 0 else "k_ens_q"
-    roof = roofline(flops_launch, bytes_launch, t_launch, kname + " (8 variants N=2..5 x blending)",
-                    measured_traffic(ENSEMBLE, kname))
+    roof = with_measured(roofline(flops_launch, bytes_launch, t_launch, kname + " (8 variants N=2..5 x blending)",
+                                  measured_traffic(ENSEMBLE, kname)), peaks)
     value = nodes * 3 * args.steps / elapsed
     line = {
         "metric": METRIC,
@@ -356,7 +357,7 @@ def run_ensemble(args):
                    "sequential_per_case": {"value": nodes * 3 * args.steps / elapsed_seq,
                                            "ms_per_step": elapsed_seq / args.steps * 1e3,
                                            "launches_per_step": 3 * len(case_list)},
-                   "speedup_vs_sequential": elapsed_seq / elapsed},
+                   "speedup_vs_sequential": elapsed_seq / elapsed, "measured_peaks": peaks},
         "roofline": roof,
     }
     if not args.no_cpu:
@@ -418,6 +419,46 @@ def mixed_comparison(name, cfg, u_host, dt, steps, warmup, autotune, dev):
             "dEk_dt_rel_diff": abs(ek_m - ek_d) / abs(ek_d)}
 
 
+def measured_peaks(dev):
+    """Achievable peaks of this GPU, measured before the timed region
+    (prandtl_peak_fma, prandtl_peak_copy): FP64 and unpacked FP32 FMA chains,
+    and device copies of 1 GiB (HBM) and of 27 MB (54 MB of traffic, the
+    RHS kernel's, which the 256 MB Infinity Cache can hold)."""
+    import ctypes
+    import torch
+    from prandtl_amd._lib import lib
+    L = lib()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    sink = torch.zeros(1, dtype=torch.float64, device=dev)
+    out = {}
+    for bits in (64, 32):
+        tf = ctypes.c_double()
+        rc = L.prandtl_peak_fma(bits, ctypes.c_void_p(sink.data_ptr()), ctypes.byref(tf), stream)
+        assert rc == 0, "prandtl_peak_fma failed (%d)" % rc
+        out["fp%d_tflops" % bits] = tf.value
+    for label, nbytes in (("1GiB", 1 << 30), ("27MB", 27000000 // 16 * 16)):
+        a = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        us = ctypes.c_double()
+        rc = L.prandtl_peak_copy(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), nbytes, 20,
+                                 ctypes.byref(us), stream)
+        assert rc == 0, "prandtl_peak_copy failed (%d)" % rc
+        out["copy_GBps_%s" % label] = 2 * nbytes / us.value / 1e3
+        del a, b
+    torch.cuda.synchronize()
+    return out
+
+
+def with_measured(roof, peaks):
+    """The roofline entry's achieved rate as a fraction of the measured peak
+    of its unit as well (the contract's peak is the datasheet one)."""
+    if peaks:
+        m = peaks["copy_GBps_1GiB"] if roof["unit"] == "GB/s" else peaks["fp64_tflops"]
+        roof["measured_peak"] = m
+        roof["frac_of_measured"] = roof["achieved"] / m
+    return roof
+
+
 def roofline(flops_launch, bytes_launch, t_launch, kernel, traffic=None):
     """Roofline entry of one kernel from its algorithmic counts per launch."""
     ai = flops_launch / bytes_launch
@@ -449,6 +490,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes of both kernels")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip the FMA-chain and copy measurements of the machine peaks")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-form comparison of a plain FP64 Navier-Stokes run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -506,6 +549,7 @@ def main():
 
     # launch shapes of both kernels: timed on this state, fastest kept
     shape, shape_us = ({"rhs": 0, "visc": 0}, None) if args.no_autotune else op.autotune(u)
+    peaks = None if args.no_peaks else measured_peaks(dev)
 
     for _ in range(args.warmup):
         op.Step(u, dt)
@@ -553,7 +597,7 @@ def main():
     by_node = 8 * nv * (1 + 1 + 2.0 / 3.0)
     kname = "k_rhs<%d,%d>" % (cfg["dim"], cfg["order"] + 1)
     traffic = measured_traffic(name, kname) if (args.precision == "fp64" and not quirks) else None
-    roof = roofline(fl_rhs * nodes, by_node * nodes, t_rhs, kname, traffic)
+    roof = with_measured(roofline(fl_rhs * nodes, by_node * nodes, t_rhs, kname, traffic), peaks)
     t_visc = ms_visc / max(n_visc, 1) * 1e-3
     kernels = {kname: {"avg_launch_us": t_rhs * 1e6, "flops_per_node": fl_rhs,
                        "tflops": fl_rhs * nodes / t_rhs / 1e12,
@@ -593,7 +637,7 @@ def main():
                    "launch_shapes": shape, "launch_shape_us": shape_us,
                    "ms_per_step_event_profiled": elapsed_prof / psteps * 1e3,
                    "kernel_ms": {"k_rhs": ms_rhs / max(n_rhs, 1), "k_visc": ms_visc / max(n_visc, 1)},
-                   "kernels": kernels, "rhs_roofline": whole},
+                   "kernels": kernels, "rhs_roofline": whole, "measured_peaks": peaks},
         "roofline": roof,
     }
     if args.precision == "fp64" and cfg.get("parabolic") and not quirks and not args.no_mixed \

@@ -14,6 +14,8 @@
 #ifndef PRANDTL_HIP_H
 #define PRANDTL_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -287,7 +289,9 @@ int prandtl_set_shape(prandtl_ctx *ctx, int kernel, int shape);
  * PRANDTL_ALGO_TILE: k_rhs, a workgroup's threads take the element's
  * two-point fluxes as jobs through LDS (every mesh kind).  PRANDTL_ALGO_LINE:
  * k_rhs_ln, one thread per tensor line (periodic Cartesian boxes in 3-D,
- * N = 2..4), the default where it exists; with it, the mixed form at N = 3
+ * N = 2..4), the default at N = 2 and 3 and, with blending, at N = 4 (the
+ * faster of the two there; unblended N = 4 defaults to the tile kernel);
+ * with it, the mixed form at N = 3
  * runs its viscous kernel's derivative contractions on the FP32 matrix cores
  * (k_visc_mx).  The two differ by rounding only;
  * launch shapes (prandtl_set_shape, prandtl_autotune) belong to the current
@@ -364,6 +368,17 @@ int prandtl_debug_gradients(prandtl_ctx *ctx, const double *u_dev, double *grad_
  * kernel. */
 int prandtl_profile_begin(prandtl_ctx *ctx);
 int prandtl_profile_end(prandtl_ctx *ctx, double *ms, int *count);
+
+/* Achievable machine peaks of the current device (no counterpart in the
+ * reference), for the roofline beside the datasheet figures.
+ * prandtl_peak_fma: vector FMA rate in TFLOP/s of bits = 64 (FP64) or 32
+ * (unpacked FP32), from 8 independent FMA chains per thread at 8 waves per
+ * SIMD; sink is a device buffer of at least 8 bytes that is never written.
+ * prandtl_peak_copy: mean microseconds of reps device-to-device copies of
+ * bytes (a multiple of 16) from src to dst, device buffers that do not
+ * overlap (read and write traffic: 2 bytes per byte copied). */
+int prandtl_peak_fma(int bits, void *sink_dev, double *tflops, void *stream);
+int prandtl_peak_copy(const void *src_dev, void *dst_dev, size_t bytes, int reps, double *us, void *stream);
 
 /* Last error message of this context (or of create when ctx is NULL). */
 const char *prandtl_last_error(const prandtl_ctx *ctx);

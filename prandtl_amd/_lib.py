@@ -39,6 +39,7 @@ EXPORTED = (
     "prandtl_set_algorithm", "prandtl_get_algorithm", "prandtl_rhs_noalloc",
     "prandtl_set_time", "prandtl_get_time", "prandtl_set_boundary_split",
     "prandtl_ensemble_set_mode", "prandtl_ensemble_autotune",
+    "prandtl_peak_fma", "prandtl_peak_copy",
 )
 
 
@@ -111,6 +112,8 @@ def lib():
     L.prandtl_set_algorithm.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.prandtl_set_time.argtypes = [vp, ctypes.c_double, vp]
     L.prandtl_ensemble_set_mode.argtypes = [vp, ctypes.c_int]
+    L.prandtl_peak_fma.argtypes = [ctypes.c_int, vp, dp, vp]
+    L.prandtl_peak_copy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, dp, vp]
     L.prandtl_ensemble_autotune.argtypes = [vp, vp, dp, ip, dp, vp]
     L.prandtl_get_time.argtypes = [vp, dp]
     L.prandtl_set_boundary_split.argtypes = [vp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, dp, dp]

@@ -839,7 +839,7 @@ int create_impl(const prandtl_mesh_desc *mesh, const prandtl_quad_mesh *qm, cons
         return PRANDTL_ERR_NOMEM;
     }
     // the line kernel where it serves this case
-    c->rhs_algo = line_ok(c) ? PRANDTL_ALGO_LINE : PRANDTL_ALGO_TILE;
+    c->rhs_algo = line_default(c) ? PRANDTL_ALGO_LINE : PRANDTL_ALGO_TILE;
     *out = c;
     return PRANDTL_OK;
 }
@@ -1005,7 +1005,7 @@ int prandtl_rhs_noalloc(const prandtl_mesh_desc *mesh, const prandtl_params *par
     c.d_visc = c.prm.parabolic ? reinterpret_cast<double *>(b + off[4]) : nullptr;
     c.d_fvf = c.prm.parabolic ? reinterpret_cast<double *>(b + off[5]) : nullptr;
     c.geo.bcs = c.geo.bcmask ? c.d_bcs : nullptr;
-    c.rhs_algo = line_ok(&c) ? PRANDTL_ALGO_LINE : PRANDTL_ALGO_TILE;
+    c.rhs_algo = line_default(&c) ? PRANDTL_ALGO_LINE : PRANDTL_ALGO_TILE;
     BoxStates bs;
     std::memcpy(bs.v, mesh->bc_state, sizeof(bs.v));
     hipStream_t s = (hipStream_t)stream;

@@ -203,6 +203,15 @@ inline bool line_ok(const prandtl_ctx *c)
     return line_built(c->dim, c->Np) && !c->gen && !c->axi && c->geo.bcmask == 0;
 }
 
+// The default algorithm where the line kernel exists: it is the faster one
+// at N = 2 and 3, and with blending at N = 4; unblended N = 4 runs the tile
+// kernel faster (12^3 boxes on MI355X: Euler 35.6 against 42.0 us per RHS,
+// Navier-Stokes 54.9 against 62.3).
+inline bool line_default(const prandtl_ctx *c)
+{
+    return line_ok(c) && (c->Np <= 4 || c->prm.blending);
+}
+
 template <int DIM, int NP, bool BLEND>
 int launch_rhs_t(prandtl_ctx *c, const KArgs &a, hipStream_t s)
 {
