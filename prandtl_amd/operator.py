@@ -551,7 +551,8 @@ class DGSEMOperator:
     def set_algorithm(self, algo):
         """Kernel algorithm of the flux-differencing RHS: "tile" (k_rhs, every
         mesh kind) or "line" (k_rhs_ln, one thread per tensor line: periodic
-        3-D boxes of order 2..4, the default there)."""
+        3-D boxes of order 2..4, the default at orders 2 and 3 and for
+        blended order 4)."""
         a = {"tile": _lib.PRANDTL_ALGO_TILE, "line": _lib.PRANDTL_ALGO_LINE}[algo]
         check(lib().prandtl_set_algorithm(self._h, _lib.PRANDTL_KERNEL_RHS, a), self._h)
 

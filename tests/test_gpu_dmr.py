@@ -150,3 +150,33 @@ def test_split_set_after_steps_uses_the_advanced_time(cuda, dmr):
           flush=True)
     assert r <= TOL
     assert moved > 1e-3
+
+
+def test_split_state_refusals(cuda):
+    """prandtl_set_boundary_split refuses what it does not serve: a
+    Cartesian box, a Navier-Stokes mesh, an attribute out of range and a
+    state with non-positive pressure; none of them changes the operator."""
+    import quad_meshes as Q
+    from prandtl_amd import PrandtlError, _lib
+    sp = cases.DMR_TOP_SPLIT
+    box = DGSEMOperator(dim=2, order=3, nelem=(4, 4), lo=(0.0, 0.0), hi=(1.0, 1.0))
+    with pytest.raises(PrandtlError) as e:
+        box.set_boundary_split(1, sp["normal"], sp["offset"], sp["speed"], sp["below"], sp["above"])
+    assert e.value.code == _lib.PRANDTL_ERR_UNSUPPORTED
+    grid = Q.grid(4, 4, periodic=(False, False))
+    bc = {1: ("specified-state", cases.DMR_PRE), 2: "supersonic-outflow", 3: "slip", 4: "slip"}
+    m = QuadMesh(**grid)
+    ns = DGSEMOperator.quad(m, 3, bc={1: "no-slip-adiabatic", 2: "no-slip-adiabatic", 3: "no-slip-adiabatic",
+                                      4: "no-slip-adiabatic"}, parabolic=True, mu=0.01, Pr=0.72)
+    with pytest.raises(PrandtlError) as e:
+        ns.set_boundary_split(1, sp["normal"], sp["offset"], sp["speed"], sp["below"], sp["above"])
+    assert e.value.code == _lib.PRANDTL_ERR_UNSUPPORTED
+    op = DGSEMOperator.quad(m, 3, bc=bc)
+    for attr in (0, 5):
+        with pytest.raises(PrandtlError) as e:
+            op.set_boundary_split(attr, sp["normal"], sp["offset"], sp["speed"], sp["below"], sp["above"])
+        assert e.value.code == _lib.PRANDTL_ERR_ARG
+    cold = np.array([1.0, 0.0, 0.0, 0.0])   # zero pressure
+    with pytest.raises(PrandtlError) as e:
+        op.set_boundary_split(1, sp["normal"], sp["offset"], sp["speed"], sp["below"], cold)
+    assert e.value.code == _lib.PRANDTL_ERR_ARG
