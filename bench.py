@@ -57,6 +57,12 @@ for _name, _case in (("ramp2d_n1", "RAMP"), ("nagashima2d_n1", "NAGASHIMA")):
 # between slip walls, blending); spin-up steps at cfl 1 form the waves
 WORKLOADS["blast1d_n3"] = dict(dim=1, order=3, nelem=(1024,), lo=(0.0,), hi=(1.0,), gamma=1.4, blending=True,
                                alpha_max=0.5, bc=[("slip", "slip")], bc_state=[(None, None)], spinup=2000)
+# the reference's Double Mach reflection on its own mesh refined once
+# (par_ref_levels 1: 147456 quadrilaterals), N = 3, blending, with the top
+# boundary's moving split state; fixed dt 1e-5 (about cfl 0.5 on this mesh:
+# the configured 1e-4 is unstable from the discontinuous start), spin-up
+# steps of that dt before the timed region
+WORKLOADS["dmr2d_n3"] = dict(dim=2, order=3, gamma=1.4, spinup=300, case="DMR", dt=1e-5)
 MESH_DIR = os.path.join(ROOT, "tests", "golden", "meshes")
 # 64 independent 2-D Euler cases (isentropic vortices and blended
 # Kelvin-Helmholtz layers, meshes 8^2..64^2, N = 2..5) advanced together
@@ -64,13 +70,18 @@ ENSEMBLE = "ensemble64"
 
 
 def resolve(cfg):
-    """A workload's settings, those of an axisymmetric case from prandtl_amd.cases."""
+    """A workload's settings, those of a mesh case from prandtl_amd.cases
+    (the axisymmetric cases, DMR); the workload's own dt, if it has one,
+    takes precedence."""
     if "case" not in cfg:
         return cfg
     from prandtl_amd import cases
     c = getattr(cases, cfg["case"])
     out = dict(cfg, mesh=c["mesh"], order=c["order"], blending=c["blending"], alpha_max=c["alpha_max"],
-               bc=c["bc"], dt=c["dt"], state=c["state"], axis_rho_floor=c["axis_rho_floor"])
+               bc=c["bc"], dt=cfg.get("dt", c["dt"]))
+    for k in ("state", "axis_rho_floor", "split", "refine"):
+        if k in c:
+            out[k] = c[k]
     del out["case"]
     return out
 
@@ -104,9 +115,12 @@ def make_operator(cfg, quirks, precision):
         return DGSEMOperator.hex(hex_mesh(cfg), cfg["order"], quirks=quirks, precision=precision,
                                  **{k: v for k, v in box_kwargs(cfg).items() if k != "order"})
     if "mesh" in cfg:
-        return DGSEMOperator.quad(quad_mesh(cfg), cfg["order"], bc=cfg["bc"], gamma=cfg["gamma"],
-                                  blending=cfg.get("blending", False), alpha_max=cfg.get("alpha_max", 0.5),
-                                  quirks=quirks, precision=precision, **axi_kwargs(cfg))
+        op = DGSEMOperator.quad(quad_mesh(cfg), cfg["order"], bc=cfg["bc"], gamma=cfg["gamma"],
+                                blending=cfg.get("blending", False), alpha_max=cfg.get("alpha_max", 0.5),
+                                quirks=quirks, precision=precision, **axi_kwargs(cfg))
+        for name, sp in cfg.get("split", {}).items():
+            op.set_boundary_split(name, sp["normal"], sp["offset"], sp["speed"], sp["below"], sp["above"])
+        return op
     return DGSEMOperator(quirks=quirks, precision=precision, **op_kwargs(cfg))
 
 
@@ -130,9 +144,13 @@ def make_oracle(cfg, quirks, threads):
                 states[a - 1] = list(v[1])
         d = dict(vert=m.vert, elem=m.elem, elem_coords=m.elem_coords, bdr=m.bdr, bdr_attr=m.bdr_attr,
                  attr_bc=kinds, attr_state=states)
-        return O.Oracle.quad(d, cfg["order"], gamma=cfg["gamma"], blending=cfg.get("blending", False),
-                             alpha_max=cfg.get("alpha_max", 0.5), quirks=quirks, threads=threads,
-                             **axi_kwargs(cfg))
+        orc = O.Oracle.quad(d, cfg["order"], gamma=cfg["gamma"], blending=cfg.get("blending", False),
+                            alpha_max=cfg.get("alpha_max", 0.5), quirks=quirks, threads=threads,
+                            **axi_kwargs(cfg))
+        for name, sp in cfg.get("split", {}).items():
+            orc.set_boundary_split(m.names[name], sp["normal"], sp["offset"], sp["speed"], sp["below"],
+                                   sp["above"])
+        return orc
     return O.Oracle(threads=threads, quirks=quirks, **op_kwargs(cfg))
 
 
@@ -144,6 +162,9 @@ def initial_state(name, cfg):
         # stream everywhere (r U in axisymmetric mode)
         m = quad_mesh(cfg)
         K = (cfg["order"] + 1) ** 2
+        if "split" in cfg:
+            # DoubleMachReflectionIC: the post-shock state left of the shock line
+            return cases.dmr_state(m.node_coords(cfg["order"])).reshape(4, -1)
         U = np.asarray(cfg["state"] if "state" in cfg else cfg["bc"]["left"][1], float)
         if cfg.get("axisymmetric"):
             return cases.axisymmetric_state(U, m.node_coords(cfg["order"])).reshape(4, -1)

@@ -765,8 +765,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             const VT *fvfp = static_cast<const VT *>(A.fvf);
             double F[NV];
             if (bt != BC_PERIODIC) {
-                const double *bs = A.geo.bcs + (code >> 4) * 5;
-                double fs[5];
+                // the attribute's outer state, copied (a pointer that could
+                // aim at either this array or the table would keep the
+                // array in scratch memory)
+                double bs[5];
+#pragma unroll
+                for (int v = 0; v < 5; v++) bs[v] = A.geo.bcs[(code >> 4) * 5 + v];
                 if (A.geo.bfn && A.geo.bfn[(code >> 4) * 16] != 0.0) {
                     // the attribute's time-dependent outer state at this
                     // point: "below" where n . x - offset - speed t < 0
@@ -781,8 +785,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     sd = __dsub_rn(sd, fn[4]);
                     sd = __dsub_rn(sd, __dmul_rn(fn[5], tt));
 #pragma unroll
-                    for (int v = 0; v < 5; v++) fs[v] = (sd < 0.0) ? fn[6 + v] : fn[11 + v];
-                    bs = fs;
+                    for (int v = 0; v < 5; v++) bs[v] = (sd < 0.0) ? fn[6 + v] : fn[11 + v];
                 }
                 const double lb = boundary_flux_vn<DIM>(bt, nvec, so, bs, P, quirk3d, F);
                 lam = (lam < lb) ? lb : lam;
@@ -823,8 +826,15 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                 for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)gn];
             }
             const bool own = code & 1;
-            double *g1 = own ? so : pn;
-            double *g2 = own ? pn : so;
+            // the face's first and second states, element by element (not a
+            // pointer to one of the two arrays, which would put both in
+            // scratch memory)
+            double g1[NV], g2[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                g1[v] = own ? so[v] : pn[v];
+                g2[v] = own ? pn[v] : so[v];
+            }
             double fvn[NV];
             fvn[0] = 0.0;
             if (A.parabolic) {
@@ -873,7 +883,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             if (bt != BC_PERIODIC) {
                 double F[NV];
                 const double ns = side ? nr : -nr;
-                const double *bs = A.geo.bcs + (2 * r + side) * 5;
+                // the side's outer state as values: a select between the
+                // table and the interior state would otherwise become one
+                // load through a selected pointer, keeping so in scratch
+                double bs[5];
+#pragma unroll
+                for (int v = 0; v < 5; v++) bs[v] = A.geo.bcs[(2 * r + side) * 5 + v];
                 const double lb = boundary_flux<DIM>(bt, r, ns, nr, so, bs, P, quirk3d, F);
                 lam = (lam < lb) ? lb : lam;
                 double d[NV];
@@ -902,8 +917,12 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                 continue;
             }
         }
-        double *s1 = side ? so : pn;
-        double *s2 = side ? pn : so;
+        double s1[NV], s2[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            s1[v] = side ? so[v] : pn[v];
+            s2[v] = side ? pn[v] : so[v];
+        }
         double fvn[NV];
         fvn[0] = 0.0;
         if (A.parabolic) {
