@@ -191,19 +191,24 @@ def test_ns_rk3_step_parity(cuda):
     assert r <= TOL
 
 
-def test_ec_entropy_conservation_gpu(cuda):
-    """EC volume + EC surface flux on a periodic mesh: sum w.J.w(u).RHS = 0."""
-    cfg = dict(dim=3, order=3, nelem=(4, 4, 4), lo=(0.0,) * 3, hi=(1.0,) * 3, ec_surface=True, quirks=False)
+@pytest.mark.parametrize("dim,algo", [(3, "line"), (3, "tile"), (2, "tile")])
+def test_ec_entropy_conservation_gpu(cuda, dim, algo):
+    """EC volume + EC surface flux on a periodic mesh: sum w.J.w(u).RHS = 0,
+    with either flux-differencing kernel."""
+    cfg = dict(dim=dim, order=3, nelem=(4,) * dim, lo=(0.0,) * dim, hi=(1.0,) * dim, ec_surface=True,
+               quirks=False)
     op = DGSEMOperator(**cfg)
+    op.set_algorithm(algo)
     X, st = states(cfg, seed=5)
     u = st["random"]
-    rhs = gpu_rhs(op, u, cuda).reshape(5, -1)
-    WJ = cases.quadrature_weights(3, 3, cfg["nelem"], cfg["lo"], cfg["hi"]).reshape(-1)
-    U = u.reshape(5, -1)
-    W = np.stack([O.cons2entropy(3, 1.4, U[:, i]) for i in range(U.shape[1])], axis=1)
+    nv = dim + 2
+    rhs = gpu_rhs(op, u, cuda).reshape(nv, -1)
+    WJ = cases.quadrature_weights(dim, 3, cfg["nelem"], cfg["lo"], cfg["hi"]).reshape(-1)
+    U = u.reshape(nv, -1)
+    W = np.stack([O.cons2entropy(dim, 1.4, U[:, i]) for i in range(U.shape[1])], axis=1)
     prod = np.sum(WJ * np.sum(W * rhs, axis=0))
     scale = np.sum(WJ * np.sum(np.abs(W * rhs), axis=0))
-    print("entropy production", prod, scale, flush=True)
+    print("entropy production (%d-D, %s)" % (dim, algo), prod, scale, flush=True)
     assert abs(prod) <= 1e-13 * scale
 
 
