@@ -950,6 +950,11 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         }
     }
 
+    // the block's wave speed: every contribution (nodes, boundary points) is
+    // in by now, and reducing it here keeps it out of the registers the pair
+    // phase needs
+    block_lam_max<BLOCK>(lam, lam_old, A.lam, sRed, slot);
+
     double accV[NV], accS[NV], accF[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) { accV[v] = 0.0; accS[v] = 0.0; accF[v] = 0.0; }
@@ -1143,7 +1148,6 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             A.out[gi] = o;
         }
     }
-    block_lam_max<BLOCK>(lam, lam_old, A.lam, sRed, slot);
 }
 
 // OPEN: the box has non-periodic sides (Euler, natural shape); the periodic
