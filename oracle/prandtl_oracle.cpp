@@ -45,7 +45,9 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <new>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -64,6 +66,14 @@ struct Cnt {
     Cnt() : v(0.0) {}
     Cnt(double x) : v(x) {}
 };
+/* Books the operations that follow to category k.  Only the counting
+ * instantiation, which runs on one thread, sets it: the other scalar types
+ * leave the shared variable alone, so their threads do not write it. */
+template <class T> inline void set_cat(int k)
+{
+    if constexpr (std::is_same<T, Cnt>::value) g_cat = k;
+    else (void)k;
+}
 inline Cnt operator+(Cnt a, Cnt b) { ++g_ops[g_cat]; return Cnt(a.v + b.v); }
 inline Cnt operator-(Cnt a, Cnt b) { ++g_ops[g_cat]; return Cnt(a.v - b.v); }
 inline Cnt operator*(Cnt a, Cnt b) { ++g_ops[g_cat]; return Cnt(a.v * b.v); }
@@ -1255,7 +1265,11 @@ template <class T> struct Engine {
     int dim, N, Np, K, NE, nv, Nfp;
     size_t NK;
     double gamma;
-    std::vector<T> U, W, G;  /* state copy (counted only), entropy vars, gradients */
+    std::vector<T> U;        /* state copy (counted only) */
+    /* entropy variables and gradients; every entry is written before it is
+     * read, so they are allocated uninitialised and first touched by the
+     * threads that fill them */
+    std::unique_ptr<T[]> W, G;
     bool axi;
     std::vector<T> Urec;     /* axisymmetric: the state recovered from r U */
 
@@ -1294,11 +1308,19 @@ template <class T> struct Engine {
 
     /* Indicator::CheckSmoothness (PerssonPeraireIndicator.cpp:21-58) and
      * DGSEMOperator::ComputeBlendingCoefficient (DGSEMOperator.cpp:60-80) */
-    void blending(const T *u)
+    void blending(const T *u, int nth)
     {
-        g_cat = CAT_IND;
-        std::vector<T> q(K), md(K);
+        set_cat<T>(CAT_IND);
         const double s = 9.21024, thr = c->thresh;
+        (void)nth;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nth)
+#endif
+        {
+        std::vector<T> q(K), md(K);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
         for (int e = 0; e < NE; e++) {
             for (int i = 0; i < K; i++) {
                 T st[5];
@@ -1331,16 +1353,33 @@ template <class T> struct Engine {
             a = std::min(a, c->p.alpha_max);
             c->alpha[e] = a;
         }
+        }
     }
 
     /* DGSEMOperator.cpp:82-94, DGSEMNonlinearForm.cpp:424-645,
      * LiftingBR1.cpp:10-66 (faces) and 156-272 (elements),
      * DGSEMOperator.cpp:96-159 (gradient conversion) */
-    void lifting(const T *u)
+    void lifting(const T *u, int nth)
     {
-        g_cat = CAT_LIFT;
-        W.assign(nv * NK, T(0.0));
-        G.assign(dim * nv * NK, T(0.0));
+        set_cat<T>(CAT_LIFT);
+        W.reset(new T[nv * NK]);
+        G.reset(new T[dim * nv * NK]);
+        const double *D = c->D.data();
+        const double w0 = c->w[0];
+        (void)nth;
+        /* Element loops are independent.  On boxes the interior faces of one
+         * direction touch disjoint nodes and a node takes at most one face
+         * term per direction, so running the directions in order, each in
+         * parallel, adds every node's terms in the serial order.  The faces
+         * of general meshes share corner nodes and run on one thread, and so
+         * do the boundary faces. */
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nth)
+#endif
+        {
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
         for (int e = 0; e < NE; e++)
             for (int i = 0; i < K; i++) {
                 T st[5], wv[5];
@@ -1348,7 +1387,9 @@ template <class T> struct Engine {
                 cons2entropy(dim, gamma, st, wv);
                 for (int v = 0; v < nv; v++) W[v * NK + (size_t)e * K + i] = wv[v];
             }
-        const double *D = c->D.data();
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
         for (int e = 0; e < NE; e++)
             for (int id = 0; id < K; id++) {
                 int ii[3] = {id % Np, (id / Np) % Np, id / (Np * Np)};
@@ -1372,8 +1413,7 @@ template <class T> struct Engine {
                         G[(cc * nv + v) * NK + (size_t)e * K + id] = a * invJ;
                     }
             }
-        const double w0 = c->w[0];
-        for (int f = 0; f < m.nf; f++)
+        auto face_terms = [&](int f) {
             for (int q = 0; q < Nfp; q++) {
                 int e1 = m.fe1[f], e2 = m.fe2[f];
                 if (e2 < 0) continue;
@@ -1390,6 +1430,24 @@ template <class T> struct Engine {
                     }
                 }
             }
+        };
+        if (m.quad) {
+#ifdef _OPENMP
+#pragma omp single
+#endif
+            for (int f = 0; f < m.nf; f++) face_terms(f);
+        } else {
+            for (int r = 0; r < dim; r++) {
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+                for (int f = r * NE; f < (r + 1) * NE; f++) face_terms(f);
+            }
+        }
+#ifdef _OPENMP
+#pragma omp single
+#endif
+        {
         /* boundary faces, after the interior ones: LiftingBR1::
          * AssembleLiftingBdrFaceVector (LiftingBR1.cpp:311-407) adds
          * f n_c / (w_0 J) with the boundary type's lifting flux f */
@@ -1431,6 +1489,10 @@ template <class T> struct Engine {
                     }
             }
         }
+        }
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
         for (int e = 0; e < NE; e++)
             for (int i = 0; i < K; i++) {
                 T st[5];
@@ -1442,13 +1504,14 @@ template <class T> struct Engine {
                     for (int v = 0; v < nv; v++) G[(cc * nv + v) * NK + (size_t)e * K + i] = g[v];
                 }
             }
+        }
     }
 
     /* DGSEMIntegrator::ComputeFVFluxes, DGSEMIntegrator.cpp:972-1196.
      * fv (K x nv, node-major) is overwritten with alpha * FV. */
     T fv_element(const T *u, int e, T alpha_e, T *fv, T lam)
     {
-        g_cat = CAT_FV;
+        set_cat<T>(CAT_FV);
         const int q = c->p.quirks;
         for (int i = 0; i < K * nv; i++) fv[i] = 0.0;
         for (int r = 0; r < dim; r++)
@@ -1503,7 +1566,7 @@ template <class T> struct Engine {
         else for (int i = 0; i < K * nv; i++) el[i] = 0.0;
         /* viscous volume flux, contravariant (DGSEMIntegrator.cpp:617-641) */
         if (par) {
-            g_cat = CAT_VISC;
+            set_cat<T>(CAT_VISC);
             for (int i = 0; i < K; i++) {
                 /* stale member id1 (line 620): the last node of the previous
                  * element loop, or of ComputeFVFluxes' zeta sweep (lines
@@ -1535,7 +1598,7 @@ template <class T> struct Engine {
         for (int k = 0; k < Nz; k++)
             for (int j = 0; j < Ny; j++)
                 for (int i = 0; i < Np; i++) {
-                    g_cat = CAT_VOL;
+                    set_cat<T>(CAT_VOL);
                     int id1 = (k * Np + j) * Np + i;
                     int ii[3] = {i, j, k};
                     T s1[5], s2[5], m1[3], m2[3];
@@ -1572,7 +1635,7 @@ template <class T> struct Engine {
                         if (r == ((c->p.plant & ORC_PLANT_ZETA_XI) ? 0 : 1))
                             for (int v = 0; v < nv; v++) glast[v] = f[v];
                     }
-                    g_cat = CAT_ASM;
+                    set_cat<T>(CAT_ASM);
                     T dU[5], dV[5];
                     for (int v = 0; v < nv; v++) { dU[v] = 0.0; dV[v] = 0.0; }
                     for (int r = 0; r < dim; r++) {
@@ -1636,7 +1699,7 @@ template <class T> struct Engine {
                 T d1[5], d2[5], fn[5];
                 if (par) {
                     /* viscous fluxes first, as DGSEMIntegrator.cpp:467-477 */
-                    g_cat = CAT_VISC;
+                    set_cat<T>(CAT_VISC);
                     T fl1[15], fl2[15], dq[3][5];
                     const T *dqp[3];
                     for (int cc = 0; cc < dim; cc++) {
@@ -1663,7 +1726,7 @@ template <class T> struct Engine {
                         for (int d = 0; d < dim; d++) { s1[1 + d] *= r1; s2[1 + d] *= r2; }
                     }
                 }
-                g_cat = CAT_FACE;
+                set_cat<T>(CAT_FACE);
                 if (ec) lam = mmax(lam, volume_flux(dim, gamma, s1, s2, nor, nor, F));
                 else lam = mmax(lam, face_flux(dim, gamma, quirks, s1, s2, nor, F));
                 if (axi) {
@@ -1674,7 +1737,7 @@ template <class T> struct Engine {
                 for (int v = 0; v < nv; v++) { d1[v] = -F[v]; d2[v] = F[v]; }
                 if (par)
                     for (int v = 0; v < nv; v++) { d1[v] += fn[v]; d2[v] -= fn[v]; }
-                g_cat = CAT_ASM;
+                set_cat<T>(CAT_ASM);
                 T a1 = 1.0 / (w0 * T(m.J[(size_t)e1 * K + n1]));
                 T a2 = 1.0 / (w0 * T(m.J[(size_t)e2 * K + n2]));
                 for (int v = 0; v < nv; v++) {
@@ -1704,7 +1767,7 @@ template <class T> struct Engine {
                 const int n1 = line_node(dim, Np, r, q, side ? N : 0);
                 T s1[5], F[5];
                 state(u, e, n1, s1);
-                g_cat = CAT_FACE;
+                set_cat<T>(CAT_FACE);
                 lam = mmax(lam, bc_flux(dim, gamma, c->p.quirks, type, s1, outer, nor, F));
                 T d[5];
                 for (int v = 0; v < nv; v++) d[v] = -F[v];
@@ -1714,7 +1777,7 @@ template <class T> struct Engine {
                      * (outflow, wall) or of the outer state with zero
                      * gradients (inflow, specified state); the adiabatic wall
                      * replaces the energy term by V.(Fv.n)_mom + qn |n| */
-                    g_cat = CAT_VISC;
+                    set_cat<T>(CAT_VISC);
                     T fl[15], dq[3][5], zero[5], fn[5];
                     const T *dqp[3];
                     const bool ext = type == ORC_BC_SUPERSONIC_INFLOW || type == ORC_BC_SPECIFIED_STATE;
@@ -1740,7 +1803,7 @@ template <class T> struct Engine {
                     }
                     for (int v = 0; v < nv; v++) d[v] += fn[v];
                 }
-                g_cat = CAT_ASM;
+                set_cat<T>(CAT_ASM);
                 T a1 = 1.0 / (w0 * T(m.J[(size_t)e * K + n1]));
                 for (int v = 0; v < nv; v++) dudt[v * NK + (size_t)e * K + n1] += a1 * d[v];
             }
@@ -1775,7 +1838,7 @@ template <class T> struct Engine {
                 T s1[5], F[5], nor[3];
                 state(u, e, n1, s1);
                 for (int d = 0; d < dim; d++) nor[d] = m.bq_nor[((size_t)b * Nfp + q) * dim + d];
-                g_cat = CAT_FACE;
+                set_cat<T>(CAT_FACE);
                 const int type = m.bq_type[b];
                 lam = mmax(lam, bc_flux(dim, gamma, c->p.quirks, type, s1, outer, nor, F));
                 if (axi) {
@@ -1787,7 +1850,7 @@ template <class T> struct Engine {
                 for (int v = 0; v < nv; v++) d[v] = -F[v];
                 if (c->p.parabolic) {
                     /* as bfaces: -F* + Fv.n */
-                    g_cat = CAT_VISC;
+                    set_cat<T>(CAT_VISC);
                     T fl[15], dq[3][5], zero[5], fn[5], sv[5];
                     const T *dqp[3];
                     const bool ext = type == ORC_BC_SUPERSONIC_INFLOW || type == ORC_BC_SPECIFIED_STATE;
@@ -1811,7 +1874,7 @@ template <class T> struct Engine {
                     }
                     for (int v = 0; v < nv; v++) d[v] += fn[v];
                 }
-                g_cat = CAT_ASM;
+                set_cat<T>(CAT_ASM);
                 T a1 = 1.0 / (w0 * T(m.J[(size_t)e * K + n1]));
                 for (int v = 0; v < nv; v++) dudt[v * NK + (size_t)e * K + n1] += a1 * d[v];
             }
@@ -1833,11 +1896,11 @@ template <class T> struct Engine {
             for (size_t i = 0; i < nv * NK; i++) Urec[i] = T(Ud[i]);
             u = Urec.data();
         }
-        if (c->p.blending) blending(u);
-        if (c->p.parabolic) lifting(u);
+        int nth = threads > 0 ? threads : 1;
+        if (c->p.blending) blending(u, nth);
+        if (c->p.parabolic) lifting(u, nth);
         for (size_t i = 0; i < nv * NK; i++) dudt[i] = 0.0;
         double lam = c->lam;
-        int nth = threads > 0 ? threads : 1;
 #ifdef _OPENMP
 #pragma omp parallel num_threads(nth) reduction(max : lam)
 #endif
@@ -2190,7 +2253,18 @@ int orc_rhs(orc_ctx *c, const double *u, double *dudt)
     c->t_stage = c->time;
     Engine<double> E(c);
     E.rhs(u, dudt, c->p.threads);
-    if (c->p.parabolic) c->grad.assign(E.G.begin(), E.G.end());
+    if (c->p.parabolic) {
+        const size_t n = (size_t)c->m.dim * c->m.nv * c->m.NE * c->m.K;
+        c->grad.resize(n);
+        double *dst = c->grad.data();
+        const double *src = E.G.get();
+        const int nth = c->p.threads > 0 ? c->p.threads : 1;
+        (void)nth;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nth) schedule(static)
+#endif
+        for (long long i = 0; i < (long long)n; i++) dst[i] = src[i];
+    }
     return 0;
 }
 
