@@ -3,7 +3,8 @@ the mesh readers and geometry builders (prandtl_amd/csrc/mesh.cpp) and the
 oracle, built from their sources by tests/sanitize/Makefile (g++), run on
 every mesh file the tests hold, on malformed files (which must be refused
 with PRANDTL_ERR_IO and a message, not crash or over-allocate) and on small
-oracle RHS/RK3 runs.  A sanitizer report makes the driver exit non-zero."""
+oracle RHS/RK3 runs.  A sanitizer report makes the test program exit
+non-zero."""
 import os
 import subprocess
 
