@@ -198,7 +198,8 @@ TRIPLE_POINT = dict(dim=2, order=3, nelem=(112, 48), lo=(0.0, 0.0), hi=(7.0, 3.0
 # [0, sx] refined ser_ref_levels times).  Each is a piecewise state: `states`
 # holds (x_end, rho, u, p) pieces in order, x <= x_end for "le" cases, x <
 # x_end otherwise (rho may be a function of x); specified-state and inflow
-# sides take the state the initial condition gives at that end.
+# sides take the state the initial condition gives at that end.  gamma and
+# alpha_max default to 1.4 and the reference's 0.5 (Simulation.cpp:477-483).
 SHOCK_TUBES = {
     # SodShockTube.hpp:9-31: nx 4, sx 1, 5 levels, t 0.25
     "sod": dict(nelem=128, hi=1.0, t_final=0.25, bc=("specified-state", "specified-state"),
@@ -215,6 +216,30 @@ SHOCK_TUBES = {
     # WoodwardColellaBlastWave.hpp: nx 4, 8 levels, slip walls, t 0.038
     "blast": dict(nelem=1024, hi=1.0, t_final=0.038, bc=("slip", "slip"), le=True,
                   states=((0.1, 1.0, 0.0, 1000.0), (0.9, 1.0, 0.0, 0.01), (None, 1.0, 0.0, 100.0))),
+    # ModifiedSodShockTube.hpp: nx 4, 5 levels, t 0.2
+    "modified_sod": dict(nelem=128, hi=1.0, t_final=0.2, bc=("specified-state", "specified-state"),
+                         states=((0.5, 1.0, 0.75, 1.0), (None, 0.125, 0.0, 0.1))),
+    # Problem123.hpp (two rarefactions moving apart): nx 4, 5 levels, t 0.15
+    "problem123": dict(nelem=128, hi=1.0, t_final=0.15, bc=("specified-state", "specified-state"),
+                       states=((0.5, 1.0, -2.0, 0.4), (None, 1.0, 2.0, 0.4))),
+    # WoodwardColellaBlastWaveLeft.hpp / ...Right.hpp / ...Collision.hpp: the
+    # left and right halves of the blast wave and the collision of the two
+    # shocks they send out; nx 4, 5 levels
+    "blast_left": dict(nelem=128, hi=1.0, t_final=0.012, bc=("specified-state", "specified-state"),
+                       states=((0.5, 1.0, 0.0, 1000.0), (None, 1.0, 0.0, 0.01))),
+    "blast_right": dict(nelem=128, hi=1.0, t_final=0.035, bc=("specified-state", "specified-state"),
+                        states=((0.5, 1.0, 0.0, 0.01), (None, 1.0, 0.0, 100.0))),
+    # (config.json of the collision: alpha_max 0.75)
+    "blast_collision": dict(nelem=128, hi=1.0, t_final=0.035, bc=("specified-state", "specified-state"),
+                            alpha_max=0.75,
+                            states=((0.5, 5.99924, 19.5975, 460.894), (None, 5.99242, -6.19633, 46.0950))),
+    # LeBlancShockTube.hpp: nx 16 on [0, 9], 5 levels, gamma 1.66667,
+    # alpha_max 1, supersonic outflow at both ends, t 6; the initial
+    # condition writes the conservative state itself, (rho, rho u, rho E) =
+    # (1, 0, 0.1) for x < 3 and (1e-3, 0, 1e-9) beyond
+    "leblanc": dict(nelem=512, hi=9.0, t_final=6.0, gamma=1.66667, alpha_max=1.0, cons=True,
+                    bc=("supersonic-outflow", "supersonic-outflow"),
+                    states=((3.0, 1.0, 0.0, 0.1), (None, 1e-3, 0.0, 1e-9))),
 }
 
 
@@ -239,7 +264,7 @@ def shock_tube(name, order=3):
     """DGSEMOperator keywords of a 1-D case of SHOCK_TUBES, and a function of
     the node coordinates X [1][NE][K] giving its initial state."""
     c = SHOCK_TUBES[name]
-    gamma = 1.4
+    gamma = c.get("gamma", 1.4)
 
     def prim_at(x):
         x = np.asarray(x, float)
@@ -251,17 +276,23 @@ def shock_tube(name, order=3):
             done |= sel
         return rho, u, p
 
+    def cons_at(x):
+        # the states as written: primitive (rho, u, p), or with "cons" the
+        # conservative (rho, rho u, rho E) themselves
+        a, b, q = prim_at(x)
+        if c.get("cons"):
+            return np.array([a, b, q])
+        return prim_to_cons(a, [b], q, gamma)
+
     def state(X):
-        rho, u, p = prim_at(X[0])
-        return prim_to_cons(rho, [u], p, gamma)
+        return cons_at(X[0])
 
     ends = []
     for x in (0.0, c["hi"]):
-        r, u, p = (float(a[0]) for a in prim_at(np.array([x])))
-        ends.append(tuple(prim_to_cons(np.array(r), [np.array(u)], np.array(p), gamma).tolist())
+        ends.append(tuple(float(v[0]) for v in cons_at(np.array([x])))
                     if c["bc"][len(ends)] in ("specified-state", "supersonic-inflow") else None)
     kw = dict(dim=1, order=order, nelem=(c["nelem"],), lo=(0.0,), hi=(c["hi"],), gamma=gamma,
-              blending=True, alpha_max=0.5, bc=[c["bc"]], bc_state=[tuple(ends)])
+              blending=True, alpha_max=c.get("alpha_max", 0.5), bc=[c["bc"]], bc_state=[tuple(ends)])
     return kw, state
 
 
@@ -308,6 +339,30 @@ DMR = dict(mesh="DoubleMachReflection.msh", order=3, blending=True, alpha_max=0.
            refine=1, bc={"left": ("specified-state", DMR_POST), "bottom1": ("specified-state", DMR_POST),
                          "bottom2": "slip", "right": "supersonic-outflow", "top": ("specified-state", DMR_PRE)},
            split={"top": DMR_TOP_SPLIT})
+
+
+# BackwardFacingStep (TestCases/Euler/2D/BackwardFacingStep: its mesh, order
+# 3, blending, alpha_max 0.5 by default, cfl 1, t 0.01): the post-shock state
+# of BackwardFacingStepIC (BackwardFacingStep.hpp:9-37) for x < 0.5, gas at
+# rest (rho 1, p 1) beyond; BackwardFacingStepLeftBCVector (line 40) at the
+# supersonic inflow, slip walls below and above, supersonic outflow.  Its
+# config steps with RK4 ("ode_solver": "RK4"), which a caller composes from
+# the operator's Mult.
+BFS_INFLOW = np.array([5.9970, 5.9970 * 98.5914, 0.0, 11666.5 / (1.4 - 1.0) + 0.5 * 5.9970 * 98.5914 * 98.5914])
+BFS = dict(mesh="BackwardFacingStep.msh", order=3, blending=True, alpha_max=0.5, t_final=0.01,
+           bc={"left": ("supersonic-inflow", BFS_INFLOW), "bottom": "slip", "right": "supersonic-outflow",
+               "top": "slip"})
+
+
+def bfs_state(X, gm1i=2.5):
+    """BackwardFacingStepIC at the node coordinates X [2][...], with 1/(gamma
+    - 1) the config's parameter x1 = 2.5 as written."""
+    x = np.asarray(X[0], float)
+    left = x < 0.5
+    rho = np.where(left, 5.9970, 1.0)
+    u = np.where(left, 98.5914, 0.0)
+    p = np.where(left, 11666.5, 1.0)
+    return np.array([rho, rho * u, np.zeros_like(x), p * gm1i + rho * 0.5 * (u * u)])
 
 
 def dmr_state(X, gamma=1.4):

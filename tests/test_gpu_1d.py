@@ -254,3 +254,89 @@ def test_shu_osher_device_loop_matches_oracle_run(cuda):
     assert n == m and r <= 1e-8
     # the inflow state is kept at the left end
     assert np.allclose(U.reshape(3, -1)[:, 0], U0.reshape(3, -1)[:, 0], rtol=1e-9, atol=0)
+
+
+NEW_TUBES = ["modified_sod", "problem123", "blast_left", "blast_right", "blast_collision", "leblanc"]
+
+
+def oracle_run(orc, U0, T, order):
+    """The oracle on Simulation::Run's loop as the device loop runs it
+    (Simulation.cpp:893-907, 1007-1024)."""
+    ref, tt, m, done = U0.copy(), 0.0, 0, False
+    orc.Mult(ref)
+    dt = orc.hmin() / (orc.GetMaxCharSpeed() * (order + 1) ** 2)
+    while not done:
+        dt_real = min(dt, T - tt)
+        ref = orc.Step(ref, dt_real)
+        tt += dt_real
+        dt = orc.hmin() / (orc.GetMaxCharSpeed() * (order + 1) ** 2)
+        m += 1
+        done = tt >= T - 1e-8 * dt
+    return ref, m
+
+
+@pytest.mark.parametrize("name", NEW_TUBES)
+def test_reference_tubes_rk3_steps_parity(cuda, name):
+    """20 RK3 steps of each remaining 1-D reference case at its full size,
+    dt at half the CFL limit of the initial state (gamma 1.66667 and
+    alpha_max 1 for LeBlanc, alpha_max 0.75 for the blast-wave collision)."""
+    import torch
+    kw, init = cases.shock_tube(name)
+    op = DGSEMOperator(**kw)
+    orc = O.Oracle(threads=8, **kw)
+    u = init(orc.coords()).reshape(-1)
+    orc.Mult(u)
+    dt = 0.5 * orc.hmin() / (orc.GetMaxCharSpeed() * (kw["order"] + 1) ** 2)
+    orc.reset_wave_speed()
+    du = to_dev(u, cuda)
+    ref = u.copy()
+    for _ in range(20):
+        op.Step(du, dt)
+        ref = orc.Step(ref, dt)
+    torch.cuda.synchronize()
+    r = rel_max(du.cpu().numpy(), ref)
+    print(name, "20 steps at dt %.3g: rel %.3e" % (dt, r), flush=True)
+    assert np.isfinite(ref).all() and r <= TOL
+
+
+@pytest.mark.parametrize("name", NEW_TUBES)
+def test_reference_tubes_device_loop_match_oracle_run(cuda, name):
+    """Each remaining 1-D reference case to its final time on the device
+    loop (cfl 1): the same number of steps as the oracle's run with the
+    same dt policy, states that agree to the rounding the shocks amplify,
+    and the exact Riemann solution within the bound the oracle meets.
+    LeBlanc's tube (a 1e8 pressure ratio, 6512 steps; 20 steps agree to
+    1e-12 above) amplifies the rounding differences of the two summation
+    orders to a few 1e-8, so its bound is 1e-7; the others keep 1e-8."""
+    import torch
+    kw, init = cases.shock_tube(name)
+    c = cases.SHOCK_TUBES[name]
+    T = c["t_final"]
+    op = DGSEMOperator(**kw)
+    orc = O.Oracle(threads=8, **kw)
+    X = orc.coords()
+    U0 = init(X).reshape(-1)
+    u = to_dev(U0, cuda)
+    t, n = op.Run(u, 0.0, T, cfl=1.0)
+    torch.cuda.synchronize()
+    ref, m = oracle_run(orc, U0, T, kw["order"])
+    U = u.cpu().numpy()
+    r = rel_max(U, ref)
+    x0, a, b, q = c["states"][0]
+    _, a2, b2, q2 = c["states"][1]
+    g = kw["gamma"]
+    if c.get("cons"):
+        left, right = (a, b / a, (g - 1) * (q - 0.5 * b * b / a)), (a2, b2 / a2, (g - 1) * (q2 - 0.5 * b2 * b2 / a2))
+    else:
+        left, right = (a, b, q), (a2, b2, q2)
+    ex = R.sample(left, right, (X[0].reshape(-1) - x0) / t, g=g)
+    W = cases.quadrature_weights(1, kw["order"], kw["nelem"], kw["lo"], kw["hi"]).reshape(-1)
+    rho = U.reshape(3, -1)[0]
+    l1 = float(np.sum(W * np.abs(rho - ex[0])) / np.sum(W * np.abs(ex[0])))
+    print("%s: device %d steps, oracle %d, rel %.3e, relative L1(rho) vs exact %.3e"
+          % (name, n, m, r, l1), flush=True)
+    assert np.isfinite(U).all() and (rho > 0).all()
+    assert abs(t - T) <= 1e-12 * T
+    assert n == m and r <= (1e-7 if name == "leblanc" else 1e-8)
+    assert l1 <= {"modified_sod": 8e-3, "problem123": 2.5e-2, "blast_left": 8e-2, "blast_right": 8e-2,
+                  "blast_collision": 3e-2, "leblanc": 1e-2}[name]

@@ -306,3 +306,65 @@ def test_lid_driven_cavity_reference_mesh(cuda):
     rg = rel_max(g.cpu().numpy(), orc.gradients())
     print("ldc rhs rel", r, "grad rel", rg, flush=True)
     assert r <= TOL and rg <= TOL
+
+
+def rk4_step(mult, x, dt):
+    """MFEM's RK4Solver::Step on a Mult: k1..k4 with y and the running sum z
+    formed in the solver's order."""
+    k = mult(x)
+    y = x + (dt / 2) * k
+    z = x + (dt / 6) * k
+    k = mult(y)
+    y = x + (dt / 2) * k
+    z = z + (dt / 3) * k
+    k = mult(y)
+    y = x + dt * k
+    z = z + (dt / 3) * k
+    k = mult(y)
+    return z + (dt / 6) * k
+
+
+def test_backward_facing_step_reference_mesh_rk4(cuda):
+    """The reference's BackwardFacingStep case on its own mesh (35000
+    quadrilaterals, order 3, blending): the post-shock inflow, slip walls and
+    supersonic outflow, with the shock at x = 0.5 initially.  Its config
+    steps with RK4, which a caller composes from the operator's Mult as
+    MFEM's RK4Solver does: 3 RK4 steps through the operator against the same
+    steps through the oracle, the RHS of the initial state within 1e-12 and
+    the stepped state within 1e-12."""
+    import torch
+    c = cases.BFS
+    mesh = QuadMesh.read(os.path.join(HERE, "golden", "meshes", c["mesh"]))
+    op = DGSEMOperator.quad(mesh, c["order"], bc=c["bc"], blending=c["blending"], alpha_max=c["alpha_max"])
+    nattr = int(mesh.bdr_attr.max())
+    kinds, states = ["slip"] * nattr, [c["bc"]["left"][1]] * nattr
+    for key, v in c["bc"].items():
+        kinds[mesh.names[key] - 1] = v if isinstance(v, str) else v[0]
+    d = dict(vert=mesh.vert, elem=mesh.elem, bdr=mesh.bdr, bdr_attr=mesh.bdr_attr, attr_bc=kinds,
+             attr_state=states)
+    orc = O.Oracle.quad(d, c["order"], blending=c["blending"], alpha_max=c["alpha_max"], threads=16)
+    u0 = cases.bfs_state(orc.coords()).reshape(-1)
+    r0 = orc.Mult(u0)
+    dt = 0.5 * orc.hmin() / (orc.GetMaxCharSpeed() * (c["order"] + 1) ** 2)
+    du = to_dev(u0, cuda)
+    out = torch.empty_like(du)
+
+    def gpu_mult(x):
+        y = torch.empty_like(x)
+        op.Mult(x, y)
+        return y
+
+    op.Mult(du, out)
+    torch.cuda.synchronize()
+    r = rel_max(out.cpu().numpy(), r0)
+    alpha, _ = op.alpha()
+    print("bfs: RHS rel %.3e, alpha > 0 on %d elements, dt %.3g" % (r, int((alpha > 0).sum()), dt), flush=True)
+    assert r <= TOL
+    x, ref = du, u0.copy()
+    for _ in range(3):
+        x = rk4_step(gpu_mult, x, dt)
+        ref = rk4_step(orc.Mult, ref, dt)
+    torch.cuda.synchronize()
+    r = rel_max(x.cpu().numpy(), ref)
+    print("bfs: 3 RK4 steps rel %.3e" % r, flush=True)
+    assert np.isfinite(ref).all() and r <= TOL
