@@ -243,11 +243,46 @@ __device__ __forceinline__ void box_index(const Geo &g, int e, int *idx)
     idx[2] = k;
 }
 
+// The step size of a stage: the device value when the caller keeps dt on the
+// device, else the argument.  Written as a branch on the pointer with the
+// loaded value passed through an empty asm: a choice between the two loads
+// would be compiled to one flat load through a selected address, and a
+// flat load still in flight makes the next memory wait drain every
+// outstanding load.
+__device__ __forceinline__ double step_dt(const KArgs &A)
+{
+    double dt = A.dt;
+    if (A.dtp) {
+        double d = *A.dtp;
+        asm volatile("" : "+v"(d));
+        dt = d;
+    }
+    return dt;
+}
+
 __device__ __forceinline__ int neighbor(const Geo &g, int e, int r, int side)
 {
     int idx[3];
     box_index(g, e, idx);
     idx[r] = side ? (idx[r] + 1 == g.n[r] ? 0 : idx[r] + 1) : (idx[r] == 0 ? g.n[r] - 1 : idx[r] - 1);
+    return (idx[2] * g.n[1] + idx[1]) * g.n[0] + idx[0];
+}
+
+// neighbor() for callers whose direction r differs between the lanes of a
+// wave (one thread per line or per face point of several directions).
+// Indexing the argument struct with a per-lane r is a vector load from
+// argument memory that the thread waits for before its state loads can
+// issue; so is a select between the three fields, which the compiler turns
+// into one load through a selected address.  The empty asm takes the three
+// scalar-loaded values as they are, and the select is then between values.
+__device__ __forceinline__ int neighbor_lane(const Geo &g, int e, int r, int side)
+{
+    int idx[3];
+    box_index(g, e, idx);
+    int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    asm volatile("" : "+s"(n0), "+s"(n1), "+s"(n2));
+    const int nr = r == 0 ? n0 : r == 1 ? n1 : n2;
+    idx[r] = side ? (idx[r] + 1 == nr ? 0 : idx[r] + 1) : (idx[r] == 0 ? nr - 1 : idx[r] - 1);
     return (idx[2] * g.n[1] + idx[1]) * g.n[0] + idx[0];
 }
 
@@ -555,7 +590,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     // ---- prologue: own node state, the neighbour data of this thread's
     // first surface point, the step size and the tile's wave-speed slot, all
     // loads in flight together ----
-    const double dt = A.dtp ? *A.dtp : A.dt;
+    const double dt = step_dt(A);
     const double lam_old = (t == 0) ? A.lam[slot] : 0.0;
     double ys[NV];
     if (act)
@@ -861,7 +896,10 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             if constexpr (AXI) {
                 // flux_num *= r (DGSEMIntegrator.cpp:190-205), the r of the
                 // first element's node on both sides
-                const double rf = own ? sR[ej][no] : A.geo.rnode[gn];
+                // (both values read, then chosen: a choice between the two
+                // loads compiles to one flat load through a selected address)
+                const double rl = sR[ej][no], rg = A.geo.rnode[gn];
+                const double rf = own ? rl : rg;
 #pragma unroll
                 for (int v = 0; v < NV; v++) F[v] *= rf;
             }
@@ -1252,7 +1290,7 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
             const int rem = j - ej * (2 * DIM * NFP);
             const int f = rem / NFP, q = rem - f * NFP;
             const int r = f >> 1, side = f & 1;
-            int en = neighbor(A.geo, eo, r, side);
+            int en = neighbor_lane(A.geo, eo, r, side);
             int nn = line_node<DIM, NP>(r, q, side ? 0 : NP - 1);
             if (OPEN && boundary_of(A.geo, eo, r, side) != BC_PERIODIC) {
                 // boundary point: this element's own face node

@@ -185,7 +185,7 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
     const size_t NK = (size_t)NE * K;
     const Phys &P = A.phys;
     const bool quirk3d = A.quirks != 0;
-    const double dt = A.dtp ? *A.dtp : A.dt;
+    const double dt = step_dt(A);
     const double lam_old = (t == 0) ? A.lam[blockIdx.x] : 0.0;
     if (BLEND)
         for (int i = t; i < NP * NP; i += BLOCK) sV1i[i] = A.tab.V1i[(i / NP) * MAXNP + i % NP];
@@ -205,48 +205,62 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
     const int ps = r == 0 ? 1 : (r == 1 ? NP + 1 : NP * (NP + 1));
     const double nr = r == 0 ? A.geo.nrm[0] : (r == 1 ? A.geo.nrm[1] : A.geo.nrm[2]);
 
-    // neighbour traces at both ends of the line, in flight during the node
-    // phase
-    double pl[NV], ph[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) pl[v] = ph[v] = 1.0;
-    const int enl = lact ? neighbor(A.geo, e, r, 0) : 0, enh = lact ? neighbor(A.geo, e, r, 1) : 0;
-    if (lact) {
-        const size_t gl = (size_t)enl * K + base + (NP - 1) * stride, gh = (size_t)enh * K + base;
-#pragma unroll
-        for (int v = 0; v < NV; v++) { pl[v] = A.y[v * NK + gl]; ph[v] = A.y[v * NK + gh]; }
-    }
-    // viscous face traces Fv.n at the line's two face points, this element's
-    // and the neighbour's, also in flight during the node phase
-    VT fto[2][NV - 1], ftn[2][NV - 1];
-#pragma unroll
-    for (int side = 0; side < 2; side++)
-#pragma unroll
-        for (int v = 0; v < NV - 1; v++) fto[side][v] = ftn[side][v] = VT(0);
-    if (A.parabolic && lact) {
-        const VT *fv = static_cast<const VT *>(A.fvf);
-#pragma unroll
-        for (int side = 0; side < 2; side++) {
-            const VT *fo = fv + (((size_t)e * 2 * DIM + 2 * r + side) * (NV - 1)) * NFP;
-            const VT *fn = fv + (((size_t)(side ? enh : enl) * 2 * DIM + 2 * r + 1 - side) * (NV - 1)) * NFP;
-#pragma unroll
-            for (int v = 0; v < NV - 1; v++) { fto[side][v] = fo[v * NFP + l]; ftn[side][v] = fn[v * NFP + l]; }
-        }
-    }
-
-    // ---- node phase: state and per-node flux quantities into LDS ----
-    double lam = 0.0;
+    // Every global load of the start is issued on every path, the workgroup's
+    // nodes first.  Loads complete in issue order, so the node phase then
+    // waits for its own loads alone while the line's traces are still in
+    // flight; a load skipped on some path would make that wait a wait for
+    // everything.  Lanes without an element load from index 0 and discard
+    // the value; without viscous terms the viscous traces are read from the
+    // state array (at most 3 NP^2 entries, within its 5 NP^3) and discarded.
     const int nact = (NE - ebase) < EPB ? (NE - ebase) * K : EPB * K;
-    // the workgroup's nodes in NPASS passes of BLOCK threads; every pass's
-    // loads are issued before the first is used
     constexpr int NPASS = (EPB * K + BLOCK - 1) / BLOCK;
     double ysp[NPASS][NV];
 #pragma unroll
     for (int p = 0; p < NPASS; p++) {
         const int i = t + p * BLOCK;
+        const size_t gi = (i < nact) ? (size_t)ebase * K + i : 0;
 #pragma unroll
-        for (int v = 0; v < NV; v++) ysp[p][v] = (i < nact) ? A.y[v * NK + (size_t)ebase * K + i] : 1.0;
+        for (int v = 0; v < NV; v++) {
+            const double y = A.y[v * NK + gi];
+            ysp[p][v] = (i < nact) ? y : 1.0;
+        }
     }
+
+    // neighbour traces at both ends of the line, in flight during the node
+    // phase
+    double pl[NV], ph[NV];
+    const int enl = lact ? neighbor_lane(A.geo, e, r, 0) : 0, enh = lact ? neighbor_lane(A.geo, e, r, 1) : 0;
+    {
+        const size_t gl = lact ? (size_t)enl * K + base + (NP - 1) * stride : 0;
+        const size_t gh = lact ? (size_t)enh * K + base : 0;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const double a = A.y[v * NK + gl], b = A.y[v * NK + gh];
+            pl[v] = lact ? a : 1.0;
+            ph[v] = lact ? b : 1.0;
+        }
+    }
+    // viscous face traces Fv.n at the line's two face points, this element's
+    // and the neighbour's, also in flight during the node phase
+    VT fto[2][NV - 1], ftn[2][NV - 1];
+    {
+        const bool vact = A.parabolic && lact;
+        const VT *fv = A.parabolic ? static_cast<const VT *>(A.fvf) : reinterpret_cast<const VT *>(A.y);
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const size_t oo = vact ? (((size_t)e * 2 * DIM + 2 * r + side) * (NV - 1)) * NFP + l : 0;
+            const size_t on = vact ? (((size_t)(side ? enh : enl) * 2 * DIM + 2 * r + 1 - side) * (NV - 1)) * NFP + l : 0;
+#pragma unroll
+            for (int v = 0; v < NV - 1; v++) {
+                const VT a = fv[oo + v * NFP], b = fv[on + v * NFP];
+                fto[side][v] = vact ? a : VT(0);
+                ftn[side][v] = vact ? b : VT(0);
+            }
+        }
+    }
+
+    // ---- node phase: state and per-node flux quantities into LDS ----
+    double lam = 0.0;
 #pragma unroll
     for (int p = 0; p < NPASS; p++) {
         const int i = t + p * BLOCK;

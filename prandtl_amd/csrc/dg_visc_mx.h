@@ -116,7 +116,7 @@ __global__ void __launch_bounds__((MxCfg<NP>::BLOCK)) k_visc_mx(const VArgs A)
             const int rem = j - ej * (2 * DIM * NFP);
             const int f = rem / NFP, q = rem - f * NFP;
             const int r = f >> 1, side = f & 1;
-            const int en = neighbor(A.geo, eo, r, side);
+            const int en = neighbor_lane(A.geo, eo, r, side);
             const int nn = line_node<DIM, NP>(r, q, side ? 0 : NP - 1);
 #pragma unroll
             for (int v = 0; v < NV; v++) tv[k][v] = A.y[v * NK + (size_t)en * K + nn];
