@@ -569,12 +569,15 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     // tid: the thread's index as the caller sees it (a loop over tiles passes
     // one the compiler cannot hoist out of the loop)
     const int t = tid < 0 ? (int)threadIdx.x : tid;
-    for (int i = t; i < NP * NP; i += BLOCK) {
-        sDh2[i] = A.tab.Dh2[(i / NP) * MAXNP + i % NP];
-        if (BLEND) sV1i[i] = A.tab.V1i[(i / NP) * MAXNP + i % NP];
-    }
-    for (int i = t; i < NP; i += BLOCK) sFvs[i] = A.tab.fvs[i];
-    for (int i = t; i < DIM * NP; i += BLOCK) sSub[i / NP][i % NP] = A.tab.subn[i / NP][i % NP];
+    // the operator's tables, one entry per thread, loaded together and first
+    // (indices clamped, so every lane loads and no load waits for another);
+    // they go to LDS once the prologue's loads are in flight
+    static_assert(NP * NP <= BLOCK, "one table entry per thread");
+    const int tq = t < NP * NP ? t : 0, tf = t < NP ? t : 0, ts = t < DIM * NP ? t : 0;
+    const double tDh2 = A.tab.Dh2[(tq / NP) * MAXNP + tq % NP];
+    const double tV1i = BLEND ? A.tab.V1i[(tq / NP) * MAXNP + tq % NP] : 0.0;
+    const double tFvs = A.tab.fvs[tf];
+    const double tSub = A.tab.subn[ts / NP][ts % NP];
 
     const Phys &P = A.phys;
     const int NE = A.geo.NE;
@@ -648,6 +651,13 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             }
         }
     }
+
+    if (t < NP * NP) {
+        sDh2[t] = tDh2;
+        if (BLEND) sV1i[t] = tV1i;
+    }
+    if (t < NP) sFvs[t] = tFvs;
+    if (t < DIM * NP) sSub[t / NP][t % NP] = tSub;
 
     // ---- node phase: state and per-node flux quantities into LDS ----
     double lam = 0.0;
