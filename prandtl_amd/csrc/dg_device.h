@@ -268,6 +268,15 @@ __device__ __forceinline__ int neighbor(const Geo &g, int e, int r, int side)
     return (idx[2] * g.n[1] + idx[1]) * g.n[0] + idx[0];
 }
 
+// a, b or c for r = 0, 1, 2 where r differs between the lanes of a wave:
+// the three values pass through an empty asm first, so the choice is between
+// values, not a load through a selected address of the argument struct
+__device__ __forceinline__ double lane_pick3(double a, double b, double c, int r)
+{
+    asm volatile("" : "+v"(a), "+v"(b), "+v"(c));
+    return r == 0 ? a : r == 1 ? b : c;
+}
+
 // neighbor() for callers whose direction r differs between the lanes of a
 // wave (one thread per line or per face point of several directions).
 // Indexing the argument struct with a per-lane r is a vector load from
@@ -639,7 +648,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             const int rem = jf - ej * JF;
             const int f = rem / NFP, l = rem - f * NFP;
             const int r = f >> 1, side = f & 1;
-            const int en = neighbor(A.geo, eo, r, side);
+            const int en = neighbor_lane(A.geo, eo, r, side);
             const int nn = line_node<DIM, NP>(r, l, side ? 0 : NP - 1);
 #pragma unroll
             for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)en * K + nn];
@@ -777,10 +786,10 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         const int rem = jf - ej * JF;
         const int f = rem / NFP, l = rem - f * NFP;
         const int r = f >> 1, side = f & 1;
-        const double nr = A.geo.nrm[r];
+        const double nr = lane_pick3(A.geo.nrm[0], A.geo.nrm[1], A.geo.nrm[2], r);
         if (!GEN && (!PREFETCH || jf != t)) {
             // not prefetched, or more surface points than threads: load now
-            const int en = neighbor(A.geo, eo, r, side);
+            const int en = neighbor_lane(A.geo, eo, r, side);
             const int nn = line_node<DIM, NP>(r, l, side ? 0 : NP - 1);
 #pragma unroll
             for (int v = 0; v < NV; v++) pn[v] = A.y[v * NK + (size_t)en * K + nn];
