@@ -527,6 +527,14 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         }
     }
 
+    // Every start load has been used by now on the lanes that have a line;
+    // saying so (vmcnt 0, which costs nothing here) keeps lanes without one
+    // from leaving those loads pending in the compiler's count, which would
+    // make it drain the assembly loads below one by one as they are issued.
+    // (The immediate is gfx9's s_waitcnt encoding: vmcnt 0, expcnt 7,
+    // lgkmcnt 15.)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+
     // ---- the assembly's inputs (viscous volume term, u0), issued before
     // the direction sums so their latency overlaps them ----
     double vsp[NPASS][NV - 1], u0p[NPASS][NV];
