@@ -1370,6 +1370,12 @@ __attribute__((amdgpu_waves_per_eu(VCfg<DIM, NP, FB>::WAVES))) k_visc(const VArg
         }
     }
     __syncthreads();
+    // every start load has been used by now; saying so (free here, before
+    // any store) keeps the compiler's count from treating a load left unused
+    // on some lane's path as pending, which would make it wait for this
+    // thread's face-trace stores to complete further down.  (gfx9's s_waitcnt
+    // immediate: vmcnt 0, expcnt 7, lgkmcnt 15.)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
 
     R gp[DIM][NV];
     R yr[NV];
