@@ -246,11 +246,12 @@ def cpu_baseline(name, cfg, quirks, u, budget_s):
     for label, threads in (("all", nthr), ("one", 1)):
         orc = make_oracle(cfg, quirks, threads)
         nodes = orc.NE * orc.K
+        orc.Mult(u)   # the first evaluation pays for first-touch allocation
         t0 = time.perf_counter()
         orc.Mult(u)
         one = max(time.perf_counter() - t0, 1e-6)
         share = budget_s * (0.4 if label == "all" else 0.6)
-        per_rep = max(1, min(20, int(share / 5 / one)))
+        per_rep = max(1, min(200, int(share / 5 / one)))
         rate, spent = median_rate(orc, u, nodes, 5, per_rep)
         out[label] = (rate, per_rep, spent, threads)
     rate, per_rep, spent, threads = out["all"]
