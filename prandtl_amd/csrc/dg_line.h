@@ -139,6 +139,13 @@ template <int DIM, int NP> struct LCfg {
     static constexpr int EPB = line_epb(TE);
     static constexpr int BLOCK = r64(EPB * TE);
     static constexpr int NQ = DIM + 3;             // v[DIM], beta, 1/beta, lambda
+    // SWZ (NP = 4): the skewed LDS slots of line_slot with 16 doubles of
+    // padding per element, and half-waves that each hold one direction's
+    // lines of two elements; otherwise node n at n + n / NP, and lanes in
+    // (element, direction, line) order
+    static constexpr bool SWZ = (NP == 4);
+    static constexpr int KP = SWZ ? K : K + K / NP;   // doubles per quantity row
+    static constexpr int EPAD = SWZ ? 16 : 0;         // doubles after each element's rows
 #ifndef PRANDTL_LINE_WAVES
 #define PRANDTL_LINE_WAVES 3
 #endif
@@ -147,6 +154,31 @@ template <int DIM, int NP> struct LCfg {
 
 // The orders the line kernel is built for (3-D, N = 2..4).
 constexpr bool line_built(int DIM, int NP) { return DIM == 3 && NP >= 3 && NP <= 5; }
+
+// LDS slot of node (i, j, k) in a quantity's row.
+//
+// NP = 4: slot 16k + 4((j+k) mod 4) + ((i+k) mod 4).  The LDS has 64 banks
+// of 4 bytes and serves a wave's 8-byte accesses 32 lanes at a time, so 32
+// lanes reading distinct doubles run in one pass when the slots are distinct
+// mod 32.  In each of the planes i = m, j = m and k = m (the node m of every
+// line of one direction) the slots are distinct mod 16; with element blocks
+// 16 (mod 32) doubles apart, a half-wave holding one direction's lines of two
+// elements reads 32 distinct banks.  Node-order passes (32 consecutive nodes,
+// k in {0, 1} or {2, 3}) are distinct mod 32 as well.
+// Other orders: node n = i + NP(j + NP k) at n + n / NP.
+template <int NP> __device__ __forceinline__ int line_slot(int i, int j, int k)
+{
+    if constexpr (NP == 4) {
+        return 16 * k + 4 * ((j + k) & 3) + ((i + k) & 3);
+    } else {
+        const int n = (k * NP + j) * NP + i;
+        return n + n / NP;
+    }
+}
+template <int NP> __device__ __forceinline__ int node_slot(int n)
+{
+    return line_slot<NP>(n % NP, (n / NP) % NP, n / (NP * NP));
+}
 
 // QNS3: the reference's NS-3D zeta-partner defect (DGSEMIntegrator.cpp:702):
 // the zeta partners m < i of node i take the eta-direction flux of node m
@@ -161,19 +193,22 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
     constexpr int NQ = C::NQ;
     static_assert(DIM == 3, "line kernel: 3-D boxes");
     static_assert(!BLEND || 4 * K <= NV * K, "indicator scratch fits the partial sums");
-    // sQ (node quantities, read in the line phase) and sP (partial sums,
-    // written after it) share storage unless blending, whose indicator uses
-    // sP as scratch while sQ is live
-    //
-    // Node n sits at n + n / NP in each quantity's row (one pad per x-row),
-    // so the lines of all three directions read distinct LDS banks.
-    constexpr int KP = K + K / NP;
-    constexpr int NQP = BLEND ? NQ + NV : (NQ > NV ? NQ : NV);
-    __shared__ double sU[EPB][NV][KP];
-    __shared__ double sQP[EPB * NQP * KP];
-    auto sQ = reinterpret_cast<double (*)[NQ][KP]>(sQP);
-    auto sP = reinterpret_cast<double (*)[NV][KP]>(sQP + (BLEND ? EPB * NQ * KP : 0));
-    auto pad = [](int n) { return n + n / NP; };
+    // Per element: NV rows of the state (sU) and NQ rows of node quantities
+    // (sQ), KP doubles each (node slots of line_slot), then EPAD doubles, so
+    // one address per node reaches every row.  The partial sums (sP, written
+    // after the line phase) take the rows of sQ unless blending, whose
+    // indicator uses sP as scratch while sQ is live.
+    constexpr int KP = C::KP;
+    constexpr int SE = (NV + NQ) * KP + C::EPAD, SPE = NV * KP + C::EPAD;
+    static_assert(NQ >= NV, "partial sums fit the node-quantity rows");
+    __shared__ double sUQ[EPB * SE];
+    __shared__ double sPb[BLEND ? EPB * SPE : 1];
+    auto sU = [&](int e2, int v, int s) -> double & { return sUQ[e2 * SE + v * KP + s]; };
+    auto sQ = [&](int e2, int q, int s) -> double & { return sUQ[e2 * SE + (NV + q) * KP + s]; };
+    auto sP = [&](int e2, int v, int s) -> double & {
+        if constexpr (BLEND) return sPb[e2 * SPE + v * KP + s];
+        else return sUQ[e2 * SE + (NV + v) * KP + s];
+    };
     __shared__ double sV1i[BLEND ? NP * NP : 1];
     __shared__ double sAl[EPB];
     __shared__ double sRed[(BLOCK + 63) / 64];
@@ -191,18 +226,42 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         for (int i = t; i < NP * NP; i += BLOCK) sV1i[i] = A.tab.V1i[(i / NP) * MAXNP + i % NP];
 
     // ---- this thread's line: element el, direction r, line l ----
-    const int el = t / TE;
-    const int rem = t - el * TE;
-    const int r = rem / NFP;
-    const int l = rem - r * NFP;
+    int el, r, l;
+    if constexpr (C::SWZ) {
+        // half-wave h holds the direction-(h mod 3) lines of the elements
+        // 2(h / 3) and 2(h / 3) + 1, 16 lines each
+        static_assert(NFP == 16 && EPB % 2 == 0 && BLOCK == EPB * TE, "line kernel: half-wave lane map");
+        const int h = t >> 5;
+        r = h % 3;
+        el = 2 * (h / 3) + ((t >> 4) & 1);
+        l = t & 15;
+    } else {
+        el = t / TE;
+        const int rem = t - el * TE;
+        r = rem / NFP;
+        l = rem - r * NFP;
+    }
     const int e = ebase + el;
     const bool lact = (el < EPB) && (e < NE);
-    // node of position m on the line: base + m * stride (x fastest)
+    // node of position m on the line: base + m * stride (x fastest); the
+    // line's nodes are (m, l mod NP, l / NP), (l mod NP, m, l / NP) or
+    // (l mod NP, l / NP, m) for r = 0, 1, 2
     const int base = r == 0 ? l * NP : (r == 1 ? (l / NP) * NP * NP + l % NP : l);
     const int stride = r == 0 ? 1 : (r == 1 ? NP : NP * NP);
-    // the same line in the padded LDS rows: pad(base + m stride) = pb + m ps
-    const int pb = r == 0 ? l * (NP + 1) : (r == 1 ? (l / NP) * NP * (NP + 1) + l % NP : l + l / NP);
+    const int la = l % NP, lb = l / NP;
+    // unskewed rows: line_slot(base + m stride) = pb + m ps
+    const int pb = r == 0 ? l * (NP + 1) : (r == 1 ? lb * NP * (NP + 1) + la : l + lb);
     const int ps = r == 0 ? 1 : (r == 1 ? NP + 1 : NP * (NP + 1));
+    // LDS slot of the line's node m: pb + m ps, or with the skewed slots
+    // lsl[m], formed at the start of the line phase (formed here, they would
+    // be held through the node phase)
+    int lsl[NP];
+    auto lnode = [&](int m) {
+        if constexpr (C::SWZ)
+            return lsl[m];
+        else
+            return pb + m * ps;
+    };
     const double nr = r == 0 ? A.geo.nrm[0] : (r == 1 ? A.geo.nrm[1] : A.geo.nrm[2]);
 
     // Every global load of the start is issued on every path, the workgroup's
@@ -269,15 +328,15 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         double ys[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) ys[v] = ysp[p][v];
-        const int np = pad(n);
+        const int np = node_slot<NP>(n);
 #pragma unroll
-        for (int v = 0; v < NV; v++) sU[e2][v][np] = ys[v];
+        for (int v = 0; v < NV; v++) sU(e2, v, np) = ys[v];
         const NodeQ<DIM> q = node_quants<DIM>(ys, P);
 #pragma unroll
-        for (int d = 0; d < DIM; d++) sQ[e2][d][np] = q.v[d];
-        sQ[e2][DIM][np] = q.beta;
-        sQ[e2][DIM + 1][np] = q.ibeta;
-        sQ[e2][DIM + 2][np] = q.lam;
+        for (int d = 0; d < DIM; d++) sQ(e2, d, np) = q.v[d];
+        sQ(e2, DIM, np) = q.beta;
+        sQ(e2, DIM + 1, np) = q.ibeta;
+        sQ(e2, DIM + 2, np) = q.lam;
         const double lq = (q.lam == q.lam) ? q.lam : 0.0;
         lam = (lam < lq) ? lq : lam;
     }
@@ -285,13 +344,13 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
 
     // ---- modal indicator and blending coefficient (element-local) ----
     if constexpr (BLEND) {
-        double *sX = &sP[0][0][0];
+        double *sX = sPb;
         double *sY = sX + EPB * K;
         for (int i = t; i < nact; i += BLOCK) {
             const int e2 = i / K, n = i - e2 * K;
             double ys[NV];
 #pragma unroll
-            for (int v = 0; v < NV; v++) ys[v] = sU[e2][v][pad(n)];
+            for (int v = 0; v < NV; v++) ys[v] = sU(e2, v, node_slot<NP>(n));
             // rho * ComputePressure(state) with V /= rho (BasicOperations.cpp:31-37)
             const double rr = 1.0 / ys[0];
             double vv = 0.0;
@@ -383,16 +442,21 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
     for (int a = 0; a < NP; a++)
 #pragma unroll
         for (int v = 0; v < NV; v++) acc[a][v] = 0.0;
+    if constexpr (C::SWZ) {
+#pragma unroll
+        for (int m = 0; m < NP; m++)
+            lsl[m] = line_slot<NP>(r == 0 ? m : la, r == 1 ? m : (r == 0 ? la : lb), r == 2 ? m : lb);
+    }
     if (lact) {
         const double aface = A.geo.aface;
         // surface points: low face (neighbour = elem1, normal +e_r) and high
         // face (this element = elem1)
 #pragma unroll
         for (int side = 0; side < 2; side++) {
-            const int no = pb + (side ? (NP - 1) * ps : 0);
+            const int no = lnode(side ? NP - 1 : 0);
             double so[NV], pn[NV];
 #pragma unroll
-            for (int v = 0; v < NV; v++) { so[v] = sU[el][v][no]; pn[v] = side ? ph[v] : pl[v]; }
+            for (int v = 0; v < NV; v++) { so[v] = sU(el, v, no); pn[v] = side ? ph[v] : pl[v]; }
             double *s1 = side ? so : pn;
             double *s2 = side ? pn : so;
             double fvn[NV];
@@ -416,10 +480,10 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
             if (own_q) {
                 qo.rho = so[0];
 #pragma unroll
-                for (int d = 0; d < DIM; d++) qo.v[d] = sQ[el][d][no];
-                qo.beta = sQ[el][DIM][no];
-                qo.ibeta = sQ[el][DIM + 1][no];
-                qo.lam = sQ[el][DIM + 2][no];
+                for (int d = 0; d < DIM; d++) qo.v[d] = sQ(el, d, no);
+                qo.beta = sQ(el, DIM, no);
+                qo.ibeta = sQ(el, DIM + 1, no);
+                qo.lam = sQ(el, DIM + 2, no);
             } else {
                 qo = node_quants<DIM>(so, P);
             }
@@ -445,19 +509,19 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
 #pragma unroll
         for (int a = 0; a < NP; a++) {
             NodeQ<DIM> qa;
-            const int na = pb + a * ps;
-            qa.rho = sU[el][0][na];
+            const int na = lnode(a);
+            qa.rho = sU(el, 0, na);
 #pragma unroll
-            for (int d = 0; d < DIM; d++) qa.v[d] = sQ[el][d][na];
-            qa.beta = sQ[el][DIM][na];
+            for (int d = 0; d < DIM; d++) qa.v[d] = sQ(el, d, na);
+            qa.beta = sQ(el, DIM, na);
 #pragma unroll
             for (int b = a + 1; b < NP; b++) {
                 NodeQ<DIM> qb;
-                const int nb = pb + b * ps;
-                qb.rho = sU[el][0][nb];
+                const int nb = lnode(b);
+                qb.rho = sU(el, 0, nb);
 #pragma unroll
-                for (int d = 0; d < DIM; d++) qb.v[d] = sQ[el][d][nb];
-                qb.beta = sQ[el][DIM][nb];
+                for (int d = 0; d < DIM; d++) qb.v[d] = sQ(el, d, nb);
+                qb.beta = sQ(el, DIM, nb);
                 double F[NV];
                 ec_flux<DIM, false>(r, nr, qa, qb, nullptr, nullptr, P, false, F);
                 const double dab = cv * A.tab.Dh2[a * MAXNP + b], dba = cv * A.tab.Dh2[b * MAXNP + a];
@@ -472,12 +536,12 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
                 // node (i, NP-1, a) of its eta line; 0 on the eta line's last node
                 const int i0 = l % NP, j0 = l / NP;
                 if (j0 < NP - 1) {
-                    const int ne = pad((a * NP + NP - 1) * NP + i0);
+                    const int ne = line_slot<NP>(i0, NP - 1, a);
                     NodeQ<DIM> qe;
-                    qe.rho = sU[el][0][ne];
+                    qe.rho = sU(el, 0, ne);
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) qe.v[d] = sQ[el][d][ne];
-                    qe.beta = sQ[el][DIM][ne];
+                    for (int d = 0; d < DIM; d++) qe.v[d] = sQ(el, d, ne);
+                    qe.beta = sQ(el, DIM, ne);
                     double g[NV];
                     ec_flux<DIM, false>(1, A.geo.nrm[1], qa, qe, nullptr, nullptr, P, false, g);
 #pragma unroll
@@ -499,17 +563,17 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
                 for (int v = 0; v < NV; v++) gprev[v] = 0.0;
 #pragma unroll
                 for (int i = 0; i < NP - 1; i++) {
-                    const int na = pb + i * ps, nb = na + ps;
+                    const int na = lnode(i), nb = lnode(i + 1);
                     double s1[NV], s2[NV];
 #pragma unroll
-                    for (int v = 0; v < NV; v++) { s1[v] = sU[el][v][na]; s2[v] = sU[el][v][nb]; }
+                    for (int v = 0; v < NV; v++) { s1[v] = sU(el, v, na); s2[v] = sU(el, v, nb); }
                     NodeQ<DIM> qa, qb;
                     qa.rho = s1[0]; qb.rho = s2[0];
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[el][d][na]; qb.v[d] = sQ[el][d][nb]; }
-                    qa.beta = sQ[el][DIM][na]; qb.beta = sQ[el][DIM][nb];
-                    qa.ibeta = sQ[el][DIM + 1][na]; qb.ibeta = sQ[el][DIM + 1][nb];
-                    qa.lam = sQ[el][DIM + 2][na]; qb.lam = sQ[el][DIM + 2][nb];
+                    for (int d = 0; d < DIM; d++) { qa.v[d] = sQ(el, d, na); qb.v[d] = sQ(el, d, nb); }
+                    qa.beta = sQ(el, DIM, na); qb.beta = sQ(el, DIM, nb);
+                    qa.ibeta = sQ(el, DIM + 1, na); qb.ibeta = sQ(el, DIM + 1, nb);
+                    qa.lam = sQ(el, DIM + 2, na); qb.lam = sQ(el, DIM + 2, nb);
                     const double si = r == 0 ? A.tab.subn[0][i] : (r == 1 ? A.tab.subn[1][i] : A.tab.subn[2][i]);
                     double G[NV];
                     ec_flux<DIM, true>(r, si, qa, qb, s1, s2, P, quirk3d, G);
@@ -556,9 +620,9 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         if (lact && r == rr) {
 #pragma unroll
             for (int a = 0; a < NP; a++) {
-                const int nd = pb + a * ps;
+                const int nd = lnode(a);
 #pragma unroll
-                for (int v = 0; v < NV; v++) sP[el][v][nd] = (rr == 0) ? acc[a][v] : sP[el][v][nd] + acc[a][v];
+                for (int v = 0; v < NV; v++) sP(el, v, nd) = (rr == 0) ? acc[a][v] : sP(el, v, nd) + acc[a][v];
             }
         }
         __syncthreads();
@@ -573,9 +637,9 @@ __attribute__((amdgpu_waves_per_eu((LCfg<DIM, NP>::WAVES)))) k_rhs_ln(const KArg
         const size_t gi = (size_t)ebase * K + i;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            double tot = sP[e2][v][pad(n)];
+            double tot = sP(e2, v, node_slot<NP>(n));
             if (A.parabolic && v > 0) tot += vsp[p][v - 1] * A.geo.invJ;
-            const double yv = sU[e2][v][pad(n)];
+            const double yv = sU(e2, v, node_slot<NP>(n));
             double o;
             if (A.mode == 0) o = tot;
             else if (A.mode == 1) o = yv + dt * tot;
