@@ -371,6 +371,20 @@ constexpr bool rhs_shape_ok(int DIM, int NP, bool BLEND, int shape)
     return f.EPB * f.K <= f.BLOCK && !(f.EPB == n.EPB && f.BLOCK == n.BLOCK);
 }
 
+// LDS slot of node n in the tile kernel's per-node rows.  A wave's pair jobs
+// read the nodes of a line, whose slots are a node stride apart; the LDS
+// serves 8-byte reads 32 lanes at a time from 64 four-byte banks, so strides
+// that are multiples of 16 doubles put a line's nodes on one or two banks.
+// 3-D N = 7 pads one double per x-row and one per xy-plane (strides 1, 9,
+// 73).  Other orders keep node n at n: their strides are odd or small, or
+// (3-D N = 3, z stride 16: two-way) the index arithmetic of a padded slot
+// costs the registers of a wave per SIMD.
+template <int DIM, int NP> __host__ __device__ constexpr int row_slot(int n)
+{
+    if (DIM == 3 && NP == 8) return n + n / 8 + n / 64;
+    return n;
+}
+
 template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
     static constexpr int NV = DIM + 2;
     static constexpr int K = npow(NP, DIM);
@@ -390,6 +404,9 @@ template <int DIM, int NP, bool BLEND, int FB = 0> struct RCfg {
     }
     static constexpr int LC = lc_pick();
     static constexpr int FXN = LC * (NPAIR + NFV);
+    // LDS row length of the per-node rows (state, node quantities, adjugate
+    // rows): see row_slot
+    static constexpr int KU = row_slot<DIM, NP>(K - 1) + 1;
     // Registers for 4 waves per SIMD (128) where a workgroup has at most two
     // waves or more than 256 nodes: there, two workgroups per CU beat the
     // spills the cap causes (3-D N=7 with blending: 3.4 ms per launch at
@@ -407,8 +424,8 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false, bool
     double V1i[BLEND ? NP * NP : 1];
     double Fvs[NP];
     double Sub[DIM][NP];
-    double U[C::EPB][C::NV][C::K];
-    double Q[C::EPB][NQ][C::K];
+    double U[C::EPB][C::NV][C::KU];
+    double Q[C::EPB][NQ][C::KU];
     // signed surface contributions (surface phase), then the pair and
     // subcell fluxes of one line chunk (direction phases): one buffer
     // (also the indicator's scratch: 4 K values per element)
@@ -418,7 +435,7 @@ template <int DIM, int NP, bool BLEND, bool QNS3, int FB, bool GEN = false, bool
     double G[GE][C::NV][GK];
     double Al[C::EPB];
     double Red[(C::BLOCK + 63) / 64];
-    double M[GEN ? C::EPB : 1][GEN ? DIM * DIM : 1][GEN ? C::K : 1];   // per-node adjugate rows (general geometry)
+    double M[GEN ? C::EPB : 1][GEN ? DIM * DIM : 1][GEN ? C::KU : 1];   // per-node adjugate rows (general geometry)
     double R[AXI ? C::EPB : 1][AXI ? C::K : 1];      // per-node r (axisymmetric)
     double Cn[GEN && DIM == 3 ? C::EPB : 1][GEN && DIM == 3 ? 24 : 1];   // hexahedron corners
 };
@@ -672,18 +689,18 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
     double lam = 0.0;
     if (act) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) sU[el][v][n] = ys[v];
+        for (int v = 0; v < NV; v++) sU[el][v][row_slot<DIM, NP>(n)] = ys[v];
         if constexpr (GEN)
 #pragma unroll
-            for (int c = 0; c < MC; c++) sM[el][c][n] = mown[c];
+            for (int c = 0; c < MC; c++) sM[el][c][row_slot<DIM, NP>(n)] = mown[c];
         if constexpr (AXI) sR[el][n] = rown;
         const NodeQ<DIM> q = node_quants<DIM>(ys, P);
 #pragma unroll
-        for (int d = 0; d < DIM; d++) sQ[el][d][n] = q.v[d];
-        sQ[el][DIM][n] = q.beta;
+        for (int d = 0; d < DIM; d++) sQ[el][d][row_slot<DIM, NP>(n)] = q.v[d];
+        sQ[el][DIM][row_slot<DIM, NP>(n)] = q.beta;
         if constexpr (BLEND) {
-            sQ[el][DIM + 1][n] = q.ibeta;
-            sQ[el][DIM + 2][n] = q.lam;
+            sQ[el][DIM + 1][row_slot<DIM, NP>(n)] = q.ibeta;
+            sQ[el][DIM + 2][row_slot<DIM, NP>(n)] = q.lam;
         }
         lam = (q.lam == q.lam) ? q.lam : 0.0;
         if (QNS3 && ii[1] == NP - 1)
@@ -803,7 +820,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         const int no = line_node<DIM, NP>(r, l, side ? NP - 1 : 0);
         double so[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) so[v] = sU[ej][v][no];
+        for (int v = 0; v < NV; v++) so[v] = sU[ej][v][row_slot<DIM, NP>(no)];
         if constexpr (GEN) {
             // general geometry: the face's normal and first element from the
             // face tables (owner computes, as on the box), boundary points by
@@ -1059,16 +1076,16 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     const int na = line_node<DIM, NP>(r, l0 + lc, a);
                     const int nb = line_node<DIM, NP>(r, l0 + lc, b);
                     NodeQ<DIM> qa, qb;
-                    qa.rho = sU[ej][0][na]; qb.rho = sU[ej][0][nb];
+                    qa.rho = sU[ej][0][row_slot<DIM, NP>(na)]; qb.rho = sU[ej][0][row_slot<DIM, NP>(nb)];
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][na]; qb.v[d] = sQ[ej][d][nb]; }
-                    qa.beta = sQ[ej][DIM][na]; qb.beta = sQ[ej][DIM][nb];
+                    for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][row_slot<DIM, NP>(na)]; qb.v[d] = sQ[ej][d][row_slot<DIM, NP>(nb)]; }
+                    qa.beta = sQ[ej][DIM][row_slot<DIM, NP>(na)]; qb.beta = sQ[ej][DIM][row_slot<DIM, NP>(nb)];
                     double F[NV];
                     if constexpr (GEN) {
                         // ComputeVolumeFlux with the mean of the two adjugate rows
                         double mm[DIM];
 #pragma unroll
-                        for (int d = 0; d < DIM; d++) mm[d] = (sM[ej][DIM * r + d][na] + sM[ej][DIM * r + d][nb]) * 0.5;
+                        for (int d = 0; d < DIM; d++) mm[d] = (sM[ej][DIM * r + d][row_slot<DIM, NP>(na)] + sM[ej][DIM * r + d][row_slot<DIM, NP>(nb)]) * 0.5;
                         two_point_flux_v<DIM, false>(mm, 0.0, qa, qb, nullptr, nullptr, P, false, F);
                         if constexpr (AXI) {
                             // f *= r_hat, r at the pair's midpoint (DGSEMIntegrator.cpp:272-292)
@@ -1102,13 +1119,13 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
                     double s1[NV], s2[NV];
                     NodeQ<DIM> qa, qb;
 #pragma unroll
-                    for (int v = 0; v < NV; v++) { s1[v] = sU[ej][v][na]; s2[v] = sU[ej][v][nb]; }
+                    for (int v = 0; v < NV; v++) { s1[v] = sU[ej][v][row_slot<DIM, NP>(na)]; s2[v] = sU[ej][v][row_slot<DIM, NP>(nb)]; }
                     qa.rho = s1[0]; qb.rho = s2[0];
 #pragma unroll
-                    for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][na]; qb.v[d] = sQ[ej][d][nb]; }
-                    qa.beta = sQ[ej][DIM][na]; qb.beta = sQ[ej][DIM][nb];
-                    qa.ibeta = sQ[ej][NQ - 2][na]; qb.ibeta = sQ[ej][NQ - 2][nb];
-                    qa.lam = sQ[ej][NQ - 1][na]; qb.lam = sQ[ej][NQ - 1][nb];
+                    for (int d = 0; d < DIM; d++) { qa.v[d] = sQ[ej][d][row_slot<DIM, NP>(na)]; qb.v[d] = sQ[ej][d][row_slot<DIM, NP>(nb)]; }
+                    qa.beta = sQ[ej][DIM][row_slot<DIM, NP>(na)]; qb.beta = sQ[ej][DIM][row_slot<DIM, NP>(nb)];
+                    qa.ibeta = sQ[ej][NQ - 2][row_slot<DIM, NP>(na)]; qb.ibeta = sQ[ej][NQ - 2][row_slot<DIM, NP>(nb)];
+                    qa.lam = sQ[ej][NQ - 1][row_slot<DIM, NP>(na)]; qb.lam = sQ[ej][NQ - 1][row_slot<DIM, NP>(nb)];
                     double F[NV];
                     if constexpr (GEN) {
                         const double *sn = A.geo.subn + (((((size_t)(ebase + ej) * DIM + r) * NFP + l0 + lc) * (NP - 1) + i) * DIM);
@@ -1182,8 +1199,8 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
         double src = 0.0;
         if constexpr (AXI) {
             // PressureFromConservative (DGSEMIntegrator.cpp:1306-1315) of this node
-            const double ke = 0.5 * ((sU[el][1][n] * sU[el][1][n] + sU[el][2][n] * sU[el][2][n]) / sU[el][0][n]);
-            src = P.gm1 * (sU[el][3][n] - ke);
+            const double ke = 0.5 * ((sU[el][1][row_slot<DIM, NP>(n)] * sU[el][1][row_slot<DIM, NP>(n)] + sU[el][2][row_slot<DIM, NP>(n)] * sU[el][2][row_slot<DIM, NP>(n)]) / sU[el][0][row_slot<DIM, NP>(n)]);
+            src = P.gm1 * (sU[el][3][row_slot<DIM, NP>(n)] - ke);
         }
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -1197,7 +1214,7 @@ __device__ __forceinline__ void rhs_body(const KArgs &A, const int tile, const i
             // ZeroAxisRadialMom (DGSEMOperator.cpp:521-533)
             if (AXI && v == 2 && axown >= 0 && (axown & 1)) tot = 0.0;
             const size_t gi = v * NK + (size_t)e * K + n;
-            const double yv = AXI ? yw[v] : sU[el][v][n];
+            const double yv = AXI ? yw[v] : sU[el][v][row_slot<DIM, NP>(n)];
             double o;
             if (A.mode == 0) o = tot;
             else if (A.mode == 1) o = yv + dt * tot;
